@@ -1,0 +1,418 @@
+"""Seeded procedural scenes and cameras for the wavefront path-tracing pass.
+
+Every random draw comes from one PCG32 stream (O'Neill 2014) of the scene seed, in
+the order documented per generator, so the GPU path and the CPU restatement receive
+identical inputs.  Geometry is computed in float64 and stored as float32.
+
+Scene layout follows the pass's inputs (SURVEY inputs of PathTracingPass):
+  meshes     object-space positions/normals (float32 xyz) + uint32 triangle indices
+  instances  3x4 row-major transforms (AccelerationStructureInstanceDesc.transform,
+             include/bisemutum/rhi/accel.hpp:48-55), mesh id, material id
+  materials  bi_pt_material records: base_color, roughness, f0, anisotropy, f90, ior,
+             two_sided -- f0 = lerp(0.04, base_color, metallic), f90 = 1, ior = 1.5
+             (src/scene_basic/menu_actions/import_model.cpp:208-230)
+  lights     DirLightData records (src/renderer/context/lights.hpp:12-22)
+  sky        RGBA32F equirect, colour multiplier, 3x3 rotation
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class Pcg32:
+    """PCG32 (pcg32_random_r / pcg32_srandom_r), Python integers."""
+
+    MULT = 6364136223846793005
+
+    def __init__(self, initstate: int, initseq: int = 0):
+        self.state = 0
+        self.inc = ((initseq << 1) | 1) & MASK64
+        self.next_u32()
+        self.state = (self.state + initstate) & MASK64
+        self.next_u32()
+
+    def next_u32(self) -> int:
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & MASK64
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self) -> float:
+        return (self.next_u32() >> 8) * (1.0 / (1 << 24))
+
+    def range(self, lo: float, hi: float) -> float:
+        return lo + (hi - lo) * self.uniform()
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    meshes: list          # list of (positions f32 [V,3], normals f32 [V,3], indices u32 [T,3])
+    instances: np.ndarray  # structured: xform f32[12], mesh u32, material u32
+    materials: np.ndarray  # f32 [M,16]
+    dir_lights: np.ndarray  # f32 [L,16] (sm_index stored as int32 bits)
+    sky_rgba: np.ndarray   # f32 [H,W,4]
+    sky_color: np.ndarray  # f32 [3]
+    sky_rot: np.ndarray    # f32 [9] row-major
+    notes: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def num_tris(self) -> int:
+        return int(sum(len(self.meshes[m][2]) for m in self.instances["mesh"]))
+
+
+INSTANCE_DTYPE = np.dtype([("xform", np.float32, (12,)), ("mesh", np.uint32), ("material", np.uint32)])
+
+
+def material(base_color, roughness, metallic, two_sided=True, anisotropy=0.0):
+    """bi_pt_material record (16 floats) with the importer's metallic mapping."""
+    base = np.asarray(base_color, np.float32)
+    m = np.float32(metallic)
+    f0 = (np.float32(0.04) + (base - np.float32(0.04)) * m).astype(np.float32)  # lerp(0.04, base, metallic)
+    rec = np.zeros(16, np.float32)
+    rec[0:3] = base
+    rec[3] = roughness
+    rec[4:7] = f0
+    rec[7] = anisotropy
+    rec[8:11] = 1.0
+    rec[11] = 1.5
+    rec[12] = 1.0 if two_sided else 0.0
+    return rec
+
+
+def dir_light(emission, direction, cast_shadow=True, shadow_strength=1.0):
+    """DirLightData record: emission, sm_index, direction (to the light), shadow_strength, ..."""
+    rec = np.zeros(16, np.float32)
+    rec[0:3] = emission
+    d = np.asarray(direction, np.float64)
+    rec[4:7] = d / np.linalg.norm(d)
+    rec[7] = shadow_strength
+    rec.view(np.int32)[3] = 0 if cast_shadow else -1
+    return rec
+
+
+def random_material(rng: Pcg32, two_sided=True):
+    """albedo U[0.1,0.9]^3, roughness U[0.05,1], metallic Bernoulli(0.3); draw order as written."""
+    albedo = [rng.range(0.1, 0.9) for _ in range(3)]
+    rough = rng.range(0.05, 1.0)
+    metal = 1.0 if rng.uniform() < 0.3 else 0.0
+    return material(albedo, rough, metal, two_sided)
+
+
+def translate_scale(t, s=(1.0, 1.0, 1.0), rot_y=0.0):
+    c, si = math.cos(rot_y), math.sin(rot_y)
+    r = np.array([[c, 0, si], [0, 1, 0], [-si, 0, c]], np.float64)
+    m = r @ np.diag(s)
+    x = np.zeros((3, 4), np.float64)
+    x[:, :3] = m
+    x[:, 3] = t
+    return x.reshape(-1).astype(np.float32)
+
+
+# ---------------------------------------------------------------- meshes
+
+def icosphere(subdiv: int):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+        new_faces = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                v = verts[a] + verts[b]
+                verts.append(v / np.linalg.norm(v))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new_faces += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = new_faces
+    p = np.array(verts, np.float64)
+    return p.astype(np.float32), p.astype(np.float32), np.array(faces, np.uint32)
+
+
+def grid_quad(origin, du, dv, nu, nv, normal):
+    """nu x nv quads spanning origin + [0,1]du + [0,1]dv, flat normal; 2*nu*nv triangles."""
+    origin, du, dv = (np.asarray(a, np.float64) for a in (origin, du, dv))
+    iu, iv = np.meshgrid(np.arange(nu + 1), np.arange(nv + 1), indexing="ij")
+    p = origin + (iu[..., None] / nu) * du + (iv[..., None] / nv) * dv
+    p = p.reshape(-1, 3)
+    n = np.tile(np.asarray(normal, np.float64), (len(p), 1))
+    idx = []
+    for a in range(nu):
+        for b in range(nv):
+            v00 = a * (nv + 1) + b
+            v10 = (a + 1) * (nv + 1) + b
+            v01 = a * (nv + 1) + b + 1
+            v11 = (a + 1) * (nv + 1) + b + 1
+            idx += [(v00, v10, v11), (v00, v11, v01)]
+    return p.astype(np.float32), n.astype(np.float32), np.array(idx, np.uint32)
+
+
+def merge(parts):
+    ps, ns, ids = [], [], []
+    base = 0
+    for p, n, i in parts:
+        ps.append(p)
+        ns.append(n)
+        ids.append(i + base)
+        base += len(p)
+    return np.concatenate(ps), np.concatenate(ns), np.concatenate(ids).astype(np.uint32)
+
+
+def lathe_column(segments=64, rings=28, radius=0.25, height=3.6, flutes=16):
+    """Fluted column: a lathe of `rings` x `segments` quads (2*rings*segments triangles)."""
+    ys = np.linspace(0.0, height, rings + 1)
+    th = np.linspace(0.0, 2 * math.pi, segments + 1)
+
+    def prof(y):
+        base = 0.35 * np.exp(-((y - 0.0) / 0.25) ** 2)      # base flare
+        cap = 0.30 * np.exp(-((y - height) / 0.25) ** 2)   # capital
+        return radius * (1.0 + base + cap)
+
+    Y, T = np.meshgrid(ys, th, indexing="ij")
+    flute = 1.0 + 0.06 * np.cos(flutes * T) * (1.0 - np.exp(-((Y - 0) / 0.3) ** 2) - np.exp(-((Y - height) / 0.3) ** 2))
+    R = prof(Y) * flute
+    P = np.stack([R * np.sin(T), Y, R * np.cos(T)], -1)
+    # analytic-ish normals by finite differences on the parametric surface
+    dY = np.gradient(P, ys, axis=0)
+    dT = np.gradient(P, th, axis=1)
+    N = np.cross(dT, dY)
+    N /= np.maximum(np.linalg.norm(N, axis=-1, keepdims=True), 1e-12)
+    idx = []
+    for r in range(rings):
+        for s in range(segments):
+            a = r * (segments + 1) + s
+            b = a + 1
+            c = a + (segments + 1)
+            d = c + 1
+            idx += [(a, b, d), (a, d, c)]
+    return P.reshape(-1, 3).astype(np.float32), N.reshape(-1, 3).astype(np.float32), np.array(idx, np.uint32)
+
+
+def unit_arch(arc_segments=14, tube_segments=14, tube_radius=0.12, rise=0.5):
+    """Semicircular tube from (0,0,0) to (1,0,0) peaking at y=rise: 2*arc*tube triangles."""
+    u = np.linspace(0.0, 1.0, arc_segments + 1)
+    a = np.linspace(0.0, 2 * math.pi, tube_segments + 1)
+    ang = u * math.pi
+    cx = 0.5 - 0.5 * np.cos(ang)
+    cy = rise * np.sin(ang)
+    tx = 0.5 * np.sin(ang)
+    ty = rise * np.cos(ang)
+    tl = np.sqrt(tx * tx + ty * ty)
+    tx, ty = tx / tl, ty / tl
+    nx, ny = -ty, tx           # in-plane normal of the centre curve
+    U, A = np.meshgrid(np.arange(arc_segments + 1), a, indexing="ij")
+    ca, sa = np.cos(A), np.sin(A)
+    Nv = np.stack([nx[U] * ca, ny[U] * ca, sa], -1)
+    P = np.stack([cx[U], cy[U], np.zeros_like(ca)], -1) + tube_radius * Nv
+    idx = []
+    for i in range(arc_segments):
+        for j in range(tube_segments):
+            p00 = i * (tube_segments + 1) + j
+            p01 = p00 + 1
+            p10 = p00 + tube_segments + 1
+            p11 = p10 + 1
+            idx += [(p00, p10, p11), (p00, p11, p01)]
+    return P.reshape(-1, 3).astype(np.float32), Nv.reshape(-1, 3).astype(np.float32), np.array(idx, np.uint32)
+
+
+# ---------------------------------------------------------------- sky
+
+def constant_sky(color=(0.5, 0.6, 0.8), w=64, h=32):
+    rgba = np.ones((h, w, 4), np.float32)
+    return rgba, np.asarray(color, np.float32)
+
+
+def preetham_like_sky(sun_dir, w=512, h=256):
+    """Zenith/horizon gradient above, dim ground below, plus a sun lobe (FP32 RGBA)."""
+    th = (np.arange(h) + 0.5) / h * math.pi
+    ph = (np.arange(w) + 0.5) / w * 2 * math.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.sin(P), np.cos(T), np.sin(T) * np.cos(P)], -1)
+    s = np.asarray(sun_dir, np.float64)
+    s = s / np.linalg.norm(s)
+    zen = np.array([0.22, 0.42, 0.95])
+    hor = np.array([0.85, 0.88, 0.95])
+    gnd = np.array([0.12, 0.11, 0.10])
+    y = d[..., 1:2]
+    t = np.clip(y, 0.0, 1.0) ** 0.4
+    up = hor * (1 - t) + zen * t
+    col = np.where(y > 0, up, gnd + (hor - gnd) * np.exp(8.0 * np.minimum(y, 0.0)))
+    cg = np.clip((d * s).sum(-1, keepdims=True), -1.0, 1.0)
+    col = col + np.array([30.0, 27.0, 22.0]) * np.exp(2000.0 * (cg - 1.0)) \
+        + np.array([0.6, 0.5, 0.35]) * np.exp(30.0 * (cg - 1.0))
+    rgba = np.concatenate([col, np.ones_like(y)], -1).astype(np.float32)
+    return rgba, np.ones(3, np.float32)
+
+
+# ---------------------------------------------------------------- configs
+
+def scene_c0(seed: int = 0xB15E0000):
+    """Box room (12 quads: floor 2x2 + 4 walls x 2, open top) + 64 icospheres (subdiv 2).
+
+    Draw order: for each sphere: centre x, y, z, radius; then materials (room first,
+    then spheres): albedo r, g, b, roughness, metallic."""
+    rng = Pcg32(seed, 0)
+    half, height = 6.0, 4.0
+    room = merge([
+        grid_quad((-half, 0, -half), (2 * half, 0, 0), (0, 0, 2 * half), 2, 2, (0, 1, 0)),
+        grid_quad((-half, 0, -half), (0, height, 0), (2 * half, 0, 0), 1, 2, (0, 0, 1)),
+        grid_quad((-half, 0, half), (2 * half, 0, 0), (0, height, 0), 2, 1, (0, 0, -1)),
+        grid_quad((-half, 0, -half), (0, 0, 2 * half), (0, height, 0), 2, 1, (1, 0, 0)),
+        grid_quad((half, 0, -half), (0, height, 0), (0, 0, 2 * half), 1, 2, (-1, 0, 0)),
+    ])
+    sphere = icosphere(2)
+    inst = []
+    centres = []
+    for _ in range(64):
+        c = (rng.range(-half + 0.6, half - 0.6), rng.range(0.6, height - 0.6), rng.range(-half + 0.6, half - 0.6))
+        r = rng.range(0.2, 0.6)
+        centres.append((c, r))
+    mats = [random_material(rng)]
+    inst.append((translate_scale((0, 0, 0)), 0, 0))
+    for i, (c, r) in enumerate(centres):
+        mats.append(random_material(rng))
+        inst.append((translate_scale(c, (r, r, r)), 1, i + 1))
+    instances = np.array(inst, INSTANCE_DTYPE)
+    sky, col = constant_sky()
+    sun = dir_light((2.6, 2.5, 2.3), (0.35, 0.85, 0.4))
+    return Scene("c0_box_room", [room, sphere], instances, np.stack(mats), sun[None], sky, col,
+                 np.eye(3, dtype=np.float32).reshape(-1),
+                 notes={"room_tris": len(room[2]), "sphere_tris": len(sphere[2])})
+
+
+def scene_c1(seed: int = 0xB15E0001):
+    """Atrium of exactly 262,144 triangles.
+
+    Split: 64 fluted lathe columns x 3,584 = 229,376; 64 arches x 392 = 25,088; ground
+    floor 2,048; gallery floor 1,536; 4 walls x 1,024 = 4,096.  Open top (sky).
+    Draw order: materials per object in instance order (columns floor 0 then 1, arches
+    floor 0 then 1, ground, gallery, 4 walls): albedo r, g, b, roughness, metallic."""
+    rng = Pcg32(seed, 0)
+    outer, court, storey = 12.0, 6.0, 4.0
+    column = lathe_column()
+    arch = unit_arch()
+    ground = grid_quad((-outer, 0, -outer), (2 * outer, 0, 0), (0, 0, 2 * outer), 32, 32, (0, 1, 0))
+    # gallery: 32x32 grid minus the central 16x16 court
+    gp, gn, gi = grid_quad((-outer, storey, -outer), (2 * outer, 0, 0), (0, 0, 2 * outer), 32, 32, (0, 1, 0))
+    keep = []
+    for a in range(32):
+        for b in range(32):
+            if not (8 <= a < 24 and 8 <= b < 24):
+                keep += [2 * (a * 32 + b), 2 * (a * 32 + b) + 1]
+    gallery = (gp, gn, gi[keep])
+    wall = grid_quad((0, 0, 0), (1, 0, 0), (0, 1, 0), 32, 16, (0, 0, 1))  # unit wall, normal +z
+    meshes = [column, arch, ground, gallery, wall]
+    # 32 column positions on the court perimeter (8 per side)
+    ring = []
+    for side in range(4):
+        for k in range(8):
+            t = -court + 1.5 * k
+            ring.append([(t, -court), (court, t), (-t, court), (-court, -t)][side])
+    inst = []
+    mats = []
+
+    def add(x, mesh):
+        mats.append(random_material(rng))
+        inst.append((x, mesh, len(mats) - 1))
+
+    for fl in range(2):
+        for (x, z) in ring:
+            add(translate_scale((x, fl * storey, z)), 0)
+    col_h = 3.6
+    for fl in range(2):
+        for k in range(32):
+            (x0, z0), (x1, z1) = ring[k], ring[(k + 1) % 32]
+            span = math.hypot(x1 - x0, z1 - z0)
+            ang = math.atan2(-(z1 - z0), x1 - x0)
+            add(translate_scale((x0, fl * storey + col_h, z0), (span, 0.8, 1.0), ang), 1)
+    add(translate_scale((0, 0, 0)), 2)
+    add(translate_scale((0, 0, 0)), 3)
+    wall_h = 2 * storey
+    for k, (pos, ang) in enumerate([((-outer, 0, -outer), 0.0), ((outer, 0, outer), math.pi),
+                                     ((-outer, 0, outer), math.pi / 2), ((outer, 0, -outer), -math.pi / 2)]):
+        add(translate_scale(pos, (2 * outer, wall_h, 1.0), ang), 4)
+    instances = np.array(inst, INSTANCE_DTYPE)
+    sun_dir = (0.45, 0.8, 0.35)
+    sky, col = preetham_like_sky(sun_dir)
+    sun = dir_light((3.2, 3.0, 2.7), sun_dir)
+    sc = Scene("c1_atrium", meshes, instances, np.stack(mats), sun[None], sky, col,
+               np.eye(3, dtype=np.float32).reshape(-1),
+               notes={"columns": 64 * len(column[2]), "arches": 64 * len(arch[2]), "ground": len(ground[2]),
+                      "gallery": len(gallery[2]), "walls": 4 * len(wall[2])})
+    assert sc.num_tris == 262144, sc.num_tris
+    return sc
+
+
+# ---------------------------------------------------------------- camera
+
+def look_at_rh(eye, center, up):
+    eye, center, up = (np.asarray(a, np.float64) for a in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    v = np.eye(4)
+    v[0, :3], v[1, :3], v[2, :3] = s, u, -f
+    v[0, 3], v[1, 3], v[2, 3] = -s @ eye, -u @ eye, f @ eye
+    return v
+
+
+def perspective_reverse_z(fovy, aspect, near, far):
+    """glm::perspective (zero-to-one, RH) with the reverse-Z patch of src/math/math.cpp:5-11."""
+    t = math.tan(fovy / 2)
+    p = np.zeros((4, 4))
+    p[0, 0] = 1.0 / (aspect * t)
+    p[1, 1] = 1.0 / t
+    p[2, 2] = near / (far - near)
+    p[2, 3] = near * far / (far - near)
+    p[3, 2] = -1.0
+    return p
+
+
+@dataclasses.dataclass
+class Camera:
+    inv_view: np.ndarray
+    inv_proj: np.ndarray
+    proj_view: np.ndarray
+    width: int
+    height: int
+
+
+def make_camera(eye, center, width, height, yfov_deg=60.0, up=(0, 1, 0), near=0.05, far=500.0):
+    view = look_at_rh(eye, center, up)
+    proj = perspective_reverse_z(math.radians(yfov_deg), width / height, near, far)
+    return Camera(np.linalg.inv(view).astype(np.float32), np.linalg.inv(proj).astype(np.float32),
+                  (proj @ view).astype(np.float32), width, height)
+
+
+def camera_c0(width=320, height=240):
+    return make_camera((0.0, 7.5, 10.5), (0.0, 0.8, 0.0), width, height, 55.0)
+
+
+def camera_c1(width=1920, height=1080):
+    return make_camera((0.0, 1.7, 11.0), (0.0, 3.2, 0.0), width, height, 65.0)
+
+
+CONFIGS = {
+    "c0": dict(scene=scene_c0, camera=camera_c0, width=320, height=240, spp=1, max_bounces=5,
+               seed=0x5EED0000),
+    "c1": dict(scene=scene_c1, camera=camera_c1, width=1920, height=1080, spp=16, max_bounces=9,
+               seed=0x5EED0001),
+}

@@ -1,0 +1,156 @@
+/*
+ * oracle.h -- CPU restatement of the wavefront path-tracing pass.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it, and never as the thing measured or shipped.
+ *
+ * Two restatements live behind this header:
+ *   ref32.cpp  scalar FP32 transcription.  Every floating-point operation is
+ *              written in the same order as the HIP kernels and built with
+ *              -ffp-contract=off, so integer stages (Morton codes, sorted
+ *              order, BVH topology, queue contents) and radiance are expected
+ *              to match the GPU bit for bit.
+ *   ref64.cpp  FP64 oracle of the same integrand (libm transcendentals, its
+ *              own median-split BVH, double-precision intersection), used for
+ *              convergence checks at high sample counts.
+ *
+ * Reference files restated (paths relative to bisemutum/):
+ *   shaders/renderer/raytracing/direction_sample/generate_camera_ray.hlsl:4-16
+ *   shaders/renderer/raytracing/direction_sample/sample_secondary_ray.hlsl:11-69
+ *   shaders/renderer/raytracing/rt_gbuffer.hlsl:7-36, hits/rt_gbuffer_hit.hlsl:6-24
+ *   shaders/renderer/raytracing/deferred_lighting_secondary.hlsl:11-60
+ *   shaders/core/material/lit.hlsl:5-35, material/utils.hlsl:40-121
+ *   shaders/core/utils/frame.hlsl:9-20, shaders/core/equitangular_to_cubemap.hlsl:28-33
+ *   shaders/core/raytracing/hit.hlsl:27-164 (attribute interpolation)
+ *   shaders/renderer/lights.hlsl:10-13,69-101 (directional light, shadow strength)
+ *   src/renderer/pass/path_tracing.cpp:224-488 (bounce loop), src/math/math.cpp:5-11
+ */
+#ifndef BI_PT_ORACLE_H
+#define BI_PT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_ctx ora_ctx;
+
+/* Scene description, all host memory.
+ * meshes are concatenated: mesh m owns vertices [vtx_off[m], vtx_off[m]+vtx_cnt[m])
+ * of positions/normals (3 floats each) and indices [idx_off[m], idx_off[m]+idx_cnt[m])
+ * (mesh-local vertex numbers).  Instance i places mesh inst_mesh[i] with the
+ * 3x4 row-major transform inst_xform[12*i..] and material inst_mat[i].
+ * materials: 16 floats each, layout of bi_pt_material (include/bi_pt.h).
+ * dir_lights: 16 floats each, layout of bi_pt_dir_light (= DirLightData).
+ * sky: sky_w*sky_h RGBA32F texels (row 0 = theta 0, the +Y pole). */
+typedef struct ora_scene_desc {
+    uint32_t num_meshes;
+    const float* positions;
+    const float* normals;
+    const uint32_t* indices;
+    const uint32_t* vtx_off;
+    const uint32_t* idx_off;
+    const uint32_t* idx_cnt;
+    uint32_t num_instances;
+    const float* inst_xform;
+    const uint32_t* inst_mesh;
+    const uint32_t* inst_mat;
+    uint32_t num_materials;
+    const float* materials;
+    uint32_t num_dir_lights;
+    const float* dir_lights;
+    uint32_t sky_w, sky_h;
+    const float* sky_rgba;
+    float sky_color[3];
+    float sky_rot[9];
+} ora_scene_desc;
+
+/* Camera: row-major 4x4 matrices (m[row*4+col]), as bi_pt_camera. */
+typedef struct ora_camera {
+    float inv_view[16];
+    float inv_proj[16];
+    float proj_view[16];
+    uint32_t width, height;
+} ora_camera;
+
+typedef struct ora_settings {
+    float ray_length;
+    uint32_t max_bounces;   /* reference semantics: clamped to [2,16]; segments = max_bounces-1 */
+    uint32_t seed;
+    uint32_t flags;         /* bit0: jitter primary rays, bit1: disable sky NEE */
+} ora_settings;
+
+ora_ctx* ora_create(const ora_scene_desc* desc);
+void ora_destroy(ora_ctx* ctx);
+uint32_t ora_num_tris(const ora_ctx* ctx);
+
+/* Flattened world-space triangles in instance order: 9 floats (v0,v1,v2),
+ * 9 floats (n0,n1,n2, un-normalised cofactor-transformed), material id. */
+void ora_get_world(const ora_ctx* ctx, float* tri_v, float* tri_n, uint32_t* tri_mat);
+
+/* LBVH of the FP32 transcription.  morton[n] per triangle (input order),
+ * order[n] sorted permutation, children[2*(n-1)] (internal k >= 0, leaf j as ~j),
+ * child_bounds[12*(n-1)]: per node, child0 lo.xyz hi.xyz then child1 lo.xyz hi.xyz,
+ * parents[2n-1]: parent of internal nodes [0,n-1) then of leaves [n-1, 2n-1). */
+void ora_get_bvh(const ora_ctx* ctx, uint32_t* morton, uint32_t* order,
+                 int32_t* children, float* child_bounds, int32_t* parents,
+                 float* scene_centroid_bounds /* 6 */);
+
+/* Batched traversal of the FP32 BVH.  rays: 8 floats each (o.xyz, tmin, d.xyz, tmax).
+ * closest: hit_t, hit_leaf (sorted position, 0xffffffff = miss), hit_u, hit_v.
+ * anyhit (any_hit != 0): hit_leaf[i] = 1 if occluded else 0. */
+void ora_trace(const ora_ctx* ctx, uint32_t n, const float* rays, int any_hit,
+               float* hit_t, uint32_t* hit_leaf, float* hit_u, float* hit_v);
+/* Brute force FP64 closest hit (all triangles, Moller-Trumbore in double). */
+void ora_trace_brute64(const ora_ctx* ctx, uint32_t n, const float* rays,
+                       double* hit_t, uint32_t* hit_tri /* input order */);
+
+/* FP32 render of samples [sample_base, sample_base+spp) for every pixel of the
+ * camera.  sum_rgb (3*W*H floats) is accumulated in sample order: the caller
+ * passes the running sums (zeros for a fresh image).
+ * path_segments (optional, spp*W*H bytes, index s_local*npix + q where q is the
+ * pixel's position in path order): number of traced extension segments.
+ * shadow_bits (optional, spp*W*H*16 bytes): per path and segment, bit l set when
+ * a shadow ray toward directional light l was queued, bit 7 for the sky ray.
+ * ray_counts (4): camera, extension, light shadow, sky shadow rays traced. */
+int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st,
+                 uint32_t sample_base, uint32_t spp, float* sum_rgb,
+                 uint8_t* path_segments, uint8_t* shadow_bits,
+                 uint64_t* ray_counts, int num_threads);
+
+/* Path-order pixel index of path-order position q (tile order when W and H are
+ * multiples of 8, else row-major). */
+uint32_t ora_pixel_of(uint32_t q, uint32_t width, uint32_t height);
+
+/* FP64 oracle: for each listed pixel, mean and variance of the per-sample
+ * radiance over samples [0, spp) (RNG seed seed64).  mean_rgb/var_rgb: 3 per pixel. */
+int ora_render64(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st,
+                 uint32_t num_pixels, const uint32_t* pixels, uint32_t spp, uint32_t seed64,
+                 double* mean_rgb, double* var_rgb, int num_threads);
+
+/* ---- unit-level entry points (for known-answer and invariant tests) ---- */
+void ora_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out);
+void ora_pcg32_keyed(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
+                     uint32_t n, float* out);
+uint32_t ora_morton3(uint32_t x, uint32_t y, uint32_t z);
+void ora_math32(int fn, uint32_t n, const float* x, const float* y, float* out); /* 0 sin 1 cos 2 acos 3 atan2(x,y) */
+/* Lit BSDF (lit.hlsl) value f(V,L)*cos for a material (16 floats), N,V,L world. */
+void ora_bsdf32(const float* material, const float* n, const float* v, const float* l, float* out3);
+void ora_bsdf64(const float* material, const double* n, const double* v, const double* l, double* out3);
+/* VNDF sample in the local frame: given local v, alpha, (u0,u1) -> local h, pdf of reflected L. */
+void ora_vndf32(const float* v_local, float alpha, float u0, float u1, float* h_out, float* pdf_out);
+double ora_vndf_pdf64(const double* v_local, const double* l_local, double alpha);
+/* Sky: radiance lookup, sample and pdf (FP32 transcription). */
+void ora_sky_eval32(const ora_ctx* ctx, const float* dir, float* rgb);
+void ora_sky_sample32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf);
+float ora_sky_pdf32(const ora_ctx* ctx, const float* dir);
+/* Camera ray for pixel (px,py) and optional jitter (jx,jy in [0,1), 0.5 = centre). */
+void ora_camera_ray32(const ora_camera* cam, uint32_t px, uint32_t py, float jx, float jy,
+                      float* o, float* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,316 @@
+"""ctypes binding of the CPU restatement (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker, never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("num_meshes", C.c_uint32), ("positions", _f32p), ("normals", _f32p), ("indices", _u32p),
+        ("vtx_off", _u32p), ("idx_off", _u32p), ("idx_cnt", _u32p),
+        ("num_instances", C.c_uint32), ("inst_xform", _f32p), ("inst_mesh", _u32p), ("inst_mat", _u32p),
+        ("num_materials", C.c_uint32), ("materials", _f32p),
+        ("num_dir_lights", C.c_uint32), ("dir_lights", _f32p),
+        ("sky_w", C.c_uint32), ("sky_h", C.c_uint32), ("sky_rgba", _f32p),
+        ("sky_color", C.c_float * 3), ("sky_rot", C.c_float * 9),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("inv_view", C.c_float * 16), ("inv_proj", C.c_float * 16), ("proj_view", C.c_float * 16),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("ray_length", C.c_float), ("max_bounces", C.c_uint32), ("seed", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.ora_create.restype = C.c_void_p
+        L.ora_create.argtypes = [C.POINTER(SceneDesc)]
+        L.ora_destroy.argtypes = [C.c_void_p]
+        L.ora_num_tris.restype = C.c_uint32
+        L.ora_num_tris.argtypes = [C.c_void_p]
+        L.ora_get_world.argtypes = [C.c_void_p, _f32p, _f32p, _u32p]
+        L.ora_get_bvh.argtypes = [C.c_void_p, _u32p, _u32p, _i32p, _f32p, _i32p, _f32p]
+        L.ora_trace.argtypes = [C.c_void_p, C.c_uint32, _f32p, C.c_int, _f32p, _u32p, _f32p, _f32p]
+        L.ora_trace_brute64.argtypes = [C.c_void_p, C.c_uint32, _f32p, _f64p, _u32p]
+        L.ora_render32.restype = C.c_int
+        L.ora_render32.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, C.c_uint32,
+                                   _f32p, _u8p, _u8p, _u64p, C.c_int]
+        L.ora_pixel_of.restype = C.c_uint32
+        L.ora_pixel_of.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ora_render64.restype = C.c_int
+        L.ora_render64.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, _u32p,
+                                   C.c_uint32, C.c_uint32, _f64p, _f64p, C.c_int]
+        L.ora_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
+        L.ora_pcg32_keyed.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        L.ora_morton3.restype = C.c_uint32
+        L.ora_morton3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ora_math32.argtypes = [C.c_int, C.c_uint32, _f32p, _f32p, _f32p]
+        L.ora_bsdf32.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p]
+        L.ora_bsdf64.argtypes = [_f32p, _f64p, _f64p, _f64p, _f64p]
+        L.ora_vndf32.argtypes = [_f32p, C.c_float, C.c_float, C.c_float, _f32p, _f32p]
+        L.ora_vndf_pdf64.restype = C.c_double
+        L.ora_vndf_pdf64.argtypes = [_f64p, _f64p, C.c_double]
+        L.ora_sky_eval32.argtypes = [C.c_void_p, _f32p, _f32p]
+        L.ora_sky_sample32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p]
+        L.ora_sky_pdf32.restype = C.c_float
+        L.ora_sky_pdf32.argtypes = [C.c_void_p, _f32p]
+        L.ora_camera_ray32.argtypes = [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                       _f32p, _f32p]
+        _lib = L
+    return _lib
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def camera_struct(cam) -> Camera:
+    c = Camera()
+    C.memmove(C.addressof(c.inv_view), np.ascontiguousarray(cam.inv_view, np.float32).ctypes.data, 64)
+    C.memmove(C.addressof(c.inv_proj), np.ascontiguousarray(cam.inv_proj, np.float32).ctypes.data, 64)
+    C.memmove(C.addressof(c.proj_view), np.ascontiguousarray(cam.proj_view, np.float32).ctypes.data, 64)
+    c.width, c.height = cam.width, cam.height
+    return c
+
+
+def settings_struct(max_bounces, seed, ray_length=1000.0, flags=0) -> Settings:
+    return Settings(ray_length, max_bounces, seed, flags)
+
+
+class OracleScene:
+    """A scenes.Scene loaded into the CPU restatement."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        pos = [np.ascontiguousarray(m[0], np.float32) for m in scene.meshes]
+        nrm = [np.ascontiguousarray(m[1], np.float32) for m in scene.meshes]
+        idx = [np.ascontiguousarray(m[2], np.uint32).reshape(-1) for m in scene.meshes]
+        self._keep = dict(
+            positions=np.concatenate(pos), normals=np.concatenate(nrm), indices=np.concatenate(idx),
+            vtx_off=np.array(np.cumsum([0] + [len(p) for p in pos[:-1]]), np.uint32),
+            idx_off=np.array(np.cumsum([0] + [len(i) for i in idx[:-1]]), np.uint32),
+            idx_cnt=np.array([len(i) for i in idx], np.uint32),
+            inst_xform=np.ascontiguousarray(scene.instances["xform"], np.float32),
+            inst_mesh=np.ascontiguousarray(scene.instances["mesh"], np.uint32),
+            inst_mat=np.ascontiguousarray(scene.instances["material"], np.uint32),
+            materials=np.ascontiguousarray(scene.materials, np.float32),
+            dir_lights=np.ascontiguousarray(scene.dir_lights, np.float32),
+            sky=np.ascontiguousarray(scene.sky_rgba, np.float32),
+        )
+        k = self._keep
+        d = SceneDesc()
+        d.num_meshes = len(pos)
+        d.positions, d.normals, d.indices = ptr(k["positions"], _f32p), ptr(k["normals"], _f32p), ptr(k["indices"], _u32p)
+        d.vtx_off, d.idx_off, d.idx_cnt = ptr(k["vtx_off"], _u32p), ptr(k["idx_off"], _u32p), ptr(k["idx_cnt"], _u32p)
+        d.num_instances = len(scene.instances)
+        d.inst_xform, d.inst_mesh, d.inst_mat = ptr(k["inst_xform"], _f32p), ptr(k["inst_mesh"], _u32p), ptr(k["inst_mat"], _u32p)
+        d.num_materials = len(scene.materials)
+        d.materials = ptr(k["materials"], _f32p)
+        d.num_dir_lights = len(scene.dir_lights)
+        d.dir_lights = ptr(k["dir_lights"], _f32p)
+        d.sky_h, d.sky_w = scene.sky_rgba.shape[0], scene.sky_rgba.shape[1]
+        d.sky_rgba = ptr(k["sky"], _f32p)
+        for i in range(3):
+            d.sky_color[i] = float(scene.sky_color[i])
+        for i in range(9):
+            d.sky_rot[i] = float(scene.sky_rot[i])
+        self.h = lib().ora_create(C.byref(d))
+        self.n = lib().ora_num_tris(self.h)
+
+    def close(self):
+        if self.h:
+            lib().ora_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def world(self):
+        v = np.zeros((self.n, 9), np.float32)
+        nn = np.zeros((self.n, 9), np.float32)
+        m = np.zeros(self.n, np.uint32)
+        lib().ora_get_world(self.h, ptr(v, _f32p), ptr(nn, _f32p), ptr(m, _u32p))
+        return v, nn, m
+
+    def bvh(self):
+        n = self.n
+        ni = max(n - 1, 1)
+        morton = np.zeros(n, np.uint32)
+        order = np.zeros(n, np.uint32)
+        children = np.zeros((ni, 2), np.int32)
+        bounds = np.zeros((ni, 12), np.float32)
+        parents = np.zeros(ni + n, np.int32)
+        cb = np.zeros(6, np.float32)
+        lib().ora_get_bvh(self.h, ptr(morton, _u32p), ptr(order, _u32p), ptr(children, _i32p),
+                          ptr(bounds, _f32p), ptr(parents, _i32p), ptr(cb, _f32p))
+        return dict(morton=morton, order=order, children=children, bounds=bounds, parents=parents,
+                    centroid_bounds=cb)
+
+    def trace(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        leaf = np.zeros(n, np.uint32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        lib().ora_trace(self.h, n, ptr(rays, _f32p), int(any_hit), ptr(t, _f32p), ptr(leaf, _u32p),
+                        ptr(u, _f32p), ptr(v, _f32p))
+        return (leaf,) if any_hit else (t, leaf, u, v)
+
+    def trace_brute64(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float64)
+        tri = np.zeros(n, np.uint32)
+        lib().ora_trace_brute64(self.h, n, ptr(rays, _f32p), ptr(t, _f64p), ptr(tri, _u32p))
+        return t, tri
+
+    def render32(self, cam, max_bounces, seed, spp, sample_base=0, sums=None, ray_length=1000.0, flags=0,
+                 record=False, threads=0):
+        npix = cam.width * cam.height
+        if sums is None:
+            sums = np.zeros((npix, 3), np.float32)
+        segs = np.zeros(spp * npix, np.uint8) if record else None
+        bits = np.zeros((spp * npix, 16), np.uint8) if record else None
+        counts = np.zeros(4, np.uint64)
+        c = camera_struct(cam)
+        s = settings_struct(max_bounces, seed, ray_length, flags)
+        rc = lib().ora_render32(self.h, C.byref(c), C.byref(s), sample_base, spp, ptr(sums, _f32p),
+                                ptr(segs, _u8p), ptr(bits, _u8p), ptr(counts, _u64p), threads)
+        assert rc == 0
+        out = dict(sums=sums, counts=counts)
+        if record:
+            out["segments"] = segs
+            out["shadow_bits"] = bits
+        return out
+
+    def render64(self, cam, max_bounces, pixels, spp, seed64=0x64, ray_length=1000.0, flags=0, threads=0):
+        pixels = np.ascontiguousarray(pixels, np.uint32)
+        mean = np.zeros((len(pixels), 3), np.float64)
+        var = np.zeros((len(pixels), 3), np.float64)
+        c = camera_struct(cam)
+        s = settings_struct(max_bounces, 0, ray_length, flags)
+        rc = lib().ora_render64(self.h, C.byref(c), C.byref(s), len(pixels), ptr(pixels, _u32p), spp, seed64,
+                                ptr(mean, _f64p), ptr(var, _f64p), threads)
+        assert rc == 0
+        return mean, var
+
+    def sky_eval(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(3, np.float32)
+        lib().ora_sky_eval32(self.h, ptr(d, _f32p), ptr(out, _f32p))
+        return out
+
+    def sky_sample(self, u0, u1):
+        d = np.zeros(3, np.float32)
+        p = np.zeros(1, np.float32)
+        lib().ora_sky_sample32(self.h, u0, u1, ptr(d, _f32p), ptr(p, _f32p))
+        return d, float(p[0])
+
+    def sky_pdf(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        return float(lib().ora_sky_pdf32(self.h, ptr(d, _f32p)))
+
+
+def pcg32(initstate, initseq, n):
+    out = np.zeros(n, np.uint32)
+    lib().ora_pcg32(initstate, initseq, n, ptr(out, _u32p))
+    return out
+
+
+def pcg32_keyed(seed, pixel, sample, bounce, n):
+    out = np.zeros(n, np.float32)
+    lib().ora_pcg32_keyed(seed, pixel, sample, bounce, n, ptr(out, _f32p))
+    return out
+
+
+def morton3(x, y, z):
+    return lib().ora_morton3(x, y, z)
+
+
+def math32(fn, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y if y is not None else np.zeros_like(x), np.float32)
+    out = np.zeros_like(x)
+    lib().ora_math32(fn, len(x), ptr(x, _f32p), ptr(y, _f32p), ptr(out, _f32p))
+    return out
+
+
+def bsdf32(mat, n, v, l):
+    out = np.zeros(3, np.float32)
+    args = [np.ascontiguousarray(a, np.float32) for a in (mat, n, v, l)]
+    lib().ora_bsdf32(*[ptr(a, _f32p) for a in args], ptr(out, _f32p))
+    return out
+
+
+def bsdf64(mat, n, v, l):
+    out = np.zeros(3, np.float64)
+    m = np.ascontiguousarray(mat, np.float32)
+    args = [np.ascontiguousarray(a, np.float64) for a in (n, v, l)]
+    lib().ora_bsdf64(ptr(m, _f32p), *[ptr(a, _f64p) for a in args], ptr(out, _f64p))
+    return out
+
+
+def vndf32(v_local, alpha, u0, u1):
+    v = np.ascontiguousarray(v_local, np.float32)
+    h = np.zeros(3, np.float32)
+    p = np.zeros(1, np.float32)
+    lib().ora_vndf32(ptr(v, _f32p), alpha, u0, u1, ptr(h, _f32p), ptr(p, _f32p))
+    return h, float(p[0])
+
+
+def vndf_pdf64(v_local, l_local, alpha):
+    v = np.ascontiguousarray(v_local, np.float64)
+    l = np.ascontiguousarray(l_local, np.float64)
+    return float(lib().ora_vndf_pdf64(ptr(v, _f64p), ptr(l, _f64p), alpha))
+
+
+def camera_ray32(cam, px, py, jx=0.5, jy=0.5):
+    c = camera_struct(cam)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().ora_camera_ray32(C.byref(c), px, py, jx, jy, ptr(o, _f32p), ptr(d, _f32p))
+    return o, d
+
+
+def pixel_order(width, height):
+    """Pixel index of each path-order position (q -> pixel)."""
+    L = lib()
+    return np.array([L.ora_pixel_of(q, width, height) for q in range(width * height)], np.uint32)

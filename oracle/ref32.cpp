@@ -1,0 +1,956 @@
+// ref32.cpp -- scalar FP32 transcription of the wavefront path-tracing pass.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).  Built with -O3 -ffp-contract=off and
+// no fast-math: every float operation below rounds once, in the order written,
+// exactly as the HIP kernels do, so the GPU is compared with this file bit for bit.
+//
+// What each part restates (reference paths relative to bisemutum/):
+//   camera_ray        shaders/renderer/raytracing/direction_sample/generate_camera_ray.hlsl:9-15,
+//                     camera_position_world shaders/core/shader_params/camera.hlsl:7-9
+//   trace_closest     rt_gbuffer.hlsl:17-35 (TMin 0.001, TMax ray_length, closest hit).  The
+//                     reference's BVH lives in the GPU driver; the LBVH here is the build's own
+//                     (Karras 2012) and has no reference counterpart.
+//   shade_surface     hits/rt_gbuffer_hit.hlsl:6-17 + core/raytracing/hit.hlsl:75-81,155-156
+//                     (interpolated normal, two-sided flip), lit.hlsl:5-35 (BSDF),
+//                     material/utils.hlsl:40-121 (Fresnel, GGX, VNDF), frame.hlsl:9-20
+//   light loop        deferred_lighting_secondary.hlsl:45-60, lights.hlsl:10-13,69-101
+//   sky               deferred_lighting_secondary.hlsl:24-29, equitangular_to_cubemap.hlsl:28-33
+//   bounce loop       path_tracing.cpp:289-460, sample_secondary_ray.hlsl:17-68
+//   accumulation      pt_accumulate.hlsl:9 as a running FP32 sum over samples
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "ref_scene.hpp"
+
+namespace ora {
+
+// ---------------------------------------------------------------- float math
+float sin32(float x);
+float cos32(float x);
+
+namespace {
+
+constexpr float kFourOverPi = 1.27323954473516268615f;
+// Cody-Waite split of pi/4 (Cephes sinf.c, S. L. Moshier)
+constexpr float kDP1 = 0.78515625f;
+constexpr float kDP2 = 2.4187564849853515625e-4f;
+constexpr float kDP3 = 3.77489497744594108e-8f;
+
+// Cephes single-precision kernels on |x| <= pi/4.
+inline float sin_kernel(float x, float z) {
+    return ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+}
+inline float cos_kernel(float z) {
+    return ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z
+        - 0.5f * z + 1.0f;
+}
+
+// Reduce |x| to octant j in {0..3} (after folding) and remainder r.
+inline float reduce_octant(float ax, int& j) {
+    int q = (int)(ax * kFourOverPi);
+    float y = (float)q;
+    if (q & 1) { q += 1; y += 1.0f; }
+    j = q & 7;
+    return ((ax - y * kDP1) - y * kDP2) - y * kDP3;
+}
+
+}  // namespace
+
+void sincos32(float x, float* s_out, float* c_out) {
+    float ax = x < 0.0f ? -x : x;
+    int j;
+    float r = reduce_octant(ax, j);
+    float z = r * r;
+    float sk = sin_kernel(r, z);
+    float ck = cos_kernel(z);
+    float s, c;
+    int sign_s = x < 0.0f ? -1 : 1;
+    int sign_c = 1;
+    if (j > 3) { j -= 4; sign_s = -sign_s; sign_c = -sign_c; }
+    if (j > 1) sign_c = -sign_c;
+    if (j == 1 || j == 2) { s = ck; c = sk; } else { s = sk; c = ck; }
+    *s_out = sign_s < 0 ? -s : s;
+    *c_out = sign_c < 0 ? -c : c;
+}
+float sin32(float x) { float s, c; sincos32(x, &s, &c); return s; }
+float cos32(float x) { float s, c; sincos32(x, &s, &c); return c; }
+
+namespace {
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kPiO2F = 1.57079632679489661923f;
+constexpr float kPiO4F = 0.785398163397448309616f;
+}
+
+// Cephes asinf.c / acosf.c
+float asin32(float x) {
+    float a = x < 0.0f ? -x : x;
+    float z, s;
+    bool big = a > 0.5f;
+    if (big) { z = 0.5f * (1.0f - a); s = std::sqrt(z); }
+    else { s = a; z = a * a; }
+    float p = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z
+                + 7.4953002686e-2f) * z + 1.6666752422e-1f) * z * s + s;
+    if (big) { p = p + p; p = kPiO2F - p; }
+    return x < 0.0f ? -p : p;
+}
+float acos32(float x) {
+    if (x < -0.5f) return kPiF - 2.0f * asin32(std::sqrt(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * asin32(std::sqrt(0.5f * (1.0f - x)));
+    return kPiO2F - asin32(x);
+}
+// Cephes atanf.c
+float atan32(float x) {
+    float a = x < 0.0f ? -x : x;
+    float y0;
+    if (a > 2.414213562373095f) { y0 = kPiO2F; a = -(1.0f / a); }
+    else if (a > 0.4142135623730950f) { y0 = kPiO4F; a = (a - 1.0f) / (a + 1.0f); }
+    else { y0 = 0.0f; }
+    float z = a * a;
+    float r = y0 + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z
+                     - 3.33329491539e-1f) * z * a + a);
+    return x < 0.0f ? -r : r;
+}
+float atan2_32(float y, float x) {
+    if (x == 0.0f) {
+        if (y > 0.0f) return kPiO2F;
+        if (y < 0.0f) return -kPiO2F;
+        return 0.0f;
+    }
+    float r = atan32(y / x);
+    if (x < 0.0f) r = (y < 0.0f) ? r - kPiF : r + kPiF;
+    return r;
+}
+
+namespace {
+
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr float kInvTwoPi = 0.159154943091895335769f;
+constexpr float kTwoPiSq = 19.7392088021787172376f;
+constexpr float kTMin = 0.001f;
+constexpr float kRobust = 1.00000036f;   // 1 + 2*gamma(3), Ize 2013 robust slab test
+constexpr float kThroughputMin = 0.001f; // sample_secondary_ray.hlsl:24
+
+struct F3 { float x, y, z; };
+inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline F3 operator*(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+inline F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
+inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 cross(F3 a, F3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline F3 normalize(F3 v) { float inv = 1.0f / std::sqrt(dot(v, v)); return v * inv; }
+inline float fmax_(float a, float b) { return a > b ? a : b; }
+inline float fmin_(float a, float b) { return a < b ? a : b; }
+inline float max3(F3 v) { return fmax_(fmax_(v.x, v.y), v.z); }
+inline bool any_pos(F3 v) { return v.x > 0.0f || v.y > 0.0f || v.z > 0.0f; }
+inline bool finite3(F3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
+// ---------------------------------------------------------------- PCG32
+// O'Neill 2014, pcg32_random_r / pcg32_srandom_r.
+struct Pcg { uint64_t state, inc; };
+inline uint32_t pcg_next(Pcg& r) {
+    uint64_t old = r.state;
+    r.state = old * 6364136223846793005ULL + r.inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+inline Pcg pcg_seed(uint64_t initstate, uint64_t initseq) {
+    Pcg r{0u, (initseq << 1u) | 1u};
+    pcg_next(r);
+    r.state += initstate;
+    pcg_next(r);
+    return r;
+}
+// Stream keyed by (seed, pixel, sample, bounce).
+inline Pcg pcg_keyed(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce) {
+    return pcg_seed(((uint64_t)pixel << 32) | sample, ((uint64_t)seed << 8) | (bounce & 0xffu));
+}
+inline float pcg_float(Pcg& r) { return (float)(pcg_next(r) >> 8) * 0x1p-24f; }
+constexpr uint32_t kJitterBounce = 255u;
+
+}  // namespace
+}  // namespace ora
+
+// ====================================================================== scene
+namespace ora {
+
+uint32_t expand10(uint32_t v) {
+    v &= 1023u;
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+}
+
+namespace {
+
+inline int clz32(uint32_t v) { return v == 0 ? 32 : __builtin_clz(v); }
+
+// Karras 2012 delta: common prefix of (code, index) keys; -1 outside [0, n).
+inline int delta(const uint32_t* codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = codes[i], b = codes[j];
+    if (a == b) return 32 + clz32((uint32_t)i ^ (uint32_t)j);
+    return clz32(a ^ b);
+}
+
+}  // namespace
+
+void build_bvh(Scene& s) {
+    const uint32_t n = s.num_tris;
+    s.morton.assign(n, 0);
+    s.order.assign(n, 0);
+    s.leaf_tri.assign(n, 0);
+    if (n == 0) { s.nodes.clear(); return; }
+    // triangle bounds and centroid bounds (min/max are exact: order free)
+    std::vector<float> lo(3 * n), hi(3 * n);
+    float cmin[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, cmax[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t t = 0; t < n; ++t) {
+        const float* v = &s.tri_v[9 * t];
+        for (int k = 0; k < 3; ++k) {
+            float a = v[k], b = v[3 + k], c = v[6 + k];
+            float l = fmin_(fmin_(a, b), c), h = fmax_(fmax_(a, b), c);
+            lo[3 * t + k] = l; hi[3 * t + k] = h;
+            float ce = (l + h) * 0.5f;
+            cmin[k] = fmin_(cmin[k], ce); cmax[k] = fmax_(cmax[k], ce);
+        }
+    }
+    for (int k = 0; k < 3; ++k) { s.cbounds[k] = cmin[k]; s.cbounds[3 + k] = cmax[k]; }
+    float inv_ext[3];
+    for (int k = 0; k < 3; ++k) {
+        float e = cmax[k] - cmin[k];
+        inv_ext[k] = e > 0.0f ? 1.0f / e : 0.0f;
+    }
+    for (uint32_t t = 0; t < n; ++t) {
+        uint32_t q[3];
+        for (int k = 0; k < 3; ++k) {
+            float ce = (lo[3 * t + k] + hi[3 * t + k]) * 0.5f;
+            float f = (ce - cmin[k]) * inv_ext[k] * 1024.0f;
+            int qi = (int)f;
+            q[k] = (uint32_t)(qi < 0 ? 0 : (qi > 1023 ? 1023 : qi));
+        }
+        s.morton[t] = morton3(q[0], q[1], q[2]);
+    }
+    for (uint32_t t = 0; t < n; ++t) s.order[t] = t;
+    std::stable_sort(s.order.begin(), s.order.end(),
+                     [&](uint32_t a, uint32_t b) { return s.morton[a] < s.morton[b]; });
+    std::vector<uint32_t> codes(n);
+    for (uint32_t k = 0; k < n; ++k) { codes[k] = s.morton[s.order[k]]; s.leaf_tri[k] = s.order[k]; }
+
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    s.nodes.assign(ni, Node{});
+    s.parent.assign(ni + n, -1);
+    if (n == 1) {
+        Node& nd = s.nodes[0];
+        nd.child[0] = ~0; nd.child[1] = ~0;
+        s.parent[ni + 0] = 0;
+    } else {
+        const int N = (int)n;
+        for (int i = 0; i < N - 1; ++i) {
+            int d = (delta(codes.data(), N, i, i + 1) - delta(codes.data(), N, i, i - 1)) >= 0 ? 1 : -1;
+            int dmin = delta(codes.data(), N, i, i - d);
+            int lmax = 2;
+            while (delta(codes.data(), N, i, i + lmax * d) > dmin) lmax *= 2;
+            int l = 0;
+            for (int t = lmax / 2; t >= 1; t /= 2)
+                if (delta(codes.data(), N, i, i + (l + t) * d) > dmin) l += t;
+            int j = i + l * d;
+            int dnode = delta(codes.data(), N, i, j);
+            int sp = 0;
+            int t = l;
+            do {
+                t = (t + 1) / 2;
+                if (delta(codes.data(), N, i, i + (sp + t) * d) > dnode) sp += t;
+            } while (t > 1);
+            int gamma = i + sp * d + (d < 0 ? -1 : 0);
+            int left = (std::min(i, j) == gamma) ? ~gamma : gamma;
+            int right = (std::max(i, j) == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+            s.nodes[i].child[0] = left;
+            s.nodes[i].child[1] = right;
+            s.parent[left < 0 ? (N - 1) + ~left : left] = i;
+            s.parent[right < 0 ? (N - 1) + ~right : right] = i;
+        }
+    }
+    // refit (post-order; min/max exact)
+    std::vector<float> nb(6 * ni);
+    auto leaf_box = [&](uint32_t leaf, float* b) {
+        uint32_t t = s.leaf_tri[leaf];
+        for (int k = 0; k < 3; ++k) { b[k] = lo[3 * t + k]; b[3 + k] = hi[3 * t + k]; }
+    };
+    // iterative post-order over internal nodes
+    std::vector<int> stack;
+    std::vector<uint8_t> visited(ni, 0);
+    stack.push_back(0);
+    while (!stack.empty()) {
+        int nd = stack.back();
+        bool ready = true;
+        for (int c = 0; c < 2; ++c) {
+            int ch = s.nodes[nd].child[c];
+            if (ch >= 0 && !visited[ch]) { stack.push_back(ch); ready = false; }
+        }
+        if (!ready) continue;
+        stack.pop_back();
+        if (visited[nd]) continue;
+        visited[nd] = 1;
+        float box[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int c = 0; c < 2; ++c) {
+            int ch = s.nodes[nd].child[c];
+            float cb[6];
+            if (ch < 0) leaf_box((uint32_t)~ch, cb);
+            else std::memcpy(cb, &nb[6 * ch], sizeof cb);
+            std::memcpy(s.nodes[nd].box[c], cb, sizeof cb);
+            for (int k = 0; k < 3; ++k) {
+                box[k] = fmin_(box[k], cb[k]);
+                box[3 + k] = fmax_(box[3 + k], cb[3 + k]);
+            }
+        }
+        std::memcpy(&nb[6 * nd], box, sizeof box);
+    }
+    // intersection records in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0
+    s.leaf_v0.resize(3 * n); s.leaf_e1.resize(3 * n); s.leaf_e2.resize(3 * n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const float* v = &s.tri_v[9 * s.leaf_tri[k]];
+        for (int a = 0; a < 3; ++a) {
+            s.leaf_v0[3 * k + a] = v[a];
+            s.leaf_e1[3 * k + a] = v[3 + a] - v[a];
+            s.leaf_e2[3 * k + a] = v[6 + a] - v[a];
+        }
+    }
+}
+
+}  // namespace ora
+
+// ================================================================ traversal
+namespace ora {
+namespace {
+
+struct Ray { F3 o, d; float tmin, tmax; };
+struct Hit { float t; uint32_t leaf; float u, v; };
+constexpr uint32_t kMiss = 0xffffffffu;
+constexpr int kStackMax = 64;
+
+inline F3 safe_inv(F3 d) {
+    auto one = [](float x) {
+        float ax = x < 0.0f ? -x : x;
+        if (ax < 1e-20f) x = x < 0.0f ? -1e-20f : 1e-20f;
+        return 1.0f / x;
+    };
+    return f3(one(d.x), one(d.y), one(d.z));
+}
+
+// Slab test of one child box against [tmin, tmax]; returns entry distance.
+inline bool box_hit(const float* b, F3 o, F3 inv, float tmin, float tmax, float* tnear) {
+    float tx0 = (b[0] - o.x) * inv.x, tx1 = (b[3] - o.x) * inv.x;
+    float ty0 = (b[1] - o.y) * inv.y, ty1 = (b[4] - o.y) * inv.y;
+    float tz0 = (b[2] - o.z) * inv.z, tz1 = (b[5] - o.z) * inv.z;
+    float tn = fmax_(fmax_(fmin_(tx0, tx1), fmin_(ty0, ty1)), fmax_(fmin_(tz0, tz1), tmin));
+    float tf = fmin_(fmin_(fmax_(tx0, tx1), fmax_(ty0, ty1)), fmin_(fmax_(tz0, tz1), tmax));
+    *tnear = tn;
+    return tn <= tf * kRobust;
+}
+
+// Moller-Trumbore with division-free barycentric acceptance; t = tn / det.
+inline bool tri_test(const Scene& s, uint32_t leaf, F3 o, F3 d, float* t_out, float* un_out,
+                     float* vn_out, float* det_out) {
+    F3 v0 = f3(s.leaf_v0[3 * leaf], s.leaf_v0[3 * leaf + 1], s.leaf_v0[3 * leaf + 2]);
+    F3 e1 = f3(s.leaf_e1[3 * leaf], s.leaf_e1[3 * leaf + 1], s.leaf_e1[3 * leaf + 2]);
+    F3 e2 = f3(s.leaf_e2[3 * leaf], s.leaf_e2[3 * leaf + 1], s.leaf_e2[3 * leaf + 2]);
+    F3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (det == 0.0f) return false;
+    F3 tv = o - v0;
+    float un = dot(tv, pv);
+    F3 qv = cross(tv, e1);
+    float vn = dot(d, qv);
+    if (det > 0.0f) {
+        if (un < 0.0f || vn < 0.0f || un + vn > det) return false;
+    } else {
+        if (un > 0.0f || vn > 0.0f || un + vn < det) return false;
+    }
+    float tn = dot(e2, qv);
+    *t_out = tn / det;
+    *un_out = un; *vn_out = vn; *det_out = det;
+    return true;
+}
+
+Hit trace_closest(const Scene& s, const Ray& r) {
+    Hit h{r.tmax, kMiss, 0.0f, 0.0f};
+    if (s.num_tris == 0) return h;
+    F3 inv = safe_inv(r.d);
+    float best_un = 0.0f, best_vn = 0.0f, best_det = 1.0f;
+    int stack[kStackMax];
+    int sp = 0;
+    int node = 0;
+    for (;;) {
+        if (node < 0) {
+            uint32_t leaf = (uint32_t)~node;
+            float t, un, vn, det;
+            if (tri_test(s, leaf, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin &&
+                (t < h.t || (t == h.t && leaf < h.leaf))) {
+                h.t = t; h.leaf = leaf; best_un = un; best_vn = vn; best_det = det;
+            }
+            if (sp == 0) break;
+            node = stack[--sp];
+            continue;
+        }
+        const Node& nd = s.nodes[node];
+        float tn0, tn1;
+        bool h0 = box_hit(nd.box[0], r.o, inv, r.tmin, h.t, &tn0);
+        bool h1 = box_hit(nd.box[1], r.o, inv, r.tmin, h.t, &tn1);
+        if (h0 && h1) {
+            int nearc = tn0 <= tn1 ? nd.child[0] : nd.child[1];
+            int farc = tn0 <= tn1 ? nd.child[1] : nd.child[0];
+            if (sp >= kStackMax) break;  // cannot happen for n < 2^34 (depth bound)
+            stack[sp++] = farc;
+            node = nearc;
+        } else if (h0) {
+            node = nd.child[0];
+        } else if (h1) {
+            node = nd.child[1];
+        } else {
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    if (h.leaf != kMiss) { h.u = best_un / best_det; h.v = best_vn / best_det; }
+    return h;
+}
+
+bool trace_any(const Scene& s, const Ray& r) {
+    if (s.num_tris == 0) return false;
+    F3 inv = safe_inv(r.d);
+    int stack[kStackMax];
+    int sp = 0;
+    int node = 0;
+    for (;;) {
+        if (node < 0) {
+            float t, un, vn, det;
+            if (tri_test(s, (uint32_t)~node, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin &&
+                t <= r.tmax)
+                return true;
+            if (sp == 0) return false;
+            node = stack[--sp];
+            continue;
+        }
+        const Node& nd = s.nodes[node];
+        float tn0, tn1;
+        bool h0 = box_hit(nd.box[0], r.o, inv, r.tmin, r.tmax, &tn0);
+        bool h1 = box_hit(nd.box[1], r.o, inv, r.tmin, r.tmax, &tn1);
+        if (h0 && h1) {
+            int nearc = tn0 <= tn1 ? nd.child[0] : nd.child[1];
+            int farc = tn0 <= tn1 ? nd.child[1] : nd.child[0];
+            if (sp >= kStackMax) return false;
+            stack[sp++] = farc;
+            node = nearc;
+        } else if (h0) {
+            node = nd.child[0];
+        } else if (h1) {
+            node = nd.child[1];
+        } else {
+            if (sp == 0) return false;
+            node = stack[--sp];
+        }
+    }
+}
+
+// ------------------------------------------------------------------- camera
+void camera_ray(const ora_camera& c, uint32_t px, uint32_t py, float jx, float jy, F3* o, F3* d) {
+    float u = ((float)px + jx) / (float)c.width;
+    float v = ((float)py + jy) / (float)c.height;
+    float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
+    const float* ip = c.inv_proj;
+    F3 q = f3(((ip[0] * nx + ip[1] * ny) + ip[2]) + ip[3],
+              ((ip[4] * nx + ip[5] * ny) + ip[6]) + ip[7],
+              ((ip[8] * nx + ip[9] * ny) + ip[10]) + ip[11]);
+    F3 dl = normalize(q);
+    const float* iv = c.inv_view;
+    F3 dw = f3(iv[0] * dl.x + iv[1] * dl.y + iv[2] * dl.z,
+               iv[4] * dl.x + iv[5] * dl.y + iv[6] * dl.z,
+               iv[8] * dl.x + iv[9] * dl.y + iv[10] * dl.z);
+    *d = normalize(dw);
+    *o = f3(iv[3], iv[7], iv[11]);
+}
+
+// ------------------------------------------------------------------ material
+struct Mat {
+    F3 base, f0, f90;
+    float rx, ry;
+    bool two_sided;
+};
+inline Mat load_mat(const float* m) {
+    Mat r;
+    r.base = f3(m[0], m[1], m[2]);
+    float rough = m[3];
+    r.f0 = f3(m[4], m[5], m[6]);
+    float aniso_p = m[7];
+    r.f90 = f3(m[8], m[9], m[10]);
+    r.two_sided = m[12] != 0.0f;
+    // get_anisotropic_roughness (utils.hlsl:54-59)
+    float aniso = std::sqrt(1.0f - aniso_p * 0.9f);
+    float r2 = rough * rough;
+    r.rx = fmax_(r2 / aniso, 0.001f);
+    r.ry = fmax_(r2 * aniso, 0.001f);
+    return r;
+}
+
+struct Frame { F3 t, b, n; };
+// create_frame(normal), frame.hlsl:9-20 (Duff et al. 2017)
+inline Frame make_frame(F3 n) {
+    float sign = n.z > 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    Frame f;
+    f.t = f3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f.b = f3(b, sign + n.y * n.y * a, -n.y);
+    f.n = n;
+    return f;
+}
+inline F3 to_local(const Frame& f, F3 v) { return f3(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+inline F3 to_world(const Frame& f, F3 v) { return (f.t * v.x + f.b * v.y) + f.n * v.z; }
+
+inline float sq(float x) { return x * x; }
+inline float ggx_ndf(F3 h, float rx, float ry) {
+    float a = (sq(h.x / rx) + sq(h.y / ry)) + sq(h.z);
+    return kInvPi / (((rx * ry) * a) * a);
+}
+inline float ggx_g1(F3 v, float rx, float ry) {
+    float a = (sq(rx * v.x) + sq(ry * v.y)) / fmax_(v.z * v.z, 0.0001f);
+    return 2.0f / (1.0f + std::sqrt(1.0f + a));
+}
+inline float ggx_vis_hc(F3 v, F3 l, float rx, float ry) {
+    float a = l.z * std::sqrt((sq(rx * v.x) + sq(ry * v.y)) + sq(v.z));
+    float b = v.z * std::sqrt((sq(rx * l.x) + sq(ry * l.y)) + sq(l.z));
+    return 0.5f / fmax_(a + b, 0.0001f);
+}
+
+// surface_eval_lit (lit.hlsl:5-35): diffuse + specular, both including cos(theta_l).
+F3 bsdf_eval(const Frame& fr, const Mat& m, F3 V, F3 L) {
+    F3 H = normalize(V + L);
+    F3 lh = to_local(fr, H), lv = to_local(fr, V), ll = to_local(fr, L);
+    if (lv.z <= 0.0f || ll.z <= 0.0f) return f3(0, 0, 0);
+    float vh = dot(V, H);
+    float c = fmax_(vh, 0.0f);
+    float x = 1.0f - c;
+    float p5 = ((x * x) * (x * x)) * x;
+    F3 fres = m.f0 + (m.f90 - m.f0) * p5;
+    float lz = fmax_(ll.z, 0.0f);
+    F3 one = f3(1.0f, 1.0f, 1.0f);
+    F3 diffuse = (((one - fres) * m.base) * kInvPi) * lz;
+    float ndf = ggx_ndf(lh, m.rx, m.ry);
+    float vis = ggx_vis_hc(lv, ll, m.rx, m.ry);
+    F3 spec = ((fres * ndf) * vis) * lz;
+    return diffuse + spec;
+}
+
+// ggx_vndf_sample (utils.hlsl:98-116)
+F3 vndf_sample(F3 v, float rx, float ry, float u0, float u1) {
+    if (v.z < 0.0f) v = -v;
+    F3 vh = normalize(f3(rx * v.x, ry * v.y, v.z));
+    float len_sqr = vh.x * vh.x + vh.y * vh.y;
+    F3 t1v;
+    if (len_sqr > 0.0f) {
+        float s = std::sqrt(len_sqr);
+        t1v = f3(-vh.y / s, vh.x / s, 0.0f / s);
+    } else {
+        t1v = f3(1.0f, 0.0f, 0.0f);
+    }
+    F3 t2v = cross(vh, t1v);
+    float r = std::sqrt(u0);
+    float phi = kTwoPi * u1;
+    float sp, cp;
+    sincos32(phi, &sp, &cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
+    float s = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - s) * std::sqrt(1.0f - t1 * t1) + s * t2;
+    F3 nh = (t1v * t1 + t2v * t2) + vh * std::sqrt(fmax_((1.0f - t1 * t1) - t2 * t2, 0.0f));
+    return normalize(f3(rx * nh.x, ry * nh.y, fmax_(nh.z, 0.0f)));
+}
+// ggx_vndf_sample_pdf (utils.hlsl:118-121) divided by 4|h.v| (sample_secondary_ray.hlsl:57)
+inline float vndf_pdf_l(F3 h, F3 v, float rx, float ry) {
+    float hv = dot(h, v);
+    float pdf_h = ((ggx_g1(v, rx, ry) * ggx_ndf(h, rx, ry)) * fmax_(hv, 0.0f)) / fmax_(v.z, 0.0001f);
+    float ahv = hv < 0.0f ? -hv : hv;
+    return pdf_h / (4.0f * ahv);
+}
+
+// ---------------------------------------------------------------------- sky
+inline F3 sky_to_local(const Scene& s, F3 d) {
+    const float* R = s.sky_rot;
+    return f3((R[0] * d.x + R[1] * d.y) + R[2] * d.z,
+              (R[3] * d.x + R[4] * d.y) + R[5] * d.z,
+              (R[6] * d.x + R[7] * d.y) + R[8] * d.z);
+}
+inline void sky_uv(F3 ds, float* u, float* v, float* theta) {
+    float y = fmin_(fmax_(ds.y, -1.0f), 1.0f);
+    float th = acos32(y);
+    float ph = ds.z == 0.0f ? 0.0f : atan2_32(ds.x, ds.z);
+    if (ph < 0.0f) ph += kTwoPi;
+    *u = ph * kInvTwoPi;
+    *v = th * kInvPi;
+    *theta = th;
+}
+F3 sky_eval(const Scene& s, F3 d) {
+    if (s.sky_w == 0) return f3(0, 0, 0);
+    float u, v, th;
+    sky_uv(sky_to_local(s, d), &u, &v, &th);
+    const int W = (int)s.sky_w, H = (int)s.sky_h;
+    float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    float fx0 = std::floor(x), fy0 = std::floor(y);
+    float fx = x - fx0, fy = y - fy0;
+    int ix0 = (int)fx0, iy0 = (int)fy0;
+    int ix1 = ix0 + 1, iy1 = iy0 + 1;
+    ix0 = ix0 < 0 ? ix0 + W : (ix0 >= W ? ix0 - W : ix0);
+    ix1 = ix1 < 0 ? ix1 + W : (ix1 >= W ? ix1 - W : ix1);
+    iy0 = iy0 < 0 ? 0 : (iy0 > H - 1 ? H - 1 : iy0);
+    iy1 = iy1 < 0 ? 0 : (iy1 > H - 1 ? H - 1 : iy1);
+    auto tex = [&](int i, int j) {
+        const float* p = &s.sky_rgba[4 * ((size_t)i * W + j)];
+        return f3(p[0], p[1], p[2]);
+    };
+    F3 c00 = tex(iy0, ix0), c10 = tex(iy0, ix1), c01 = tex(iy1, ix0), c11 = tex(iy1, ix1);
+    F3 top = c00 + (c10 - c00) * fx;
+    F3 bot = c01 + (c11 - c01) * fx;
+    F3 c = top + (bot - top) * fy;
+    return c * f3(s.sky_color[0], s.sky_color[1], s.sky_color[2]);
+}
+float sky_pdf(const Scene& s, F3 d) {
+    if (!(s.sky_total > 0.0f)) return 0.0f;
+    float u, v, th;
+    sky_uv(sky_to_local(s, d), &u, &v, &th);
+    const int W = (int)s.sky_w, H = (int)s.sky_h;
+    int j = (int)(u * (float)W), i = (int)(v * (float)H);
+    j = j < 0 ? 0 : (j > W - 1 ? W - 1 : j);
+    i = i < 0 ? 0 : (i > H - 1 ? H - 1 : i);
+    float w = s.sky_w_tex[(size_t)i * W + j];
+    float st = sin32(th);
+    if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
+    return (w * s.sky_wh_over_total) / (kTwoPiSq * st);
+}
+// Piecewise-constant (luminance * sin(theta)) importance sampling of the equirect.
+bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
+    *pdf = 0.0f;
+    if (!(s.sky_total > 0.0f)) return false;
+    const int W = (int)s.sky_w, H = (int)s.sky_h;
+    float t0 = u0 * s.sky_total;
+    int lo = 0, hi = H - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (s.sky_marg[mid + 1] > t0) hi = mid; else lo = mid + 1;
+    }
+    int i = lo;
+    float rs = s.sky_rows[i];
+    if (!(rs > 0.0f)) return false;
+    float dv = (t0 - s.sky_marg[i]) / rs;
+    dv = dv < 0.0f ? 0.0f : (dv >= 1.0f ? 0x1.fffffep-1f : dv);
+    float t1 = u1 * rs;
+    const float* cc = &s.sky_ccdf[(size_t)i * (W + 1)];
+    lo = 0; hi = W - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cc[mid + 1] > t1) hi = mid; else lo = mid + 1;
+    }
+    int j = lo;
+    float w = s.sky_w_tex[(size_t)i * W + j];
+    if (!(w > 0.0f)) return false;
+    float du = (t1 - cc[j]) / w;
+    du = du < 0.0f ? 0.0f : (du >= 1.0f ? 0x1.fffffep-1f : du);
+    float u = ((float)j + du) / (float)W;
+    float v = ((float)i + dv) / (float)H;
+    float st, ct, sp, cp;
+    sincos32(v * kPiF, &st, &ct);
+    sincos32(u * kTwoPi, &sp, &cp);
+    if (!(st > 0.0f)) return false;
+    F3 ds = f3(st * sp, ct, st * cp);
+    const float* R = s.sky_rot;
+    *dir = f3((R[0] * ds.x + R[3] * ds.y) + R[6] * ds.z,
+              (R[1] * ds.x + R[4] * ds.y) + R[7] * ds.z,
+              (R[2] * ds.x + R[5] * ds.y) + R[8] * ds.z);
+    *pdf = (w * s.sky_wh_over_total) / (kTwoPiSq * st);
+    return true;
+}
+
+inline float power_heuristic(float a, float b) {
+    float a2 = a * a;
+    return a2 / (a2 + b * b);
+}
+
+// ------------------------------------------------------------- path tracing
+struct PathOut {
+    F3 radiance;
+    uint32_t segments;
+    uint8_t shadow_bits[16];
+    uint32_t n_light_shadow, n_sky_shadow;
+};
+
+constexpr int kMaxDirLights = 4;
+
+PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st,
+                   uint32_t pixel, uint32_t sample) {
+    PathOut out{};
+    const uint32_t M = st.max_bounces < 2 ? 2 : (st.max_bounces > 16 ? 16 : st.max_bounces);
+    const uint32_t D = M - 1;
+    const bool sky_nee = (st.flags & 2u) == 0 && s.sky_total > 0.0f;
+    const uint32_t nl = s.num_dir_lights < (uint32_t)kMaxDirLights ? s.num_dir_lights : kMaxDirLights;
+    uint32_t px = pixel % cam.width, py = pixel / cam.width;
+    float jx = 0.5f, jy = 0.5f;
+    if (st.flags & 1u) {
+        Pcg jr = pcg_keyed(st.seed, pixel, sample, kJitterBounce);
+        jx = pcg_float(jr);
+        jy = pcg_float(jr);
+    }
+    Ray ray;
+    camera_ray(cam, px, py, jx, jy, &ray.o, &ray.d);
+    ray.tmin = kTMin;
+    ray.tmax = st.ray_length;
+    F3 beta = f3(1.0f, 1.0f, 1.0f);
+    float pdf_prev = 0.0f;
+    F3 acc_miss = f3(0, 0, 0), acc_sky = f3(0, 0, 0);
+    F3 acc_dir[kMaxDirLights];
+    for (int l = 0; l < kMaxDirLights; ++l) acc_dir[l] = f3(0, 0, 0);
+    for (uint32_t b = 0; b < D; ++b) {
+        out.segments = b + 1;
+        Hit h = trace_closest(s, ray);
+        if (h.leaf == kMiss) {
+            F3 ls = sky_eval(s, ray.d);
+            float w = 1.0f;
+            if (b > 0 && sky_nee) w = power_heuristic(pdf_prev, sky_pdf(s, ray.d));
+            acc_miss = (beta * ls) * w;
+            break;
+        }
+        uint32_t tri = s.leaf_tri[h.leaf];
+        F3 P = ray.o + ray.d * h.t;
+        const float* tn = &s.tri_n[9 * tri];
+        F3 n0 = f3(tn[0], tn[1], tn[2]), n1 = f3(tn[3], tn[4], tn[5]), n2 = f3(tn[6], tn[7], tn[8]);
+        F3 N = normalize((n0 + (n1 - n0) * h.u) + (n2 - n0) * h.v);
+        Mat m = load_mat(&s.materials[16 * s.tri_mat[tri]]);
+        if (m.two_sided && dot(ray.d, N) > 0.0f) N = -N;
+        F3 V = normalize(ray.o - P);
+        Frame fr = make_frame(N);
+        Pcg rng = pcg_keyed(st.seed, pixel, sample, b);
+        float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
+        // directional lights (deferred_lighting_secondary.hlsl:45-60) with shadow rays
+        for (uint32_t l = 0; l < nl; ++l) {
+            const float* L = &s.dir_lights[16 * l];
+            F3 le = f3(L[0], L[1], L[2]);
+            F3 ldir = f3(L[4], L[5], L[6]);
+            F3 f = bsdf_eval(fr, m, V, ldir);
+            F3 c = (le * f) * beta;
+            if (!any_pos(c)) continue;
+            int sm_index;
+            std::memcpy(&sm_index, &L[3], sizeof sm_index);
+            if (sm_index < 0) { acc_dir[l] = acc_dir[l] + c; continue; }
+            out.shadow_bits[b] |= (uint8_t)(1u << l);
+            out.n_light_shadow++;
+            Ray sr{P, ldir, kTMin, FLT_MAX};
+            if (!trace_any(s, sr)) {
+                acc_dir[l] = acc_dir[l] + c;
+            } else {
+                float keep = 1.0f - L[7];
+                if (keep > 0.0f) acc_dir[l] = acc_dir[l] + c * keep;
+            }
+        }
+        // sky next-event estimation with power-heuristic MIS against BSDF sampling
+        if (sky_nee) {
+            F3 wi;
+            float pl;
+            if (sky_sample(s, u0, u1, &wi, &pl)) {
+                F3 f = bsdf_eval(fr, m, V, wi);
+                if (any_pos(f)) {
+                    F3 hl = to_local(fr, normalize(V + wi));
+                    float pb = vndf_pdf_l(hl, to_local(fr, V), m.rx, m.ry);
+                    float w = power_heuristic(pl, pb);
+                    F3 c = ((sky_eval(s, wi) * f) * (w / pl)) * beta;
+                    if (any_pos(c) && finite3(c)) {
+                        out.shadow_bits[b] |= 0x80u;
+                        out.n_sky_shadow++;
+                        Ray sr{P, wi, kTMin, FLT_MAX};
+                        if (!trace_any(s, sr)) acc_sky = acc_sky + c;
+                    }
+                }
+            }
+        }
+        // extension ray (sample_secondary_ray.hlsl:17-68)
+        if (b + 1 >= D || max3(beta) < kThroughputMin) break;
+        F3 vl = to_local(fr, V);
+        F3 hh = vndf_sample(vl, m.rx, m.ry, u2, u3);
+        float hv = dot(vl, hh);
+        F3 ll = hh * (2.0f * hv) - vl;  // reflect(-V, h)
+        float pdf = vndf_pdf_l(hh, vl, m.rx, m.ry);
+        F3 L = to_world(fr, ll);
+        F3 f = bsdf_eval(fr, m, V, L);
+        F3 w = f3(f.x / pdf, f.y / pdf, f.z / pdf);
+        if (!finite3(w)) w = f3(0, 0, 0);
+        F3 nb = beta * w;
+        if (!(max3(nb) > 0.0f)) break;
+        beta = nb;
+        pdf_prev = pdf;
+        ray.o = P;
+        ray.d = L;
+    }
+    F3 r = acc_miss;
+    for (uint32_t l = 0; l < nl; ++l) r = r + acc_dir[l];
+    r = r + acc_sky;
+    out.radiance = r;
+    return out;
+}
+
+}  // namespace
+
+uint32_t pixel_of(uint32_t q, uint32_t W, uint32_t H) {
+    if ((W & 7u) == 0 && (H & 7u) == 0) {
+        uint32_t tile = q >> 6, within = q & 63u;
+        uint32_t tiles_x = W >> 3;
+        uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+        return ((ty << 3) + (within >> 3)) * W + (tx << 3) + (within & 7u);
+    }
+    return q;
+}
+
+}  // namespace ora
+
+// ================================================================== C API
+using namespace ora;
+
+extern "C" {
+
+uint32_t ora_pixel_of(uint32_t q, uint32_t width, uint32_t height) { return pixel_of(q, width, height); }
+
+int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st,
+                 uint32_t sample_base, uint32_t spp, float* sum_rgb, uint8_t* path_segments,
+                 uint8_t* shadow_bits, uint64_t* ray_counts, int num_threads) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    const uint32_t W = cam->width, H = cam->height, npix = W * H;
+    if (W == 0 || H == 0) return -1;
+    if (num_threads <= 0) num_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint64_t> n_ext{0}, n_ls{0}, n_ss{0};
+    std::atomic<uint32_t> next{0};
+    const uint32_t chunk = 64;
+    auto worker = [&]() {
+        uint64_t e = 0, ls = 0, ss = 0;
+        for (;;) {
+            uint32_t q0 = next.fetch_add(chunk);
+            if (q0 >= npix) break;
+            uint32_t q1 = std::min(npix, q0 + chunk);
+            for (uint32_t q = q0; q < q1; ++q) {
+                uint32_t p = pixel_of(q, W, H);
+                F3 sum = f3(sum_rgb[3 * p], sum_rgb[3 * p + 1], sum_rgb[3 * p + 2]);
+                for (uint32_t sl = 0; sl < spp; ++sl) {
+                    PathOut po = trace_path(s, *cam, *st, p, sample_base + sl);
+                    sum = sum + po.radiance;
+                    size_t pi = (size_t)sl * npix + q;
+                    if (path_segments) path_segments[pi] = (uint8_t)po.segments;
+                    if (shadow_bits) std::memcpy(&shadow_bits[16 * pi], po.shadow_bits, 16);
+                    e += po.segments - 1;
+                    ls += po.n_light_shadow;
+                    ss += po.n_sky_shadow;
+                }
+                sum_rgb[3 * p] = sum.x; sum_rgb[3 * p + 1] = sum.y; sum_rgb[3 * p + 2] = sum.z;
+            }
+        }
+        n_ext += e; n_ls += ls; n_ss += ss;
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < num_threads; ++i) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    if (ray_counts) {
+        ray_counts[0] = (uint64_t)npix * spp;
+        ray_counts[1] = n_ext.load();
+        ray_counts[2] = n_ls.load();
+        ray_counts[3] = n_ss.load();
+    }
+    return 0;
+}
+
+void ora_trace(const ora_ctx* ctx, uint32_t n, const float* rays, int any_hit, float* hit_t,
+               uint32_t* hit_leaf, float* hit_u, float* hit_v) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = &rays[8 * i];
+        Ray ray{f3(r[0], r[1], r[2]), f3(r[4], r[5], r[6]), r[3], r[7]};
+        if (any_hit) {
+            hit_leaf[i] = trace_any(s, ray) ? 1u : 0u;
+        } else {
+            Hit h = trace_closest(s, ray);
+            hit_t[i] = h.t; hit_leaf[i] = h.leaf; hit_u[i] = h.u; hit_v[i] = h.v;
+        }
+    }
+}
+
+void ora_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out) {
+    Pcg r = pcg_seed(initstate, initseq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = pcg_next(r);
+}
+void ora_pcg32_keyed(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t n,
+                     float* out) {
+    Pcg r = pcg_keyed(seed, pixel, sample, bounce);
+    for (uint32_t i = 0; i < n; ++i) out[i] = pcg_float(r);
+}
+uint32_t ora_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
+
+void ora_math32(int fn, uint32_t n, const float* x, const float* y, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = sin32(x[i]); break;
+            case 1: out[i] = cos32(x[i]); break;
+            case 2: out[i] = acos32(x[i]); break;
+            default: out[i] = atan2_32(x[i], y[i]); break;
+        }
+    }
+}
+
+void ora_bsdf32(const float* material, const float* n, const float* v, const float* l, float* out3) {
+    Mat m = load_mat(material);
+    Frame fr = make_frame(f3(n[0], n[1], n[2]));
+    F3 r = bsdf_eval(fr, m, f3(v[0], v[1], v[2]), f3(l[0], l[1], l[2]));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+
+void ora_vndf32(const float* v_local, float alpha, float u0, float u1, float* h_out, float* pdf_out) {
+    F3 v = f3(v_local[0], v_local[1], v_local[2]);
+    F3 h = vndf_sample(v, alpha, alpha, u0, u1);
+    h_out[0] = h.x; h_out[1] = h.y; h_out[2] = h.z;
+    *pdf_out = vndf_pdf_l(h, v, alpha, alpha);
+}
+
+void ora_sky_eval32(const ora_ctx* ctx, const float* dir, float* rgb) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    F3 c = sky_eval(s, f3(dir[0], dir[1], dir[2]));
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+void ora_sky_sample32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    F3 d = f3(0, 0, 0);
+    sky_sample(s, u0, u1, &d, pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+float ora_sky_pdf32(const ora_ctx* ctx, const float* dir) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    return sky_pdf(s, f3(dir[0], dir[1], dir[2]));
+}
+void ora_camera_ray32(const ora_camera* cam, uint32_t px, uint32_t py, float jx, float jy, float* o,
+                      float* d) {
+    F3 oo, dd;
+    camera_ray(*cam, px, py, jx, jy, &oo, &dd);
+    o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
+    d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// ref_scene.cpp -- scene set-up for the CPU restatement: instance flattening,
+// sky importance table, LBVH build entry.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// Flattening restates the two-level scene of src/graphics/accel.cpp:104-132 (per-instance
+// 3x4 row-major transforms, include/bisemutum/rhi/accel.hpp:48-55) as one world-space
+// triangle list; normals use the cofactor of the 3x3 part, which is proportional to the
+// inverse transpose that hit.hlsl:155 applies (WorldToObject4x3 * n).
+#include <cstring>
+
+#include "oracle.h"
+#include "ref_scene.hpp"
+
+namespace ora {
+namespace {
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+void flatten(const ora_scene_desc& d, Scene& s) {
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < d.num_instances; ++i) total += d.idx_cnt[d.inst_mesh[i]] / 3;
+    s.num_tris = total;
+    s.tri_v.resize((size_t)9 * total);
+    s.tri_n.resize((size_t)9 * total);
+    s.tri_mat.resize(total);
+    uint32_t t = 0;
+    for (uint32_t i = 0; i < d.num_instances; ++i) {
+        const float* m = &d.inst_xform[12 * i];
+        const uint32_t mesh = d.inst_mesh[i];
+        // cofactor matrix of the linear part
+        float c00 = m[5] * m[10] - m[6] * m[9], c01 = m[6] * m[8] - m[4] * m[10], c02 = m[4] * m[9] - m[5] * m[8];
+        float c10 = m[2] * m[9] - m[1] * m[10], c11 = m[0] * m[10] - m[2] * m[8], c12 = m[1] * m[8] - m[0] * m[9];
+        float c20 = m[1] * m[6] - m[2] * m[5], c21 = m[2] * m[4] - m[0] * m[6], c22 = m[0] * m[5] - m[1] * m[4];
+        float det = (m[0] * c00 + m[1] * c01) + m[2] * c02;
+        float sgn = det < 0.0f ? -1.0f : 1.0f;
+        const uint32_t ntri = d.idx_cnt[mesh] / 3;
+        const uint32_t* idx = &d.indices[d.idx_off[mesh]];
+        const float* P = &d.positions[3 * (size_t)d.vtx_off[mesh]];
+        const float* Nn = &d.normals[3 * (size_t)d.vtx_off[mesh]];
+        for (uint32_t k = 0; k < ntri; ++k, ++t) {
+            for (int c = 0; c < 3; ++c) {
+                uint32_t vi = idx[3 * k + c];
+                float x = P[3 * vi], y = P[3 * vi + 1], z = P[3 * vi + 2];
+                s.tri_v[9 * t + 3 * c + 0] = ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
+                s.tri_v[9 * t + 3 * c + 1] = ((m[4] * x + m[5] * y) + m[6] * z) + m[7];
+                s.tri_v[9 * t + 3 * c + 2] = ((m[8] * x + m[9] * y) + m[10] * z) + m[11];
+                float nx = Nn[3 * vi], ny = Nn[3 * vi + 1], nz = Nn[3 * vi + 2];
+                s.tri_n[9 * t + 3 * c + 0] = ((c00 * nx + c01 * ny) + c02 * nz) * sgn;
+                s.tri_n[9 * t + 3 * c + 1] = ((c10 * nx + c11 * ny) + c12 * nz) * sgn;
+                s.tri_n[9 * t + 3 * c + 2] = ((c20 * nx + c21 * ny) + c22 * nz) * sgn;
+            }
+            s.tri_mat[t] = d.inst_mat[i];
+        }
+    }
+}
+
+// Importance table over the equirect: w = luminance(texel * sky_color) * sin(theta_row).
+void build_sky_table(Scene& s) {
+    const uint32_t W = s.sky_w, H = s.sky_h;
+    s.sky_w_tex.assign((size_t)W * H, 0.0f);
+    s.sky_ccdf.assign((size_t)(W + 1) * H, 0.0f);
+    s.sky_rows.assign(H, 0.0f);
+    s.sky_marg.assign(H + 1, 0.0f);
+    if (W == 0 || H == 0) { s.sky_total = 0.0f; return; }
+    const float pi_over_h = kPiF / (float)H;
+    for (uint32_t i = 0; i < H; ++i) {
+        float st = sin32(((float)i + 0.5f) * pi_over_h);
+        float run = 0.0f;
+        for (uint32_t j = 0; j < W; ++j) {
+            const float* p = &s.sky_rgba[4 * ((size_t)i * W + j)];
+            float r = p[0] * s.sky_color[0], g = p[1] * s.sky_color[1], b = p[2] * s.sky_color[2];
+            float lum = (0.2126f * r + 0.7152f * g) + 0.0722f * b;
+            float w = lum * st;
+            if (!(w > 0.0f)) w = 0.0f;
+            s.sky_w_tex[(size_t)i * W + j] = w;
+            s.sky_ccdf[(size_t)i * (W + 1) + j] = run;
+            run = run + w;
+        }
+        s.sky_ccdf[(size_t)i * (W + 1) + W] = run;
+        s.sky_rows[i] = run;
+        s.sky_marg[i + 1] = s.sky_marg[i] + run;
+    }
+    s.sky_total = s.sky_marg[H];
+    s.sky_wh_over_total = s.sky_total > 0.0f ? (float)(W * H) / s.sky_total : 0.0f;
+}
+
+}  // namespace
+}  // namespace ora
+
+using namespace ora;
+
+extern "C" {
+
+ora_ctx* ora_create(const ora_scene_desc* d) {
+    Scene* s = new Scene();
+    flatten(*d, *s);
+    s->materials.assign(d->materials, d->materials + 16 * (size_t)d->num_materials);
+    s->num_dir_lights = d->num_dir_lights;
+    s->dir_lights.assign(d->dir_lights, d->dir_lights + 16 * (size_t)d->num_dir_lights);
+    s->sky_w = d->sky_rgba ? d->sky_w : 0;
+    s->sky_h = d->sky_rgba ? d->sky_h : 0;
+    if (d->sky_rgba) s->sky_rgba.assign(d->sky_rgba, d->sky_rgba + 4 * (size_t)d->sky_w * d->sky_h);
+    std::memcpy(s->sky_color, d->sky_color, sizeof s->sky_color);
+    std::memcpy(s->sky_rot, d->sky_rot, sizeof s->sky_rot);
+    build_sky_table(*s);
+    build_bvh(*s);
+    return reinterpret_cast<ora_ctx*>(s);
+}
+
+void ora_destroy(ora_ctx* ctx) {
+    Scene* s = reinterpret_cast<Scene*>(ctx);
+    free_bvh64(*s);
+    delete s;
+}
+
+uint32_t ora_num_tris(const ora_ctx* ctx) { return reinterpret_cast<const Scene*>(ctx)->num_tris; }
+
+void ora_get_world(const ora_ctx* ctx, float* tri_v, float* tri_n, uint32_t* tri_mat) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    std::memcpy(tri_v, s.tri_v.data(), s.tri_v.size() * sizeof(float));
+    std::memcpy(tri_n, s.tri_n.data(), s.tri_n.size() * sizeof(float));
+    std::memcpy(tri_mat, s.tri_mat.data(), s.tri_mat.size() * sizeof(uint32_t));
+}
+
+void ora_get_bvh(const ora_ctx* ctx, uint32_t* morton, uint32_t* order, int32_t* children,
+                 float* child_bounds, int32_t* parents, float* scene_centroid_bounds) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    const uint32_t n = s.num_tris;
+    if (n == 0) return;
+    std::memcpy(morton, s.morton.data(), n * sizeof(uint32_t));
+    std::memcpy(order, s.order.data(), n * sizeof(uint32_t));
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    for (uint32_t i = 0; i < ni; ++i) {
+        children[2 * i] = s.nodes[i].child[0];
+        children[2 * i + 1] = s.nodes[i].child[1];
+        std::memcpy(&child_bounds[12 * i], s.nodes[i].box[0], 6 * sizeof(float));
+        std::memcpy(&child_bounds[12 * i + 6], s.nodes[i].box[1], 6 * sizeof(float));
+    }
+    if (parents) std::memcpy(parents, s.parent.data(), s.parent.size() * sizeof(int32_t));
+    if (scene_centroid_bounds) std::memcpy(scene_centroid_bounds, s.cbounds, 6 * sizeof(float));
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// ref_scene.hpp -- scene state shared by the FP32 and FP64 restatements.
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace ora {
+
+struct Node {
+    int32_t child[2] = {0, 0};   // internal k >= 0, leaf j encoded as ~j
+    float box[2][6] = {};        // per child: lo.xyz, hi.xyz
+};
+
+struct Scene {
+    uint32_t num_tris = 0;
+    std::vector<float> tri_v;      // 9 per triangle, world space, instance order
+    std::vector<float> tri_n;      // 9 per triangle, cofactor-transformed normals
+    std::vector<uint32_t> tri_mat;
+    std::vector<float> materials;  // 16 per material
+    uint32_t num_dir_lights = 0;
+    std::vector<float> dir_lights; // 16 per light
+    // sky
+    uint32_t sky_w = 0, sky_h = 0;
+    std::vector<float> sky_rgba;
+    float sky_color[3] = {0, 0, 0};
+    float sky_rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    std::vector<float> sky_w_tex;  // per texel luminance * sin(theta_row)
+    std::vector<float> sky_ccdf;   // per row W+1 running sums
+    std::vector<float> sky_rows;   // per row sum
+    std::vector<float> sky_marg;   // H+1 running sums of rows
+    float sky_total = 0.0f;
+    float sky_wh_over_total = 0.0f;
+    // LBVH (FP32 transcription)
+    float cbounds[6] = {};
+    std::vector<uint32_t> morton, order, leaf_tri;
+    std::vector<Node> nodes;
+    std::vector<int32_t> parent;
+    std::vector<float> leaf_v0, leaf_e1, leaf_e2;
+    // FP64 oracle BVH (built lazily by ref64.cpp)
+    mutable void* bvh64 = nullptr;
+};
+
+uint32_t expand10(uint32_t v);
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
+void build_bvh(Scene& s);
+uint32_t pixel_of(uint32_t q, uint32_t W, uint32_t H);
+void sincos32(float x, float* s, float* c);
+float sin32(float x);
+float cos32(float x);
+float acos32(float x);
+float atan2_32(float y, float x);
+void free_bvh64(const Scene& s);
+
+}  // namespace ora
