@@ -1,0 +1,266 @@
+"""Python binding (ctypes) of the C ABI in include/bi_pt.h.
+
+The pass runs in libbi_pt.so (hand-written HIP kernels for gfx950).  Output
+buffers live in device memory allocated through torch, which is only plumbing
+here; there is no CPU fallback: a missing library or a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbi_pt.so")
+
+FLAG_JITTER = 1 << 0
+FLAG_NO_SKY_NEE = 1 << 1
+FLAG_RECORD_QUEUES = 1 << 8
+FLAG_TIMING = 1 << 9
+
+STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_SCENE", 5: "DEVICE_CHECK",
+          6: "UNSUPPORTED"}
+
+# every entry point declared in include/bi_pt.h
+EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
+           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_debug_get_bvh", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
+           "bi_pt_debug_trace"]
+
+
+class PtError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"bi_pt {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class DeviceDesc(C.Structure):
+    _fields_ = [("device", C.c_int32), ("_pad", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("transform", C.c_float * 12), ("mesh", C.c_uint32), ("material", C.c_uint32)]
+
+
+class CameraC(C.Structure):
+    _fields_ = [("matrix_inv_view", C.c_float * 16), ("matrix_inv_proj", C.c_float * 16),
+                ("matrix_proj_view", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("ray_length", C.c_float), ("max_bounces", C.c_uint32), ("accumulate", C.c_uint32),
+                ("spp", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("wave_paths", C.c_uint32),
+                ("stack_lds", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("rays_camera", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_light_shadow", C.c_uint64),
+                ("rays_sky_shadow", C.c_uint64), ("frames", C.c_uint64), ("samples_accumulated", C.c_uint64),
+                ("ms_build", C.c_float), ("ms_trace_closest", C.c_float), ("ms_shade", C.c_float),
+                ("ms_trace_shadow", C.c_float), ("ms_resolve", C.c_float), ("ms_total", C.c_float),
+                ("launches_trace_closest", C.c_uint32), ("launches_trace_shadow", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libbi_pt.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run __graft_entry__.build() (make -C bisemutum-engine_amd)")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, i32, f32p, u32p, i32p = C.c_void_p, C.c_uint32, C.c_int32, C.POINTER(C.c_float), \
+            C.POINTER(C.c_uint32), C.POINTER(C.c_int32)
+        L.bi_pt_create.argtypes = [C.POINTER(DeviceDesc), C.POINTER(vp)]
+        L.bi_pt_destroy.argtypes = [vp]
+        L.bi_pt_last_error.argtypes = [vp]
+        L.bi_pt_last_error.restype = C.c_char_p
+        L.bi_pt_set_scene.argtypes = [vp, C.POINTER(Mesh), u32, C.POINTER(Instance), u32, f32p, u32]
+        L.bi_pt_update_transforms.argtypes = [vp, f32p, u32]
+        L.bi_pt_set_lights.argtypes = [vp, f32p, u32]
+        L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
+        L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
+        L.bi_pt_reset_accumulation.argtypes = [vp]
+        L.bi_pt_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.bi_pt_debug_get_bvh.argtypes = [vp, u32p, u32p, i32p, f32p, i32p, f32p]
+        L.bi_pt_debug_num_tris.argtypes = [vp]
+        L.bi_pt_debug_num_tris.restype = u32
+        L.bi_pt_debug_queue.argtypes = [vp, u32, u32, u32, u32p, u32, u32p]
+        L.bi_pt_debug_get_sums.argtypes = [vp, f32p]
+        L.bi_pt_debug_trace.argtypes = [vp, u32, f32p, C.c_int, f32p, u32p, f32p, f32p]
+        for name in EXPORTS:
+            if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_float):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def camera_struct(cam) -> CameraC:
+    c = CameraC()
+    C.memmove(C.addressof(c.matrix_inv_view), np.ascontiguousarray(cam.inv_view, np.float32).ctypes.data, 64)
+    C.memmove(C.addressof(c.matrix_inv_proj), np.ascontiguousarray(cam.inv_proj, np.float32).ctypes.data, 64)
+    C.memmove(C.addressof(c.matrix_proj_view), np.ascontiguousarray(cam.proj_view, np.float32).ctypes.data, 64)
+    c.width, c.height = cam.width, cam.height
+    return c
+
+
+class PathTracer:
+    """One bi_pt context on one GPU."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("bi_pt needs a ROCm GPU; none is visible")
+        self.torch = torch
+        self.device = device
+        h = C.c_void_p()
+        st = lib().bi_pt_create(C.byref(DeviceDesc(device, 0, hbm_budget_bytes)), C.byref(h))
+        if st != 0:
+            raise PtError(st, "bi_pt_create failed")
+        self.h = h
+        self._out = None
+        self._depth = None
+
+    def _check(self, st):
+        if st != 0:
+            raise PtError(st, lib().bi_pt_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().bi_pt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- inputs
+    def set_scene(self, scene):
+        keep = []
+        meshes = (Mesh * len(scene.meshes))()
+        for i, (p, n, idx) in enumerate(scene.meshes):
+            p = np.ascontiguousarray(p, np.float32)
+            n = np.ascontiguousarray(n, np.float32)
+            idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+            keep += [p, n, idx]
+            meshes[i] = Mesh(_p(p), _p(n), _p(idx, C.c_uint32), len(p), len(idx))
+        inst = (Instance * len(scene.instances))()
+        for i, rec in enumerate(scene.instances):
+            x = np.ascontiguousarray(rec["xform"], np.float32)
+            inst[i].transform[:] = list(map(float, x))
+            inst[i].mesh = int(rec["mesh"])
+            inst[i].material = int(rec["material"])
+        mats = np.ascontiguousarray(scene.materials, np.float32)
+        self._check(lib().bi_pt_set_scene(self.h, meshes, len(scene.meshes), inst, len(scene.instances), _p(mats),
+                                          len(mats)))
+        self.set_lights(scene.dir_lights)
+        self.set_sky(scene.sky_rgba, scene.sky_color, scene.sky_rot)
+
+    def update_transforms(self, xforms):
+        x = np.ascontiguousarray(xforms, np.float32)
+        self._check(lib().bi_pt_update_transforms(self.h, _p(x), len(x)))
+
+    def set_lights(self, lights):
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
+        self._check(lib().bi_pt_set_lights(self.h, _p(lights), len(lights)))
+
+    def set_sky(self, rgba, color, rot):
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        col = np.ascontiguousarray(color, np.float32)
+        rot = np.ascontiguousarray(rot, np.float32)
+        h, w = (rgba.shape[0], rgba.shape[1]) if rgba.size else (0, 0)
+        self._check(lib().bi_pt_set_sky(self.h, _p(rgba) if rgba.size else None, w, h, _p(col), _p(rot)))
+
+    # ---------------------------------------------------------------- render
+    def render(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
+               wave_paths=0, stack_lds=0, depth=False, stream=None):
+        torch = self.torch
+        npix = cam.width * cam.height
+        dev = torch.device("cuda", self.device)
+        if self._out is None or self._out.numel() != npix * 4:
+            self._out = torch.empty(npix * 4, dtype=torch.float32, device=dev)
+        dptr = None
+        if depth:
+            if self._depth is None or self._depth.numel() != npix:
+                self._depth = torch.empty(npix, dtype=torch.float32, device=dev)
+            dptr = C.c_void_p(self._depth.data_ptr())
+        st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, wave_paths, stack_lds)
+        s = C.c_void_p(stream) if stream else None
+        self._check(lib().bi_pt_render(self.h, C.byref(camera_struct(cam)), C.byref(st),
+                                       C.c_void_p(self._out.data_ptr()), dptr, s))
+        return self._out
+
+    def image(self):
+        self.torch.cuda.synchronize()
+        return self._out.cpu().numpy().reshape(-1, 4)
+
+    def depth_image(self):
+        self.torch.cuda.synchronize()
+        return self._depth.cpu().numpy()
+
+    def reset(self):
+        self._check(lib().bi_pt_reset_accumulation(self.h))
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().bi_pt_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    # ---------------------------------------------------------------- inspection
+    def num_tris(self):
+        return lib().bi_pt_debug_num_tris(self.h)
+
+    def bvh(self):
+        n = self.num_tris()
+        ni = max(n - 1, 1)
+        morton = np.zeros(n, np.uint32)
+        order = np.zeros(n, np.uint32)
+        children = np.zeros((ni, 2), np.int32)
+        bounds = np.zeros((ni, 12), np.float32)
+        parents = np.zeros(ni + n, np.int32)
+        cb = np.zeros(6, np.float32)
+        self._check(lib().bi_pt_debug_get_bvh(self.h, _p(morton, C.c_uint32), _p(order, C.c_uint32),
+                                              _p(children, C.c_int32), _p(bounds), _p(parents, C.c_int32), _p(cb)))
+        return dict(morton=morton, order=order, children=children, bounds=bounds, parents=parents,
+                    centroid_bounds=cb)
+
+    def queue(self, wave, bounce, kind):
+        cnt = C.c_uint32()
+        self._check(lib().bi_pt_debug_queue(self.h, wave, bounce, kind, None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, np.uint32)
+        self._check(lib().bi_pt_debug_queue(self.h, wave, bounce, kind, _p(out, C.c_uint32), cnt.value,
+                                            C.byref(cnt)))
+        return out
+
+    def sums(self, npix):
+        out = np.zeros((npix, 3), np.float32)
+        self._check(lib().bi_pt_debug_get_sums(self.h, _p(out)))
+        return out
+
+    def trace(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        leaf = np.zeros(n, np.uint32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        self._check(lib().bi_pt_debug_trace(self.h, n, _p(rays), int(any_hit), _p(t), _p(leaf, C.c_uint32),
+                                            _p(u), _p(v)))
+        return (leaf,) if any_hit else (t, leaf, u, v)

@@ -1,0 +1,351 @@
+// lbvh.hip -- per-frame LBVH build on gfx950.
+//
+// Replaces the driver-side BLAS/TLAS build the reference requests every frame
+// (src/graphics/accel.cpp:11-159 -> vkCmdBuildAccelerationStructuresKHR,
+// src/rhi/backend_vulkan/command.cpp:425,483).  The reference has no BVH code of
+// its own; this is the build's design:
+//   1. flatten    instances -> world-space triangles (+ cofactor normals), centroid bounds
+//   2. morton     30-bit Morton code of each triangle's bounding-box centre
+//   3. sort       stable LSD radix sort of (code, triangle) with 8-bit digits:
+//                 LDS histogram -> device scan -> LDS-ranked scatter, the rank of
+//                 equal digits inside a wavefront taken from 8 ballots
+//   4. karras     binary radix tree over the sorted codes (Karras 2012), index
+//                 tie-break for equal codes
+//   5. leaves     leaf intersection/shading records in sorted order, then the
+//                 bottom-up AABB refit (second arrival at each node continues)
+// All steps are integer or exact min/max except the Morton quantisation, whose
+// float operations are written in a fixed order.
+#include "pt_kernels.hpp"
+
+namespace bipt {
+namespace {
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__global__ void k_init_bounds(uint32_t* cb) {
+    int t = threadIdx.x;
+    if (t < 3) cb[t] = 0xffffffffu;       // min slots
+    else if (t < 6) cb[t] = 0u;           // max slots
+}
+
+__global__ __launch_bounds__(256) void k_flatten(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                 const uint32_t* __restrict__ idx,
+                                                 const InstanceRec* __restrict__ inst, uint32_t ninst,
+                                                 uint32_t n, float* __restrict__ tri_v,
+                                                 float* __restrict__ tri_n, uint32_t* __restrict__ tri_mat,
+                                                 uint32_t* __restrict__ cb) {
+    uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    float cmin[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+    float cmax[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    if (t < n) {
+        uint32_t lo = 0, hi = ninst - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (inst[mid].first_tri <= t) lo = mid; else hi = mid - 1;
+        }
+        const InstanceRec& I = inst[lo];
+        const float* m = I.xform;
+        float c00 = m[5] * m[10] - m[6] * m[9], c01 = m[6] * m[8] - m[4] * m[10], c02 = m[4] * m[9] - m[5] * m[8];
+        float c10 = m[2] * m[9] - m[1] * m[10], c11 = m[0] * m[10] - m[2] * m[8], c12 = m[1] * m[8] - m[0] * m[9];
+        float c20 = m[1] * m[6] - m[2] * m[5], c21 = m[2] * m[4] - m[0] * m[6], c22 = m[0] * m[5] - m[1] * m[4];
+        float det = (m[0] * c00 + m[1] * c01) + m[2] * c02;
+        float sgn = det < 0.0f ? -1.0f : 1.0f;
+        uint32_t k = t - I.first_tri;
+        float w[9];
+        for (int c = 0; c < 3; ++c) {
+            uint32_t vi = I.vtx_off + idx[I.idx_off + 3 * k + c];
+            float x = pos[3 * (size_t)vi], y = pos[3 * (size_t)vi + 1], z = pos[3 * (size_t)vi + 2];
+            w[3 * c + 0] = ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
+            w[3 * c + 1] = ((m[4] * x + m[5] * y) + m[6] * z) + m[7];
+            w[3 * c + 2] = ((m[8] * x + m[9] * y) + m[10] * z) + m[11];
+            float nx = nrm[3 * (size_t)vi], ny = nrm[3 * (size_t)vi + 1], nz = nrm[3 * (size_t)vi + 2];
+            tri_n[9 * (size_t)t + 3 * c + 0] = ((c00 * nx + c01 * ny) + c02 * nz) * sgn;
+            tri_n[9 * (size_t)t + 3 * c + 1] = ((c10 * nx + c11 * ny) + c12 * nz) * sgn;
+            tri_n[9 * (size_t)t + 3 * c + 2] = ((c20 * nx + c21 * ny) + c22 * nz) * sgn;
+        }
+        for (int a = 0; a < 9; ++a) tri_v[9 * (size_t)t + a] = w[a];
+        tri_mat[t] = I.material;
+        for (int a = 0; a < 3; ++a) {
+            float l = sel_min(sel_min(w[a], w[3 + a]), w[6 + a]);
+            float h = sel_max(sel_max(w[a], w[3 + a]), w[6 + a]);
+            float ce = (l + h) * 0.5f;
+            cmin[a] = ce;
+            cmax[a] = ce;
+        }
+    }
+    // wavefront reduction of the centroid bounds, then one atomic per wave and axis
+    for (int a = 0; a < 3; ++a) {
+        float mn = cmin[a], mx = cmax[a];
+        for (int off = 32; off >= 1; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off));
+            mx = fmaxf(mx, __shfl_xor(mx, off));
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            atomicMin(&cb[a], f2ord(mn));
+            atomicMax(&cb[3 + a], f2ord(mx));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_v, uint32_t n,
+                                                const uint32_t* __restrict__ cb, uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals, float* __restrict__ cb_out) {
+    uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    float cmin[3], inv_ext[3];
+    for (int a = 0; a < 3; ++a) {
+        cmin[a] = ord2f(cb[a]);
+        float e = ord2f(cb[3 + a]) - cmin[a];
+        inv_ext[a] = e > 0.0f ? 1.0f / e : 0.0f;
+    }
+    if (t == 0 && cb_out) {
+        for (int a = 0; a < 6; ++a) cb_out[a] = ord2f(cb[a]);
+    }
+    if (t >= n) return;
+    const float* w = tri_v + 9 * (size_t)t;
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+        float l = sel_min(sel_min(w[a], w[3 + a]), w[6 + a]);
+        float h = sel_max(sel_max(w[a], w[3 + a]), w[6 + a]);
+        float ce = (l + h) * 0.5f;
+        float f = (ce - cmin[a]) * inv_ext[a] * 1024.0f;
+        int qi = (int)f;
+        q[a] = (uint32_t)(qi < 0 ? 0 : (qi > 1023 ? 1023 : qi));
+    }
+    keys[t] = morton3(q[0], q[1], q[2]);
+    vals[t] = t;
+}
+
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortItems = 16;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ kin, uint32_t n, uint32_t shift,
+                                                    uint32_t nblocks, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kSortTile;
+    for (uint32_t r = 0; r < kSortItems; ++r) {
+        uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(kin[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m entries in place, one workgroup of 1024 lanes.
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ a, uint32_t m) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (uint32_t start = 0; start < m; start += 1024u) {
+        uint32_t i = start + threadIdx.x;
+        uint32_t v = i < m ? a[i] : 0u;
+        uint32_t inc = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(inc, off);
+            if (lane >= (uint32_t)off) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t wbase = 0, total = 0;
+        for (uint32_t w = 0; w < 16; ++w) {
+            uint32_t s = wsum[w];
+            if (w < wave) wbase += s;
+            total += s;
+        }
+        uint32_t carry = carry_s;
+        if (i < m) a[i] = carry + wbase + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       const uint32_t* __restrict__ hist, uint32_t n,
+                                                       uint32_t nblocks, uint32_t shift) {
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_wcnt[4][256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    s_off[tid] = hist[tid * nblocks + blockIdx.x];
+    for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t r = 0; r < kSortItems; ++r) {
+        uint32_t i = base + r * kSortThreads + tid;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = __ballot(bit);
+            m &= bit ? bal : ~bal;
+        }
+        uint32_t rank = __popcll(m & lt);
+        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t dest = s_off[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) dest += s_wcnt[w][d];
+            kout[dest] = key;
+            vout[dest] = val;
+        }
+        __syncthreads();
+        s_off[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
+        for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = codes[i], b = codes[j];
+    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, int n,
+                                                BvhNode* __restrict__ nodes, int32_t* __restrict__ parent,
+                                                uint32_t* __restrict__ refit_cnt) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (n == 1) {
+        if (i == 0) {
+            nodes[0].child[0] = ~0;
+            nodes[0].child[1] = ~0;
+            nodes[0].pad[0] = nodes[0].pad[1] = 0;
+            parent[0] = -1;
+            parent[1] = 0;
+            refit_cnt[0] = 0;
+        }
+        return;
+    }
+    if (i >= n - 1) return;
+    int d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = kdelta(codes, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(codes, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) / 2;
+        if (kdelta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? -1 : 0);
+    int left = (min(i, j) == gamma) ? ~gamma : gamma;
+    int right = (max(i, j) == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    nodes[i].child[0] = left;
+    nodes[i].child[1] = right;
+    nodes[i].pad[0] = nodes[i].pad[1] = 0;
+    parent[left < 0 ? (n - 1) + ~left : left] = i;
+    parent[right < 0 ? (n - 1) + ~right : right] = i;
+    if (i == 0) parent[0] = -1;
+    refit_cnt[i] = 0;
+}
+
+__device__ __forceinline__ float ld_coherent(const float* p) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ order, const float* __restrict__ tri_v,
+                                                const float* __restrict__ tri_n,
+                                                const uint32_t* __restrict__ tri_mat, int n,
+                                                LeafGeo* __restrict__ geo, LeafShade* __restrict__ shade,
+                                                BvhNode* nodes, const int32_t* __restrict__ parent,
+                                                uint32_t* refit_cnt) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    uint32_t t = order[k];
+    const float* w = tri_v + 9 * (size_t)t;
+    float v[9];
+    for (int a = 0; a < 9; ++a) v[a] = w[a];
+    LeafGeo g;
+    for (int a = 0; a < 3; ++a) {
+        g.v[a] = v[a];
+        g.v[3 + a] = v[3 + a] - v[a];
+        g.v[6 + a] = v[6 + a] - v[a];
+    }
+    g.v[9] = g.v[10] = g.v[11] = 0.0f;
+    geo[k] = g;
+    LeafShade sh;
+    for (int a = 0; a < 9; ++a) sh.n[a] = tri_n[9 * (size_t)t + a];
+    sh.material = tri_mat[t];
+    sh.pad[0] = sh.pad[1] = 0;
+    shade[k] = sh;
+    float box[6];
+    for (int a = 0; a < 3; ++a) {
+        box[a] = sel_min(sel_min(v[a], v[3 + a]), v[6 + a]) + 0.0f;
+        box[3 + a] = sel_max(sel_max(v[a], v[3 + a]), v[6 + a]) + 0.0f;
+    }
+    if (n == 1) {
+        for (int a = 0; a < 6; ++a) { nodes[0].box[a] = box[a]; nodes[0].box[6 + a] = box[a]; }
+        return;
+    }
+    int p = parent[(n - 1) + k];
+    int slot = nodes[p].child[0] == ~k ? 0 : 1;
+    for (;;) {
+        for (int a = 0; a < 6; ++a) nodes[p].box[6 * slot + a] = box[a];
+        __threadfence();
+        uint32_t old = __hip_atomic_fetch_add(&refit_cnt[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;   // the sibling subtree is still being built; it continues upward
+        __threadfence();
+        const float* ob = nodes[p].box + 6 * (1 - slot);
+        for (int a = 0; a < 3; ++a) {
+            box[a] = sel_min(box[a], ld_coherent(ob + a)) + 0.0f;
+            box[3 + a] = sel_max(box[3 + a], ld_coherent(ob + 3 + a)) + 0.0f;
+        }
+        if (p == 0) return;
+        int q = parent[p];
+        slot = nodes[q].child[0] == p ? 0 : 1;
+        p = q;
+    }
+}
+
+}  // namespace
+
+void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
+    const uint32_t n = a.num_tris;
+    if (n == 0) return;
+    const uint32_t g256 = (n + 255) / 256;
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, a.cbounds_ord);
+    hipLaunchKernelGGL(k_flatten, dim3(g256), dim3(256), 0, s, a.positions, a.normals, a.indices, a.instances,
+                       a.num_instances, n, a.tri_v, a.tri_n, a.tri_mat, a.cbounds_ord);
+    hipLaunchKernelGGL(k_morton, dim3(g256), dim3(256), 0, s, a.tri_v, n, a.cbounds_ord, a.keys[0], a.vals[0],
+                       a.cbounds_out);
+    const uint32_t nblocks = (n + kSortTile - 1) / kSortTile;
+    int cur = 0;
+    for (uint32_t shift = 0; shift < 30; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, s, a.keys[cur], n, shift, nblocks, a.hist);
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, a.hist, 256u * nblocks);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(256), 0, s, a.keys[cur], a.vals[cur],
+                           a.keys[cur ^ 1], a.vals[cur ^ 1], a.hist, n, nblocks, shift);
+        cur ^= 1;
+    }
+    // 4 passes: the sorted keys/values end in buffer 0
+    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, a.keys[cur], (int)n, a.nodes, a.parent, a.refit_cnt);
+    hipLaunchKernelGGL(k_leaves, dim3(g256), dim3(256), 0, s, a.vals[cur], a.tri_v, a.tri_n, a.tri_mat, (int)n,
+                       a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt);
+}
+
+uint32_t sort_hist_entries(uint32_t n) { return 256u * ((n + kSortTile - 1) / kSortTile); }
+
+}  // namespace bipt

@@ -1,0 +1,368 @@
+// pt_device.hpp -- device-side arithmetic of the path-tracing pass (gfx950).
+//
+// Every function rounds each float operation once, in the order written: the
+// library is compiled with -ffp-contract=off, correctly rounded division and
+// square root (HIP's default), and no fast-math, so the results are a fixed
+// function of the inputs.  Where the pass uses min/max on values that may be
+// NaN, comparisons are written out (a < b ? a : b) so NaN propagation is fixed too.
+//
+// Reference formulas (paths relative to bisemutum/):
+//   lit BSDF                shaders/core/material/lit.hlsl:5-35
+//   Fresnel / GGX / VNDF    shaders/core/material/utils.hlsl:40-121
+//   shading frame           shaders/core/utils/frame.hlsl:9-20 (Duff et al. 2017)
+//   camera ray              shaders/renderer/raytracing/direction_sample/generate_camera_ray.hlsl:9-15
+//   equirect mapping        shaders/core/equitangular_to_cubemap.hlsl:28-33
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pt_layout.hpp"
+
+namespace bipt {
+
+#define PT_HD __host__ __device__ __forceinline__
+
+struct V3 { float x, y, z; };
+PT_HD V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+PT_HD V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+PT_HD V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+PT_HD V3 operator*(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+PT_HD V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+PT_HD V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+PT_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PT_HD V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+PT_HD float sel_max(float a, float b) { return a > b ? a : b; }
+PT_HD float sel_min(float a, float b) { return a < b ? a : b; }
+PT_HD V3 normalize(V3 v) { float inv = 1.0f / __builtin_sqrtf(dot(v, v)); return v * inv; }
+PT_HD float max3(V3 v) { return sel_max(sel_max(v.x, v.y), v.z); }
+PT_HD bool any_pos(V3 v) { return v.x > 0.0f || v.y > 0.0f || v.z > 0.0f; }
+PT_HD bool finite1(float x) { return __builtin_isfinite(x); }
+PT_HD bool finite3(V3 v) { return finite1(v.x) && finite1(v.y) && finite1(v.z); }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kPiO2 = 1.57079632679489661923f;
+constexpr float kPiO4 = 0.785398163397448309616f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr float kInvTwoPi = 0.159154943091895335769f;
+constexpr float kTwoPiSq = 19.7392088021787172376f;
+constexpr float kTMin = 0.001f;            // rt_gbuffer.hlsl:18 (TMin)
+constexpr float kRobust = 1.00000036f;     // 1 + 2*gamma(3): conservative slab test (Ize 2013)
+constexpr float kThroughputMin = 0.001f;   // sample_secondary_ray.hlsl:24
+constexpr float kShadowTMax = 3.40282347e38f;
+
+// ------------------------------------------------------------ transcendentals
+// sin/cos/asin/acos/atan in single precision after the Cephes library (S. L.
+// Moshier): Cody-Waite reduction by pi/4 in three parts, then the published
+// minimax polynomials, evaluated by Horner without fused multiply-add.
+PT_HD void pt_sincos(float x, float* s_out, float* c_out) {
+    float ax = x < 0.0f ? -x : x;
+    int q = (int)(ax * 1.27323954473516268615f);
+    float y = (float)q;
+    if (q & 1) { q += 1; y += 1.0f; }
+    int j = q & 7;
+    float r = ((ax - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
+    float z = r * r;
+    float sk = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * r + r;
+    float ck = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z
+        - 0.5f * z + 1.0f;
+    int sign_s = x < 0.0f ? -1 : 1;
+    int sign_c = 1;
+    if (j > 3) { j -= 4; sign_s = -sign_s; sign_c = -sign_c; }
+    if (j > 1) sign_c = -sign_c;
+    float s, c;
+    if (j == 1 || j == 2) { s = ck; c = sk; } else { s = sk; c = ck; }
+    *s_out = sign_s < 0 ? -s : s;
+    *c_out = sign_c < 0 ? -c : c;
+}
+PT_HD float pt_sin(float x) { float s, c; pt_sincos(x, &s, &c); return s; }
+PT_HD float pt_asin(float x) {
+    float a = x < 0.0f ? -x : x;
+    float z, s;
+    bool big = a > 0.5f;
+    if (big) { z = 0.5f * (1.0f - a); s = __builtin_sqrtf(z); }
+    else { s = a; z = a * a; }
+    float p = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z
+                + 7.4953002686e-2f) * z + 1.6666752422e-1f) * z * s + s;
+    if (big) { p = p + p; p = kPiO2 - p; }
+    return x < 0.0f ? -p : p;
+}
+PT_HD float pt_acos(float x) {
+    if (x < -0.5f) return kPi - 2.0f * pt_asin(__builtin_sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * pt_asin(__builtin_sqrtf(0.5f * (1.0f - x)));
+    return kPiO2 - pt_asin(x);
+}
+PT_HD float pt_atan(float x) {
+    float a = x < 0.0f ? -x : x;
+    float y0;
+    if (a > 2.414213562373095f) { y0 = kPiO2; a = -(1.0f / a); }
+    else if (a > 0.4142135623730950f) { y0 = kPiO4; a = (a - 1.0f) / (a + 1.0f); }
+    else { y0 = 0.0f; }
+    float z = a * a;
+    float r = y0 + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z
+                     - 3.33329491539e-1f) * z * a + a);
+    return x < 0.0f ? -r : r;
+}
+PT_HD float pt_atan2(float y, float x) {
+    if (x == 0.0f) {
+        if (y > 0.0f) return kPiO2;
+        if (y < 0.0f) return -kPiO2;
+        return 0.0f;
+    }
+    float r = pt_atan(y / x);
+    if (x < 0.0f) r = (y < 0.0f) ? r - kPi : r + kPi;
+    return r;
+}
+
+// -------------------------------------------------------------------- PCG32
+// O'Neill 2014 (pcg32_random_r / pcg32_srandom_r); one stream per
+// (seed, pixel, sample, bounce): initstate = pixel << 32 | sample,
+// initseq = seed << 8 | bounce.  Floats are the top 24 bits times 2^-24.
+struct Pcg { uint64_t state, inc; };
+PT_HD uint32_t pcg_next(Pcg& r) {
+    uint64_t old = r.state;
+    r.state = old * 6364136223846793005ULL + r.inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+PT_HD Pcg pcg_keyed(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce) {
+    Pcg r{0u, ((((uint64_t)seed << 8) | (bounce & 0xffu)) << 1u) | 1u};
+    pcg_next(r);
+    r.state += ((uint64_t)pixel << 32) | sample;
+    pcg_next(r);
+    return r;
+}
+PT_HD float pcg_float(Pcg& r) { return (float)(pcg_next(r) >> 8) * 0x1p-24f; }
+constexpr uint32_t kJitterBounce = 255u;
+
+// ------------------------------------------------------------------ camera
+PT_HD void camera_ray(const CameraParams& c, uint32_t px, uint32_t py, float jx, float jy, V3* o, V3* d) {
+    float u = ((float)px + jx) / (float)c.width;
+    float v = ((float)py + jy) / (float)c.height;
+    float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
+    const float* ip = c.inv_proj;
+    V3 q = v3(((ip[0] * nx + ip[1] * ny) + ip[2]) + ip[3],
+              ((ip[4] * nx + ip[5] * ny) + ip[6]) + ip[7],
+              ((ip[8] * nx + ip[9] * ny) + ip[10]) + ip[11]);
+    V3 dl = normalize(q);
+    const float* iv = c.inv_view;
+    V3 dw = v3(iv[0] * dl.x + iv[1] * dl.y + iv[2] * dl.z,
+               iv[4] * dl.x + iv[5] * dl.y + iv[6] * dl.z,
+               iv[8] * dl.x + iv[9] * dl.y + iv[10] * dl.z);
+    *d = normalize(dw);
+    *o = v3(iv[3], iv[7], iv[11]);
+}
+
+// Position in path order -> pixel index (8x8 tiles when both sides are multiples of 8).
+PT_HD uint32_t pixel_of(uint32_t q, uint32_t W, uint32_t H) {
+    if ((W & 7u) == 0 && (H & 7u) == 0) {
+        uint32_t tile = q >> 6, within = q & 63u;
+        uint32_t tiles_x = W >> 3;
+        uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+        return ((ty << 3) + (within >> 3)) * W + (tx << 3) + (within & 7u);
+    }
+    return q;
+}
+
+// ---------------------------------------------------------------- material
+struct Mat {
+    V3 base, f0, f90;
+    float rx, ry;
+    bool two_sided;
+};
+PT_HD Mat load_mat(const MaterialRec& m) {
+    Mat r;
+    r.base = v3(m.base_color[0], m.base_color[1], m.base_color[2]);
+    r.f0 = v3(m.f0_color[0], m.f0_color[1], m.f0_color[2]);
+    r.f90 = v3(m.f90_color[0], m.f90_color[1], m.f90_color[2]);
+    r.two_sided = m.two_sided != 0u;
+    float aniso = __builtin_sqrtf(1.0f - m.anisotropy * 0.9f);   // get_anisotropic_roughness
+    float r2 = m.roughness * m.roughness;
+    r.rx = sel_max(r2 / aniso, 0.001f);
+    r.ry = sel_max(r2 * aniso, 0.001f);
+    return r;
+}
+
+struct Frame { V3 t, b, n; };
+PT_HD Frame make_frame(V3 n) {
+    float sign = n.z > 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    Frame f;
+    f.t = v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f.b = v3(b, sign + n.y * n.y * a, -n.y);
+    f.n = n;
+    return f;
+}
+PT_HD V3 to_local(const Frame& f, V3 v) { return v3(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+PT_HD V3 to_world(const Frame& f, V3 v) { return (f.t * v.x + f.b * v.y) + f.n * v.z; }
+
+PT_HD float sq(float x) { return x * x; }
+PT_HD float ggx_ndf(V3 h, float rx, float ry) {
+    float a = (sq(h.x / rx) + sq(h.y / ry)) + sq(h.z);
+    return kInvPi / (((rx * ry) * a) * a);
+}
+PT_HD float ggx_g1(V3 v, float rx, float ry) {
+    float a = (sq(rx * v.x) + sq(ry * v.y)) / sel_max(v.z * v.z, 0.0001f);
+    return 2.0f / (1.0f + __builtin_sqrtf(1.0f + a));
+}
+PT_HD float ggx_vis_hc(V3 v, V3 l, float rx, float ry) {
+    float a = l.z * __builtin_sqrtf((sq(rx * v.x) + sq(ry * v.y)) + sq(v.z));
+    float b = v.z * __builtin_sqrtf((sq(rx * l.x) + sq(ry * l.y)) + sq(l.z));
+    return 0.5f / sel_max(a + b, 0.0001f);
+}
+// surface_eval_lit: diffuse + specular, both including cos(theta_l).
+PT_HD V3 bsdf_eval(const Frame& fr, const Mat& m, V3 V, V3 L) {
+    V3 H = normalize(V + L);
+    V3 lh = to_local(fr, H), lv = to_local(fr, V), ll = to_local(fr, L);
+    if (lv.z <= 0.0f || ll.z <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    float vh = dot(V, H);
+    float c = sel_max(vh, 0.0f);
+    float x = 1.0f - c;
+    float p5 = ((x * x) * (x * x)) * x;
+    V3 fres = m.f0 + (m.f90 - m.f0) * p5;
+    float lz = sel_max(ll.z, 0.0f);
+    V3 one = v3(1.0f, 1.0f, 1.0f);
+    V3 diffuse = (((one - fres) * m.base) * kInvPi) * lz;
+    float ndf = ggx_ndf(lh, m.rx, m.ry);
+    float vis = ggx_vis_hc(lv, ll, m.rx, m.ry);
+    V3 spec = ((fres * ndf) * vis) * lz;
+    return diffuse + spec;
+}
+// ggx_vndf_sample (Heitz 2018)
+PT_HD V3 vndf_sample(V3 v, float rx, float ry, float u0, float u1) {
+    if (v.z < 0.0f) v = -v;
+    V3 vh = normalize(v3(rx * v.x, ry * v.y, v.z));
+    float len_sqr = vh.x * vh.x + vh.y * vh.y;
+    V3 t1v;
+    if (len_sqr > 0.0f) {
+        float s = __builtin_sqrtf(len_sqr);
+        t1v = v3(-vh.y / s, vh.x / s, 0.0f / s);
+    } else {
+        t1v = v3(1.0f, 0.0f, 0.0f);
+    }
+    V3 t2v = cross(vh, t1v);
+    float r = __builtin_sqrtf(u0);
+    float phi = kTwoPi * u1;
+    float sp, cp;
+    pt_sincos(phi, &sp, &cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
+    float s = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - s) * __builtin_sqrtf(1.0f - t1 * t1) + s * t2;
+    V3 nh = (t1v * t1 + t2v * t2) + vh * __builtin_sqrtf(sel_max((1.0f - t1 * t1) - t2 * t2, 0.0f));
+    return normalize(v3(rx * nh.x, ry * nh.y, sel_max(nh.z, 0.0f)));
+}
+// ggx_vndf_sample_pdf / (4 |h.v|): density of the reflected direction.
+PT_HD float vndf_pdf_l(V3 h, V3 v, float rx, float ry) {
+    float hv = dot(h, v);
+    float pdf_h = ((ggx_g1(v, rx, ry) * ggx_ndf(h, rx, ry)) * sel_max(hv, 0.0f)) / sel_max(v.z, 0.0001f);
+    float ahv = hv < 0.0f ? -hv : hv;
+    return pdf_h / (4.0f * ahv);
+}
+PT_HD float power_heuristic(float a, float b) {
+    float a2 = a * a;
+    return a2 / (a2 + b * b);
+}
+
+// --------------------------------------------------------------------- sky
+PT_HD V3 sky_to_local(const SkyParams& s, V3 d) {
+    const float* R = s.rot;
+    return v3((R[0] * d.x + R[1] * d.y) + R[2] * d.z,
+              (R[3] * d.x + R[4] * d.y) + R[5] * d.z,
+              (R[6] * d.x + R[7] * d.y) + R[8] * d.z);
+}
+PT_HD void sky_uv(V3 ds, float* u, float* v, float* theta) {
+    float y = sel_min(sel_max(ds.y, -1.0f), 1.0f);
+    float th = pt_acos(y);
+    float ph = ds.z == 0.0f ? 0.0f : pt_atan2(ds.x, ds.z);   // equitangular_to_cubemap.hlsl:29
+    if (ph < 0.0f) ph += kTwoPi;
+    *u = ph * kInvTwoPi;
+    *v = th * kInvPi;
+    *theta = th;
+}
+__device__ __forceinline__ V3 sky_texel(const SkyParams& s, int i, int j) {
+    float4 t = s.texels[(size_t)i * s.width + j];
+    return v3(t.x, t.y, t.z);
+}
+// Bilinear lookup (texel centres at (i+0.5)/W, wrap in u, clamp in v) times the sky colour.
+__device__ __forceinline__ V3 sky_eval(const SkyParams& s, V3 d) {
+    if (s.width == 0) return v3(0.0f, 0.0f, 0.0f);
+    float u, v, th;
+    sky_uv(sky_to_local(s, d), &u, &v, &th);
+    const int W = (int)s.width, H = (int)s.height;
+    float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y);
+    float fx = x - fx0, fy = y - fy0;
+    int ix0 = (int)fx0, iy0 = (int)fy0;
+    int ix1 = ix0 + 1, iy1 = iy0 + 1;
+    ix0 = ix0 < 0 ? ix0 + W : (ix0 >= W ? ix0 - W : ix0);
+    ix1 = ix1 < 0 ? ix1 + W : (ix1 >= W ? ix1 - W : ix1);
+    iy0 = iy0 < 0 ? 0 : (iy0 > H - 1 ? H - 1 : iy0);
+    iy1 = iy1 < 0 ? 0 : (iy1 > H - 1 ? H - 1 : iy1);
+    V3 c00 = sky_texel(s, iy0, ix0), c10 = sky_texel(s, iy0, ix1);
+    V3 c01 = sky_texel(s, iy1, ix0), c11 = sky_texel(s, iy1, ix1);
+    V3 top = c00 + (c10 - c00) * fx;
+    V3 bot = c01 + (c11 - c01) * fx;
+    V3 c = top + (bot - top) * fy;
+    return c * v3(s.color[0], s.color[1], s.color[2]);
+}
+__device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
+    if (!(s.total > 0.0f)) return 0.0f;
+    float u, v, th;
+    sky_uv(sky_to_local(s, d), &u, &v, &th);
+    const int W = (int)s.width, H = (int)s.height;
+    int j = (int)(u * (float)W), i = (int)(v * (float)H);
+    j = j < 0 ? 0 : (j > W - 1 ? W - 1 : j);
+    i = i < 0 ? 0 : (i > H - 1 ? H - 1 : i);
+    float w = s.weight[(size_t)i * W + j];
+    float st = pt_sin(th);
+    if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
+    return (w * s.wh_over_total) / (kTwoPiSq * st);
+}
+// Piecewise-constant importance sampling of luminance * sin(theta) over the equirect.
+__device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u1, V3* dir, float* pdf) {
+    *pdf = 0.0f;
+    if (!(s.total > 0.0f)) return false;
+    const int W = (int)s.width, H = (int)s.height;
+    float t0 = u0 * s.total;
+    int lo = 0, hi = H - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (s.marg[mid + 1] > t0) hi = mid; else lo = mid + 1;
+    }
+    int i = lo;
+    float rs = s.rows[i];
+    if (!(rs > 0.0f)) return false;
+    float dv = (t0 - s.marg[i]) / rs;
+    dv = dv < 0.0f ? 0.0f : (dv >= 1.0f ? 0x1.fffffep-1f : dv);
+    float t1 = u1 * rs;
+    const float* cc = s.ccdf + (size_t)i * (W + 1);
+    lo = 0; hi = W - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cc[mid + 1] > t1) hi = mid; else lo = mid + 1;
+    }
+    int j = lo;
+    float w = s.weight[(size_t)i * W + j];
+    if (!(w > 0.0f)) return false;
+    float du = (t1 - cc[j]) / w;
+    du = du < 0.0f ? 0.0f : (du >= 1.0f ? 0x1.fffffep-1f : du);
+    float u = ((float)j + du) / (float)W;
+    float v = ((float)i + dv) / (float)H;
+    float st, ct, sp, cp;
+    pt_sincos(v * kPi, &st, &ct);
+    pt_sincos(u * kTwoPi, &sp, &cp);
+    if (!(st > 0.0f)) return false;
+    V3 ds = v3(st * sp, ct, st * cp);
+    const float* R = s.rot;
+    *dir = v3((R[0] * ds.x + R[3] * ds.y) + R[6] * ds.z,
+              (R[1] * ds.x + R[4] * ds.y) + R[7] * ds.z,
+              (R[2] * ds.x + R[5] * ds.y) + R[8] * ds.z);
+    *pdf = (w * s.wh_over_total) / (kTwoPiSq * st);
+    return true;
+}
+
+}  // namespace bipt

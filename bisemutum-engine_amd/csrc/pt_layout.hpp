@@ -1,0 +1,132 @@
+// pt_layout.hpp -- device data layout of the path-tracing pass (host and device).
+//
+// HBM layout (one context):
+//   scene      mesh pool (float xyz positions/normals, uint32 indices), instance records,
+//              material table (64 B each), equirect sky + importance tables
+//   per frame  world triangles (36 B) + normals (36 B) in instance order, Morton keys,
+//              sorted order, BVH2 nodes (64 B: both child boxes + child links, one node
+//              per 128-B line half), leaf intersection records (48 B: v0, e1, e2) and
+//              leaf shading records (48 B: n0, n1, n2, material) in sorted leaf order
+//   per wave   SoA ray queues (o.xyz, d.xyz, path id), SoA hit records (t, leaf, u, v),
+//              SoA path state (throughput, last BSDF pdf), per-kind radiance accumulators,
+//              SoA shadow queue (o.xyz, d.xyz, contribution.xyz, path<<3|kind)
+// Nodes are read whole by one lane (4 x 16-B loads); rays, hits and path state are
+// read by consecutive lanes at consecutive addresses (SoA, coalesced).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bipt {
+
+constexpr int kMaxDirLights = 4;
+constexpr int kMaxKinds = kMaxDirLights + 2;   // miss, lights..., sky
+constexpr int kStackMax = 64;                  // >= 30 Morton + 34 index levels (Karras depth bound)
+constexpr uint32_t kLeafMiss = 0xffffffffu;
+constexpr uint32_t kKindSky = 7u;
+
+struct alignas(16) BvhNode {
+    float box[12];     // child0 lo.xyz hi.xyz, child1 lo.xyz hi.xyz
+    int32_t child[2];  // internal k >= 0, leaf j as ~j
+    int32_t pad[2];
+};
+static_assert(sizeof(BvhNode) == 64, "node record is 64 bytes");
+
+struct alignas(16) LeafGeo {
+    float v[12];       // v0.xyz, e1.xyz, e2.xyz, unused
+};
+struct alignas(16) LeafShade {
+    float n[9];        // n0, n1, n2 (cofactor-transformed, not normalised)
+    uint32_t material;
+    uint32_t pad[2];
+};
+static_assert(sizeof(LeafGeo) == 48 && sizeof(LeafShade) == 48, "leaf records are 48 bytes");
+
+struct MaterialRec {   // = bi_pt_material
+    float base_color[3];
+    float roughness;
+    float f0_color[3];
+    float anisotropy;
+    float f90_color[3];
+    float ior;
+    uint32_t two_sided;
+    uint32_t pad[3];
+};
+static_assert(sizeof(MaterialRec) == 64, "material record is 64 bytes");
+
+struct LightRec {      // = bi_pt_dir_light (DirLightData)
+    float emission[3];
+    int32_t sm_index;
+    float direction[3];
+    float shadow_strength;
+    float cascade[4];
+    float depth_bias, normal_bias;
+    float pad[2];
+};
+static_assert(sizeof(LightRec) == 64, "light record is 64 bytes");
+
+struct InstanceRec {
+    float xform[12];
+    uint32_t first_tri;   // first world triangle of this instance
+    uint32_t idx_off;     // first index of its mesh in the index pool
+    uint32_t vtx_off;     // first vertex of its mesh in the vertex pool
+    uint32_t material;
+};
+static_assert(sizeof(InstanceRec) == 64, "instance record is 64 bytes");
+
+struct CameraParams {
+    float inv_view[16];
+    float inv_proj[16];
+    float proj_view[16];
+    uint32_t width, height;
+};
+
+struct SkyParams {
+    const float4* texels;
+    const float* weight;   // luminance * sin(theta) per texel
+    const float* ccdf;     // per row: W+1 running sums
+    const float* rows;     // per row sum
+    const float* marg;     // H+1 running sums of rows
+    uint32_t width, height;
+    float total, wh_over_total;
+    float color[3];
+    float rot[9];
+};
+
+struct BvhView {
+    const BvhNode* nodes;
+    const LeafGeo* leaf_geo;
+    uint32_t num_tris;
+};
+
+// Ray queue in SoA form.
+struct RayQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    uint32_t* path;
+};
+struct HitBuf {
+    float *t, *u, *v;
+    uint32_t* leaf;
+};
+struct ShadowQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *cx, *cy, *cz;
+    uint32_t* pk;   // path << 3 | kind
+};
+struct PathState {
+    float *bx, *by, *bz, *pdf;
+    float* acc;       // [kind][3][cap]
+    uint32_t cap;
+};
+
+// Device-side counters of one wave (uint32 slots).
+enum CounterSlot : uint32_t {
+    kCntExtA = 0,       // entries in extension queue A
+    kCntExtB = 1,       // entries in extension queue B
+    kCntShadow = 2,     // entries in the shadow queue
+    kCntTicket = 3,     // dynamic tile ticket of the compaction kernel
+    kCntError = 4,      // nonzero: a kernel hit an internal limit
+    kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
+    kCntSlots = 16
+};
+
+}  // namespace bipt

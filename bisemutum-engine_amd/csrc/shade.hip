@@ -1,0 +1,331 @@
+// shade.hip -- surface shading, next-event estimation and queue compaction (gfx950).
+//
+// One kernel per segment replaces the reference's per-bounce passes
+// sample_secondary_ray_cs (sample_secondary_ray.hlsl:11-69) and
+// deferred_lighting_secondary_cs (deferred_lighting_secondary.hlsl:11-111):
+//   miss   -> sky radiance * throughput (MIS-weighted against sky NEE after bounce 0)
+//   hit    -> interpolated normal, lit BSDF; one shadow ray per directional light
+//             (exact visibility replaces the shadow map); one sky NEE shadow ray;
+//             VNDF-sampled extension ray while max(throughput) >= 0.001
+// Instead of full-screen passes with dead lanes, live rays are compacted into
+// the next queue: a wavefront ballot/prefix scan inside each 256-lane tile, and a
+// single-pass decoupled look-back (Merrill & Garland 2016) across tiles taken in
+// ticket order, so queue order equals input order (deterministic contents).
+#include "pt_kernels.hpp"
+
+namespace bipt {
+namespace {
+
+constexpr uint32_t kTile = 256;
+constexpr uint64_t kStatusAgg = 1ull << 62;
+constexpr uint64_t kStatusPrefix = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 31) - 1ull;
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+__device__ __forceinline__ uint64_t pack_desc(uint64_t status, uint32_t e, uint32_t s) {
+    return status | ((uint64_t)e << 31) | (uint64_t)s;
+}
+
+struct ShadowOut {
+    V3 c;
+    V3 d;
+    uint32_t kind;
+};
+
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_excl[2];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const bool primary = a.bounce == 0;
+    const uint32_t count = primary ? a.wave.num_paths : *a.count_in;
+    const uint32_t ntiles = (count + kTile - 1) / kTile;
+    const uint32_t nl = a.num_lights;
+    const uint32_t nslots = nl + 2u;
+    const size_t cap = a.ps.cap;
+    if (count == 0) {
+        if (blockIdx.x == 0 && tid == 0) { *a.count_out = 0u; *a.count_shadow = 0u; }
+        return;
+    }
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        __syncthreads();
+        if (tile >= ntiles) break;
+        const uint32_t i = tile * kTile + tid;
+        const bool active = i < count;
+
+        uint32_t n_ext = 0, n_sh = 0;
+        uint32_t path = 0;
+        V3 P = v3(0.f, 0.f, 0.f), Lext = v3(0.f, 0.f, 0.f), beta_next = v3(0.f, 0.f, 0.f);
+        float pdf_next = 0.0f;
+        ShadowOut sh[kMaxDirLights + 1];
+        if (active) {
+            path = primary ? i : a.qin.path[i];
+            const uint32_t q = path % a.wave.npix, sl = path / a.wave.npix;
+            const uint32_t pixel = pixel_of(q, a.wave.cam.width, a.wave.cam.height);
+            const uint32_t sample = a.wave.sample0 + sl;
+            V3 o, d;
+            V3 beta;
+            float pdf_prev;
+            if (primary) {
+                float jx = 0.5f, jy = 0.5f;
+                if (a.wave.flags & 1u) {
+                    Pcg jr = pcg_keyed(a.wave.seed, pixel, sample, kJitterBounce);
+                    jx = pcg_float(jr);
+                    jy = pcg_float(jr);
+                }
+                camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &o, &d);
+                beta = v3(1.0f, 1.0f, 1.0f);
+                pdf_prev = 0.0f;
+                for (uint32_t k = 0; k < nslots; ++k) {
+                    float* acc = a.ps.acc + (size_t)k * 3u * cap;
+                    acc[path] = 0.0f;
+                    acc[cap + path] = 0.0f;
+                    acc[2 * cap + path] = 0.0f;
+                }
+            } else {
+                o = v3(a.qin.ox[i], a.qin.oy[i], a.qin.oz[i]);
+                d = v3(a.qin.dx[i], a.qin.dy[i], a.qin.dz[i]);
+                beta = v3(a.ps.bx[path], a.ps.by[path], a.ps.bz[path]);
+                pdf_prev = a.ps.pdf[path];
+            }
+            const uint32_t leaf = a.hits.leaf[i];
+            const bool want_depth = primary && a.depth_out != nullptr && sample == a.depth_sample;
+            if (leaf == kLeafMiss) {
+                V3 ls = sky_eval(a.sky, d);
+                float w = 1.0f;
+                if (!primary && a.sky_nee) w = power_heuristic(pdf_prev, sky_pdf(a.sky, d));
+                V3 c = (beta * ls) * w;
+                a.ps.acc[path] = c.x;
+                a.ps.acc[cap + path] = c.y;
+                a.ps.acc[2 * cap + path] = c.z;
+                if (want_depth) a.depth_out[pixel] = 0.0f;
+            } else {
+                const float t = a.hits.t[i], hu = a.hits.u[i], hv = a.hits.v[i];
+                P = o + d * t;
+                const LeafShade& ls = a.leaf_shade[leaf];
+                V3 n0 = v3(ls.n[0], ls.n[1], ls.n[2]), n1 = v3(ls.n[3], ls.n[4], ls.n[5]);
+                V3 n2 = v3(ls.n[6], ls.n[7], ls.n[8]);
+                V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
+                Mat m = load_mat(a.materials[ls.material]);
+                if (m.two_sided && dot(d, N) > 0.0f) N = -N;
+                V3 V = normalize(o - P);
+                Frame fr = make_frame(N);
+                Pcg rng = pcg_keyed(a.wave.seed, pixel, sample, a.bounce);
+                float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
+                for (uint32_t l = 0; l < nl; ++l) {
+                    const LightRec& L = a.lights[l];
+                    V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
+                    V3 ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
+                    V3 f = bsdf_eval(fr, m, V, ldir);
+                    V3 c = (le * f) * beta;
+                    if (!any_pos(c)) continue;
+                    if (L.sm_index < 0) {
+                        float* acc = a.ps.acc + (size_t)(l + 1u) * 3u * cap;
+                        acc[path] = acc[path] + c.x;
+                        acc[cap + path] = acc[cap + path] + c.y;
+                        acc[2 * cap + path] = acc[2 * cap + path] + c.z;
+                        continue;
+                    }
+                    sh[n_sh].c = c;
+                    sh[n_sh].d = ldir;
+                    sh[n_sh].kind = l;
+                    ++n_sh;
+                }
+                // sky NEE only where the next segment is sampled too (the reference sees the
+                // sky only through a traced segment that escapes)
+                const bool extend = a.bounce < a.last_bounce && !(max3(beta) < kThroughputMin);
+                if (a.sky_nee && extend) {
+                    V3 wi;
+                    float pl;
+                    if (sky_sample(a.sky, u0, u1, &wi, &pl)) {
+                        V3 f = bsdf_eval(fr, m, V, wi);
+                        if (any_pos(f)) {
+                            V3 hl = to_local(fr, normalize(V + wi));
+                            float pb = vndf_pdf_l(hl, to_local(fr, V), m.rx, m.ry);
+                            float w = power_heuristic(pl, pb);
+                            V3 c = ((sky_eval(a.sky, wi) * f) * (w / pl)) * beta;
+                            if (any_pos(c) && finite3(c)) {
+                                sh[n_sh].c = c;
+                                sh[n_sh].d = wi;
+                                sh[n_sh].kind = kKindSky;
+                                ++n_sh;
+                            }
+                        }
+                    }
+                }
+                if (extend) {
+                    V3 vl = to_local(fr, V);
+                    V3 hh = vndf_sample(vl, m.rx, m.ry, u2, u3);
+                    float hvd = dot(vl, hh);
+                    V3 ll = hh * (2.0f * hvd) - vl;
+                    float pdf = vndf_pdf_l(hh, vl, m.rx, m.ry);
+                    Lext = to_world(fr, ll);
+                    V3 f = bsdf_eval(fr, m, V, Lext);
+                    V3 w = v3(f.x / pdf, f.y / pdf, f.z / pdf);
+                    if (!finite3(w)) w = v3(0.0f, 0.0f, 0.0f);
+                    V3 nb = beta * w;
+                    if (max3(nb) > 0.0f) {
+                        n_ext = 1;
+                        beta_next = nb;
+                        pdf_next = pdf;
+                    }
+                }
+                if (want_depth) {
+                    const float* pv = a.wave.cam.proj_view;
+                    float z = ((pv[8] * P.x + pv[9] * P.y) + pv[10] * P.z) + pv[11];
+                    float w = ((pv[12] * P.x + pv[13] * P.y) + pv[14] * P.z) + pv[15];
+                    a.depth_out[pixel] = z / w;
+                }
+            }
+        }
+
+        // ---- tile scan of (extension, shadow) emissions: wavefront prefix, then LDS
+        uint32_t packed = (n_ext << 16) | n_sh;
+        uint32_t inc = packed;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(inc, off);
+            if (lane >= (uint32_t)off) inc += y;
+        }
+        if (lane == 63) s_wsum[wave] = inc;
+        __syncthreads();
+        uint32_t wbase = 0, agg = 0;
+        for (uint32_t w = 0; w < 4; ++w) {
+            uint32_t s = s_wsum[w];
+            if (w < wave) wbase += s;
+            agg += s;
+        }
+        const uint32_t excl_local = wbase + inc - packed;
+
+        // ---- decoupled look-back across tiles (thread 0)
+        if (tid == 0) {
+            const uint32_t agg_e = agg >> 16, agg_s = agg & 0xffffu;
+            uint32_t pe = 0, ps = 0;
+            if (tile == 0) {
+                __hip_atomic_store(&a.desc[0], pack_desc(kStatusPrefix, agg_e, agg_s), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&a.desc[tile], pack_desc(kStatusAgg, agg_e, agg_s), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                int j = (int)tile - 1;
+                uint32_t spins = 0;
+                while (j >= 0) {
+                    uint64_t dsc = __hip_atomic_load(&a.desc[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    uint64_t st = dsc >> 62;
+                    if (st == 0) {
+                        if (++spins > kSpinLimit) { atomicOr(a.error, 2u); break; }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    pe += (uint32_t)((dsc >> 31) & kValMask);
+                    ps += (uint32_t)(dsc & kValMask);
+                    if (st == 2) break;
+                    --j;
+                }
+                __hip_atomic_store(&a.desc[tile], pack_desc(kStatusPrefix, pe + agg_e, ps + agg_s), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_excl[0] = pe;
+            s_excl[1] = ps;
+            if (tile == ntiles - 1) {
+                *a.count_out = pe + agg_e;
+                *a.count_shadow = ps + agg_s;
+                atomicAdd((unsigned long long*)&a.ray_counters[1], (unsigned long long)(pe + agg_e));
+            }
+        }
+        __syncthreads();
+        const uint32_t base_e = s_excl[0] + (excl_local >> 16);
+        const uint32_t base_s = s_excl[1] + (excl_local & 0xffffu);
+        if (n_ext) {
+            a.qout.ox[base_e] = P.x; a.qout.oy[base_e] = P.y; a.qout.oz[base_e] = P.z;
+            a.qout.dx[base_e] = Lext.x; a.qout.dy[base_e] = Lext.y; a.qout.dz[base_e] = Lext.z;
+            a.qout.path[base_e] = path;
+            a.ps.bx[path] = beta_next.x; a.ps.by[path] = beta_next.y; a.ps.bz[path] = beta_next.z;
+            a.ps.pdf[path] = pdf_next;
+        }
+        uint32_t n_light = 0, n_sky = 0;
+        for (uint32_t k = 0; k < n_sh; ++k) {
+            const uint32_t o = base_s + k;
+            a.sq.ox[o] = P.x; a.sq.oy[o] = P.y; a.sq.oz[o] = P.z;
+            a.sq.dx[o] = sh[k].d.x; a.sq.dy[o] = sh[k].d.y; a.sq.dz[o] = sh[k].d.z;
+            a.sq.cx[o] = sh[k].c.x; a.sq.cy[o] = sh[k].c.y; a.sq.cz[o] = sh[k].c.z;
+            a.sq.pk[o] = (path << 3) | sh[k].kind;
+            if (sh[k].kind == kKindSky) ++n_sky; else ++n_light;
+        }
+        // per-tile ray statistics: one atomic per wavefront
+        uint32_t cl = n_light, cs = n_sky;
+        for (int off = 32; off >= 1; off >>= 1) {
+            cl += __shfl_xor(cl, off);
+            cs += __shfl_xor(cs, off);
+        }
+        if (lane == 0) {
+            if (cl) atomicAdd((unsigned long long*)&a.ray_counters[2], (unsigned long long)cl);
+            if (cs) atomicAdd((unsigned long long*)&a.ray_counters[3], (unsigned long long)cs);
+        }
+    }
+}
+
+// Per pixel (path order q): running sum += radiance of each sample in sample order.
+__global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= a.npix) return;
+    const uint32_t pixel = pixel_of(q, a.width, a.height);
+    const size_t cap = a.ps.cap;
+    V3 sum = v3(a.sums[3 * (size_t)pixel], a.sums[3 * (size_t)pixel + 1], a.sums[3 * (size_t)pixel + 2]);
+    for (uint32_t s = 0; s < a.samples; ++s) {
+        const size_t p = (size_t)s * a.npix + q;
+        V3 r = v3(0.f, 0.f, 0.f);
+        for (uint32_t k = 0; k < a.num_slots; ++k) {
+            const float* acc = a.ps.acc + (size_t)k * 3u * cap;
+            V3 c = v3(acc[p], acc[cap + p], acc[2 * cap + p]);
+            r = k == 0 ? c : r + c;
+        }
+        sum = sum + r;
+    }
+    a.sums[3 * (size_t)pixel] = sum.x;
+    a.sums[3 * (size_t)pixel + 1] = sum.y;
+    a.sums[3 * (size_t)pixel + 2] = sum.z;
+}
+
+__global__ __launch_bounds__(256) void k_output(const float* __restrict__ sums, uint32_t npix, float den,
+                                                float* __restrict__ out) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= npix) return;
+    float4 o;
+    o.x = sums[3 * (size_t)p] / den;
+    o.y = sums[3 * (size_t)p + 1] / den;
+    o.z = sums[3 * (size_t)p + 2] / den;
+    o.w = 1.0f;
+    reinterpret_cast<float4*>(out)[p] = o;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s) {
+    uint32_t tiles = (capacity + kTile - 1) / kTile;
+    uint32_t grid = tiles < 4096u ? tiles : 4096u;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_resolve(const ResolveArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve, dim3((a.npix + 255u) / 256u), dim3(256), 0, s, a);
+}
+
+void launch_output(const float* sums, uint32_t npix, float den, uint32_t, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_output, dim3((npix + 255u) / 256u), dim3(256), 0, s, sums, npix, den, out);
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255u) / 256u), dim3(256), 0, s, p, value, n);
+}
+
+}  // namespace bipt

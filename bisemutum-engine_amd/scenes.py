@@ -82,7 +82,7 @@ def material(base_color, roughness, metallic, two_sided=True, anisotropy=0.0):
     rec[7] = anisotropy
     rec[8:11] = 1.0
     rec[11] = 1.5
-    rec[12] = 1.0 if two_sided else 0.0
+    rec.view(np.uint32)[12] = 1 if two_sided else 0
     return rec
 
 

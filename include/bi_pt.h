@@ -1,0 +1,209 @@
+/*
+ * bi_pt.h -- C ABI of the MI355X-native wavefront path-tracing pass.
+ *
+ * Drop-in boundary for bisemutum's builtin-renderer path-tracing pass
+ * (paths relative to the reference's bisemutum/ directory):
+ *   PathTracingPass::update_params   src/renderer/pass/path_tracing.cpp:183-222
+ *   PathTracingPass::render          src/renderer/pass/path_tracing.hpp:27-35, .cpp:224-488
+ *   InputData / OutputData           src/renderer/pass/path_tracing.hpp:13-23
+ *   PathTracingSettings              include/bisemutum/renderer/basic.hpp:76-81
+ * The pass's render-graph inputs become explicit calls: scene geometry and
+ * materials (bi_pt_set_scene, replacing the TLAS/BLAS inputs of
+ * src/graphics/accel.cpp:11-159 and the scene buffers of
+ * src/graphics/gpu_scene_data.hpp:10-24), directional lights (bi_pt_set_lights,
+ * LightsContext, src/renderer/context/lights.cpp:48-95), the sky
+ * (bi_pt_set_sky, SkyboxContext / SkyboxSystem, src/scene_basic/skybox.cpp:43-73)
+ * and the camera (bi_pt_render, gfx::Camera matrices, src/graphics/camera.cpp:96-110).
+ * The output colour texture (rgba16f in the reference) is an RGBA32F device
+ * buffer; the optional depth output is reverse-Z device depth as pt_depth.hlsl.
+ *
+ * Conventions: matrices are row-major (m[row*4 + col]); device pointers are
+ * HIP allocations on the context's device; calls on one context are not
+ * thread-safe (externally serialised, as the reference records on one thread);
+ * work is stream-ordered on the caller's hipStream_t.  Every call returns a
+ * bi_pt_status; bi_pt_last_error(ctx) describes the last failure.
+ */
+#ifndef BI_PT_H
+#define BI_PT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* bi_pt_stream;  /* hipStream_t; NULL = default stream */
+
+typedef enum bi_pt_status {
+    BI_PT_OK = 0,
+    BI_PT_ERROR_INVALID_ARGUMENT = 1,
+    BI_PT_ERROR_OUT_OF_MEMORY = 2,     /* device allocation or HBM budget exhausted */
+    BI_PT_ERROR_HIP = 3,               /* a HIP runtime call failed */
+    BI_PT_ERROR_NO_SCENE = 4,
+    BI_PT_ERROR_DEVICE_CHECK = 5,      /* a kernel reported an internal-limit violation */
+    BI_PT_ERROR_UNSUPPORTED = 6
+} bi_pt_status;
+
+typedef struct bi_pt_ctx bi_pt_ctx;
+
+/* Device selection and memory policy. */
+typedef struct bi_pt_device_desc {
+    int32_t device;              /* HIP device ordinal */
+    uint32_t _pad;
+    uint64_t hbm_budget_bytes;   /* 0 = no limit; allocations beyond it fail with OUT_OF_MEMORY */
+} bi_pt_device_desc;
+
+/* One mesh (host memory, copied at bi_pt_set_scene): float xyz positions and
+ * normals (flat float buffers of 3 per vertex, as the reference's
+ * positions_buffer / normals_buffer), uint32 triangle-list indices. */
+typedef struct bi_pt_mesh {
+    const float* positions;
+    const float* normals;
+    const uint32_t* indices;
+    uint32_t num_vertices;
+    uint32_t num_indices;
+} bi_pt_mesh;
+
+/* One drawable: 3x4 row-major object-to-world transform, laid out like
+ * rhi::AccelerationStructureInstanceDesc::transform (include/bisemutum/rhi/accel.hpp:48-55). */
+typedef struct bi_pt_instance {
+    float transform[12];
+    uint32_t mesh;
+    uint32_t material;
+} bi_pt_instance;
+
+/* Lit material (SurfaceData of shaders/core/material/utils.hlsl:5-16 restricted
+ * to parameter materials).  64 bytes.  anisotropy follows
+ * get_anisotropic_roughness; ior is only used for cos < 0 (never for reflection). */
+typedef struct bi_pt_material {
+    float base_color[3];
+    float roughness;
+    float f0_color[3];
+    float anisotropy;
+    float f90_color[3];
+    float ior;
+    uint32_t two_sided;
+    uint32_t _pad[3];
+} bi_pt_material;
+
+/* Byte-identical to DirLightData (shaders/renderer/lights_struct.hlsl:3-14,
+ * src/renderer/context/lights.hpp:12-22).  64 bytes.  emission = colour*strength,
+ * direction points TO the light.  sm_index < 0: casts no shadow (no shadow ray);
+ * otherwise an exact shadow ray replaces the shadow map and an occluded sample
+ * keeps (1 - shadow_strength) of the light (lights.hlsl:62-63).  The cascade and
+ * bias fields are accepted and ignored. */
+typedef struct bi_pt_dir_light {
+    float emission[3];
+    int32_t sm_index;
+    float direction[3];
+    float shadow_strength;
+    float cascade_shadow_radius_sqr[4];
+    float shadow_depth_bias;
+    float shadow_normal_bias;
+    float _pad1[2];
+} bi_pt_dir_light;
+
+/* Camera: matrices as gfx::Camera computes them (camera.cpp:96-110). */
+typedef struct bi_pt_camera {
+    float matrix_inv_view[16];
+    float matrix_inv_proj[16];
+    float matrix_proj_view[16];
+    uint32_t width;
+    uint32_t height;
+} bi_pt_camera;
+
+enum {
+    BI_PT_FLAG_JITTER = 1u << 0,         /* jitter primary rays inside the pixel (reference: pixel centre) */
+    BI_PT_FLAG_NO_SKY_NEE = 1u << 1,     /* sky only on BSDF-sampled misses, as the reference */
+    BI_PT_FLAG_RECORD_QUEUES = 1u << 8,  /* keep every bounce's queue contents for bi_pt_debug_queue */
+    BI_PT_FLAG_TIMING = 1u << 9          /* time each kernel class with HIP events (bi_pt_get_stats) */
+};
+
+/* PathTracingSettings (basic.hpp:76-81) plus the sample schedule.
+ * max_bounces keeps the reference's meaning: clamped to [2,16], the pass traces
+ * max_bounces-1 path segments.  accumulate: keep summing into the running image
+ * while the camera is unchanged (pt_accumulate.hlsl); otherwise restart. */
+typedef struct bi_pt_settings {
+    float ray_length;         /* TMax of camera/extension rays; farther geometry reads as sky */
+    uint32_t max_bounces;
+    uint32_t accumulate;
+    uint32_t spp;             /* samples per pixel rendered by this call */
+    uint32_t seed;            /* PCG32 streams keyed by (seed, pixel, sample, bounce) */
+    uint32_t flags;
+    uint32_t wave_paths;      /* paths in flight per wave; 0 = automatic */
+    uint32_t stack_lds;       /* traversal stack entries kept in LDS (0 = automatic, <= 32) */
+} bi_pt_settings;
+
+typedef struct bi_pt_stats {
+    uint64_t rays_camera;
+    uint64_t rays_extension;
+    uint64_t rays_light_shadow;
+    uint64_t rays_sky_shadow;
+    uint64_t frames;
+    uint64_t samples_accumulated;  /* per pixel, in the running image */
+    /* milliseconds of the last render call (BI_PT_FLAG_TIMING), per kernel class */
+    float ms_build;
+    float ms_trace_closest;
+    float ms_shade;
+    float ms_trace_shadow;
+    float ms_resolve;
+    float ms_total;
+    uint32_t launches_trace_closest;
+    uint32_t launches_trace_shadow;
+} bi_pt_stats;
+
+/* ---- lifecycle (PathTracingPass ctor, path_tracing.cpp:128-181) ---- */
+bi_pt_status bi_pt_create(const bi_pt_device_desc* desc, bi_pt_ctx** out_ctx);
+bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx);
+const char* bi_pt_last_error(const bi_pt_ctx* ctx);
+
+/* ---- inputs (update_params, path_tracing.cpp:183-222, and the TLAS/scene inputs) ---- */
+bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t num_meshes,
+                             const bi_pt_instance* instances, uint32_t num_instances,
+                             const bi_pt_material* materials, uint32_t num_materials);
+/* Replace instance transforms only (same count/meshes/materials); the LBVH is rebuilt by the next render. */
+bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* transforms_3x4, uint32_t num_instances);
+bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uint32_t num_lights);
+/* Equirect RGBA32F sky (host memory, row 0 = +Y pole), colour multiplier
+ * (SkyboxSystem color*strength) and 3x3 row-major rotation applied to directions
+ * before lookup (deferred_lighting_secondary.hlsl:25). */
+bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba32f, uint32_t width, uint32_t height,
+                           const float color[3], const float rotation_3x3[9]);
+
+/* ---- the pass (PathTracingPass::render, path_tracing.cpp:224-488) ----
+ * Rebuilds the LBVH, traces settings->spp samples per pixel and writes the
+ * running mean to out_rgba32f (device, width*height*4 floats).  out_depth
+ * (device, width*height floats, nullable) receives the primary hit's reverse-Z
+ * depth (0 on miss).  Stream-ordered; returns once the work is enqueued
+ * (unless statistics or queue recording require a synchronisation). */
+bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_pt_settings* settings,
+                          float* out_rgba32f, float* out_depth, bi_pt_stream stream);
+bi_pt_status bi_pt_reset_accumulation(bi_pt_ctx* ctx);
+bi_pt_status bi_pt_get_stats(const bi_pt_ctx* ctx, bi_pt_stats* out);
+
+/* ---- inspection hooks for parity tests (no reference counterpart) ----
+ * Copy the last-built LBVH to host memory: morton[n] (input triangle order),
+ * order[n] (sorted permutation), children[2*max(n-1,1)] (internal k >= 0, leaf j as ~j),
+ * child_bounds[12*max(n-1,1)] (per node: child0 lo.xyz hi.xyz, child1 lo.xyz hi.xyz),
+ * parents[max(n-1,1)+n] (internal nodes, then leaves; root -1), centroid_bounds[6]. */
+bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* order, int32_t* children,
+                                 float* child_bounds, int32_t* parents, float* centroid_bounds);
+uint32_t bi_pt_debug_num_tris(const bi_pt_ctx* ctx);
+/* Queue contents recorded by the last render with BI_PT_FLAG_RECORD_QUEUES:
+ * kind 0 = extension queue entering segment `bounce` (path ids), kind 1 = shadow
+ * queue of segment `bounce` (path_id << 3 | ray kind; kinds 0-3 lights, 7 sky).
+ * wave selects the wave of the call.  *count receives the length; at most capacity entries copied. */
+bi_pt_status bi_pt_debug_queue(bi_pt_ctx* ctx, uint32_t wave, uint32_t bounce, uint32_t kind,
+                               uint32_t* out, uint32_t capacity, uint32_t* count);
+/* Running per-pixel radiance sums (device -> host, 3 floats per pixel). */
+bi_pt_status bi_pt_debug_get_sums(bi_pt_ctx* ctx, float* sums_rgb);
+/* Batched traversal of the current LBVH: rays are host memory, 8 floats each
+ * (o.xyz, tmin, d.xyz, tmax); closest -> (t, leaf, u, v), any-hit -> leaf = occluded. */
+bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, int any_hit, float* hit_t,
+                               uint32_t* hit_leaf, float* hit_u, float* hit_v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -146,6 +146,11 @@ double ora_vndf_pdf64(const double* v_local, const double* l_local, double alpha
 void ora_sky_eval32(const ora_ctx* ctx, const float* dir, float* rgb);
 void ora_sky_sample32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf);
 float ora_sky_pdf32(const ora_ctx* ctx, const float* dir);
+/* Batched forms for quadrature tests. */
+void ora_bsdf64_batch(const float* material, const double* n, const double* v, uint32_t count, const double* l,
+                      double* out3);
+void ora_vndf_pdf64_batch(const double* v_local, uint32_t count, const double* l_local, double alpha, double* out);
+void ora_sky_batch32(const ora_ctx* ctx, uint32_t count, const float* dirs, float* rgb, float* pdf);
 /* Camera ray for pixel (px,py) and optional jitter (jx,jy in [0,1), 0.5 = centre). */
 void ora_camera_ray32(const ora_camera* cam, uint32_t px, uint32_t py, float jx, float jy,
                       float* o, float* d);

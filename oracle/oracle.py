@@ -90,6 +90,9 @@ def lib():
         L.ora_sky_pdf32.argtypes = [C.c_void_p, _f32p]
         L.ora_camera_ray32.argtypes = [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                        _f32p, _f32p]
+        L.ora_bsdf64_batch.argtypes = [_f32p, _f64p, _f64p, C.c_uint32, _f64p, _f64p]
+        L.ora_vndf_pdf64_batch.argtypes = [_f64p, C.c_uint32, _f64p, C.c_double, _f64p]
+        L.ora_sky_batch32.argtypes = [C.c_void_p, C.c_uint32, _f32p, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -300,6 +303,32 @@ def vndf_pdf64(v_local, l_local, alpha):
     v = np.ascontiguousarray(v_local, np.float64)
     l = np.ascontiguousarray(l_local, np.float64)
     return float(lib().ora_vndf_pdf64(ptr(v, _f64p), ptr(l, _f64p), alpha))
+
+
+def bsdf64_batch(mat, n, v, ls):
+    m = np.ascontiguousarray(mat, np.float32)
+    n = np.ascontiguousarray(n, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    ls = np.ascontiguousarray(ls, np.float64)
+    out = np.zeros((len(ls), 3), np.float64)
+    lib().ora_bsdf64_batch(ptr(m, _f32p), ptr(n, _f64p), ptr(v, _f64p), len(ls), ptr(ls, _f64p), ptr(out, _f64p))
+    return out
+
+
+def vndf_pdf64_batch(v_local, ls, alpha):
+    v = np.ascontiguousarray(v_local, np.float64)
+    ls = np.ascontiguousarray(ls, np.float64)
+    out = np.zeros(len(ls), np.float64)
+    lib().ora_vndf_pdf64_batch(ptr(v, _f64p), len(ls), ptr(ls, _f64p), alpha, ptr(out, _f64p))
+    return out
+
+
+def sky_batch32(scene, dirs):
+    d = np.ascontiguousarray(dirs, np.float32)
+    rgb = np.zeros((len(d), 3), np.float32)
+    pdf = np.zeros(len(d), np.float32)
+    lib().ora_sky_batch32(scene.h, len(d), ptr(d, _f32p), ptr(rgb, _f32p), ptr(pdf, _f32p))
+    return rgb, pdf
 
 
 def camera_ray32(cam, px, py, jx=0.5, jy=0.5):

@@ -226,7 +226,7 @@ void build_bvh(Scene& s) {
         for (int k = 0; k < 3; ++k) {
             float a = v[k], b = v[3 + k], c = v[6 + k];
             float l = fmin_(fmin_(a, b), c), h = fmax_(fmax_(a, b), c);
-            lo[3 * t + k] = l; hi[3 * t + k] = h;
+            lo[3 * t + k] = l + 0.0f; hi[3 * t + k] = h + 0.0f;  // -0 -> +0: box bits independent of order
             float ce = (l + h) * 0.5f;
             cmin[k] = fmin_(cmin[k], ce); cmax[k] = fmax_(cmax[k], ce);
         }
@@ -500,7 +500,9 @@ inline Mat load_mat(const float* m) {
     r.f0 = f3(m[4], m[5], m[6]);
     float aniso_p = m[7];
     r.f90 = f3(m[8], m[9], m[10]);
-    r.two_sided = m[12] != 0.0f;
+    uint32_t ts;
+    std::memcpy(&ts, &m[12], sizeof ts);
+    r.two_sided = ts != 0u;
     // get_anisotropic_roughness (utils.hlsl:54-59)
     float aniso = std::sqrt(1.0f - aniso_p * 0.9f);
     float r2 = rough * rough;
@@ -766,8 +768,12 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
                 if (keep > 0.0f) acc_dir[l] = acc_dir[l] + c * keep;
             }
         }
-        // sky next-event estimation with power-heuristic MIS against BSDF sampling
-        if (sky_nee) {
+        // Sky next-event estimation with power-heuristic MIS against BSDF sampling, only
+        // where the path also samples the next segment: the reference receives sky light
+        // only through a traced segment that escapes (deferred_lighting_secondary.hlsl:24-29),
+        // so the last vertex and terminated paths get none.
+        const bool extend = b + 1 < D && !(max3(beta) < kThroughputMin);
+        if (sky_nee && extend) {
             F3 wi;
             float pl;
             if (sky_sample(s, u0, u1, &wi, &pl)) {
@@ -787,7 +793,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
             }
         }
         // extension ray (sample_secondary_ray.hlsl:17-68)
-        if (b + 1 >= D || max3(beta) < kThroughputMin) break;
+        if (!extend) break;
         F3 vl = to_local(fr, V);
         F3 hh = vndf_sample(vl, m.rx, m.ry, u2, u3);
         float hv = dot(vl, hh);
@@ -951,6 +957,17 @@ void ora_camera_ray32(const ora_camera* cam, uint32_t px, uint32_t py, float jx,
     camera_ray(*cam, px, py, jx, jy, &oo, &dd);
     o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
     d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+void ora_sky_batch32(const ora_ctx* ctx, uint32_t count, const float* dirs, float* rgb, float* pdf) {
+    for (uint32_t i = 0; i < count; ++i) {
+        if (rgb) ora_sky_eval32(ctx, dirs + 3 * (size_t)i, rgb + 3 * (size_t)i);
+        if (pdf) pdf[i] = ora_sky_pdf32(ctx, dirs + 3 * (size_t)i);
+    }
 }
 
 }  // extern "C"

@@ -212,7 +212,9 @@ Mat64 load_mat64(const float* m) {
     double r2 = (double)m[3] * (double)m[3];
     r.rx = std::max(r2 / aniso, 0.001);
     r.ry = std::max(r2 * aniso, 0.001);
-    r.two_sided = m[12] != 0.0f;
+    uint32_t ts;
+    std::memcpy(&ts, &m[12], sizeof ts);
+    r.two_sided = ts != 0u;
     return r;
 }
 struct Frame64 { D3 t, b, n; };
@@ -433,7 +435,8 @@ D3 path64(const Scene& s, const Bvh64& b, const Sky64& k, const ora_camera& cam,
             L = L + c * vis;
         }
         double u0 = pcg_double(rng), u1 = pcg_double(rng), u2 = pcg_double(rng), u3 = pcg_double(rng);
-        if (sky_nee) {
+        const bool extend = bo + 1 < D && !(max3(beta) < 0.001);
+        if (sky_nee && extend) {
             D3 wi;
             double pl;
             if (sky_sample64(s, k, u0, u1, &wi, &pl)) {
@@ -445,7 +448,7 @@ D3 path64(const Scene& s, const Bvh64& b, const Sky64& k, const ora_camera& cam,
                 }
             }
         }
-        if (bo + 1 >= D || max3(beta) < 0.001) break;
+        if (!extend) break;
         D3 vl = loc(fr, V);
         D3 hh = vndf_sample64(vl, m.rx, m.ry, u2, u3);
         D3 ll = hh * (2 * dot(vl, hh)) - vl;
@@ -539,6 +542,19 @@ void ora_bsdf64(const float* material, const double* n, const double* v, const d
 double ora_vndf_pdf64(const double* v_local, const double* l_local, double alpha) {
     D3 v = d3(v_local[0], v_local[1], v_local[2]), l = d3(l_local[0], l_local[1], l_local[2]);
     return vndf_pdf_l64(normalize(v + l), v, alpha, alpha);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+void ora_bsdf64_batch(const float* material, const double* n, const double* v, uint32_t count, const double* l,
+                      double* out3) {
+    for (uint32_t i = 0; i < count; ++i) ora_bsdf64(material, n, v, l + 3 * (size_t)i, out3 + 3 * (size_t)i);
+}
+
+void ora_vndf_pdf64_batch(const double* v_local, uint32_t count, const double* l_local, double alpha, double* out) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = ora_vndf_pdf64(v_local, l_local + 3 * (size_t)i, alpha);
 }
 
 }  // extern "C"
