@@ -1,0 +1,193 @@
+"""Benchmark of the wavefront path-tracing pass on one MI355X.
+
+A step is one bi_pt_render call: the per-frame LBVH rebuild plus 16 spp of the
+1920x1080 atrium (262,144 triangles, depth 8), as one render job.  Prints one JSON
+line with Mrays/s (all traced rays: camera, extension, light-shadow, sky-shadow),
+p50/p99 job latency, the trace kernel's roofline figures and the CPU baseline
+(the CPU restatement timed on this host's cores on a bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("bisemutum-engine_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+BYTES_PER_TRACED_RAY = 40      # SoA ray read (o, d: 24 B) + hit write (t, leaf, u, v: 16 B)
+BYTES_PER_CAMERA_RAY = 16      # camera rays are generated in the kernel: hit write only
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--wave-paths", type=int, default=0)
+    ap.add_argument("--stack-lds", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this pass runs on one GPU only; multi-GPU scaling is not part of this build "
+              "(image tiles would shard with no exchange but the final gather).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import bi_pt
+    import scenes
+
+    cfg = scenes.CONFIGS[args.config]
+    t = time.perf_counter()
+    scene = cfg["scene"]()
+    cam = cfg["camera"]()
+    log(f"scene {scene.name}: {scene.num_tris} tris, generated in {time.perf_counter() - t:.1f}s")
+    pt = bi_pt.PathTracer(0)
+    pt.set_scene(scene)
+    stream = torch.cuda.current_stream().cuda_stream
+    spp, mb, seed = cfg["spp"], cfg["max_bounces"], cfg["seed"]
+
+    def step(k, flags=0):
+        pt.render(cam, spp=spp, max_bounces=mb, seed=seed + k, flags=flags, wave_paths=args.wave_paths,
+                  stack_lds=args.stack_lds, stream=stream)
+
+    for k in range(args.warmup):
+        step(k)
+        torch.cuda.synchronize()
+        log(f"warmup step {k} done")
+    before = pt.stats()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record()
+        step(1000 + k)
+        ends[k].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    after = pt.stats()
+    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    rays = sum(after[k] - before[k] for k in ("rays_camera", "rays_extension", "rays_light_shadow",
+                                                "rays_sky_shadow"))
+    secs = t1 - t0
+    mrays = rays / secs / 1e6
+    log(f"timed {args.steps} steps: {secs:.3f}s, {rays} rays, {mrays:.1f} Mrays/s")
+
+    # roofline leg: one more frame with per-kernel HIP events on the render stream
+    s0 = pt.stats()
+    step(5000, flags=bi_pt.FLAG_TIMING)
+    s1 = pt.stats()
+    n = pt.num_tris()
+    bvh_bytes = 64 * max(n - 1, 1) + 48 * n
+    cam_rays = s1["rays_camera"] - s0["rays_camera"]
+    ext_rays = s1["rays_extension"] - s0["rays_extension"]
+    launches = s1["launches_trace_closest"]
+    alg = cam_rays * BYTES_PER_CAMERA_RAY + ext_rays * BYTES_PER_TRACED_RAY + launches * bvh_bytes
+    ms_trace = s1["ms_trace_closest"]
+    avg_launch_ms = ms_trace / max(launches, 1)
+    achieved = (alg / launches) / (avg_launch_ms * 1e-3) / 1e9
+    timing = {k: round(s1[k], 3) for k in ("ms_build", "ms_trace_closest", "ms_shade", "ms_trace_shadow",
+                                           "ms_resolve", "ms_total")}
+    log(f"kernel classes (ms, one frame): {timing}")
+
+    result = {
+        "metric": "Mrays/s and p99 render-job latency at equal RMSE vs FP64 oracle",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, generated in-repo)",
+        "config": {
+            "workload": f"plain wavefront path tracing, {scene.name} ({n} tris, LBVH rebuilt per frame), "
+                        f"{cam.width}x{cam.height}, {spp} spp, depth {mb - 1}",
+            "width": cam.width, "height": cam.height, "spp": spp, "max_bounces": mb,
+            "wave_paths": args.wave_paths, "stack_lds": args.stack_lds,
+        },
+        "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3)},
+        "rays_per_step": int(rays // args.steps),
+        "kernel_ms_one_frame": timing,
+        "roofline": {
+            "kernel": "k_trace_closest (camera + extension rays)",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "alg_bytes_per_launch": int(alg / launches),
+            "avg_launch_ms": round(avg_launch_ms, 4),
+            "launches": int(launches),
+        },
+    }
+
+    if not args.no_rmse:
+        result["rmse_vs_fp64"] = rmse_leg(pt, scene, cam, cfg)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_leg(scene, cfg)
+    pt.close()
+    print(json.dumps(result), flush=True)
+
+
+def rmse_leg(pt, scene, cam, cfg):
+    """Relative RMSE of one 16-spp GPU frame against the FP64 oracle at 4096 spp on a seeded pixel subset."""
+    import numpy as np
+
+    import oracle
+    t = time.perf_counter()
+    ors = oracle.OracleScene(scene)
+    rng = np.random.default_rng(0x5EED)
+    pix = np.sort(rng.choice(cam.width * cam.height, 16, replace=False)).astype(np.uint32)
+    m64, _ = ors.render64(cam, cfg["max_bounces"], pix, 4096, threads=min(16, os.cpu_count() or 1))
+    pt.render(cam, spp=cfg["spp"], max_bounces=cfg["max_bounces"], seed=cfg["seed"])
+    img = pt.image()[:, :3]
+    err = img[pix] - m64
+    rel = float(np.sqrt((err ** 2).mean()) / max(np.sqrt((m64 ** 2).mean()), 1e-12))
+    log(f"rmse leg: {time.perf_counter() - t:.1f}s")
+    ors.close()
+    return {"relative_rmse": round(rel, 5), "pixels": int(len(pix)), "gpu_spp": cfg["spp"], "oracle_spp": 4096}
+
+
+def cpu_leg(scene, cfg):
+    """The FP32 CPU restatement on this host, a bounded sample of the same workload."""
+    import oracle
+    import scenes
+    threads = min(16, os.cpu_count() or 1)
+    w, h = cfg["width"], cfg["height"]
+    cam = cfg["camera"](w, h)
+    t = time.perf_counter()
+    ors = oracle.OracleScene(scene)
+    build_s = time.perf_counter() - t
+    spp = max(1, cfg["spp"] // 2)
+    t = time.perf_counter()
+    out = ors.render32(cam, cfg["max_bounces"], cfg["seed"], spp=spp, threads=threads)
+    secs = time.perf_counter() - t
+    rays = int(out["counts"].sum())
+    ors.close()
+    log(f"cpu leg: {rays} rays in {secs:.2f}s on {threads} threads")
+    return {"value": round(rays / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"same scene and camera, {w}x{h}, {spp} of the {cfg['spp']} spp, depth "
+                      f"{cfg['max_bounces'] - 1}; render only, LBVH build {build_s:.2f}s excluded",
+            "seconds": round(secs, 2)}
+
+
+if __name__ == "__main__":
+    main()

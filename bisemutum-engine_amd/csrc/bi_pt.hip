@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -81,6 +82,7 @@ struct bi_pt_ctx {
     // statistics
     bi_pt_stats stats{};
     uint64_t frames = 0;
+    uint64_t camera_rays = 0;
     hipEvent_t ev[2 * 8] = {};
     bool events_ready = false;
     // recorded queues [wave][bounce]
@@ -221,6 +223,7 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s) {
     a.refit_cnt = ctx->refit_cnt.p;
     a.leaf_geo = ctx->leaf_geo.p;
     a.leaf_shade = ctx->leaf_shade.p;
+    a.error = &ctx->counters.p[kCntError];
     launch_build_lbvh(a, s);
     HIP_TRY(hipGetLastError());
     ctx->bvh_built = true;
@@ -328,6 +331,9 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
         for (uint32_t k = 0; k < meshes[m].num_indices; ++k)
             if (meshes[m].indices[k] >= meshes[m].num_vertices)
                 return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
+        for (size_t k = 0; k < 3 * (size_t)meshes[m].num_vertices; ++k)
+            if (!std::isfinite(meshes[m].positions[k]) || !std::isfinite(meshes[m].normals[k]))
+                return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite vertex position or normal");
         vtx_off[m] = (uint32_t)nv;
         idx_off[m] = (uint32_t)ni;
         nv += meshes[m].num_vertices;
@@ -339,6 +345,9 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
         const bi_pt_instance& I = instances[i];
         if (I.mesh >= num_meshes || I.material >= num_materials)
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "instance mesh/material out of range");
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(I.transform[k]))
+                return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite instance transform");
         std::memcpy(ih[i].xform, I.transform, sizeof ih[i].xform);
         ih[i].first_tri = (uint32_t)tris;
         ih[i].idx_off = idx_off[I.mesh];
@@ -386,6 +395,8 @@ bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* xf, uint32_t n
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     if (num_instances != ctx->num_inst || (num_instances && !xf))
         return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "instance count differs from the scene");
+    for (size_t k = 0; k < 12 * (size_t)num_instances; ++k)
+        if (!std::isfinite(xf[k])) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite instance transform");
     for (uint32_t i = 0; i < num_instances; ++i) std::memcpy(ctx->inst_host[i].xform, xf + 12 * (size_t)i, 48);
     HIP_TRY(hipSetDevice(ctx->device));
     if (num_instances)
@@ -507,12 +518,12 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     uint32_t lds_stack = st->stack_lds ? std::min(st->stack_lds, 32u) : 16u;
     TRY(ensure_wave_buffers(ctx, cap, nslots, sh_mult));
 
+    // device counters: queue counts, ticket and error flag per call; the ray statistics
+    // (slots kCntRays..) accumulate over the context's lifetime
+    HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, kCntRays * sizeof(uint32_t), s));
     if (timing) HIP_TRY(hipEventRecord(ctx->ev[0], s));
     TRY(build_bvh(ctx, s));
     if (timing) HIP_TRY(hipEventRecord(ctx->ev[1], s));
-
-    // device counters: queue counts, ticket, error, ray statistics
-    HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, kCntSlots * sizeof(uint32_t), s));
     uint32_t* cnt = ctx->counters.p;
     uint64_t* rays = reinterpret_cast<uint64_t*>(cnt + kCntRays);
 
@@ -649,7 +660,9 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     launch_output(ctx->sums.p, npix, (float)ctx->acc_count, 0, out_rgba, s);
     HIP_TRY(hipGetLastError());
     ctx->frames++;
+    ctx->camera_rays += (uint64_t)npix * st->spp;
     ctx->stats.frames = ctx->frames;
+    ctx->stats.rays_camera = ctx->camera_rays;
     ctx->stats.samples_accumulated = ctx->acc_count;
     ctx->stats.launches_trace_closest = launches_closest;
     ctx->stats.launches_trace_shadow = launches_shadow;
@@ -695,7 +708,8 @@ bi_pt_status bi_pt_get_stats(const bi_pt_ctx* cctx, bi_pt_stats* out) {
         *out = ctx->stats;
         return fail(ctx, BI_PT_ERROR_DEVICE_CHECK,
                     std::string("device check failed: ") + ((hc[kCntError] & 1u) ? "traversal stack overflow " : "") +
-                        ((hc[kCntError] & 2u) ? "look-back spin limit" : ""));
+                        ((hc[kCntError] & 2u) ? "look-back spin limit " : "") +
+                        ((hc[kCntError] & 4u) ? "refit wait limit" : ""));
     }
     return BI_PT_OK;
 }

@@ -20,6 +20,8 @@
 namespace bipt {
 namespace {
 
+constexpr uint32_t kBoxPending = 0x7fc00000u;   // quiet NaN: never a valid box coordinate
+
 __device__ __forceinline__ uint32_t f2ord(float f) {
     uint32_t b = __float_as_uint(f);
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
@@ -255,15 +257,22 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     nodes[i].child[0] = left;
     nodes[i].child[1] = right;
     nodes[i].pad[0] = nodes[i].pad[1] = 0;
+    // child boxes start as NaN: the refit reads a sibling's box once it is no longer NaN
+    for (int a = 0; a < 12; ++a) nodes[i].box[a] = __uint_as_float(kBoxPending);
     parent[left < 0 ? (n - 1) + ~left : left] = i;
     parent[right < 0 ? (n - 1) + ~right : right] = i;
     if (i == 0) parent[0] = -1;
     refit_cnt[i] = 0;
 }
 
-__device__ __forceinline__ float ld_coherent(const float* p) {
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
+// Device-coherent (agent scope) relaxed accesses of box words: no cache write-back or
+// invalidation is needed because a reader waits for the word itself to change.
+__device__ __forceinline__ uint32_t ld_box(const float* p) {
+    return __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_box(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ order, const float* __restrict__ tri_v,
@@ -271,7 +280,7 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
                                                 const uint32_t* __restrict__ tri_mat, int n,
                                                 LeafGeo* __restrict__ geo, LeafShade* __restrict__ shade,
                                                 BvhNode* nodes, const int32_t* __restrict__ parent,
-                                                uint32_t* refit_cnt) {
+                                                uint32_t* refit_cnt, uint32_t* error) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     uint32_t t = order[k];
@@ -303,15 +312,25 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
     int p = parent[(n - 1) + k];
     int slot = nodes[p].child[0] == ~k ? 0 : 1;
     for (;;) {
-        for (int a = 0; a < 6; ++a) nodes[p].box[6 * slot + a] = box[a];
-        __threadfence();
-        uint32_t old = __hip_atomic_fetch_add(&refit_cnt[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        for (int a = 0; a < 6; ++a) st_box(&nodes[p].box[6 * slot + a], box[a]);
+        uint32_t old = __hip_atomic_fetch_add(&refit_cnt[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0) return;   // the sibling subtree is still being built; it continues upward
-        __threadfence();
+        // second arrival: the sibling has issued its box; wait until every word is visible
         const float* ob = nodes[p].box + 6 * (1 - slot);
+        float other[6];
+        for (int a = 0; a < 6; ++a) {
+            uint32_t bits = ld_box(ob + a);
+            uint32_t spins = 0;
+            while (bits == kBoxPending) {
+                if (++spins > (1u << 20)) { atomicOr(error, 4u); bits = 0u; break; }
+                __builtin_amdgcn_s_sleep(1);
+                bits = ld_box(ob + a);
+            }
+            other[a] = __uint_as_float(bits);
+        }
         for (int a = 0; a < 3; ++a) {
-            box[a] = sel_min(box[a], ld_coherent(ob + a)) + 0.0f;
-            box[3 + a] = sel_max(box[3 + a], ld_coherent(ob + 3 + a)) + 0.0f;
+            box[a] = sel_min(box[a], other[a]) + 0.0f;
+            box[3 + a] = sel_max(box[3 + a], other[3 + a]) + 0.0f;
         }
         if (p == 0) return;
         int q = parent[p];
@@ -343,7 +362,7 @@ void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
     // 4 passes: the sorted keys/values end in buffer 0
     hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, a.keys[cur], (int)n, a.nodes, a.parent, a.refit_cnt);
     hipLaunchKernelGGL(k_leaves, dim3(g256), dim3(256), 0, s, a.vals[cur], a.tri_v, a.tri_n, a.tri_mat, (int)n,
-                       a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt);
+                       a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.error);
 }
 
 uint32_t sort_hist_entries(uint32_t n) { return 256u * ((n + kSortTile - 1) / kSortTile); }

@@ -41,6 +41,7 @@ struct BuildArgs {
     uint32_t* refit_cnt;
     LeafGeo* leaf_geo;
     LeafShade* leaf_shade;
+    uint32_t* error;
 };
 void launch_build_lbvh(const BuildArgs& a, hipStream_t s);
 uint32_t sort_hist_entries(uint32_t n);

@@ -8,9 +8,10 @@
 //             (exact visibility replaces the shadow map); one sky NEE shadow ray;
 //             VNDF-sampled extension ray while max(throughput) >= 0.001
 // Instead of full-screen passes with dead lanes, live rays are compacted into
-// the next queue: a wavefront ballot/prefix scan inside each 256-lane tile, and a
+// the next queue: a wavefront prefix scan inside each 256-lane tile, and a
 // single-pass decoupled look-back (Merrill & Garland 2016) across tiles taken in
-// ticket order, so queue order equals input order (deterministic contents).
+// ticket order, walked 64 predecessors at a time by one wavefront, so queue
+// order equals input order (deterministic contents).
 #include "pt_kernels.hpp"
 
 namespace bipt {
@@ -20,17 +21,59 @@ constexpr uint32_t kTile = 256;
 constexpr uint64_t kStatusAgg = 1ull << 62;
 constexpr uint64_t kStatusPrefix = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 31) - 1ull;
-constexpr uint32_t kSpinLimit = 1u << 24;
+constexpr uint32_t kSpinLimit = 1u << 20;   // bounded wait: a stuck predecessor becomes an error, not a hang
+constexpr uint32_t kShadeGridMax = 2048;
 
 __device__ __forceinline__ uint64_t pack_desc(uint64_t status, uint32_t e, uint32_t s) {
     return status | ((uint64_t)e << 31) | (uint64_t)s;
 }
 
-struct ShadowOut {
-    V3 c;
-    V3 d;
-    uint32_t kind;
-};
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Exclusive prefix (extension, shadow) of all tiles before `tile`, computed by one
+// wavefront: lane l inspects descriptor tile-1-l of the current window.
+__device__ __forceinline__ void look_back(uint64_t* desc, uint32_t tile, uint32_t lane, uint32_t* error,
+                                          uint32_t* pe_out, uint32_t* ps_out) {
+    uint32_t pe = 0, ps = 0;
+    int base = (int)tile - 1;
+    uint32_t spins = 0;
+    while (base >= 0) {
+        const int j = base - (int)lane;
+        uint64_t d = 0;
+        uint64_t st = 2;   // lanes before tile 0 read as an inclusive prefix of zero
+        if (j >= 0) {
+            // relaxed: the descriptor word is the only datum exchanged between tiles, so no
+            // acquire (and no cache invalidation) is needed; device scope keeps it coherent
+            d = __hip_atomic_load(&desc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            st = d >> 62;
+        }
+        if (__ballot(st == 0) != 0ull) {
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(error, 2u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const uint64_t inclusive = __ballot(st == 2);
+        uint32_t e = (uint32_t)((d >> 31) & kValMask), s = (uint32_t)(d & kValMask);
+        if (inclusive != 0ull) {
+            const uint32_t stop = (uint32_t)__builtin_ctzll(inclusive);   // nearest inclusive predecessor
+            if (lane > stop) { e = 0; s = 0; }
+            pe += wave_sum(e);
+            ps += wave_sum(s);
+            break;
+        }
+        pe += wave_sum(e);
+        ps += wave_sum(s);
+        base -= 64;
+    }
+    *pe_out = pe;
+    *ps_out = ps;
+}
 
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     __shared__ uint32_t s_tile;
@@ -56,11 +99,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         const uint32_t i = tile * kTile + tid;
         const bool active = i < count;
 
-        uint32_t n_ext = 0, n_sh = 0;
+        uint32_t n_ext = 0;
+        uint32_t sh_mask = 0;   // bit l: shadow ray toward light l; bit kMaxDirLights: sky ray
         uint32_t path = 0;
         V3 P = v3(0.f, 0.f, 0.f), Lext = v3(0.f, 0.f, 0.f), beta_next = v3(0.f, 0.f, 0.f);
         float pdf_next = 0.0f;
-        ShadowOut sh[kMaxDirLights + 1];
+        V3 c_slot[kMaxDirLights + 1];
+        V3 wi_sky = v3(0.f, 0.f, 0.f);
+        for (int k = 0; k <= kMaxDirLights; ++k) c_slot[k] = v3(0.f, 0.f, 0.f);
         if (active) {
             path = primary ? i : a.qin.path[i];
             const uint32_t q = path % a.wave.npix, sl = path / a.wave.npix;
@@ -105,17 +151,20 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             } else {
                 const float t = a.hits.t[i], hu = a.hits.u[i], hv = a.hits.v[i];
                 P = o + d * t;
-                const LeafShade& ls = a.leaf_shade[leaf];
-                V3 n0 = v3(ls.n[0], ls.n[1], ls.n[2]), n1 = v3(ls.n[3], ls.n[4], ls.n[5]);
-                V3 n2 = v3(ls.n[6], ls.n[7], ls.n[8]);
+                const float4* lsp = reinterpret_cast<const float4*>(a.leaf_shade + leaf);
+                const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
+                V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
+                const uint32_t mat_id = __float_as_uint(s2.y);
                 V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
-                Mat m = load_mat(a.materials[ls.material]);
+                Mat m = load_mat(a.materials[mat_id]);
                 if (m.two_sided && dot(d, N) > 0.0f) N = -N;
                 V3 V = normalize(o - P);
                 Frame fr = make_frame(N);
                 Pcg rng = pcg_keyed(a.wave.seed, pixel, sample, a.bounce);
                 float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
-                for (uint32_t l = 0; l < nl; ++l) {
+#pragma unroll
+                for (int l = 0; l < kMaxDirLights; ++l) {
+                    if ((uint32_t)l >= nl) break;
                     const LightRec& L = a.lights[l];
                     V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
                     V3 ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
@@ -123,16 +172,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
                     V3 c = (le * f) * beta;
                     if (!any_pos(c)) continue;
                     if (L.sm_index < 0) {
-                        float* acc = a.ps.acc + (size_t)(l + 1u) * 3u * cap;
+                        float* acc = a.ps.acc + (size_t)(l + 1) * 3u * cap;
                         acc[path] = acc[path] + c.x;
                         acc[cap + path] = acc[cap + path] + c.y;
                         acc[2 * cap + path] = acc[2 * cap + path] + c.z;
                         continue;
                     }
-                    sh[n_sh].c = c;
-                    sh[n_sh].d = ldir;
-                    sh[n_sh].kind = l;
-                    ++n_sh;
+                    c_slot[l] = c;
+                    sh_mask |= 1u << l;
                 }
                 // sky NEE only where the next segment is sampled too (the reference sees the
                 // sky only through a traced segment that escapes)
@@ -148,10 +195,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
                             float w = power_heuristic(pl, pb);
                             V3 c = ((sky_eval(a.sky, wi) * f) * (w / pl)) * beta;
                             if (any_pos(c) && finite3(c)) {
-                                sh[n_sh].c = c;
-                                sh[n_sh].d = wi;
-                                sh[n_sh].kind = kKindSky;
-                                ++n_sh;
+                                c_slot[kMaxDirLights] = c;
+                                wi_sky = wi;
+                                sh_mask |= 1u << kMaxDirLights;
                             }
                         }
                     }
@@ -181,9 +227,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
                 }
             }
         }
+        const uint32_t n_sh = __popc(sh_mask);
 
         // ---- tile scan of (extension, shadow) emissions: wavefront prefix, then LDS
-        uint32_t packed = (n_ext << 16) | n_sh;
+        const uint32_t packed = (n_ext << 16) | n_sh;
         uint32_t inc = packed;
         for (int off = 1; off < 64; off <<= 1) {
             uint32_t y = __shfl_up(inc, off);
@@ -198,46 +245,31 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             agg += s;
         }
         const uint32_t excl_local = wbase + inc - packed;
+        const uint32_t agg_e = agg >> 16, agg_s = agg & 0xffffu;
 
-        // ---- decoupled look-back across tiles (thread 0)
-        if (tid == 0) {
-            const uint32_t agg_e = agg >> 16, agg_s = agg & 0xffffu;
+        // ---- decoupled look-back across tiles (wavefront 0)
+        if (wave == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&a.desc[tile], pack_desc(tile == 0 ? kStatusPrefix : kStatusAgg, agg_e, agg_s),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t pe = 0, ps = 0;
-            if (tile == 0) {
-                __hip_atomic_store(&a.desc[0], pack_desc(kStatusPrefix, agg_e, agg_s), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_store(&a.desc[tile], pack_desc(kStatusAgg, agg_e, agg_s), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                int j = (int)tile - 1;
-                uint32_t spins = 0;
-                while (j >= 0) {
-                    uint64_t dsc = __hip_atomic_load(&a.desc[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    uint64_t st = dsc >> 62;
-                    if (st == 0) {
-                        if (++spins > kSpinLimit) { atomicOr(a.error, 2u); break; }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    pe += (uint32_t)((dsc >> 31) & kValMask);
-                    ps += (uint32_t)(dsc & kValMask);
-                    if (st == 2) break;
-                    --j;
+            if (tile > 0) look_back(a.desc, tile, lane, a.error, &pe, &ps);
+            if (lane == 0) {
+                if (tile > 0)
+                    __hip_atomic_store(&a.desc[tile], pack_desc(kStatusPrefix, pe + agg_e, ps + agg_s),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_excl[0] = pe;
+                s_excl[1] = ps;
+                if (tile == ntiles - 1) {
+                    *a.count_out = pe + agg_e;
+                    *a.count_shadow = ps + agg_s;
+                    atomicAdd((unsigned long long*)&a.ray_counters[1], (unsigned long long)(pe + agg_e));
                 }
-                __hip_atomic_store(&a.desc[tile], pack_desc(kStatusPrefix, pe + agg_e, ps + agg_s), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-            s_excl[0] = pe;
-            s_excl[1] = ps;
-            if (tile == ntiles - 1) {
-                *a.count_out = pe + agg_e;
-                *a.count_shadow = ps + agg_s;
-                atomicAdd((unsigned long long*)&a.ray_counters[1], (unsigned long long)(pe + agg_e));
             }
         }
         __syncthreads();
         const uint32_t base_e = s_excl[0] + (excl_local >> 16);
-        const uint32_t base_s = s_excl[1] + (excl_local & 0xffffu);
+        uint32_t o_sh = s_excl[1] + (excl_local & 0xffffu);
         if (n_ext) {
             a.qout.ox[base_e] = P.x; a.qout.oy[base_e] = P.y; a.qout.oz[base_e] = P.z;
             a.qout.dx[base_e] = Lext.x; a.qout.dy[base_e] = Lext.y; a.qout.dz[base_e] = Lext.z;
@@ -245,21 +277,23 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             a.ps.bx[path] = beta_next.x; a.ps.by[path] = beta_next.y; a.ps.bz[path] = beta_next.z;
             a.ps.pdf[path] = pdf_next;
         }
-        uint32_t n_light = 0, n_sky = 0;
-        for (uint32_t k = 0; k < n_sh; ++k) {
-            const uint32_t o = base_s + k;
-            a.sq.ox[o] = P.x; a.sq.oy[o] = P.y; a.sq.oz[o] = P.z;
-            a.sq.dx[o] = sh[k].d.x; a.sq.dy[o] = sh[k].d.y; a.sq.dz[o] = sh[k].d.z;
-            a.sq.cx[o] = sh[k].c.x; a.sq.cy[o] = sh[k].c.y; a.sq.cz[o] = sh[k].c.z;
-            a.sq.pk[o] = (path << 3) | sh[k].kind;
-            if (sh[k].kind == kKindSky) ++n_sky; else ++n_light;
+        // shadow rays in kind order: lights 0..3, then the sky
+#pragma unroll
+        for (int k = 0; k <= kMaxDirLights; ++k) {
+            if (!(sh_mask & (1u << k))) continue;
+            const bool sky = k == kMaxDirLights;
+            const V3 dir = sky ? wi_sky : v3(a.lights[k < kMaxDirLights ? k : 0].direction[0],
+                                            a.lights[k < kMaxDirLights ? k : 0].direction[1],
+                                            a.lights[k < kMaxDirLights ? k : 0].direction[2]);
+            a.sq.ox[o_sh] = P.x; a.sq.oy[o_sh] = P.y; a.sq.oz[o_sh] = P.z;
+            a.sq.dx[o_sh] = dir.x; a.sq.dy[o_sh] = dir.y; a.sq.dz[o_sh] = dir.z;
+            a.sq.cx[o_sh] = c_slot[k].x; a.sq.cy[o_sh] = c_slot[k].y; a.sq.cz[o_sh] = c_slot[k].z;
+            a.sq.pk[o_sh] = (path << 3) | (sky ? kKindSky : (uint32_t)k);
+            ++o_sh;
         }
-        // per-tile ray statistics: one atomic per wavefront
-        uint32_t cl = n_light, cs = n_sky;
-        for (int off = 32; off >= 1; off >>= 1) {
-            cl += __shfl_xor(cl, off);
-            cs += __shfl_xor(cs, off);
-        }
+        // ray statistics: one atomic per wavefront and kind
+        const uint32_t cl = wave_sum(__popc(sh_mask & ((1u << kMaxDirLights) - 1u)));
+        const uint32_t cs = wave_sum((sh_mask >> kMaxDirLights) & 1u);
         if (lane == 0) {
             if (cl) atomicAdd((unsigned long long*)&a.ray_counters[2], (unsigned long long)cl);
             if (cs) atomicAdd((unsigned long long*)&a.ray_counters[3], (unsigned long long)cs);
@@ -310,7 +344,7 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
 
 void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s) {
     uint32_t tiles = (capacity + kTile - 1) / kTile;
-    uint32_t grid = tiles < 4096u ? tiles : 4096u;
+    uint32_t grid = tiles < kShadeGridMax ? tiles : kShadeGridMax;
     if (grid == 0) grid = 1;
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a);
 }

@@ -136,6 +136,7 @@ typedef struct bi_pt_settings {
 } bi_pt_settings;
 
 typedef struct bi_pt_stats {
+    /* traced rays, cumulative over the context's lifetime */
     uint64_t rays_camera;
     uint64_t rays_extension;
     uint64_t rays_light_shadow;
