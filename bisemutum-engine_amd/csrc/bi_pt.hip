@@ -57,6 +57,8 @@ struct bi_pt_ctx {
     DevBuf<float4> sky_tex;
     DevBuf<float> sky_w, sky_ccdf, sky_rows, sky_marg;
     SkyParams sky{};
+    std::vector<float> sky_host;
+    bool sky_set = false;
     // build buffers
     uint32_t build_cap = 0;
     DevBuf<float> tri_v, tri_n, cb_out;
@@ -412,9 +414,12 @@ bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uin
     if (num_lights > (uint32_t)kMaxDirLights)
         return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 4 directional lights");
     if (num_lights && !lights) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null lights");
-    ctx->lights.assign(num_lights, LightRec{});
-    if (num_lights) std::memcpy(ctx->lights.data(), lights, num_lights * sizeof(LightRec));
-    ctx->acc_count = 0;
+    std::vector<LightRec> next(num_lights);
+    if (num_lights) std::memcpy(next.data(), lights, num_lights * sizeof(LightRec));
+    const bool changed = next.size() != ctx->lights.size() ||
+                         (num_lights && std::memcmp(next.data(), ctx->lights.data(), num_lights * sizeof(LightRec)) != 0);
+    ctx->lights.swap(next);
+    if (changed) ctx->acc_count = 0;   // unchanged lights (set every frame) keep the running image
     return BI_PT_OK;
 }
 
@@ -424,6 +429,14 @@ bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32
     if ((w == 0) != (h == 0) || ((w || h) && !rgba) || !color || !rot)
         return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "bad sky arguments");
     HIP_TRY(hipSetDevice(ctx->device));
+    const size_t ntex = (size_t)w * h * 4;
+    if (ctx->sky_set && ctx->sky.width == w && ctx->sky.height == h &&
+        std::memcmp(ctx->sky.color, color, sizeof ctx->sky.color) == 0 &&
+        std::memcmp(ctx->sky.rot, rot, sizeof ctx->sky.rot) == 0 && ctx->sky_host.size() == ntex &&
+        (ntex == 0 || std::memcmp(ctx->sky_host.data(), rgba, ntex * sizeof(float)) == 0))
+        return BI_PT_OK;   // unchanged sky (set every frame): keep tables and the running image
+    ctx->sky_host.assign(rgba, rgba + ntex);
+    ctx->sky_set = true;
     SkyParams sp{};
     sp.width = w;
     sp.height = h;
