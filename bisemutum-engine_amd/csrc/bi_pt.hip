@@ -596,6 +596,7 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             ta.hits = hits;
             ta.desc_clear = ctx->desc.p;
             ta.ticket_clear = &cnt[kCntTicket];
+            ta.work = &cnt[kCntTraceWork];
             ta.error = &cnt[kCntError];
             TRY(mark(0));
             launch_trace_closest(ta, wp.num_paths, primary, lds_stack, s);
@@ -620,6 +621,8 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             sa.ps = ps;
             sa.desc = ctx->desc.p;
             sa.ticket = &cnt[kCntTicket];
+            sa.trace_work_reset = &cnt[kCntTraceWork];
+            sa.shadow_work_reset = &cnt[kCntShadowWork];
             sa.error = &cnt[kCntError];
             sa.ray_counters = rays;
             sa.bounce = b;
@@ -636,6 +639,7 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             sha.ps = ps;
             for (uint32_t l = 0; l < nl; ++l) sha.keep[l] = 1.0f - ctx->lights[l].shadow_strength;
             sha.num_lights = nl;
+            sha.work = &cnt[kCntShadowWork];
             sha.error = &cnt[kCntError];
             launch_trace_shadow(sha, wp.num_paths * sh_mult, lds_stack, s);
             ++launches_shadow;
@@ -822,6 +826,7 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             ta.wave.ray_length = tmax;
             ta.q = q;
             ta.count_in = cnt.p;
+            ta.work = cnt.p + 1;   // zero
             ta.hits = HitBuf{ht[0].p, ht[1].p, ht[2].p, out.p};
             ta.error = &ctx->counters.p[kCntError];
             launch_trace_closest(ta, n, false, 16, nullptr);

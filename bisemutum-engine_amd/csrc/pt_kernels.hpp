@@ -65,6 +65,7 @@ struct TraceArgs {
     HitBuf hits;
     uint64_t* desc_clear;       // look-back descriptors cleared for the following shade
     uint32_t* ticket_clear;     // tile ticket reset for the following shade
+    uint32_t* work;             // ray counter of the persistent lanes (zero at launch)
     uint32_t* error;
 };
 void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, hipStream_t s);
@@ -79,6 +80,7 @@ struct ShadowArgs {
     PathState ps;
     float keep[kMaxDirLights];  // 1 - shadow_strength per light
     uint32_t num_lights;
+    uint32_t* work;             // ray counter of the persistent lanes (zero at launch)
     uint32_t* error;
 };
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, hipStream_t s);
@@ -101,6 +103,8 @@ struct ShadeArgs {
     PathState ps;
     uint64_t* desc;
     uint32_t* ticket;
+    uint32_t* trace_work_reset;   // zeroed here for the next closest-hit launch
+    uint32_t* shadow_work_reset;  // zeroed here for this segment's shadow launch
     uint32_t* error;
     uint64_t* ray_counters;     // [camera, extension, light shadow, sky shadow]
     uint32_t bounce;

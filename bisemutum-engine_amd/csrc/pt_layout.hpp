@@ -125,6 +125,8 @@ enum CounterSlot : uint32_t {
     kCntShadow = 2,     // entries in the shadow queue
     kCntTicket = 3,     // dynamic tile ticket of the compaction kernel
     kCntError = 4,      // nonzero: a kernel hit an internal limit
+    kCntTraceWork = 5,  // next ray of the persistent closest-hit lanes
+    kCntShadowWork = 6, // next ray of the persistent shadow lanes
     kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
     kCntSlots = 16
 };

@@ -86,6 +86,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const uint32_t nl = a.num_lights;
     const uint32_t nslots = nl + 2u;
     const size_t cap = a.ps.cap;
+    if (blockIdx.x == 0 && tid == 0) {
+        // the closest-hit launch of this segment has finished and this segment's shadow
+        // launch has not started: reset their persistent-lane ray counters
+        *a.trace_work_reset = 0u;
+        *a.shadow_work_reset = 0u;
+    }
     if (count == 0) {
         if (blockIdx.x == 0 && tid == 0) { *a.count_out = 0u; *a.count_shadow = 0u; }
         return;

@@ -2,23 +2,26 @@
 //
 // Replaces TraceRay of rt_gbuffer.hlsl:25 (closest hit, TMin 0.001, TMax
 // ray_length) and the shadow-map lookups of lights.hlsl:69-101 (exact any-hit
-// shadow rays).  One lane per ray, rays read from SoA queues (coalesced),
-// nodes read whole (4 x 16 B).  Short stack: the top `L` entries live in LDS
-// ([slot][lane], bank-conflict free for equal depths), deeper entries spill to
-// a per-lane private array; the total depth bound (64) covers any Karras tree
-// over fewer than 2^34 triangles.
+// shadow rays).  Rays come from SoA queues (coalesced); nodes are read whole
+// (4 x 16 B).  Short stack: the top `L` entries live in LDS ([slot][lane], bank
+// conflict free for equal depths), deeper entries spill to a per-lane private
+// array; the total depth bound (64) covers any Karras tree over < 2^34 triangles.
 //
-// Closest hit uses the "while-while" loop of Aila & Laine 2009: a lane descends
-// internal nodes until it reaches a leaf, then tests leaves until it pops an
-// internal node.  Each lane's visiting order is a function of its own ray only
-// (nearer child first, ties to child 0; stack LIFO), the same order the CPU
-// restatement uses, so hits agree bit for bit.
+// Persistent lanes with dynamic ray fetch: a workgroup-resident grid of wavefronts
+// keeps every lane busy -- a lane whose ray has finished takes the next ray index
+// from a global counter (one atomic per wavefront and refill, lanes ranked by a
+// ballot), so secondary rays of very different lengths do not idle the SIMD.
+// Inside a ray, the "while-while" loop of Aila & Laine 2009 descends internal nodes
+// until a leaf, then tests leaves until an internal node is popped.  Each lane's
+// visiting order is a function of its own ray only (nearer child first, ties to
+// child 0; stack LIFO) -- the order of the CPU restatement -- so hits agree bit for bit.
 #include "pt_kernels.hpp"
 
 namespace bipt {
 namespace {
 
 constexpr int kDone = (int)0x80000000;
+constexpr uint32_t kNoRay = 0xffffffffu;
 
 __device__ __forceinline__ V3 safe_inv(V3 d) {
     auto one = [](float x) {
@@ -93,106 +96,175 @@ __device__ __forceinline__ bool tri_test(const LeafGeo* geo, uint32_t leaf, V3 o
     return true;
 }
 
+// The stack depth lives in a register of the caller (`sp`), apart from the spill
+// array: an array indexed at run time is placed in private memory, and a counter
+// stored next to it would follow it there.
 template <int L>
 struct ShortStack {
     int spill[kStackMax - L];
-    int sp = 0;
-    __device__ __forceinline__ bool push(int (*lds)[256], int x) {
-        if (sp >= kStackMax) return false;
-        if (sp < L) lds[sp][threadIdx.x] = x;
-        else spill[sp - L] = x;
-        ++sp;
+    __device__ __forceinline__ bool push(int (*lds)[256], int& sp, int x) {
+        const int s = sp;
+        if (s >= kStackMax) return false;
+        if (s < L) lds[s][threadIdx.x] = x;
+        else spill[s - L] = x;
+        sp = s + 1;
         return true;
     }
-    __device__ __forceinline__ int pop(int (*lds)[256]) {
-        --sp;
-        return sp < L ? lds[sp][threadIdx.x] : spill[sp - L];
+    __device__ __forceinline__ int pop(int (*lds)[256], int& sp) {
+        const int s = sp - 1;
+        sp = s;
+        return s < L ? lds[s][threadIdx.x] : spill[s - L];
     }
 };
 
-template <bool PRIMARY, int L>
-__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
-    __shared__ int lds[L > 0 ? L : 1][256];
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t count = PRIMARY ? a.wave.num_paths : *a.count_in;
-    if (i == 0 && a.ticket_clear) *a.ticket_clear = 0u;
-    if (a.desc_clear && i < (count + 255u) / 256u) a.desc_clear[i] = 0ull;
-    if (i >= count) return;
-    V3 o, d;
-    if (PRIMARY) {
-        const uint32_t q = i % a.wave.npix, sl = i / a.wave.npix;
-        const uint32_t pixel = pixel_of(q, a.wave.cam.width, a.wave.cam.height);
-        float jx = 0.5f, jy = 0.5f;
-        if (a.wave.flags & 1u) {
-            Pcg jr = pcg_keyed(a.wave.seed, pixel, a.wave.sample0 + sl, kJitterBounce);
-            jx = pcg_float(jr);
-            jy = pcg_float(jr);
-        }
-        camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &o, &d);
-    } else {
-        o = v3(a.q.ox[i], a.q.oy[i], a.q.oz[i]);
-        d = v3(a.q.dx[i], a.q.dy[i], a.q.dz[i]);
+// Refill idle lanes of the wavefront from the global ray counter, once at least half
+// the wavefront is idle (or no lane is busy): one atomic per refill.  Returns the new
+// ray index for a lane that asked and got one, kNoRay otherwise; sets *exhausted when
+// the queue is drained for this lane.
+constexpr int kRefillLanes = 32;
+__device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bool want, bool* exhausted,
+                                              bool any_busy) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return kNoRay;
+    if (any_busy && __popcll(m) < kRefillLanes) return kNoRay;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(work, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    if (!want) return kNoRay;
+    const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (idx >= count) {
+        *exhausted = true;
+        return kNoRay;
     }
-    const float tmin = kTMin;
-    float tbest = a.wave.ray_length;
-    uint32_t best = kLeafMiss;
-    float bun = 0.0f, bvn = 0.0f, bdet = 1.0f;
-    if (a.bvh.num_tris > 0) {
-        const V3 inv = safe_inv(d);
-        ShortStack<L> st;
-        int node = 0;
-        for (;;) {
-            while (node >= 0) {
-                NodeBoxes nb = load_node(a.bvh.nodes, node);
-                bool h0, h1;
-                float tn0, tn1;
-                box2(nb, o, inv, tmin, tbest, &h0, &tn0, &h1, &tn1);
-                if (h0 && h1) {
-                    bool first0 = tn0 <= tn1;
-                    int nearc = first0 ? nb.ch.x : nb.ch.y;
-                    int farc = first0 ? nb.ch.y : nb.ch.x;
-                    if (!st.push(lds, farc)) { atomicOr(a.error, 1u); node = kDone; break; }
-                    node = nearc;
-                } else if (h0) {
-                    node = nb.ch.x;
-                } else if (h1) {
-                    node = nb.ch.y;
-                } else {
-                    node = st.sp == 0 ? kDone : st.pop(lds);
-                    if (node == kDone) break;
-                }
-            }
-            if (node == kDone) break;
-            while (node < 0 && node != kDone) {
-                const uint32_t leaf = (uint32_t)~node;
-                float t, un, vn, det;
-                if (tri_test(a.bvh.leaf_geo, leaf, o, d, &t, &un, &vn, &det) && t >= tmin &&
-                    (t < tbest || (t == tbest && leaf < best))) {
-                    tbest = t;
-                    best = leaf;
-                    bun = un;
-                    bvn = vn;
-                    bdet = det;
-                }
-                node = st.sp == 0 ? kDone : st.pop(lds);
-            }
-            if (node == kDone) break;
-        }
-    }
-    a.hits.t[i] = tbest;
-    a.hits.leaf[i] = best;
-    a.hits.u[i] = best != kLeafMiss ? bun / bdet : 0.0f;
-    a.hits.v[i] = best != kLeafMiss ? bvn / bdet : 0.0f;
+    return idx;
 }
 
-// Any-hit: the answer does not depend on the visiting order (no pruning by a
-// running distance), so children are taken in stored order without sorting.
+__device__ __forceinline__ void clear_following_shade(const TraceArgs& a, uint32_t count) {
+    const uint32_t ntiles = (count + 255u) / 256u;
+    if (a.desc_clear)
+        for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < ntiles; t += gridDim.x * 256u) a.desc_clear[t] = 0ull;
+    if (a.ticket_clear && blockIdx.x == 0 && threadIdx.x == 0) *a.ticket_clear = 0u;
+}
+
+template <bool PRIMARY, int L>
+__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
+    __shared__ int lds[L][256];
+    const uint32_t count = PRIMARY ? a.wave.num_paths : *a.count_in;
+    clear_following_shade(a, count);
+    const float tmin = kTMin;
+    if (a.bvh.num_tris == 0) {
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+            a.hits.t[i] = a.wave.ray_length;
+            a.hits.leaf[i] = kLeafMiss;
+            a.hits.u[i] = 0.0f;
+            a.hits.v[i] = 0.0f;
+        }
+        return;
+    }
+    uint32_t ray = kNoRay;
+    bool exhausted = false;
+    V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
+    float tbest = 0.0f, bun = 0.0f, bvn = 0.0f, bdet = 1.0f;
+    uint32_t best = kLeafMiss;
+    int node = kDone;
+    ShortStack<L> st;
+    int sp = 0;
+    NodeBoxes cur{};
+    for (;;) {
+        const uint32_t got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted,
+                                       __ballot(ray != kNoRay) != 0ull);
+        if (got != kNoRay) {
+            ray = got;
+            if (PRIMARY) {
+                const uint32_t q = ray % a.wave.npix, sl = ray / a.wave.npix;
+                const uint32_t pixel = pixel_of(q, a.wave.cam.width, a.wave.cam.height);
+                float jx = 0.5f, jy = 0.5f;
+                if (a.wave.flags & 1u) {
+                    Pcg jr = pcg_keyed(a.wave.seed, pixel, a.wave.sample0 + sl, kJitterBounce);
+                    jx = pcg_float(jr);
+                    jy = pcg_float(jr);
+                }
+                camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &o, &d);
+            } else {
+                o = v3(a.q.ox[ray], a.q.oy[ray], a.q.oz[ray]);
+                d = v3(a.q.dx[ray], a.q.dy[ray], a.q.dz[ray]);
+            }
+            inv = safe_inv(d);
+            tbest = a.wave.ray_length;
+            best = kLeafMiss;
+            bun = bvn = 0.0f;
+            bdet = 1.0f;
+            node = 0;
+            sp = 0;
+            cur = load_node(a.bvh.nodes, 0);
+        }
+        if (__ballot(ray != kNoRay) == 0ull) {
+            if (__ballot(!exhausted) == 0ull) break;
+            continue;
+        }
+        if (ray != kNoRay) {
+            // one node-centric step: both boxes against [tmin, tbest], hit leaf children
+            // intersected at once (nearer first), then the nearer internal child entered.
+            // The next node is chosen from the box tests alone, so its load is issued
+            // before the leaf tests and overlaps them.
+            const NodeBoxes nb = cur;
+            bool h0, h1;
+            float tn0, tn1;
+            box2(nb, o, inv, tmin, tbest, &h0, &tn0, &h1, &tn1);
+            const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
+            const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
+            if (i0 && i1) {
+                if (!st.push(lds, sp, first == 0 ? nb.ch.y : nb.ch.x)) { atomicOr(a.error, 1u); node = kDone; }
+                else node = first == 0 ? nb.ch.x : nb.ch.y;
+            } else if (i0) {
+                node = nb.ch.x;
+            } else if (i1) {
+                node = nb.ch.y;
+            } else {
+                node = sp == 0 ? kDone : st.pop(lds, sp);
+            }
+            if (node != kDone) cur = load_node(a.bvh.nodes, node);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int c = k == 0 ? first : 1 - first;
+                const bool hit = c == 0 ? h0 : h1;
+                const int ch = c == 0 ? nb.ch.x : nb.ch.y;
+                if (hit && ch < 0) {
+                    const uint32_t leaf = (uint32_t)~ch;
+                    float t, un, vn, det;
+                    if (tri_test(a.bvh.leaf_geo, leaf, o, d, &t, &un, &vn, &det) && t >= tmin &&
+                        (t < tbest || (t == tbest && leaf < best))) {
+                        tbest = t;
+                        best = leaf;
+                        bun = un;
+                        bvn = vn;
+                        bdet = det;
+                    }
+                }
+            }
+            if (node == kDone) {
+                a.hits.t[ray] = tbest;
+                a.hits.leaf[ray] = best;
+                a.hits.u[ray] = best != kLeafMiss ? bun / bdet : 0.0f;
+                a.hits.v[ray] = best != kLeafMiss ? bvn / bdet : 0.0f;
+                ray = kNoRay;
+            }
+        }
+    }
+}
+
+// Any-hit traversal of one ray, resumable: returns once the ray is decided.
+// The answer does not depend on the visiting order (no pruning by a running
+// distance), so children are taken in stored order without sorting.
 template <int L>
 __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3 o, V3 d, float tmin, float tmax,
                                          uint32_t* error) {
     if (bvh.num_tris == 0) return false;
     const V3 inv = safe_inv(d);
     ShortStack<L> st;
+    int sp = 0;
     int node = 0;
     for (;;) {
         while (node >= 0) {
@@ -201,38 +273,31 @@ __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3
             float tn0, tn1;
             box2(nb, o, inv, tmin, tmax, &h0, &tn0, &h1, &tn1);
             if (h0 && h1) {
-                if (!st.push(lds, nb.ch.y)) { atomicOr(error, 1u); return false; }
+                if (!st.push(lds, sp, nb.ch.y)) { atomicOr(error, 1u); return false; }
                 node = nb.ch.x;
             } else if (h0) {
                 node = nb.ch.x;
             } else if (h1) {
                 node = nb.ch.y;
             } else {
-                if (st.sp == 0) return false;
-                node = st.pop(lds);
+                if (sp == 0) return false;
+                node = st.pop(lds, sp);
             }
         }
         while (node < 0) {
             float t, un, vn, det;
             if (tri_test(bvh.leaf_geo, (uint32_t)~node, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
                 return true;
-            if (st.sp == 0) return false;
-            node = st.pop(lds);
+            if (sp == 0) return false;
+            node = st.pop(lds, sp);
         }
     }
 }
 
-template <int L>
-__global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
-    __shared__ int lds[L > 0 ? L : 1][256];
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= *a.count_in) return;
-    V3 o = v3(a.q.ox[i], a.q.oy[i], a.q.oz[i]);
-    V3 d = v3(a.q.dx[i], a.q.dy[i], a.q.dz[i]);
+__device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bool occ) {
     const uint32_t pk = a.q.pk[i];
     const uint32_t path = pk >> 3, kind = pk & 7u;
     V3 c = v3(a.q.cx[i], a.q.cy[i], a.q.cz[i]);
-    bool occ = occluded<L>(a.bvh, lds, o, d, kTMin, kShadowTMax, a.error);
     if (occ) {
         if (kind == kKindSky) return;
         float keep = a.keep[kind];
@@ -248,6 +313,79 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
 }
 
 template <int L>
+__global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
+    __shared__ int lds[L][256];
+    const uint32_t count = *a.count_in;
+    const float tmin = kTMin, tmax = kShadowTMax;
+    if (a.bvh.num_tris == 0) {
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) apply_shadow(a, i, false);
+        return;
+    }
+    uint32_t ray = kNoRay;
+    bool exhausted = false;
+    V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
+    int node = kDone;
+    bool occ = false;
+    ShortStack<L> st;
+    int sp = 0;
+    NodeBoxes cur{};
+    for (;;) {
+        const uint32_t got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted,
+                                       __ballot(ray != kNoRay) != 0ull);
+        if (got != kNoRay) {
+            ray = got;
+            o = v3(a.q.ox[ray], a.q.oy[ray], a.q.oz[ray]);
+            d = v3(a.q.dx[ray], a.q.dy[ray], a.q.dz[ray]);
+            inv = safe_inv(d);
+            node = 0;
+            sp = 0;
+            occ = false;
+            cur = load_node(a.bvh.nodes, 0);
+        }
+        if (__ballot(ray != kNoRay) == 0ull) {
+            if (__ballot(!exhausted) == 0ull) break;
+            continue;
+        }
+        if (ray != kNoRay) {
+            // node-centric any-hit step: the next internal node is chosen from the box
+            // tests (child 0 first; the answer is order-independent) and its load issued,
+            // then the hit leaf children are intersected
+            const NodeBoxes nb = cur;
+            bool h0, h1;
+            float tn0, tn1;
+            box2(nb, o, inv, tmin, tmax, &h0, &tn0, &h1, &tn1);
+            const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
+            if (i0 && i1) {
+                if (!st.push(lds, sp, nb.ch.y)) { atomicOr(a.error, 1u); node = kDone; }
+                else node = nb.ch.x;
+            } else if (i0) {
+                node = nb.ch.x;
+            } else if (i1) {
+                node = nb.ch.y;
+            } else {
+                node = sp == 0 ? kDone : st.pop(lds, sp);
+            }
+            if (node != kDone) cur = load_node(a.bvh.nodes, node);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const bool hit = c == 0 ? h0 : h1;
+                const int ch = c == 0 ? nb.ch.x : nb.ch.y;
+                if (hit && ch < 0 && !occ) {
+                    float t, un, vn, det;
+                    if (tri_test(a.bvh.leaf_geo, (uint32_t)~ch, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
+                        occ = true;
+                }
+            }
+            if (occ) node = kDone;
+            if (node == kDone) {
+                apply_shadow(a, ray, occ);
+                ray = kNoRay;
+            }
+        }
+    }
+}
+
+template <int L>
 __global__ __launch_bounds__(256) void k_debug_anyhit(BvhView bvh, RayQueue q, uint32_t n, float tmax,
                                                       uint32_t* out, uint32_t* error) {
     __shared__ int lds[L][256];
@@ -258,21 +396,54 @@ __global__ __launch_bounds__(256) void k_debug_anyhit(BvhView bvh, RayQueue q, u
     out[i] = occluded<L>(bvh, lds, o, d, kTMin, tmax, error) ? 1u : 0u;
 }
 
+// Workgroups that stay resident on the whole chip for a kernel (persistent grid).
+template <class K>
+uint32_t resident_groups(K kernel) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+    return (uint32_t)(cus * per_cu);
+}
+
+template <bool P, int L>
+void closest_l(const TraceArgs& a, uint32_t capacity, hipStream_t s) {
+    uint32_t groups = (capacity + 255u) / 256u;
+    static uint32_t resident = 0;
+    if (resident == 0) resident = resident_groups(k_trace_closest<P, L>);
+    if (groups > resident) groups = resident;
+    if (groups == 0) groups = 1;
+    hipLaunchKernelGGL((k_trace_closest<P, L>), dim3(groups), dim3(256), 0, s, a);
+}
+
+template <int L>
+void shadow_l(const ShadowArgs& a, uint32_t capacity, hipStream_t s) {
+    uint32_t groups = (capacity + 255u) / 256u;
+    static uint32_t resident = 0;
+    if (resident == 0) resident = resident_groups(k_trace_shadow<L>);
+    if (groups > resident) groups = resident;
+    if (groups == 0) groups = 1;
+    hipLaunchKernelGGL(k_trace_shadow<L>, dim3(groups), dim3(256), 0, s, a);
+}
+
 template <bool P>
-void dispatch_closest(const TraceArgs& a, dim3 g, uint32_t lds, hipStream_t s) {
-    if (lds <= 8) hipLaunchKernelGGL((k_trace_closest<P, 8>), g, dim3(256), 0, s, a);
-    else if (lds <= 16) hipLaunchKernelGGL((k_trace_closest<P, 16>), g, dim3(256), 0, s, a);
-    else if (lds <= 24) hipLaunchKernelGGL((k_trace_closest<P, 24>), g, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_trace_closest<P, 32>), g, dim3(256), 0, s, a);
+void dispatch_closest(const TraceArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
+    if (lds <= 8) closest_l<P, 8>(a, capacity, s);
+    else if (lds <= 16) closest_l<P, 16>(a, capacity, s);
+    else if (lds <= 24) closest_l<P, 24>(a, capacity, s);
+    else closest_l<P, 32>(a, capacity, s);
 }
 
 }  // namespace
 
 void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, hipStream_t s) {
-    dim3 g((capacity + 255u) / 256u);
-    if (g.x == 0) return;
-    if (primary) dispatch_closest<true>(a, g, lds_stack, s);
-    else dispatch_closest<false>(a, g, lds_stack, s);
+    if (capacity == 0) return;
+    if (primary) dispatch_closest<true>(a, capacity, lds_stack, s);
+    else dispatch_closest<false>(a, capacity, lds_stack, s);
 }
 
 void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, float tmax, uint32_t* out,
@@ -282,12 +453,11 @@ void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, floa
 }
 
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
-    dim3 g((capacity + 255u) / 256u);
-    if (g.x == 0) return;
-    if (lds <= 8) hipLaunchKernelGGL(k_trace_shadow<8>, g, dim3(256), 0, s, a);
-    else if (lds <= 16) hipLaunchKernelGGL(k_trace_shadow<16>, g, dim3(256), 0, s, a);
-    else if (lds <= 24) hipLaunchKernelGGL(k_trace_shadow<24>, g, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_trace_shadow<32>, g, dim3(256), 0, s, a);
+    if (capacity == 0) return;
+    if (lds <= 8) shadow_l<8>(a, capacity, s);
+    else if (lds <= 16) shadow_l<16>(a, capacity, s);
+    else if (lds <= 24) shadow_l<24>(a, capacity, s);
+    else shadow_l<32>(a, capacity, s);
 }
 
 }  // namespace bipt

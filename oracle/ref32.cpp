@@ -389,6 +389,10 @@ inline bool tri_test(const Scene& s, uint32_t leaf, F3 o, F3 d, float* t_out, fl
     return true;
 }
 
+// Node-centric traversal: every step processes one internal node -- both child boxes
+// are tested against [tmin, tbest], hit leaf children are intersected at once (nearer
+// child first), then the nearer hit internal child is entered and the farther one
+// pushed.  The stack holds internal nodes only.  The HIP kernel follows the same order.
 Hit trace_closest(const Scene& s, const Ray& r) {
     Hit h{r.tmax, kMiss, 0.0f, 0.0f};
     if (s.num_tris == 0) return h;
@@ -398,30 +402,31 @@ Hit trace_closest(const Scene& s, const Ray& r) {
     int sp = 0;
     int node = 0;
     for (;;) {
-        if (node < 0) {
-            uint32_t leaf = (uint32_t)~node;
+        const Node& nd = s.nodes[node];
+        float tn0, tn1;
+        const bool h0 = box_hit(nd.box[0], r.o, inv, r.tmin, h.t, &tn0);
+        const bool h1 = box_hit(nd.box[1], r.o, inv, r.tmin, h.t, &tn1);
+        const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
+        for (int k = 0; k < 2; ++k) {
+            const int c = k == 0 ? first : 1 - first;
+            const bool hit = c == 0 ? h0 : h1;
+            const int ch = nd.child[c];
+            if (!hit || ch >= 0) continue;
+            const uint32_t leaf = (uint32_t)~ch;
             float t, un, vn, det;
             if (tri_test(s, leaf, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin &&
                 (t < h.t || (t == h.t && leaf < h.leaf))) {
                 h.t = t; h.leaf = leaf; best_un = un; best_vn = vn; best_det = det;
             }
-            if (sp == 0) break;
-            node = stack[--sp];
-            continue;
         }
-        const Node& nd = s.nodes[node];
-        float tn0, tn1;
-        bool h0 = box_hit(nd.box[0], r.o, inv, r.tmin, h.t, &tn0);
-        bool h1 = box_hit(nd.box[1], r.o, inv, r.tmin, h.t, &tn1);
-        if (h0 && h1) {
-            int nearc = tn0 <= tn1 ? nd.child[0] : nd.child[1];
-            int farc = tn0 <= tn1 ? nd.child[1] : nd.child[0];
+        const bool i0 = h0 && nd.child[0] >= 0, i1 = h1 && nd.child[1] >= 0;
+        if (i0 && i1) {
             if (sp >= kStackMax) break;  // cannot happen for n < 2^34 (depth bound)
-            stack[sp++] = farc;
-            node = nearc;
-        } else if (h0) {
+            stack[sp++] = first == 0 ? nd.child[1] : nd.child[0];
+            node = first == 0 ? nd.child[0] : nd.child[1];
+        } else if (i0) {
             node = nd.child[0];
-        } else if (h1) {
+        } else if (i1) {
             node = nd.child[1];
         } else {
             if (sp == 0) break;
