@@ -72,7 +72,10 @@ struct bi_pt_ctx {
     uint32_t wave_cap = 0, wave_slots = 0, wave_sh_mult = 0;
     DevBuf<float> qa[6], qb[6], hit_f[3], ps_f[4], acc, sq_f[9];
     DevBuf<uint32_t> qa_path, qb_path, hit_leaf, sq_pk;
-    DevBuf<uint64_t> desc;
+    DevBuf<float> st_f[9], st_c;      // shade staging: hit point, extension dir, sky dir; contributions
+    DevBuf<uint32_t> st_path, st_mask;
+    DevBuf<uint64_t> tile_sums;
+    DevBuf<uint32_t> tile_sky;
     DevBuf<uint32_t> counters;   // kCntSlots uint32 (ray counters at kCntRays as 4 x u64)
     // accumulation
     DevBuf<float> sums;
@@ -250,7 +253,12 @@ bi_pt_status ensure_wave_buffers(bi_pt_ctx* ctx, uint32_t cap, uint32_t slots, u
     const size_t shcap = (size_t)cap * sh_mult;
     for (int k = 0; k < 9; ++k) TRY(dalloc(ctx, ctx->sq_f[k], shcap));
     TRY(dalloc(ctx, ctx->sq_pk, shcap));
-    TRY(dalloc(ctx, ctx->desc, (cap + 255u) / 256u + 1u));
+    for (int k = 0; k < 9; ++k) TRY(dalloc(ctx, ctx->st_f[k], cap));
+    TRY(dalloc(ctx, ctx->st_c, (size_t)3 * (slots - 1u) * cap));
+    TRY(dalloc(ctx, ctx->st_path, cap));
+    TRY(dalloc(ctx, ctx->st_mask, cap));
+    TRY(dalloc(ctx, ctx->tile_sums, (cap + 255u) / 256u + 1u));
+    TRY(dalloc(ctx, ctx->tile_sky, (cap + 255u) / 256u + 1u));
     ctx->wave_cap = cap;
     ctx->wave_slots = slots;
     ctx->wave_sh_mult = sh_mult;
@@ -306,7 +314,9 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     for (int k = 0; k < 4; ++k) dfree(ctx, ctx->ps_f[k]);
     for (int k = 0; k < 9; ++k) dfree(ctx, ctx->sq_f[k]);
     dfree(ctx, ctx->qa_path); dfree(ctx, ctx->qb_path); dfree(ctx, ctx->hit_leaf); dfree(ctx, ctx->acc);
-    dfree(ctx, ctx->sq_pk); dfree(ctx, ctx->desc); dfree(ctx, ctx->counters); dfree(ctx, ctx->sums);
+    dfree(ctx, ctx->sq_pk);
+    for (auto& b : ctx->st_f) dfree(ctx, b);
+    dfree(ctx, ctx->st_c); dfree(ctx, ctx->st_path); dfree(ctx, ctx->st_mask); dfree(ctx, ctx->tile_sums); dfree(ctx, ctx->tile_sky); dfree(ctx, ctx->counters); dfree(ctx, ctx->sums);
     if (ctx->events_ready)
         for (auto& ev : ctx->ev) (void)hipEventDestroy(ev);
     delete ctx;
@@ -531,9 +541,10 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     uint32_t lds_stack = st->stack_lds ? std::min(st->stack_lds, 32u) : 16u;
     TRY(ensure_wave_buffers(ctx, cap, nslots, sh_mult));
 
-    // device counters: queue counts, ticket and error flag per call; the ray statistics
+    // device counters: queue counts, persistent-lane ray counters and error flag per call; the ray statistics
     // (slots kCntRays..) accumulate over the context's lifetime
     HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, kCntRays * sizeof(uint32_t), s));
+    HIP_TRY(hipMemsetAsync(ctx->counters.p + kCntTraceWork, 0, (kCntSlots - kCntTraceWork) * sizeof(uint32_t), s));
     if (timing) HIP_TRY(hipEventRecord(ctx->ev[0], s));
     TRY(build_bvh(ctx, s));
     if (timing) HIP_TRY(hipEventRecord(ctx->ev[1], s));
@@ -546,6 +557,12 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     RayQueue qA = queue_of(ctx->qa, ctx->qa_path), qB = queue_of(ctx->qb, ctx->qb_path);
     ShadowQueue sq{ctx->sq_f[0].p, ctx->sq_f[1].p, ctx->sq_f[2].p, ctx->sq_f[3].p, ctx->sq_f[4].p,
                    ctx->sq_f[5].p, ctx->sq_f[6].p, ctx->sq_f[7].p, ctx->sq_f[8].p, ctx->sq_pk.p};
+    const size_t wc = ctx->wave_cap;
+    ShadeStage stage{ctx->st_f[0].p, ctx->st_f[1].p, ctx->st_f[2].p, ctx->st_path.p,
+                     ctx->st_f[3].p, ctx->st_f[4].p, ctx->st_f[5].p,
+                     ctx->st_f[6].p, ctx->st_f[7].p, ctx->st_f[8].p,
+                     ctx->st_c.p, ctx->st_c.p + (size_t)(nl + 1u) * wc, ctx->st_c.p + (size_t)2 * (nl + 1u) * wc,
+                     ctx->st_mask.p, ctx->tile_sums.p, ctx->tile_sky.p};
     if (record) { ctx->rec_ext.clear(); ctx->rec_sh.clear(); }
 
     // the launches of one kind are timed together: events around each class of kernel
@@ -594,8 +611,6 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             ta.q = qin;
             ta.count_in = primary ? nullptr : cnt_in;
             ta.hits = hits;
-            ta.desc_clear = ctx->desc.p;
-            ta.ticket_clear = &cnt[kCntTicket];
             ta.work = &cnt[kCntTraceWork];
             ta.error = &cnt[kCntError];
             TRY(mark(0));
@@ -614,22 +629,37 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             sa.qin = qin;
             sa.count_in = primary ? nullptr : cnt_in;
             sa.hits = hits;
-            sa.qout = qout;
-            sa.count_out = cnt_out;
-            sa.sq = sq;
-            sa.count_shadow = &cnt[kCntShadow];
+            sa.st = stage;
             sa.ps = ps;
-            sa.desc = ctx->desc.p;
-            sa.ticket = &cnt[kCntTicket];
             sa.trace_work_reset = &cnt[kCntTraceWork];
             sa.shadow_work_reset = &cnt[kCntShadowWork];
-            sa.error = &cnt[kCntError];
-            sa.ray_counters = rays;
             sa.bounce = b;
             sa.last_bounce = D - 1;
             sa.depth_out = out_depth;
             sa.depth_sample = (uint32_t)ctx->acc_count;
             launch_shade(sa, wp.num_paths, s);
+
+            ScanArgs sc{};
+            sc.tile_sums = stage.tile_sums;
+            sc.tile_sky = stage.tile_sky;
+            sc.count_in = primary ? nullptr : cnt_in;
+            sc.num_entries = wp.num_paths;
+            sc.count_ext = cnt_out;
+            sc.count_shadow = &cnt[kCntShadow];
+            sc.ray_counters = rays;
+            launch_scan_tiles(sc, s);
+
+            CompactArgs ca{};
+            ca.st = stage;
+            ca.count_in = primary ? nullptr : cnt_in;
+            ca.num_entries = wp.num_paths;
+            ca.num_lights = nl;
+            ca.cap = ctx->wave_cap;
+            for (uint32_t l = 0; l < nl; ++l)
+                for (int c = 0; c < 3; ++c) ca.ldir[l][c] = ctx->lights[l].direction[c];
+            ca.qout = qout;
+            ca.sq = sq;
+            launch_compact(ca, wp.num_paths, s);
             TRY(mark(2));
 
             ShadowArgs sha{};
@@ -804,7 +834,7 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
     for (int k = 0; k < 6; ++k) TRY(dalloc(ctx, f[k], n));
     for (int k = 0; k < 3; ++k) TRY(dalloc(ctx, ht[k], n));
     TRY(dalloc(ctx, path, n));
-    TRY(dalloc(ctx, cnt, 4));
+    TRY(dalloc(ctx, cnt, 4 + kWorkParts * kWorkStride));
     TRY(dalloc(ctx, out, n));
     std::vector<float> col(n);
     const int src[6] = {0, 1, 2, 4, 5, 6};
@@ -813,8 +843,10 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
         for (uint32_t i = 0; i < n; ++i) col[i] = rays[8 * i + src[k]];
         if (hipMemcpy(f[k].p, col.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess) status = BI_PT_ERROR_HIP;
     }
-    uint32_t hn[4] = {n, 0, 0, 0};
-    if (status == BI_PT_OK && hipMemcpy(cnt.p, hn, 16, hipMemcpyHostToDevice) != hipSuccess) status = BI_PT_ERROR_HIP;
+    std::vector<uint32_t> hn(4 + kWorkParts * kWorkStride, 0u);
+    hn[0] = n;
+    if (status == BI_PT_OK && hipMemcpy(cnt.p, hn.data(), hn.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        status = BI_PT_ERROR_HIP;
     RayQueue q{f[0].p, f[1].p, f[2].p, f[3].p, f[4].p, f[5].p, path.p};
     if (status == BI_PT_OK) {
         if (any_hit) {
@@ -826,7 +858,7 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             ta.wave.ray_length = tmax;
             ta.q = q;
             ta.count_in = cnt.p;
-            ta.work = cnt.p + 1;   // zero
+            ta.work = cnt.p + 4;   // zeroed part counters
             ta.hits = HitBuf{ht[0].p, ht[1].p, ht[2].p, out.p};
             ta.error = &ctx->counters.p[kCntError];
             launch_trace_closest(ta, n, false, 16, nullptr);

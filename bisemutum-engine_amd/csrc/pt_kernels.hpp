@@ -63,9 +63,7 @@ struct TraceArgs {
     RayQueue q;                 // input queue (unused for primary rays)
     const uint32_t* count_in;   // device count (null: primary, count = wave.num_paths)
     HitBuf hits;
-    uint64_t* desc_clear;       // look-back descriptors cleared for the following shade
-    uint32_t* ticket_clear;     // tile ticket reset for the following shade
-    uint32_t* work;             // ray counter of the persistent lanes (zero at launch)
+    uint32_t* work;             // kWorkParts part counters of the persistent lanes (zero at launch)
     uint32_t* error;
 };
 void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, hipStream_t s);
@@ -80,10 +78,22 @@ struct ShadowArgs {
     PathState ps;
     float keep[kMaxDirLights];  // 1 - shadow_strength per light
     uint32_t num_lights;
-    uint32_t* work;             // ray counter of the persistent lanes (zero at launch)
+    uint32_t* work;             // kWorkParts part counters of the persistent lanes (zero at launch)
     uint32_t* error;
 };
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, hipStream_t s);
+
+// Per-entry staging written by the shade kernel and read by the compaction kernel.
+struct ShadeStage {
+    float *px, *py, *pz;    // hit point (entries with any output)
+    uint32_t* path;
+    float *ex, *ey, *ez;    // extension direction
+    float *sx, *sy, *sz;    // sky shadow-ray direction
+    float *cx, *cy, *cz;    // shadow contribution of kind k (lights 0..n-1, sky n) at k * cap + entry
+    uint32_t* mask;         // bit 0 extension ray, bit 1 + l light l, bit 1 + kMaxDirLights sky
+    uint64_t* tile_sums;    // per 256-entry tile: extension << 32 | shadow (exclusive prefix after the scan)
+    uint32_t* tile_sky;     // per tile: sky shadow rays
+};
 
 struct ShadeArgs {
     WaveParams wave;
@@ -96,23 +106,39 @@ struct ShadeArgs {
     RayQueue qin;
     const uint32_t* count_in;   // null at bounce 0
     HitBuf hits;
-    RayQueue qout;
-    uint32_t* count_out;
-    ShadowQueue sq;
-    uint32_t* count_shadow;
+    ShadeStage st;
     PathState ps;
-    uint64_t* desc;
-    uint32_t* ticket;
     uint32_t* trace_work_reset;   // zeroed here for the next closest-hit launch
     uint32_t* shadow_work_reset;  // zeroed here for this segment's shadow launch
-    uint32_t* error;
-    uint64_t* ray_counters;     // [camera, extension, light shadow, sky shadow]
     uint32_t bounce;
     uint32_t last_bounce;       // segments - 1
     float* depth_out;           // nullable: reverse-Z depth of the primary hit
     uint32_t depth_sample;
 };
 void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s);
+
+struct ScanArgs {
+    uint64_t* tile_sums;
+    const uint32_t* tile_sky;
+    const uint32_t* count_in;   // null: num_entries
+    uint32_t num_entries;
+    uint32_t* count_ext;
+    uint32_t* count_shadow;
+    uint64_t* ray_counters;     // [camera, extension, light shadow, sky shadow]
+};
+void launch_scan_tiles(const ScanArgs& a, hipStream_t s);
+
+struct CompactArgs {
+    ShadeStage st;
+    const uint32_t* count_in;   // null: num_entries
+    uint32_t num_entries;
+    uint32_t num_lights;
+    size_t cap;
+    float ldir[kMaxDirLights][3];
+    RayQueue qout;
+    ShadowQueue sq;
+};
+void launch_compact(const CompactArgs& a, uint32_t capacity, hipStream_t s);
 
 struct ResolveArgs {
     PathState ps;

@@ -119,16 +119,17 @@ struct PathState {
 };
 
 // Device-side counters of one wave (uint32 slots).
+constexpr uint32_t kWorkParts = 8;     // ray-queue parts of the persistent lanes (one per XCD)
+constexpr uint32_t kWorkStride = 32;   // uint32 between part counters: one 128-B line each
 enum CounterSlot : uint32_t {
     kCntExtA = 0,       // entries in extension queue A
     kCntExtB = 1,       // entries in extension queue B
     kCntShadow = 2,     // entries in the shadow queue
-    kCntTicket = 3,     // dynamic tile ticket of the compaction kernel
     kCntError = 4,      // nonzero: a kernel hit an internal limit
-    kCntTraceWork = 5,  // next ray of the persistent closest-hit lanes
-    kCntShadowWork = 6, // next ray of the persistent shadow lanes
     kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
-    kCntSlots = 16
+    kCntTraceWork = 32,                                       // part counters of the closest-hit lanes
+    kCntShadowWork = kCntTraceWork + kWorkParts * kWorkStride, // part counters of the shadow lanes
+    kCntSlots = kCntShadowWork + kWorkParts * kWorkStride
 };
 
 }  // namespace bipt

@@ -117,42 +117,53 @@ struct ShortStack {
     }
 };
 
-// Refill idle lanes of the wavefront from the global ray counter, once at least half
-// the wavefront is idle (or no lane is busy): one atomic per refill.  Returns the new
-// ray index for a lane that asked and got one, kNoRay otherwise; sets *exhausted when
-// the queue is drained for this lane.
+// Refill idle lanes of the wavefront, once at least half the wavefront is idle (or no
+// lane is busy): one atomic per refill.  The queue is split into kWorkParts contiguous
+// parts, each with its own counter on its own 128-B line; a wavefront starts on the
+// part of its workgroup's XCD (workgroups go to the XCDs round-robin) and moves on to
+// the following parts when its part runs dry, so no single counter sees every
+// refill.  Returns the new ray index for a lane that asked and got one, kNoRay
+// otherwise; sets *exhausted when every part is drained.
 constexpr int kRefillLanes = 32;
+struct WorkCursor {
+    uint32_t part;   // part the wavefront draws from
+    uint32_t left;   // parts not yet found empty
+};
+__device__ __forceinline__ WorkCursor work_cursor() { return WorkCursor{blockIdx.x % kWorkParts, kWorkParts}; }
 __device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bool want, bool* exhausted,
-                                              bool any_busy) {
+                                              bool any_busy, WorkCursor& wc) {
     const uint64_t m = __ballot(want);
     if (m == 0ull) return kNoRay;
+    if (wc.left == 0) {
+        if (want) *exhausted = true;
+        return kNoRay;
+    }
     if (any_busy && __popcll(m) < kRefillLanes) return kNoRay;
     const uint32_t lane = __lane_id();
     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(work, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    if (!want) return kNoRay;
-    const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (idx >= count) {
-        *exhausted = true;
-        return kNoRay;
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    for (;;) {
+        const uint32_t lo = (uint32_t)(((uint64_t)count * wc.part) / kWorkParts);
+        const uint32_t hi = (uint32_t)(((uint64_t)count * (wc.part + 1u)) / kWorkParts);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(work + wc.part * kWorkStride, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        if (base < hi - lo) {
+            const uint32_t idx = base + rank;
+            return (want && idx < hi - lo) ? lo + idx : kNoRay;
+        }
+        wc.part = wc.part + 1u == kWorkParts ? 0u : wc.part + 1u;
+        if (--wc.left == 0) {
+            if (want) *exhausted = true;
+            return kNoRay;
+        }
     }
-    return idx;
-}
-
-__device__ __forceinline__ void clear_following_shade(const TraceArgs& a, uint32_t count) {
-    const uint32_t ntiles = (count + 255u) / 256u;
-    if (a.desc_clear)
-        for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < ntiles; t += gridDim.x * 256u) a.desc_clear[t] = 0ull;
-    if (a.ticket_clear && blockIdx.x == 0 && threadIdx.x == 0) *a.ticket_clear = 0u;
 }
 
 template <bool PRIMARY, int L>
 __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     __shared__ int lds[L][256];
     const uint32_t count = PRIMARY ? a.wave.num_paths : *a.count_in;
-    clear_following_shade(a, count);
     const float tmin = kTMin;
     if (a.bvh.num_tris == 0) {
         for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
@@ -165,6 +176,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     }
     uint32_t ray = kNoRay;
     bool exhausted = false;
+    WorkCursor wc = work_cursor();
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     float tbest = 0.0f, bun = 0.0f, bvn = 0.0f, bdet = 1.0f;
     uint32_t best = kLeafMiss;
@@ -174,7 +186,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     NodeBoxes cur{};
     for (;;) {
         const uint32_t got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted,
-                                       __ballot(ray != kNoRay) != 0ull);
+                                       __ballot(ray != kNoRay) != 0ull, wc);
         if (got != kNoRay) {
             ray = got;
             if (PRIMARY) {
@@ -323,6 +335,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     }
     uint32_t ray = kNoRay;
     bool exhausted = false;
+    WorkCursor wc = work_cursor();
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     int node = kDone;
     bool occ = false;
@@ -331,7 +344,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     NodeBoxes cur{};
     for (;;) {
         const uint32_t got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted,
-                                       __ballot(ray != kNoRay) != 0ull);
+                                       __ballot(ray != kNoRay) != 0ull, wc);
         if (got != kNoRay) {
             ray = got;
             o = v3(a.q.ox[ray], a.q.oy[ray], a.q.oz[ray]);
