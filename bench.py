@@ -34,6 +34,8 @@ def main():
     ap.add_argument("--config", default="c1", choices=["c0", "c1"])
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--stack-lds", type=int, default=0)
+    ap.add_argument("--refill-lanes", type=int, default=0)
+    ap.add_argument("--work-parts", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
     args = ap.parse_args()
@@ -60,7 +62,8 @@ def main():
 
     def step(k, flags=0):
         pt.render(cam, spp=spp, max_bounces=mb, seed=seed + k, flags=flags, wave_paths=args.wave_paths,
-                  stack_lds=args.stack_lds, stream=stream)
+                  stack_lds=args.stack_lds, refill_lanes=args.refill_lanes, work_parts=args.work_parts,
+                  stream=stream)
 
     for k in range(args.warmup):
         step(k)
@@ -120,6 +123,7 @@ def main():
                         f"{cam.width}x{cam.height}, {spp} spp, depth {mb - 1}",
             "width": cam.width, "height": cam.height, "spp": spp, "max_bounces": mb,
             "wave_paths": args.wave_paths, "stack_lds": args.stack_lds,
+            "refill_lanes": args.refill_lanes, "work_parts": args.work_parts,
         },
         "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                            "p99": round(float(np.percentile(lat, 99)), 3)},

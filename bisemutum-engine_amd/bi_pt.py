@@ -56,7 +56,7 @@ class CameraC(C.Structure):
 class Settings(C.Structure):
     _fields_ = [("ray_length", C.c_float), ("max_bounces", C.c_uint32), ("accumulate", C.c_uint32),
                 ("spp", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("wave_paths", C.c_uint32),
-                ("stack_lds", C.c_uint32)]
+                ("stack_lds", C.c_uint32), ("refill_lanes", C.c_uint32), ("work_parts", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -190,7 +190,7 @@ class PathTracer:
 
     # ---------------------------------------------------------------- render
     def render(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
-               wave_paths=0, stack_lds=0, depth=False, stream=None):
+               wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, depth=False, stream=None):
         torch = self.torch
         npix = cam.width * cam.height
         dev = torch.device("cuda", self.device)
@@ -201,7 +201,8 @@ class PathTracer:
             if self._depth is None or self._depth.numel() != npix:
                 self._depth = torch.empty(npix, dtype=torch.float32, device=dev)
             dptr = C.c_void_p(self._depth.data_ptr())
-        st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, wave_paths, stack_lds)
+        st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, wave_paths, stack_lds,
+                      refill_lanes, work_parts)
         s = C.c_void_p(stream) if stream else None
         self._check(lib().bi_pt_render(self.h, C.byref(camera_struct(cam)), C.byref(st),
                                        C.c_void_p(self._out.data_ptr()), dptr, s))

@@ -539,6 +539,8 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     uint32_t sh_mult = 1;
     for (const LightRec& L : ctx->lights) sh_mult += L.sm_index >= 0 ? 1u : 0u;
     uint32_t lds_stack = st->stack_lds ? std::min(st->stack_lds, 32u) : 16u;
+    const uint32_t refill = st->refill_lanes ? std::min(st->refill_lanes, 64u) : 32u;
+    const uint32_t parts = st->work_parts ? std::min(st->work_parts, kWorkPartsMax) : kWorkParts;
     TRY(ensure_wave_buffers(ctx, cap, nslots, sh_mult));
 
     // device counters: queue counts, persistent-lane ray counters and error flag per call; the ray statistics
@@ -612,6 +614,8 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             ta.count_in = primary ? nullptr : cnt_in;
             ta.hits = hits;
             ta.work = &cnt[kCntTraceWork];
+            ta.parts = parts;
+            ta.refill = refill;
             ta.error = &cnt[kCntError];
             TRY(mark(0));
             launch_trace_closest(ta, wp.num_paths, primary, lds_stack, s);
@@ -670,6 +674,8 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
             for (uint32_t l = 0; l < nl; ++l) sha.keep[l] = 1.0f - ctx->lights[l].shadow_strength;
             sha.num_lights = nl;
             sha.work = &cnt[kCntShadowWork];
+            sha.parts = parts;
+            sha.refill = refill;
             sha.error = &cnt[kCntError];
             launch_trace_shadow(sha, wp.num_paths * sh_mult, lds_stack, s);
             ++launches_shadow;
@@ -859,6 +865,8 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             ta.q = q;
             ta.count_in = cnt.p;
             ta.work = cnt.p + 4;   // zeroed part counters
+            ta.parts = kWorkParts;
+            ta.refill = 32;
             ta.hits = HitBuf{ht[0].p, ht[1].p, ht[2].p, out.p};
             ta.error = &ctx->counters.p[kCntError];
             launch_trace_closest(ta, n, false, 16, nullptr);

@@ -63,7 +63,9 @@ struct TraceArgs {
     RayQueue q;                 // input queue (unused for primary rays)
     const uint32_t* count_in;   // device count (null: primary, count = wave.num_paths)
     HitBuf hits;
-    uint32_t* work;             // kWorkParts part counters of the persistent lanes (zero at launch)
+    uint32_t* work;             // part counters of the persistent lanes (zero at launch)
+    uint32_t parts;             // queue parts (1..kWorkPartsMax)
+    uint32_t refill;            // idle lanes before a refill (1..64)
     uint32_t* error;
 };
 void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, hipStream_t s);
@@ -78,7 +80,9 @@ struct ShadowArgs {
     PathState ps;
     float keep[kMaxDirLights];  // 1 - shadow_strength per light
     uint32_t num_lights;
-    uint32_t* work;             // kWorkParts part counters of the persistent lanes (zero at launch)
+    uint32_t* work;             // part counters of the persistent lanes (zero at launch)
+    uint32_t parts;             // queue parts (1..kWorkPartsMax)
+    uint32_t refill;            // idle lanes before a refill (1..64)
     uint32_t* error;
 };
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, hipStream_t s);

@@ -119,7 +119,8 @@ struct PathState {
 };
 
 // Device-side counters of one wave (uint32 slots).
-constexpr uint32_t kWorkParts = 8;     // ray-queue parts of the persistent lanes (one per XCD)
+constexpr uint32_t kWorkParts = 8;     // default ray-queue parts of the persistent lanes (one per XCD)
+constexpr uint32_t kWorkPartsMax = 32;
 constexpr uint32_t kWorkStride = 32;   // uint32 between part counters: one 128-B line each
 enum CounterSlot : uint32_t {
     kCntExtA = 0,       // entries in extension queue A
@@ -128,8 +129,8 @@ enum CounterSlot : uint32_t {
     kCntError = 4,      // nonzero: a kernel hit an internal limit
     kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
     kCntTraceWork = 32,                                       // part counters of the closest-hit lanes
-    kCntShadowWork = kCntTraceWork + kWorkParts * kWorkStride, // part counters of the shadow lanes
-    kCntSlots = kCntShadowWork + kWorkParts * kWorkStride
+    kCntShadowWork = kCntTraceWork + kWorkPartsMax * kWorkStride, // part counters of the shadow lanes
+    kCntSlots = kCntShadowWork + kWorkPartsMax * kWorkStride
 };
 
 }  // namespace bipt

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const uint32_t nl = a.num_lights;
     const uint32_t nslots = nl + 2u;
     const size_t cap = a.ps.cap;
-    if (blockIdx.x == 0 && tid < kWorkParts) {
+    if (blockIdx.x == 0 && tid < kWorkPartsMax) {
         // the closest-hit launch of this segment has finished and this segment's shadow
         // launch has not started: reset their persistent-lane part counters
         a.trace_work_reset[tid * kWorkStride] = 0u;

@@ -35,6 +35,7 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
 struct NodeBoxes {
     float4 a, b, c;
     int2 ch;
+    int2 pad;
 };
 __device__ __forceinline__ NodeBoxes load_node(const BvhNode* nodes, int i) {
     const float4* p = reinterpret_cast<const float4*>(nodes + i);
@@ -42,10 +43,16 @@ __device__ __forceinline__ NodeBoxes load_node(const BvhNode* nodes, int i) {
     n.a = p[0];
     n.b = p[1];
     n.c = p[2];
-    int4 d = reinterpret_cast<const int4*>(p)[3];
+    const int4 d = reinterpret_cast<const int4*>(p)[3];
     n.ch = make_int2(d.x, d.y);
+    n.pad = make_int2(d.z, d.w);
     return n;
 }
+// The last 16 B of a node are loaded whole (the compiler widens a partial load to
+// it); keeping the two padding words formally live until the node is consumed stops
+// the register allocator from reusing them while the load is in flight, which would
+// force a wait for the load right after issuing it.
+__device__ __forceinline__ void consume_pad(const NodeBoxes& n) { asm volatile("" ::"v"(n.pad.x), "v"(n.pad.y)); }
 
 // Slab test of both children.  No operand can be NaN ((b - o) is finite and
 // inv is finite and nonzero), so fminf/fmaxf select exactly what the CPU's
@@ -68,14 +75,26 @@ __device__ __forceinline__ void box2(const NodeBoxes& n, V3 o, V3 inv, float tmi
     *h1 = tn <= tf * kRobust;
 }
 
-// Moller-Trumbore with division-free barycentric acceptance; t = tn / det.
-__device__ __forceinline__ bool tri_test(const LeafGeo* geo, uint32_t leaf, V3 o, V3 d, float* t_out, float* un_out,
-                                         float* vn_out, float* det_out) {
+// Leaf intersection record held in registers: v0, e1, e2.
+struct LeafRegs {
+    float4 g0, g1;
+    float g2;
+};
+__device__ __forceinline__ LeafRegs load_leaf(const LeafGeo* geo, uint32_t leaf) {
     const float4* p = reinterpret_cast<const float4*>(geo + leaf);
-    float4 g0 = p[0], g1 = p[1], g2 = p[2];
-    V3 v0 = v3(g0.x, g0.y, g0.z);
-    V3 e1 = v3(g0.w, g1.x, g1.y);
-    V3 e2 = v3(g1.z, g1.w, g2.x);
+    LeafRegs r;
+    r.g0 = p[0];
+    r.g1 = p[1];
+    r.g2 = reinterpret_cast<const float*>(p + 2)[0];
+    return r;
+}
+
+// Moller-Trumbore with division-free barycentric acceptance; t = tn / det.
+__device__ __forceinline__ bool tri_test(const LeafRegs& g, V3 o, V3 d, float* t_out, float* un_out, float* vn_out,
+                                         float* det_out) {
+    V3 v0 = v3(g.g0.x, g.g0.y, g.g0.z);
+    V3 e1 = v3(g.g0.w, g.g1.x, g.g1.y);
+    V3 e2 = v3(g.g1.z, g.g1.w, g.g2);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
     if (det == 0.0f) return false;
@@ -95,10 +114,18 @@ __device__ __forceinline__ bool tri_test(const LeafGeo* geo, uint32_t leaf, V3 o
     *det_out = det;
     return true;
 }
+__device__ __forceinline__ bool tri_test(const LeafGeo* geo, uint32_t leaf, V3 o, V3 d, float* t_out, float* un_out,
+                                         float* vn_out, float* det_out) {
+    return tri_test(load_leaf(geo, leaf), o, d, t_out, un_out, vn_out, det_out);
+}
 
 // The stack depth lives in a register of the caller (`sp`), apart from the spill
 // array: an array indexed at run time is placed in private memory, and a counter
-// stored next to it would follow it there.
+// stored next to it would follow it there.  The spill array is touched only inside
+// branches taken when some lane of the wavefront is that deep, and its loaded value is
+// consumed inside the branch: the common path is then pure LDS, with no generic
+// (flat) access and no wait on the vector-memory counter, which would also wait for
+// the node load issued ahead of it.
 template <int L>
 struct ShortStack {
     int spill[kStackMax - L];
@@ -106,30 +133,41 @@ struct ShortStack {
         const int s = sp;
         if (s >= kStackMax) return false;
         if (s < L) lds[s][threadIdx.x] = x;
-        else spill[s - L] = x;
+        if (__ballot(s >= L) != 0ull) {
+            if (s >= L) spill[s - L] = x;
+        }
         sp = s + 1;
         return true;
     }
     __device__ __forceinline__ int pop(int (*lds)[256], int& sp) {
         const int s = sp - 1;
         sp = s;
-        return s < L ? lds[s][threadIdx.x] : spill[s - L];
+        int v = lds[s < L ? s : L - 1][threadIdx.x];
+        if (__ballot(s >= L) != 0ull) {
+            const int w = spill[s >= L ? s - L : 0];
+            v = s >= L ? w : v;
+        }
+        return v;
     }
 };
 
-// Refill idle lanes of the wavefront, once at least half the wavefront is idle (or no
-// lane is busy): one atomic per refill.  The queue is split into kWorkParts contiguous
+// Refill idle lanes of the wavefront, once `refill` lanes are idle (or no lane is
+// busy): one atomic per refill.  The queue is split into `parts` contiguous
 // parts, each with its own counter on its own 128-B line; a wavefront starts on the
 // part of its workgroup's XCD (workgroups go to the XCDs round-robin) and moves on to
 // the following parts when its part runs dry, so no single counter sees every
 // refill.  Returns the new ray index for a lane that asked and got one, kNoRay
 // otherwise; sets *exhausted when every part is drained.
-constexpr int kRefillLanes = 32;
 struct WorkCursor {
-    uint32_t part;   // part the wavefront draws from
-    uint32_t left;   // parts not yet found empty
+    uint32_t part;    // part the wavefront draws from
+    uint32_t left;    // parts not yet found empty
+    uint32_t parts;
+    uint32_t chunk;   // rays per part (the last parts may hold fewer)
+    uint32_t refill;
 };
-__device__ __forceinline__ WorkCursor work_cursor() { return WorkCursor{blockIdx.x % kWorkParts, kWorkParts}; }
+__device__ __forceinline__ WorkCursor work_cursor(uint32_t count, uint32_t parts, uint32_t refill) {
+    return WorkCursor{blockIdx.x % parts, parts, parts, (count + parts - 1u) / parts, refill};
+}
 __device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bool want, bool* exhausted,
                                               bool any_busy, WorkCursor& wc) {
     const uint64_t m = __ballot(want);
@@ -138,13 +176,13 @@ __device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bo
         if (want) *exhausted = true;
         return kNoRay;
     }
-    if (any_busy && __popcll(m) < kRefillLanes) return kNoRay;
+    if (any_busy && (uint32_t)__popcll(m) < wc.refill) return kNoRay;
     const uint32_t lane = __lane_id();
     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
     const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     for (;;) {
-        const uint32_t lo = (uint32_t)(((uint64_t)count * wc.part) / kWorkParts);
-        const uint32_t hi = (uint32_t)(((uint64_t)count * (wc.part + 1u)) / kWorkParts);
+        const uint32_t lo = min(wc.part * wc.chunk, count);
+        const uint32_t hi = min(lo + wc.chunk, count);
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(work + wc.part * kWorkStride, (uint32_t)__popcll(m));
         base = __shfl(base, (int)leader);
@@ -152,7 +190,7 @@ __device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bo
             const uint32_t idx = base + rank;
             return (want && idx < hi - lo) ? lo + idx : kNoRay;
         }
-        wc.part = wc.part + 1u == kWorkParts ? 0u : wc.part + 1u;
+        wc.part = wc.part + 1u == wc.parts ? 0u : wc.part + 1u;
         if (--wc.left == 0) {
             if (want) *exhausted = true;
             return kNoRay;
@@ -176,7 +214,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     }
     uint32_t ray = kNoRay;
     bool exhausted = false;
-    WorkCursor wc = work_cursor();
+    WorkCursor wc = work_cursor(count, a.parts, a.refill);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     float tbest = 0.0f, bun = 0.0f, bvn = 0.0f, bdet = 1.0f;
     uint32_t best = kLeafMiss;
@@ -219,17 +257,26 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
         if (ray != kNoRay) {
             // one node-centric step: both boxes against [tmin, tbest], hit leaf children
             // intersected at once (nearer first), then the nearer internal child entered.
-            // The next node is chosen from the box tests alone, so its load is issued
-            // before the leaf tests and overlaps them.
+            // The leaf records are loaded first and the next node (chosen from the box
+            // tests alone) after them, so waiting for the leaves leaves the node load in
+            // flight across the leaf tests; the node load is issued on every path (node 0
+            // for a finished ray) to keep that wait count fixed.
             const NodeBoxes nb = cur;
+            consume_pad(nb);
             bool h0, h1;
             float tn0, tn1;
             box2(nb, o, inv, tmin, tbest, &h0, &tn0, &h1, &tn1);
             const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
+            const int cn = first == 0 ? nb.ch.x : nb.ch.y, cf = first == 0 ? nb.ch.y : nb.ch.x;
+            const bool hn = first == 0 ? h0 : h1, hf = first == 0 ? h1 : h0;
+            const bool ln = hn && cn < 0, lf = hf && cf < 0;
+            LeafRegs gn{}, gf{};
+            if (ln) gn = load_leaf(a.bvh.leaf_geo, (uint32_t)~cn);
+            if (lf) gf = load_leaf(a.bvh.leaf_geo, (uint32_t)~cf);
             const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
             if (i0 && i1) {
-                if (!st.push(lds, sp, first == 0 ? nb.ch.y : nb.ch.x)) { atomicOr(a.error, 1u); node = kDone; }
-                else node = first == 0 ? nb.ch.x : nb.ch.y;
+                if (!st.push(lds, sp, cf)) { atomicOr(a.error, 1u); node = kDone; }
+                else node = cn;
             } else if (i0) {
                 node = nb.ch.x;
             } else if (i1) {
@@ -237,23 +284,19 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
             } else {
                 node = sp == 0 ? kDone : st.pop(lds, sp);
             }
-            if (node != kDone) cur = load_node(a.bvh.nodes, node);
+            cur = load_node(a.bvh.nodes, node != kDone ? node : 0);
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                const int c = k == 0 ? first : 1 - first;
-                const bool hit = c == 0 ? h0 : h1;
-                const int ch = c == 0 ? nb.ch.x : nb.ch.y;
-                if (hit && ch < 0) {
-                    const uint32_t leaf = (uint32_t)~ch;
-                    float t, un, vn, det;
-                    if (tri_test(a.bvh.leaf_geo, leaf, o, d, &t, &un, &vn, &det) && t >= tmin &&
-                        (t < tbest || (t == tbest && leaf < best))) {
-                        tbest = t;
-                        best = leaf;
-                        bun = un;
-                        bvn = vn;
-                        bdet = det;
-                    }
+                if (!(k == 0 ? ln : lf)) continue;
+                const uint32_t leaf = (uint32_t)~(k == 0 ? cn : cf);
+                float t, un, vn, det;
+                if (tri_test(k == 0 ? gn : gf, o, d, &t, &un, &vn, &det) && t >= tmin &&
+                    (t < tbest || (t == tbest && leaf < best))) {
+                    tbest = t;
+                    best = leaf;
+                    bun = un;
+                    bvn = vn;
+                    bdet = det;
                 }
             }
             if (node == kDone) {
@@ -335,7 +378,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     }
     uint32_t ray = kNoRay;
     bool exhausted = false;
-    WorkCursor wc = work_cursor();
+    WorkCursor wc = work_cursor(count, a.parts, a.refill);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     int node = kDone;
     bool occ = false;
@@ -360,13 +403,19 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
             continue;
         }
         if (ray != kNoRay) {
-            // node-centric any-hit step: the next internal node is chosen from the box
-            // tests (child 0 first; the answer is order-independent) and its load issued,
-            // then the hit leaf children are intersected
+            // node-centric any-hit step: leaf records of the hit leaf children are loaded,
+            // then the next internal node (child 0 first; the answer is order-independent)
+            // is chosen from the box tests and its load issued on every path, then the
+            // leaves are intersected while the node load is in flight
             const NodeBoxes nb = cur;
+            consume_pad(nb);
             bool h0, h1;
             float tn0, tn1;
             box2(nb, o, inv, tmin, tmax, &h0, &tn0, &h1, &tn1);
+            const bool l0 = h0 && nb.ch.x < 0, l1 = h1 && nb.ch.y < 0;
+            LeafRegs g0{}, g1{};
+            if (l0) g0 = load_leaf(a.bvh.leaf_geo, (uint32_t)~nb.ch.x);
+            if (l1) g1 = load_leaf(a.bvh.leaf_geo, (uint32_t)~nb.ch.y);
             const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
             if (i0 && i1) {
                 if (!st.push(lds, sp, nb.ch.y)) { atomicOr(a.error, 1u); node = kDone; }
@@ -378,16 +427,12 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
             } else {
                 node = sp == 0 ? kDone : st.pop(lds, sp);
             }
-            if (node != kDone) cur = load_node(a.bvh.nodes, node);
+            cur = load_node(a.bvh.nodes, node != kDone ? node : 0);
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const bool hit = c == 0 ? h0 : h1;
-                const int ch = c == 0 ? nb.ch.x : nb.ch.y;
-                if (hit && ch < 0 && !occ) {
-                    float t, un, vn, det;
-                    if (tri_test(a.bvh.leaf_geo, (uint32_t)~ch, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
-                        occ = true;
-                }
+                if (!(c == 0 ? l0 : l1) || occ) continue;
+                float t, un, vn, det;
+                if (tri_test(c == 0 ? g0 : g1, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax) occ = true;
             }
             if (occ) node = kDone;
             if (node == kDone) {

@@ -133,6 +133,8 @@ typedef struct bi_pt_settings {
     uint32_t flags;
     uint32_t wave_paths;      /* paths in flight per wave; 0 = automatic */
     uint32_t stack_lds;       /* traversal stack entries kept in LDS (0 = automatic, <= 32) */
+    uint32_t refill_lanes;    /* idle lanes before a wavefront fetches new rays (0 = automatic, <= 64) */
+    uint32_t work_parts;      /* ray-queue parts with their own fetch counter (0 = automatic, <= 32) */
 } bi_pt_settings;
 
 typedef struct bi_pt_stats {
