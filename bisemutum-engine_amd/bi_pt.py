@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbi_pt.so")
+# BI_PT_LIBRARY names another build of the library (for example one made with
+# different compile-time options); by default the one beside this file is used.
+LIB_PATH = os.environ.get("BI_PT_LIBRARY") or os.path.join(HERE, "libbi_pt.so")
 
 FLAG_JITTER = 1 << 0
 FLAG_NO_SKY_NEE = 1 << 1

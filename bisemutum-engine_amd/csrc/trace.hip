@@ -212,23 +212,63 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
         }
         return;
     }
+    // Each iteration: (1) the lanes holding a ray test both child boxes of `cur` and
+    // choose the next node, issuing the loads of the hit leaf children; (2) lanes whose
+    // ray ends here and idle lanes ask for new rays (a new ray starts at the root) and
+    // issue the loads of their ray data; (3) the next node is loaded, at this single
+    // place, so the loop keeps it in the same registers; (4) the leaf tests of (1) and
+    // the ray set-up of (2) run while that load is in flight.
     uint32_t ray = kNoRay;
     bool exhausted = false;
     WorkCursor wc = work_cursor(count, a.parts, a.refill);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     float tbest = 0.0f, bun = 0.0f, bvn = 0.0f, bdet = 1.0f;
     uint32_t best = kLeafMiss;
-    int node = kDone;
     ShortStack<L> st;
     int sp = 0;
-    NodeBoxes cur{};
+    NodeBoxes cur = load_node(a.bvh.nodes, 0);
     for (;;) {
-        const uint32_t got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted,
-                                       __ballot(ray != kNoRay) != 0ull, wc);
-        if (got != kNoRay) {
-            ray = got;
+        const bool active = ray != kNoRay;
+        int next = kDone;
+        int cn, cf;              // set, with the leaf records, where ln / lf are set
+        bool ln = false, lf = false;
+        LeafRegs gn, gf;
+        if (active) {
+            // node-centric step: both boxes against [tmin, tbest]; hit leaf children are
+            // intersected below (nearer first), the nearer internal child is entered
+            const NodeBoxes nb = cur;
+            consume_pad(nb);
+            bool h0, h1;
+            float tn0, tn1;
+            box2(nb, o, inv, tmin, tbest, &h0, &tn0, &h1, &tn1);
+            const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
+            cn = first == 0 ? nb.ch.x : nb.ch.y;
+            cf = first == 0 ? nb.ch.y : nb.ch.x;
+            const bool hn = first == 0 ? h0 : h1, hf = first == 0 ? h1 : h0;
+            ln = hn && cn < 0;
+            lf = hf && cf < 0;
+            if (ln) gn = load_leaf(a.bvh.leaf_geo, (uint32_t)~cn);
+            if (lf) gf = load_leaf(a.bvh.leaf_geo, (uint32_t)~cf);
+            const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
+            if (i0 && i1) {
+                if (!st.push(lds, sp, cf)) atomicOr(a.error, 1u);
+                else next = cn;
+            } else if (i0) {
+                next = nb.ch.x;
+            } else if (i1) {
+                next = nb.ch.y;
+            } else if (sp > 0) {
+                next = st.pop(lds, sp);
+            }
+        }
+        const bool done = active && next == kDone;
+        const uint32_t got = fetch_ray(a.work, count, (!active || done) && !exhausted, &exhausted,
+                                       __ballot(active && !done) != 0ull, wc);
+        const bool fresh = got != kNoRay;
+        V3 no = o, nd = d;
+        if (fresh) {
             if (PRIMARY) {
-                const uint32_t q = ray % a.wave.npix, sl = ray / a.wave.npix;
+                const uint32_t q = got % a.wave.npix, sl = got / a.wave.npix;
                 const uint32_t pixel = pixel_of(q, a.wave.cam.width, a.wave.cam.height);
                 float jx = 0.5f, jy = 0.5f;
                 if (a.wave.flags & 1u) {
@@ -236,55 +276,17 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
                     jx = pcg_float(jr);
                     jy = pcg_float(jr);
                 }
-                camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &o, &d);
+                camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &no, &nd);
             } else {
-                o = v3(a.q.ox[ray], a.q.oy[ray], a.q.oz[ray]);
-                d = v3(a.q.dx[ray], a.q.dy[ray], a.q.dz[ray]);
+                no = v3(a.q.ox[got], a.q.oy[got], a.q.oz[got]);
+                nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
             }
-            inv = safe_inv(d);
-            tbest = a.wave.ray_length;
-            best = kLeafMiss;
-            bun = bvn = 0.0f;
-            bdet = 1.0f;
-            node = 0;
-            sp = 0;
-            cur = load_node(a.bvh.nodes, 0);
+            next = 0;
         }
-        if (__ballot(ray != kNoRay) == 0ull) {
-            if (__ballot(!exhausted) == 0ull) break;
-            continue;
-        }
-        if (ray != kNoRay) {
-            // one node-centric step: both boxes against [tmin, tbest], hit leaf children
-            // intersected at once (nearer first), then the nearer internal child entered.
-            // The leaf records are loaded first and the next node (chosen from the box
-            // tests alone) after them, so waiting for the leaves leaves the node load in
-            // flight across the leaf tests; the node load is issued on every path (node 0
-            // for a finished ray) to keep that wait count fixed.
-            const NodeBoxes nb = cur;
-            consume_pad(nb);
-            bool h0, h1;
-            float tn0, tn1;
-            box2(nb, o, inv, tmin, tbest, &h0, &tn0, &h1, &tn1);
-            const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
-            const int cn = first == 0 ? nb.ch.x : nb.ch.y, cf = first == 0 ? nb.ch.y : nb.ch.x;
-            const bool hn = first == 0 ? h0 : h1, hf = first == 0 ? h1 : h0;
-            const bool ln = hn && cn < 0, lf = hf && cf < 0;
-            LeafRegs gn{}, gf{};
-            if (ln) gn = load_leaf(a.bvh.leaf_geo, (uint32_t)~cn);
-            if (lf) gf = load_leaf(a.bvh.leaf_geo, (uint32_t)~cf);
-            const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
-            if (i0 && i1) {
-                if (!st.push(lds, sp, cf)) { atomicOr(a.error, 1u); node = kDone; }
-                else node = cn;
-            } else if (i0) {
-                node = nb.ch.x;
-            } else if (i1) {
-                node = nb.ch.y;
-            } else {
-                node = sp == 0 ? kDone : st.pop(lds, sp);
-            }
-            cur = load_node(a.bvh.nodes, node != kDone ? node : 0);
+        cur = load_node(a.bvh.nodes, next != kDone ? next : 0);
+        if (active) {
+            // the running best is replaced only by a strictly nearer hit or, at equal
+            // distance, by the lower leaf: the result does not depend on the order
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 if (!(k == 0 ? ln : lf)) continue;
@@ -299,14 +301,26 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
                     bdet = det;
                 }
             }
-            if (node == kDone) {
-                a.hits.t[ray] = tbest;
-                a.hits.leaf[ray] = best;
-                a.hits.u[ray] = best != kLeafMiss ? bun / bdet : 0.0f;
-                a.hits.v[ray] = best != kLeafMiss ? bvn / bdet : 0.0f;
-                ray = kNoRay;
-            }
         }
+        if (done) {
+            a.hits.t[ray] = tbest;
+            a.hits.leaf[ray] = best;
+            a.hits.u[ray] = best != kLeafMiss ? bun / bdet : 0.0f;
+            a.hits.v[ray] = best != kLeafMiss ? bvn / bdet : 0.0f;
+            ray = kNoRay;
+        }
+        if (fresh) {
+            ray = got;
+            o = no;
+            d = nd;
+            inv = safe_inv(nd);
+            tbest = a.wave.ray_length;
+            best = kLeafMiss;
+            bun = bvn = 0.0f;
+            bdet = 1.0f;
+            sp = 0;
+        }
+        if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
 }
 
@@ -368,7 +382,7 @@ __device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bo
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_shadow(ShadowArgs a) {
     __shared__ int lds[L][256];
     const uint32_t count = *a.count_in;
     const float tmin = kTMin, tmax = kShadowTMax;
@@ -376,70 +390,81 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
         for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) apply_shadow(a, i, false);
         return;
     }
+    // Same iteration as the closest-hit kernel: box tests and the next node, refill,
+    // the single node load, then the leaf tests and ray set-up while it is in flight.
+    // A ray found occluded by its leaf tests ends after the load was issued for it;
+    // its lane takes a new ray at the next iteration.
     uint32_t ray = kNoRay;
     bool exhausted = false;
     WorkCursor wc = work_cursor(count, a.parts, a.refill);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
-    int node = kDone;
-    bool occ = false;
     ShortStack<L> st;
     int sp = 0;
-    NodeBoxes cur{};
+    NodeBoxes cur = load_node(a.bvh.nodes, 0);
     for (;;) {
-        const uint32_t got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted,
-                                       __ballot(ray != kNoRay) != 0ull, wc);
-        if (got != kNoRay) {
-            ray = got;
-            o = v3(a.q.ox[ray], a.q.oy[ray], a.q.oz[ray]);
-            d = v3(a.q.dx[ray], a.q.dy[ray], a.q.dz[ray]);
-            inv = safe_inv(d);
-            node = 0;
-            sp = 0;
-            occ = false;
-            cur = load_node(a.bvh.nodes, 0);
-        }
-        if (__ballot(ray != kNoRay) == 0ull) {
-            if (__ballot(!exhausted) == 0ull) break;
-            continue;
-        }
-        if (ray != kNoRay) {
-            // node-centric any-hit step: leaf records of the hit leaf children are loaded,
-            // then the next internal node (child 0 first; the answer is order-independent)
-            // is chosen from the box tests and its load issued on every path, then the
-            // leaves are intersected while the node load is in flight
+        const bool active = ray != kNoRay;
+        int next = kDone;
+        int c0, c1;              // set, with the leaf records, where l0 / l1 are set
+        bool l0 = false, l1 = false;
+        LeafRegs g0, g1;
+        if (active) {
+            // node-centric any-hit step: the next internal node is chosen from the box
+            // tests (child 0 first; the answer is order-independent)
             const NodeBoxes nb = cur;
             consume_pad(nb);
             bool h0, h1;
             float tn0, tn1;
             box2(nb, o, inv, tmin, tmax, &h0, &tn0, &h1, &tn1);
-            const bool l0 = h0 && nb.ch.x < 0, l1 = h1 && nb.ch.y < 0;
-            LeafRegs g0{}, g1{};
-            if (l0) g0 = load_leaf(a.bvh.leaf_geo, (uint32_t)~nb.ch.x);
-            if (l1) g1 = load_leaf(a.bvh.leaf_geo, (uint32_t)~nb.ch.y);
-            const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
+            c0 = nb.ch.x;
+            c1 = nb.ch.y;
+            l0 = h0 && c0 < 0;
+            l1 = h1 && c1 < 0;
+            if (l0) g0 = load_leaf(a.bvh.leaf_geo, (uint32_t)~c0);
+            if (l1) g1 = load_leaf(a.bvh.leaf_geo, (uint32_t)~c1);
+            const bool i0 = h0 && c0 >= 0, i1 = h1 && c1 >= 0;
             if (i0 && i1) {
-                if (!st.push(lds, sp, nb.ch.y)) { atomicOr(a.error, 1u); node = kDone; }
-                else node = nb.ch.x;
+                if (!st.push(lds, sp, c1)) atomicOr(a.error, 1u);
+                else next = c0;
             } else if (i0) {
-                node = nb.ch.x;
+                next = c0;
             } else if (i1) {
-                node = nb.ch.y;
-            } else {
-                node = sp == 0 ? kDone : st.pop(lds, sp);
+                next = c1;
+            } else if (sp > 0) {
+                next = st.pop(lds, sp);
             }
-            cur = load_node(a.bvh.nodes, node != kDone ? node : 0);
+        }
+        const bool done = active && next == kDone;
+        const uint32_t got = fetch_ray(a.work, count, (!active || done) && !exhausted, &exhausted,
+                                       __ballot(active && !done) != 0ull, wc);
+        const bool fresh = got != kNoRay;
+        V3 no = o, nd = d;
+        if (fresh) {
+            no = v3(a.q.ox[got], a.q.oy[got], a.q.oz[got]);
+            nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
+            next = 0;
+        }
+        cur = load_node(a.bvh.nodes, next != kDone ? next : 0);
+        bool occ = false;
+        if (active) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 if (!(c == 0 ? l0 : l1) || occ) continue;
                 float t, un, vn, det;
                 if (tri_test(c == 0 ? g0 : g1, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax) occ = true;
             }
-            if (occ) node = kDone;
-            if (node == kDone) {
-                apply_shadow(a, ray, occ);
-                ray = kNoRay;
-            }
         }
+        if (done || occ) {
+            apply_shadow(a, ray, occ);
+            ray = kNoRay;
+        }
+        if (fresh) {
+            ray = got;
+            o = no;
+            d = nd;
+            inv = safe_inv(nd);
+            sp = 0;
+        }
+        if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
 }
 
