@@ -103,7 +103,10 @@ def main():
     achieved = (alg / launches) / (avg_launch_ms * 1e-3) / 1e9
     timing = {k: round(s1[k], 3) for k in ("ms_build", "ms_trace_closest", "ms_shade", "ms_trace_shadow",
                                            "ms_resolve", "ms_total")}
+    rays_by_kind = {k: int(s1[k] - s0[k]) for k in ("rays_camera", "rays_extension", "rays_light_shadow",
+                                                    "rays_sky_shadow")}
     log(f"kernel classes (ms, one frame): {timing}")
+    log(f"rays by kind (one frame): {rays_by_kind}")
 
     result = {
         "metric": "Mrays/s and p99 render-job latency at equal RMSE vs FP64 oracle",
@@ -129,6 +132,7 @@ def main():
                            "p99": round(float(np.percentile(lat, 99)), 3)},
         "rays_per_step": int(rays // args.steps),
         "kernel_ms_one_frame": timing,
+        "rays_by_kind_one_frame": rays_by_kind,
         "roofline": {
             "kernel": "k_trace_closest (camera + extension rays)",
             "bound": "hbm",

@@ -27,7 +27,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
            "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
-           "bi_pt_debug_get_bvh", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
+           "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace"]
 
 
@@ -96,6 +96,7 @@ def lib():
         L.bi_pt_reset_accumulation.argtypes = [vp]
         L.bi_pt_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.bi_pt_debug_get_bvh.argtypes = [vp, u32p, u32p, i32p, f32p, i32p, f32p]
+        L.bi_pt_debug_get_bvh4.argtypes = [vp, i32p, f32p]
         L.bi_pt_debug_num_tris.argtypes = [vp]
         L.bi_pt_debug_num_tris.restype = u32
         L.bi_pt_debug_queue.argtypes = [vp, u32, u32, u32, u32p, u32, u32p]
@@ -243,6 +244,13 @@ class PathTracer:
                                               _p(children, C.c_int32), _p(bounds), _p(parents, C.c_int32), _p(cb)))
         return dict(morton=morton, order=order, children=children, bounds=bounds, parents=parents,
                     centroid_bounds=cb)
+
+    def bvh4(self):
+        ni = max(self.num_tris() - 1, 1)
+        child4 = np.zeros((ni, 4), np.int32)
+        box4 = np.zeros((ni, 24), np.float32)
+        self._check(lib().bi_pt_debug_get_bvh4(self.h, _p(child4, C.c_int32), _p(box4)))
+        return child4, box4
 
     def queue(self, wave, bounce, kind):
         cnt = C.c_uint32()

@@ -64,6 +64,7 @@ struct bi_pt_ctx {
     DevBuf<float> tri_v, tri_n, cb_out;
     DevBuf<uint32_t> tri_mat, cb_ord, keys0, keys1, vals0, vals1, hist, refit_cnt;
     DevBuf<BvhNode> nodes;
+    DevBuf<Bvh4Node> nodes4;
     DevBuf<int32_t> parent;
     DevBuf<LeafGeo> leaf_geo;
     DevBuf<LeafShade> leaf_shade;
@@ -178,7 +179,7 @@ void build_sky_table(const float* rgba, uint32_t W, uint32_t H, const float colo
 }
 
 BvhView bvh_view(const bi_pt_ctx* ctx) {
-    return BvhView{ctx->nodes.p, ctx->leaf_geo.p, ctx->num_tris};
+    return BvhView{ctx->nodes4.p, ctx->leaf_geo.p, ctx->num_tris};
 }
 
 bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
@@ -196,6 +197,7 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
     TRY(dalloc(ctx, ctx->vals1, n));
     TRY(dalloc(ctx, ctx->hist, sort_hist_entries(n)));
     TRY(dalloc(ctx, ctx->nodes, ni));
+    TRY(dalloc(ctx, ctx->nodes4, ni));
     TRY(dalloc(ctx, ctx->parent, ni + n));
     TRY(dalloc(ctx, ctx->refit_cnt, ni));
     TRY(dalloc(ctx, ctx->leaf_geo, n));
@@ -224,6 +226,7 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s) {
     a.vals[1] = ctx->vals1.p;
     a.hist = ctx->hist.p;
     a.nodes = ctx->nodes.p;
+    a.nodes4 = ctx->nodes4.p;
     a.parent = ctx->parent.p;
     a.refit_cnt = ctx->refit_cnt.p;
     a.leaf_geo = ctx->leaf_geo.p;
@@ -307,7 +310,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     dfree(ctx, ctx->sky_marg);
     dfree(ctx, ctx->tri_v); dfree(ctx, ctx->tri_n); dfree(ctx, ctx->cb_out); dfree(ctx, ctx->tri_mat);
     dfree(ctx, ctx->cb_ord); dfree(ctx, ctx->keys0); dfree(ctx, ctx->keys1); dfree(ctx, ctx->vals0);
-    dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes);
+    dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes); dfree(ctx, ctx->nodes4);
     dfree(ctx, ctx->parent); dfree(ctx, ctx->leaf_geo); dfree(ctx, ctx->leaf_shade);
     for (int k = 0; k < 6; ++k) { dfree(ctx, ctx->qa[k]); dfree(ctx, ctx->qb[k]); }
     for (int k = 0; k < 3; ++k) dfree(ctx, ctx->hit_f[k]);
@@ -494,7 +497,8 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
         return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "empty image or zero samples");
     const uint64_t npix64 = (uint64_t)camera->width * camera->height;
     if (npix64 > (1ull << 26)) return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "image larger than 2^26 pixels");
-    if (!(st->ray_length > 0.0f)) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "ray_length must be positive");
+    if (!(st->ray_length > 0.0f) || !(st->ray_length <= kShadowTMax))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "ray_length must be positive and at most 1.7e38");
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const uint32_t npix = (uint32_t)npix64;
@@ -761,8 +765,8 @@ bi_pt_status bi_pt_get_stats(const bi_pt_ctx* cctx, bi_pt_stats* out) {
         *out = ctx->stats;
         return fail(ctx, BI_PT_ERROR_DEVICE_CHECK,
                     std::string("device check failed: ") + ((hc[kCntError] & 1u) ? "traversal stack overflow " : "") +
-                        ((hc[kCntError] & 2u) ? "look-back spin limit " : "") +
-                        ((hc[kCntError] & 4u) ? "refit wait limit" : ""));
+                        ((hc[kCntError] & 4u) ? "refit wait limit " : "") +
+                        ((hc[kCntError] & 8u) ? "traversal step bound exceeded" : ""));
     }
     return BI_PT_OK;
 }
@@ -799,6 +803,24 @@ bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* ord
     return BI_PT_OK;
 }
 
+bi_pt_status bi_pt_debug_get_bvh4(bi_pt_ctx* ctx, int32_t* child4, float* box4) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->bvh_built) TRY(build_bvh(ctx, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    const uint32_t n = ctx->num_tris;
+    if (n == 0) return BI_PT_OK;
+    const size_t ni = n > 1 ? n - 1 : 1;
+    std::vector<Bvh4Node> nodes(ni);
+    HIP_TRY(hipMemcpy(nodes.data(), ctx->nodes4.p, ni * sizeof(Bvh4Node), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ni; ++i) {
+        if (child4) std::memcpy(&child4[4 * i], nodes[i].child, 16);
+        if (box4) std::memcpy(&box4[24 * i], nodes[i].lox, 96);
+    }
+    return BI_PT_OK;
+}
+
 uint32_t bi_pt_debug_num_tris(const bi_pt_ctx* ctx) { return ctx ? ctx->num_tris : 0; }
 
 bi_pt_status bi_pt_debug_queue(bi_pt_ctx* ctx, uint32_t wave, uint32_t bounce, uint32_t kind, uint32_t* out,
@@ -831,6 +853,8 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
     if (!ctx->bvh_built) TRY(build_bvh(ctx, nullptr));
     // all rays share tmin = 0.001 and the first ray's tmax
     const float tmax = rays[7];
+    if (!(tmax > 0.001f) || !(tmax <= kShadowTMax))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "debug trace needs 0.001 < tmax <= 1.7e38");
     for (uint32_t i = 0; i < n; ++i)
         if (rays[8 * i + 3] != 0.001f || rays[8 * i + 7] != tmax)
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "debug trace needs tmin = 0.001 and one tmax");

@@ -339,6 +339,61 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
     }
 }
 
+// BVH4 collapse.  Every internal BVH2 node k is turned into a BVH4 node stored at
+// index k, independently of the others: starting from k's two children, the internal
+// child with the largest box surface (ties: the lower slot) is replaced in its slot by
+// its first child and its second child is appended, until four slots are filled or
+// only leaves are left.  Traversal starts at node 0 and follows internal links, which
+// are BVH2 node indices and so name the BVH4 node built from that node; nodes not
+// reached that way are never read.  Runs after the refit: every box is final.
+__device__ __forceinline__ float half_area(const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return (dx * dy + dy * dz) + dz * dx;
+}
+
+__global__ __launch_bounds__(256) void k_collapse4(const BvhNode* __restrict__ nodes, int ni,
+                                                   Bvh4Node* __restrict__ out) {
+    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (k >= ni) return;
+    int id[4];
+    float box[4][6];
+    int cnt = 2;
+    for (int c = 0; c < 2; ++c) {
+        id[c] = nodes[k].child[c];
+        for (int a = 0; a < 6; ++a) box[c][a] = nodes[k].box[6 * c + a];
+    }
+    while (cnt < 4) {
+        int pick = -1;
+        float best = -1.0f;
+        for (int j = 0; j < cnt; ++j) {
+            if (id[j] < 0) continue;
+            const float ar = half_area(box[j]);
+            if (ar > best) { best = ar; pick = j; }
+        }
+        if (pick < 0) break;
+        const BvhNode& m = nodes[id[pick]];
+        id[cnt] = m.child[1];
+        for (int a = 0; a < 6; ++a) box[cnt][a] = m.box[6 + a];
+        id[pick] = m.child[0];
+        for (int a = 0; a < 6; ++a) box[pick][a] = m.box[a];
+        ++cnt;
+    }
+    Bvh4Node r;
+    const float inf = __builtin_inff();
+    for (int j = 0; j < 4; ++j) {
+        const bool used = j < cnt;
+        r.lox[j] = used ? box[j][0] : inf;
+        r.loy[j] = used ? box[j][1] : inf;
+        r.loz[j] = used ? box[j][2] : inf;
+        r.hix[j] = used ? box[j][3] : inf;
+        r.hiy[j] = used ? box[j][4] : inf;
+        r.hiz[j] = used ? box[j][5] : inf;
+        r.child[j] = used ? id[j] : 0;
+        r.pad[j] = 0;
+    }
+    out[k] = r;
+}
+
 }  // namespace
 
 void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
@@ -363,6 +418,8 @@ void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, a.keys[cur], (int)n, a.nodes, a.parent, a.refit_cnt);
     hipLaunchKernelGGL(k_leaves, dim3(g256), dim3(256), 0, s, a.vals[cur], a.tri_v, a.tri_n, a.tri_mat, (int)n,
                        a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.error);
+    const int ni = n > 1 ? (int)n - 1 : 1;
+    hipLaunchKernelGGL(k_collapse4, dim3((ni + 255) / 256), dim3(256), 0, s, a.nodes, ni, a.nodes4);
 }
 
 uint32_t sort_hist_entries(uint32_t n) { return 256u * ((n + kSortTile - 1) / kSortTile); }

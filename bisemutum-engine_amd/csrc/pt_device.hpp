@@ -50,7 +50,9 @@ constexpr float kTwoPiSq = 19.7392088021787172376f;
 constexpr float kTMin = 0.001f;            // rt_gbuffer.hlsl:18 (TMin)
 constexpr float kRobust = 1.00000036f;     // 1 + 2*gamma(3): conservative slab test (Ize 2013)
 constexpr float kThroughputMin = 0.001f;   // sample_secondary_ray.hlsl:24
-constexpr float kShadowTMax = 3.40282347e38f;
+// Largest ray extent: half of FLT_MAX, so tmax * kRobust stays finite and the empty
+// BVH4 slot (+inf box) is rejected by every slab test.
+constexpr float kShadowTMax = 1.70141173e38f;
 
 // ------------------------------------------------------------ transcendentals
 // sin/cos/asin/acos/atan in single precision after the Cephes library (S. L.

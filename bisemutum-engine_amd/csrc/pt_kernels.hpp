@@ -37,6 +37,7 @@ struct BuildArgs {
     uint32_t* vals[2];
     uint32_t* hist;
     BvhNode* nodes;
+    Bvh4Node* nodes4;
     int32_t* parent;
     uint32_t* refit_cnt;
     LeafGeo* leaf_geo;

@@ -4,13 +4,16 @@
 //   scene      mesh pool (float xyz positions/normals, uint32 indices), instance records,
 //              material table (64 B each), equirect sky + importance tables
 //   per frame  world triangles (36 B) + normals (36 B) in instance order, Morton keys,
-//              sorted order, BVH2 nodes (64 B: both child boxes + child links, one node
-//              per 128-B line half), leaf intersection records (48 B: v0, e1, e2) and
-//              leaf shading records (48 B: n0, n1, n2, material) in sorted leaf order
+//              sorted order, BVH2 nodes (64 B: both child boxes + child links; the build
+//              structure), BVH4 nodes (128 B, one 128-B line: four child boxes as six
+//              4-wide planes + four child links; the traversal structure, stored at the
+//              index of the BVH2 node it collapses), leaf intersection records (48 B:
+//              v0, e1, e2) and leaf shading records (48 B: n0, n1, n2, material) in sorted
+//              leaf order
 //   per wave   SoA ray queues (o.xyz, d.xyz, path id), SoA hit records (t, leaf, u, v),
 //              SoA path state (throughput, last BSDF pdf), per-kind radiance accumulators,
 //              SoA shadow queue (o.xyz, d.xyz, contribution.xyz, path<<3|kind)
-// Nodes are read whole by one lane (4 x 16-B loads); rays, hits and path state are
+// Nodes are read whole by one lane (7 x 16-B loads); rays, hits and path state are
 // read by consecutive lanes at consecutive addresses (SoA, coalesced).
 #pragma once
 
@@ -21,7 +24,10 @@ namespace bipt {
 
 constexpr int kMaxDirLights = 4;
 constexpr int kMaxKinds = kMaxDirLights + 2;   // miss, lights..., sky
-constexpr int kStackMax = 64;                  // >= 30 Morton + 34 index levels (Karras depth bound)
+// Traversal stack bound: a Karras tree over 30-bit Morton codes with index tie-breaks
+// is at most 63 levels deep, a BVH4 level spans at least one of them, and a step pushes
+// at most 3 siblings.
+constexpr int kStackMax = 192;
 constexpr uint32_t kLeafMiss = 0xffffffffu;
 constexpr uint32_t kKindSky = 7u;
 
@@ -31,6 +37,16 @@ struct alignas(16) BvhNode {
     int32_t pad[2];
 };
 static_assert(sizeof(BvhNode) == 64, "node record is 64 bytes");
+
+// BVH4 node: child k's box is (lox[k], loy[k], loz[k]) - (hix[k], hiy[k], hiz[k]);
+// child k is internal (BVH4 node index >= 0) or leaf j (stored as ~j).  An empty slot
+// has the box (+inf,+inf,+inf)-(+inf,+inf,+inf), which every slab test rejects.
+struct alignas(128) Bvh4Node {
+    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "BVH4 node record is 128 bytes");
 
 struct alignas(16) LeafGeo {
     float v[12];       // v0.xyz, e1.xyz, e2.xyz, unused
@@ -94,7 +110,7 @@ struct SkyParams {
 };
 
 struct BvhView {
-    const BvhNode* nodes;
+    const Bvh4Node* nodes;
     const LeafGeo* leaf_geo;
     uint32_t num_tris;
 };
@@ -126,7 +142,7 @@ enum CounterSlot : uint32_t {
     kCntExtA = 0,       // entries in extension queue A
     kCntExtB = 1,       // entries in extension queue B
     kCntShadow = 2,     // entries in the shadow queue
-    kCntError = 4,      // nonzero: a kernel hit an internal limit
+    kCntError = 4,      // nonzero: a kernel hit an internal limit (1 stack, 4 refit wait, 8 step bound)
     kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
     kCntTraceWork = 32,                                       // part counters of the closest-hit lanes
     kCntShadowWork = kCntTraceWork + kWorkPartsMax * kWorkStride, // part counters of the shadow lanes

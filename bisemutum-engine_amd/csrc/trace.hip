@@ -2,19 +2,20 @@
 //
 // Replaces TraceRay of rt_gbuffer.hlsl:25 (closest hit, TMin 0.001, TMax
 // ray_length) and the shadow-map lookups of lights.hlsl:69-101 (exact any-hit
-// shadow rays).  Rays come from SoA queues (coalesced); nodes are read whole
-// (4 x 16 B).  Short stack: the top `L` entries live in LDS ([slot][lane], bank
-// conflict free for equal depths), deeper entries spill to a per-lane private
-// array; the total depth bound (64) covers any Karras tree over < 2^34 triangles.
+// shadow rays).  Rays come from SoA queues (coalesced); the traversal structure is
+// the BVH4 collapsed from the LBVH, one 128-B node per step (7 x 16-B loads).  Short
+// stack: the top `L` entries live in LDS ([slot][lane], bank conflict free for equal
+// depths), deeper entries spill to a per-lane private array; the total depth bound
+// kStackMax covers any tree the build can produce.
 //
 // Persistent lanes with dynamic ray fetch: a workgroup-resident grid of wavefronts
-// keeps every lane busy -- a lane whose ray has finished takes the next ray index
-// from a global counter (one atomic per wavefront and refill, lanes ranked by a
-// ballot), so secondary rays of very different lengths do not idle the SIMD.
-// Inside a ray, the "while-while" loop of Aila & Laine 2009 descends internal nodes
-// until a leaf, then tests leaves until an internal node is popped.  Each lane's
-// visiting order is a function of its own ray only (nearer child first, ties to
-// child 0; stack LIFO) -- the order of the CPU restatement -- so hits agree bit for bit.
+// keeps lanes busy -- lanes whose rays have finished take the next ray indices from
+// per-XCD counters (one atomic per wavefront and refill, lanes ranked by a ballot), so
+// secondary rays of very different lengths do not idle the SIMD.  Each step is
+// node-centric: the four child boxes of one node are tested, hit leaves intersected,
+// the nearest hit internal child entered and the other hit internal children pushed
+// (ascending slot order).  Each lane's visiting order is a function of its own ray
+// only, the order of the CPU restatement, so hits and error flags agree bit for bit.
 #include "pt_kernels.hpp"
 
 namespace bipt {
@@ -32,47 +33,38 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
     return v3(one(d.x), one(d.y), one(d.z));
 }
 
-struct NodeBoxes {
-    float4 a, b, c;
-    int2 ch;
-    int2 pad;
+struct Node4 {
+    float4 lx, ly, lz, hx, hy, hz;   // child k's box in component k
+    int4 ch;
 };
-__device__ __forceinline__ NodeBoxes load_node(const BvhNode* nodes, int i) {
+__device__ __forceinline__ Node4 load_node(const Bvh4Node* nodes, int i) {
     const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    NodeBoxes n;
-    n.a = p[0];
-    n.b = p[1];
-    n.c = p[2];
-    const int4 d = reinterpret_cast<const int4*>(p)[3];
-    n.ch = make_int2(d.x, d.y);
-    n.pad = make_int2(d.z, d.w);
+    Node4 n;
+    n.lx = p[0];
+    n.ly = p[1];
+    n.lz = p[2];
+    n.hx = p[3];
+    n.hy = p[4];
+    n.hz = p[5];
+    n.ch = reinterpret_cast<const int4*>(p)[6];
     return n;
 }
-// The last 16 B of a node are loaded whole (the compiler widens a partial load to
-// it); keeping the two padding words formally live until the node is consumed stops
-// the register allocator from reusing them while the load is in flight, which would
-// force a wait for the load right after issuing it.
-__device__ __forceinline__ void consume_pad(const NodeBoxes& n) { asm volatile("" ::"v"(n.pad.x), "v"(n.pad.y)); }
 
-// Slab test of both children.  No operand can be NaN ((b - o) is finite and
-// inv is finite and nonzero), so fminf/fmaxf select exactly what the CPU's
-// comparisons select, up to the sign of zero, which only ever reaches comparisons.
-__device__ __forceinline__ void box2(const NodeBoxes& n, V3 o, V3 inv, float tmin, float tmax, bool* h0, float* tn0,
-                                     bool* h1, float* tn1) {
-    float x0 = (n.a.x - o.x) * inv.x, x1 = (n.a.w - o.x) * inv.x;
-    float y0 = (n.a.y - o.y) * inv.y, y1 = (n.b.x - o.y) * inv.y;
-    float z0 = (n.a.z - o.z) * inv.z, z1 = (n.b.y - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-    float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-    *tn0 = tn;
-    *h0 = tn <= tf * kRobust;
-    x0 = (n.b.z - o.x) * inv.x; x1 = (n.c.y - o.x) * inv.x;
-    y0 = (n.b.w - o.y) * inv.y; y1 = (n.c.z - o.y) * inv.y;
-    z0 = (n.c.x - o.z) * inv.z; z1 = (n.c.w - o.z) * inv.z;
-    tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-    tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-    *tn1 = tn;
-    *h1 = tn <= tf * kRobust;
+__device__ __forceinline__ float comp(float4 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+__device__ __forceinline__ int comp(int4 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// Slab test of child k (k a compile-time constant after unrolling).  No operand can
+// be NaN ((b - o) is finite or +inf and inv is finite and nonzero), so fminf/fmaxf
+// select exactly what the CPU's comparisons select, up to the sign of zero, which
+// only ever reaches comparisons.
+__device__ __forceinline__ bool box_k(const Node4& n, int k, V3 o, V3 inv, float tmin, float tmax, float* tn_out) {
+    const float x0 = (comp(n.lx, k) - o.x) * inv.x, x1 = (comp(n.hx, k) - o.x) * inv.x;
+    const float y0 = (comp(n.ly, k) - o.y) * inv.y, y1 = (comp(n.hy, k) - o.y) * inv.y;
+    const float z0 = (comp(n.lz, k) - o.z) * inv.z, z1 = (comp(n.hz, k) - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    *tn_out = tn;
+    return tn <= tf * kRobust;
 }
 
 // Leaf intersection record held in registers: v0, e1, e2.
@@ -151,32 +143,30 @@ struct ShortStack {
     }
 };
 
-// Refill idle lanes of the wavefront, once `refill` lanes are idle (or no lane is
-// busy): one atomic per refill.  The queue is split into `parts` contiguous
-// parts, each with its own counter on its own 128-B line; a wavefront starts on the
-// part of its workgroup's XCD (workgroups go to the XCDs round-robin) and moves on to
-// the following parts when its part runs dry, so no single counter sees every
-// refill.  Returns the new ray index for a lane that asked and got one, kNoRay
-// otherwise; sets *exhausted when every part is drained.
+// Ray fetch for the lanes that ask (the caller decides when a wavefront refills: once
+// `refill` lanes are idle, or no lane is busy): one atomic per refill.  The queue is
+// split into `parts` contiguous parts, each with its own counter on its own 128-B
+// line; a wavefront starts on the part of its workgroup's XCD (workgroups go to the
+// XCDs round-robin) and moves on to the following parts when its part runs dry, so
+// no single counter sees every refill.  Returns the new ray index for a lane that
+// asked and got one, kNoRay otherwise; sets *exhausted when every part is drained.
 struct WorkCursor {
     uint32_t part;    // part the wavefront draws from
     uint32_t left;    // parts not yet found empty
     uint32_t parts;
     uint32_t chunk;   // rays per part (the last parts may hold fewer)
-    uint32_t refill;
 };
-__device__ __forceinline__ WorkCursor work_cursor(uint32_t count, uint32_t parts, uint32_t refill) {
-    return WorkCursor{blockIdx.x % parts, parts, parts, (count + parts - 1u) / parts, refill};
+__device__ __forceinline__ WorkCursor work_cursor(uint32_t count, uint32_t parts) {
+    return WorkCursor{blockIdx.x % parts, parts, parts, (count + parts - 1u) / parts};
 }
 __device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bool want, bool* exhausted,
-                                              bool any_busy, WorkCursor& wc) {
+                                              WorkCursor& wc) {
     const uint64_t m = __ballot(want);
     if (m == 0ull) return kNoRay;
     if (wc.left == 0) {
         if (want) *exhausted = true;
         return kNoRay;
     }
-    if (any_busy && (uint32_t)__popcll(m) < wc.refill) return kNoRay;
     const uint32_t lane = __lane_id();
     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
     const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -198,9 +188,53 @@ __device__ __forceinline__ uint32_t fetch_ray(uint32_t* work, uint32_t count, bo
     }
 }
 
+// Deferred leaf tests.  A box step does not intersect the leaves it reaches: it appends
+// (leaf, owner lane) to its wavefront's queue in LDS, and the queue is flushed -- 64
+// tests at a time, one per lane, each lane testing a leaf of any owner with the
+// owner's ray fetched by a cross-lane read -- once it holds a wavefront's worth of
+// tests, before lanes take new rays, or when no lane has a node left to visit.  So the
+// triangle tests run with full wavefronts instead of the few lanes that reach a leaf
+// in a given step.  A closest-hit candidate is accepted only where its leaf's own box
+// passes the slab test up to the candidate's distance t.  Boxes nest and the slab
+// arithmetic rounds monotonically, so every ancestor box of an accepted candidate then
+// passes for any running distance >= t: each accepted candidate is reached whatever the
+// pruning schedule, and the closest hit -- the minimum of (t, leaf) over accepted
+// candidates -- does not depend on when or in which order leaves are tested.  An
+// occlusion is an OR over leaves tested against a fixed extent.  A deferred hit only
+// prunes later box tests a little later.
+constexpr uint32_t kLeafQueue = 64u + 4u * 64u;   // < 64 before a step, which adds <= 4 per lane
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Appends the hit leaf children (mask lm over the slots of ch) of every lane, with their
+// box entry distances when qtn is given; returns the new queue length (uniform).
+__device__ __forceinline__ uint32_t append_leaves(uint32_t* qleaf, uint8_t* qown, float* qtn, uint32_t qn,
+                                                  uint32_t lm, int4 ch, float4 tn, uint32_t lane) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool bit = (lm >> k) & 1u;
+        const uint64_t m = __ballot(bit);
+        if (m == 0ull) continue;
+        if (bit) {
+            const uint32_t idx = qn + lanes_below(m);
+            qleaf[idx] = (uint32_t)~comp(ch, k);
+            qown[idx] = (uint8_t)lane;
+            if (qtn) qtn[idx] = comp(tn, k);
+        }
+        qn += (uint32_t)__popcll(m);
+    }
+    return qn;
+}
+
 template <bool PRIMARY, int L>
 __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     __shared__ int lds[L][256];
+    __shared__ uint32_t s_qleaf[4][kLeafQueue];
+    __shared__ uint8_t s_qown[4][kLeafQueue];
+    __shared__ float s_qtn[4][kLeafQueue];
+    __shared__ unsigned long long s_key[4][64];
     const uint32_t count = PRIMARY ? a.wave.num_paths : *a.count_in;
     const float tmin = kTMin;
     if (a.bvh.num_tris == 0) {
@@ -212,58 +246,130 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
         }
         return;
     }
-    // Each iteration: (1) the lanes holding a ray test both child boxes of `cur` and
-    // choose the next node, issuing the loads of the hit leaf children; (2) lanes whose
-    // ray ends here and idle lanes ask for new rays (a new ray starts at the root) and
-    // issue the loads of their ray data; (3) the next node is loaded, at this single
-    // place, so the loop keeps it in the same registers; (4) the leaf tests of (1) and
-    // the ray set-up of (2) run while that load is in flight.
+    // Each iteration: (1) lanes with a node to visit test its four child boxes, push the
+    // hit internal children but the nearest, queue the hit leaves and move to the
+    // nearest (a lane with nothing left is parked until its queued leaves are tested);
+    // (2) the leaf queue is flushed when full, before a refill, or when no lane is busy,
+    // and parked lanes with no queued leaves write their hit; (3) idle lanes take new
+    // rays (starting at the root) once enough of them are idle; (4) the next node is
+    // loaded, at this single place, so the loop keeps it in the same registers.
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t* qleaf = s_qleaf[w];
+    uint8_t* qown = s_qown[w];
+    float* qtn = s_qtn[w];
+    unsigned long long* key = s_key[w];
+    key[lane] = ~0ull;
+    uint32_t qn = 0;
     uint32_t ray = kNoRay;
-    bool exhausted = false;
-    WorkCursor wc = work_cursor(count, a.parts, a.refill);
+    bool parked = false, exhausted = false;
+    WorkCursor wc = work_cursor(count, a.parts);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
-    float tbest = 0.0f, bun = 0.0f, bvn = 0.0f, bdet = 1.0f;
+    float tbest = 0.0f;
     uint32_t best = kLeafMiss;
     ShortStack<L> st;
     int sp = 0;
-    NodeBoxes cur = load_node(a.bvh.nodes, 0);
+    uint32_t steps = 0;
+    Node4 cur = load_node(a.bvh.nodes, 0);
     for (;;) {
-        const bool active = ray != kNoRay;
         int next = kDone;
-        int cn, cf;              // set, with the leaf records, where ln / lf are set
-        bool ln = false, lf = false;
-        LeafRegs gn, gf;
-        if (active) {
-            // node-centric step: both boxes against [tmin, tbest]; hit leaf children are
-            // intersected below (nearer first), the nearer internal child is entered
-            const NodeBoxes nb = cur;
-            consume_pad(nb);
-            bool h0, h1;
-            float tn0, tn1;
-            box2(nb, o, inv, tmin, tbest, &h0, &tn0, &h1, &tn1);
-            const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
-            cn = first == 0 ? nb.ch.x : nb.ch.y;
-            cf = first == 0 ? nb.ch.y : nb.ch.x;
-            const bool hn = first == 0 ? h0 : h1, hf = first == 0 ? h1 : h0;
-            ln = hn && cn < 0;
-            lf = hf && cf < 0;
-            if (ln) gn = load_leaf(a.bvh.leaf_geo, (uint32_t)~cn);
-            if (lf) gf = load_leaf(a.bvh.leaf_geo, (uint32_t)~cf);
-            const bool i0 = h0 && nb.ch.x >= 0, i1 = h1 && nb.ch.y >= 0;
-            if (i0 && i1) {
-                if (!st.push(lds, sp, cf)) atomicOr(a.error, 1u);
-                else next = cn;
-            } else if (i0) {
-                next = nb.ch.x;
-            } else if (i1) {
-                next = nb.ch.y;
-            } else if (sp > 0) {
-                next = st.pop(lds, sp);
+        uint32_t lm = 0;
+        int4 ch = make_int4(0, 0, 0, 0);
+        float4 tnv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ray != kNoRay && !parked) {
+            const Node4 nb = cur;
+            ch = nb.ch;
+            float tn[4];
+            bool h[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) h[k] = box_k(nb, k, o, inv, tmin, tbest, &tn[k]);
+            tnv = make_float4(tn[0], tn[1], tn[2], tn[3]);
+            int nearest = -1, nearc = kDone;
+            float tnear = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = comp(ch, k);
+                if (h[k] && c < 0) lm |= 1u << k;
+                if (h[k] && c >= 0 && (nearest < 0 || tn[k] < tnear)) {
+                    nearest = k;
+                    nearc = c;
+                    tnear = tn[k];
+                }
+            }
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = comp(ch, k);
+                if (ok && k != nearest && h[k] && c >= 0) ok = st.push(lds, sp, c);
+            }
+            if (!ok) atomicOr(a.error, 1u);
+            else if (nearest >= 0) next = nearc;
+            else if (sp > 0) next = st.pop(lds, sp);
+            // a ray enters each node at most once: more steps than nodes means corrupt
+            // links, and the ray is ended instead of looping
+            if (++steps > a.bvh.num_tris) {
+                atomicOr(a.error, 8u);
+                next = kDone;
+            }
+            parked = next == kDone;
+        }
+        qn = append_leaves(qleaf, qown, qtn, qn, lm, ch, tnv, lane);
+        const bool busy = ray != kNoRay && !parked;
+        const uint64_t wm = __ballot((ray == kNoRay || parked) && !exhausted);
+        const bool any_busy = __ballot(busy) != 0ull;
+        const bool refill_now = wm != 0ull && (!any_busy || (uint32_t)__popcll(wm) >= a.refill);
+        if (qn != 0u && (qn >= 64u || refill_now || !any_busy)) {
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t b = 0; b < qn; b += 64u) {
+                const uint32_t e = b + lane;
+                const bool valid = e < qn;
+                const uint32_t leaf = valid ? qleaf[e] : 0u;
+                const float tnl = valid ? qtn[e] : 0.0f;
+                const int owner = valid ? (int)qown[e] : (int)lane;
+                const V3 oo = v3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
+                const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
+                const float tb = __shfl(tbest, owner);
+                if (valid) {
+                    // a hit counts only where the leaf's own box passes the slab test up to
+                    // the hit distance (the acceptance rule that makes the closest hit
+                    // independent of the order leaves are found in)
+                    float t, un, vn, det;
+                    if (tri_test(a.bvh.leaf_geo, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tb &&
+                        tnl <= t * kRobust)
+                        atomicMin(&key[owner], ((unsigned long long)__float_as_uint(t) << 32) | leaf);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            qn = 0;
+            const unsigned long long kb = key[lane];
+            if (kb != ~0ull) {
+                key[lane] = ~0ull;
+                const float t = __uint_as_float((uint32_t)(kb >> 32));
+                const uint32_t leaf = (uint32_t)kb;
+                if (t < tbest || (t == tbest && leaf < best)) {
+                    tbest = t;
+                    best = leaf;
+                }
             }
         }
-        const bool done = active && next == kDone;
-        const uint32_t got = fetch_ray(a.work, count, (!active || done) && !exhausted, &exhausted,
-                                       __ballot(active && !done) != 0ull, wc);
+        if (parked && qn == 0u) {
+            // every leaf of the ray is tested: write the hit; u, v come from repeating
+            // the winning test
+            float u = 0.0f, v = 0.0f;
+            if (best != kLeafMiss) {
+                float t, un, vn, det;
+                tri_test(a.bvh.leaf_geo, best, o, d, &t, &un, &vn, &det);
+                u = un / det;
+                v = vn / det;
+            }
+            a.hits.t[ray] = tbest;
+            a.hits.leaf[ray] = best;
+            a.hits.u[ray] = u;
+            a.hits.v[ray] = v;
+            ray = kNoRay;
+            parked = false;
+        }
+        uint32_t got = kNoRay;
+        if (refill_now) got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted, wc);
         const bool fresh = got != kNoRay;
         V3 no = o, nd = d;
         if (fresh) {
@@ -284,31 +390,6 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
             next = 0;
         }
         cur = load_node(a.bvh.nodes, next != kDone ? next : 0);
-        if (active) {
-            // the running best is replaced only by a strictly nearer hit or, at equal
-            // distance, by the lower leaf: the result does not depend on the order
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (!(k == 0 ? ln : lf)) continue;
-                const uint32_t leaf = (uint32_t)~(k == 0 ? cn : cf);
-                float t, un, vn, det;
-                if (tri_test(k == 0 ? gn : gf, o, d, &t, &un, &vn, &det) && t >= tmin &&
-                    (t < tbest || (t == tbest && leaf < best))) {
-                    tbest = t;
-                    best = leaf;
-                    bun = un;
-                    bvn = vn;
-                    bdet = det;
-                }
-            }
-        }
-        if (done) {
-            a.hits.t[ray] = tbest;
-            a.hits.leaf[ray] = best;
-            a.hits.u[ray] = best != kLeafMiss ? bun / bdet : 0.0f;
-            a.hits.v[ray] = best != kLeafMiss ? bvn / bdet : 0.0f;
-            ray = kNoRay;
-        }
         if (fresh) {
             ray = got;
             o = no;
@@ -316,17 +397,16 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
             inv = safe_inv(nd);
             tbest = a.wave.ray_length;
             best = kLeafMiss;
-            bun = bvn = 0.0f;
-            bdet = 1.0f;
             sp = 0;
+            steps = 0;
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
 }
 
-// Any-hit traversal of one ray, resumable: returns once the ray is decided.
-// The answer does not depend on the visiting order (no pruning by a running
-// distance), so children are taken in stored order without sorting.
+// Any-hit traversal of one ray (inspection kernel).  The answer does not depend on the
+// visiting order (no pruning by a running distance); the order is the shadow kernel's:
+// first hit internal child entered, the others pushed in slot order.
 template <int L>
 __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3 o, V3 d, float tmin, float tmax,
                                          uint32_t* error) {
@@ -335,31 +415,38 @@ __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3
     ShortStack<L> st;
     int sp = 0;
     int node = 0;
-    for (;;) {
-        while (node >= 0) {
-            NodeBoxes nb = load_node(bvh.nodes, node);
-            bool h0, h1;
-            float tn0, tn1;
-            box2(nb, o, inv, tmin, tmax, &h0, &tn0, &h1, &tn1);
-            if (h0 && h1) {
-                if (!st.push(lds, sp, nb.ch.y)) { atomicOr(error, 1u); return false; }
-                node = nb.ch.x;
-            } else if (h0) {
-                node = nb.ch.x;
-            } else if (h1) {
-                node = nb.ch.y;
-            } else {
-                if (sp == 0) return false;
-                node = st.pop(lds, sp);
+    for (uint32_t steps = 1;; ++steps) {
+        if (steps > bvh.num_tris) {   // corrupt links: end the ray instead of looping
+            atomicOr(error, 8u);
+            return false;
+        }
+        const Node4 nb = load_node(bvh.nodes, node);
+        int next = kDone;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float tn;
+            if (!box_k(nb, k, o, inv, tmin, tmax, &tn)) continue;
+            const int c = comp(nb.ch, k);
+            if (c < 0) {
+                float t, un, vn, det;
+                if (tri_test(bvh.leaf_geo, (uint32_t)~c, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
+                    return true;
+            } else if (next == kDone) {
+                next = c;
+            } else if (ok) {
+                ok = st.push(lds, sp, c);
             }
         }
-        while (node < 0) {
-            float t, un, vn, det;
-            if (tri_test(bvh.leaf_geo, (uint32_t)~node, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
-                return true;
-            if (sp == 0) return false;
-            node = st.pop(lds, sp);
+        if (!ok) {
+            atomicOr(error, 1u);
+            return false;
         }
+        if (next == kDone) {
+            if (sp == 0) return false;
+            next = st.pop(lds, sp);
+        }
+        node = next;
     }
 }
 
@@ -382,60 +469,103 @@ __device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bo
 }
 
 template <int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_shadow(ShadowArgs a) {
+__global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     __shared__ int lds[L][256];
+    __shared__ uint32_t s_qleaf[4][kLeafQueue];
+    __shared__ uint8_t s_qown[4][kLeafQueue];
+    __shared__ uint32_t s_occ[4][64];
     const uint32_t count = *a.count_in;
     const float tmin = kTMin, tmax = kShadowTMax;
     if (a.bvh.num_tris == 0) {
         for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) apply_shadow(a, i, false);
         return;
     }
-    // Same iteration as the closest-hit kernel: box tests and the next node, refill,
-    // the single node load, then the leaf tests and ray set-up while it is in flight.
-    // A ray found occluded by its leaf tests ends after the load was issued for it;
-    // its lane takes a new ray at the next iteration.
+    // Same iteration as the closest-hit kernel: box steps queue their leaves, the queue
+    // is flushed when full, before a refill or when no lane is busy, and parked lanes
+    // finish once their leaves are tested.  The first hit internal child is entered and
+    // the others pushed in slot order (the answer is order-independent).  A flush that
+    // finds a ray occluded ends it at once, busy or parked.
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t* qleaf = s_qleaf[w];
+    uint8_t* qown = s_qown[w];
+    uint32_t* occf = s_occ[w];
+    occf[lane] = 0u;
+    uint32_t qn = 0;
     uint32_t ray = kNoRay;
-    bool exhausted = false;
-    WorkCursor wc = work_cursor(count, a.parts, a.refill);
+    bool parked = false, exhausted = false;
+    WorkCursor wc = work_cursor(count, a.parts);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     ShortStack<L> st;
     int sp = 0;
-    NodeBoxes cur = load_node(a.bvh.nodes, 0);
+    uint32_t steps = 0;
+    Node4 cur = load_node(a.bvh.nodes, 0);
     for (;;) {
-        const bool active = ray != kNoRay;
         int next = kDone;
-        int c0, c1;              // set, with the leaf records, where l0 / l1 are set
-        bool l0 = false, l1 = false;
-        LeafRegs g0, g1;
-        if (active) {
-            // node-centric any-hit step: the next internal node is chosen from the box
-            // tests (child 0 first; the answer is order-independent)
-            const NodeBoxes nb = cur;
-            consume_pad(nb);
-            bool h0, h1;
-            float tn0, tn1;
-            box2(nb, o, inv, tmin, tmax, &h0, &tn0, &h1, &tn1);
-            c0 = nb.ch.x;
-            c1 = nb.ch.y;
-            l0 = h0 && c0 < 0;
-            l1 = h1 && c1 < 0;
-            if (l0) g0 = load_leaf(a.bvh.leaf_geo, (uint32_t)~c0);
-            if (l1) g1 = load_leaf(a.bvh.leaf_geo, (uint32_t)~c1);
-            const bool i0 = h0 && c0 >= 0, i1 = h1 && c1 >= 0;
-            if (i0 && i1) {
-                if (!st.push(lds, sp, c1)) atomicOr(a.error, 1u);
-                else next = c0;
-            } else if (i0) {
-                next = c0;
-            } else if (i1) {
-                next = c1;
-            } else if (sp > 0) {
+        uint32_t lm = 0;
+        int4 ch = make_int4(0, 0, 0, 0);
+        if (ray != kNoRay && !parked) {
+            const Node4 nb = cur;
+            ch = nb.ch;
+            bool h[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float tn;
+                h[k] = box_k(nb, k, o, inv, tmin, tmax, &tn);
+            }
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = comp(ch, k);
+                if (!h[k]) continue;
+                if (c < 0) lm |= 1u << k;
+                else if (next == kDone) next = c;
+                else if (ok) ok = st.push(lds, sp, c);
+            }
+            if (!ok) {
+                atomicOr(a.error, 1u);
+                next = kDone;
+            } else if (next == kDone && sp > 0) {
                 next = st.pop(lds, sp);
             }
+            if (++steps > a.bvh.num_tris) {   // corrupt links: end the ray instead of looping
+                atomicOr(a.error, 8u);
+                next = kDone;
+            }
+            parked = next == kDone;
         }
-        const bool done = active && next == kDone;
-        const uint32_t got = fetch_ray(a.work, count, (!active || done) && !exhausted, &exhausted,
-                                       __ballot(active && !done) != 0ull, wc);
+        qn = append_leaves(qleaf, qown, nullptr, qn, lm, ch, make_float4(0.f, 0.f, 0.f, 0.f), lane);
+        const bool busy = ray != kNoRay && !parked;
+        const uint64_t wm = __ballot((ray == kNoRay || parked) && !exhausted);
+        const bool any_busy = __ballot(busy) != 0ull;
+        const bool refill_now = wm != 0ull && (!any_busy || (uint32_t)__popcll(wm) >= a.refill);
+        bool occ = false;
+        if (qn != 0u && (qn >= 64u || refill_now || !any_busy)) {
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t b = 0; b < qn; b += 64u) {
+                const uint32_t e = b + lane;
+                const bool valid = e < qn;
+                const uint32_t leaf = valid ? qleaf[e] : 0u;
+                const int owner = valid ? (int)qown[e] : (int)lane;
+                const V3 oo = v3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
+                const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
+                if (valid) {
+                    float t, un, vn, det;
+                    if (tri_test(a.bvh.leaf_geo, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
+                        occf[owner] = 1u;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            qn = 0;
+            occ = occf[lane] != 0u;
+            if (occ) occf[lane] = 0u;
+        }
+        if (ray != kNoRay && (occ || (parked && qn == 0u))) {
+            apply_shadow(a, ray, occ);
+            ray = kNoRay;
+            parked = false;
+        }
+        uint32_t got = kNoRay;
+        if (refill_now) got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted, wc);
         const bool fresh = got != kNoRay;
         V3 no = o, nd = d;
         if (fresh) {
@@ -444,25 +574,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             next = 0;
         }
         cur = load_node(a.bvh.nodes, next != kDone ? next : 0);
-        bool occ = false;
-        if (active) {
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (!(c == 0 ? l0 : l1) || occ) continue;
-                float t, un, vn, det;
-                if (tri_test(c == 0 ? g0 : g1, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax) occ = true;
-            }
-        }
-        if (done || occ) {
-            apply_shadow(a, ray, occ);
-            ray = kNoRay;
-        }
         if (fresh) {
             ray = got;
             o = no;
             d = nd;
             inv = safe_inv(nd);
             sp = 0;
+            steps = 0;
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }

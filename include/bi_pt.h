@@ -192,6 +192,11 @@ bi_pt_status bi_pt_get_stats(const bi_pt_ctx* ctx, bi_pt_stats* out);
  * parents[max(n-1,1)+n] (internal nodes, then leaves; root -1), centroid_bounds[6]. */
 bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* order, int32_t* children,
                                  float* child_bounds, int32_t* parents, float* centroid_bounds);
+/* Copy the BVH4 traversal nodes built from the last LBVH: for each BVH2 internal node
+ * k (max(n-1,1) of them), child4[4k..4k+3] (internal: node index, leaf j as ~j, empty
+ * slot 0) and box4[24k..24k+23] (lo.x[4], lo.y[4], lo.z[4], hi.x[4], hi.y[4], hi.z[4];
+ * an empty slot holds +inf). */
+bi_pt_status bi_pt_debug_get_bvh4(bi_pt_ctx* ctx, int32_t* child4, float* box4);
 uint32_t bi_pt_debug_num_tris(const bi_pt_ctx* ctx);
 /* Queue contents recorded by the last render with BI_PT_FLAG_RECORD_QUEUES:
  * kind 0 = extension queue entering segment `bounce` (path ids), kind 1 = shadow

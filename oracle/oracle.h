@@ -98,6 +98,11 @@ void ora_get_bvh(const ora_ctx* ctx, uint32_t* morton, uint32_t* order,
                  int32_t* children, float* child_bounds, int32_t* parents,
                  float* scene_centroid_bounds /* 6 */);
 
+/* BVH4 traversal nodes collapsed from the LBVH, one per internal BVH2 node k
+ * (max(n-1,1) of them): child4[4k+j] (internal: node index, leaf j as ~j, empty slot 0),
+ * box4[24k..]: lo.x[4], lo.y[4], lo.z[4], hi.x[4], hi.y[4], hi.z[4] (empty slot +inf). */
+void ora_get_bvh4(const ora_ctx* ctx, int32_t* child4, float* box4);
+
 /* Batched traversal of the FP32 BVH.  rays: 8 floats each (o.xyz, tmin, d.xyz, tmax).
  * closest: hit_t, hit_leaf (sorted position, 0xffffffff = miss), hit_u, hit_v.
  * anyhit (any_hit != 0): hit_leaf[i] = 1 if occluded else 0. */

@@ -64,6 +64,7 @@ def lib():
         L.ora_num_tris.argtypes = [C.c_void_p]
         L.ora_get_world.argtypes = [C.c_void_p, _f32p, _f32p, _u32p]
         L.ora_get_bvh.argtypes = [C.c_void_p, _u32p, _u32p, _i32p, _f32p, _i32p, _f32p]
+        L.ora_get_bvh4.argtypes = [C.c_void_p, _i32p, _f32p]
         L.ora_trace.argtypes = [C.c_void_p, C.c_uint32, _f32p, C.c_int, _f32p, _u32p, _f32p, _f32p]
         L.ora_trace_brute64.argtypes = [C.c_void_p, C.c_uint32, _f32p, _f64p, _u32p]
         L.ora_render32.restype = C.c_int
@@ -185,6 +186,13 @@ class OracleScene:
                           ptr(bounds, _f32p), ptr(parents, _i32p), ptr(cb, _f32p))
         return dict(morton=morton, order=order, children=children, bounds=bounds, parents=parents,
                     centroid_bounds=cb)
+
+    def bvh4(self):
+        ni = max(self.n - 1, 1)
+        child4 = np.zeros((ni, 4), np.int32)
+        box4 = np.zeros((ni, 24), np.float32)
+        lib().ora_get_bvh4(self.h, ptr(child4, _i32p), ptr(box4, _f32p))
+        return child4, box4
 
     def trace(self, rays, any_hit=False):
         rays = np.ascontiguousarray(rays, np.float32)

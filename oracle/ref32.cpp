@@ -135,6 +135,7 @@ constexpr float kTwoPi = 6.28318530717958647692f;
 constexpr float kInvTwoPi = 0.159154943091895335769f;
 constexpr float kTwoPiSq = 19.7392088021787172376f;
 constexpr float kTMin = 0.001f;
+constexpr float kShadowTMax = 1.70141173e38f;   // half of FLT_MAX: tmax * kRobust stays finite
 constexpr float kRobust = 1.00000036f;   // 1 + 2*gamma(3), Ize 2013 robust slab test
 constexpr float kThroughputMin = 0.001f; // sample_secondary_ray.hlsl:24
 
@@ -322,6 +323,47 @@ void build_bvh(Scene& s) {
         }
         std::memcpy(&nb[6 * nd], box, sizeof box);
     }
+    // BVH4 collapse: every internal node k independently -- from its two children, the
+    // internal child with the largest box surface (ties: the lower slot) is replaced in
+    // place by its first child and its second child appended, until four slots are
+    // filled or only leaves remain.  Internal links keep BVH2 indices, so they name the
+    // BVH4 node collapsed from that node.
+    s.nodes4.assign(ni, Node4{});
+    for (uint32_t k = 0; k < ni; ++k) {
+        int id[4];
+        float bx[4][6];
+        int cnt = 2;
+        for (int c = 0; c < 2; ++c) {
+            id[c] = s.nodes[k].child[c];
+            std::memcpy(bx[c], s.nodes[k].box[c], sizeof bx[c]);
+        }
+        while (cnt < 4) {
+            int pick = -1;
+            float best = -1.0f;
+            for (int j = 0; j < cnt; ++j) {
+                if (id[j] < 0) continue;
+                const float dx = bx[j][3] - bx[j][0], dy = bx[j][4] - bx[j][1], dz = bx[j][5] - bx[j][2];
+                const float ar = (dx * dy + dy * dz) + dz * dx;
+                if (ar > best) { best = ar; pick = j; }
+            }
+            if (pick < 0) break;
+            const Node& m = s.nodes[id[pick]];
+            id[cnt] = m.child[1];
+            std::memcpy(bx[cnt], m.box[1], sizeof bx[cnt]);
+            id[pick] = m.child[0];
+            std::memcpy(bx[pick], m.box[0], sizeof bx[pick]);
+            ++cnt;
+        }
+        Node4& q = s.nodes4[k];
+        for (int j = 0; j < 4; ++j) {
+            const bool used = j < cnt;
+            for (int a = 0; a < 3; ++a) {
+                q.lo[a][j] = used ? bx[j][a] : INFINITY;
+                q.hi[a][j] = used ? bx[j][3 + a] : INFINITY;
+            }
+            q.child[j] = used ? id[j] : 0;
+        }
+    }
     // intersection records in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0
     s.leaf_v0.resize(3 * n); s.leaf_e1.resize(3 * n); s.leaf_e2.resize(3 * n);
     for (uint32_t k = 0; k < n; ++k) {
@@ -343,7 +385,7 @@ namespace {
 struct Ray { F3 o, d; float tmin, tmax; };
 struct Hit { float t; uint32_t leaf; float u, v; };
 constexpr uint32_t kMiss = 0xffffffffu;
-constexpr int kStackMax = 64;
+constexpr int kStackMax = 192;   // 3 pushes per BVH4 level over at most 63 levels
 
 inline F3 safe_inv(F3 d) {
     auto one = [](float x) {
@@ -389,10 +431,24 @@ inline bool tri_test(const Scene& s, uint32_t leaf, F3 o, F3 d, float* t_out, fl
     return true;
 }
 
-// Node-centric traversal: every step processes one internal node -- both child boxes
-// are tested against [tmin, tbest], hit leaf children are intersected at once (nearer
-// child first), then the nearer hit internal child is entered and the farther one
-// pushed.  The stack holds internal nodes only.  The HIP kernel follows the same order.
+// Slab test of BVH4 child j against [tmin, tmax]; returns the entry distance.
+inline bool box4_hit(const Node4& nd, int j, F3 o, F3 inv, float tmin, float tmax, float* tnear) {
+    const float b[6] = {nd.lo[0][j], nd.lo[1][j], nd.lo[2][j], nd.hi[0][j], nd.hi[1][j], nd.hi[2][j]};
+    return box_hit(b, o, inv, tmin, tmax, tnear);
+}
+
+// Node-centric BVH4 traversal: every step tests the four child boxes of one node
+// against [tmin, tbest], pushes the hit internal children other than the nearest
+// (ascending slot order; the nearest is the lowest slot among equal entry distances),
+// intersects the hit leaves and enters the nearest hit internal child, or pops.
+// A leaf's hit at distance t is a candidate only if the leaf's own box passes the slab
+// test up to t (entry distance <= t * kRobust); a candidate replaces the best if nearer
+// or, at equal distance, of lower leaf index.  Boxes nest and the slab arithmetic
+// rounds monotonically, so every candidate's ancestors pass for any running distance
+// >= t and every candidate is reached in any visiting order: the result is the minimum
+// of (t, leaf) over all candidates, whatever the order (the HIP kernel defers and
+// batches its leaf tests).  A ray enters each node at most once, so a step count above
+// the triangle count ends the ray (it cannot happen on a valid tree).
 Hit trace_closest(const Scene& s, const Ray& r) {
     Hit h{r.tmax, kMiss, 0.0f, 0.0f};
     if (s.num_tris == 0) return h;
@@ -401,76 +457,71 @@ Hit trace_closest(const Scene& s, const Ray& r) {
     int stack[kStackMax];
     int sp = 0;
     int node = 0;
-    for (;;) {
-        const Node& nd = s.nodes[node];
-        float tn0, tn1;
-        const bool h0 = box_hit(nd.box[0], r.o, inv, r.tmin, h.t, &tn0);
-        const bool h1 = box_hit(nd.box[1], r.o, inv, r.tmin, h.t, &tn1);
-        const int first = (h0 && h1) ? (tn0 <= tn1 ? 0 : 1) : (h0 ? 0 : 1);
-        for (int k = 0; k < 2; ++k) {
-            const int c = k == 0 ? first : 1 - first;
-            const bool hit = c == 0 ? h0 : h1;
-            const int ch = nd.child[c];
-            if (!hit || ch >= 0) continue;
-            const uint32_t leaf = (uint32_t)~ch;
+    for (uint32_t steps = 1;; ++steps) {
+        const Node4& nd = s.nodes4[node];
+        float tn[4];
+        bool hit[4];
+        for (int j = 0; j < 4; ++j) hit[j] = box4_hit(nd, j, r.o, inv, r.tmin, h.t, &tn[j]);
+        int nearest = -1;
+        for (int j = 0; j < 4; ++j)
+            if (hit[j] && nd.child[j] >= 0 && (nearest < 0 || tn[j] < tn[nearest])) nearest = j;
+        bool ok = true;
+        for (int j = 0; j < 4; ++j) {
+            if (!ok || j == nearest || !hit[j] || nd.child[j] < 0) continue;
+            if (sp >= kStackMax) ok = false;
+            else stack[sp++] = nd.child[j];
+        }
+        for (int j = 0; j < 4; ++j) {
+            if (!hit[j] || nd.child[j] >= 0) continue;
+            const uint32_t leaf = (uint32_t)~nd.child[j];
             float t, un, vn, det;
-            if (tri_test(s, leaf, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin &&
+            if (tri_test(s, leaf, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin && tn[j] <= t * kRobust &&
                 (t < h.t || (t == h.t && leaf < h.leaf))) {
                 h.t = t; h.leaf = leaf; best_un = un; best_vn = vn; best_det = det;
             }
         }
-        const bool i0 = h0 && nd.child[0] >= 0, i1 = h1 && nd.child[1] >= 0;
-        if (i0 && i1) {
-            if (sp >= kStackMax) break;  // cannot happen for n < 2^34 (depth bound)
-            stack[sp++] = first == 0 ? nd.child[1] : nd.child[0];
-            node = first == 0 ? nd.child[0] : nd.child[1];
-        } else if (i0) {
-            node = nd.child[0];
-        } else if (i1) {
-            node = nd.child[1];
-        } else {
-            if (sp == 0) break;
-            node = stack[--sp];
-        }
+        if (!ok || steps > s.num_tris) break;
+        if (nearest >= 0) node = nd.child[nearest];
+        else if (sp > 0) node = stack[--sp];
+        else break;
     }
     if (h.leaf != kMiss) { h.u = best_un / best_det; h.v = best_vn / best_det; }
     return h;
 }
 
+// Any-hit BVH4 traversal in the shadow kernel's order: the first hit internal child
+// (lowest slot) is entered and the other hit internal children pushed in slot order,
+// the hit leaves intersected in slot order.  The answer does not depend on the order.
 bool trace_any(const Scene& s, const Ray& r) {
     if (s.num_tris == 0) return false;
     F3 inv = safe_inv(r.d);
     int stack[kStackMax];
     int sp = 0;
     int node = 0;
-    for (;;) {
-        if (node < 0) {
+    for (uint32_t steps = 1;; ++steps) {
+        const Node4& nd = s.nodes4[node];
+        int next = -1;
+        bool ok = true;
+        bool hit[4];
+        for (int j = 0; j < 4; ++j) {
+            float tn;
+            hit[j] = box4_hit(nd, j, r.o, inv, r.tmin, r.tmax, &tn);
+            if (!hit[j] || nd.child[j] < 0) continue;
+            if (next < 0) next = nd.child[j];
+            else if (!ok) continue;
+            else if (sp >= kStackMax) ok = false;
+            else stack[sp++] = nd.child[j];
+        }
+        for (int j = 0; j < 4; ++j) {
+            if (!hit[j] || nd.child[j] >= 0) continue;
             float t, un, vn, det;
-            if (tri_test(s, (uint32_t)~node, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin &&
-                t <= r.tmax)
+            if (tri_test(s, (uint32_t)~nd.child[j], r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin && t <= r.tmax)
                 return true;
-            if (sp == 0) return false;
-            node = stack[--sp];
-            continue;
         }
-        const Node& nd = s.nodes[node];
-        float tn0, tn1;
-        bool h0 = box_hit(nd.box[0], r.o, inv, r.tmin, r.tmax, &tn0);
-        bool h1 = box_hit(nd.box[1], r.o, inv, r.tmin, r.tmax, &tn1);
-        if (h0 && h1) {
-            int nearc = tn0 <= tn1 ? nd.child[0] : nd.child[1];
-            int farc = tn0 <= tn1 ? nd.child[1] : nd.child[0];
-            if (sp >= kStackMax) return false;
-            stack[sp++] = farc;
-            node = nearc;
-        } else if (h0) {
-            node = nd.child[0];
-        } else if (h1) {
-            node = nd.child[1];
-        } else {
-            if (sp == 0) return false;
-            node = stack[--sp];
-        }
+        if (!ok || steps > s.num_tris) return false;
+        if (next >= 0) node = next;
+        else if (sp > 0) node = stack[--sp];
+        else return false;
     }
 }
 
@@ -765,7 +816,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
             if (sm_index < 0) { acc_dir[l] = acc_dir[l] + c; continue; }
             out.shadow_bits[b] |= (uint8_t)(1u << l);
             out.n_light_shadow++;
-            Ray sr{P, ldir, kTMin, FLT_MAX};
+            Ray sr{P, ldir, kTMin, kShadowTMax};
             if (!trace_any(s, sr)) {
                 acc_dir[l] = acc_dir[l] + c;
             } else {
@@ -791,7 +842,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
                     if (any_pos(c) && finite3(c)) {
                         out.shadow_bits[b] |= 0x80u;
                         out.n_sky_shadow++;
-                        Ray sr{P, wi, kTMin, FLT_MAX};
+                        Ray sr{P, wi, kTMin, kShadowTMax};
                         if (!trace_any(s, sr)) acc_sky = acc_sky + c;
                     }
                 }

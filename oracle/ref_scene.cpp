@@ -139,4 +139,16 @@ void ora_get_bvh(const ora_ctx* ctx, uint32_t* morton, uint32_t* order, int32_t*
     if (scene_centroid_bounds) std::memcpy(scene_centroid_bounds, s.cbounds, 6 * sizeof(float));
 }
 
+void ora_get_bvh4(const ora_ctx* ctx, int32_t* child4, float* box4) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    for (size_t i = 0; i < s.nodes4.size(); ++i) {
+        const Node4& q = s.nodes4[i];
+        std::memcpy(&child4[4 * i], q.child, 4 * sizeof(int32_t));
+        for (int a = 0; a < 3; ++a) {
+            std::memcpy(&box4[24 * i + 4 * a], q.lo[a], 4 * sizeof(float));
+            std::memcpy(&box4[24 * i + 12 + 4 * a], q.hi[a], 4 * sizeof(float));
+        }
+    }
+}
+
 }  // extern "C"

@@ -12,6 +12,15 @@ struct Node {
     float box[2][6] = {};        // per child: lo.xyz, hi.xyz
 };
 
+// BVH4 traversal node collapsed from BVH2 node k (stored at index k): child j's box is
+// (lo[0][j], lo[1][j], lo[2][j]) - (hi[0][j], hi[1][j], hi[2][j]); an empty slot has the
+// box (+inf)^3 - (+inf)^3 and child 0.
+struct Node4 {
+    float lo[3][4];
+    float hi[3][4];
+    int32_t child[4];
+};
+
 struct Scene {
     uint32_t num_tris = 0;
     std::vector<float> tri_v;      // 9 per triangle, world space, instance order
@@ -35,6 +44,7 @@ struct Scene {
     float cbounds[6] = {};
     std::vector<uint32_t> morton, order, leaf_tri;
     std::vector<Node> nodes;
+    std::vector<Node4> nodes4;
     std::vector<int32_t> parent;
     std::vector<float> leaf_v0, leaf_e1, leaf_e2;
     // FP64 oracle BVH (built lazily by ref64.cpp)

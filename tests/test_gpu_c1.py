@@ -5,7 +5,7 @@ import pytest
 import bi_pt
 import oracle
 import scenes
-from test_gpu_parity import compare_bvh, radiance_check, random_rays, expected_queues
+from test_gpu_parity import compare_bvh, compare_bvh4, radiance_check, random_rays, expected_queues
 
 pytestmark = pytest.mark.gpu
 
@@ -28,6 +28,7 @@ def test_c1_bvh_bit_exact(c1, c1_gpu):
     print("case: c1 bvh 262144 tris", flush=True)
     assert c1_gpu.num_tris() == 262144
     compare_bvh(c1_gpu.bvh(), c1[1].bvh())
+    compare_bvh4(c1_gpu, c1[1])
 
 
 def test_c1_trace_bit_exact(c1, c1_gpu):

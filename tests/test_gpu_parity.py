@@ -40,9 +40,17 @@ def compare_bvh(g, o):
     assert np.array_equal(g["centroid_bounds"], o["centroid_bounds"])
 
 
+def compare_bvh4(gpu, ora):
+    gc, gb = gpu.bvh4()
+    oc, ob = ora.bvh4()
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gb.view(np.uint32), ob.view(np.uint32))
+
+
 def test_bvh_bit_exact_c0(c0_gpu, c0_oracle):
     print("case: c0 bvh", flush=True)
     compare_bvh(c0_gpu.bvh(), c0_oracle.bvh())
+    compare_bvh4(c0_gpu, c0_oracle)
 
 
 def random_rays(rng, n, centre, radius, tmax=1000.0):
@@ -69,6 +77,27 @@ def test_trace_bit_exact_c0(c0_gpu, c0_oracle):
     ga = c0_gpu.trace(rays, any_hit=True)[0]
     oa = c0_oracle.trace(rays, any_hit=True)[0]
     assert np.array_equal(ga, oa)
+
+
+def test_anyhit_longest_rays_all_octants(c0_gpu, c0_oracle):
+    """Longest shadow-ray extent (1.7e38) in every direction octant: empty BVH4 slots
+    are rejected and the GPU answers match the oracle bit for bit."""
+    print("case: c0 any-hit tmax 1.7e38, 8 octants", flush=True)
+    rng = np.random.default_rng(2)
+    rays = random_rays(rng, 16000, np.array([0.0, 2.0, 0.0]), 6.0, tmax=1.70141173e38)
+    signs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float32)
+    rays[:, 4:7] = np.abs(rays[:, 4:7]) * signs[np.arange(len(rays)) % 8]
+    ga = c0_gpu.trace(rays, any_hit=True)[0]
+    oa = c0_oracle.trace(rays, any_hit=True)[0]
+    assert np.array_equal(ga, oa)
+    g = c0_gpu.trace(rays)
+    o = c0_oracle.trace(rays)
+    for a, b in zip(g, o):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    too_long = rays[:4].copy()
+    too_long[:, 7] = np.float32(3.0e38)
+    with pytest.raises(bi_pt.PtError):
+        c0_gpu.trace(too_long, any_hit=True)
 
 
 def radiance_check(gs, os_):

@@ -4,7 +4,9 @@ The reference has no BVH code (the GPU driver builds it), so these checks pin th
 build's own LBVH against hand-computed values, a bit-by-bit Morton interleave,
 numpy's stable sort, a brute-force characterisation of the binary radix tree
 (every internal node splits its key range at the first differing key bit), exact
-box containment, and FP64 brute-force closest hits.
+box containment, the invariants of the greedy BVH4 collapse (same leaves and boxes
+as the BVH2 subtree, full nodes unless only leaves remain, no unexpanded child
+larger than an expanded one), and FP64 brute-force closest hits.
 """
 import numpy as np
 import pytest
@@ -117,6 +119,68 @@ def check_tree(bvh, tri_v):
             assert np.array_equal(box[:3], lo) and np.array_equal(box[3:], hi)
 
 
+def half_area32(b):
+    """Surface measure of boxes (lo.xyz, hi.xyz) in float32, as the collapse orders them."""
+    b = np.asarray(b, np.float32)
+    dx, dy, dz = b[..., 3] - b[..., 0], b[..., 4] - b[..., 1], b[..., 5] - b[..., 2]
+    return (dx * dy + dy * dz) + dz * dx
+
+
+def check_bvh4(bvh, child4, box4):
+    """BVH4 node k collapses BVH2 node k: checked against the BVH2 tree it came from."""
+    children, bounds, parents = bvh["children"], bvh["bounds"], bvh["parents"]
+    ni = len(children)
+    n = len(bvh["order"])
+
+    def box_of(c):
+        """Box of BVH2 child c (internal index or ~leaf) as stored in its parent."""
+        p = parents[c] if c >= 0 else parents[(n - 1 if n > 1 else 1) + ~c]
+        slot = 0 if children[p, 0] == c else 1
+        return bounds[p, 6 * slot:6 * slot + 6]
+
+    def leaves2(k):
+        out, st = [], [k]
+        while st:
+            x = st.pop()
+            for c in children[x]:
+                (out.append(~c) if c < 0 else st.append(c))
+        return sorted(out)
+
+    for k in range(ni):
+        ch = child4[k]
+        lo = box4[k, 0:12].reshape(3, 4).T
+        hi = box4[k, 12:24].reshape(3, 4).T
+        used = ~np.isinf(lo[:, 0])
+        cnt = int(used.sum())
+        assert used[:cnt].all() and not used[cnt:].any()
+        assert np.all(np.isinf(lo[cnt:])) and np.all(np.isinf(hi[cnt:])) and np.all(ch[cnt:] == 0)
+        assert 2 <= cnt <= 4
+        if cnt < 4:
+            assert np.all(ch[:cnt] < 0)      # stopped early only when every slot is a leaf
+        if n == 1:
+            continue
+        # same leaves as the BVH2 subtree; every slot's box is its BVH2 child box
+        got = []
+        for j in range(cnt):
+            c = int(ch[j])
+            got += [~c] if c < 0 else leaves2(c)
+            np.testing.assert_array_equal(np.concatenate([lo[j], hi[j]]).view(np.uint32), box_of(c).view(np.uint32))
+        assert sorted(got) == leaves2(k)
+        # greedy order: no internal slot larger than a BVH2 node that was expanded
+        expanded = set()
+        for j in range(cnt):
+            x = int(ch[j])
+            x = parents[x] if x >= 0 else parents[(n - 1) + ~x]
+            while x != k:
+                expanded.add(x)
+                x = parents[x]
+        if expanded:
+            exp_min = min(half_area32(box_of(e)) for e in expanded)
+            for j in range(cnt):
+                if ch[j] >= 0:
+                    assert half_area32(box_of(int(ch[j]))) <= exp_min
+
+
 def test_c0_morton_sort_tree(c0_oracle):
     bvh = c0_oracle.bvh()
     tri_v, _, _ = c0_oracle.world()
@@ -134,6 +198,7 @@ def test_c0_morton_sort_tree(c0_oracle):
     assert np.max(np.abs(q - q64)) <= 1
     assert np.mean(q == q64) > 0.999
     check_tree(bvh, tri_v)
+    check_bvh4(bvh, *c0_oracle.bvh4())
 
 
 def test_duplicate_codes_and_tiny_trees(oracle_lib):
@@ -144,10 +209,13 @@ def test_duplicate_codes_and_tiny_trees(oracle_lib):
     sc = oracle_lib.OracleScene(tiny_scene(np.concatenate(tris)))
     bvh = sc.bvh()
     check_tree(bvh, sc.world()[0])
-    for count in (1, 2, 3):
+    check_bvh4(bvh, *sc.bvh4())
+    for count in (1, 2, 3, 5, 9):
         sc = oracle_lib.OracleScene(tiny_scene(rng.uniform(-1, 1, size=(9 * count,))))
         bvh = sc.bvh()
-        check_tree(bvh, sc.world()[0])
+        if count <= 3:
+            check_tree(bvh, sc.world()[0])
+        check_bvh4(bvh, *sc.bvh4())
 
 
 def random_rays(rng, n, centre, radius, tmax=1000.0):
@@ -182,6 +250,91 @@ def test_traversal_matches_fp64_brute_force(c0_oracle):
     short[:, 7] = np.where(sel, t64 * 0.999, 1000.0)
     occ_short = c0_oracle.trace(short[sel], any_hit=True)[0]
     assert occ_short.mean() < 0.01
+
+
+def test_anyhit_longest_rays_all_octants(c0_oracle):
+    """Shadow rays reach t = 1.7e38 (half of FLT_MAX): tmax * kRobust stays finite,
+    so empty BVH4 slots (+inf boxes) are rejected for every direction octant."""
+    rng = np.random.default_rng(12)
+    rays = random_rays(rng, 4000, np.array([0.0, 2.0, 0.0]), 5.0, tmax=1.70141173e38)
+    signs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float32)
+    rays[:, 4:7] = np.abs(rays[:, 4:7]) * signs[np.arange(len(rays)) % 8]
+    occ = c0_oracle.trace(rays, any_hit=True)[0]
+    t64, tri64 = c0_oracle.trace_brute64(rays)
+    agree = occ.astype(bool) == (tri64 != 0xFFFFFFFF)
+    assert agree.mean() > 0.999
+    closest = c0_oracle.trace(rays)[1]
+    assert np.mean((closest != 0xFFFFFFFF) == occ.astype(bool)) > 0.999
+
+
+def brute_closest32(tri_sorted, rays, kRobust=np.float32(1.00000036)):
+    """Closest hit over every triangle with single-precision arithmetic written out
+    operation by operation: Moller-Trumbore with division-free acceptance, a candidate
+    only where the triangle's bounding box passes the robust slab test up to the hit
+    distance, minimum of (t, leaf).  No BVH: the traversal must find the same answer."""
+    f = np.float32
+    v = tri_sorted.reshape(-1, 3, 3).astype(f)
+    v0, e1, e2 = v[:, 0], v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+    lo = np.minimum(np.minimum(v[:, 0], v[:, 1]), v[:, 2]) + f(0)
+    hi = np.maximum(np.maximum(v[:, 0], v[:, 1]), v[:, 2]) + f(0)
+    out_t = np.zeros(len(rays), f)
+    out_leaf = np.full(len(rays), 0xFFFFFFFF, np.uint32)
+    for i, r in enumerate(rays.astype(f)):
+        o, tmin, d, tmax = r[0:3], r[3], r[4:7], r[7]
+        ad = np.abs(d)
+        dd = np.where(ad < f(1e-20), np.where(d < 0, f(-1e-20), f(1e-20)), d)
+        inv = (f(1) / dd).astype(f)
+        pv = np.stack([d[1] * e2[:, 2] - d[2] * e2[:, 1], d[2] * e2[:, 0] - d[0] * e2[:, 2],
+                       d[0] * e2[:, 1] - d[1] * e2[:, 0]], 1)
+        det = (e1[:, 0] * pv[:, 0] + e1[:, 1] * pv[:, 1]) + e1[:, 2] * pv[:, 2]
+        tv = o - v0
+        un = (tv[:, 0] * pv[:, 0] + tv[:, 1] * pv[:, 1]) + tv[:, 2] * pv[:, 2]
+        qv = np.stack([tv[:, 1] * e1[:, 2] - tv[:, 2] * e1[:, 1], tv[:, 2] * e1[:, 0] - tv[:, 0] * e1[:, 2],
+                       tv[:, 0] * e1[:, 1] - tv[:, 1] * e1[:, 0]], 1)
+        vn = (d[0] * qv[:, 0] + d[1] * qv[:, 1]) + d[2] * qv[:, 2]
+        pos = (det > 0) & ~((un < 0) | (vn < 0) | (un + vn > det))
+        neg = (det < 0) & ~((un > 0) | (vn > 0) | (un + vn < det))
+        tn = (e2[:, 0] * qv[:, 0] + e2[:, 1] * qv[:, 1]) + e2[:, 2] * qv[:, 2]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = tn / det
+        x0, x1 = (lo[:, 0] - o[0]) * inv[0], (hi[:, 0] - o[0]) * inv[0]
+        y0, y1 = (lo[:, 1] - o[1]) * inv[1], (hi[:, 1] - o[1]) * inv[1]
+        z0, z1 = (lo[:, 2] - o[2]) * inv[2], (hi[:, 2] - o[2]) * inv[2]
+        entry = np.maximum(np.maximum(np.minimum(x0, x1), np.minimum(y0, y1)), np.maximum(np.minimum(z0, z1), tmin))
+        ok = (pos | neg) & (t >= tmin) & (t <= tmax) & (entry <= t * kRobust)
+        if ok.any():
+            idx = np.nonzero(ok)[0]
+            best = idx[np.lexsort((idx, t[idx]))[0]]
+            out_t[i], out_leaf[i] = t[best], best
+        else:
+            out_t[i] = tmax
+    return out_t, out_leaf
+
+
+def test_closest_hit_is_order_independent(oracle_lib, c0_oracle):
+    """The BVH traversal returns exactly the brute-force minimum over all candidates --
+    on a tie-rich grid (rays aimed at shared vertices and edges) and on C0."""
+    grid = scenes.grid_quad((-2, 0, -2), (4, 0, 0), (0, 0, 4), 8, 8, (0, 1, 0))
+    tri = grid[0][grid[2]].reshape(-1)
+    sc = oracle_lib.OracleScene(tiny_scene(tri))
+    ts = sc.world()[0][sc.bvh()["order"]]
+    rng = np.random.default_rng(21)
+    pts = np.array([(-2 + 0.5 * i, 0.0, -2 + 0.5 * j) for i in range(9) for j in range(9)] +
+                   [(-2 + 0.25 + 0.5 * i, 0.0, -2 + 0.5 * j) for i in range(8) for j in range(9)], np.float32)
+    o = pts + rng.uniform(-1.5, 1.5, size=pts.shape).astype(np.float32) * np.array([1, 0, 1], np.float32) \
+        + np.array([0, 2, 0], np.float32)
+    d = pts - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((len(pts), 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, 0.001, d, 100.0
+    t, leaf, _, _ = sc.trace(rays)
+    bt, bleaf = brute_closest32(ts, rays)
+    assert np.array_equal(leaf, bleaf) and np.array_equal(t.view(np.uint32), bt.view(np.uint32))
+    assert np.mean(leaf != 0xFFFFFFFF) > 0.9
+    rays = random_rays(rng, 300, np.array([0.0, 2.0, 0.0]), 5.0)
+    t, leaf, _, _ = c0_oracle.trace(rays)
+    bt, bleaf = brute_closest32(c0_oracle.world()[0][c0_oracle.bvh()["order"]], rays)
+    assert np.array_equal(leaf, bleaf) and np.array_equal(t.view(np.uint32), bt.view(np.uint32))
 
 
 def test_traversal_empty_and_single(oracle_lib):
