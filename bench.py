@@ -26,6 +26,37 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_k_trace_closest_c1.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_trace_closest launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def reachable_bvh4_nodes(pt):
+    """BVH4 nodes reachable from the root (the nodes a traversal can read)."""
+    import numpy as np
+    child4, _ = pt.bvh4()
+    seen = np.zeros(len(child4), bool)
+    stack = [0]
+    while stack:
+        k = stack.pop()
+        if seen[k]:
+            continue
+        seen[k] = True
+        stack.extend(int(c) for c in child4[k] if c > 0)   # 0 = the root: only empty slots name it
+    return int(seen.sum())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,6 +69,8 @@ def main():
     ap.add_argument("--work-parts", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device-to-device copy first (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this pass runs on one GPU only; multi-GPU scaling is not part of this build "
@@ -55,6 +88,13 @@ def main():
     scene = cfg["scene"]()
     cam = cfg["camera"]()
     log(f"scene {scene.name}: {scene.num_tris} tris, generated in {time.perf_counter() - t:.1f}s")
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
+        del src, dst
     pt = bi_pt.PathTracer(0)
     pt.set_scene(scene)
     stream = torch.cuda.current_stream().cuda_stream
@@ -93,7 +133,8 @@ def main():
     step(5000, flags=bi_pt.FLAG_TIMING)
     s1 = pt.stats()
     n = pt.num_tris()
-    bvh_bytes = 64 * max(n - 1, 1) + 48 * n
+    nodes4 = reachable_bvh4_nodes(pt)
+    bvh_bytes = 128 * nodes4 + 48 * n   # reachable BVH4 nodes + leaf intersection records
     cam_rays = s1["rays_camera"] - s0["rays_camera"]
     ext_rays = s1["rays_extension"] - s0["rays_extension"]
     launches = s1["launches_trace_closest"]
@@ -140,7 +181,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "alg_bytes_per_launch": int(alg / launches),
             "avg_launch_ms": round(avg_launch_ms, 4),
             "launches": int(launches),

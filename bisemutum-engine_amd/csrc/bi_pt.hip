@@ -71,8 +71,9 @@ struct bi_pt_ctx {
     bool bvh_built = false;
     // wave buffers
     uint32_t wave_cap = 0, wave_slots = 0, wave_sh_mult = 0;
-    DevBuf<float> qa[6], qb[6], hit_f[3], ps_f[4], acc, sq_f[9];
-    DevBuf<uint32_t> qa_path, qb_path, hit_leaf, sq_pk;
+    DevBuf<float> qa[6], qb[6], ps_f[4], acc, sq_f[9];
+    DevBuf<uint32_t> qa_path, qb_path, sq_pk;
+    DevBuf<HitRec> hit_rec;
     DevBuf<float> st_f[9], st_c;      // shade staging: hit point, extension dir, sky dir; contributions
     DevBuf<uint32_t> st_path, st_mask;
     DevBuf<uint64_t> tile_sums;
@@ -249,8 +250,7 @@ bi_pt_status ensure_wave_buffers(bi_pt_ctx* ctx, uint32_t cap, uint32_t slots, u
     }
     TRY(dalloc(ctx, ctx->qa_path, cap));
     TRY(dalloc(ctx, ctx->qb_path, cap));
-    for (int k = 0; k < 3; ++k) TRY(dalloc(ctx, ctx->hit_f[k], cap));
-    TRY(dalloc(ctx, ctx->hit_leaf, cap));
+    TRY(dalloc(ctx, ctx->hit_rec, cap));
     for (int k = 0; k < 4; ++k) TRY(dalloc(ctx, ctx->ps_f[k], cap));
     TRY(dalloc(ctx, ctx->acc, (size_t)slots * 3u * cap));
     const size_t shcap = (size_t)cap * sh_mult;
@@ -313,10 +313,9 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes); dfree(ctx, ctx->nodes4);
     dfree(ctx, ctx->parent); dfree(ctx, ctx->leaf_geo); dfree(ctx, ctx->leaf_shade);
     for (int k = 0; k < 6; ++k) { dfree(ctx, ctx->qa[k]); dfree(ctx, ctx->qb[k]); }
-    for (int k = 0; k < 3; ++k) dfree(ctx, ctx->hit_f[k]);
     for (int k = 0; k < 4; ++k) dfree(ctx, ctx->ps_f[k]);
     for (int k = 0; k < 9; ++k) dfree(ctx, ctx->sq_f[k]);
-    dfree(ctx, ctx->qa_path); dfree(ctx, ctx->qb_path); dfree(ctx, ctx->hit_leaf); dfree(ctx, ctx->acc);
+    dfree(ctx, ctx->qa_path); dfree(ctx, ctx->qb_path); dfree(ctx, ctx->hit_rec); dfree(ctx, ctx->acc);
     dfree(ctx, ctx->sq_pk);
     for (auto& b : ctx->st_f) dfree(ctx, b);
     dfree(ctx, ctx->st_c); dfree(ctx, ctx->st_path); dfree(ctx, ctx->st_mask); dfree(ctx, ctx->tile_sums); dfree(ctx, ctx->tile_sky); dfree(ctx, ctx->counters); dfree(ctx, ctx->sums);
@@ -559,7 +558,7 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
 
     const BvhView bvh = bvh_view(ctx);
     PathState ps{ctx->ps_f[0].p, ctx->ps_f[1].p, ctx->ps_f[2].p, ctx->ps_f[3].p, ctx->acc.p, ctx->wave_cap};
-    HitBuf hits{ctx->hit_f[0].p, ctx->hit_f[1].p, ctx->hit_f[2].p, ctx->hit_leaf.p};
+    HitBuf hits{ctx->hit_rec.p};
     RayQueue qA = queue_of(ctx->qa, ctx->qa_path), qB = queue_of(ctx->qb, ctx->qb_path);
     ShadowQueue sq{ctx->sq_f[0].p, ctx->sq_f[1].p, ctx->sq_f[2].p, ctx->sq_f[3].p, ctx->sq_f[4].p,
                    ctx->sq_f[5].p, ctx->sq_f[6].p, ctx->sq_f[7].p, ctx->sq_f[8].p, ctx->sq_pk.p};
@@ -860,9 +859,9 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "debug trace needs tmin = 0.001 and one tmax");
     DevBuf<float> f[6];
     DevBuf<uint32_t> path, cnt, out;
-    DevBuf<float> ht[3];
+    DevBuf<HitRec> hrec;
     for (int k = 0; k < 6; ++k) TRY(dalloc(ctx, f[k], n));
-    for (int k = 0; k < 3; ++k) TRY(dalloc(ctx, ht[k], n));
+    TRY(dalloc(ctx, hrec, n));
     TRY(dalloc(ctx, path, n));
     TRY(dalloc(ctx, cnt, 4 + kWorkParts * kWorkStride));
     TRY(dalloc(ctx, out, n));
@@ -891,18 +890,24 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             ta.work = cnt.p + 4;   // zeroed part counters
             ta.parts = kWorkParts;
             ta.refill = 32;
-            ta.hits = HitBuf{ht[0].p, ht[1].p, ht[2].p, out.p};
+            ta.hits = HitBuf{hrec.p};
             ta.error = &ctx->counters.p[kCntError];
             launch_trace_closest(ta, n, false, 16, nullptr);
-            if (hipMemcpy(hit_leaf, out.p, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-                (hit_t && hipMemcpy(hit_t, ht[0].p, n * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
-                (hit_u && hipMemcpy(hit_u, ht[1].p, n * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
-                (hit_v && hipMemcpy(hit_v, ht[2].p, n * 4, hipMemcpyDeviceToHost) != hipSuccess))
+            std::vector<HitRec> h(n);
+            if (hipMemcpy(h.data(), hrec.p, n * sizeof(HitRec), hipMemcpyDeviceToHost) != hipSuccess) {
                 status = BI_PT_ERROR_HIP;
+            } else {
+                for (uint32_t i = 0; i < n; ++i) {
+                    hit_leaf[i] = h[i].leaf;
+                    if (hit_t) hit_t[i] = h[i].t;
+                    if (hit_u) hit_u[i] = h[i].u;
+                    if (hit_v) hit_v[i] = h[i].v;
+                }
+            }
         }
     }
     for (int k = 0; k < 6; ++k) dfree(ctx, f[k]);
-    for (int k = 0; k < 3; ++k) dfree(ctx, ht[k]);
+    dfree(ctx, hrec);
     dfree(ctx, path);
     dfree(ctx, cnt);
     dfree(ctx, out);

@@ -10,11 +10,11 @@
 //              index of the BVH2 node it collapses), leaf intersection records (48 B:
 //              v0, e1, e2) and leaf shading records (48 B: n0, n1, n2, material) in sorted
 //              leaf order
-//   per wave   SoA ray queues (o.xyz, d.xyz, path id), SoA hit records (t, leaf, u, v),
+//   per wave   SoA ray queues (o.xyz, d.xyz, path id), 16-B hit records (t, leaf, u, v),
 //              SoA path state (throughput, last BSDF pdf), per-kind radiance accumulators,
 //              SoA shadow queue (o.xyz, d.xyz, contribution.xyz, path<<3|kind)
 // Nodes are read whole by one lane (7 x 16-B loads); rays, hits and path state are
-// read by consecutive lanes at consecutive addresses (SoA, coalesced).
+// read by consecutive lanes at consecutive addresses (coalesced).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -120,9 +120,16 @@ struct RayQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz;
     uint32_t* path;
 };
+// Hit record of one ray, one 16-B store: a lane finishes its ray at its own time, so
+// the records of a wavefront land scattered; one record per ray keeps that to one
+// partial cache line instead of four.
+struct alignas(16) HitRec {
+    float t;
+    uint32_t leaf;
+    float u, v;
+};
 struct HitBuf {
-    float *t, *u, *v;
-    uint32_t* leaf;
+    HitRec* rec;
 };
 struct ShadowQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz, *cx, *cy, *cz;

@@ -78,7 +78,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             beta = v3(a.ps.bx[path], a.ps.by[path], a.ps.bz[path]);
             pdf_prev = a.ps.pdf[path];
         }
-        const uint32_t leaf = a.hits.leaf[i];
+        const HitRec hit = a.hits.rec[i];
+        const uint32_t leaf = hit.leaf;
         const bool want_depth = primary && a.depth_out != nullptr && sample == a.depth_sample;
         if (leaf == kLeafMiss) {
             V3 ls = sky_eval(a.sky, d);
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             a.ps.acc[2 * cap + path] = c.z;
             if (want_depth) a.depth_out[pixel] = 0.0f;
         } else {
-            const float t = a.hits.t[i], hu = a.hits.u[i], hv = a.hits.v[i];
+            const float t = hit.t, hu = hit.u, hv = hit.v;
             P = o + d * t;
             const float4* lsp = reinterpret_cast<const float4*>(a.leaf_shade + leaf);
             const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];

@@ -239,10 +239,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     const float tmin = kTMin;
     if (a.bvh.num_tris == 0) {
         for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
-            a.hits.t[i] = a.wave.ray_length;
-            a.hits.leaf[i] = kLeafMiss;
-            a.hits.u[i] = 0.0f;
-            a.hits.v[i] = 0.0f;
+            a.hits.rec[i] = HitRec{a.wave.ray_length, kLeafMiss, 0.0f, 0.0f};
         }
         return;
     }
@@ -361,10 +358,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
                 u = un / det;
                 v = vn / det;
             }
-            a.hits.t[ray] = tbest;
-            a.hits.leaf[ray] = best;
-            a.hits.u[ray] = u;
-            a.hits.v[ray] = v;
+            a.hits.rec[ray] = HitRec{tbest, best, u, v};
             ray = kNoRay;
             parked = false;
         }
