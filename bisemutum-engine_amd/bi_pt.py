@@ -20,6 +20,7 @@ FLAG_JITTER = 1 << 0
 FLAG_NO_SKY_NEE = 1 << 1
 FLAG_RECORD_QUEUES = 1 << 8
 FLAG_TIMING = 1 << 9
+FLAG_REUSE_BVH = 1 << 10
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_SCENE", 5: "DEVICE_CHECK",
           6: "UNSUPPORTED"}
@@ -28,7 +29,9 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
            "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
-           "bi_pt_debug_trace"]
+           "bi_pt_debug_trace", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
+           "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
+           "bi_pt_batch_autotune"]
 
 
 class PtError(RuntimeError):
@@ -59,6 +62,19 @@ class Settings(C.Structure):
     _fields_ = [("ray_length", C.c_float), ("max_bounces", C.c_uint32), ("accumulate", C.c_uint32),
                 ("spp", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("wave_paths", C.c_uint32),
                 ("stack_lds", C.c_uint32), ("refill_lanes", C.c_uint32), ("work_parts", C.c_uint32)]
+
+
+class JobC(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("camera", C.POINTER(CameraC)), ("settings", C.POINTER(Settings)),
+                ("out_rgba32f", C.c_void_p), ("out_depth", C.c_void_p)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("wave_paths", C.c_uint32), ("stack_lds", C.c_uint32), ("refill_lanes", C.c_uint32),
+                ("work_parts", C.c_uint32), ("use_graph", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Stats(C.Structure):
@@ -102,8 +118,17 @@ def lib():
         L.bi_pt_debug_queue.argtypes = [vp, u32, u32, u32, u32p, u32, u32p]
         L.bi_pt_debug_get_sums.argtypes = [vp, f32p]
         L.bi_pt_debug_trace.argtypes = [vp, u32, f32p, C.c_int, f32p, u32p, f32p, f32p]
+        L.bi_pt_batch_create.argtypes = [C.POINTER(DeviceDesc), C.POINTER(vp)]
+        L.bi_pt_batch_destroy.argtypes = [vp]
+        L.bi_pt_batch_last_error.argtypes = [vp]
+        L.bi_pt_batch_last_error.restype = C.c_char_p
+        L.bi_pt_batch_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.bi_pt_batch_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.bi_pt_render_batch.argtypes = [vp, C.POINTER(JobC), u32, vp]
+        L.bi_pt_batch_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.bi_pt_batch_autotune.argtypes = [vp, C.POINTER(JobC), u32, u32, vp, C.POINTER(Tuning)]
         for name in EXPORTS:
-            if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris"):
+            if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris", "bi_pt_batch_last_error"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -275,3 +300,92 @@ class PathTracer:
         self._check(lib().bi_pt_debug_trace(self.h, n, _p(rays), int(any_hit), _p(t), _p(leaf, C.c_uint32),
                                             _p(u), _p(v)))
         return (leaf,) if any_hit else (t, leaf, u, v)
+
+
+class Job:
+    """One render job of a batch: a PathTracer (its scene, lights, sky and running
+    image), a camera and the settings of bi_pt_render."""
+
+    def __init__(self, tracer, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
+                 depth=False):
+        self.tracer = tracer
+        self.cam = cam
+        self.settings = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, 0, 0, 0, 0)
+        self.depth = depth
+
+    def _buffers(self):
+        t = self.tracer
+        torch = t.torch
+        npix = self.cam.width * self.cam.height
+        dev = torch.device("cuda", t.device)
+        if t._out is None or t._out.numel() != npix * 4:
+            t._out = torch.empty(npix * 4, dtype=torch.float32, device=dev)
+        if self.depth and (t._depth is None or t._depth.numel() != npix):
+            t._depth = torch.empty(npix, dtype=torch.float32, device=dev)
+        return t._out, (t._depth if self.depth else None)
+
+
+class Batch:
+    """bi_pt_batch: renders the jobs of several PathTracers in shared waves."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("bi_pt needs a ROCm GPU; none is visible")
+        self.device = device
+        h = C.c_void_p()
+        st = lib().bi_pt_batch_create(C.byref(DeviceDesc(device, 0, hbm_budget_bytes)), C.byref(h))
+        if st != 0:
+            raise PtError(st, "bi_pt_batch_create failed")
+        self.h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise PtError(st, lib().bi_pt_batch_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().bi_pt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _jobs(self, jobs):
+        arr = (JobC * len(jobs))()
+        keep = []
+        for i, j in enumerate(jobs):
+            out, depth = j._buffers()
+            cam = camera_struct(j.cam)
+            keep.append(cam)
+            arr[i] = JobC(j.tracer.h.value, C.pointer(cam), C.pointer(j.settings), out.data_ptr(),
+                          depth.data_ptr() if depth is not None else None)
+        return arr, keep
+
+    def render(self, jobs, stream=None):
+        arr, keep = self._jobs(jobs)
+        self._check(lib().bi_pt_render_batch(self.h, arr, len(jobs), C.c_void_p(stream) if stream else None))
+
+    def set_tuning(self, wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, use_graph=False):
+        t = Tuning(wave_paths, stack_lds, refill_lanes, work_parts, 1 if use_graph else 0)
+        self._check(lib().bi_pt_batch_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().bi_pt_batch_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, jobs, rounds=1, stream=None):
+        arr, keep = self._jobs(jobs)
+        t = Tuning()
+        self._check(lib().bi_pt_batch_autotune(self.h, arr, len(jobs), rounds,
+                                               C.c_void_p(stream) if stream else None, C.byref(t)))
+        return t.as_dict()
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().bi_pt_batch_get_stats(self.h, C.byref(s)))
+        return s.as_dict()

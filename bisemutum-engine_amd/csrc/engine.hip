@@ -1,0 +1,570 @@
+// engine.hip -- the wave engine: plans the waves of a call (one or several render
+// jobs), sizes the queues, and issues the per-segment kernels on one stream; the batch
+// object (bi_pt_render_batch) with its hipGraph capture of a whole call and the
+// autotuner of the engine's knobs.
+//
+// Frame structure (PathTracingPass::render, src/renderer/pass/path_tracing.cpp:224-488,
+// re-designed as compacted waves instead of full-screen passes):
+//   LBVH build of every job's scene (every call unless the caller keeps a built one:
+//   the reference rebuilds the TLAS every frame, accel.cpp:156-158)
+//   for each wave (whole samples of whole images, possibly of several jobs):
+//     segment 0:   trace camera rays (generated in the kernel) -> shade -> shadow rays
+//     segment b:   trace extension queue -> shade -> shadow rays
+//     resolve:     per-pixel running sums += radiance of each sample, in sample order
+//   output:        running mean -> RGBA32F (pt_accumulate.hlsl:9 as sum / count)
+// A job's samples are visited in increasing order across waves and each path's
+// arithmetic depends only on (seed, pixel, sample, bounce) and its own scene, so a job
+// renders bit for bit the same alone, in any wave split, or batched with other jobs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pt_host.hpp"
+
+namespace bipt {
+
+bi_pt_status engine_init(Owner* own, Engine& eng) {
+    if (eng.ready) return BI_PT_OK;
+    BI_TRY(dalloc(own, eng.counters, kCntSlots));
+    BI_HIP_TRY(own, hipMemset(eng.counters.p, 0, kCntSlots * sizeof(uint32_t)));
+    BI_TRY(dalloc(own, eng.empty_tree, 1));
+    Bvh4Node e{};
+    for (int j = 0; j < 4; ++j) {
+        e.lox[j] = e.loy[j] = e.loz[j] = e.hix[j] = e.hiy[j] = e.hiz[j] = INFINITY;
+        e.child[j] = 0;
+        e.pad[j] = 0;
+    }
+    BI_HIP_TRY(own, hipMemcpy(eng.empty_tree.p, &e, sizeof e, hipMemcpyHostToDevice));
+    for (auto& ev : eng.ev) BI_HIP_TRY(own, hipEventCreate(&ev));
+    BI_HIP_TRY(own, hipEventCreate(&eng.plan_ev));
+    eng.ready = true;
+    return BI_PT_OK;
+}
+
+void engine_free(Owner* own, Engine& eng) {
+    for (int k = 0; k < 6; ++k) { dfree(own, eng.qa[k]); dfree(own, eng.qb[k]); }
+    for (int k = 0; k < 4; ++k) dfree(own, eng.ps_f[k]);
+    for (int k = 0; k < 9; ++k) { dfree(own, eng.sq_f[k]); dfree(own, eng.st_f[k]); }
+    dfree(own, eng.acc); dfree(own, eng.st_c);
+    dfree(own, eng.qa_path); dfree(own, eng.qb_path); dfree(own, eng.sq_pk); dfree(own, eng.st_path);
+    dfree(own, eng.st_mask); dfree(own, eng.tile_sky); dfree(own, eng.hit_rec); dfree(own, eng.tile_sums);
+    dfree(own, eng.counters); dfree(own, eng.plan); dfree(own, eng.empty_tree);
+    if (eng.ready) {
+        for (auto& ev : eng.ev) (void)hipEventDestroy(ev);
+        (void)hipEventDestroy(eng.plan_ev);
+    }
+    if (eng.plan_host) (void)hipHostFree(eng.plan_host);
+    eng.plan_host = nullptr;
+    eng.plan_host_n = 0;
+    eng.ready = false;
+}
+
+bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out) {
+    BI_HIP_TRY(own, hipDeviceSynchronize());
+    uint32_t hc[kCntSlots];
+    BI_HIP_TRY(own, hipMemcpy(hc, eng.counters.p, sizeof hc, hipMemcpyDeviceToHost));
+    uint64_t r[4];
+    std::memcpy(r, &hc[kCntRays], sizeof r);
+    eng.stats.rays_extension = r[1];
+    eng.stats.rays_light_shadow = r[2];
+    eng.stats.rays_sky_shadow = r[3];
+    if (out) *out = eng.stats;
+    if (hc[kCntError])
+        return fail(own, BI_PT_ERROR_DEVICE_CHECK,
+                    std::string("device check failed: ") + ((hc[kCntError] & 1u) ? "traversal stack overflow " : "") +
+                        ((hc[kCntError] & 4u) ? "refit wait limit " : "") +
+                        ((hc[kCntError] & 8u) ? "traversal step bound exceeded" : ""));
+    return BI_PT_OK;
+}
+
+namespace {
+
+bi_pt_status ensure_wave_buffers(Owner* own, Engine& eng, uint32_t cap, uint32_t slots, uint32_t sh_mult) {
+    if (cap <= eng.wave_cap && slots <= eng.wave_slots && sh_mult <= eng.wave_sh_mult) return BI_PT_OK;
+    cap = std::max(cap, eng.wave_cap);
+    slots = std::max(slots, eng.wave_slots);
+    sh_mult = std::max(sh_mult, eng.wave_sh_mult);
+    for (int k = 0; k < 6; ++k) {
+        BI_TRY(dalloc(own, eng.qa[k], cap));
+        BI_TRY(dalloc(own, eng.qb[k], cap));
+    }
+    BI_TRY(dalloc(own, eng.qa_path, cap));
+    BI_TRY(dalloc(own, eng.qb_path, cap));
+    BI_TRY(dalloc(own, eng.hit_rec, cap));
+    for (int k = 0; k < 4; ++k) BI_TRY(dalloc(own, eng.ps_f[k], cap));
+    BI_TRY(dalloc(own, eng.acc, (size_t)slots * 3u * cap));
+    const size_t shcap = (size_t)cap * sh_mult;
+    for (int k = 0; k < 9; ++k) BI_TRY(dalloc(own, eng.sq_f[k], shcap));
+    BI_TRY(dalloc(own, eng.sq_pk, shcap));
+    for (int k = 0; k < 9; ++k) BI_TRY(dalloc(own, eng.st_f[k], cap));
+    BI_TRY(dalloc(own, eng.st_c, (size_t)3 * (slots - 1u) * cap));
+    BI_TRY(dalloc(own, eng.st_path, cap));
+    BI_TRY(dalloc(own, eng.st_mask, cap));
+    BI_TRY(dalloc(own, eng.tile_sums, (cap + 255u) / 256u + 1u));
+    BI_TRY(dalloc(own, eng.tile_sky, (cap + 255u) / 256u + 1u));
+    eng.wave_cap = cap;
+    eng.wave_slots = slots;
+    eng.wave_sh_mult = sh_mult;
+    return BI_PT_OK;
+}
+
+RayQueue queue_of(DevBuf<float>* f, DevBuf<uint32_t>& path) {
+    return RayQueue{f[0].p, f[1].p, f[2].p, f[3].p, f[4].p, f[5].p, path.p};
+}
+
+// Per-job facts of a call.
+struct Prep {
+    bi_pt_ctx* ctx;
+    CameraParams cam;
+    uint32_t npix, M, D, nl, spp, sh;
+    bool sky_nee, reset, rebuild;
+    uint64_t acc_before;
+};
+
+// One job's share of one wave.
+struct Item {
+    uint32_t job, sample_off, samples;
+};
+
+// Byte layout of the plan buffer: per wave, JobRec[k] then path_base[k+1] and
+// pix_base[k+1]; then the output records JobRec[n] and pix_base[n+1].
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n, hipStream_t s,
+                         const RenderOpts& opt, bi_pt_batch* graph_owner) {
+    if (n == 0) return BI_PT_OK;
+    if (!jobs) return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "null job array");
+    BI_TRY(engine_init(own, eng));
+    const bool timing = opt.timing && n == 1;
+    const bool record = opt.record && n == 1;
+    std::vector<Prep> prep(n);
+    uint32_t max_npix = 0, slots = 2, sh_mult = 1;
+    for (uint32_t j = 0; j < n; ++j) {
+        const JobIn& J = jobs[j];
+        bi_pt_ctx* ctx = J.ctx;
+        if (!ctx || !J.camera || !J.st || !J.out_rgba)
+            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "null context/camera/settings/output in job " + std::to_string(j));
+        for (uint32_t k = 0; k < j; ++k)
+            if (jobs[k].ctx == ctx) return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "a context appears twice in one call");
+        if (!ctx->has_scene) return fail(own, BI_PT_ERROR_NO_SCENE, "no scene set (job " + std::to_string(j) + ")");
+        const bi_pt_camera* c = J.camera;
+        const bi_pt_settings* st = J.st;
+        if (c->width == 0 || c->height == 0 || st->spp == 0)
+            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "empty image or zero samples");
+        const uint64_t npix64 = (uint64_t)c->width * c->height;
+        if (npix64 > (1ull << 26)) return fail(own, BI_PT_ERROR_UNSUPPORTED, "image larger than 2^26 pixels");
+        if (!(st->ray_length > 0.0f) || !(st->ray_length <= kShadowTMax))
+            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "ray_length must be positive and at most 1.7e38");
+        if (ctx->acc_count + st->spp > 0xffffffffull)
+            return fail(own, BI_PT_ERROR_UNSUPPORTED, "more than 2^32 accumulated samples");
+        Prep& p = prep[j];
+        p.ctx = ctx;
+        std::memcpy(p.cam.inv_view, c->matrix_inv_view, 64);
+        std::memcpy(p.cam.inv_proj, c->matrix_inv_proj, 64);
+        std::memcpy(p.cam.proj_view, c->matrix_proj_view, 64);
+        p.cam.width = c->width;
+        p.cam.height = c->height;
+        p.npix = (uint32_t)npix64;
+        p.M = std::min(std::max(st->max_bounces, 2u), 16u);
+        p.D = p.M - 1;
+        p.nl = (uint32_t)ctx->lights.size();
+        p.spp = st->spp;
+        p.sky_nee = !(st->flags & BI_PT_FLAG_NO_SKY_NEE) && ctx->sky.total > 0.0f;
+        p.sh = 1;
+        for (const LightRec& L : ctx->lights) p.sh += L.sm_index >= 0 ? 1u : 0u;
+        // accumulation history: continue only with the same view, size and sampling settings
+        const bool same = ctx->acc_count > 0 && st->accumulate && ctx->acc_w == p.cam.width &&
+                          ctx->acc_h == p.cam.height && std::memcmp(&ctx->last_cam, &p.cam, sizeof p.cam) == 0 &&
+                          ctx->last_seed == st->seed && ctx->last_flags == (st->flags & 3u) &&
+                          ctx->last_bounces == p.M && ctx->last_ray_length == st->ray_length;
+        p.reset = !same;
+        p.rebuild = !(opt.reuse_bvh && ctx->bvh_built);
+        max_npix = std::max(max_npix, p.npix);
+        slots = std::max(slots, p.nl + 2u);
+        sh_mult = std::max(sh_mult, p.sh);
+    }
+
+    // wave plan: whole samples of whole images, jobs in call order, each job's samples in order
+    const Knobs& kn = opt.knobs;
+    uint32_t cap = kn.wave_paths ? kn.wave_paths : (1u << 22);
+    cap = std::max(cap, max_npix);
+    std::vector<std::vector<Item>> waves;
+    {
+        std::vector<uint32_t> left(n), done(n, 0);
+        for (uint32_t j = 0; j < n; ++j) left[j] = prep[j].spp;
+        for (;;) {
+            std::vector<Item> w;
+            uint64_t room = cap;
+            for (uint32_t j = 0; j < n; ++j) {
+                if (left[j] == 0) continue;
+                const uint32_t ws = (uint32_t)std::min<uint64_t>(left[j], room / prep[j].npix);
+                if (ws == 0) continue;
+                w.push_back(Item{j, done[j], ws});
+                room -= (uint64_t)ws * prep[j].npix;
+                done[j] += ws;
+                left[j] -= ws;
+            }
+            if (w.empty()) break;
+            waves.push_back(std::move(w));
+        }
+    }
+    uint32_t wave_max = 0;
+    for (const auto& w : waves) {
+        uint64_t paths = 0;
+        for (const Item& it : w) paths += (uint64_t)it.samples * prep[it.job].npix;
+        wave_max = std::max(wave_max, (uint32_t)paths);
+    }
+    const uint32_t lds_stack = kn.stack_lds ? std::min(kn.stack_lds, 32u) : 16u;
+    const uint32_t refill = kn.refill_lanes ? std::min(kn.refill_lanes, 64u) : 32u;
+    const uint32_t parts = kn.work_parts ? std::min(kn.work_parts, kWorkPartsMax) : kWorkParts;
+
+    // device memory: per-job images, BVH builds, wave buffers, plan
+    for (uint32_t j = 0; j < n; ++j) {
+        bi_pt_ctx* ctx = prep[j].ctx;
+        bi_pt_status e = dalloc(ctx, ctx->sums, (size_t)3 * prep[j].npix);
+        if (e == BI_PT_OK && prep[j].rebuild) e = ensure_build_buffers(ctx);
+        if (e != BI_PT_OK) return fail(own, e, ctx->err);
+    }
+    BI_TRY(ensure_wave_buffers(own, eng, wave_max, slots, sh_mult));
+    const size_t rec_align = alignof(JobRec);
+    auto wave_bytes = [&](size_t k) {
+        return align_up(k * sizeof(JobRec) + 2 * (k + 1) * sizeof(uint32_t), rec_align);
+    };
+    std::vector<size_t> wave_off(waves.size() + 1, 0);
+    for (size_t w = 0; w < waves.size(); ++w) wave_off[w + 1] = wave_off[w] + wave_bytes(waves[w].size());
+    const size_t plan_bytes = wave_off.back() + wave_bytes(n);
+    BI_TRY(dalloc(own, eng.plan, plan_bytes));
+
+    // host image of the plan, in pinned memory the previous call's copy has finished with
+    if (eng.plan_pending) BI_HIP_TRY(own, hipEventSynchronize(eng.plan_ev));
+    eng.plan_pending = false;
+    if (eng.plan_host_n < plan_bytes) {
+        if (eng.plan_host) BI_HIP_TRY(own, hipHostFree(eng.plan_host));
+        eng.plan_host = nullptr;
+        eng.plan_host_n = 0;
+        BI_HIP_TRY(own, hipHostMalloc(reinterpret_cast<void**>(&eng.plan_host), plan_bytes, hipHostMallocDefault));
+        eng.plan_host_n = plan_bytes;
+    }
+    unsigned char* const host = eng.plan_host;
+    std::memset(host, 0, plan_bytes);
+    auto fill_rec = [&](JobRec& r, uint32_t j, uint32_t sample_off, uint32_t samples, uint32_t pix_base) {
+        const Prep& p = prep[j];
+        bi_pt_ctx* ctx = p.ctx;
+        const bool empty = ctx->num_tris == 0;
+        r.nodes = empty ? eng.empty_tree.p : ctx->nodes4.p;
+        r.leaf_geo = empty ? nullptr : ctx->leaf_geo.p;
+        r.leaf_shade = empty ? nullptr : ctx->leaf_shade.p;
+        r.materials = ctx->mats.p;
+        r.num_tris = ctx->num_tris;
+        r.num_lights = p.nl;
+        r.sky_nee = p.sky_nee ? 1u : 0u;
+        r.last_bounce = p.D - 1;
+        for (uint32_t l = 0; l < p.nl; ++l) {
+            r.lights[l] = ctx->lights[l];
+            r.keep[l] = 1.0f - ctx->lights[l].shadow_strength;
+        }
+        r.sky = ctx->sky;
+        r.cam = p.cam;
+        r.npix = p.npix;
+        r.sample0 = (uint32_t)(p.reset ? 0 : ctx->acc_count) + sample_off;
+        r.samples = samples;
+        r.seed = jobs[j].st->seed;
+        r.flags = jobs[j].st->flags & 3u;
+        r.ray_length = jobs[j].st->ray_length;
+        r.pix_base = pix_base;
+        r.sums = ctx->sums.p;
+        r.depth_out = jobs[j].out_depth;
+        r.depth_sample = (uint32_t)(p.reset ? 0 : ctx->acc_count);
+        r.out_rgba = jobs[j].out_rgba;
+        r.out_den = (float)((p.reset ? 0 : ctx->acc_count) + p.spp);
+    };
+    std::vector<WaveJobs> wj(waves.size());
+    for (size_t w = 0; w < waves.size(); ++w) {
+        const size_t k = waves[w].size();
+        unsigned char* base = host + wave_off[w];
+        JobRec* recs = reinterpret_cast<JobRec*>(base);
+        uint32_t* pb = reinterpret_cast<uint32_t*>(base + k * sizeof(JobRec));
+        uint32_t* xb = pb + (k + 1);
+        uint32_t paths = 0, pix = 0;
+        for (size_t i = 0; i < k; ++i) {
+            const Item& it = waves[w][i];
+            new (&recs[i]) JobRec{};
+            fill_rec(recs[i], it.job, it.sample_off, it.samples, pix);
+            pb[i] = paths;
+            xb[i] = pix;
+            paths += it.samples * prep[it.job].npix;
+            pix += prep[it.job].npix;
+        }
+        pb[k] = paths;
+        xb[k] = pix;
+        unsigned char* dbase = eng.plan.p + wave_off[w];
+        wj[w] = WaveJobs{reinterpret_cast<const JobRec*>(dbase),
+                         reinterpret_cast<const uint32_t*>(dbase + k * sizeof(JobRec)),
+                         reinterpret_cast<const uint32_t*>(dbase + k * sizeof(JobRec)) + (k + 1), (uint32_t)k, paths};
+    }
+    WaveJobs out_jobs{};
+    uint32_t out_pix = 0;
+    {
+        unsigned char* base = host + wave_off.back();
+        JobRec* recs = reinterpret_cast<JobRec*>(base);
+        uint32_t* pb = reinterpret_cast<uint32_t*>(base + n * sizeof(JobRec));
+        uint32_t* xb = pb + (n + 1);
+        for (uint32_t j = 0; j < n; ++j) {
+            new (&recs[j]) JobRec{};
+            fill_rec(recs[j], j, 0, 0, out_pix);
+            pb[j] = 0;
+            xb[j] = out_pix;
+            out_pix += prep[j].npix;
+        }
+        pb[n] = 0;
+        xb[n] = out_pix;
+        unsigned char* dbase = eng.plan.p + wave_off.back();
+        out_jobs = WaveJobs{reinterpret_cast<const JobRec*>(dbase),
+                            reinterpret_cast<const uint32_t*>(dbase + n * sizeof(JobRec)),
+                            reinterpret_cast<const uint32_t*>(dbase + n * sizeof(JobRec)) + (n + 1), n, 0};
+    }
+    BI_HIP_TRY(own, hipMemcpyAsync(eng.plan.p, host, plan_bytes, hipMemcpyHostToDevice, s));
+    BI_HIP_TRY(own, hipEventRecord(eng.plan_ev, s));
+    eng.plan_pending = true;
+
+    // graph replay: the same launch sequence with the same buffers (the plan's contents,
+    // uploaded above, carry everything that changes between calls)
+    const bool graph = graph_owner && graph_owner->use_graph && !timing && !record;
+    std::vector<uint64_t> key;
+    if (graph) {
+        key = {n, cap, lds_stack, refill, parts, (uint64_t)(uintptr_t)eng.plan.p, eng.wave_cap, eng.wave_slots,
+               eng.wave_sh_mult, plan_bytes};
+        for (uint32_t j = 0; j < n; ++j) {
+            const Prep& p = prep[j];
+            key.insert(key.end(), {(uint64_t)(uintptr_t)p.ctx, (uint64_t)(uintptr_t)p.ctx->nodes4.p,
+                                   (uint64_t)(uintptr_t)p.ctx->sums.p, p.npix, p.spp, p.M, p.nl, p.sh,
+                                   (uint64_t)p.reset, (uint64_t)p.rebuild, p.ctx->num_tris});
+        }
+        if (graph_owner->graph_exec && key == graph_owner->graph_key) {
+            BI_HIP_TRY(own, hipGraphLaunch(graph_owner->graph_exec, s));
+            goto finish;
+        }
+        if (graph_owner->graph_exec) { (void)hipGraphExecDestroy(graph_owner->graph_exec); graph_owner->graph_exec = nullptr; }
+        if (graph_owner->graph) { (void)hipGraphDestroy(graph_owner->graph); graph_owner->graph = nullptr; }
+        graph_owner->graph_key.clear();
+        BI_HIP_TRY(own, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    }
+    {
+        uint32_t* cnt = eng.counters.p;
+        uint64_t* rays = reinterpret_cast<uint64_t*>(cnt + kCntRays);
+        // per call: queue lengths and error flag (the ray statistics accumulate), the
+        // persistent lanes' part counters, and the images of jobs that start over
+        launch_fill_u32(cnt, 0u, kCntRays, s);
+        launch_fill_u32(cnt + kCntTraceWork, 0u, kCntSlots - kCntTraceWork, s);
+        for (uint32_t j = 0; j < n; ++j)
+            if (prep[j].reset) launch_fill_u32(reinterpret_cast<uint32_t*>(prep[j].ctx->sums.p), 0u, 3 * prep[j].npix, s);
+        if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[0], s));
+        for (uint32_t j = 0; j < n; ++j) {
+            if (!prep[j].rebuild) continue;
+            if (bi_pt_status e = build_bvh(prep[j].ctx, s, &cnt[kCntError]); e != BI_PT_OK) {
+                if (graph) { hipGraph_t g; (void)hipStreamEndCapture(s, &g); if (g) (void)hipGraphDestroy(g); }
+                return fail(own, e, prep[j].ctx->err);
+            }
+        }
+        if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[1], s));
+
+        PathState ps{eng.ps_f[0].p, eng.ps_f[1].p, eng.ps_f[2].p, eng.ps_f[3].p, eng.acc.p, eng.wave_cap};
+        HitBuf hits{eng.hit_rec.p};
+        RayQueue qA = queue_of(eng.qa, eng.qa_path), qB = queue_of(eng.qb, eng.qb_path);
+        ShadowQueue sq{eng.sq_f[0].p, eng.sq_f[1].p, eng.sq_f[2].p, eng.sq_f[3].p, eng.sq_f[4].p,
+                       eng.sq_f[5].p, eng.sq_f[6].p, eng.sq_f[7].p, eng.sq_f[8].p, eng.sq_pk.p};
+        const size_t wc = eng.wave_cap;
+        const size_t cslots = eng.wave_slots - 1u;
+        ShadeStage stage{eng.st_f[0].p, eng.st_f[1].p, eng.st_f[2].p, eng.st_path.p,
+                         eng.st_f[3].p, eng.st_f[4].p, eng.st_f[5].p,
+                         eng.st_f[6].p, eng.st_f[7].p, eng.st_f[8].p,
+                         eng.st_c.p, eng.st_c.p + cslots * wc, eng.st_c.p + 2 * cslots * wc,
+                         eng.st_mask.p, eng.tile_sums.p, eng.tile_sky.p};
+        if (record) { eng.rec_ext.clear(); eng.rec_sh.clear(); }
+
+        // the launches of one kind are timed together: events around each class of kernel
+        std::vector<hipEvent_t> tev;
+        std::vector<int> tcls;
+        auto mark = [&](int cls) -> bi_pt_status {
+            if (!timing) return BI_PT_OK;
+            hipEvent_t e;
+            BI_HIP_TRY(own, hipEventCreate(&e));
+            BI_HIP_TRY(own, hipEventRecord(e, s));
+            tev.push_back(e);
+            tcls.push_back(cls);
+            return BI_PT_OK;
+        };
+        uint32_t launches_closest = 0, launches_shadow = 0;
+
+        for (size_t w = 0; w < waves.size(); ++w) {
+            const WaveJobs& J = wj[w];
+            uint32_t D = 0;
+            for (const Item& it : waves[w]) D = std::max(D, prep[it.job].D);
+            if (record) {
+                eng.rec_ext.emplace_back(D);
+                eng.rec_sh.emplace_back(D);
+                std::vector<uint32_t> all(J.num_paths);
+                for (uint32_t k = 0; k < J.num_paths; ++k) all[k] = k;
+                eng.rec_ext.back()[0] = all;
+            }
+            for (uint32_t b = 0; b < D; ++b) {
+                const bool primary = b == 0;
+                const RayQueue& qin = (b & 1u) ? qA : qB;      // segment b reads what segment b-1 wrote
+                const RayQueue& qout = (b & 1u) ? qB : qA;
+                uint32_t* cnt_in = (b & 1u) ? &cnt[kCntExtA] : &cnt[kCntExtB];
+                uint32_t* cnt_out = (b & 1u) ? &cnt[kCntExtB] : &cnt[kCntExtA];
+
+                TraceArgs ta{};
+                ta.jobs = J;
+                ta.q = qin;
+                ta.count_in = primary ? nullptr : cnt_in;
+                ta.hits = hits;
+                ta.work = &cnt[kCntTraceWork];
+                ta.parts = parts;
+                ta.refill = refill;
+                ta.error = &cnt[kCntError];
+                BI_TRY(mark(0));
+                launch_trace_closest(ta, J.num_paths, primary, lds_stack, s);
+                ++launches_closest;
+                BI_TRY(mark(1));
+
+                ShadeArgs sa{};
+                sa.jobs = J;
+                sa.qin = qin;
+                sa.count_in = primary ? nullptr : cnt_in;
+                sa.hits = hits;
+                sa.st = stage;
+                sa.ps = ps;
+                sa.trace_work_reset = &cnt[kCntTraceWork];
+                sa.shadow_work_reset = &cnt[kCntShadowWork];
+                sa.bounce = b;
+                launch_shade(sa, J.num_paths, s);
+
+                ScanArgs sc{};
+                sc.tile_sums = stage.tile_sums;
+                sc.tile_sky = stage.tile_sky;
+                sc.count_in = primary ? nullptr : cnt_in;
+                sc.num_entries = J.num_paths;
+                sc.count_ext = cnt_out;
+                sc.count_shadow = &cnt[kCntShadow];
+                sc.ray_counters = rays;
+                launch_scan_tiles(sc, s);
+
+                CompactArgs ca{};
+                ca.jobs = J;
+                ca.st = stage;
+                ca.count_in = primary ? nullptr : cnt_in;
+                ca.num_entries = J.num_paths;
+                ca.cap = wc;
+                ca.qout = qout;
+                ca.sq = sq;
+                launch_compact(ca, J.num_paths, s);
+                BI_TRY(mark(2));
+
+                ShadowArgs sha{};
+                sha.jobs = J;
+                sha.q = sq;
+                sha.count_in = &cnt[kCntShadow];
+                sha.ps = ps;
+                sha.work = &cnt[kCntShadowWork];
+                sha.parts = parts;
+                sha.refill = refill;
+                sha.error = &cnt[kCntError];
+                launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, s);
+                ++launches_shadow;
+                BI_TRY(mark(3));
+                BI_HIP_TRY(own, hipGetLastError());
+                if (record) {
+                    uint32_t hc[3];
+                    BI_HIP_TRY(own, hipMemcpyAsync(hc, cnt, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                    BI_HIP_TRY(own, hipStreamSynchronize(s));
+                    const uint32_t next = hc[(b & 1u) ? kCntExtB : kCntExtA];
+                    auto& shq = eng.rec_sh.back()[b];
+                    shq.resize(hc[kCntShadow]);
+                    if (!shq.empty())
+                        BI_HIP_TRY(own, hipMemcpy(shq.data(), eng.sq_pk.p, shq.size() * 4, hipMemcpyDeviceToHost));
+                    if (b + 1 < D) {
+                        auto& ex = eng.rec_ext.back()[b + 1];
+                        ex.resize(next);
+                        if (next) BI_HIP_TRY(own, hipMemcpy(ex.data(), qout.path, next * 4, hipMemcpyDeviceToHost));
+                    }
+                }
+            }
+            uint32_t wave_pix = 0;
+            for (const Item& it : waves[w]) wave_pix += prep[it.job].npix;
+            ResolveArgs ra{};
+            ra.jobs = J;
+            ra.ps = ps;
+            ra.num_pix = wave_pix;
+            launch_resolve(ra, s);
+            BI_TRY(mark(4));
+        }
+        launch_output(out_jobs, out_pix, s);
+        BI_HIP_TRY(own, hipGetLastError());
+        eng.stats.launches_trace_closest = launches_closest;
+        eng.stats.launches_trace_shadow = launches_shadow;
+        if (timing) {
+            BI_HIP_TRY(own, hipEventRecord(eng.ev[2], s));
+            BI_HIP_TRY(own, hipEventSynchronize(eng.ev[2]));
+            float ms_build = 0, total = 0;
+            BI_HIP_TRY(own, hipEventElapsedTime(&ms_build, eng.ev[0], eng.ev[1]));
+            BI_HIP_TRY(own, hipEventElapsedTime(&total, eng.ev[0], eng.ev[2]));
+            float cls_ms[5] = {0, 0, 0, 0, 0};
+            // class of the interval (tev[k-1], tev[k]) is tcls[k]: 1 trace, 2 shade, 3 shadow, 4 resolve
+            for (size_t k = 1; k < tev.size(); ++k) {
+                float ms = 0;
+                BI_HIP_TRY(own, hipEventElapsedTime(&ms, tev[k - 1], tev[k]));
+                int c = tcls[k];
+                if (c >= 1 && c <= 4) cls_ms[c] += ms;
+            }
+            for (auto e : tev) (void)hipEventDestroy(e);
+            eng.stats.ms_build = ms_build;
+            eng.stats.ms_trace_closest = cls_ms[1];
+            eng.stats.ms_shade = cls_ms[2];
+            eng.stats.ms_trace_shadow = cls_ms[3];
+            eng.stats.ms_resolve = cls_ms[4];
+            eng.stats.ms_total = total;
+        }
+    }
+    if (graph) {
+        hipGraph_t g = nullptr;
+        BI_HIP_TRY(own, hipStreamEndCapture(s, &g));
+        graph_owner->graph = g;
+        BI_HIP_TRY(own, hipGraphInstantiate(&graph_owner->graph_exec, g, nullptr, nullptr, 0));
+        graph_owner->graph_key = key;
+        BI_HIP_TRY(own, hipGraphLaunch(graph_owner->graph_exec, s));
+    }
+finish:
+    // host bookkeeping of the running images
+    for (uint32_t j = 0; j < n; ++j) {
+        const Prep& p = prep[j];
+        bi_pt_ctx* ctx = p.ctx;
+        const bi_pt_settings* st = jobs[j].st;
+        if (p.reset) ctx->acc_count = 0;
+        ctx->acc_w = p.cam.width;
+        ctx->acc_h = p.cam.height;
+        ctx->last_cam = p.cam;
+        ctx->last_seed = st->seed;
+        ctx->last_flags = st->flags & 3u;
+        ctx->last_bounces = p.M;
+        ctx->last_ray_length = st->ray_length;
+        ctx->acc_count += p.spp;
+        if (p.rebuild) ctx->bvh_built = true;
+        eng.camera_rays += (uint64_t)p.npix * p.spp;
+    }
+    eng.frames++;
+    eng.stats.frames = eng.frames;
+    eng.stats.rays_camera = eng.camera_rays;
+    eng.stats.samples_accumulated = n == 1 ? prep[0].ctx->acc_count : 0;
+    return BI_PT_OK;
+}
+
+}  // namespace bipt

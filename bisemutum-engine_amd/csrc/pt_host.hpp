@@ -1,0 +1,198 @@
+// pt_host.hpp -- host-side state of the C ABI: contexts (one scene + its running
+// image), the wave engine (queues and per-wave buffers, shared by every job a call
+// renders) and the batch object (an engine that renders several contexts' jobs in one
+// set of waves).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bi_pt.h"
+#include "pt_kernels.hpp"
+
+namespace bipt {
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+// Device memory owner: HBM budget accounting and the last error message.
+struct Owner {
+    int device = 0;
+    uint64_t budget = 0;      // 0 = unlimited
+    uint64_t allocated = 0;
+    std::string err;
+};
+
+inline bi_pt_status fail(Owner* o, bi_pt_status st, const std::string& msg) {
+    if (o) o->err = msg;
+    return st;
+}
+
+#define BI_HIP_TRY(own, expr)                                                                       \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return ::bipt::fail((own), BI_PT_ERROR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define BI_TRY(expr)                            \
+    do {                                        \
+        bi_pt_status s_ = (expr);               \
+        if (s_ != BI_PT_OK) return s_;          \
+    } while (0)
+
+template <class T>
+bi_pt_status dalloc(Owner* o, DevBuf<T>& b, size_t n) {
+    if (b.p && b.n >= n) return BI_PT_OK;
+    if (b.p) {
+        (void)hipFree(b.p);
+        o->allocated -= b.n * sizeof(T);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    if (n == 0) return BI_PT_OK;
+    const size_t bytes = n * sizeof(T);
+    if (o->budget && o->allocated + bytes > o->budget)
+        return fail(o, BI_PT_ERROR_OUT_OF_MEMORY,
+                    "HBM budget exceeded: need " + std::to_string(bytes) + " B more, budget " +
+                        std::to_string(o->budget) + " B, in use " + std::to_string(o->allocated) + " B");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(o, BI_PT_ERROR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    b.p = static_cast<T*>(p);
+    b.n = n;
+    o->allocated += bytes;
+    return BI_PT_OK;
+}
+
+template <class T>
+void dfree(Owner* o, DevBuf<T>& b) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        o->allocated -= b.n * sizeof(T);
+    }
+    b.p = nullptr;
+    b.n = 0;
+}
+
+// Tuning knobs of the wave machinery (0 = automatic).
+struct Knobs {
+    uint32_t wave_paths = 0;
+    uint32_t stack_lds = 0;
+    uint32_t refill_lanes = 0;
+    uint32_t work_parts = 0;
+};
+
+// Queues and per-wave buffers, sized for the largest wave rendered so far.
+struct Engine {
+    uint32_t wave_cap = 0, wave_slots = 0, wave_sh_mult = 0;
+    DevBuf<float> qa[6], qb[6], ps_f[4], acc, sq_f[9], st_f[9], st_c;
+    DevBuf<uint32_t> qa_path, qb_path, sq_pk, st_path, st_mask, tile_sky;
+    DevBuf<HitRec> hit_rec;
+    DevBuf<uint64_t> tile_sums;
+    DevBuf<uint32_t> counters;             // kCntSlots (ray counters at kCntRays as 4 x u64)
+    DevBuf<unsigned char> plan;            // job records and bases of every wave of a call
+    unsigned char* plan_host = nullptr;    // pinned staging of the plan (copied asynchronously)
+    size_t plan_host_n = 0;
+    bool plan_pending = false;             // plan_ev marks the end of the last plan copy
+    DevBuf<Bvh4Node> empty_tree;           // one node of empty slots (jobs without triangles)
+    hipEvent_t ev[3] = {};
+    hipEvent_t plan_ev = nullptr;
+    bool ready = false;
+    // statistics
+    bi_pt_stats stats{};
+    uint64_t frames = 0, camera_rays = 0;
+    // queues recorded by a single-job call with BI_PT_FLAG_RECORD_QUEUES: [wave][segment]
+    std::vector<std::vector<std::vector<uint32_t>>> rec_ext, rec_sh;
+};
+
+bi_pt_status engine_init(Owner* own, Engine& eng);
+void engine_free(Owner* own, Engine& eng);
+// Error flags of the engine's last calls (synchronises the device).
+bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out);
+
+}  // namespace bipt
+
+// One scene, its per-frame BVH and its running image.
+struct bi_pt_ctx : bipt::Owner {
+    // scene
+    bool has_scene = false;
+    bipt::DevBuf<float> pos, nrm;
+    bipt::DevBuf<uint32_t> idx;
+    bipt::DevBuf<bipt::InstanceRec> inst;
+    bipt::DevBuf<bipt::MaterialRec> mats;
+    std::vector<bipt::InstanceRec> inst_host;
+    uint32_t num_tris = 0, num_inst = 0, num_mats = 0;
+    // lights
+    std::vector<bipt::LightRec> lights;
+    // sky
+    bipt::DevBuf<float4> sky_tex;
+    bipt::DevBuf<float> sky_w, sky_ccdf, sky_rows, sky_marg;
+    bipt::SkyParams sky{};
+    std::vector<float> sky_host;
+    bool sky_set = false;
+    // build buffers
+    bipt::DevBuf<float> tri_v, tri_n, cb_out;
+    bipt::DevBuf<uint32_t> tri_mat, cb_ord, keys0, keys1, vals0, vals1, hist, refit_cnt;
+    bipt::DevBuf<bipt::BvhNode> nodes;
+    bipt::DevBuf<bipt::Bvh4Node> nodes4;
+    bipt::DevBuf<int32_t> parent;
+    bipt::DevBuf<bipt::LeafGeo> leaf_geo;
+    bipt::DevBuf<bipt::LeafShade> leaf_shade;
+    bool bvh_built = false;
+    // accumulation
+    bipt::DevBuf<float> sums;
+    uint32_t acc_w = 0, acc_h = 0;
+    uint64_t acc_count = 0;
+    bipt::CameraParams last_cam{};
+    uint32_t last_seed = 0, last_flags = 0, last_bounces = 0;
+    float last_ray_length = 0.0f;
+    // wave machinery of bi_pt_render
+    bipt::Engine eng;
+};
+
+// Several contexts' jobs rendered in one set of waves.
+struct bi_pt_batch : bipt::Owner {
+    bipt::Engine eng;
+    bipt::Knobs knobs;
+    bool use_graph = false;
+    // captured call (hipGraph) and the plan it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+};
+
+namespace bipt {
+
+bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx);
+bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error);
+BvhView bvh_view(const bi_pt_ctx* ctx);
+
+// One job of a call.
+struct JobIn {
+    bi_pt_ctx* ctx;
+    const bi_pt_camera* camera;
+    const bi_pt_settings* st;
+    float* out_rgba;
+    float* out_depth;
+};
+
+struct RenderOpts {
+    Knobs knobs;
+    bool timing = false;      // per-kernel-class events (single-job calls)
+    bool record = false;      // record queue contents (single-job calls)
+    bool reuse_bvh = false;   // keep a built BVH whose scene is unchanged
+};
+
+// Renders every job (spp samples each, added to its context's running image) in shared
+// waves on one stream.  Errors are reported on `own`.
+bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n, hipStream_t s,
+                         const RenderOpts& opt, bi_pt_batch* graph_owner);
+
+}  // namespace bipt

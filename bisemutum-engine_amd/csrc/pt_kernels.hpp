@@ -47,22 +47,31 @@ struct BuildArgs {
 void launch_build_lbvh(const BuildArgs& a, hipStream_t s);
 uint32_t sort_hist_entries(uint32_t n);
 
-// Wave description shared by the trace and shade kernels.
-struct WaveParams {
-    CameraParams cam;
-    uint32_t num_paths;   // paths in this wave: (samples in wave) * npix
-    uint32_t npix;
-    uint32_t sample0;     // global sample index of the wave's first sample
-    uint32_t seed;
-    uint32_t flags;
-    float ray_length;
+// Jobs of a wave: records, path bases (num_jobs + 1 entries, the last = paths in the
+// wave) and pixel bases (num_jobs + 1 entries) for the resolve pass.
+struct WaveJobs {
+    const JobRec* rec;
+    const uint32_t* path_base;
+    const uint32_t* pix_base;
+    uint32_t num_jobs;
+    uint32_t num_paths;
 };
 
+// Job of a path (or pixel slot): the last j with base[j] <= x.
+PT_HD uint32_t job_of(const uint32_t* base, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;   // base[lo] <= x < base[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 struct TraceArgs {
-    BvhView bvh;
-    WaveParams wave;
+    WaveJobs jobs;
     RayQueue q;                 // input queue (unused for primary rays)
-    const uint32_t* count_in;   // device count (null: primary, count = wave.num_paths)
+    const uint32_t* count_in;   // device count (null: primary, count = jobs.num_paths)
     HitBuf hits;
     uint32_t* work;             // part counters of the persistent lanes (zero at launch)
     uint32_t parts;             // queue parts (1..kWorkPartsMax)
@@ -75,12 +84,10 @@ void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, floa
                          uint32_t* error, hipStream_t s);
 
 struct ShadowArgs {
-    BvhView bvh;
+    WaveJobs jobs;
     ShadowQueue q;
     const uint32_t* count_in;
     PathState ps;
-    float keep[kMaxDirLights];  // 1 - shadow_strength per light
-    uint32_t num_lights;
     uint32_t* work;             // part counters of the persistent lanes (zero at launch)
     uint32_t parts;             // queue parts (1..kWorkPartsMax)
     uint32_t refill;            // idle lanes before a refill (1..64)
@@ -101,13 +108,7 @@ struct ShadeStage {
 };
 
 struct ShadeArgs {
-    WaveParams wave;
-    const LeafShade* leaf_shade;
-    const MaterialRec* materials;
-    LightRec lights[kMaxDirLights];
-    uint32_t num_lights;
-    SkyParams sky;
-    uint32_t sky_nee;
+    WaveJobs jobs;
     RayQueue qin;
     const uint32_t* count_in;   // null at bounce 0
     HitBuf hits;
@@ -116,9 +117,6 @@ struct ShadeArgs {
     uint32_t* trace_work_reset;   // zeroed here for the next closest-hit launch
     uint32_t* shadow_work_reset;  // zeroed here for this segment's shadow launch
     uint32_t bounce;
-    uint32_t last_bounce;       // segments - 1
-    float* depth_out;           // nullable: reverse-Z depth of the primary hit
-    uint32_t depth_sample;
 };
 void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s);
 
@@ -134,28 +132,24 @@ struct ScanArgs {
 void launch_scan_tiles(const ScanArgs& a, hipStream_t s);
 
 struct CompactArgs {
+    WaveJobs jobs;
     ShadeStage st;
     const uint32_t* count_in;   // null: num_entries
     uint32_t num_entries;
-    uint32_t num_lights;
     size_t cap;
-    float ldir[kMaxDirLights][3];
     RayQueue qout;
     ShadowQueue sq;
 };
 void launch_compact(const CompactArgs& a, uint32_t capacity, hipStream_t s);
 
 struct ResolveArgs {
+    WaveJobs jobs;
     PathState ps;
-    uint32_t num_slots;   // accumulator kinds in use: miss, lights, sky
-    uint32_t npix;
-    uint32_t samples;     // samples in this wave
-    uint32_t width, height;
-    float* sums;          // running per-pixel sums (3 floats)
+    uint32_t num_pix;     // resolve slots of the wave (sum of the jobs' pixels)
 };
 void launch_resolve(const ResolveArgs& a, hipStream_t s);
-void launch_output(const float* sums, uint32_t npix, float inv_count_den, uint32_t count, float* out_rgba,
-                   hipStream_t s);
+// Running mean of every job of the wave -> its RGBA32F output.
+void launch_output(const WaveJobs& jobs, uint32_t num_pix, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
 
 }  // namespace bipt

@@ -115,6 +115,38 @@ struct BvhView {
     uint32_t num_tris;
 };
 
+// One render job of a wave: a scene (BVH, shading records, materials, lights, sky), a
+// camera and sampling parameters, and where its paths sit in the wave.  A wave may hold
+// paths of several jobs (batched rendering); job j owns the paths [path_base[j],
+// path_base[j+1]) and the pixel slots [pix_base, pix_base + npix) of the resolve pass.
+// Kernels find a path's job by a binary search over the wave's path bases.
+struct JobRec {
+    const Bvh4Node* nodes;
+    const LeafGeo* leaf_geo;
+    const LeafShade* leaf_shade;
+    const MaterialRec* materials;
+    uint32_t num_tris;
+    uint32_t num_lights;
+    uint32_t sky_nee;       // sky next-event estimation on
+    uint32_t last_bounce;   // segments - 1
+    float keep[kMaxDirLights];   // 1 - shadow_strength per light
+    LightRec lights[kMaxDirLights];
+    SkyParams sky;
+    CameraParams cam;
+    uint32_t npix;
+    uint32_t sample0;       // sample index of the job's first sample in this wave
+    uint32_t samples;       // samples of the job in this wave
+    uint32_t seed;
+    uint32_t flags;         // bit 0: pixel jitter
+    float ray_length;
+    uint32_t pix_base;      // first resolve slot of the job in this wave
+    float* sums;            // running per-pixel sums (3 floats per pixel)
+    float* depth_out;       // nullable: reverse-Z depth of the primary hit
+    uint32_t depth_sample;  // the sample whose primary hit gives the depth
+    float* out_rgba;        // output mean (RGBA32F)
+    float out_den;          // samples accumulated after this call
+};
+
 // Ray queue in SoA form.
 struct RayQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz;

@@ -30,14 +30,146 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Shades queue entry i (path `path`, the job's local path `local`) of job J: writes the
+// entry's staging slots and returns its emission mask.
+__device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec& J, uint32_t i, uint32_t path,
+                                                uint32_t local, bool primary) {
+    const size_t cap = a.ps.cap;
+    uint32_t mask = 0;
+    const uint32_t npix = J.npix;
+    const uint32_t q = local % npix, sl = local / npix;
+    const uint32_t pixel = pixel_of(q, J.cam.width, J.cam.height);
+    const uint32_t sample = J.sample0 + sl;
+    const uint32_t nl = J.num_lights;
+    const uint32_t nslots = nl + 2u;
+    V3 o, d, P = v3(0.0f, 0.0f, 0.0f);
+    V3 beta;
+    float pdf_prev;
+    if (primary) {
+        float jx = 0.5f, jy = 0.5f;
+        if (J.flags & 1u) {
+            Pcg jr = pcg_keyed(J.seed, pixel, sample, kJitterBounce);
+            jx = pcg_float(jr);
+            jy = pcg_float(jr);
+        }
+        camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, jx, jy, &o, &d);
+        beta = v3(1.0f, 1.0f, 1.0f);
+        pdf_prev = 0.0f;
+        for (uint32_t k = 0; k < nslots; ++k) {
+            float* acc = a.ps.acc + (size_t)k * 3u * cap;
+            acc[path] = 0.0f;
+            acc[cap + path] = 0.0f;
+            acc[2 * cap + path] = 0.0f;
+        }
+    } else {
+        o = v3(a.qin.ox[i], a.qin.oy[i], a.qin.oz[i]);
+        d = v3(a.qin.dx[i], a.qin.dy[i], a.qin.dz[i]);
+        beta = v3(a.ps.bx[path], a.ps.by[path], a.ps.bz[path]);
+        pdf_prev = a.ps.pdf[path];
+    }
+    const HitRec hit = a.hits.rec[i];
+    const uint32_t leaf = hit.leaf;
+    const bool want_depth = primary && J.depth_out != nullptr && sample == J.depth_sample;
+    if (leaf == kLeafMiss) {
+        V3 ls = sky_eval(J.sky, d);
+        float w = 1.0f;
+        if (!primary && J.sky_nee) w = power_heuristic(pdf_prev, sky_pdf(J.sky, d));
+        V3 c = (beta * ls) * w;
+        a.ps.acc[path] = c.x;
+        a.ps.acc[cap + path] = c.y;
+        a.ps.acc[2 * cap + path] = c.z;
+        if (want_depth) J.depth_out[pixel] = 0.0f;
+    } else {
+        const float t = hit.t, hu = hit.u, hv = hit.v;
+        P = o + d * t;
+        const float4* lsp = reinterpret_cast<const float4*>(J.leaf_shade + leaf);
+        const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
+        V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
+        const uint32_t mat_id = __float_as_uint(s2.y);
+        V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
+        Mat m = load_mat(J.materials[mat_id]);
+        if (m.two_sided && dot(d, N) > 0.0f) N = -N;
+        V3 V = normalize(o - P);
+        Frame fr = make_frame(N);
+        Pcg rng = pcg_keyed(J.seed, pixel, sample, a.bounce);
+        float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
+        for (uint32_t l = 0; l < nl; ++l) {
+            const LightRec& L = J.lights[l];
+            V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
+            V3 ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
+            V3 f = bsdf_eval(fr, m, V, ldir);
+            V3 c = (le * f) * beta;
+            if (!any_pos(c)) continue;
+            if (L.sm_index < 0) {
+                float* acc = a.ps.acc + (size_t)(l + 1u) * 3u * cap;
+                acc[path] = acc[path] + c.x;
+                acc[cap + path] = acc[cap + path] + c.y;
+                acc[2 * cap + path] = acc[2 * cap + path] + c.z;
+                continue;
+            }
+            const size_t s = (size_t)l * cap + i;
+            a.st.cx[s] = c.x; a.st.cy[s] = c.y; a.st.cz[s] = c.z;
+            mask |= 2u << l;
+        }
+        // sky NEE only where the next segment is sampled too (the reference sees the
+        // sky only through a traced segment that escapes)
+        const bool extend = a.bounce < J.last_bounce && !(max3(beta) < kThroughputMin);
+        if (J.sky_nee && extend) {
+            V3 wi;
+            float pl;
+            if (sky_sample(J.sky, u0, u1, &wi, &pl)) {
+                V3 f = bsdf_eval(fr, m, V, wi);
+                if (any_pos(f)) {
+                    V3 hl = to_local(fr, normalize(V + wi));
+                    float pb = vndf_pdf_l(hl, to_local(fr, V), m.rx, m.ry);
+                    float w = power_heuristic(pl, pb);
+                    V3 c = ((sky_eval(J.sky, wi) * f) * (w / pl)) * beta;
+                    if (any_pos(c) && finite3(c)) {
+                        const size_t s = (size_t)nl * cap + i;
+                        a.st.cx[s] = c.x; a.st.cy[s] = c.y; a.st.cz[s] = c.z;
+                        a.st.sx[i] = wi.x; a.st.sy[i] = wi.y; a.st.sz[i] = wi.z;
+                        mask |= 2u << kMaxDirLights;
+                    }
+                }
+            }
+        }
+        if (extend) {
+            V3 vl = to_local(fr, V);
+            V3 hh = vndf_sample(vl, m.rx, m.ry, u2, u3);
+            float hvd = dot(vl, hh);
+            V3 ll = hh * (2.0f * hvd) - vl;
+            float pdf = vndf_pdf_l(hh, vl, m.rx, m.ry);
+            V3 Lext = to_world(fr, ll);
+            V3 f = bsdf_eval(fr, m, V, Lext);
+            V3 w = v3(f.x / pdf, f.y / pdf, f.z / pdf);
+            if (!finite3(w)) w = v3(0.0f, 0.0f, 0.0f);
+            V3 nb = beta * w;
+            if (max3(nb) > 0.0f) {
+                a.st.ex[i] = Lext.x; a.st.ey[i] = Lext.y; a.st.ez[i] = Lext.z;
+                a.ps.bx[path] = nb.x; a.ps.by[path] = nb.y; a.ps.bz[path] = nb.z;
+                a.ps.pdf[path] = pdf;
+                mask |= kMaskExt;
+            }
+        }
+        if (want_depth) {
+            const float* pv = J.cam.proj_view;
+            float z = ((pv[8] * P.x + pv[9] * P.y) + pv[10] * P.z) + pv[11];
+            float w = ((pv[12] * P.x + pv[13] * P.y) + pv[14] * P.z) + pv[15];
+            J.depth_out[pixel] = z / w;
+        }
+    }
+    if (mask) {
+        a.st.px[i] = P.x; a.st.py[i] = P.y; a.st.pz[i] = P.z;
+        a.st.path[i] = path;
+    }
+    return mask;
+}
+
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     __shared__ uint32_t s_wsum[4], s_wsky[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const bool primary = a.bounce == 0;
-    const uint32_t count = primary ? a.wave.num_paths : *a.count_in;
-    const uint32_t nl = a.num_lights;
-    const uint32_t nslots = nl + 2u;
-    const size_t cap = a.ps.cap;
+    const uint32_t count = primary ? a.jobs.num_paths : *a.count_in;
     if (blockIdx.x == 0 && tid < kWorkPartsMax) {
         // the closest-hit launch of this segment has finished and this segment's shadow
         // launch has not started: reset their persistent-lane part counters
@@ -50,128 +182,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     uint32_t mask = 0;
     if (i < count) {
         const uint32_t path = primary ? i : a.qin.path[i];
-        const uint32_t q = path % a.wave.npix, sl = path / a.wave.npix;
-        const uint32_t pixel = pixel_of(q, a.wave.cam.width, a.wave.cam.height);
-        const uint32_t sample = a.wave.sample0 + sl;
-        V3 o, d, P = v3(0.0f, 0.0f, 0.0f);
-        V3 beta;
-        float pdf_prev;
-        if (primary) {
-            float jx = 0.5f, jy = 0.5f;
-            if (a.wave.flags & 1u) {
-                Pcg jr = pcg_keyed(a.wave.seed, pixel, sample, kJitterBounce);
-                jx = pcg_float(jr);
-                jy = pcg_float(jr);
-            }
-            camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &o, &d);
-            beta = v3(1.0f, 1.0f, 1.0f);
-            pdf_prev = 0.0f;
-            for (uint32_t k = 0; k < nslots; ++k) {
-                float* acc = a.ps.acc + (size_t)k * 3u * cap;
-                acc[path] = 0.0f;
-                acc[cap + path] = 0.0f;
-                acc[2 * cap + path] = 0.0f;
-            }
+        if (a.jobs.num_jobs == 1u) {
+            mask = shade_entry(a, a.jobs.rec[0], i, path, path, primary);   // uniform record: scalar loads
         } else {
-            o = v3(a.qin.ox[i], a.qin.oy[i], a.qin.oz[i]);
-            d = v3(a.qin.dx[i], a.qin.dy[i], a.qin.dz[i]);
-            beta = v3(a.ps.bx[path], a.ps.by[path], a.ps.bz[path]);
-            pdf_prev = a.ps.pdf[path];
-        }
-        const HitRec hit = a.hits.rec[i];
-        const uint32_t leaf = hit.leaf;
-        const bool want_depth = primary && a.depth_out != nullptr && sample == a.depth_sample;
-        if (leaf == kLeafMiss) {
-            V3 ls = sky_eval(a.sky, d);
-            float w = 1.0f;
-            if (!primary && a.sky_nee) w = power_heuristic(pdf_prev, sky_pdf(a.sky, d));
-            V3 c = (beta * ls) * w;
-            a.ps.acc[path] = c.x;
-            a.ps.acc[cap + path] = c.y;
-            a.ps.acc[2 * cap + path] = c.z;
-            if (want_depth) a.depth_out[pixel] = 0.0f;
-        } else {
-            const float t = hit.t, hu = hit.u, hv = hit.v;
-            P = o + d * t;
-            const float4* lsp = reinterpret_cast<const float4*>(a.leaf_shade + leaf);
-            const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
-            V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
-            const uint32_t mat_id = __float_as_uint(s2.y);
-            V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
-            Mat m = load_mat(a.materials[mat_id]);
-            if (m.two_sided && dot(d, N) > 0.0f) N = -N;
-            V3 V = normalize(o - P);
-            Frame fr = make_frame(N);
-            Pcg rng = pcg_keyed(a.wave.seed, pixel, sample, a.bounce);
-            float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
-            for (uint32_t l = 0; l < nl; ++l) {
-                const LightRec& L = a.lights[l];
-                V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
-                V3 ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
-                V3 f = bsdf_eval(fr, m, V, ldir);
-                V3 c = (le * f) * beta;
-                if (!any_pos(c)) continue;
-                if (L.sm_index < 0) {
-                    float* acc = a.ps.acc + (size_t)(l + 1u) * 3u * cap;
-                    acc[path] = acc[path] + c.x;
-                    acc[cap + path] = acc[cap + path] + c.y;
-                    acc[2 * cap + path] = acc[2 * cap + path] + c.z;
-                    continue;
-                }
-                const size_t s = (size_t)l * cap + i;
-                a.st.cx[s] = c.x; a.st.cy[s] = c.y; a.st.cz[s] = c.z;
-                mask |= 2u << l;
-            }
-            // sky NEE only where the next segment is sampled too (the reference sees the
-            // sky only through a traced segment that escapes)
-            const bool extend = a.bounce < a.last_bounce && !(max3(beta) < kThroughputMin);
-            if (a.sky_nee && extend) {
-                V3 wi;
-                float pl;
-                if (sky_sample(a.sky, u0, u1, &wi, &pl)) {
-                    V3 f = bsdf_eval(fr, m, V, wi);
-                    if (any_pos(f)) {
-                        V3 hl = to_local(fr, normalize(V + wi));
-                        float pb = vndf_pdf_l(hl, to_local(fr, V), m.rx, m.ry);
-                        float w = power_heuristic(pl, pb);
-                        V3 c = ((sky_eval(a.sky, wi) * f) * (w / pl)) * beta;
-                        if (any_pos(c) && finite3(c)) {
-                            const size_t s = (size_t)nl * cap + i;
-                            a.st.cx[s] = c.x; a.st.cy[s] = c.y; a.st.cz[s] = c.z;
-                            a.st.sx[i] = wi.x; a.st.sy[i] = wi.y; a.st.sz[i] = wi.z;
-                            mask |= 2u << kMaxDirLights;
-                        }
-                    }
-                }
-            }
-            if (extend) {
-                V3 vl = to_local(fr, V);
-                V3 hh = vndf_sample(vl, m.rx, m.ry, u2, u3);
-                float hvd = dot(vl, hh);
-                V3 ll = hh * (2.0f * hvd) - vl;
-                float pdf = vndf_pdf_l(hh, vl, m.rx, m.ry);
-                V3 Lext = to_world(fr, ll);
-                V3 f = bsdf_eval(fr, m, V, Lext);
-                V3 w = v3(f.x / pdf, f.y / pdf, f.z / pdf);
-                if (!finite3(w)) w = v3(0.0f, 0.0f, 0.0f);
-                V3 nb = beta * w;
-                if (max3(nb) > 0.0f) {
-                    a.st.ex[i] = Lext.x; a.st.ey[i] = Lext.y; a.st.ez[i] = Lext.z;
-                    a.ps.bx[path] = nb.x; a.ps.by[path] = nb.y; a.ps.bz[path] = nb.z;
-                    a.ps.pdf[path] = pdf;
-                    mask |= kMaskExt;
-                }
-            }
-            if (want_depth) {
-                const float* pv = a.wave.cam.proj_view;
-                float z = ((pv[8] * P.x + pv[9] * P.y) + pv[10] * P.z) + pv[11];
-                float w = ((pv[12] * P.x + pv[13] * P.y) + pv[14] * P.z) + pv[15];
-                a.depth_out[pixel] = z / w;
-            }
-        }
-        if (mask) {
-            a.st.px[i] = P.x; a.st.py[i] = P.y; a.st.pz[i] = P.z;
-            a.st.path[i] = path;
+            const uint32_t jid = job_of(a.jobs.path_base, a.jobs.num_jobs, path);
+            mask = shade_entry(a, a.jobs.rec[jid], i, path, path - a.jobs.path_base[jid], primary);
         }
         a.st.mask[i] = mask;
     }
@@ -273,16 +288,19 @@ __global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
             a.qout.path[e] = path;
         }
         // shadow rays in kind order: lights 0..3, then the sky
+        const JobRec& J = a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, path)];
+        const uint32_t nl = J.num_lights;
         uint32_t o = (uint32_t)(base & 0xffffffffull) + (excl & 0xffffu);
-        for (uint32_t k = 0; k <= a.num_lights; ++k) {
-            const bool sky = k == a.num_lights;
+        for (uint32_t k = 0; k <= nl; ++k) {
+            const bool sky = k == nl;
             if (!(mask & (2u << (sky ? (uint32_t)kMaxDirLights : k)))) continue;
             const size_t s = (size_t)k * a.cap + i;
             a.sq.ox[o] = px; a.sq.oy[o] = py; a.sq.oz[o] = pz;
             if (sky) {
                 a.sq.dx[o] = a.st.sx[i]; a.sq.dy[o] = a.st.sy[i]; a.sq.dz[o] = a.st.sz[i];
             } else {
-                a.sq.dx[o] = a.ldir[k][0]; a.sq.dy[o] = a.ldir[k][1]; a.sq.dz[o] = a.ldir[k][2];
+                const float* ld = J.lights[k].direction;
+                a.sq.dx[o] = ld[0]; a.sq.dy[o] = ld[1]; a.sq.dz[o] = ld[2];
             }
             a.sq.cx[o] = a.st.cx[s]; a.sq.cy[o] = a.st.cy[s]; a.sq.cz[o] = a.st.cz[s];
             a.sq.pk[o] = (path << 3) | (sky ? kKindSky : k);
@@ -291,38 +309,52 @@ __global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
     }
 }
 
-// Per pixel (path order q): running sum += radiance of each sample in sample order.
+// Per pixel slot (job j's slot q in path order): running sum += radiance of each of the
+// job's samples in this wave, in sample order; the radiance of a path is the sum of its
+// kinds in kind order (miss, lights, sky).
 __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= a.npix) return;
-    const uint32_t pixel = pixel_of(q, a.width, a.height);
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+    if (slot >= a.num_pix) return;
+    const uint32_t jid = job_of(a.jobs.pix_base, a.jobs.num_jobs, slot);
+    const JobRec& J = a.jobs.rec[jid];
+    const uint32_t q = slot - J.pix_base;
+    const uint32_t npix = J.npix;
+    const uint32_t pixel = pixel_of(q, J.cam.width, J.cam.height);
+    const uint32_t nslots = J.num_lights + 2u;
+    const uint32_t samples = J.samples;
+    const uint32_t pbase = a.jobs.path_base[jid];
+    float* sums = J.sums;
     const size_t cap = a.ps.cap;
-    V3 sum = v3(a.sums[3 * (size_t)pixel], a.sums[3 * (size_t)pixel + 1], a.sums[3 * (size_t)pixel + 2]);
-    for (uint32_t s = 0; s < a.samples; ++s) {
-        const size_t p = (size_t)s * a.npix + q;
+    V3 sum = v3(sums[3 * (size_t)pixel], sums[3 * (size_t)pixel + 1], sums[3 * (size_t)pixel + 2]);
+    for (uint32_t s = 0; s < samples; ++s) {
+        const size_t p = (size_t)pbase + (size_t)s * npix + q;
         V3 r = v3(0.f, 0.f, 0.f);
-        for (uint32_t k = 0; k < a.num_slots; ++k) {
+        for (uint32_t k = 0; k < nslots; ++k) {
             const float* acc = a.ps.acc + (size_t)k * 3u * cap;
             V3 c = v3(acc[p], acc[cap + p], acc[2 * cap + p]);
             r = k == 0 ? c : r + c;
         }
         sum = sum + r;
     }
-    a.sums[3 * (size_t)pixel] = sum.x;
-    a.sums[3 * (size_t)pixel + 1] = sum.y;
-    a.sums[3 * (size_t)pixel + 2] = sum.z;
+    sums[3 * (size_t)pixel] = sum.x;
+    sums[3 * (size_t)pixel + 1] = sum.y;
+    sums[3 * (size_t)pixel + 2] = sum.z;
 }
 
-__global__ __launch_bounds__(256) void k_output(const float* __restrict__ sums, uint32_t npix, float den,
-                                                float* __restrict__ out) {
-    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-    if (p >= npix) return;
+__global__ __launch_bounds__(256) void k_output(WaveJobs jobs, uint32_t num_pix) {
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+    if (slot >= num_pix) return;
+    const uint32_t jid = job_of(jobs.pix_base, jobs.num_jobs, slot);
+    const JobRec& J = jobs.rec[jid];
+    const uint32_t p = slot - J.pix_base;
+    const float den = J.out_den;
+    const float* sums = J.sums;
     float4 o;
     o.x = sums[3 * (size_t)p] / den;
     o.y = sums[3 * (size_t)p + 1] / den;
     o.z = sums[3 * (size_t)p + 2] / den;
     o.w = 1.0f;
-    reinterpret_cast<float4*>(out)[p] = o;
+    reinterpret_cast<float4*>(J.out_rgba)[p] = o;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
@@ -347,11 +379,13 @@ void launch_compact(const CompactArgs& a, uint32_t capacity, hipStream_t s) {
 }
 
 void launch_resolve(const ResolveArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_resolve, dim3((a.npix + 255u) / 256u), dim3(256), 0, s, a);
+    if (a.num_pix == 0) return;
+    hipLaunchKernelGGL(k_resolve, dim3((a.num_pix + 255u) / 256u), dim3(256), 0, s, a);
 }
 
-void launch_output(const float* sums, uint32_t npix, float den, uint32_t, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_output, dim3((npix + 255u) / 256u), dim3(256), 0, s, sums, npix, den, out);
+void launch_output(const WaveJobs& jobs, uint32_t num_pix, hipStream_t s) {
+    if (num_pix == 0) return;
+    hipLaunchKernelGGL(k_output, dim3((num_pix + 255u) / 256u), dim3(256), 0, s, jobs, num_pix);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {

@@ -208,6 +208,14 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Pointer held by another lane.
+template <class T>
+__device__ __forceinline__ T* shfl_ptr(T* p, int src) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src), hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
 // Appends the hit leaf children (mask lm over the slots of ch) of every lane, with their
 // box entry distances when qtn is given; returns the new queue length (uniform).
 __device__ __forceinline__ uint32_t append_leaves(uint32_t* qleaf, uint8_t* qown, float* qtn, uint32_t qn,
@@ -235,14 +243,11 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     __shared__ uint8_t s_qown[4][kLeafQueue];
     __shared__ float s_qtn[4][kLeafQueue];
     __shared__ unsigned long long s_key[4][64];
-    const uint32_t count = PRIMARY ? a.wave.num_paths : *a.count_in;
+    const uint32_t count = PRIMARY ? a.jobs.num_paths : *a.count_in;
     const float tmin = kTMin;
-    if (a.bvh.num_tris == 0) {
-        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
-            a.hits.rec[i] = HitRec{a.wave.ray_length, kLeafMiss, 0.0f, 0.0f};
-        }
-        return;
-    }
+    // Each lane keeps its ray's scene (BVH nodes, leaf records, triangle count): a wave
+    // may hold rays of several jobs.  A job without triangles points at a one-node tree
+    // whose slots are all empty.
     // Each iteration: (1) lanes with a node to visit test its four child boxes, push the
     // hit internal children but the nearest, queue the hit leaves and move to the
     // nearest (a lane with nothing left is parked until its queued leaves are tested);
@@ -263,10 +268,13 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     float tbest = 0.0f;
     uint32_t best = kLeafMiss;
+    const Bvh4Node* nodes = a.jobs.rec[0].nodes;
+    const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
+    uint32_t max_steps = 1;
     ShortStack<L> st;
     int sp = 0;
     uint32_t steps = 0;
-    Node4 cur = load_node(a.bvh.nodes, 0);
+    Node4 cur = load_node(nodes, 0);
     for (;;) {
         int next = kDone;
         uint32_t lm = 0;
@@ -303,7 +311,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
             else if (sp > 0) next = st.pop(lds, sp);
             // a ray enters each node at most once: more steps than nodes means corrupt
             // links, and the ray is ended instead of looping
-            if (++steps > a.bvh.num_tris) {
+            if (++steps > max_steps) {
                 atomicOr(a.error, 8u);
                 next = kDone;
             }
@@ -325,12 +333,13 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
                 const V3 oo = v3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
                 const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
                 const float tb = __shfl(tbest, owner);
+                const LeafGeo* og = shfl_ptr(geo, owner);
                 if (valid) {
                     // a hit counts only where the leaf's own box passes the slab test up to
                     // the hit distance (the acceptance rule that makes the closest hit
                     // independent of the order leaves are found in)
                     float t, un, vn, det;
-                    if (tri_test(a.bvh.leaf_geo, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tb &&
+                    if (tri_test(og, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tb &&
                         tnl <= t * kRobust)
                         atomicMin(&key[owner], ((unsigned long long)__float_as_uint(t) << 32) | leaf);
                 }
@@ -354,7 +363,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
             float u = 0.0f, v = 0.0f;
             if (best != kLeafMiss) {
                 float t, un, vn, det;
-                tri_test(a.bvh.leaf_geo, best, o, d, &t, &un, &vn, &det);
+                tri_test(geo, best, o, d, &t, &un, &vn, &det);
                 u = un / det;
                 v = vn / det;
             }
@@ -366,30 +375,47 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
         if (refill_now) got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted, wc);
         const bool fresh = got != kNoRay;
         V3 no = o, nd = d;
+        float tmax = 0.0f;
         if (fresh) {
-            if (PRIMARY) {
-                const uint32_t q = got % a.wave.npix, sl = got / a.wave.npix;
-                const uint32_t pixel = pixel_of(q, a.wave.cam.width, a.wave.cam.height);
-                float jx = 0.5f, jy = 0.5f;
-                if (a.wave.flags & 1u) {
-                    Pcg jr = pcg_keyed(a.wave.seed, pixel, a.wave.sample0 + sl, kJitterBounce);
-                    jx = pcg_float(jr);
-                    jy = pcg_float(jr);
+            // one job: its record sits at a uniform address (scalar loads); several: the
+            // path's job is looked up per lane
+            auto setup = [&](const JobRec& J, uint32_t local) {
+                nodes = J.nodes;
+                geo = J.leaf_geo;
+                max_steps = J.num_tris > 1u ? J.num_tris : 1u;
+                tmax = J.ray_length;
+                if (PRIMARY) {
+                    const uint32_t q = local % J.npix, sl = local / J.npix;
+                    const uint32_t pixel = pixel_of(q, J.cam.width, J.cam.height);
+                    float jx = 0.5f, jy = 0.5f;
+                    if (J.flags & 1u) {
+                        Pcg jr = pcg_keyed(J.seed, pixel, J.sample0 + sl, kJitterBounce);
+                        jx = pcg_float(jr);
+                        jy = pcg_float(jr);
+                    }
+                    camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, jx, jy, &no, &nd);
                 }
-                camera_ray(a.wave.cam, pixel % a.wave.cam.width, pixel / a.wave.cam.width, jx, jy, &no, &nd);
+            };
+            if (a.jobs.num_jobs == 1u) {
+                setup(a.jobs.rec[0], got);
             } else {
+                const uint32_t path = PRIMARY ? got : a.q.path[got];
+                const uint32_t jid = job_of(a.jobs.path_base, a.jobs.num_jobs, path);
+                setup(a.jobs.rec[jid], path - a.jobs.path_base[jid]);
+            }
+            if (!PRIMARY) {
                 no = v3(a.q.ox[got], a.q.oy[got], a.q.oz[got]);
                 nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
             }
             next = 0;
         }
-        cur = load_node(a.bvh.nodes, next != kDone ? next : 0);
+        cur = load_node(nodes, next != kDone ? next : 0);
         if (fresh) {
             ray = got;
             o = no;
             d = nd;
             inv = safe_inv(nd);
-            tbest = a.wave.ray_length;
+            tbest = tmax;
             best = kLeafMiss;
             sp = 0;
             steps = 0;
@@ -447,14 +473,23 @@ __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3
 __device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bool occ) {
     const uint32_t pk = a.q.pk[i];
     const uint32_t path = pk >> 3, kind = pk & 7u;
+    float keep = 1.0f;
+    uint32_t nl;
+    if (a.jobs.num_jobs == 1u) {   // uniform record address: scalar loads
+        nl = a.jobs.rec[0].num_lights;
+        if (occ && kind != kKindSky) keep = a.jobs.rec[0].keep[kind];
+    } else {
+        const JobRec& J = a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, path)];
+        nl = J.num_lights;
+        if (occ && kind != kKindSky) keep = J.keep[kind];
+    }
     V3 c = v3(a.q.cx[i], a.q.cy[i], a.q.cz[i]);
     if (occ) {
         if (kind == kKindSky) return;
-        float keep = a.keep[kind];
         if (!(keep > 0.0f)) return;
         c = c * keep;
     }
-    const uint32_t slot = kind == kKindSky ? a.num_lights + 1u : kind + 1u;
+    const uint32_t slot = kind == kKindSky ? nl + 1u : kind + 1u;
     const size_t cap = a.ps.cap;
     float* acc = a.ps.acc + (size_t)slot * 3u * cap;
     acc[path] = acc[path] + c.x;
@@ -470,10 +505,6 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     __shared__ uint32_t s_occ[4][64];
     const uint32_t count = *a.count_in;
     const float tmin = kTMin, tmax = kShadowTMax;
-    if (a.bvh.num_tris == 0) {
-        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) apply_shadow(a, i, false);
-        return;
-    }
     // Same iteration as the closest-hit kernel: box steps queue their leaves, the queue
     // is flushed when full, before a refill or when no lane is busy, and parked lanes
     // finish once their leaves are tested.  The first hit internal child is entered and
@@ -489,10 +520,13 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     bool parked = false, exhausted = false;
     WorkCursor wc = work_cursor(count, a.parts);
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
+    const Bvh4Node* nodes = a.jobs.rec[0].nodes;   // the lane's ray's scene (see k_trace_closest)
+    const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
+    uint32_t max_steps = 1;
     ShortStack<L> st;
     int sp = 0;
     uint32_t steps = 0;
-    Node4 cur = load_node(a.bvh.nodes, 0);
+    Node4 cur = load_node(nodes, 0);
     for (;;) {
         int next = kDone;
         uint32_t lm = 0;
@@ -521,7 +555,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
             } else if (next == kDone && sp > 0) {
                 next = st.pop(lds, sp);
             }
-            if (++steps > a.bvh.num_tris) {   // corrupt links: end the ray instead of looping
+            if (++steps > max_steps) {   // corrupt links: end the ray instead of looping
                 atomicOr(a.error, 8u);
                 next = kDone;
             }
@@ -542,9 +576,10 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
                 const int owner = valid ? (int)qown[e] : (int)lane;
                 const V3 oo = v3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
                 const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
+                const LeafGeo* og = shfl_ptr(geo, owner);
                 if (valid) {
                     float t, un, vn, det;
-                    if (tri_test(a.bvh.leaf_geo, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
+                    if (tri_test(og, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
                         occf[owner] = 1u;
                 }
             }
@@ -563,11 +598,18 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
         const bool fresh = got != kNoRay;
         V3 no = o, nd = d;
         if (fresh) {
+            auto setup = [&](const JobRec& J) {
+                nodes = J.nodes;
+                geo = J.leaf_geo;
+                max_steps = J.num_tris > 1u ? J.num_tris : 1u;
+            };
+            if (a.jobs.num_jobs == 1u) setup(a.jobs.rec[0]);
+            else setup(a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, a.q.pk[got] >> 3)]);
             no = v3(a.q.ox[got], a.q.oy[got], a.q.oz[got]);
             nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
             next = 0;
         }
-        cur = load_node(a.bvh.nodes, next != kDone ? next : 0);
+        cur = load_node(nodes, next != kDone ? next : 0);
         if (fresh) {
             ray = got;
             o = no;

@@ -117,7 +117,8 @@ enum {
     BI_PT_FLAG_JITTER = 1u << 0,         /* jitter primary rays inside the pixel (reference: pixel centre) */
     BI_PT_FLAG_NO_SKY_NEE = 1u << 1,     /* sky only on BSDF-sampled misses, as the reference */
     BI_PT_FLAG_RECORD_QUEUES = 1u << 8,  /* keep every bounce's queue contents for bi_pt_debug_queue */
-    BI_PT_FLAG_TIMING = 1u << 9          /* time each kernel class with HIP events (bi_pt_get_stats) */
+    BI_PT_FLAG_TIMING = 1u << 9,         /* time each kernel class with HIP events (bi_pt_get_stats) */
+    BI_PT_FLAG_REUSE_BVH = 1u << 10      /* keep the built LBVH while the scene and transforms are unchanged */
 };
 
 /* PathTracingSettings (basic.hpp:76-81) plus the sample schedule.
@@ -184,6 +185,46 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
                           float* out_rgba32f, float* out_depth, bi_pt_stream stream);
 bi_pt_status bi_pt_reset_accumulation(bi_pt_ctx* ctx);
 bi_pt_status bi_pt_get_stats(const bi_pt_ctx* ctx, bi_pt_stats* out);
+
+/* ---- batched rendering (no reference counterpart: the reference renders one view per
+ * pass invocation; this serves several render jobs at once) ----
+ * A batch renders the jobs of several contexts in one set of waves: their ray queues
+ * are shared, every kernel launch carries rays of all jobs.  Each job is exactly what
+ * bi_pt_render(job.ctx, job.camera, job.settings, ...) would produce, bit for bit
+ * (samples are added to the context's running image in the same order).  A context
+ * may appear once per call.  The queue and traversal knobs of the jobs' settings are
+ * ignored: the batch's tuning applies. */
+typedef struct bi_pt_batch bi_pt_batch;
+typedef struct bi_pt_job {
+    bi_pt_ctx* ctx;                    /* scene, lights, sky and running image of the job */
+    const bi_pt_camera* camera;
+    const bi_pt_settings* settings;
+    float* out_rgba32f;                /* device, width*height*4 */
+    float* out_depth;                  /* device, width*height, nullable */
+} bi_pt_job;
+typedef struct bi_pt_tuning {
+    uint32_t wave_paths;    /* paths in flight per wave (0 = automatic) */
+    uint32_t stack_lds;     /* traversal stack entries in LDS (0 = automatic, <= 32) */
+    uint32_t refill_lanes;  /* idle lanes before a wavefront fetches rays (0 = automatic, <= 64) */
+    uint32_t work_parts;    /* ray-queue parts with their own fetch counter (0 = automatic, <= 32) */
+    uint32_t use_graph;     /* capture a call's launches in a hipGraph and replay it while the
+                               jobs' shapes (sizes, spp, bounces, lights, scenes) repeat */
+} bi_pt_tuning;
+bi_pt_status bi_pt_batch_create(const bi_pt_device_desc* desc, bi_pt_batch** out_batch);
+bi_pt_status bi_pt_batch_destroy(bi_pt_batch* batch);
+const char* bi_pt_batch_last_error(const bi_pt_batch* batch);
+bi_pt_status bi_pt_batch_set_tuning(bi_pt_batch* batch, const bi_pt_tuning* tuning);
+bi_pt_status bi_pt_batch_get_tuning(const bi_pt_batch* batch, bi_pt_tuning* tuning);
+/* Renders every job; stream-ordered.  BI_PT_FLAG_REUSE_BVH in a job's flags keeps that
+ * context's built LBVH; BI_PT_FLAG_TIMING / RECORD_QUEUES are ignored here. */
+bi_pt_status bi_pt_render_batch(bi_pt_batch* batch, const bi_pt_job* jobs, uint32_t num_jobs, bi_pt_stream stream);
+/* Ray statistics of the batch (all its calls); checks the device error flags. */
+bi_pt_status bi_pt_batch_get_stats(bi_pt_batch* batch, bi_pt_stats* out);
+/* Picks the batch's tuning for these jobs by timing candidate knob settings (wave
+ * capacity, LDS stack depth, refill threshold) on them, `rounds` calls each; writes the
+ * best to *best (nullable) and installs it.  The jobs' running images are restarted. */
+bi_pt_status bi_pt_batch_autotune(bi_pt_batch* batch, const bi_pt_job* jobs, uint32_t num_jobs, uint32_t rounds,
+                                  bi_pt_stream stream, bi_pt_tuning* best);
 
 /* ---- inspection hooks for parity tests (no reference counterpart) ----
  * Copy the last-built LBVH to host memory: morton[n] (input triangle order),
