@@ -31,7 +31,9 @@ EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
-           "bi_pt_batch_autotune"]
+           "bi_pt_batch_autotune", "bi_pt_create_view", "bi_pt_server_create", "bi_pt_server_destroy",
+           "bi_pt_server_last_error", "bi_pt_server_clock", "bi_pt_server_add_scene", "bi_pt_server_submit",
+           "bi_pt_server_step", "bi_pt_server_poll", "bi_pt_server_take_result", "bi_pt_server_get_stats"]
 
 
 class PtError(RuntimeError):
@@ -75,6 +77,26 @@ class Tuning(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_jobs_per_round", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
+                ("spp_per_round", C.c_uint32), ("wave_paths", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene_id", C.c_uint32), ("_pad", C.c_uint32), ("arrival_s", C.c_double), ("camera", CameraC),
+                ("settings", Settings)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("spp_done", C.c_uint32), ("submit_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("rounds", C.c_uint64), ("waiting", C.c_uint32), ("running", C.c_uint32),
+                ("hbm_used", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -127,8 +149,22 @@ def lib():
         L.bi_pt_render_batch.argtypes = [vp, C.POINTER(JobC), u32, vp]
         L.bi_pt_batch_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.bi_pt_batch_autotune.argtypes = [vp, C.POINTER(JobC), u32, u32, vp, C.POINTER(Tuning)]
+        L.bi_pt_create_view.argtypes = [vp, C.POINTER(vp)]
+        L.bi_pt_server_create.argtypes = [C.POINTER(ServerDesc), C.POINTER(vp)]
+        L.bi_pt_server_destroy.argtypes = [vp]
+        L.bi_pt_server_last_error.argtypes = [vp]
+        L.bi_pt_server_last_error.restype = C.c_char_p
+        L.bi_pt_server_clock.argtypes = [vp]
+        L.bi_pt_server_clock.restype = C.c_double
+        L.bi_pt_server_add_scene.argtypes = [vp, vp, u32p]
+        L.bi_pt_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.bi_pt_server_step.argtypes = [vp, vp, u32p]
+        L.bi_pt_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(JobStatus)]
+        L.bi_pt_server_take_result.argtypes = [vp, C.c_uint64, f32p, C.c_uint64]
+        L.bi_pt_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
         for name in EXPORTS:
-            if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris", "bi_pt_batch_last_error"):
+            if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris", "bi_pt_batch_last_error",
+                            "bi_pt_server_last_error", "bi_pt_server_clock"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -150,16 +186,24 @@ def camera_struct(cam) -> CameraC:
 class PathTracer:
     """One bi_pt context on one GPU."""
 
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, view_of=None):
+        """A context on `device`; with view_of=<PathTracer>, a view of that tracer's scene."""
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("bi_pt needs a ROCm GPU; none is visible")
         self.torch = torch
         self.device = device
         h = C.c_void_p()
-        st = lib().bi_pt_create(C.byref(DeviceDesc(device, 0, hbm_budget_bytes)), C.byref(h))
-        if st != 0:
-            raise PtError(st, "bi_pt_create failed")
+        if view_of is not None:
+            self.device = view_of.device
+            st = lib().bi_pt_create_view(view_of.h, C.byref(h))
+            if st != 0:
+                raise PtError(st, lib().bi_pt_last_error(view_of.h).decode())
+            self._scene_owner = view_of   # the scene context outlives the view
+        else:
+            st = lib().bi_pt_create(C.byref(DeviceDesc(device, 0, hbm_budget_bytes)), C.byref(h))
+            if st != 0:
+                raise PtError(st, "bi_pt_create failed")
         self.h = h
         self._out = None
         self._depth = None
@@ -389,3 +433,70 @@ class Batch:
         s = Stats()
         self._check(lib().bi_pt_batch_get_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+
+class Server:
+    """bi_pt_server: the serving loop (admission by HBM budget, continuous batching)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_jobs_per_round=0, spp_per_round=0, wave_paths=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("bi_pt needs a ROCm GPU; none is visible")
+        h = C.c_void_p()
+        st = lib().bi_pt_server_create(C.byref(ServerDesc(device, max_jobs_per_round, hbm_budget_bytes,
+                                                          spp_per_round, wave_paths)), C.byref(h))
+        if st != 0:
+            raise PtError(st, "bi_pt_server_create failed")
+        self.h = h
+        self.scenes = []
+
+    def _check(self, st):
+        if st != 0:
+            raise PtError(st, lib().bi_pt_server_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().bi_pt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self):
+        return lib().bi_pt_server_clock(self.h)
+
+    def add_scene(self, tracer):
+        sid = C.c_uint32()
+        self._check(lib().bi_pt_server_add_scene(self.h, tracer.h, C.byref(sid)))
+        self.scenes.append(tracer)
+        return sid.value
+
+    def submit(self, scene_id, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, arrival_s=-1.0):
+        r = Request(scene_id, 0, arrival_s, camera_struct(cam),
+                    Settings(ray_length, max_bounces, 0, spp, seed, flags, 0, 0, 0, 0))
+        jid = C.c_uint64()
+        self._check(lib().bi_pt_server_submit(self.h, C.byref(r), C.byref(jid)))
+        return jid.value
+
+    def step(self, stream=None):
+        act = C.c_uint32()
+        self._check(lib().bi_pt_server_step(self.h, C.c_void_p(stream) if stream else None, C.byref(act)))
+        return act.value
+
+    def poll(self, job_id):
+        s = JobStatus()
+        self._check(lib().bi_pt_server_poll(self.h, job_id, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def take_result(self, job_id, npix):
+        out = np.zeros((npix, 4), np.float32)
+        self._check(lib().bi_pt_server_take_result(self.h, job_id, _p(out), out.size))
+        return out
+
+    def stats(self):
+        s = ServerStats()
+        self._check(lib().bi_pt_server_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}

@@ -174,6 +174,7 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
                              const bi_pt_instance* instances, uint32_t num_instances,
                              const bi_pt_material* materials, uint32_t num_materials) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
     if ((num_meshes && !meshes) || (num_instances && !instances) || (num_materials && !materials))
         return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null scene array");
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -244,11 +245,13 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
     ctx->has_scene = true;
     ctx->bvh_built = false;
     ctx->acc_count = 0;
+    ctx->scene_version++;
     return BI_PT_OK;
 }
 
 bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* xf, uint32_t num_instances) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     if (num_instances != ctx->num_inst || (num_instances && !xf))
         return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "instance count differs from the scene");
@@ -261,11 +264,13 @@ bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* xf, uint32_t n
                           hipMemcpyHostToDevice));
     ctx->bvh_built = false;
     ctx->acc_count = 0;
+    ctx->scene_version++;
     return BI_PT_OK;
 }
 
 bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uint32_t num_lights) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
     if (num_lights > (uint32_t)kMaxDirLights)
         return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 4 directional lights");
     if (num_lights && !lights) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null lights");
@@ -274,13 +279,17 @@ bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uin
     const bool changed = next.size() != ctx->lights.size() ||
                          (num_lights && std::memcmp(next.data(), ctx->lights.data(), num_lights * sizeof(LightRec)) != 0);
     ctx->lights.swap(next);
-    if (changed) ctx->acc_count = 0;   // unchanged lights (set every frame) keep the running image
+    if (changed) {   // unchanged lights (set every frame) keep the running image
+        ctx->acc_count = 0;
+        ctx->scene_version++;
+    }
     return BI_PT_OK;
 }
 
 bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, const float color[3],
                            const float rot[9]) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
     if ((w == 0) != (h == 0) || ((w || h) && !rgba) || !color || !rot)
         return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "bad sky arguments");
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -321,6 +330,22 @@ bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32
     }
     ctx->sky = sp;
     ctx->acc_count = 0;
+    ctx->scene_version++;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_create_view(bi_pt_ctx* scene, bi_pt_ctx** out_view) {
+    if (!scene || !out_view) return BI_PT_ERROR_INVALID_ARGUMENT;
+    *out_view = nullptr;
+    if (scene->scene_src) return fail(scene, BI_PT_ERROR_INVALID_ARGUMENT, "a view of a view");
+    bi_pt_device_desc d{};
+    d.device = scene->device;
+    d.hbm_budget_bytes = scene->budget;
+    bi_pt_ctx* v = nullptr;
+    BI_TRY(bi_pt_create(&d, &v));
+    v->scene_src = scene;
+    v->has_scene = true;
+    *out_view = v;
     return BI_PT_OK;
 }
 
@@ -355,6 +380,7 @@ bi_pt_status bi_pt_get_stats(const bi_pt_ctx* cctx, bi_pt_stats* out) {
 
 bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* order, int32_t* children,
                                  float* child_bounds, int32_t* parents, float* centroid_bounds) {
+    if (ctx && ctx->scene_src) ctx = ctx->scene_src;   // a view is inspected through its scene
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -386,6 +412,7 @@ bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* ord
 }
 
 bi_pt_status bi_pt_debug_get_bvh4(bi_pt_ctx* ctx, int32_t* child4, float* box4) {
+    if (ctx && ctx->scene_src) ctx = ctx->scene_src;   // a view is inspected through its scene
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -403,7 +430,10 @@ bi_pt_status bi_pt_debug_get_bvh4(bi_pt_ctx* ctx, int32_t* child4, float* box4) 
     return BI_PT_OK;
 }
 
-uint32_t bi_pt_debug_num_tris(const bi_pt_ctx* ctx) { return ctx ? ctx->num_tris : 0; }
+uint32_t bi_pt_debug_num_tris(const bi_pt_ctx* ctx) {
+    if (ctx && ctx->scene_src) ctx = ctx->scene_src;
+    return ctx ? ctx->num_tris : 0;
+}
 
 bi_pt_status bi_pt_debug_queue(bi_pt_ctx* ctx, uint32_t wave, uint32_t bounce, uint32_t kind, uint32_t* out,
                                uint32_t capacity, uint32_t* count) {
@@ -428,6 +458,7 @@ bi_pt_status bi_pt_debug_get_sums(bi_pt_ctx* ctx, float* sums) {
 
 bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, int any_hit, float* hit_t,
                                uint32_t* hit_leaf, float* hit_u, float* hit_v) {
+    if (ctx && ctx->scene_src) ctx = ctx->scene_src;   // a view is inspected through its scene
     if (!ctx || (n && (!rays || !hit_leaf))) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     if (n == 0) return BI_PT_OK;

@@ -83,6 +83,19 @@ bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out) {
     return BI_PT_OK;
 }
 
+uint64_t engine_bytes(uint32_t cap, uint32_t slots, uint32_t sh_mult) {
+    const uint64_t c = cap, tiles = (cap + 255u) / 256u + 1u;
+    uint64_t b = 0;
+    b += 12 * c * 4 + 2 * c * 4;                 // queues A and B: 6 floats + path id each
+    b += c * sizeof(HitRec);                     // hit records
+    b += 4 * c * 4;                              // path state
+    b += (uint64_t)slots * 3 * c * 4;            // radiance accumulators
+    b += (uint64_t)sh_mult * c * (9 * 4 + 4);    // shadow queue
+    b += 9 * c * 4 + 3 * (uint64_t)(slots - 1) * c * 4 + 2 * c * 4;   // shade staging
+    b += tiles * (8 + 4);                        // tile totals
+    return b;
+}
+
 namespace {
 
 bi_pt_status ensure_wave_buffers(Owner* own, Engine& eng, uint32_t cap, uint32_t slots, uint32_t sh_mult) {
@@ -120,7 +133,8 @@ RayQueue queue_of(DevBuf<float>* f, DevBuf<uint32_t>& path) {
 
 // Per-job facts of a call.
 struct Prep {
-    bi_pt_ctx* ctx;
+    bi_pt_ctx* ctx;       // running image
+    bi_pt_ctx* scene;     // scene, lights, sky, BVH
     CameraParams cam;
     uint32_t npix, M, D, nl, spp, sh;
     bool sky_nee, reset, rebuild;
@@ -154,7 +168,8 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
             return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "null context/camera/settings/output in job " + std::to_string(j));
         for (uint32_t k = 0; k < j; ++k)
             if (jobs[k].ctx == ctx) return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "a context appears twice in one call");
-        if (!ctx->has_scene) return fail(own, BI_PT_ERROR_NO_SCENE, "no scene set (job " + std::to_string(j) + ")");
+        bi_pt_ctx* S = scene_of(ctx);
+        if (!S->has_scene) return fail(own, BI_PT_ERROR_NO_SCENE, "no scene set (job " + std::to_string(j) + ")");
         const bi_pt_camera* c = J.camera;
         const bi_pt_settings* st = J.st;
         if (c->width == 0 || c->height == 0 || st->spp == 0)
@@ -175,18 +190,22 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         p.npix = (uint32_t)npix64;
         p.M = std::min(std::max(st->max_bounces, 2u), 16u);
         p.D = p.M - 1;
-        p.nl = (uint32_t)ctx->lights.size();
+        p.scene = S;
+        p.nl = (uint32_t)S->lights.size();
         p.spp = st->spp;
-        p.sky_nee = !(st->flags & BI_PT_FLAG_NO_SKY_NEE) && ctx->sky.total > 0.0f;
+        p.sky_nee = !(st->flags & BI_PT_FLAG_NO_SKY_NEE) && S->sky.total > 0.0f;
         p.sh = 1;
-        for (const LightRec& L : ctx->lights) p.sh += L.sm_index >= 0 ? 1u : 0u;
+        for (const LightRec& L : S->lights) p.sh += L.sm_index >= 0 ? 1u : 0u;
         // accumulation history: continue only with the same view, size and sampling settings
         const bool same = ctx->acc_count > 0 && st->accumulate && ctx->acc_w == p.cam.width &&
                           ctx->acc_h == p.cam.height && std::memcmp(&ctx->last_cam, &p.cam, sizeof p.cam) == 0 &&
                           ctx->last_seed == st->seed && ctx->last_flags == (st->flags & 3u) &&
-                          ctx->last_bounces == p.M && ctx->last_ray_length == st->ray_length;
+                          ctx->last_bounces == p.M && ctx->last_ray_length == st->ray_length &&
+                          ctx->last_version == S->scene_version;
         p.reset = !same;
-        p.rebuild = !(opt.reuse_bvh && ctx->bvh_built);
+        p.rebuild = !(opt.reuse_bvh && S->bvh_built);
+        for (uint32_t k = 0; k < j; ++k)
+            if (prep[k].scene == S) p.rebuild = false;   // one build per scene and call
         max_npix = std::max(max_npix, p.npix);
         slots = std::max(slots, p.nl + 2u);
         sh_mult = std::max(sh_mult, p.sh);
@@ -230,8 +249,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     for (uint32_t j = 0; j < n; ++j) {
         bi_pt_ctx* ctx = prep[j].ctx;
         bi_pt_status e = dalloc(ctx, ctx->sums, (size_t)3 * prep[j].npix);
-        if (e == BI_PT_OK && prep[j].rebuild) e = ensure_build_buffers(ctx);
         if (e != BI_PT_OK) return fail(own, e, ctx->err);
+        if (prep[j].rebuild && (e = ensure_build_buffers(prep[j].scene)) != BI_PT_OK)
+            return fail(own, e, prep[j].scene->err);
     }
     BI_TRY(ensure_wave_buffers(own, eng, wave_max, slots, sh_mult));
     const size_t rec_align = alignof(JobRec);
@@ -258,20 +278,21 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     auto fill_rec = [&](JobRec& r, uint32_t j, uint32_t sample_off, uint32_t samples, uint32_t pix_base) {
         const Prep& p = prep[j];
         bi_pt_ctx* ctx = p.ctx;
-        const bool empty = ctx->num_tris == 0;
-        r.nodes = empty ? eng.empty_tree.p : ctx->nodes4.p;
-        r.leaf_geo = empty ? nullptr : ctx->leaf_geo.p;
-        r.leaf_shade = empty ? nullptr : ctx->leaf_shade.p;
-        r.materials = ctx->mats.p;
-        r.num_tris = ctx->num_tris;
+        bi_pt_ctx* S = p.scene;
+        const bool empty = S->num_tris == 0;
+        r.nodes = empty ? eng.empty_tree.p : S->nodes4.p;
+        r.leaf_geo = empty ? nullptr : S->leaf_geo.p;
+        r.leaf_shade = empty ? nullptr : S->leaf_shade.p;
+        r.materials = S->mats.p;
+        r.num_tris = S->num_tris;
         r.num_lights = p.nl;
         r.sky_nee = p.sky_nee ? 1u : 0u;
         r.last_bounce = p.D - 1;
         for (uint32_t l = 0; l < p.nl; ++l) {
-            r.lights[l] = ctx->lights[l];
-            r.keep[l] = 1.0f - ctx->lights[l].shadow_strength;
+            r.lights[l] = S->lights[l];
+            r.keep[l] = 1.0f - S->lights[l].shadow_strength;
         }
-        r.sky = ctx->sky;
+        r.sky = S->sky;
         r.cam = p.cam;
         r.npix = p.npix;
         r.sample0 = (uint32_t)(p.reset ? 0 : ctx->acc_count) + sample_off;
@@ -344,9 +365,10 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                eng.wave_sh_mult, plan_bytes};
         for (uint32_t j = 0; j < n; ++j) {
             const Prep& p = prep[j];
-            key.insert(key.end(), {(uint64_t)(uintptr_t)p.ctx, (uint64_t)(uintptr_t)p.ctx->nodes4.p,
-                                   (uint64_t)(uintptr_t)p.ctx->sums.p, p.npix, p.spp, p.M, p.nl, p.sh,
-                                   (uint64_t)p.reset, (uint64_t)p.rebuild, p.ctx->num_tris});
+            key.insert(key.end(), {(uint64_t)(uintptr_t)p.ctx, (uint64_t)(uintptr_t)p.scene,
+                                   (uint64_t)(uintptr_t)p.scene->nodes4.p, (uint64_t)(uintptr_t)p.ctx->sums.p,
+                                   p.npix, p.spp, p.M, p.nl, p.sh, (uint64_t)p.reset, (uint64_t)p.rebuild,
+                                   p.scene->num_tris});
         }
         if (graph_owner->graph_exec && key == graph_owner->graph_key) {
             BI_HIP_TRY(own, hipGraphLaunch(graph_owner->graph_exec, s));
@@ -369,9 +391,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[0], s));
         for (uint32_t j = 0; j < n; ++j) {
             if (!prep[j].rebuild) continue;
-            if (bi_pt_status e = build_bvh(prep[j].ctx, s, &cnt[kCntError]); e != BI_PT_OK) {
+            if (bi_pt_status e = build_bvh(prep[j].scene, s, &cnt[kCntError]); e != BI_PT_OK) {
                 if (graph) { hipGraph_t g; (void)hipStreamEndCapture(s, &g); if (g) (void)hipGraphDestroy(g); }
-                return fail(own, e, prep[j].ctx->err);
+                return fail(own, e, prep[j].scene->err);
             }
         }
         if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[1], s));
@@ -557,7 +579,8 @@ finish:
         ctx->last_bounces = p.M;
         ctx->last_ray_length = st->ray_length;
         ctx->acc_count += p.spp;
-        if (p.rebuild) ctx->bvh_built = true;
+        ctx->last_version = p.scene->scene_version;
+        if (p.rebuild) p.scene->bvh_built = true;
         eng.camera_rays += (uint64_t)p.npix * p.spp;
     }
     eng.frames++;

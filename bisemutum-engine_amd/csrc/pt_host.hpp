@@ -113,14 +113,22 @@ struct Engine {
 };
 
 bi_pt_status engine_init(Owner* own, Engine& eng);
+// Device bytes of the wave buffers for waves of `cap` paths, `slots` radiance kinds and
+// `sh_mult` shadow rays per path (what ensure_wave_buffers allocates).
+uint64_t engine_bytes(uint32_t cap, uint32_t slots, uint32_t sh_mult);
 void engine_free(Owner* own, Engine& eng);
 // Error flags of the engine's last calls (synchronises the device).
 bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out);
 
 }  // namespace bipt
 
-// One scene, its per-frame BVH and its running image.
+// One scene, its per-frame BVH and its running image.  A view (bi_pt_create_view)
+// renders the scene of another context (scene_src: scene, lights, sky and BVH are
+// read from there) into a running image of its own.
 struct bi_pt_ctx : bipt::Owner {
+    bi_pt_ctx* scene_src = nullptr;
+    uint64_t scene_version = 0;     // bumped by every change of scene, transforms, lights or sky
+    uint64_t last_version = 0;      // scene version of the running image
     // scene
     bool has_scene = false;
     bipt::DevBuf<float> pos, nrm;
@@ -169,6 +177,9 @@ struct bi_pt_batch : bipt::Owner {
 };
 
 namespace bipt {
+
+// The context that holds the scene a context renders.
+inline bi_pt_ctx* scene_of(bi_pt_ctx* c) { return c->scene_src ? c->scene_src : c; }
 
 bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx);
 bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error);

@@ -161,6 +161,13 @@ typedef struct bi_pt_stats {
 bi_pt_status bi_pt_create(const bi_pt_device_desc* desc, bi_pt_ctx** out_ctx);
 bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx);
 const char* bi_pt_last_error(const bi_pt_ctx* ctx);
+/* A view renders the scene (meshes, instances, materials, lights, sky and LBVH) of
+ * `scene` into a running image of its own: several cameras or render jobs of one
+ * resident scene.  The view's scene inputs cannot be set; changing the scene context's
+ * inputs restarts its views' running images.  The scene context must outlive its views.
+ * (No reference counterpart: the reference keeps one history per camera,
+ * camera.cpp:176-204.) */
+bi_pt_status bi_pt_create_view(bi_pt_ctx* scene, bi_pt_ctx** out_view);
 
 /* ---- inputs (update_params, path_tracing.cpp:183-222, and the TLAS/scene inputs) ---- */
 bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t num_meshes,
@@ -225,6 +232,58 @@ bi_pt_status bi_pt_batch_get_stats(bi_pt_batch* batch, bi_pt_stats* out);
  * best to *best (nullable) and installs it.  The jobs' running images are restarted. */
 bi_pt_status bi_pt_batch_autotune(bi_pt_batch* batch, const bi_pt_job* jobs, uint32_t num_jobs, uint32_t rounds,
                                   bi_pt_stream stream, bi_pt_tuning* best);
+
+/* ---- serving loop (no reference counterpart: the reference renders one camera per frame,
+ * basic.cpp:157-161) ----
+ * Render jobs of uneven scene size, resolution and spp are submitted at any time.  A
+ * job is admitted once the HBM budget has room for its running image and output (the
+ * resident scenes and the wave buffers are counted too); each bi_pt_server_step renders
+ * the next samples of every admitted job in one batched call (shared ray queues), so
+ * jobs join and leave at round boundaries.  A finished job's image equals, bit for bit,
+ * a bi_pt_render of the same camera and settings on a context holding its scene. */
+typedef struct bi_pt_server bi_pt_server;
+typedef struct bi_pt_server_desc {
+    int32_t device;
+    uint32_t max_jobs_per_round;   /* 0 = 64 */
+    uint64_t hbm_budget_bytes;     /* 0 = unlimited */
+    uint32_t spp_per_round;        /* samples of each job per round; 0 = 4 */
+    uint32_t wave_paths;           /* paths in flight per wave; 0 = automatic */
+} bi_pt_server_desc;
+typedef struct bi_pt_request {
+    uint32_t scene_id;             /* from bi_pt_server_add_scene */
+    uint32_t _pad;
+    double arrival_s;              /* arrival time on the server clock; < 0 = now */
+    bi_pt_camera camera;
+    bi_pt_settings settings;       /* spp = the job's total samples; accumulate and queue knobs ignored */
+} bi_pt_request;
+enum { BI_PT_JOB_WAITING = 0, BI_PT_JOB_RUNNING = 1, BI_PT_JOB_DONE = 2 };
+typedef struct bi_pt_job_status {
+    uint32_t state;
+    uint32_t spp_done;
+    double submit_s, admit_s, done_s;   /* server clock, seconds */
+} bi_pt_job_status;
+typedef struct bi_pt_server_stats {
+    uint64_t rays;                 /* all traced rays */
+    uint64_t rounds;
+    uint32_t waiting, running;
+    uint64_t hbm_used;
+} bi_pt_server_stats;
+bi_pt_status bi_pt_server_create(const bi_pt_server_desc* desc, bi_pt_server** out_server);
+bi_pt_status bi_pt_server_destroy(bi_pt_server* server);
+const char* bi_pt_server_last_error(const bi_pt_server* server);
+double bi_pt_server_clock(const bi_pt_server* server);
+/* Makes a context's scene resident (its LBVH is built now and kept); the context must
+ * outlive the server and keep its scene unchanged. */
+bi_pt_status bi_pt_server_add_scene(bi_pt_server* server, bi_pt_ctx* scene, uint32_t* scene_id);
+bi_pt_status bi_pt_server_submit(bi_pt_server* server, const bi_pt_request* request, uint64_t* job_id);
+/* One scheduling round (admission, one batched call, retirement); waits for the round.
+ * *active (nullable) = jobs still waiting or running. */
+bi_pt_status bi_pt_server_step(bi_pt_server* server, bi_pt_stream stream, uint32_t* active);
+bi_pt_status bi_pt_server_poll(bi_pt_server* server, uint64_t job_id, bi_pt_job_status* status);
+/* Copies a finished job's RGBA32F image to host memory (nullable: discard) and frees the job. */
+bi_pt_status bi_pt_server_take_result(bi_pt_server* server, uint64_t job_id, float* host_rgba,
+                                      uint64_t capacity_floats);
+bi_pt_status bi_pt_server_get_stats(bi_pt_server* server, bi_pt_server_stats* out);
 
 /* ---- inspection hooks for parity tests (no reference counterpart) ----
  * Copy the last-built LBVH to host memory: morton[n] (input triangle order),
