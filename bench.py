@@ -5,6 +5,12 @@ A step is one bi_pt_render call: the per-frame LBVH rebuild plus 16 spp of the
 line with Mrays/s (all traced rays: camera, extension, light-shadow, sky-shadow),
 p50/p99 job latency, the trace kernel's roofline figures and the CPU baseline
 (the CPU restatement timed on this host's cores on a bounded sample).
+
+Two more legs report the other forms beside the plain pass: "batched" (64 scenes of
+1k-500k triangles in one bi_pt_render_batch call, autotuned, replayed from a hipGraph,
+against the same jobs rendered one by one) and "serving" (Poisson arrivals over 8
+resident scenes through bi_pt_server, p50/p99 job latency against first-come
+first-served single renders).
 """
 import argparse
 import json
@@ -69,6 +75,10 @@ def main():
     ap.add_argument("--work-parts", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
+    ap.add_argument("--batch-calls", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="arrival window of the serving leg (0 skips it; 20 = the full 4,000-job run)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy first (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -190,10 +200,187 @@ def main():
 
     if not args.no_rmse:
         result["rmse_vs_fp64"] = rmse_leg(pt, scene, cam, cfg)
+    if not args.no_batch:
+        result["batched"] = batch_leg(args.batch_calls)
+    if args.serve_seconds > 0:
+        result["serving"] = serve_leg(args.serve_seconds)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_leg(scene, cfg)
     pt.close()
     print(json.dumps(result), flush=True)
+
+
+RAY_KINDS = ("rays_camera", "rays_extension", "rays_light_shadow", "rays_sky_shadow")
+
+
+def batch_leg(steps):
+    """Multi-scene form: 64 c0-style scenes (1k-500k tris, log-uniform), one job each
+    (256x256 / 640x480 / 1280x720, 1-8 spp, depth 6), LBVHs rebuilt every call.
+    The plain form renders the 64 jobs one after another with bi_pt_render; the
+    batched form renders them in one bi_pt_render_batch call (shared waves) with the
+    autotuned knobs, replayed from a hipGraph."""
+    import torch
+
+    import bi_pt
+    import scenes
+    t = time.perf_counter()
+    specs = scenes.batch_c2()
+    tracers = []
+    for sc, _, _ in specs:
+        tr = bi_pt.PathTracer()
+        tr.set_scene(sc)
+        tracers.append(tr)
+    tris = sum(sc.num_tris for sc, _, _ in specs)
+    paths = sum(cam.width * cam.height * kw["spp"] for _, cam, kw in specs)
+    log(f"batch leg: {len(specs)} scenes, {tris} tris, {paths} paths, set up in {time.perf_counter() - t:.1f}s")
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def rays_of(objs):
+        return sum(sum(o.stats()[k] for k in RAY_KINDS) for o in objs)
+
+    # plain: one bi_pt_render per job, back to back on one stream
+    def plain():
+        for (sc, cam, kw), tr in zip(specs, tracers):
+            tr.render(cam, stream=sh, **kw)
+
+    with torch.cuda.stream(stream):
+        plain()
+        stream.synchronize()
+        r0 = rays_of(tracers)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plain()
+        stream.synchronize()
+        plain_s = (time.perf_counter() - t0) / steps
+        plain_rays = (rays_of(tracers) - r0) // steps
+
+        b = bi_pt.Batch()
+        jobs = [bi_pt.Job(tr, cam, **kw) for (sc, cam, kw), tr in zip(specs, tracers)]
+        t = time.perf_counter()
+        tuned = b.autotune(jobs, rounds=1, stream=sh)
+        tune_s = time.perf_counter() - t
+        b.set_tuning(use_graph=True, **{k: tuned[k] for k in ("wave_paths", "stack_lds", "refill_lanes",
+                                                              "work_parts")})
+        b.render(jobs, stream=sh)          # capture
+        stream.synchronize()
+        r0 = rays_of([b])
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.render(jobs, stream=sh)      # replay
+        stream.synchronize()
+        batch_s = (time.perf_counter() - t0) / steps
+        batch_rays = (rays_of([b]) - r0) // steps
+    b.close()
+    for tr in tracers:
+        tr.close()
+    log(f"batch leg: plain {plain_s * 1e3:.1f} ms ({plain_rays / plain_s / 1e6:.0f} Mrays/s), "
+        f"batched {batch_s * 1e3:.1f} ms ({batch_rays / batch_s / 1e6:.0f} Mrays/s), autotune {tune_s:.1f}s")
+    return {"workload": f"{len(specs)} c0-style scenes, {tris} tris in all (1k-500k each, log-uniform), one job "
+                        f"each: 256x256 / 640x480 / 1280x720, 1-8 spp, depth 6, LBVHs rebuilt per call; "
+                        f"{paths} paths per call",
+            "calls": steps, "rays_per_call": int(batch_rays),
+            "plain_ms_per_call": round(plain_s * 1e3, 2), "plain_mrays_s": round(plain_rays / plain_s / 1e6, 1),
+            "batched_ms_per_call": round(batch_s * 1e3, 2), "batched_mrays_s": round(batch_rays / batch_s / 1e6, 1),
+            "tuning": {k: int(tuned[k]) for k in ("wave_paths", "stack_lds", "refill_lanes")},
+            "autotune_s": round(tune_s, 1)}
+
+
+def serve_leg(seconds, rate=200.0):
+    """Serving form: Poisson arrivals at `rate` jobs/s for `seconds` over 8 resident
+    templates (10k-1M tris); orbit cameras, 256x256 / 512x512 / 1280x720, spp 1/4/16,
+    depth 6, 96 GB HBM budget.  The plain form serves the same arrivals first-come
+    first-served with one bi_pt_render per job (latencies from measured job times)."""
+    import numpy as np
+    import torch
+
+    import bi_pt
+    import scenes
+    templates = scenes.templates_c3()
+    tracers = []
+    for sc in templates:
+        tr = bi_pt.PathTracer()
+        tr.set_scene(sc)
+        tracers.append(tr)
+    rng = scenes.Pcg32(0xA11C0000, 0)
+    jobs = []
+    t_arr = 0.0
+    while True:
+        t_arr += -math.log(1.0 - rng.uniform()) / rate
+        if t_arr >= seconds:
+            break
+        k = min(int(rng.uniform() * len(templates)), len(templates) - 1)
+        w, h = scenes.C3_RESOLUTIONS[min(int(rng.uniform() * 3), 2)]
+        spp = scenes.C3_SPP[min(int(rng.uniform() * 3), 2)]
+        cam = scenes.camera_orbit(rng, w, h)
+        jobs.append((t_arr, k, cam, spp))
+    kw = dict(max_bounces=7, seed=0x5EED0003)
+    log(f"serve leg: {len(jobs)} jobs over {seconds}s, templates {[s.num_tris for s in templates]}")
+
+    srv = bi_pt.Server(hbm_budget_bytes=96 << 30, spp_per_round=4)
+    sids = [srv.add_scene(tr) for tr in tracers]
+    torch.cuda.synchronize()
+    ids = []
+    i = 0
+    t0 = srv.clock()
+    peak = 0
+    while True:
+        now = srv.clock() - t0
+        while i < len(jobs) and jobs[i][0] <= now:
+            a, k, cam, spp = jobs[i]
+            ids.append(srv.submit(sids[k], cam, spp=spp, arrival_s=t0 + a, **kw))
+            i += 1
+        active = srv.step()
+        peak = max(peak, srv.stats()["running"])
+        if active == 0:
+            if i == len(jobs):
+                break
+            time.sleep(max(0.0, jobs[i][0] - (srv.clock() - t0)))
+    t_end = srv.clock() - t0
+    lat = []
+    for jid in ids:
+        s = srv.poll(jid)
+        lat.append(s["done_s"] - s["submit_s"])
+        srv.take_result(jid, None)
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat) * 1e3
+
+    # plain form: first come first served, one bi_pt_render per job, BVHs kept resident
+    dur = []
+    rays0 = sum(sum(tr.stats()[k] for k in RAY_KINDS) for tr in tracers)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter()
+    for a, k, cam, spp in jobs:
+        t = time.perf_counter()
+        tracers[k].render(cam, spp=spp, flags=bi_pt.FLAG_REUSE_BVH, **kw)
+        torch.cuda.synchronize()
+        dur.append(time.perf_counter() - t)
+    t_plain = time.perf_counter() - t_plain
+    plain_rays = sum(sum(tr.stats()[k] for k in RAY_KINDS) for tr in tracers) - rays0
+    done = 0.0
+    plat = []
+    for (a, _, _, _), d in zip(jobs, dur):
+        done = max(done, a) + d
+        plat.append(done - a)
+    plat = np.array(plat) * 1e3
+    for tr in tracers:
+        tr.close()
+    log(f"serve leg: served {len(ids)} jobs in {t_end:.2f}s, p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms; plain FCFS p50 {np.percentile(plat, 50):.1f} ms, "
+        f"p99 {np.percentile(plat, 99):.1f} ms")
+    return {"workload": f"Poisson arrivals {rate:g} jobs/s for {seconds:g}s ({len(jobs)} jobs) over 8 resident "
+                        f"c0-style templates ({min(s.num_tris for s in templates)}-"
+                        f"{max(s.num_tris for s in templates)} tris), orbit cameras, 256x256 / 512x512 / "
+                        f"1280x720, spp 1/4/16, depth 6, 96 GB HBM budget, 4 spp per job per round",
+            "jobs": len(jobs), "served_s": round(t_end, 3), "rays": int(st["rays"]),
+            "mrays_s": round(st["rays"] / t_end / 1e6, 1),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2)},
+            "peak_running_jobs": int(peak), "rounds": int(st["rounds"]),
+            "plain_fcfs": {"latency_ms": {"p50": round(float(np.percentile(plat, 50)), 2),
+                                          "p99": round(float(np.percentile(plat, 99)), 2)},
+                           "busy_s": round(t_plain, 3), "mrays_s": round(plain_rays / t_plain / 1e6, 1)}}
 
 
 def rmse_leg(pt, scene, cam, cfg):

@@ -492,6 +492,10 @@ class Server:
         return {k: getattr(s, k) for k, _ in s._fields_}
 
     def take_result(self, job_id, npix):
+        """The finished job's RGBA32F image ([npix, 4]); npix None discards it."""
+        if npix is None:
+            self._check(lib().bi_pt_server_take_result(self.h, job_id, None, 0))
+            return None
         out = np.zeros((npix, 4), np.float32)
         self._check(lib().bi_pt_server_take_result(self.h, job_id, _p(out), out.size))
         return out

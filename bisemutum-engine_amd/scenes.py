@@ -262,6 +262,17 @@ def preetham_like_sky(sun_dir, w=512, h=256):
 
 # ---------------------------------------------------------------- configs
 
+def room_c0(half=6.0, height=4.0):
+    """Open-top box room: floor 2x2 quads + 4 walls of 1x2 / 2x1 quads (24 triangles)."""
+    return merge([
+        grid_quad((-half, 0, -half), (2 * half, 0, 0), (0, 0, 2 * half), 2, 2, (0, 1, 0)),
+        grid_quad((-half, 0, -half), (0, height, 0), (2 * half, 0, 0), 1, 2, (0, 0, 1)),
+        grid_quad((-half, 0, half), (2 * half, 0, 0), (0, height, 0), 2, 1, (0, 0, -1)),
+        grid_quad((-half, 0, -half), (0, 0, 2 * half), (0, height, 0), 2, 1, (1, 0, 0)),
+        grid_quad((half, 0, -half), (0, height, 0), (0, 0, 2 * half), 1, 2, (-1, 0, 0)),
+    ])
+
+
 def scene_c0(seed: int = 0xB15E0000):
     """Box room (12 quads: floor 2x2 + 4 walls x 2, open top) + 64 icospheres (subdiv 2).
 
@@ -269,13 +280,7 @@ def scene_c0(seed: int = 0xB15E0000):
     then spheres): albedo r, g, b, roughness, metallic."""
     rng = Pcg32(seed, 0)
     half, height = 6.0, 4.0
-    room = merge([
-        grid_quad((-half, 0, -half), (2 * half, 0, 0), (0, 0, 2 * half), 2, 2, (0, 1, 0)),
-        grid_quad((-half, 0, -half), (0, height, 0), (2 * half, 0, 0), 1, 2, (0, 0, 1)),
-        grid_quad((-half, 0, half), (2 * half, 0, 0), (0, height, 0), 2, 1, (0, 0, -1)),
-        grid_quad((-half, 0, -half), (0, 0, 2 * half), (0, height, 0), 2, 1, (1, 0, 0)),
-        grid_quad((half, 0, -half), (0, height, 0), (0, 0, 2 * half), 1, 2, (-1, 0, 0)),
-    ])
+    room = room_c0(half, height)
     sphere = icosphere(2)
     inst = []
     centres = []
@@ -294,6 +299,93 @@ def scene_c0(seed: int = 0xB15E0000):
     return Scene("c0_box_room", [room, sphere], instances, np.stack(mats), sun[None], sky, col,
                  np.eye(3, dtype=np.float32).reshape(-1),
                  notes={"room_tris": len(room[2]), "sphere_tris": len(sphere[2])})
+
+
+_ICO_CACHE: dict = {}
+
+
+def icosphere_cached(subdiv: int):
+    if subdiv not in _ICO_CACHE:
+        _ICO_CACHE[subdiv] = icosphere(subdiv)
+    return _ICO_CACHE[subdiv]
+
+
+def sized_room_plan(target_tris: int):
+    """(icosphere subdivision, sphere count) for a c0-style room of about target_tris:
+    the finest subdivision that still leaves at least 64 spheres (subdivision 0 below
+    64 x 20 triangles), then as many spheres as fill the target."""
+    body = max(target_tris - 24, 20)
+    s = 0
+    while s < 6 and 64 * 20 * 4 ** (s + 1) <= body:
+        s += 1
+    return s, max(1, int(round(body / (20 * 4 ** s))))
+
+
+def scene_sized(target_tris: int, seed: int):
+    """c0-style room (24 triangles) with icospheres adding up to about target_tris.
+
+    Spheres of one subdivision (sized_room_plan); draw order as scene_c0: for each sphere
+    centre x, y, z, radius; then materials (room first, then spheres)."""
+    rng = Pcg32(seed, 0)
+    half, height = 6.0, 4.0
+    room = room_c0(half, height)
+    subdiv, count = sized_room_plan(target_tris)
+    sphere = icosphere_cached(subdiv)
+    centres = []
+    for _ in range(count):
+        c = (rng.range(-half + 0.6, half - 0.6), rng.range(0.6, height - 0.6), rng.range(-half + 0.6, half - 0.6))
+        centres.append((c, rng.range(0.2, 0.6)))
+    mats = [random_material(rng)]
+    inst = [(translate_scale((0, 0, 0)), 0, 0)]
+    for i, (c, r) in enumerate(centres):
+        mats.append(random_material(rng))
+        inst.append((translate_scale(c, (r, r, r)), 1, i + 1))
+    sky, col = constant_sky()
+    sun = dir_light((2.6, 2.5, 2.3), (0.35, 0.85, 0.4))
+    return Scene(f"room_{count}x{20 * 4 ** subdiv}", [room, sphere], np.array(inst, INSTANCE_DTYPE), np.stack(mats),
+                 sun[None], sky, col, np.eye(3, dtype=np.float32).reshape(-1),
+                 notes={"spheres": count, "sphere_tris": 20 * 4 ** subdiv})
+
+
+def log_uniform_tris(rng: Pcg32, lo: float, hi: float) -> int:
+    return int(round(math.exp(rng.range(math.log(lo), math.log(hi)))))
+
+
+def camera_orbit(rng: Pcg32, width: int, height: int, radius=12.0, centre=(0.0, 2.0, 0.0)):
+    """Camera on a sphere around a c0-style room, looking at its centre over the walls:
+    azimuth U[0, 2pi), elevation U[30, 75] degrees (draw order as written)."""
+    az = rng.range(0.0, 2.0 * math.pi)
+    el = math.radians(rng.range(30.0, 75.0))
+    eye = (centre[0] + radius * math.cos(el) * math.sin(az), centre[1] + radius * math.sin(el),
+           centre[2] + radius * math.cos(el) * math.cos(az))
+    return make_camera(eye, centre, width, height, 55.0)
+
+
+C2_RESOLUTIONS = [(256, 256), (640, 480), (1280, 720)]
+C3_RESOLUTIONS = [(256, 256), (512, 512), (1280, 720)]
+C3_SPP = [1, 4, 16]
+
+
+def batch_c2(count: int = 64, seed: int = 0xB15E0002):
+    """Multi-scene batch: `count` c0-style scenes with triangle counts log-uniform in
+    [1k, 500k], each with one job: resolution uniform over C2_RESOLUTIONS, spp U{1..8},
+    depth 6 (max_bounces 7), an orbit camera.  Draws per scene: triangle count, then
+    resolution, spp, camera; each scene's own geometry uses seed + 1 + index."""
+    rng = Pcg32(seed, 1)
+    out = []
+    for i in range(count):
+        tris = log_uniform_tris(rng, 1e3, 5e5)
+        w, h = C2_RESOLUTIONS[min(int(rng.uniform() * 3), 2)]
+        spp = 1 + min(int(rng.uniform() * 8), 7)
+        cam = camera_orbit(rng, w, h)
+        out.append((scene_sized(tris, seed + 1 + i), cam, dict(spp=spp, max_bounces=7, seed=0x5EED0002 + i)))
+    return out
+
+
+def templates_c3(count: int = 8, seed: int = 0xB15E0003):
+    """Serving templates: `count` c0-style scenes, triangle counts log-uniform in [10k, 1M]."""
+    rng = Pcg32(seed, 1)
+    return [scene_sized(log_uniform_tris(rng, 1e4, 1e6), seed + 1 + i) for i in range(count)]
 
 
 def scene_c1(seed: int = 0xB15E0001):
