@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--work-parts", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache leg")
+    ap.add_argument("--rmse-frames", type=int, default=6, help="frames averaged in each RMSE figure")
+    ap.add_argument("--nrc-warmup", type=int, default=48, help="frames of online training before the cache's RMSE")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-calls", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=2.0,
@@ -198,8 +201,11 @@ def main():
         },
     }
 
-    if not args.no_rmse:
-        result["rmse_vs_fp64"] = rmse_leg(pt, scene, cam, cfg)
+    ref = fp64_reference(scene) if not args.no_rmse else None
+    if ref is not None:
+        result["rmse_vs_fp64"] = rmse_leg(pt, cam, cfg, ref, args.rmse_frames)
+        if not args.no_nrc:
+            result["nrc"] = nrc_leg(pt, cam, cfg, ref, result["rmse_vs_fp64"], args.rmse_frames, args.nrc_warmup)
     if not args.no_batch:
         result["batched"] = batch_leg(args.batch_calls)
     if args.serve_seconds > 0:
@@ -383,23 +389,106 @@ def serve_leg(seconds, rate=200.0):
                            "busy_s": round(t_plain, 3), "mrays_s": round(plain_rays / t_plain / 1e6, 1)}}
 
 
-def rmse_leg(pt, scene, cam, cfg):
-    """Relative RMSE of one 16-spp GPU frame against the FP64 oracle at 4096 spp on a seeded pixel subset."""
+FP64_REFERENCE = os.path.join(ROOT, "tests", "golden", "c1_fp64_4096spp.npz")
+
+
+def fp64_reference(scene):
+    """FP64 oracle means at 4096 spp on 2048 seeded pixels of the atrium view (a committed
+    fixture, tests/golden/make_c1_fp64.py), or None when the scene differs from it."""
+    import hashlib
+
+    import numpy as np
+    if not os.path.exists(FP64_REFERENCE):
+        return None
+    ref = np.load(FP64_REFERENCE)
+    h = hashlib.sha256()
+    for p, n, i in scene.meshes:
+        h.update(p.tobytes()); h.update(n.tobytes()); h.update(i.tobytes())
+    h.update(scene.instances.tobytes()); h.update(scene.materials.tobytes()); h.update(scene.dir_lights.tobytes())
+    h.update(scene.sky_rgba.tobytes())
+    return ref if str(ref["scene_sha256"]) == h.hexdigest() else None
+
+
+def image_errors(img, ref):
+    """(relative RMSE, relative MSE) of an image's reference pixels against the FP64 means."""
+    import numpy as np
+    m = ref["mean"]
+    e = img[ref["pixels"], :3].astype(np.float64) - m
+    return float(np.sqrt((e ** 2).mean() / (m ** 2).mean())), float((e ** 2 / (m ** 2 + 1e-2)).mean())
+
+
+def timed(fn):
+    """Milliseconds of fn's work on the current stream (HIP events)."""
+    import torch
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(torch.cuda.current_stream())
+    fn()
+    b.record(torch.cuda.current_stream())
+    b.synchronize()
+    return a.elapsed_time(b)
+
+
+def rmse_leg(pt, cam, cfg, ref, frames):
+    """Plain frames (16 spp, LBVH rebuilt) against the FP64 reference: RMSE averaged over frames."""
+    import numpy as np
+    ms, mse, rel = [], [], []
+    for k in range(frames):
+        t = timed(lambda: pt.render(cam, spp=cfg["spp"], max_bounces=cfg["max_bounces"], seed=cfg["seed"] + 7000 + k),
+                  )
+        r, q = image_errors(pt.image(), ref)
+        ms.append(t)
+        mse.append(r * r)
+        rel.append(q)
+    out = {"relative_rmse": round(float(np.sqrt(np.mean(mse))), 4), "relative_mse": round(float(np.mean(rel)), 4),
+           "frames": frames, "ms_per_frame": round(float(np.mean(ms)), 2), "spp": cfg["spp"],
+           "pixels": int(len(ref["pixels"])), "oracle_spp": int(ref["spp"])}
+    log(f"rmse leg: plain {out}")
+    return out
+
+
+def nrc_leg(pt, cam, cfg, ref, plain, frames, warmup):
+    """Neural radiance cache form at equal time: paths end in the cache after 2 bounces,
+    65,536 training paths and 4 Adam steps per frame (online).  After `warmup` frames of
+    online training, the cached frame's spp is set so that its time matches the plain
+    16-spp frame; RMSE against the FP64 reference, averaged over `frames` frames."""
     import numpy as np
 
-    import oracle
-    t = time.perf_counter()
-    ors = oracle.OracleScene(scene)
-    rng = np.random.default_rng(0x5EED)
-    pix = np.sort(rng.choice(cam.width * cam.height, 16, replace=False)).astype(np.uint32)
-    m64, _ = ors.render64(cam, cfg["max_bounces"], pix, 4096, threads=min(16, os.cpu_count() or 1))
-    pt.render(cam, spp=cfg["spp"], max_bounces=cfg["max_bounces"], seed=cfg["seed"])
-    img = pt.image()[:, :3]
-    err = img[pix] - m64
-    rel = float(np.sqrt((err ** 2).mean()) / max(np.sqrt((m64 ** 2).mean()), 1e-12))
-    log(f"rmse leg: {time.perf_counter() - t:.1f}s")
-    ors.close()
-    return {"relative_rmse": round(rel, 5), "pixels": int(len(pix)), "gpu_spp": cfg["spp"], "oracle_spp": 4096}
+    import bi_pt
+    mb = cfg["max_bounces"]
+    nrc = bi_pt.Nrc(pt, train_samples=65536, train_steps=4, cache_bounce=2, seed=1)
+    t0 = time.perf_counter()
+    for k in range(warmup):
+        nrc.render(cam, spp=1, max_bounces=mb, seed=cfg["seed"] + 9000 + k)
+    nrc.stats()
+    warm_s = time.perf_counter() - t0
+    t16 = timed(lambda: nrc.render(cam, spp=16, max_bounces=mb, seed=cfg["seed"] + 9500))
+    spp_eq = max(1, int(cfg["spp"] * plain["ms_per_frame"] / t16))
+
+    def run(spp, base):
+        ms, mse, rel = [], [], []
+        for k in range(frames):
+            ms.append(timed(lambda: nrc.render(cam, spp=spp, max_bounces=mb, seed=base + k)))
+            r, q = image_errors(pt.image(), ref)
+            mse.append(r * r)
+            rel.append(q)
+        return {"spp": spp, "ms_per_frame": round(float(np.mean(ms)), 2),
+                "relative_rmse": round(float(np.sqrt(np.mean(mse))), 4), "relative_mse": round(float(np.mean(rel)), 4)}
+
+    eq = run(spp_eq, cfg["seed"] + 10000)
+    same_spp = run(cfg["spp"], cfg["seed"] + 11000)
+    st = nrc.stats()
+    nrc.close()
+    out = {"workload": f"cache after 2 bounces, 65,536 training paths and 4 Adam steps per frame, "
+                       f"{warmup} warm-up frames of online training at 1 spp",
+           "equal_time": eq, "equal_spp": same_spp, "plain": {k: plain[k] for k in ("spp", "ms_per_frame",
+                                                                                   "relative_rmse", "relative_mse")},
+           "rmse_ratio_equal_time": round(eq["relative_rmse"] / plain["relative_rmse"], 4),
+           "relmse_ratio_equal_time": round(eq["relative_mse"] / plain["relative_mse"], 4),
+           "warmup_s": round(warm_s, 2), "train_steps": int(st["steps"]), "last_loss": round(st["last_loss"], 4),
+           "valid_training_samples": int(st["last_valid_samples"])}
+    log(f"nrc leg: {out}")
+    return out
 
 
 def cpu_leg(scene, cfg):

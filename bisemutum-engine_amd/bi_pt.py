@@ -33,7 +33,12 @@ EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
            "bi_pt_batch_autotune", "bi_pt_create_view", "bi_pt_server_create", "bi_pt_server_destroy",
            "bi_pt_server_last_error", "bi_pt_server_clock", "bi_pt_server_add_scene", "bi_pt_server_submit",
-           "bi_pt_server_step", "bi_pt_server_poll", "bi_pt_server_take_result", "bi_pt_server_get_stats"]
+           "bi_pt_server_step", "bi_pt_server_poll", "bi_pt_server_take_result", "bi_pt_server_get_stats",
+           "bi_pt_nrc_create", "bi_pt_nrc_destroy", "bi_pt_nrc_last_error", "bi_pt_render_nrc",
+           "bi_pt_nrc_get_stats", "bi_pt_nrc_get_params", "bi_pt_nrc_set_params", "bi_pt_nrc_set_bounds",
+           "bi_pt_nrc_infer", "bi_pt_nrc_train", "bi_pt_nrc_debug_gradient", "bi_pt_nrc_debug_samples"]
+NRC_PARAMS = 17408
+NRC_FREEZE = 1
 
 
 class PtError(RuntimeError):
@@ -99,6 +104,17 @@ class ServerStats(C.Structure):
                 ("hbm_used", C.c_uint64)]
 
 
+class NrcDesc(C.Structure):
+    _fields_ = [("train_samples", C.c_uint32), ("train_steps", C.c_uint32), ("cache_bounce", C.c_uint32),
+                ("learning_rate", C.c_float), ("use_after_steps", C.c_uint32), ("seed", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class NrcStats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("steps", C.c_uint64), ("last_loss", C.c_float), ("_pad", C.c_uint32),
+                ("last_valid_samples", C.c_uint64)]
+
+
 class Stats(C.Structure):
     _fields_ = [("rays_camera", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_light_shadow", C.c_uint64),
                 ("rays_sky_shadow", C.c_uint64), ("frames", C.c_uint64), ("samples_accumulated", C.c_uint64),
@@ -162,9 +178,22 @@ def lib():
         L.bi_pt_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(JobStatus)]
         L.bi_pt_server_take_result.argtypes = [vp, C.c_uint64, f32p, C.c_uint64]
         L.bi_pt_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.bi_pt_nrc_create.argtypes = [vp, C.POINTER(NrcDesc), C.POINTER(vp)]
+        L.bi_pt_nrc_destroy.argtypes = [vp]
+        L.bi_pt_nrc_last_error.argtypes = [vp]
+        L.bi_pt_nrc_last_error.restype = C.c_char_p
+        L.bi_pt_render_nrc.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
+        L.bi_pt_nrc_get_stats.argtypes = [vp, C.POINTER(NrcStats)]
+        L.bi_pt_nrc_get_params.argtypes = [vp, f32p, u32]
+        L.bi_pt_nrc_set_params.argtypes = [vp, f32p, u32]
+        L.bi_pt_nrc_set_bounds.argtypes = [vp, f32p]
+        L.bi_pt_nrc_infer.argtypes = [vp, vp, u32, vp, vp]
+        L.bi_pt_nrc_train.argtypes = [vp, vp, vp, u32, vp]
+        L.bi_pt_nrc_debug_gradient.argtypes = [vp, f32p, u32]
+        L.bi_pt_nrc_debug_samples.argtypes = [vp, f32p, f32p, u32]
         for name in EXPORTS:
             if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris", "bi_pt_batch_last_error",
-                            "bi_pt_server_last_error", "bi_pt_server_clock"):
+                            "bi_pt_server_last_error", "bi_pt_server_clock", "bi_pt_nrc_last_error"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -504,3 +533,98 @@ class Server:
         s = ServerStats()
         self._check(lib().bi_pt_server_get_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+class Nrc:
+    """bi_pt_nrc: the neural radiance cache of one PathTracer's scene."""
+
+    def __init__(self, tracer, train_samples=0, train_steps=0, cache_bounce=0, learning_rate=0.0,
+                 use_after_steps=0, seed=0, flags=0):
+        self.tracer = tracer
+        self.torch = tracer.torch
+        h = C.c_void_p()
+        st = lib().bi_pt_nrc_create(tracer.h, C.byref(NrcDesc(train_samples, train_steps, cache_bounce,
+                                                                learning_rate, use_after_steps, seed, flags)),
+                                    C.byref(h))
+        if st != 0:
+            raise PtError(st, "bi_pt_nrc_create failed")
+        self.h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise PtError(st, lib().bi_pt_nrc_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().bi_pt_nrc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
+               wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, stream=None):
+        """bi_pt_render_nrc into the tracer's output buffer (tracer.image() reads it)."""
+        t = self.tracer
+        torch = self.torch
+        npix = cam.width * cam.height
+        if t._out is None or t._out.numel() != npix * 4:
+            t._out = torch.empty(npix * 4, dtype=torch.float32, device=torch.device("cuda", t.device))
+        st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, wave_paths, stack_lds,
+                      refill_lanes, work_parts)
+        self._check(lib().bi_pt_render_nrc(self.h, C.byref(camera_struct(cam)), C.byref(st),
+                                           C.c_void_p(t._out.data_ptr()), None,
+                                           C.c_void_p(stream) if stream else None))
+        return t._out
+
+    def stats(self):
+        s = NrcStats()
+        self._check(lib().bi_pt_nrc_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_ if k != "_pad"}
+
+    def params(self):
+        out = np.zeros(NRC_PARAMS, np.float32)
+        self._check(lib().bi_pt_nrc_get_params(self.h, _p(out), NRC_PARAMS))
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(lib().bi_pt_nrc_set_params(self.h, _p(w), w.size))
+
+    def set_bounds(self, lo, hi):
+        b = np.ascontiguousarray(list(lo) + list(hi), np.float32)
+        self._check(lib().bi_pt_nrc_set_bounds(self.h, _p(b)))
+
+    def infer(self, records):
+        """Raw outputs [n, 3] for device records (torch float32 [n, 16])."""
+        torch = self.torch
+        out = torch.empty((records.shape[0], 3), dtype=torch.float32, device=records.device)
+        self._check(lib().bi_pt_nrc_infer(self.h, C.c_void_p(records.data_ptr()), records.shape[0],
+                                          C.c_void_p(out.data_ptr()), None))
+        torch.cuda.synchronize()
+        return out
+
+    def train(self, records, targets):
+        """One Adam step on device records [n, 16] and targets [n, 3] (torch float32)."""
+        self._check(lib().bi_pt_nrc_train(self.h, C.c_void_p(records.data_ptr()), C.c_void_p(targets.data_ptr()),
+                                          records.shape[0], None))
+
+    def gradient(self):
+        out = np.zeros(NRC_PARAMS, np.float32)
+        self._check(lib().bi_pt_nrc_debug_gradient(self.h, _p(out), NRC_PARAMS))
+        return out
+
+    def samples(self, n):
+        """The last frame's first n training records [n, 16] and targets [n, 3]."""
+        rec = np.zeros((n, 16), np.float32)
+        tgt = np.zeros((n, 3), np.float32)
+        self._check(lib().bi_pt_nrc_debug_samples(self.h, _p(rec), _p(tgt), n))
+        return rec, tgt
+
+
+def nrc_train_seed(seed, frame):
+    """Seed of the training paths of frame `frame` of bi_pt_render_nrc with render seed `seed`."""
+    return ((seed * 0x9E3779B1) & 0xFFFFFFFF) ^ ((frame * 0x85EBCA77 + 0x7A1B) & 0xFFFFFFFF)

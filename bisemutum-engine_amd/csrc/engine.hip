@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "pt_host.hpp"
+#include "pt_nrc.hpp"
 
 namespace bipt {
 
@@ -136,9 +137,9 @@ struct Prep {
     bi_pt_ctx* ctx;       // running image
     bi_pt_ctx* scene;     // scene, lights, sky, BVH
     CameraParams cam;
-    uint32_t npix, M, D, nl, spp, sh;
+    uint32_t npix, M, D, nl, spp, sh, role;
     bool sky_nee, reset, rebuild;
-    uint64_t acc_before;
+    float* sums;          // running sums (training paths: the cache's targets)
 };
 
 // One job's share of one wave.
@@ -161,13 +162,18 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     const bool record = opt.record && n == 1;
     std::vector<Prep> prep(n);
     uint32_t max_npix = 0, slots = 2, sh_mult = 1;
+    bi_pt_nrc* const nrc = opt.nrc;
     for (uint32_t j = 0; j < n; ++j) {
         const JobIn& J = jobs[j];
         bi_pt_ctx* ctx = J.ctx;
-        if (!ctx || !J.camera || !J.st || !J.out_rgba)
+        const bool train = J.role == kRoleTrain;
+        if (J.role != kRoleImage && !nrc)
+            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "a cached or training job needs a radiance cache");
+        if (!ctx || !J.camera || !J.st || (!J.out_rgba && !train))
             return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "null context/camera/settings/output in job " + std::to_string(j));
-        for (uint32_t k = 0; k < j; ++k)
-            if (jobs[k].ctx == ctx) return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "a context appears twice in one call");
+        for (uint32_t k = 0; k < j && !train; ++k)
+            if (jobs[k].ctx == ctx && jobs[k].role != kRoleTrain)
+                return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "a context appears twice in one call");
         bi_pt_ctx* S = scene_of(ctx);
         if (!S->has_scene) return fail(own, BI_PT_ERROR_NO_SCENE, "no scene set (job " + std::to_string(j) + ")");
         const bi_pt_camera* c = J.camera;
@@ -178,10 +184,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         if (npix64 > (1ull << 26)) return fail(own, BI_PT_ERROR_UNSUPPORTED, "image larger than 2^26 pixels");
         if (!(st->ray_length > 0.0f) || !(st->ray_length <= kShadowTMax))
             return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "ray_length must be positive and at most 1.7e38");
-        if (ctx->acc_count + st->spp > 0xffffffffull)
+        if (!train && ctx->acc_count + st->spp > 0xffffffffull)
             return fail(own, BI_PT_ERROR_UNSUPPORTED, "more than 2^32 accumulated samples");
         Prep& p = prep[j];
         p.ctx = ctx;
+        p.role = J.role;
         std::memcpy(p.cam.inv_view, c->matrix_inv_view, 64);
         std::memcpy(p.cam.inv_proj, c->matrix_inv_proj, 64);
         std::memcpy(p.cam.proj_view, c->matrix_proj_view, 64);
@@ -201,8 +208,14 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                           ctx->acc_h == p.cam.height && std::memcmp(&ctx->last_cam, &p.cam, sizeof p.cam) == 0 &&
                           ctx->last_seed == st->seed && ctx->last_flags == (st->flags & 3u) &&
                           ctx->last_bounces == p.M && ctx->last_ray_length == st->ray_length &&
-                          ctx->last_version == S->scene_version;
-        p.reset = !same;
+                          ctx->last_version == S->scene_version && ctx->last_role == J.role;
+        p.reset = !same || train;
+        p.sums = nullptr;
+        if (train) {
+            p.npix = nrc->train_samples;
+            p.spp = 1;
+            p.sums = nrc->targets.p;
+        }
         p.rebuild = !(opt.reuse_bvh && S->bvh_built);
         for (uint32_t k = 0; k < j; ++k)
             if (prep[k].scene == S) p.rebuild = false;   // one build per scene and call
@@ -246,14 +259,21 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     const uint32_t parts = kn.work_parts ? std::min(kn.work_parts, kWorkPartsMax) : kWorkParts;
 
     // device memory: per-job images, BVH builds, wave buffers, plan
+    bool any_cached = false, any_train = false;
     for (uint32_t j = 0; j < n; ++j) {
         bi_pt_ctx* ctx = prep[j].ctx;
-        bi_pt_status e = dalloc(ctx, ctx->sums, (size_t)3 * prep[j].npix);
-        if (e != BI_PT_OK) return fail(own, e, ctx->err);
+        bi_pt_status e = BI_PT_OK;
+        any_cached |= prep[j].role == kRoleCached;
+        any_train |= prep[j].role == kRoleTrain;
+        if (prep[j].role != kRoleTrain) {
+            if ((e = dalloc(ctx, ctx->sums, (size_t)3 * prep[j].npix)) != BI_PT_OK) return fail(own, e, ctx->err);
+            prep[j].sums = ctx->sums.p;
+        }
         if (prep[j].rebuild && (e = ensure_build_buffers(prep[j].scene)) != BI_PT_OK)
             return fail(own, e, prep[j].scene->err);
     }
     BI_TRY(ensure_wave_buffers(own, eng, wave_max, slots, sh_mult));
+    if (any_cached) BI_TRY(nrc_ensure_query(nrc, eng.wave_cap));
     const size_t rec_align = alignof(JobRec);
     auto wave_bytes = [&](size_t k) {
         return align_up(k * sizeof(JobRec) + 2 * (k + 1) * sizeof(uint32_t), rec_align);
@@ -298,10 +318,13 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         r.sample0 = (uint32_t)(p.reset ? 0 : ctx->acc_count) + sample_off;
         r.samples = samples;
         r.seed = jobs[j].st->seed;
-        r.flags = jobs[j].st->flags & 3u;
+        r.flags = (jobs[j].st->flags & 3u) | (p.role == kRoleCached ? kJobCache : 0u) |
+                  (p.role == kRoleTrain ? kJobTrain : 0u);
+        r.cache_bounce = p.role != kRoleImage ? nrc->cache_bounce : 0xffffffffu;
+        r.cache_rec = p.role == kRoleTrain ? nrc->train_rec.p : nullptr;
         r.ray_length = jobs[j].st->ray_length;
         r.pix_base = pix_base;
-        r.sums = ctx->sums.p;
+        r.sums = p.sums;
         r.depth_out = jobs[j].out_depth;
         r.depth_sample = (uint32_t)(p.reset ? 0 : ctx->acc_count);
         r.out_rgba = jobs[j].out_rgba;
@@ -386,8 +409,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         // persistent lanes' part counters, and the images of jobs that start over
         launch_fill_u32(cnt, 0u, kCntRays, s);
         launch_fill_u32(cnt + kCntTraceWork, 0u, kCntSlots - kCntTraceWork, s);
-        for (uint32_t j = 0; j < n; ++j)
-            if (prep[j].reset) launch_fill_u32(reinterpret_cast<uint32_t*>(prep[j].ctx->sums.p), 0u, 3 * prep[j].npix, s);
+        for (uint32_t j = 0; j < n; ++j) {
+            if (prep[j].reset) launch_fill_u32(reinterpret_cast<uint32_t*>(prep[j].sums), 0u, 3 * prep[j].npix, s);
+            if (prep[j].role == kRoleTrain)   // training samples start unrecorded
+                launch_fill_u32(reinterpret_cast<uint32_t*>(nrc->train_rec.p), 0u, 16 * prep[j].npix, s);
+        }
         if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[0], s));
         for (uint32_t j = 0; j < n; ++j) {
             if (!prep[j].rebuild) continue;
@@ -396,6 +422,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 return fail(own, e, prep[j].scene->err);
             }
         }
+        if ((any_cached || any_train) && prep[0].scene->cb_out.p)   // the cache's position box
+            BI_HIP_TRY(own, hipMemcpyAsync(nrc->bounds.p, prep[0].scene->cb_out.p, 6 * sizeof(float),
+                                           hipMemcpyDeviceToDevice, s));
         if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[1], s));
 
         PathState ps{eng.ps_f[0].p, eng.ps_f[1].p, eng.ps_f[2].p, eng.ps_f[3].p, eng.acc.p, eng.wave_cap};
@@ -429,7 +458,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         for (size_t w = 0; w < waves.size(); ++w) {
             const WaveJobs& J = wj[w];
             uint32_t D = 0;
-            for (const Item& it : waves[w]) D = std::max(D, prep[it.job].D);
+            bool wave_cached = false;
+            for (const Item& it : waves[w]) {
+                D = std::max(D, prep[it.job].D);
+                wave_cached |= prep[it.job].role == kRoleCached;
+            }
             if (record) {
                 eng.rec_ext.emplace_back(D);
                 eng.rec_sh.emplace_back(D);
@@ -468,7 +501,20 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sa.trace_work_reset = &cnt[kCntTraceWork];
                 sa.shadow_work_reset = &cnt[kCntShadowWork];
                 sa.bounce = b;
+                const bool cache_now = wave_cached && b == nrc->cache_bounce;
+                sa.nrc_query = cache_now ? nrc->query.p : nullptr;
                 launch_shade(sa, J.num_paths, s);
+                if (cache_now) {
+                    // paths that ended in the cache at this segment: their radiance estimate
+                    NrcInferArgs na{};
+                    na.rec = nrc->query.p;
+                    na.count_in = primary ? nullptr : cnt_in;
+                    na.num = J.num_paths;
+                    na.fwd = nrc->fwd.p;
+                    na.bounds = nrc->bounds.p;
+                    na.ps = ps;
+                    launch_nrc_infer(na, s);
+                }
 
                 ScanArgs sc{};
                 sc.tile_sums = stage.tile_sums;
@@ -531,6 +577,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         }
         launch_output(out_jobs, out_pix, s);
         BI_HIP_TRY(own, hipGetLastError());
+        if (any_train) {   // the cache learns from this frame's training paths
+            if (bi_pt_status e = nrc_train_frame(nrc, s); e != BI_PT_OK) return fail(own, e, nrc->err);
+        }
         eng.stats.launches_trace_closest = launches_closest;
         eng.stats.launches_trace_shadow = launches_shadow;
         if (timing) {
@@ -566,10 +615,18 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     }
 finish:
     // host bookkeeping of the running images
+    uint32_t images = 0;
+    bi_pt_ctx* image_ctx = nullptr;
     for (uint32_t j = 0; j < n; ++j) {
         const Prep& p = prep[j];
         bi_pt_ctx* ctx = p.ctx;
         const bi_pt_settings* st = jobs[j].st;
+        eng.camera_rays += (uint64_t)p.npix * p.spp;
+        if (p.rebuild) p.scene->bvh_built = true;
+        if (p.role == kRoleTrain) continue;
+        ++images;
+        image_ctx = ctx;
+        ctx->last_role = p.role;
         if (p.reset) ctx->acc_count = 0;
         ctx->acc_w = p.cam.width;
         ctx->acc_h = p.cam.height;
@@ -580,13 +637,11 @@ finish:
         ctx->last_ray_length = st->ray_length;
         ctx->acc_count += p.spp;
         ctx->last_version = p.scene->scene_version;
-        if (p.rebuild) p.scene->bvh_built = true;
-        eng.camera_rays += (uint64_t)p.npix * p.spp;
     }
     eng.frames++;
     eng.stats.frames = eng.frames;
     eng.stats.rays_camera = eng.camera_rays;
-    eng.stats.samples_accumulated = n == 1 ? prep[0].ctx->acc_count : 0;
+    eng.stats.samples_accumulated = images == 1 ? image_ctx->acc_count : 0;
     return BI_PT_OK;
 }
 

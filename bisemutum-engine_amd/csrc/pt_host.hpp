@@ -159,7 +159,7 @@ struct bi_pt_ctx : bipt::Owner {
     uint32_t acc_w = 0, acc_h = 0;
     uint64_t acc_count = 0;
     bipt::CameraParams last_cam{};
-    uint32_t last_seed = 0, last_flags = 0, last_bounces = 0;
+    uint32_t last_seed = 0, last_flags = 0, last_bounces = 0, last_role = 0;
     float last_ray_length = 0.0f;
     // wave machinery of bi_pt_render
     bipt::Engine eng;
@@ -185,13 +185,17 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx);
 bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error);
 BvhView bvh_view(const bi_pt_ctx* ctx);
 
-// One job of a call.
+// One job of a call.  Roles: a plain image; an image whose paths end in the call's
+// radiance cache; the cache's training paths (one sample per slot into the cache's
+// targets, no image, no accumulation; its context only names the scene).
+enum : uint32_t { kRoleImage = 0, kRoleCached = 1, kRoleTrain = 2 };
 struct JobIn {
     bi_pt_ctx* ctx;
     const bi_pt_camera* camera;
     const bi_pt_settings* st;
     float* out_rgba;
     float* out_depth;
+    uint32_t role = kRoleImage;
 };
 
 struct RenderOpts {
@@ -199,6 +203,7 @@ struct RenderOpts {
     bool timing = false;      // per-kernel-class events (single-job calls)
     bool record = false;      // record queue contents (single-job calls)
     bool reuse_bvh = false;   // keep a built BVH whose scene is unchanged
+    bi_pt_nrc* nrc = nullptr; // radiance cache of kRoleCached / kRoleTrain jobs
 };
 
 // Renders every job (spp samples each, added to its context's running image) in shared

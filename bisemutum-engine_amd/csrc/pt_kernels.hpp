@@ -68,6 +68,24 @@ PT_HD uint32_t job_of(const uint32_t* base, uint32_t n, uint32_t x) {
     return lo;
 }
 
+// Slot, pixel and sample of a job's local path: images walk their pixels in 8x8 tile
+// order, sample after sample; the cache's training paths take one pixel per slot drawn
+// uniformly from the image (PCG32 keyed by the job seed and slot) and use the slot as
+// their sample index, so no two training paths share a random stream.
+constexpr uint32_t kTrainPixelBounce = 254u;
+struct PathKey {
+    uint32_t q, pixel, sample;
+};
+PT_HD PathKey path_key(const JobRec& J, uint32_t local) {
+    const uint32_t q = local % J.npix, sl = local / J.npix;
+    if (J.flags & kJobTrain) {
+        Pcg r = pcg_keyed(J.seed, q, 0u, kTrainPixelBounce);
+        const uint64_t npix = (uint64_t)J.cam.width * J.cam.height;
+        return PathKey{q, (uint32_t)(((uint64_t)pcg_next(r) * npix) >> 32), q};
+    }
+    return PathKey{q, pixel_of(q, J.cam.width, J.cam.height), J.sample0 + sl};
+}
+
 struct TraceArgs {
     WaveJobs jobs;
     RayQueue q;                 // input queue (unused for primary rays)
@@ -117,6 +135,8 @@ struct ShadeArgs {
     uint32_t* trace_work_reset;   // zeroed here for the next closest-hit launch
     uint32_t* shadow_work_reset;  // zeroed here for this segment's shadow launch
     uint32_t bounce;
+    NrcRecord* nrc_query;         // non-null: the cache kernel runs after this segment's
+                                  // shading; entry i's query (or kNoPath) goes to [i]
 };
 void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s);
 

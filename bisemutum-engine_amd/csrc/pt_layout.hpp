@@ -115,6 +115,26 @@ struct BvhView {
     uint32_t num_tris;
 };
 
+// Job kinds beside plain images (JobRec::flags).
+constexpr uint32_t kJobCache = 1u << 2;   // image paths end in the radiance cache at cache_bounce
+constexpr uint32_t kJobTrain = 1u << 3;   // the cache's training paths: one per slot, random pixels,
+                                          // radiance after cache_bounce goes to the slot's target
+constexpr uint32_t kNoPath = 0xffffffffu;
+
+// Surface sample of the radiance cache (16 floats, 64 B): a query of an image path
+// (tag = path, kNoPath when the entry has none) or a training sample (tag = 1 when the
+// path reached its record segment, else 0).
+struct alignas(16) NrcRecord {
+    float p[3];       // position (world)
+    float wo[3];      // direction towards the previous vertex
+    float n[3];       // shading normal
+    float roughness;
+    float albedo[3];  // base colour
+    uint32_t tag;
+    float pad[2];
+};
+static_assert(sizeof(NrcRecord) == 64, "cache record is 64 bytes");
+
 // One render job of a wave: a scene (BVH, shading records, materials, lights, sky), a
 // camera and sampling parameters, and where its paths sit in the wave.  A wave may hold
 // paths of several jobs (batched rendering); job j owns the paths [path_base[j],
@@ -137,7 +157,10 @@ struct JobRec {
     uint32_t sample0;       // sample index of the job's first sample in this wave
     uint32_t samples;       // samples of the job in this wave
     uint32_t seed;
-    uint32_t flags;         // bit 0: pixel jitter
+    uint32_t flags;         // bit 0: pixel jitter; kJobCache, kJobTrain
+    uint32_t cache_bounce;  // segment at which the radiance cache takes over (kJobCache) or
+                            // at which a training path records its sample (kJobTrain)
+    NrcRecord* cache_rec;   // kJobTrain: one record per slot
     float ray_length;
     uint32_t pix_base;      // first resolve slot of the job in this wave
     float* sums;            // running per-pixel sums (3 floats per pixel)

@@ -36,11 +36,11 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
                                                 uint32_t local, bool primary) {
     const size_t cap = a.ps.cap;
     uint32_t mask = 0;
-    const uint32_t npix = J.npix;
-    const uint32_t q = local % npix, sl = local / npix;
-    const uint32_t pixel = pixel_of(q, J.cam.width, J.cam.height);
-    const uint32_t sample = J.sample0 + sl;
+    const PathKey key = path_key(J, local);
+    const uint32_t pixel = key.pixel;
+    const uint32_t sample = key.sample;
     const uint32_t nl = J.num_lights;
+    if (a.nrc_query) a.nrc_query[i].tag = kNoPath;   // entries without a cache query
     const uint32_t nslots = nl + 2u;
     V3 o, d, P = v3(0.0f, 0.0f, 0.0f);
     V3 beta;
@@ -90,6 +90,34 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
         Mat m = load_mat(J.materials[mat_id]);
         if (m.two_sided && dot(d, N) > 0.0f) N = -N;
         V3 V = normalize(o - P);
+        if (a.bounce == J.cache_bounce && (J.flags & (kJobCache | kJobTrain))) {
+            NrcRecord r;
+            r.p[0] = P.x; r.p[1] = P.y; r.p[2] = P.z;
+            r.wo[0] = V.x; r.wo[1] = V.y; r.wo[2] = V.z;
+            r.n[0] = N.x; r.n[1] = N.y; r.n[2] = N.z;
+            r.roughness = J.materials[mat_id].roughness;
+            r.albedo[0] = m.base.x; r.albedo[1] = m.base.y; r.albedo[2] = m.base.z;
+            r.pad[0] = 0.0f; r.pad[1] = 0.0f;
+            if (J.flags & kJobCache) {
+                // the path ends here: the cache's estimate of the radiance leaving P
+                // towards V (added by the cache kernel, times the throughput) stands
+                // for this vertex's lighting and everything after it
+                r.tag = path;
+                a.nrc_query[i] = r;
+                return 0u;
+            }
+            // training path: the radiance gathered from here on, per unit throughput,
+            // is the target of this sample
+            r.tag = 1u;
+            J.cache_rec[key.q] = r;
+            beta = v3(1.0f, 1.0f, 1.0f);
+            for (uint32_t k = 0; k < nslots; ++k) {
+                float* acc = a.ps.acc + (size_t)k * 3u * cap;
+                acc[path] = 0.0f;
+                acc[cap + path] = 0.0f;
+                acc[2 * cap + path] = 0.0f;
+            }
+        }
         Frame fr = make_frame(N);
         Pcg rng = pcg_keyed(J.seed, pixel, sample, a.bounce);
         float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
@@ -319,7 +347,8 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
     const JobRec& J = a.jobs.rec[jid];
     const uint32_t q = slot - J.pix_base;
     const uint32_t npix = J.npix;
-    const uint32_t pixel = pixel_of(q, J.cam.width, J.cam.height);
+    // a training job's slot q sums into target q
+    const uint32_t pixel = (J.flags & kJobTrain) ? q : pixel_of(q, J.cam.width, J.cam.height);
     const uint32_t nslots = J.num_lights + 2u;
     const uint32_t samples = J.samples;
     const uint32_t pbase = a.jobs.path_base[jid];
@@ -346,6 +375,7 @@ __global__ __launch_bounds__(256) void k_output(WaveJobs jobs, uint32_t num_pix)
     if (slot >= num_pix) return;
     const uint32_t jid = job_of(jobs.pix_base, jobs.num_jobs, slot);
     const JobRec& J = jobs.rec[jid];
+    if (!J.out_rgba) return;   // training targets have no image
     const uint32_t p = slot - J.pix_base;
     const float den = J.out_den;
     const float* sums = J.sums;

@@ -385,15 +385,14 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
                 max_steps = J.num_tris > 1u ? J.num_tris : 1u;
                 tmax = J.ray_length;
                 if (PRIMARY) {
-                    const uint32_t q = local % J.npix, sl = local / J.npix;
-                    const uint32_t pixel = pixel_of(q, J.cam.width, J.cam.height);
+                    const PathKey k = path_key(J, local);
                     float jx = 0.5f, jy = 0.5f;
                     if (J.flags & 1u) {
-                        Pcg jr = pcg_keyed(J.seed, pixel, J.sample0 + sl, kJitterBounce);
+                        Pcg jr = pcg_keyed(J.seed, k.pixel, k.sample, kJitterBounce);
                         jx = pcg_float(jr);
                         jy = pcg_float(jr);
                     }
-                    camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, jx, jy, &no, &nd);
+                    camera_ray(J.cam, k.pixel % J.cam.width, k.pixel / J.cam.width, jx, jy, &no, &nd);
                 }
             };
             if (a.jobs.num_jobs == 1u) {

@@ -285,6 +285,62 @@ bi_pt_status bi_pt_server_take_result(bi_pt_server* server, uint64_t job_id, flo
                                       uint64_t capacity_floats);
 bi_pt_status bi_pt_server_get_stats(bi_pt_server* server, bi_pt_server_stats* out);
 
+/* ---- neural radiance cache (no reference counterpart: the reference traces every
+ * path to its last segment, path_tracing.cpp:290-343) ----
+ * A small MLP (64 inputs, 4 hidden layers of 64 with ReLU, RGB out; BF16 on the matrix
+ * cores with FP32 accumulation) predicts the radiance leaving a surface point.  In
+ * bi_pt_render_nrc the image's paths end in the cache at segment `cache_bounce`; the
+ * same waves trace `train_samples` extra paths from random pixels to the full depth,
+ * whose radiance gathered after that segment (per unit throughput) is the target of
+ * their sample there; after the frame the cache takes `train_steps` Adam steps on those
+ * samples (relative L2 loss), so it trains online on the tracer's own path suffixes.
+ * Until `use_after_steps` steps have been taken the image is rendered plainly. */
+typedef struct bi_pt_nrc bi_pt_nrc;
+typedef struct bi_pt_nrc_desc {
+    uint32_t train_samples;       /* training paths per frame; 0 = 65,536 (a multiple of 64 * train_steps) */
+    uint32_t train_steps;         /* Adam steps per frame; 0 = 4 */
+    uint32_t cache_bounce;        /* segment at which paths end in the cache; 0 = 2 */
+    float learning_rate;          /* 0 = 1e-2 (beta1 0.9, beta2 0.99, eps 1e-15) */
+    uint32_t use_after_steps;     /* plain rendering before this many training steps */
+    uint32_t seed;                /* initial weights */
+    uint32_t flags;               /* BI_PT_NRC_FREEZE: render with the cache, no training */
+} bi_pt_nrc_desc;
+enum { BI_PT_NRC_FREEZE = 1u << 0 };
+typedef struct bi_pt_nrc_stats {
+    uint64_t frames, steps;
+    float last_loss;              /* mean relative L2 of the last frame's (or step's) valid samples */
+    uint32_t _pad;
+    uint64_t last_valid_samples;  /* training paths that reached their sample */
+} bi_pt_nrc_stats;
+/* The cache renders the scene of `ctx` into ctx's running image. */
+bi_pt_status bi_pt_nrc_create(bi_pt_ctx* ctx, const bi_pt_nrc_desc* desc, bi_pt_nrc** out_nrc);
+bi_pt_status bi_pt_nrc_destroy(bi_pt_nrc* nrc);
+const char* bi_pt_nrc_last_error(const bi_pt_nrc* nrc);
+/* bi_pt_render with the cache (settings and outputs as there), then one frame of training. */
+bi_pt_status bi_pt_render_nrc(bi_pt_nrc* nrc, const bi_pt_camera* camera, const bi_pt_settings* settings,
+                              float* out_rgba32f, float* out_depth, bi_pt_stream stream);
+bi_pt_status bi_pt_nrc_get_stats(bi_pt_nrc* nrc, bi_pt_nrc_stats* out);
+/* FP32 master weights, 17,408 floats: W_l[k][i] row-major for l = 0..3 (64 x 64), then
+ * W_4 (64 x 16, outputs 3..15 zero).  Setting them restarts Adam. */
+bi_pt_status bi_pt_nrc_get_params(bi_pt_nrc* nrc, float* params, uint32_t count);
+bi_pt_status bi_pt_nrc_set_params(bi_pt_nrc* nrc, const float* params, uint32_t count);
+/* Box that normalises positions (lo.xyz, hi.xyz); each render sets it to the scene's
+ * triangle-centroid bounds. */
+bi_pt_status bi_pt_nrc_set_bounds(bi_pt_nrc* nrc, const float lo_hi[6]);
+/* Direct use of the network on device records (64 B each: position, direction towards
+ * the viewer, normal, roughness, base colour, tag; a training sample counts when tag
+ * == 1): raw outputs (3 floats per record, device), or one Adam step on count samples
+ * with device targets (3 floats each). */
+bi_pt_status bi_pt_nrc_infer(bi_pt_nrc* nrc, const void* records, uint32_t count, float* out_rgb,
+                             bi_pt_stream stream);
+bi_pt_status bi_pt_nrc_train(bi_pt_nrc* nrc, const void* records, const float* targets, uint32_t count,
+                             bi_pt_stream stream);
+/* Gradient of the last step's loss (the sum of its chunk partials), 17,408 floats. */
+bi_pt_status bi_pt_nrc_debug_gradient(bi_pt_nrc* nrc, float* grad, uint32_t count);
+/* The last frame's first `count` training samples: records (16 floats each) and
+ * targets (3 floats each), to host memory. */
+bi_pt_status bi_pt_nrc_debug_samples(bi_pt_nrc* nrc, float* records, float* targets, uint32_t count);
+
 /* ---- inspection hooks for parity tests (no reference counterpart) ----
  * Copy the last-built LBVH to host memory: morton[n] (input triangle order),
  * order[n] (sorted permutation), children[2*max(n-1,1)] (internal k >= 0, leaf j as ~j),

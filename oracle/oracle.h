@@ -125,6 +125,22 @@ int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
                  uint8_t* path_segments, uint8_t* shadow_bits,
                  uint64_t* ray_counts, int num_threads);
 
+/* Radiance-cache modes of bi_pt_render_nrc (src: the build's cache; no reference
+ * counterpart), for a cache whose network output is the constant cache_rgb:
+ * ora_render32_cached renders like ora_render32 but every path that hits at segment
+ * cache_bounce ends there with throughput x max(cache_rgb, 0) as its radiance of that
+ * vertex and beyond.  ora_train_samples traces training slot q = 0..n-1 (pixel drawn by
+ * PCG32 keyed (seed, q, 0, 254) uniformly over the image, sample q, jittered): rec[16q..]
+ * receives the record of the hit at segment cache_bounce (all zero when the path does
+ * not reach it) and target[3q..] the radiance gathered after it with throughput 1 --
+ * with suffixes > 1, the mean over that many continuations from the recorded vertex
+ * (continuation k draws with the seed xor k * 0x9E3779B9; k = 0 is the path's own). */
+int ora_render32_cached(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t cache_bounce,
+                        const float* cache_rgb, uint32_t sample_base, uint32_t spp, float* sum_rgb,
+                        int num_threads);
+int ora_train_samples(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t cache_bounce,
+                      uint32_t n, uint32_t suffixes, float* rec, float* target);
+
 /* Path-order pixel index of path-order position q (tile order when W and H are
  * multiples of 8, else row-major). */
 uint32_t ora_pixel_of(uint32_t q, uint32_t width, uint32_t height);

@@ -70,6 +70,12 @@ def lib():
         L.ora_render32.restype = C.c_int
         L.ora_render32.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, C.c_uint32,
                                    _f32p, _u8p, _u8p, _u64p, C.c_int]
+        L.ora_render32_cached.restype = C.c_int
+        L.ora_render32_cached.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, _f32p,
+                                          C.c_uint32, C.c_uint32, _f32p, C.c_int]
+        L.ora_train_samples.restype = C.c_int
+        L.ora_train_samples.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32,
+                                        C.c_uint32, C.c_uint32, _f32p, _f32p]
         L.ora_pixel_of.restype = C.c_uint32
         L.ora_pixel_of.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.ora_render64.restype = C.c_int
@@ -231,6 +237,28 @@ class OracleScene:
             out["segments"] = segs
             out["shadow_bits"] = bits
         return out
+
+    def render32_cached(self, cam, max_bounces, seed, spp, cache_bounce, cache_rgb, flags=0, ray_length=1000.0,
+                        threads=0):
+        """Running sums of paths that end at segment cache_bounce in a constant cache."""
+        sums = np.zeros((cam.width * cam.height, 3), np.float32)
+        rgb = np.ascontiguousarray(cache_rgb, np.float32)
+        rc = lib().ora_render32_cached(self.h, C.byref(camera_struct(cam)),
+                                       C.byref(settings_struct(max_bounces, seed, ray_length, flags)), cache_bounce,
+                                       ptr(rgb, _f32p), 0, spp, ptr(sums, _f32p), threads)
+        assert rc == 0
+        return sums
+
+    def train_samples(self, cam, max_bounces, seed, cache_bounce, n, suffixes=1, flags=0, ray_length=1000.0):
+        """Records [n, 16] and targets [n, 3] of the cache's training slots 0..n-1 (targets
+        averaged over `suffixes` continuations from each recorded vertex)."""
+        rec = np.zeros((n, 16), np.float32)
+        tgt = np.zeros((n, 3), np.float32)
+        rc = lib().ora_train_samples(self.h, C.byref(camera_struct(cam)),
+                                     C.byref(settings_struct(max_bounces, seed, ray_length, flags)), cache_bounce, n,
+                                     suffixes, ptr(rec, _f32p), ptr(tgt, _f32p))
+        assert rc == 0
+        return rec, tgt
 
     def render64(self, cam, max_bounces, pixels, spp, seed64=0x64, ray_length=1000.0, flags=0, threads=0):
         pixels = np.ascontiguousarray(pixels, np.uint32)

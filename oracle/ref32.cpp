@@ -759,8 +759,25 @@ struct PathOut {
 
 constexpr int kMaxDirLights = 4;
 
+// Radiance-cache modes of a path (bi_pt_render_nrc): at the hit of segment `bounce` the
+// path either ends in a cache whose output is the constant `rgb` (its radiance of that
+// vertex and beyond = throughput x rgb, in the miss slot), or -- a training path --
+// records the vertex (16 floats: position, direction to the previous vertex, normal,
+// roughness, base colour, tag 1) and restarts with throughput 1 and no radiance, so
+// what it gathers afterwards is the sample's target.
+// A training path's segments from `bounce` on draw their random numbers with the seed
+// xor suffix_key (0: the path's own stream, as the GPU traces it), so several runs share
+// the recorded vertex and differ only after it.
+struct CacheMode {
+    int bounce = -1;
+    bool train = false;
+    F3 rgb = {0.0f, 0.0f, 0.0f};
+    float* rec = nullptr;
+    uint32_t suffix_key = 0;
+};
+
 PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st,
-                   uint32_t pixel, uint32_t sample) {
+                   uint32_t pixel, uint32_t sample, const CacheMode* cm = nullptr) {
     PathOut out{};
     const uint32_t M = st.max_bounces < 2 ? 2 : (st.max_bounces > 16 ? 16 : st.max_bounces);
     const uint32_t D = M - 1;
@@ -800,8 +817,30 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         Mat m = load_mat(&s.materials[16 * s.tri_mat[tri]]);
         if (m.two_sided && dot(ray.d, N) > 0.0f) N = -N;
         F3 V = normalize(ray.o - P);
+        if (cm && (int)b == cm->bounce) {
+            if (!cm->train) {
+                F3 c = f3(beta.x * fmax_(cm->rgb.x, 0.0f), beta.y * fmax_(cm->rgb.y, 0.0f),
+                          beta.z * fmax_(cm->rgb.z, 0.0f));
+                acc_miss = finite3(c) ? c : f3(0, 0, 0);
+                break;
+            }
+            const float* mr = &s.materials[16 * s.tri_mat[tri]];
+            float* r = cm->rec;
+            r[0] = P.x; r[1] = P.y; r[2] = P.z;
+            r[3] = V.x; r[4] = V.y; r[5] = V.z;
+            r[6] = N.x; r[7] = N.y; r[8] = N.z;
+            r[9] = mr[3];
+            r[10] = mr[0]; r[11] = mr[1]; r[12] = mr[2];
+            const uint32_t tag = 1u;
+            std::memcpy(&r[13], &tag, 4);
+            r[14] = r[15] = 0.0f;
+            beta = f3(1.0f, 1.0f, 1.0f);
+            acc_miss = acc_sky = f3(0, 0, 0);
+            for (int l = 0; l < kMaxDirLights; ++l) acc_dir[l] = f3(0, 0, 0);
+        }
         Frame fr = make_frame(N);
-        Pcg rng = pcg_keyed(st.seed, pixel, sample, b);
+        const bool suffix = cm && cm->train && (int)b >= cm->bounce;
+        Pcg rng = pcg_keyed(suffix ? st.seed ^ cm->suffix_key : st.seed, pixel, sample, b);
         float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
         // directional lights (deferred_lighting_secondary.hlsl:45-60) with shadow rays
         for (uint32_t l = 0; l < nl; ++l) {
@@ -894,9 +933,70 @@ extern "C" {
 
 uint32_t ora_pixel_of(uint32_t q, uint32_t width, uint32_t height) { return pixel_of(q, width, height); }
 
+namespace {
+int render32_impl(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st,
+                  uint32_t sample_base, uint32_t spp, float* sum_rgb, uint8_t* path_segments,
+                  uint8_t* shadow_bits, uint64_t* ray_counts, int num_threads, const CacheMode* cm);
+}
+
 int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st,
                  uint32_t sample_base, uint32_t spp, float* sum_rgb, uint8_t* path_segments,
                  uint8_t* shadow_bits, uint64_t* ray_counts, int num_threads) {
+    return render32_impl(ctx, cam, st, sample_base, spp, sum_rgb, path_segments, shadow_bits, ray_counts,
+                         num_threads, nullptr);
+}
+
+int ora_render32_cached(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t cache_bounce,
+                        const float* cache_rgb, uint32_t sample_base, uint32_t spp, float* sum_rgb, int num_threads) {
+    CacheMode cm;
+    cm.bounce = (int)cache_bounce;
+    cm.rgb = f3(cache_rgb[0], cache_rgb[1], cache_rgb[2]);
+    return render32_impl(ctx, cam, st, sample_base, spp, sum_rgb, nullptr, nullptr, nullptr, num_threads, &cm);
+}
+
+int ora_train_samples(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t cache_bounce,
+                      uint32_t n, uint32_t suffixes, float* rec, float* target) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    if (cam->width == 0 || cam->height == 0) return -1;
+    const uint64_t npix = (uint64_t)cam->width * cam->height;
+    ora_settings js = *st;
+    js.flags |= 1u;   // training paths are jittered
+    for (uint32_t q = 0; q < n; ++q) {
+        Pcg r = pcg_keyed(st->seed, q, 0u, 254u);
+        const uint32_t pixel = (uint32_t)(((uint64_t)pcg_next(r) * npix) >> 32);
+        CacheMode cm;
+        cm.bounce = (int)cache_bounce;
+        cm.train = true;
+        cm.rec = &rec[16 * (size_t)q];
+        std::memset(cm.rec, 0, 64);
+        PathOut po = trace_path(s, *cam, js, pixel, q, &cm);
+        F3 t = po.radiance;
+        uint32_t tag = 0;
+        std::memcpy(&tag, &cm.rec[13], 4);
+        if (suffixes > 1 && tag == 1u) {
+            // mean over `suffixes` continuations from the recorded vertex (run 0 above)
+            double m[3] = {t.x, t.y, t.z};
+            float keep[16];
+            std::memcpy(keep, cm.rec, 64);
+            for (uint32_t k = 1; k < suffixes; ++k) {
+                cm.suffix_key = k * 0x9E3779B9u;
+                PathOut pk = trace_path(s, *cam, js, pixel, q, &cm);
+                m[0] += pk.radiance.x; m[1] += pk.radiance.y; m[2] += pk.radiance.z;
+            }
+            std::memcpy(cm.rec, keep, 64);
+            t = f3((float)(m[0] / suffixes), (float)(m[1] / suffixes), (float)(m[2] / suffixes));
+        }
+        target[3 * (size_t)q] = t.x;
+        target[3 * (size_t)q + 1] = t.y;
+        target[3 * (size_t)q + 2] = t.z;
+    }
+    return 0;
+}
+
+namespace {
+int render32_impl(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st,
+                  uint32_t sample_base, uint32_t spp, float* sum_rgb, uint8_t* path_segments,
+                  uint8_t* shadow_bits, uint64_t* ray_counts, int num_threads, const CacheMode* cm) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);
     const uint32_t W = cam->width, H = cam->height, npix = W * H;
     if (W == 0 || H == 0) return -1;
@@ -914,7 +1014,7 @@ int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
                 uint32_t p = pixel_of(q, W, H);
                 F3 sum = f3(sum_rgb[3 * p], sum_rgb[3 * p + 1], sum_rgb[3 * p + 2]);
                 for (uint32_t sl = 0; sl < spp; ++sl) {
-                    PathOut po = trace_path(s, *cam, *st, p, sample_base + sl);
+                    PathOut po = trace_path(s, *cam, *st, p, sample_base + sl, cm);
                     sum = sum + po.radiance;
                     size_t pi = (size_t)sl * npix + q;
                     if (path_segments) path_segments[pi] = (uint8_t)po.segments;
@@ -940,6 +1040,7 @@ int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
     }
     return 0;
 }
+}  // namespace
 
 void ora_trace(const ora_ctx* ctx, uint32_t n, const float* rays, int any_hit, float* hit_t,
                uint32_t* hit_leaf, float* hit_u, float* hit_v) {
