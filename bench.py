@@ -288,7 +288,8 @@ def batch_leg(steps):
             "calls": steps, "rays_per_call": int(batch_rays),
             "plain_ms_per_call": round(plain_s * 1e3, 2), "plain_mrays_s": round(plain_rays / plain_s / 1e6, 1),
             "batched_ms_per_call": round(batch_s * 1e3, 2), "batched_mrays_s": round(batch_rays / batch_s / 1e6, 1),
-            "tuning": {k: int(tuned[k]) for k in ("wave_paths", "stack_lds", "refill_lanes")},
+            "tuning": {"wave_paths": int(tuned["wave_paths"]) or 1 << 25,   # 0 = the engine's default
+                       "stack_lds": int(tuned["stack_lds"]), "refill_lanes": int(tuned["refill_lanes"])},
             "autotune_s": round(tune_s, 1)}
 
 

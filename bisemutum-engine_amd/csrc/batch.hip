@@ -126,7 +126,7 @@ bi_pt_status bi_pt_batch_get_stats(bi_pt_batch* b, bi_pt_stats* out) {
     return engine_check(b, b->eng, out);
 }
 
-// Coordinate descent over the knobs that matter most (wave capacity, LDS stack depth,
+// Coordinate descent over the knobs that matter most (wave capacity 8M / 16M / 32M paths, LDS stack depth,
 // refill threshold), each candidate timed with HIP events over `rounds` calls; the
 // jobs render with accumulation off so each call does the same work.
 bi_pt_status bi_pt_batch_autotune(bi_pt_batch* b, const bi_pt_job* jobs, uint32_t n, uint32_t rounds,
@@ -164,7 +164,7 @@ bi_pt_status bi_pt_batch_autotune(bi_pt_batch* b, const bi_pt_job* jobs, uint32_
     Knobs best = b->knobs;
     float best_ms = 0.0f;
     bi_pt_status status = time_knobs(best, &best_ms);
-    const uint32_t waves[] = {1u << 21, 1u << 22, 1u << 23};
+    const uint32_t waves[] = {1u << 23, 1u << 24, 1u << 25};
     const uint32_t stacks[] = {8u, 16u, 32u};
     const uint32_t refills[] = {24u, 32u, 48u};
     for (int axis = 0; axis < 3 && status == BI_PT_OK; ++axis) {

@@ -197,6 +197,8 @@ struct PathState {
 };
 
 // Device-side counters of one wave (uint32 slots).
+constexpr uint32_t kDefaultWavePaths = 1u << 25;   // paths per wave unless a job asks otherwise
+constexpr uint32_t kMaxWavePaths = 1u << 28;       // path << 3 | kind stays within 32 bits
 constexpr uint32_t kWorkParts = 8;     // default ray-queue parts of the persistent lanes (one per XCD)
 constexpr uint32_t kWorkPartsMax = 32;
 constexpr uint32_t kWorkStride = 32;   // uint32 between part counters: one 128-B line each

@@ -55,7 +55,7 @@ struct bi_pt_server : bipt::Owner {
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     uint64_t reserve = 0;          // wave buffers at full capacity for the resident scenes
-    uint32_t wave_cap = 1u << 22;
+    uint32_t wave_cap = kDefaultWavePaths;   // the wave engine's capacity (what `reserve` covers)
     uint64_t used() const { return scene_bytes + job_bytes + std::max<uint64_t>(reserve, batch->allocated); }
 };
 
@@ -93,10 +93,10 @@ bi_pt_status bi_pt_server_create(const bi_pt_server_desc* d, bi_pt_server** out)
         delete s;
         return st;
     }
+    if (d->wave_paths) s->wave_cap = std::min(d->wave_paths, kMaxWavePaths);
     bi_pt_tuning t{};
-    t.wave_paths = d->wave_paths;
+    t.wave_paths = s->wave_cap;
     (void)bi_pt_batch_set_tuning(s->batch, &t);
-    if (d->wave_paths) s->wave_cap = d->wave_paths;
     *out = s;
     return BI_PT_OK;
 }

@@ -132,7 +132,7 @@ typedef struct bi_pt_settings {
     uint32_t spp;             /* samples per pixel rendered by this call */
     uint32_t seed;            /* PCG32 streams keyed by (seed, pixel, sample, bounce) */
     uint32_t flags;
-    uint32_t wave_paths;      /* paths in flight per wave; 0 = automatic */
+    uint32_t wave_paths;      /* paths in flight per wave; 0 = 2^25 (about 270 B of queues per path), at most 2^28 */
     uint32_t stack_lds;       /* traversal stack entries kept in LDS (0 = automatic, <= 32) */
     uint32_t refill_lanes;    /* idle lanes before a wavefront fetches new rays (0 = automatic, <= 64) */
     uint32_t work_parts;      /* ray-queue parts with their own fetch counter (0 = automatic, <= 32) */
@@ -247,7 +247,7 @@ typedef struct bi_pt_server_desc {
     uint32_t max_jobs_per_round;   /* 0 = 64 */
     uint64_t hbm_budget_bytes;     /* 0 = unlimited */
     uint32_t spp_per_round;        /* samples of each job per round; 0 = 4 */
-    uint32_t wave_paths;           /* paths in flight per wave; 0 = automatic */
+    uint32_t wave_paths;           /* paths in flight per wave; 0 = 2^25 (the budget reserves its queues) */
 } bi_pt_server_desc;
 typedef struct bi_pt_request {
     uint32_t scene_id;             /* from bi_pt_server_add_scene */
