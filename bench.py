@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--stack-lds", type=int, default=0)
     ap.add_argument("--refill-lanes", type=int, default=0)
     ap.add_argument("--work-parts", type=int, default=0)
+    ap.add_argument("--block-threads", type=int, default=0, help="traversal workgroup size: 128, 256 or 512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache leg")
@@ -116,7 +117,7 @@ def main():
     def step(k, flags=0):
         pt.render(cam, spp=spp, max_bounces=mb, seed=seed + k, flags=flags, wave_paths=args.wave_paths,
                   stack_lds=args.stack_lds, refill_lanes=args.refill_lanes, work_parts=args.work_parts,
-                  stream=stream)
+                  block_threads=args.block_threads, stream=stream)
 
     for k in range(args.warmup):
         step(k)
@@ -180,7 +181,7 @@ def main():
                         f"{cam.width}x{cam.height}, {spp} spp, depth {mb - 1}",
             "width": cam.width, "height": cam.height, "spp": spp, "max_bounces": mb,
             "wave_paths": args.wave_paths, "stack_lds": args.stack_lds,
-            "refill_lanes": args.refill_lanes, "work_parts": args.work_parts,
+            "refill_lanes": args.refill_lanes, "work_parts": args.work_parts, "block_threads": args.block_threads,
         },
         "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                            "p99": round(float(np.percentile(lat, 99)), 3)},
@@ -267,7 +268,7 @@ def batch_leg(steps):
         tuned = b.autotune(jobs, rounds=1, stream=sh)
         tune_s = time.perf_counter() - t
         b.set_tuning(use_graph=True, **{k: tuned[k] for k in ("wave_paths", "stack_lds", "refill_lanes",
-                                                              "work_parts")})
+                                                              "work_parts", "block_threads")})
         b.render(jobs, stream=sh)          # capture
         stream.synchronize()
         r0 = rays_of([b])
@@ -289,7 +290,8 @@ def batch_leg(steps):
             "plain_ms_per_call": round(plain_s * 1e3, 2), "plain_mrays_s": round(plain_rays / plain_s / 1e6, 1),
             "batched_ms_per_call": round(batch_s * 1e3, 2), "batched_mrays_s": round(batch_rays / batch_s / 1e6, 1),
             "tuning": {"wave_paths": int(tuned["wave_paths"]) or 1 << 25,   # 0 = the engine's default
-                       "stack_lds": int(tuned["stack_lds"]), "refill_lanes": int(tuned["refill_lanes"])},
+                       "stack_lds": int(tuned["stack_lds"]) or 16, "refill_lanes": int(tuned["refill_lanes"]) or 32,
+                       "block_threads": int(tuned["block_threads"]) or 256},
             "autotune_s": round(tune_s, 1)}
 
 

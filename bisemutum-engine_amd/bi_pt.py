@@ -68,7 +68,8 @@ class CameraC(C.Structure):
 class Settings(C.Structure):
     _fields_ = [("ray_length", C.c_float), ("max_bounces", C.c_uint32), ("accumulate", C.c_uint32),
                 ("spp", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("wave_paths", C.c_uint32),
-                ("stack_lds", C.c_uint32), ("refill_lanes", C.c_uint32), ("work_parts", C.c_uint32)]
+                ("stack_lds", C.c_uint32), ("refill_lanes", C.c_uint32), ("work_parts", C.c_uint32),
+                ("block_threads", C.c_uint32)]
 
 
 class JobC(C.Structure):
@@ -78,7 +79,7 @@ class JobC(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wave_paths", C.c_uint32), ("stack_lds", C.c_uint32), ("refill_lanes", C.c_uint32),
-                ("work_parts", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("work_parts", C.c_uint32), ("use_graph", C.c_uint32), ("block_threads", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -291,7 +292,7 @@ class PathTracer:
 
     # ---------------------------------------------------------------- render
     def render(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
-               wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, depth=False, stream=None):
+               wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, block_threads=0, depth=False, stream=None):
         torch = self.torch
         npix = cam.width * cam.height
         dev = torch.device("cuda", self.device)
@@ -303,7 +304,7 @@ class PathTracer:
                 self._depth = torch.empty(npix, dtype=torch.float32, device=dev)
             dptr = C.c_void_p(self._depth.data_ptr())
         st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, wave_paths, stack_lds,
-                      refill_lanes, work_parts)
+                      refill_lanes, work_parts, block_threads)
         s = C.c_void_p(stream) if stream else None
         self._check(lib().bi_pt_render(self.h, C.byref(camera_struct(cam)), C.byref(st),
                                        C.c_void_p(self._out.data_ptr()), dptr, s))
@@ -442,8 +443,8 @@ class Batch:
         arr, keep = self._jobs(jobs)
         self._check(lib().bi_pt_render_batch(self.h, arr, len(jobs), C.c_void_p(stream) if stream else None))
 
-    def set_tuning(self, wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, use_graph=False):
-        t = Tuning(wave_paths, stack_lds, refill_lanes, work_parts, 1 if use_graph else 0)
+    def set_tuning(self, wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, use_graph=False, block_threads=0):
+        t = Tuning(wave_paths, stack_lds, refill_lanes, work_parts, 1 if use_graph else 0, block_threads)
         self._check(lib().bi_pt_batch_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
@@ -566,7 +567,7 @@ class Nrc:
             pass
 
     def render(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
-               wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, stream=None):
+               wave_paths=0, stack_lds=0, refill_lanes=0, work_parts=0, block_threads=0, stream=None):
         """bi_pt_render_nrc into the tracer's output buffer (tracer.image() reads it)."""
         t = self.tracer
         torch = self.torch
@@ -574,7 +575,7 @@ class Nrc:
         if t._out is None or t._out.numel() != npix * 4:
             t._out = torch.empty(npix * 4, dtype=torch.float32, device=torch.device("cuda", t.device))
         st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, wave_paths, stack_lds,
-                      refill_lanes, work_parts)
+                      refill_lanes, work_parts, block_threads)
         self._check(lib().bi_pt_render_nrc(self.h, C.byref(camera_struct(cam)), C.byref(st),
                                            C.c_void_p(t._out.data_ptr()), None,
                                            C.c_void_p(stream) if stream else None))

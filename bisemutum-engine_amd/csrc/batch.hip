@@ -15,7 +15,9 @@ using namespace bipt;
 
 namespace {
 
-Knobs knobs_of(const bi_pt_tuning& t) { return Knobs{t.wave_paths, t.stack_lds, t.refill_lanes, t.work_parts}; }
+Knobs knobs_of(const bi_pt_tuning& t) {
+    return Knobs{t.wave_paths, t.stack_lds, t.refill_lanes, t.work_parts, t.block_threads};
+}
 
 bi_pt_tuning tuning_of(const bi_pt_batch* b) {
     bi_pt_tuning t{};
@@ -23,6 +25,7 @@ bi_pt_tuning tuning_of(const bi_pt_batch* b) {
     t.stack_lds = b->knobs.stack_lds;
     t.refill_lanes = b->knobs.refill_lanes;
     t.work_parts = b->knobs.work_parts;
+    t.block_threads = b->knobs.block_threads;
     t.use_graph = b->use_graph ? 1u : 0u;
     return t;
 }
@@ -90,7 +93,8 @@ const char* bi_pt_batch_last_error(const bi_pt_batch* b) { return b ? b->err.c_s
 
 bi_pt_status bi_pt_batch_set_tuning(bi_pt_batch* b, const bi_pt_tuning* t) {
     if (!b || !t) return BI_PT_ERROR_INVALID_ARGUMENT;
-    if (t->stack_lds > 32 || t->refill_lanes > 64 || t->work_parts > kWorkPartsMax)
+    if (t->stack_lds > 32 || t->refill_lanes > 64 || t->work_parts > kWorkPartsMax ||
+        (t->block_threads != 0 && t->block_threads != 128 && t->block_threads != 256 && t->block_threads != 512))
         return fail(b, BI_PT_ERROR_INVALID_ARGUMENT, "tuning out of range");
     b->knobs = knobs_of(*t);
     b->use_graph = t->use_graph != 0;
@@ -126,8 +130,9 @@ bi_pt_status bi_pt_batch_get_stats(bi_pt_batch* b, bi_pt_stats* out) {
     return engine_check(b, b->eng, out);
 }
 
-// Coordinate descent over the knobs that matter most (wave capacity 8M / 16M / 32M paths, LDS stack depth,
-// refill threshold), each candidate timed with HIP events over `rounds` calls; the
+// Coordinate descent over the knobs that matter most (wave capacity 8M / 16M / 32M paths,
+// traversal block shape 128 / 256 / 512 threads, LDS stack depth, refill threshold), each
+// candidate timed with HIP events over `rounds` calls; the
 // jobs render with accumulation off so each call does the same work.
 bi_pt_status bi_pt_batch_autotune(bi_pt_batch* b, const bi_pt_job* jobs, uint32_t n, uint32_t rounds,
                                   bi_pt_stream stream, bi_pt_tuning* best_out) {
@@ -167,12 +172,14 @@ bi_pt_status bi_pt_batch_autotune(bi_pt_batch* b, const bi_pt_job* jobs, uint32_
     const uint32_t waves[] = {1u << 23, 1u << 24, 1u << 25};
     const uint32_t stacks[] = {8u, 16u, 32u};
     const uint32_t refills[] = {24u, 32u, 48u};
-    for (int axis = 0; axis < 3 && status == BI_PT_OK; ++axis) {
+    const uint32_t blocks[] = {128u, 256u, 512u};
+    for (int axis = 0; axis < 4 && status == BI_PT_OK; ++axis) {
         for (int c = 0; c < 3 && status == BI_PT_OK; ++c) {
             Knobs k = best;
             if (axis == 0) k.wave_paths = waves[c];
             if (axis == 1) k.stack_lds = stacks[c];
             if (axis == 2) k.refill_lanes = refills[c];
+            if (axis == 3) k.block_threads = blocks[c];
             float ms = 0.0f;
             status = time_knobs(k, &ms);
             if (status == BI_PT_OK && ms < best_ms) {

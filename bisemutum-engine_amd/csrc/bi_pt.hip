@@ -362,7 +362,7 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
     JobIn job{ctx, camera, st, out_rgba, out_depth};
     RenderOpts opt;
-    opt.knobs = Knobs{st->wave_paths, st->stack_lds, st->refill_lanes, st->work_parts};
+    opt.knobs = Knobs{st->wave_paths, st->stack_lds, st->refill_lanes, st->work_parts, st->block_threads};
     opt.timing = (st->flags & BI_PT_FLAG_TIMING) != 0;
     opt.record = (st->flags & BI_PT_FLAG_RECORD_QUEUES) != 0;
     opt.reuse_bvh = (st->flags & BI_PT_FLAG_REUSE_BVH) != 0;
@@ -519,7 +519,7 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             ta.refill = 32;
             ta.hits = HitBuf{hrec.p};
             ta.error = &ctx->eng.counters.p[kCntError];
-            if (status == BI_PT_OK) launch_trace_closest(ta, n, false, 16, nullptr);
+            if (status == BI_PT_OK) launch_trace_closest(ta, n, false, 16, 256, nullptr);
             std::vector<HitRec> h(n);
             if (hipMemcpy(h.data(), hrec.p, n * sizeof(HitRec), hipMemcpyDeviceToHost) != hipSuccess) {
                 status = BI_PT_ERROR_HIP;

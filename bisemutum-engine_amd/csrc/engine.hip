@@ -261,6 +261,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     const uint32_t lds_stack = kn.stack_lds ? std::min(kn.stack_lds, 32u) : 16u;
     const uint32_t refill = kn.refill_lanes ? std::min(kn.refill_lanes, 64u) : 32u;
     const uint32_t parts = kn.work_parts ? std::min(kn.work_parts, kWorkPartsMax) : kWorkParts;
+    const uint32_t block = kn.block_threads == 128 || kn.block_threads == 512 ? kn.block_threads : 256u;
 
     // device memory: per-job images, BVH builds, wave buffers, plan
     bool any_cached = false, any_train = false;
@@ -388,7 +389,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     const bool graph = graph_owner && graph_owner->use_graph && !timing && !record;
     std::vector<uint64_t> key;
     if (graph) {
-        key = {n, cap, lds_stack, refill, parts, (uint64_t)(uintptr_t)eng.plan.p, eng.wave_cap, eng.wave_slots,
+        key = {n, cap, lds_stack, refill, parts, block, (uint64_t)(uintptr_t)eng.plan.p, eng.wave_cap, eng.wave_slots,
                eng.wave_sh_mult, plan_bytes};
         for (uint32_t j = 0; j < n; ++j) {
             const Prep& p = prep[j];
@@ -491,7 +492,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 ta.refill = refill;
                 ta.error = &cnt[kCntError];
                 BI_TRY(mark(0));
-                launch_trace_closest(ta, J.num_paths, primary, lds_stack, s);
+                launch_trace_closest(ta, J.num_paths, primary, lds_stack, block, s);
                 ++launches_closest;
                 BI_TRY(mark(1));
 
@@ -550,7 +551,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sha.parts = parts;
                 sha.refill = refill;
                 sha.error = &cnt[kCntError];
-                launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, s);
+                launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, block, s);
                 ++launches_shadow;
                 BI_TRY(mark(3));
                 BI_HIP_TRY(own, hipGetLastError());

@@ -618,7 +618,8 @@ bi_pt_status bi_pt_render_nrc(bi_pt_nrc* n, const bi_pt_camera* camera, const bi
     jobs[1] = JobIn{ctx, camera, &tst, nullptr, nullptr};
     jobs[1].role = kRoleTrain;
     RenderOpts opt;
-    opt.knobs = Knobs{settings->wave_paths, settings->stack_lds, settings->refill_lanes, settings->work_parts};
+    opt.knobs = Knobs{settings->wave_paths, settings->stack_lds, settings->refill_lanes, settings->work_parts,
+                      settings->block_threads};
     opt.reuse_bvh = (settings->flags & BI_PT_FLAG_REUSE_BVH) != 0;
     opt.nrc = n;
     BI_TRY(render_jobs(n, ctx->eng, jobs, train ? 2u : 1u, reinterpret_cast<hipStream_t>(stream), opt, nullptr));

@@ -87,6 +87,7 @@ struct Knobs {
     uint32_t stack_lds = 0;
     uint32_t refill_lanes = 0;
     uint32_t work_parts = 0;
+    uint32_t block_threads = 0;   // traversal workgroup size: 128, 256 or 512 (0 = 256)
 };
 
 // Queues and per-wave buffers, sized for the largest wave rendered so far.

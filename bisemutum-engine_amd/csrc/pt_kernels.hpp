@@ -96,7 +96,9 @@ struct TraceArgs {
     uint32_t refill;            // idle lanes before a refill (1..64)
     uint32_t* error;
 };
-void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, hipStream_t s);
+// block: threads per workgroup of the persistent grid (128, 256 or 512; else 256).
+void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, uint32_t block,
+                          hipStream_t s);
 
 void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, float tmax, uint32_t* out,
                          uint32_t* error, hipStream_t s);
@@ -111,7 +113,7 @@ struct ShadowArgs {
     uint32_t refill;            // idle lanes before a refill (1..64)
     uint32_t* error;
 };
-void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, hipStream_t s);
+void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, uint32_t block, hipStream_t s);
 
 // Per-entry staging written by the shade kernel and read by the compaction kernel.
 struct ShadeStage {

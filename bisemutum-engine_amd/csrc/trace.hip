@@ -118,10 +118,10 @@ __device__ __forceinline__ bool tri_test(const LeafGeo* geo, uint32_t leaf, V3 o
 // consumed inside the branch: the common path is then pure LDS, with no generic
 // (flat) access and no wait on the vector-memory counter, which would also wait for
 // the node load issued ahead of it.
-template <int L>
+template <int L, int B = 256>
 struct ShortStack {
     int spill[kStackMax - L];
-    __device__ __forceinline__ bool push(int (*lds)[256], int& sp, int x) {
+    __device__ __forceinline__ bool push(int (*lds)[B], int& sp, int x) {
         const int s = sp;
         if (s >= kStackMax) return false;
         if (s < L) lds[s][threadIdx.x] = x;
@@ -131,7 +131,7 @@ struct ShortStack {
         sp = s + 1;
         return true;
     }
-    __device__ __forceinline__ int pop(int (*lds)[256], int& sp) {
+    __device__ __forceinline__ int pop(int (*lds)[B], int& sp) {
         const int s = sp - 1;
         sp = s;
         int v = lds[s < L ? s : L - 1][threadIdx.x];
@@ -236,13 +236,13 @@ __device__ __forceinline__ uint32_t append_leaves(uint32_t* qleaf, uint8_t* qown
     return qn;
 }
 
-template <bool PRIMARY, int L>
-__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
-    __shared__ int lds[L][256];
-    __shared__ uint32_t s_qleaf[4][kLeafQueue];
-    __shared__ uint8_t s_qown[4][kLeafQueue];
-    __shared__ float s_qtn[4][kLeafQueue];
-    __shared__ unsigned long long s_key[4][64];
+template <bool PRIMARY, int L, int B>
+__global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
+    __shared__ int lds[L][B];
+    __shared__ uint32_t s_qleaf[B / 64][kLeafQueue];
+    __shared__ uint8_t s_qown[B / 64][kLeafQueue];
+    __shared__ float s_qtn[B / 64][kLeafQueue];
+    __shared__ unsigned long long s_key[B / 64][64];
     const uint32_t count = PRIMARY ? a.jobs.num_paths : *a.count_in;
     const float tmin = kTMin;
     // Each lane keeps its ray's scene (BVH nodes, leaf records, triangle count): a wave
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a) {
     const Bvh4Node* nodes = a.jobs.rec[0].nodes;
     const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
     uint32_t max_steps = 1;
-    ShortStack<L> st;
+    ShortStack<L, B> st;
     int sp = 0;
     uint32_t steps = 0;
     Node4 cur = load_node(nodes, 0);
@@ -496,12 +496,12 @@ __device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bo
     acc[2 * cap + path] = acc[2 * cap + path] + c.z;
 }
 
-template <int L>
-__global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
-    __shared__ int lds[L][256];
-    __shared__ uint32_t s_qleaf[4][kLeafQueue];
-    __shared__ uint8_t s_qown[4][kLeafQueue];
-    __shared__ uint32_t s_occ[4][64];
+template <int L, int B>
+__global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
+    __shared__ int lds[L][B];
+    __shared__ uint32_t s_qleaf[B / 64][kLeafQueue];
+    __shared__ uint8_t s_qown[B / 64][kLeafQueue];
+    __shared__ uint32_t s_occ[B / 64][64];
     const uint32_t count = *a.count_in;
     const float tmin = kTMin, tmax = kShadowTMax;
     // Same iteration as the closest-hit kernel: box steps queue their leaves, the queue
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow(ShadowArgs a) {
     const Bvh4Node* nodes = a.jobs.rec[0].nodes;   // the lane's ray's scene (see k_trace_closest)
     const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
     uint32_t max_steps = 1;
-    ShortStack<L> st;
+    ShortStack<L, B> st;
     int sp = 0;
     uint32_t steps = 0;
     Node4 cur = load_node(nodes, 0);
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_debug_anyhit(BvhView bvh, RayQueue q, u
 
 // Workgroups that stay resident on the whole chip for a kernel (persistent grid).
 template <class K>
-uint32_t resident_groups(K kernel) {
+uint32_t resident_groups(K kernel, int block) {
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
@@ -642,44 +642,62 @@ uint32_t resident_groups(K kernel) {
             cus = 256;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1024 / block;
     return (uint32_t)(cus * per_cu);
 }
 
-template <bool P, int L>
-void closest_l(const TraceArgs& a, uint32_t capacity, hipStream_t s) {
-    uint32_t groups = (capacity + 255u) / 256u;
+template <bool P, int L, int B>
+void closest_lb(const TraceArgs& a, uint32_t capacity, hipStream_t s) {
+    uint32_t groups = (capacity + B - 1u) / B;
     static uint32_t resident = 0;
-    if (resident == 0) resident = resident_groups(k_trace_closest<P, L>);
+    if (resident == 0) resident = resident_groups(k_trace_closest<P, L, B>, B);
     if (groups > resident) groups = resident;
     if (groups == 0) groups = 1;
-    hipLaunchKernelGGL((k_trace_closest<P, L>), dim3(groups), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_trace_closest<P, L, B>), dim3(groups), dim3(B), 0, s, a);
 }
 
-template <int L>
-void shadow_l(const ShadowArgs& a, uint32_t capacity, hipStream_t s) {
-    uint32_t groups = (capacity + 255u) / 256u;
+template <int L, int B>
+void shadow_lb(const ShadowArgs& a, uint32_t capacity, hipStream_t s) {
+    uint32_t groups = (capacity + B - 1u) / B;
     static uint32_t resident = 0;
-    if (resident == 0) resident = resident_groups(k_trace_shadow<L>);
+    if (resident == 0) resident = resident_groups(k_trace_shadow<L, B>, B);
     if (groups > resident) groups = resident;
     if (groups == 0) groups = 1;
-    hipLaunchKernelGGL(k_trace_shadow<L>, dim3(groups), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_trace_shadow<L, B>), dim3(groups), dim3(B), 0, s, a);
+}
+
+// Block shapes: 128, 256 (default) or 512 threads; LDS stack depths 8, 16 (default), 24, 32.
+template <bool P, int B>
+void closest_b(const TraceArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
+    if (lds <= 8) closest_lb<P, 8, B>(a, capacity, s);
+    else if (lds <= 16) closest_lb<P, 16, B>(a, capacity, s);
+    else if (lds <= 24) closest_lb<P, 24, B>(a, capacity, s);
+    else closest_lb<P, 32, B>(a, capacity, s);
 }
 
 template <bool P>
-void dispatch_closest(const TraceArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
-    if (lds <= 8) closest_l<P, 8>(a, capacity, s);
-    else if (lds <= 16) closest_l<P, 16>(a, capacity, s);
-    else if (lds <= 24) closest_l<P, 24>(a, capacity, s);
-    else closest_l<P, 32>(a, capacity, s);
+void dispatch_closest(const TraceArgs& a, uint32_t capacity, uint32_t lds, uint32_t block, hipStream_t s) {
+    if (block == 128) closest_b<P, 128>(a, capacity, lds, s);
+    else if (block == 512) closest_b<P, 512>(a, capacity, lds, s);
+    else closest_b<P, 256>(a, capacity, lds, s);
+}
+
+template <int B>
+void shadow_b(const ShadowArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
+    if (lds <= 8) shadow_lb<8, B>(a, capacity, s);
+    else if (lds <= 16) shadow_lb<16, B>(a, capacity, s);
+    else if (lds <= 24) shadow_lb<24, B>(a, capacity, s);
+    else shadow_lb<32, B>(a, capacity, s);
 }
 
 }  // namespace
 
-void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, hipStream_t s) {
+void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, uint32_t block,
+                          hipStream_t s) {
     if (capacity == 0) return;
-    if (primary) dispatch_closest<true>(a, capacity, lds_stack, s);
-    else dispatch_closest<false>(a, capacity, lds_stack, s);
+    if (primary) dispatch_closest<true>(a, capacity, lds_stack, block, s);
+    else dispatch_closest<false>(a, capacity, lds_stack, block, s);
 }
 
 void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, float tmax, uint32_t* out,
@@ -688,12 +706,11 @@ void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, floa
     hipLaunchKernelGGL(k_debug_anyhit<16>, dim3((n + 255u) / 256u), dim3(256), 0, s, bvh, q, n, tmax, out, error);
 }
 
-void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
+void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds, uint32_t block, hipStream_t s) {
     if (capacity == 0) return;
-    if (lds <= 8) shadow_l<8>(a, capacity, s);
-    else if (lds <= 16) shadow_l<16>(a, capacity, s);
-    else if (lds <= 24) shadow_l<24>(a, capacity, s);
-    else shadow_l<32>(a, capacity, s);
+    if (block == 128) shadow_b<128>(a, capacity, lds, s);
+    else if (block == 512) shadow_b<512>(a, capacity, lds, s);
+    else shadow_b<256>(a, capacity, lds, s);
 }
 
 }  // namespace bipt

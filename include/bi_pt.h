@@ -136,6 +136,7 @@ typedef struct bi_pt_settings {
     uint32_t stack_lds;       /* traversal stack entries kept in LDS (0 = automatic, <= 32) */
     uint32_t refill_lanes;    /* idle lanes before a wavefront fetches new rays (0 = automatic, <= 64) */
     uint32_t work_parts;      /* ray-queue parts with their own fetch counter (0 = automatic, <= 32) */
+    uint32_t block_threads;   /* traversal workgroup size: 128, 256 or 512 (0 = 256) */
 } bi_pt_settings;
 
 typedef struct bi_pt_stats {
@@ -216,6 +217,7 @@ typedef struct bi_pt_tuning {
     uint32_t work_parts;    /* ray-queue parts with their own fetch counter (0 = automatic, <= 32) */
     uint32_t use_graph;     /* capture a call's launches in a hipGraph and replay it while the
                                jobs' shapes (sizes, spp, bounces, lights, scenes) repeat */
+    uint32_t block_threads; /* traversal workgroup size: 128, 256 or 512 (0 = 256) */
 } bi_pt_tuning;
 bi_pt_status bi_pt_batch_create(const bi_pt_device_desc* desc, bi_pt_batch** out_batch);
 bi_pt_status bi_pt_batch_destroy(bi_pt_batch* batch);
@@ -228,7 +230,7 @@ bi_pt_status bi_pt_render_batch(bi_pt_batch* batch, const bi_pt_job* jobs, uint3
 /* Ray statistics of the batch (all its calls); checks the device error flags. */
 bi_pt_status bi_pt_batch_get_stats(bi_pt_batch* batch, bi_pt_stats* out);
 /* Picks the batch's tuning for these jobs by timing candidate knob settings (wave
- * capacity, LDS stack depth, refill threshold) on them, `rounds` calls each; writes the
+ * capacity, traversal block shape, LDS stack depth, refill threshold) on them, `rounds` calls each; writes the
  * best to *best (nullable) and installs it.  The jobs' running images are restarted. */
 bi_pt_status bi_pt_batch_autotune(bi_pt_batch* batch, const bi_pt_job* jobs, uint32_t num_jobs, uint32_t rounds,
                                   bi_pt_stream stream, bi_pt_tuning* best);
