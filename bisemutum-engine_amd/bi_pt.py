@@ -36,7 +36,8 @@ EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene
            "bi_pt_server_step", "bi_pt_server_poll", "bi_pt_server_take_result", "bi_pt_server_get_stats",
            "bi_pt_nrc_create", "bi_pt_nrc_destroy", "bi_pt_nrc_last_error", "bi_pt_render_nrc",
            "bi_pt_nrc_get_stats", "bi_pt_nrc_get_params", "bi_pt_nrc_set_params", "bi_pt_nrc_set_bounds",
-           "bi_pt_nrc_infer", "bi_pt_nrc_train", "bi_pt_nrc_debug_gradient", "bi_pt_nrc_debug_samples"]
+           "bi_pt_nrc_infer", "bi_pt_nrc_train", "bi_pt_nrc_debug_gradient", "bi_pt_nrc_debug_samples",
+           "bi_pt_set_build_options"]
 NRC_PARAMS = 17408
 NRC_FREEZE = 1
 
@@ -149,6 +150,7 @@ def lib():
         L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
         L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
         L.bi_pt_reset_accumulation.argtypes = [vp]
+        L.bi_pt_set_build_options.argtypes = [vp, C.POINTER(C.c_uint32 * 4)]
         L.bi_pt_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.bi_pt_debug_get_bvh.argtypes = [vp, u32p, u32p, i32p, f32p, i32p, f32p]
         L.bi_pt_debug_get_bvh4.argtypes = [vp, i32p, f32p]
@@ -320,6 +322,11 @@ class PathTracer:
 
     def reset(self):
         self._check(lib().bi_pt_reset_accumulation(self.h))
+
+    def set_build_options(self, treelet_rounds=1):
+        """Rounds of treelet restructuring after the LBVH (0: the Morton/Karras tree)."""
+        opt = (C.c_uint32 * 4)(treelet_rounds, 0, 0, 0)
+        self._check(lib().bi_pt_set_build_options(self.h, C.byref(opt)))
 
     def stats(self):
         s = Stats()

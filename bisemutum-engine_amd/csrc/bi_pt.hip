@@ -74,6 +74,10 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
     BI_TRY(dalloc(ctx, ctx->refit_cnt, ni));
     BI_TRY(dalloc(ctx, ctx->leaf_geo, n));
     BI_TRY(dalloc(ctx, ctx->leaf_shade, n));
+    BI_TRY(dalloc(ctx, ctx->tl_cost, ni));
+    BI_TRY(dalloc(ctx, ctx->tl_height, ni));
+    BI_TRY(dalloc(ctx, ctx->tl_order, ni));
+    BI_TRY(dalloc(ctx, ctx->tl_bins, 192));
     return BI_PT_OK;
 }
 
@@ -105,6 +109,11 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error) {
     a.leaf_geo = ctx->leaf_geo.p;
     a.leaf_shade = ctx->leaf_shade.p;
     a.error = error;
+    a.treelet_rounds = ctx->treelet_rounds;
+    a.tl_cost = ctx->tl_cost.p;
+    a.tl_height = ctx->tl_height.p;
+    a.tl_order = ctx->tl_order.p;
+    a.tl_bins = ctx->tl_bins.p;
     launch_build_lbvh(a, s);
     BI_HIP_TRY(ctx, hipGetLastError());
     return BI_PT_OK;
@@ -162,6 +171,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     dfree(ctx, ctx->cb_ord); dfree(ctx, ctx->keys0); dfree(ctx, ctx->keys1); dfree(ctx, ctx->vals0);
     dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes);
     dfree(ctx, ctx->nodes4); dfree(ctx, ctx->parent); dfree(ctx, ctx->leaf_geo); dfree(ctx, ctx->leaf_shade);
+    dfree(ctx, ctx->tl_cost); dfree(ctx, ctx->tl_height); dfree(ctx, ctx->tl_order); dfree(ctx, ctx->tl_bins);
     dfree(ctx, ctx->sums);
     engine_free(ctx, ctx->eng);
     delete ctx;
@@ -346,6 +356,17 @@ bi_pt_status bi_pt_create_view(bi_pt_ctx* scene, bi_pt_ctx** out_view) {
     v->scene_src = scene;
     v->has_scene = true;
     *out_view = v;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_set_build_options(bi_pt_ctx* ctx, const bi_pt_build_options* opt) {
+    if (!ctx || !opt) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if (opt->treelet_rounds > 16) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "at most 16 treelet rounds");
+    if (opt->treelet_rounds != ctx->treelet_rounds) {
+        ctx->treelet_rounds = opt->treelet_rounds;
+        ctx->bvh_built = false;   // the next render builds again (its images do not change)
+    }
     return BI_PT_OK;
 }
 

@@ -13,8 +13,11 @@
 //                 tie-break for equal codes
 //   5. leaves     leaf intersection/shading records in sorted order, then the
 //                 bottom-up AABB refit (second arrival at each node continues)
-// All steps are integer or exact min/max except the Morton quantisation, whose
-// float operations are written in a fixed order.
+//   6. treelets   rounds of treelet restructuring of the refit tree (surface-area
+//                 cost; bi_pt_set_build_options, default 1 round)
+//   7. collapse   BVH4 traversal nodes
+// All steps are integer or exact min/max except the Morton quantisation and the
+// surface-area costs, whose float operations are written in a fixed order.
 #include "pt_kernels.hpp"
 
 namespace bipt {
@@ -222,7 +225,8 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n,
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, int n,
                                                 BvhNode* __restrict__ nodes, int32_t* __restrict__ parent,
-                                                uint32_t* __restrict__ refit_cnt) {
+                                                uint32_t* __restrict__ refit_cnt, float* __restrict__ sub_cost,
+                                                int32_t* __restrict__ sub_height) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (n == 1) {
         if (i == 0) {
@@ -263,6 +267,9 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     parent[right < 0 ? (n - 1) + ~right : right] = i;
     if (i == 0) parent[0] = -1;
     refit_cnt[i] = 0;
+    // subtree cost and height, published by the refit when the node is complete
+    sub_cost[i] = __uint_as_float(kBoxPending);
+    sub_height[i] = -1;
 }
 
 // Device-coherent (agent scope) relaxed accesses of box words: no cache write-back or
@@ -274,13 +281,19 @@ __device__ __forceinline__ void st_box(float* p, float v) {
     __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
+// Half surface of a box (lo.xyz, hi.xyz), the surface-area cost unit.
+__device__ __forceinline__ float half_area_box(const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return (dx * dy + dy * dz) + dz * dx;
+}
 
 __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ order, const float* __restrict__ tri_v,
                                                 const float* __restrict__ tri_n,
                                                 const uint32_t* __restrict__ tri_mat, int n,
                                                 LeafGeo* __restrict__ geo, LeafShade* __restrict__ shade,
                                                 BvhNode* nodes, const int32_t* __restrict__ parent,
-                                                uint32_t* refit_cnt, uint32_t* error) {
+                                                uint32_t* refit_cnt, float* sub_cost, int32_t* sub_height,
+                                                uint32_t* error) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     uint32_t t = order[k];
@@ -311,6 +324,10 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
     }
     int p = parent[(n - 1) + k];
     int slot = nodes[p].child[0] == ~k ? 0 : 1;
+    // surface-area cost and height of the subtree this thread carries upward (a
+    // triangle: its box's surface, height 0)
+    float cost = half_area_box(box);
+    int height = 0;
     for (;;) {
         for (int a = 0; a < 6; ++a) st_box(&nodes[p].box[6 * slot + a], box[a]);
         uint32_t old = __hip_atomic_fetch_add(&refit_cnt[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -332,6 +349,36 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
             box[a] = sel_min(box[a], other[a]) + 0.0f;
             box[3 + a] = sel_max(box[3 + a], other[3 + a]) + 0.0f;
         }
+        // the sibling subtree's cost and height: a triangle's from its box, an internal
+        // node's once its thread has published them
+        const int sib = nodes[p].child[1 - slot];
+        float sib_cost;
+        int sib_height;
+        if (sib < 0) {
+            sib_cost = half_area_box(other);
+            sib_height = 0;
+        } else {
+            uint32_t bits = ld_box(sub_cost + sib);
+            uint32_t spins = 0;
+            while (bits == kBoxPending) {
+                if (++spins > (1u << 20)) { atomicOr(error, 4u); bits = 0u; break; }
+                __builtin_amdgcn_s_sleep(1);
+                bits = ld_box(sub_cost + sib);
+            }
+            sib_cost = __uint_as_float(bits);
+            sib_height = __hip_atomic_load(sub_height + sib, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            spins = 0;
+            while (sib_height < 0) {
+                if (++spins > (1u << 20)) { atomicOr(error, 4u); sib_height = 0; break; }
+                __builtin_amdgcn_s_sleep(1);
+                sib_height = __hip_atomic_load(sub_height + sib, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        const float c0 = slot == 0 ? cost : sib_cost, c1 = slot == 0 ? sib_cost : cost;
+        cost = (half_area_box(box) + c0) + c1;
+        height = 1 + max(height, sib_height);
+        st_box(sub_cost + p, cost);
+        __hip_atomic_store(sub_height + p, height, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (p == 0) return;
         int q = parent[p];
         slot = nodes[q].child[0] == p ? 0 : 1;
@@ -349,6 +396,283 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
 __device__ __forceinline__ float half_area(const float* b) {
     const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
     return (dx * dy + dy * dz) + dz * dx;
+}
+
+// ---------------------------------------------------------------- treelet restructuring
+// Rounds of treelet restructuring (Karras & Aila 2013, "Fast Parallel Construction of
+// High-Quality Bounding Volume Hierarchies") lower the tree's surface-area cost after
+// the refit.  Node n's treelet starts as its two children; the treelet leaf that is an
+// internal node with the largest box surface (ties: the lower position) is replaced in
+// place by its first child and its second child appended, until 7 treelet leaves (or
+// only triangles) remain.  Over the subsets of those leaves, cost(s) = area(s) + the
+// cheapest split of s into two parts (the part holding the lowest leaf first, parts
+// enumerated as p = (p - 1) & s, first minimum kept), a single leaf costing its
+// subtree's cost (a triangle: its box's surface).  If the best topology is cheaper than
+// the current one, and does not raise the subtree's height or keeps it <= 48 (so the
+// tree stays within 63 levels and the traversal stack bound holds), the treelet is
+// rebuilt from the same internal nodes, assigned in pre-order (left part first).
+// Only nodes whose subtree is at least kTreeletMinHeight levels tall at the start of a
+// round are restructured.  They are processed by that height, lowest first, one launch
+// per height: nodes of one height have disjoint subtrees and every node finds its
+// subtree already processed, so the result equals a children-first pass in any order --
+// the CPU restatement's recursion.  The refit publishes every subtree's cost and height
+// for the first round; a round leaves them current for the next.
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletMaxHeight = 48;
+constexpr int kTreeletMinHeight = 5;   // lower subtrees are left as built (their share of the cost is small)
+constexpr int kDepthBins = 64;
+
+__device__ __forceinline__ float half_area6(const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return (dx * dy + dy * dz) + dz * dx;
+}
+
+struct TreeletArgs {
+    BvhNode* nodes;
+    int32_t* parent;
+    int ni, n;
+    float* cost;        // per internal node: surface-area cost of its subtree
+    int32_t* height;    // per internal node: levels below it
+    uint32_t* order;    // internal nodes of height >= kTreeletMinHeight grouped by height
+    uint32_t* bins;     // [count[64], cursor[64], start[64]]
+    uint32_t* error;
+};
+
+// Nodes by height, counted and placed with one global atomic per workgroup and height.
+__global__ __launch_bounds__(256) void k_tl_hist(TreeletArgs a) {
+    __shared__ uint32_t h[kDepthBins];
+    if (threadIdx.x < kDepthBins) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (k < a.ni) {
+        const int ht = a.height[k];
+        if (ht >= kTreeletMinHeight && ht < kDepthBins) atomicAdd(&h[ht], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kDepthBins && h[threadIdx.x]) atomicAdd(&a.bins[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_tl_scan(uint32_t* bins) {
+    if (threadIdx.x != 0) return;
+    uint32_t run = 0;
+    for (int d = 0; d < kDepthBins; ++d) {
+        bins[kDepthBins + d] = run;
+        bins[2 * kDepthBins + d] = run;
+        run += bins[d];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tl_scatter(TreeletArgs a) {
+    __shared__ uint32_t cnt[kDepthBins], base[kDepthBins];
+    if (threadIdx.x < kDepthBins) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
+    int ht = -1;
+    uint32_t rank = 0;
+    if (k < a.ni) {
+        ht = a.height[k];
+        if (ht >= kTreeletMinHeight && ht < kDepthBins) rank = atomicAdd(&cnt[ht], 1u);
+        else ht = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDepthBins && cnt[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&a.bins[kDepthBins + threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (ht >= 0) a.order[base[ht] + rank] = (uint32_t)k;
+}
+
+// One treelet per wavefront.  Every lane forms the same treelet (uniform values); the
+// dynamic programme runs over subset sizes, lane l owning subsets l + 1 and l + 65, each
+// subset's cheapest split found by its owning lane in the enumeration order above, the
+// table in LDS; lane 0 rebuilds an accepted treelet.  Register arrays are indexed with
+// compile-time indices only (selects), so nothing goes to scratch memory.
+struct TreeletLds {
+    float cost[1 << kTreeletLeaves];
+    uint8_t split[1 << kTreeletLeaves];
+    uint8_t hh[1 << kTreeletLeaves];
+    int stk[3][kTreeletLeaves];
+};
+
+__device__ __forceinline__ void tl_union(const float (&bx)[kTreeletLeaves][6], int k, int s, float* u) {
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < kTreeletLeaves; ++i) {
+        if (i >= k || !((s >> i) & 1)) continue;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            u[e] = first ? bx[i][e] : sel_min(u[e], bx[i][e]);
+            u[3 + e] = first ? bx[i][3 + e] : sel_max(u[3 + e], bx[i][3 + e]);
+        }
+        first = false;
+    }
+}
+
+template <class T, int N>
+__device__ __forceinline__ T sel_idx(const T (&v)[N], int i) {
+    T r = v[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j) r = i == j ? v[j] : r;
+    return r;
+}
+
+__device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, TreeletLds& L) {
+    BvhNode* nodes = a.nodes;
+    int ref[kTreeletLeaves];
+    float bx[kTreeletLeaves][6];
+    int inner[kTreeletLeaves - 1];
+#pragma unroll
+    for (int i = 0; i < kTreeletLeaves; ++i) {
+        ref[i] = -1;
+        for (int e = 0; e < 6; ++e) bx[i][e] = 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < kTreeletLeaves - 1; ++i) inner[i] = n;
+    int k = 2, ninner = 1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        ref[c] = nodes[n].child[c];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) bx[c][e] = nodes[n].box[6 * c + e];
+    }
+    for (int step = 0; step < kTreeletLeaves - 2; ++step) {
+        int best = -1;
+        float ba = -1.0f;
+#pragma unroll
+        for (int i = 0; i < kTreeletLeaves; ++i) {
+            if (i >= k || ref[i] < 0) continue;
+            const float ar = half_area6(bx[i]);
+            if (ar > ba) { ba = ar; best = i; }
+        }
+        if (best < 0) break;
+        const int m = sel_idx(ref, best);
+        const int c0 = nodes[m].child[0], c1 = nodes[m].child[1];
+        float b0[6], b1[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            b0[e] = nodes[m].box[e];
+            b1[e] = nodes[m].box[6 + e];
+        }
+#pragma unroll
+        for (int i = 0; i < kTreeletLeaves; ++i) {
+            if (i == best) {
+                ref[i] = c0;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) bx[i][e] = b0[e];
+            }
+            if (i == k) {
+                ref[i] = c1;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) bx[i][e] = b1[e];
+            }
+        }
+#pragma unroll
+        for (int i = 1; i < kTreeletLeaves - 1; ++i)
+            if (i == ninner) inner[i] = m;
+        ++ninner;
+        ++k;
+    }
+    float all[6];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        all[e] = sel_min(nodes[n].box[e], nodes[n].box[6 + e]);
+        all[3 + e] = sel_max(nodes[n].box[3 + e], nodes[n].box[9 + e]);
+    }
+    const int r0 = nodes[n].child[0], r1 = nodes[n].child[1];
+    const float cc0 = r0 >= 0 ? a.cost[r0] : half_area6(nodes[n].box);
+    const float cc1 = r1 >= 0 ? a.cost[r1] : half_area6(nodes[n].box + 6);
+    const float cur = (half_area6(all) + cc0) + cc1;
+    const int curh = 1 + max(r0 >= 0 ? a.height[r0] : 0, r1 >= 0 ? a.height[r1] : 0);
+    bool rebuilt = false;
+    if (k >= 3) {
+        const int full = (1 << k) - 1;
+        // subsets by size; lane l owns l + 1 and l + 65
+        for (int size = 1; size <= k; ++size) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int sub = (int)lane + 1 + 64 * h;
+                if (sub > full || __popc(sub) != size) continue;
+                if (size == 1) {
+                    const int i = __builtin_ctz(sub);
+                    const int r = sel_idx(ref, i);
+                    float b[6];
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) {
+                        b[e] = bx[0][e];
+#pragma unroll
+                        for (int j = 1; j < kTreeletLeaves; ++j) b[e] = i == j ? bx[j][e] : b[e];
+                    }
+                    L.cost[sub] = r >= 0 ? a.cost[r] : half_area6(b);
+                    L.hh[sub] = (uint8_t)(r >= 0 ? a.height[r] : 0);
+                    L.split[sub] = 0;
+                    continue;
+                }
+                float u[6];
+                tl_union(bx, k, sub, u);
+                const int low = sub & -sub;
+                float best = 3.0e38f;
+                int bp = 0;
+                for (int p = (sub - 1) & sub; p > 0; p = (p - 1) & sub) {
+                    if (!(p & low)) continue;
+                    const float c = L.cost[p] + L.cost[sub ^ p];
+                    if (c < best) { best = c; bp = p; }
+                }
+                L.cost[sub] = half_area6(u) + best;
+                L.split[sub] = (uint8_t)bp;
+                L.hh[sub] = (uint8_t)(1 + max((int)L.hh[bp], (int)L.hh[sub ^ bp]));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        const float best_cost = L.cost[full];
+        const int best_h = L.hh[full];
+        if (best_cost < cur && (best_h <= curh || best_h <= kTreeletMaxHeight)) {
+            rebuilt = true;
+            if (lane == 0) {
+                // rebuild in pre-order (left part first) from the treelet's internal nodes
+                int sp = 0, next = 0;
+                L.stk[0][0] = full; L.stk[1][0] = -1; L.stk[2][0] = 0; sp = 1;
+                while (sp > 0) {
+                    --sp;
+                    const int sub = L.stk[0][sp], par = L.stk[1][sp], side = L.stk[2][sp];
+                    int me;
+                    if ((sub & (sub - 1)) == 0) {
+                        me = sel_idx(ref, __builtin_ctz(sub));
+                    } else {
+                        me = sel_idx(inner, next);
+                        ++next;
+                        a.cost[me] = L.cost[sub];
+                        a.height[me] = L.hh[sub];
+                        const int p = L.split[sub];
+                        L.stk[0][sp] = sub ^ p; L.stk[1][sp] = me; L.stk[2][sp] = 1; ++sp;
+                        L.stk[0][sp] = p; L.stk[1][sp] = me; L.stk[2][sp] = 0; ++sp;
+                    }
+                    if (par < 0) continue;
+                    nodes[par].child[side] = me;
+                    float u[6];
+                    tl_union(bx, k, sub, u);
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) nodes[par].box[6 * side + e] = u[e];
+                    a.parent[me >= 0 ? me : (a.n - 1) + ~me] = par;
+                }
+            }
+        }
+    }
+    if (!rebuilt && lane == 0) {
+        a.cost[n] = cur;
+        a.height[n] = curh;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tl_process(TreeletArgs a, uint32_t d) {
+    __shared__ TreeletLds s_tl[4];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t start = a.bins[2 * kDepthBins + d], count = a.bins[d];
+    for (uint32_t i = blockIdx.x * 4u + w; i < count; i += gridDim.x * 4u) {
+        const int n = __builtin_amdgcn_readfirstlane((int)a.order[start + i]);
+        tl_process_wave(a, n, lane, s_tl[w]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 __global__ __launch_bounds__(256) void k_collapse4(const BvhNode* __restrict__ nodes, int ni,
@@ -415,10 +739,25 @@ void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
         cur ^= 1;
     }
     // 4 passes: the sorted keys/values end in buffer 0
-    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, a.keys[cur], (int)n, a.nodes, a.parent, a.refit_cnt);
+    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, a.keys[cur], (int)n, a.nodes, a.parent, a.refit_cnt,
+                       a.tl_cost, a.tl_height);
     hipLaunchKernelGGL(k_leaves, dim3(g256), dim3(256), 0, s, a.vals[cur], a.tri_v, a.tri_n, a.tri_mat, (int)n,
-                       a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.error);
+                       a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.tl_cost, a.tl_height, a.error);
     const int ni = n > 1 ? (int)n - 1 : 1;
+    if (a.treelet_rounds > 0 && n >= 3) {
+        TreeletArgs t{a.nodes, a.parent, ni, (int)n, a.tl_cost, a.tl_height, a.tl_order, a.tl_bins, a.error};
+        const uint32_t gi = (uint32_t)(ni + 255) / 256u;
+        for (uint32_t r = 0; r < a.treelet_rounds; ++r) {
+            launch_fill_u32(a.tl_bins, 0u, 3 * kDepthBins, s);
+            hipLaunchKernelGGL(k_tl_hist, dim3(gi), dim3(256), 0, s, t);
+            hipLaunchKernelGGL(k_tl_scan, dim3(1), dim3(64), 0, s, a.tl_bins);
+            hipLaunchKernelGGL(k_tl_scatter, dim3(gi), dim3(256), 0, s, t);
+            // lowest first; heights the tree does not reach find no node (fewer nodes
+            // stand at each height above the lowest ones: smaller grids there)
+            for (int h = kTreeletMinHeight; h < kDepthBins; ++h)
+                hipLaunchKernelGGL(k_tl_process, dim3(h < 16 ? 1024 : 64), dim3(256), 0, s, t, (uint32_t)h);
+        }
+    }
     hipLaunchKernelGGL(k_collapse4, dim3((ni + 255) / 256), dim3(256), 0, s, a.nodes, ni, a.nodes4);
 }
 

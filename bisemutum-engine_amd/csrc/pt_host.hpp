@@ -154,6 +154,10 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<int32_t> parent;
     bipt::DevBuf<bipt::LeafGeo> leaf_geo;
     bipt::DevBuf<bipt::LeafShade> leaf_shade;
+    bipt::DevBuf<float> tl_cost;
+    bipt::DevBuf<int32_t> tl_height;
+    bipt::DevBuf<uint32_t> tl_order, tl_bins;
+    uint32_t treelet_rounds = 1;    // bi_pt_set_build_options
     bool bvh_built = false;
     // accumulation
     bipt::DevBuf<float> sums;

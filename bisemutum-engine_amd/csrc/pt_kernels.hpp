@@ -43,6 +43,12 @@ struct BuildArgs {
     LeafGeo* leaf_geo;
     LeafShade* leaf_shade;
     uint32_t* error;
+    // treelet restructuring after the refit (0 rounds: the Morton/Karras tree as built)
+    uint32_t treelet_rounds;
+    float* tl_cost;
+    int32_t* tl_height;
+    uint32_t* tl_order;
+    uint32_t* tl_bins;   // 192 entries
 };
 void launch_build_lbvh(const BuildArgs& a, hipStream_t s);
 uint32_t sort_hist_entries(uint32_t n);

@@ -192,6 +192,16 @@ bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba32f, uint32_t width,
 bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_pt_settings* settings,
                           float* out_rgba32f, float* out_depth, bi_pt_stream stream);
 bi_pt_status bi_pt_reset_accumulation(bi_pt_ctx* ctx);
+/* How the per-frame BVH is built (no reference counterpart: the reference's BVH is the
+ * driver's).  After the LBVH (Morton codes, radix sort, Karras hierarchy, refit), rounds
+ * of treelet restructuring lower the tree's surface-area cost (7-leaf treelets, optimal
+ * topology by dynamic programming; Karras & Aila 2013).  Closest hits do not depend on
+ * the tree, so images are the same for any setting; only the speed changes. */
+typedef struct bi_pt_build_options {
+    uint32_t treelet_rounds;   /* 0 = the Morton/Karras tree as built; default 1; at most 16 */
+    uint32_t _reserved[3];
+} bi_pt_build_options;
+bi_pt_status bi_pt_set_build_options(bi_pt_ctx* ctx, const bi_pt_build_options* options);
 bi_pt_status bi_pt_get_stats(const bi_pt_ctx* ctx, bi_pt_stats* out);
 
 /* ---- batched rendering (no reference counterpart: the reference renders one view per

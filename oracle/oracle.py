@@ -60,6 +60,7 @@ def lib():
         L.ora_create.restype = C.c_void_p
         L.ora_create.argtypes = [C.POINTER(SceneDesc)]
         L.ora_destroy.argtypes = [C.c_void_p]
+        L.ora_set_treelet_rounds.argtypes = [C.c_void_p, C.c_uint32]
         L.ora_num_tris.restype = C.c_uint32
         L.ora_num_tris.argtypes = [C.c_void_p]
         L.ora_get_world.argtypes = [C.c_void_p, _f32p, _f32p, _u32p]
@@ -160,6 +161,10 @@ class OracleScene:
             d.sky_rot[i] = float(scene.sky_rot[i])
         self.h = lib().ora_create(C.byref(d))
         self.n = lib().ora_num_tris(self.h)
+
+    def set_treelet_rounds(self, rounds):
+        """Rebuild the BVH with `rounds` rounds of treelet restructuring (0: the LBVH as built)."""
+        lib().ora_set_treelet_rounds(self.h, rounds)
 
     def close(self):
         if self.h:

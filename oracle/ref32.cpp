@@ -213,6 +213,169 @@ inline int delta(const uint32_t* codes, int n, int i, int j) {
 
 }  // namespace
 
+// Treelet restructuring (Karras & Aila 2013) of the refit tree, children before
+// parents: node n's treelet starts as its two children; the treelet leaf that is an
+// internal node with the largest box surface (ties: the lower position) is replaced in
+// place by its first child and its second child appended, until 7 leaves or only
+// triangles remain.  cost(s) = area(s) + the cheapest split of s (parts holding the
+// lowest leaf first, p = (p - 1) & s, first minimum kept); a single leaf costs its
+// subtree's cost (a triangle: its box's surface).  A cheaper topology that does not
+// raise the subtree's height, or keeps it <= 48, replaces the treelet, its internal
+// nodes assigned in pre-order (left part first).  Only nodes whose subtree is at least
+// 5 levels tall at the start of the round are restructured.  The GPU processes them
+// by that height, one launch per height, which gives the same result.
+namespace {
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletMaxHeight = 48;
+constexpr int kTreeletMinHeight = 5;   // lower subtrees are left as built
+inline float half_area6(const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return (dx * dy + dy * dz) + dz * dx;
+}
+void treelet_union(const float (*bx)[6], int k, int s, float* u) {
+    bool first = true;
+    for (int i = 0; i < k; ++i) {
+        if (!((s >> i) & 1)) continue;
+        for (int e = 0; e < 3; ++e) {
+            u[e] = first ? bx[i][e] : fmin_(u[e], bx[i][e]);
+            u[3 + e] = first ? bx[i][3 + e] : fmax_(u[3 + e], bx[i][3 + e]);
+        }
+        first = false;
+    }
+}
+void treelet_process(Scene& s, std::vector<float>& tcost, std::vector<int>& theight, int n, bool optimize) {
+    const int N = (int)s.num_tris;
+    std::vector<Node>& nodes = s.nodes;
+    int ref[kTreeletLeaves];
+    float bx[kTreeletLeaves][6];
+    int inner[kTreeletLeaves - 1];
+    int k = 2, ninner = 1;
+    inner[0] = n;
+    for (int c = 0; c < 2; ++c) {
+        ref[c] = nodes[n].child[c];
+        std::memcpy(bx[c], nodes[n].box[c], sizeof bx[c]);
+    }
+    while (k < kTreeletLeaves) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < k; ++i) {
+            if (ref[i] < 0) continue;
+            const float ar = half_area6(bx[i]);
+            if (ar > ba) { ba = ar; best = i; }
+        }
+        if (best < 0) break;
+        const int m = ref[best];
+        inner[ninner++] = m;
+        ref[best] = nodes[m].child[0];
+        ref[k] = nodes[m].child[1];
+        std::memcpy(bx[best], nodes[m].box[0], sizeof bx[best]);
+        std::memcpy(bx[k], nodes[m].box[1], sizeof bx[k]);
+        ++k;
+    }
+    auto leaf_cost = [&](int i) { return ref[i] >= 0 ? tcost[ref[i]] : half_area6(bx[i]); };
+    auto leaf_height = [&](int i) { return ref[i] >= 0 ? theight[ref[i]] : 0; };
+    float all[6];
+    for (int e = 0; e < 3; ++e) {
+        all[e] = fmin_(nodes[n].box[0][e], nodes[n].box[1][e]);
+        all[3 + e] = fmax_(nodes[n].box[0][3 + e], nodes[n].box[1][3 + e]);
+    }
+    auto child_cost = [&](int c) {
+        const int r = nodes[n].child[c];
+        return r >= 0 ? tcost[r] : half_area6(nodes[n].box[c]);
+    };
+    auto child_height = [&](int c) {
+        const int r = nodes[n].child[c];
+        return r >= 0 ? theight[r] : 0;
+    };
+    const float cur = (half_area6(all) + child_cost(0)) + child_cost(1);
+    const int curh = 1 + std::max(child_height(0), child_height(1));
+    if (optimize && k >= 3) {
+        const int full = (1 << k) - 1;
+        float cost[1 << kTreeletLeaves];
+        int split[1 << kTreeletLeaves], hh[1 << kTreeletLeaves];
+        for (int sub = 1; sub <= full; ++sub) {
+            if ((sub & (sub - 1)) == 0) {
+                const int i = __builtin_ctz(sub);
+                cost[sub] = leaf_cost(i);
+                hh[sub] = leaf_height(i);
+                split[sub] = 0;
+                continue;
+            }
+            float u[6];
+            treelet_union(bx, k, sub, u);
+            const int low = sub & -sub;
+            float best = 3.0e38f;
+            int bp = 0;
+            for (int p = (sub - 1) & sub; p > 0; p = (p - 1) & sub) {
+                if (!(p & low)) continue;
+                const float c = cost[p] + cost[sub ^ p];
+                if (c < best) { best = c; bp = p; }
+            }
+            cost[sub] = half_area6(u) + best;
+            split[sub] = bp;
+            hh[sub] = 1 + std::max(hh[bp], hh[sub ^ bp]);
+        }
+        if (cost[full] < cur && (hh[full] <= curh || hh[full] <= kTreeletMaxHeight)) {
+            int next = 0;
+            // pre-order: the subset's node, then its left part, then its right part
+            struct Rebuild {
+                Scene& s; std::vector<float>& tc; std::vector<int>& th; const int* ref; const float (*bx)[6];
+                int k, N; const int* inner; int& next; const float* cost; const int* split; const int* hh;
+                int operator()(int sub) {
+                    if ((sub & (sub - 1)) == 0) return ref[__builtin_ctz(sub)];
+                    const int me = inner[next++];
+                    tc[me] = cost[sub];
+                    th[me] = hh[sub];
+                    const int parts[2] = {split[sub], sub ^ split[sub]};
+                    for (int side = 0; side < 2; ++side) {
+                        const int c = (*this)(parts[side]);
+                        s.nodes[me].child[side] = c;
+                        treelet_union(bx, k, parts[side], s.nodes[me].box[side]);
+                        s.parent[c >= 0 ? c : (N - 1) + ~c] = me;
+                    }
+                    return me;
+                }
+            } rebuild{s, tcost, theight, ref, bx, k, N, inner, next, cost, split, hh};
+            rebuild(full);
+            return;
+        }
+    }
+    tcost[n] = cur;
+    theight[n] = curh;
+}
+// Cost and height of every subtree as the refit publishes them (children first).
+void subtree_costs(const Scene& s, std::vector<float>& tcost, std::vector<int>& theight, int n) {
+    float c[2];
+    int h[2];
+    for (int k = 0; k < 2; ++k) {
+        const int r = s.nodes[n].child[k];
+        if (r >= 0) {
+            subtree_costs(s, tcost, theight, r);
+            c[k] = tcost[r];
+            h[k] = theight[r];
+        } else {
+            c[k] = half_area6(s.nodes[n].box[k]);
+            h[k] = 0;
+        }
+    }
+    float all[6];
+    for (int e = 0; e < 3; ++e) {
+        all[e] = fmin_(s.nodes[n].box[0][e], s.nodes[n].box[1][e]);
+        all[3 + e] = fmax_(s.nodes[n].box[0][3 + e], s.nodes[n].box[1][3 + e]);
+    }
+    tcost[n] = (half_area6(all) + c[0]) + c[1];
+    theight[n] = 1 + std::max(h[0], h[1]);
+}
+// Children first; nodes whose height at the start of the round (h0) is below the
+// threshold are left alone, their stored cost and height included.
+void treelet_pass(Scene& s, std::vector<float>& tcost, std::vector<int>& theight, const std::vector<int>& h0,
+                  int n) {
+    for (int c = 0; c < 2; ++c)
+        if (s.nodes[n].child[c] >= 0) treelet_pass(s, tcost, theight, h0, s.nodes[n].child[c]);
+    if (h0[n] >= kTreeletMinHeight) treelet_process(s, tcost, theight, n, true);
+}
+}  // namespace
+
 void build_bvh(Scene& s) {
     const uint32_t n = s.num_tris;
     s.morton.assign(n, 0);
@@ -322,6 +485,15 @@ void build_bvh(Scene& s) {
             }
         }
         std::memcpy(&nb[6 * nd], box, sizeof box);
+    }
+    if (n >= 3) {
+        std::vector<float> cost(ni);
+        std::vector<int> height(ni);
+        subtree_costs(s, cost, height, 0);
+        for (uint32_t r = 0; r < s.treelet_rounds; ++r) {
+            const std::vector<int> h0 = height;   // heights at the start of the round select the nodes
+            treelet_pass(s, cost, height, h0, 0);
+        }
     }
     // BVH4 collapse: every internal node k independently -- from its two children, the
     // internal child with the largest box surface (ties: the lower slot) is replaced in

@@ -106,6 +106,12 @@ ora_ctx* ora_create(const ora_scene_desc* d) {
     return reinterpret_cast<ora_ctx*>(s);
 }
 
+void ora_set_treelet_rounds(ora_ctx* ctx, uint32_t rounds) {
+    Scene* s = reinterpret_cast<Scene*>(ctx);
+    s->treelet_rounds = rounds;
+    build_bvh(*s);
+}
+
 void ora_destroy(ora_ctx* ctx) {
     Scene* s = reinterpret_cast<Scene*>(ctx);
     free_bvh64(*s);

@@ -48,9 +48,31 @@ def compare_bvh4(gpu, ora):
 
 
 def test_bvh_bit_exact_c0(c0_gpu, c0_oracle):
-    print("case: c0 bvh", flush=True)
+    print("case: c0 bvh (LBVH + 1 round of treelet restructuring)", flush=True)
     compare_bvh(c0_gpu.bvh(), c0_oracle.bvh())
     compare_bvh4(c0_gpu, c0_oracle)
+
+
+@pytest.mark.parametrize("rounds", [0, 2, 5])
+def test_bvh_bit_exact_c0_rounds(oracle_lib, c0_scene, rounds):
+    print(f"case: c0 bvh, {rounds} treelet rounds", flush=True)
+    g = bi_pt.PathTracer()
+    g.set_scene(c0_scene)
+    g.set_build_options(treelet_rounds=rounds)
+    o = oracle_lib.OracleScene(c0_scene)
+    o.set_treelet_rounds(rounds)
+    compare_bvh(g.bvh(), o.bvh())
+    compare_bvh4(g, o)
+    # images do not depend on the tree: compare a small render with the default build
+    cam = scenes.camera_c0(48, 32)
+    g.render(cam, spp=2, max_bounces=5, seed=3)
+    a = g.image().copy()
+    d = bi_pt.PathTracer()
+    d.set_scene(c0_scene)
+    d.render(cam, spp=2, max_bounces=5, seed=3)
+    assert np.array_equal(a.view(np.uint32), d.image().view(np.uint32))
+    g.close()
+    d.close()
 
 
 def random_rays(rng, n, centre, radius, tmax=1000.0):

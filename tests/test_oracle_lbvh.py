@@ -119,6 +119,135 @@ def check_tree(bvh, tri_v):
             assert np.array_equal(box[:3], lo) and np.array_equal(box[3:], hi)
 
 
+def check_tree_any(bvh, tri_v):
+    """Invariants of any binary BVH over the sorted leaves: every leaf once, parents
+    consistent with children, every child box exactly the bounds of the triangles below
+    it, at most 63 levels.  Returns (internal-node half-area sum, depth)."""
+    children, par = bvh["children"], bvh["parents"]
+    n = len(bvh["order"])
+    if n == 1:
+        return 0.0, 0
+    v = tri_v.reshape(-1, 3, 3)
+    leaf_lo = v.min(axis=1)[bvh["order"]]
+    leaf_hi = v.max(axis=1)[bvh["order"]]
+    seen = np.zeros(n, np.int32)
+    post, stack, depth = [], [(0, 0)], 0
+    while stack:
+        nd, d = stack.pop()
+        post.append(nd)
+        depth = max(depth, d + 1)
+        for c in children[nd]:
+            if c < 0:
+                seen[~c] += 1
+            else:
+                stack.append((c, d + 1))
+    assert np.all(seen == 1) and len(set(post)) == n - 1 == len(post)
+    assert par[0] == -1
+    for nd in range(n - 1):
+        for c in children[nd]:
+            assert par[(n - 1) + ~c if c < 0 else c] == nd
+    box = {}
+    area = 0.0
+    for nd in reversed(post):
+        los, his = [], []
+        for k, c in enumerate(children[nd]):
+            lo, hi = (leaf_lo[~c], leaf_hi[~c]) if c < 0 else box[c]
+            b = bvh["bounds"][nd, 6 * k:6 * k + 6]
+            assert np.array_equal(b[:3], lo) and np.array_equal(b[3:], hi)
+            los.append(lo)
+            his.append(hi)
+        box[nd] = (np.minimum(*los), np.maximum(*his))
+        e = (box[nd][1] - box[nd][0]).astype(np.float64)
+        area += e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+    assert depth <= 63
+    return area, depth
+
+
+def test_treelet_restructuring_c0(oracle_lib, c0_scene, c0_oracle):
+    """The default build restructures the LBVH's treelets (one round): a valid tree over
+    the same leaves with a lower surface-area cost, more rounds lower it further, and
+    the closest hits stay the same bit for bit."""
+    raw = oracle_lib.OracleScene(c0_scene)
+    raw.set_treelet_rounds(0)
+    tri_v = raw.world()[0]
+    a0, d0 = check_tree_any(raw.bvh(), tri_v)
+    a1, d1 = check_tree_any(c0_oracle.bvh(), tri_v)   # the default: one round
+    three = oracle_lib.OracleScene(c0_scene)
+    three.set_treelet_rounds(3)
+    a3, d3 = check_tree_any(three.bvh(), tri_v)
+    print(f"internal half-area sum: LBVH {a0:.1f}, 1 round {a1:.1f}, 3 rounds {a3:.1f}; depth {d0} -> {d3}")
+    assert a3 <= a1 < 0.95 * a0
+    assert np.array_equal(raw.bvh()["order"], c0_oracle.bvh()["order"])
+    check_bvh4(three.bvh(), *three.bvh4())
+    rays = random_rays(np.random.default_rng(5), 2000, np.array([0.0, 2.0, 0.0]), 5.0)
+    t0, l0, u0, v0 = raw.trace(rays)
+    t3, l3, u3, v3 = three.trace(rays)
+    assert np.array_equal(l0, l3) and np.array_equal(t0.view(np.uint32), t3.view(np.uint32))
+    assert np.array_equal(u0.view(np.uint32), u3.view(np.uint32))
+    assert np.array_equal(raw.trace(rays, any_hit=True)[0], three.trace(rays, any_hit=True)[0])
+    assert np.array_equal(c0_oracle.trace(rays)[1], l0)
+
+
+def all_tree_costs(boxes):
+    """Half-area sums of the internal nodes of every binary tree over the given leaf
+    boxes (enumerated explicitly: (2k - 3)!! trees for k leaves)."""
+    def union(idx):
+        b = boxes[list(idx)]
+        return np.concatenate([b[:, :3].min(axis=0), b[:, 3:].max(axis=0)])
+
+    def area(b):
+        e = (b[3:] - b[:3]).astype(np.float64)
+        return e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+
+    def trees(items):
+        if len(items) == 1:
+            yield 0.0
+            return
+        first, rest = items[0], items[1:]
+        own = area(union(items))
+        for mask in range(2 ** len(rest)):
+            left = (first,) + tuple(r for i, r in enumerate(rest) if (mask >> i) & 1)
+            right = tuple(r for i, r in enumerate(rest) if not (mask >> i) & 1)
+            if not right:
+                continue
+            for cl in trees(left):
+                for cr in trees(right):
+                    yield own + cl + cr
+
+    return list(trees(tuple(range(len(boxes)))))
+
+
+def test_treelet_root_is_optimal_for_seven_leaves(oracle_lib):
+    """With 7 triangles the root's treelet holds every leaf, so the restructured tree
+    must reach the minimum over all 10,395 binary trees (explicit enumeration).  The
+    triangles sit at halving distances along x, so the Morton order builds a chain
+    tall enough (height 6) for the root to be restructured."""
+    rng = np.random.default_rng(11)
+    for trial in range(4):
+        # boxes centred on y = z = 0, so only x reaches the Morton codes
+        cx = 3.0 * 0.5 ** np.arange(7) + rng.uniform(0.0, 0.001, 7)
+        sz = 0.3 * 0.5 ** np.arange(7) * rng.uniform(0.5, 1.0, 7)
+        sy = rng.uniform(0.05, 0.5, 7)
+        tri = np.stack([np.stack([cx - sz, -sy, -sz], 1), np.stack([cx + sz, sy, -sz], 1),
+                        np.stack([cx, 0.2 * sy, sz], 1)], 1).astype(np.float32).reshape(-1)
+        sc = oracle_lib.OracleScene(tiny_scene(tri))
+        sc.set_treelet_rounds(0)
+        _, lbvh_height = check_tree_any(sc.bvh(), sc.world()[0])
+        assert lbvh_height >= 5
+        sc.set_treelet_rounds(1)
+        v = sc.world()[0].reshape(-1, 3, 3)[sc.bvh()["order"]]
+        boxes = np.concatenate([v.min(axis=1), v.max(axis=1)], axis=1)
+        costs = all_tree_costs(boxes)
+        assert len(costs) == 10395
+        got, _ = check_tree_any(sc.bvh(), sc.world()[0])
+        sc.set_treelet_rounds(0)
+        lbvh, _ = check_tree_any(sc.bvh(), sc.world()[0])
+        print(f"trial {trial}: LBVH {lbvh:.4f} (height {lbvh_height}), restructured {got:.4f}, "
+              f"optimum {min(costs):.4f}")
+        assert abs(got - min(costs)) <= 1e-5 * min(costs)
+        assert got <= lbvh + 1e-9
+
+
 def half_area32(b):
     """Surface measure of boxes (lo.xyz, hi.xyz) in float32, as the collapse orders them."""
     b = np.asarray(b, np.float32)
@@ -181,7 +310,9 @@ def check_bvh4(bvh, child4, box4):
                     assert half_area32(box_of(int(ch[j]))) <= exp_min
 
 
-def test_c0_morton_sort_tree(c0_oracle):
+def test_c0_morton_sort_tree(oracle_lib, c0_scene):
+    c0_oracle = oracle_lib.OracleScene(c0_scene)
+    c0_oracle.set_treelet_rounds(0)   # the Morton/Karras tree as built
     bvh = c0_oracle.bvh()
     tri_v, _, _ = c0_oracle.world()
     n = c0_oracle.n
@@ -207,11 +338,18 @@ def test_duplicate_codes_and_tiny_trees(oracle_lib):
     # 37 identical triangles plus a few distinct ones: equal Morton codes everywhere
     tris = [base] * 37 + [rng.uniform(-1, 1, size=(9,)) for _ in range(5)]
     sc = oracle_lib.OracleScene(tiny_scene(np.concatenate(tris)))
+    sc.set_treelet_rounds(0)
     bvh = sc.bvh()
     check_tree(bvh, sc.world()[0])
     check_bvh4(bvh, *sc.bvh4())
+    sc.set_treelet_rounds(2)
+    check_tree_any(sc.bvh(), sc.world()[0])
+    check_bvh4(sc.bvh(), *sc.bvh4())
     for count in (1, 2, 3, 5, 9):
         sc = oracle_lib.OracleScene(tiny_scene(rng.uniform(-1, 1, size=(9 * count,))))
+        check_tree_any(sc.bvh(), sc.world()[0])
+        check_bvh4(sc.bvh(), *sc.bvh4())
+        sc.set_treelet_rounds(0)
         bvh = sc.bvh()
         if count <= 3:
             check_tree(bvh, sc.world()[0])
