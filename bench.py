@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--refill-lanes", type=int, default=0)
     ap.add_argument("--work-parts", type=int, default=0)
     ap.add_argument("--block-threads", type=int, default=0, help="traversal workgroup size: 128, 256 or 512")
+    ap.add_argument("--treelet-rounds", type=int, default=-1,
+                    help="rounds of treelet restructuring after the LBVH (-1: the library's default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache leg")
@@ -111,6 +113,8 @@ def main():
         del src, dst
     pt = bi_pt.PathTracer(0)
     pt.set_scene(scene)
+    if args.treelet_rounds >= 0:
+        pt.set_build_options(treelet_rounds=args.treelet_rounds)
     stream = torch.cuda.current_stream().cuda_stream
     spp, mb, seed = cfg["spp"], cfg["max_bounces"], cfg["seed"]
 
@@ -182,6 +186,7 @@ def main():
             "width": cam.width, "height": cam.height, "spp": spp, "max_bounces": mb,
             "wave_paths": args.wave_paths, "stack_lds": args.stack_lds,
             "refill_lanes": args.refill_lanes, "work_parts": args.work_parts, "block_threads": args.block_threads,
+            "treelet_rounds": args.treelet_rounds,
         },
         "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                            "p99": round(float(np.percentile(lat, 99)), 3)},

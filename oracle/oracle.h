@@ -84,7 +84,7 @@ typedef struct ora_settings {
 
 ora_ctx* ora_create(const ora_scene_desc* desc);
 /* Rebuilds the BVH with `rounds` rounds of treelet restructuring after the LBVH (the
- * default is 1, as bi_pt_set_build_options; 0 keeps the Morton/Karras tree). */
+ * default is 2, as bi_pt_set_build_options; 0 keeps the Morton/Karras tree). */
 void ora_set_treelet_rounds(ora_ctx* ctx, uint32_t rounds);
 void ora_destroy(ora_ctx* ctx);
 uint32_t ora_num_tris(const ora_ctx* ctx);
