@@ -76,8 +76,8 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
     BI_TRY(dalloc(ctx, ctx->leaf_shade, n));
     BI_TRY(dalloc(ctx, ctx->tl_cost, ni));
     BI_TRY(dalloc(ctx, ctx->tl_height, ni));
-    BI_TRY(dalloc(ctx, ctx->tl_order, ni));
-    BI_TRY(dalloc(ctx, ctx->tl_bins, 192));
+    BI_TRY(dalloc(ctx, ctx->tl_snap, ni));
+    BI_TRY(dalloc(ctx, ctx->tl_starts, 1 + treelet_start_cap(n)));
     return BI_PT_OK;
 }
 
@@ -112,8 +112,8 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error) {
     a.treelet_rounds = ctx->treelet_rounds;
     a.tl_cost = ctx->tl_cost.p;
     a.tl_height = ctx->tl_height.p;
-    a.tl_order = ctx->tl_order.p;
-    a.tl_bins = ctx->tl_bins.p;
+    a.tl_snap = ctx->tl_snap.p;
+    a.tl_starts = ctx->tl_starts.p;
     launch_build_lbvh(a, s);
     BI_HIP_TRY(ctx, hipGetLastError());
     return BI_PT_OK;
@@ -171,7 +171,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     dfree(ctx, ctx->cb_ord); dfree(ctx, ctx->keys0); dfree(ctx, ctx->keys1); dfree(ctx, ctx->vals0);
     dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes);
     dfree(ctx, ctx->nodes4); dfree(ctx, ctx->parent); dfree(ctx, ctx->leaf_geo); dfree(ctx, ctx->leaf_shade);
-    dfree(ctx, ctx->tl_cost); dfree(ctx, ctx->tl_height); dfree(ctx, ctx->tl_order); dfree(ctx, ctx->tl_bins);
+    dfree(ctx, ctx->tl_cost); dfree(ctx, ctx->tl_height); dfree(ctx, ctx->tl_snap); dfree(ctx, ctx->tl_starts);
     dfree(ctx, ctx->sums);
     engine_free(ctx, ctx->eng);
     delete ctx;

@@ -80,7 +80,8 @@ bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out) {
         return fail(own, BI_PT_ERROR_DEVICE_CHECK,
                     std::string("device check failed: ") + ((hc[kCntError] & 1u) ? "traversal stack overflow " : "") +
                         ((hc[kCntError] & 4u) ? "refit wait limit " : "") +
-                        ((hc[kCntError] & 8u) ? "traversal step bound exceeded" : ""));
+                        ((hc[kCntError] & 8u) ? "traversal step bound exceeded " : "") +
+                        ((hc[kCntError] & 16u) ? "treelet start list overflow" : ""));
     return BI_PT_OK;
 }
 

@@ -412,15 +412,18 @@ __device__ __forceinline__ float half_area(const float* b) {
 // tree stays within 63 levels and the traversal stack bound holds), the treelet is
 // rebuilt from the same internal nodes, assigned in pre-order (left part first).
 // Only nodes whose subtree is at least kTreeletMinHeight levels tall at the start of a
-// round are restructured.  They are processed by that height, lowest first, one launch
-// per height: nodes of one height have disjoint subtrees and every node finds its
-// subtree already processed, so the result equals a children-first pass in any order --
-// the CPU restatement's recursion.  The refit publishes every subtree's cost and height
-// for the first round; a round leaves them current for the next.
+// round are restructured, children first: restructuring a node changes nothing outside
+// its subtree and subtrees of nodes where neither contains the other are disjoint, so
+// every children-first order gives the same tree -- the CPU restatement's recursion.
+// One launch per round schedules them bottom-up like the refit: a wavefront starts at
+// each node of exactly that height (its subtree holds no other such node), restructures
+// it, then counts itself in at the parent; the arrival that completes the parent's count
+// (one per child that is restructured this round) carries on there, the other ends.  No
+// wavefront waits for another.  The refit publishes every subtree's cost and height for
+// the first round; a round leaves them current for the next.
 constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletMaxHeight = 48;
 constexpr int kTreeletMinHeight = 5;   // lower subtrees are left as built (their share of the cost is small)
-constexpr int kDepthBins = 64;
 
 __device__ __forceinline__ float half_area6(const float* b) {
     const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
@@ -433,52 +436,37 @@ struct TreeletArgs {
     int ni, n;
     float* cost;        // per internal node: surface-area cost of its subtree
     int32_t* height;    // per internal node: levels below it
-    uint32_t* order;    // internal nodes of height >= kTreeletMinHeight grouped by height
-    uint32_t* bins;     // [count[64], cursor[64], start[64]]
+    int32_t* snap;      // per internal node: its height at the start of the round
+    uint32_t* arrivals; // per internal node: 2 - children restructured this round, + arrivals
+    uint32_t* starts;   // [0]: count, then the nodes a climb starts from (at most n / 6)
+    uint32_t start_cap;
     uint32_t* error;
 };
 
-// Nodes by height, counted and placed with one global atomic per workgroup and height.
-__global__ __launch_bounds__(256) void k_tl_hist(TreeletArgs a) {
-    __shared__ uint32_t h[kDepthBins];
-    if (threadIdx.x < kDepthBins) h[threadIdx.x] = 0;
-    __syncthreads();
+__global__ __launch_bounds__(256) void k_tl_prep(TreeletArgs a) {
     const int k = (int)(blockIdx.x * 256u + threadIdx.x);
-    if (k < a.ni) {
-        const int ht = a.height[k];
-        if (ht >= kTreeletMinHeight && ht < kDepthBins) atomicAdd(&h[ht], 1u);
+    if (k >= a.ni) return;
+    a.snap[k] = a.height[k];
+    uint32_t t = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int ch = a.nodes[k].child[c];
+        t += (ch >= 0 && a.height[ch] >= kTreeletMinHeight) ? 1u : 0u;
     }
-    __syncthreads();
-    if (threadIdx.x < kDepthBins && h[threadIdx.x]) atomicAdd(&a.bins[threadIdx.x], h[threadIdx.x]);
-}
-
-__global__ void k_tl_scan(uint32_t* bins) {
-    if (threadIdx.x != 0) return;
-    uint32_t run = 0;
-    for (int d = 0; d < kDepthBins; ++d) {
-        bins[kDepthBins + d] = run;
-        bins[2 * kDepthBins + d] = run;
-        run += bins[d];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_tl_scatter(TreeletArgs a) {
-    __shared__ uint32_t cnt[kDepthBins], base[kDepthBins];
-    if (threadIdx.x < kDepthBins) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
-    int ht = -1;
-    uint32_t rank = 0;
-    if (k < a.ni) {
-        ht = a.height[k];
-        if (ht >= kTreeletMinHeight && ht < kDepthBins) rank = atomicAdd(&cnt[ht], 1u);
-        else ht = -1;
-    }
-    __syncthreads();
-    if (threadIdx.x < kDepthBins && cnt[threadIdx.x])
-        base[threadIdx.x] = atomicAdd(&a.bins[kDepthBins + threadIdx.x], cnt[threadIdx.x]);
-    __syncthreads();
-    if (ht >= 0) a.order[base[ht] + rank] = (uint32_t)k;
+    a.arrivals[k] = 2u - t;
+    // a climb starts at every node of height exactly kTreeletMinHeight (their subtrees
+    // are disjoint and hold at least 6 triangles each); one atomic per wavefront
+    const bool start = a.snap[k] == kTreeletMinHeight;
+    const uint64_t m = __ballot(start);
+    if (!start) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t first = (uint32_t)__builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(a.starts, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)first);
+    const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (idx < a.start_cap) a.starts[1 + idx] = (uint32_t)k;
+    else atomicOr(a.error, 16u);
 }
 
 // One treelet per wavefront.  Every lane forms the same treelet (uniform values); the
@@ -487,11 +475,33 @@ __global__ __launch_bounds__(256) void k_tl_scatter(TreeletArgs a) {
 // table in LDS; lane 0 rebuilds an accepted treelet.  Register arrays are indexed with
 // compile-time indices only (selects), so nothing goes to scratch memory.
 struct TreeletLds {
+    float box[1 << kTreeletLeaves][6];   // union of every subset's boxes
+    float area[1 << kTreeletLeaves];     // its half surface
     float cost[1 << kTreeletLeaves];
     uint8_t split[1 << kTreeletLeaves];
     uint8_t hh[1 << kTreeletLeaves];
-    int stk[3][kTreeletLeaves];
+    int8_t slot[1 << kTreeletLeaves];   // rebuild: pre-order slot of an internal subset, else -1
+    int8_t up[1 << kTreeletLeaves];     // rebuild: slot of the parent of a subset in the tree, else -1
+    int8_t side[1 << kTreeletLeaves];   // rebuild: 0 left part, 1 right part
+    int ref[8];     // treelet leaves: internal node index, or ~triangle
+    int inner[8];   // the treelet's internal nodes, root first
 };
+
+// Subsets of 7 leaves by size (1..5): entry [s][l] is the l-th subset of s leaves in
+// increasing order, 0 past the last; lane l of the dynamic programme owns [s][l].
+struct TreeletBySize {
+    uint8_t v[6][64];
+};
+constexpr TreeletBySize make_treelet_by_size() {
+    TreeletBySize t{};
+    int cnt[6] = {};
+    for (int q = 1; q < (1 << kTreeletLeaves); ++q) {
+        const int pc = __builtin_popcount(q);
+        if (pc <= 5) t.v[pc][cnt[pc]++] = (uint8_t)q;
+    }
+    return t;
+}
+__constant__ TreeletBySize kTreeletBySize = make_treelet_by_size();
 
 __device__ __forceinline__ void tl_union(const float (&bx)[kTreeletLeaves][6], int k, int s, float* u) {
     bool first = true;
@@ -507,44 +517,142 @@ __device__ __forceinline__ void tl_union(const float (&bx)[kTreeletLeaves][6], i
     }
 }
 
-template <class T, int N>
-__device__ __forceinline__ T sel_idx(const T (&v)[N], int i) {
-    T r = v[0];
+// Cheapest split of a subset: the minimum of cost[p] + cost[sub ^ p] over the parts p
+// holding the lowest leaf, the largest p among equal minima (the one the serial
+// enumeration p = (p - 1) & sub meets first), 3e38 and p = 0 if no sum is below 3e38.
+// Candidate c of a subset deposits its bits, lowest first, at the subset's set bits
+// above the lowest; the candidates are evaluated unrolled so their LDS reads overlap.
+__device__ __forceinline__ void tl_take(float v, int p, float& best, int& bp) {
+    if (v < best || (v == best && bp != 0 && p > bp)) {
+        best = v;
+        bp = p;
+    }
+}
+__device__ __forceinline__ int tl_deposit(int c, int low, const int (&b)[kTreeletLeaves - 1]) {
+    int p = low;
 #pragma unroll
-    for (int j = 1; j < N; ++j) r = i == j ? v[j] : r;
-    return r;
+    for (int j = 0; j < kTreeletLeaves - 1; ++j) p |= ((c >> j) & 1) ? b[j] : 0;
+    return p;
+}
+__device__ __forceinline__ void tl_bits(int sub, int& low, int (&b)[kTreeletLeaves - 1]) {
+    low = sub & -sub;
+    int m = sub ^ low;
+#pragma unroll
+    for (int j = 0; j < kTreeletLeaves - 1; ++j) {
+        b[j] = m & -m;
+        m ^= b[j];
+    }
 }
 
-__device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, TreeletLds& L) {
-    BvhNode* nodes = a.nodes;
-    int ref[kTreeletLeaves];
-    float bx[kTreeletLeaves][6];
-    int inner[kTreeletLeaves - 1];
+// All candidates of one subset of S leaves in one lane.
+template <int S>
+__device__ __forceinline__ void tl_split_lane(const TreeletLds& L, int sub, float& best, int& bp) {
+    constexpr int kCand = (1 << (S - 1)) - 1;
+    int low, b[kTreeletLeaves - 1];
+    tl_bits(sub, low, b);
+    float v[kCand];
+    int pp[kCand];
 #pragma unroll
-    for (int i = 0; i < kTreeletLeaves; ++i) {
-        ref[i] = -1;
-        for (int e = 0; e < 6; ++e) bx[i][e] = 0.0f;
+    for (int c = 0; c < kCand; ++c) {
+        pp[c] = tl_deposit(c, low, b);
+        v[c] = L.cost[pp[c]] + L.cost[sub ^ pp[c]];
     }
+    best = 3.0e38f;
+    bp = 0;
 #pragma unroll
-    for (int i = 0; i < kTreeletLeaves - 1; ++i) inner[i] = n;
-    int k = 2, ninner = 1;
+    for (int c = 0; c < kCand; ++c) tl_take(v[c], pp[c], best, bp);
+}
+
+// The candidates of `sub` (0: none) spread over a group of `width` lanes (a power of two,
+// lane j of the group takes candidates j, j + width, ...; at most kPer each), the
+// result shuffled to every lane of the group.
+template <int kPer>
+__device__ __forceinline__ void tl_split_group(const TreeletLds& L, int sub, int j, int width, float& best, int& bp) {
+    best = 3.0e38f;
+    bp = 0;
+    if (sub != 0) {
+        const int ncand = (1 << (__popc(sub) - 1)) - 1;
+        int low, b[kTreeletLeaves - 1];
+        tl_bits(sub, low, b);
+        float v[kPer];
+        int pp[kPer];
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int c = j + width * t;
+            pp[t] = c < ncand ? tl_deposit(c, low, b) : 0;
+            v[t] = c < ncand ? L.cost[pp[t]] + L.cost[sub ^ pp[t]] : 3.0e38f;
+        }
+#pragma unroll
+        for (int t = 0; t < kPer; ++t)
+            if (pp[t] != 0) tl_take(v[t], pp[t], best, bp);
+    }
+    for (int o = 1; o < width; o <<= 1) {
+        const float ob = __shfl_xor(best, o);
+        const int op = __shfl_xor(bp, o);
+        if (op != 0 && (bp == 0 || ob < best || (ob == best && op > bp))) {
+            best = ob;
+            bp = op;
+        }
+    }
+}
+
+// One subset size of the dynamic programme: `sub` is the lane's subset of S leaves.
+template <int S>
+__device__ __forceinline__ void tl_dp_size(const TreeletArgs& a, TreeletLds& L, int sub, int full) {
+    if (sub != 0 && sub <= full) {
+        if constexpr (S == 1) {
+            const int r = L.ref[__builtin_ctz(sub)];
+            L.cost[sub] = r >= 0 ? a.cost[r] : L.area[sub];
+            L.hh[sub] = (uint8_t)(r >= 0 ? a.height[r] : 0);
+            L.split[sub] = 0;
+        } else {
+            float best;
+            int bp;
+            tl_split_lane<S>(L, sub, best, bp);
+            L.cost[sub] = L.area[sub] + best;
+            L.split[sub] = (uint8_t)bp;
+            L.hh[sub] = (uint8_t)(1 + max((int)L.hh[bp], (int)L.hh[sub ^ bp]));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// subs[s]: the lane's subset of s leaves (kTreeletBySize), s = 1..5.
+__device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, TreeletLds& L, const int (&subs)[6]) {
+    BvhNode* nodes = a.nodes;
+    // the treelet's leaf boxes stay in registers (compile-time indices only); leaf and
+    // internal node indices sit in LDS, identical in every lane; bit i of `internal`
+    // marks a leaf that is an internal node
+    float bx[kTreeletLeaves][6];
+#pragma unroll
+    for (int i = 0; i < kTreeletLeaves; ++i)
+#pragma unroll
+        for (int e = 0; e < 6; ++e) bx[i][e] = 0.0f;
+    uint32_t internal = 0;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        ref[c] = nodes[n].child[c];
+        const int ch = nodes[n].child[c];
+        L.ref[c] = ch;
+        internal |= ch >= 0 ? 1u << c : 0u;
 #pragma unroll
         for (int e = 0; e < 6; ++e) bx[c][e] = nodes[n].box[6 * c + e];
     }
+    L.inner[0] = n;
+    int k = 2;
     for (int step = 0; step < kTreeletLeaves - 2; ++step) {
         int best = -1;
         float ba = -1.0f;
 #pragma unroll
         for (int i = 0; i < kTreeletLeaves; ++i) {
-            if (i >= k || ref[i] < 0) continue;
+            if (!((internal >> i) & 1u)) continue;
             const float ar = half_area6(bx[i]);
             if (ar > ba) { ba = ar; best = i; }
         }
         if (best < 0) break;
-        const int m = sel_idx(ref, best);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int m = L.ref[best];
         const int c0 = nodes[m].child[0], c1 = nodes[m].child[1];
         float b0[6], b1[6];
 #pragma unroll
@@ -552,25 +660,27 @@ __device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, Tree
             b0[e] = nodes[m].box[e];
             b1[e] = nodes[m].box[6 + e];
         }
+        // every lane has read ref[best] before any lane overwrites it
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int i = 0; i < kTreeletLeaves; ++i) {
             if (i == best) {
-                ref[i] = c0;
 #pragma unroll
                 for (int e = 0; e < 6; ++e) bx[i][e] = b0[e];
             }
             if (i == k) {
-                ref[i] = c1;
 #pragma unroll
                 for (int e = 0; e < 6; ++e) bx[i][e] = b1[e];
             }
         }
-#pragma unroll
-        for (int i = 1; i < kTreeletLeaves - 1; ++i)
-            if (i == ninner) inner[i] = m;
-        ++ninner;
+        internal = (internal & ~(1u << best)) | (c0 >= 0 ? 1u << best : 0u) | (c1 >= 0 ? 1u << k : 0u);
+        L.ref[best] = c0;
+        L.ref[k] = c1;
+        L.inner[step + 1] = m;
         ++k;
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     float all[6];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -585,40 +695,51 @@ __device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, Tree
     bool rebuilt = false;
     if (k >= 3) {
         const int full = (1 << k) - 1;
-        // subsets by size; lane l owns l + 1 and l + 65
-        for (int size = 1; size <= k; ++size) {
+        // every subset's union and its half surface once, lane l owning subsets l + 1
+        // and l + 65
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int sub = (int)lane + 1 + 64 * h;
-                if (sub > full || __popc(sub) != size) continue;
-                if (size == 1) {
-                    const int i = __builtin_ctz(sub);
-                    const int r = sel_idx(ref, i);
-                    float b[6];
+        for (int h = 0; h < 2; ++h) {
+            const int sub = (int)lane + 1 + 64 * h;
+            float u[6];
+            tl_union(bx, k, sub, u);
+            if (sub <= full) {
 #pragma unroll
-                    for (int e = 0; e < 6; ++e) {
-                        b[e] = bx[0][e];
-#pragma unroll
-                        for (int j = 1; j < kTreeletLeaves; ++j) b[e] = i == j ? bx[j][e] : b[e];
-                    }
-                    L.cost[sub] = r >= 0 ? a.cost[r] : half_area6(b);
-                    L.hh[sub] = (uint8_t)(r >= 0 ? a.height[r] : 0);
-                    L.split[sub] = 0;
-                    continue;
-                }
-                float u[6];
-                tl_union(bx, k, sub, u);
-                const int low = sub & -sub;
-                float best = 3.0e38f;
-                int bp = 0;
-                for (int p = (sub - 1) & sub; p > 0; p = (p - 1) & sub) {
-                    if (!(p & low)) continue;
-                    const float c = L.cost[p] + L.cost[sub ^ p];
-                    if (c < best) { best = c; bp = p; }
-                }
-                L.cost[sub] = half_area6(u) + best;
+                for (int e = 0; e < 6; ++e) L.box[sub][e] = u[e];
+                L.area[sub] = half_area6(u);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // subsets of up to 5 leaves by size, one lane per subset
+        tl_dp_size<1>(a, L, subs[1], full);
+        tl_dp_size<2>(a, L, subs[2], full);
+        tl_dp_size<3>(a, L, subs[3], full);
+        if (k >= 4) tl_dp_size<4>(a, L, subs[4], full);
+        if (k >= 5) tl_dp_size<5>(a, L, subs[5], full);
+        // subsets of 6 and 7 leaves: 8 lanes for each subset of 6 out of 7 leaves, the
+        // whole wavefront for the full set
+        if (k == 7) {
+            const int g = (int)(lane >> 3);
+            const int sub = g < kTreeletLeaves ? full ^ (1 << g) : 0;
+            float best;
+            int bp;
+            tl_split_group<4>(L, sub, (int)(lane & 7u), 8, best, bp);
+            if (sub != 0 && (lane & 7u) == 0) {
+                L.cost[sub] = L.area[sub] + best;
                 L.split[sub] = (uint8_t)bp;
                 L.hh[sub] = (uint8_t)(1 + max((int)L.hh[bp], (int)L.hh[sub ^ bp]));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (k >= 6) {
+            float best;
+            int bp;
+            tl_split_group<1>(L, full, (int)lane, 64, best, bp);
+            if (lane == 0) {
+                L.cost[full] = L.area[full] + best;
+                L.split[full] = (uint8_t)bp;
+                L.hh[full] = (uint8_t)(1 + max((int)L.hh[bp], (int)L.hh[full ^ bp]));
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -627,33 +748,54 @@ __device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, Tree
         const int best_h = L.hh[full];
         if (best_cost < cur && (best_h <= curh || best_h <= kTreeletMaxHeight)) {
             rebuilt = true;
-            if (lane == 0) {
-                // rebuild in pre-order (left part first) from the treelet's internal nodes
-                int sp = 0, next = 0;
-                L.stk[0][0] = full; L.stk[1][0] = -1; L.stk[2][0] = 0; sp = 1;
-                while (sp > 0) {
-                    --sp;
-                    const int sub = L.stk[0][sp], par = L.stk[1][sp], side = L.stk[2][sp];
-                    int me;
-                    if ((sub & (sub - 1)) == 0) {
-                        me = sel_idx(ref, __builtin_ctz(sub));
-                    } else {
-                        me = sel_idx(inner, next);
-                        ++next;
-                        a.cost[me] = L.cost[sub];
-                        a.height[me] = L.hh[sub];
-                        const int p = L.split[sub];
-                        L.stk[0][sp] = sub ^ p; L.stk[1][sp] = me; L.stk[2][sp] = 1; ++sp;
-                        L.stk[0][sp] = p; L.stk[1][sp] = me; L.stk[2][sp] = 0; ++sp;
-                    }
-                    if (par < 0) continue;
-                    nodes[par].child[side] = me;
-                    float u[6];
-                    tl_union(bx, k, sub, u);
+            // rebuild in pre-order (left part first) from the treelet's internal nodes:
+            // the root takes slot 0; the parts of an internal subset at slot t take
+            // t + 1 (left) and t + popc(left) (right: the left part's subtree holds
+            // popc(left) - 1 internal nodes), leaves no slot.  Filled top-down, one tree
+            // level per pass (at most 6 levels below the root); then every subset in
+            // the tree writes its link, box and parent in parallel.
 #pragma unroll
-                    for (int e = 0; e < 6; ++e) nodes[par].box[6 * side + e] = u[e];
-                    a.parent[me >= 0 ? me : (a.n - 1) + ~me] = par;
+            for (int h = 0; h < 2; ++h) {
+                const int q = (int)lane + 64 * h;
+                L.slot[q] = q == full ? 0 : -1;
+                L.up[q] = -1;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int pass = 0; pass < kTreeletLeaves - 1; ++pass) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int q = (int)lane + 64 * h;
+                    const int t = q <= full ? L.slot[q] : -1;
+                    if (t < 0) continue;
+                    const int lp = L.split[q], rp = q ^ lp;
+                    L.up[lp] = (int8_t)t;
+                    L.side[lp] = 0;
+                    L.up[rp] = (int8_t)t;
+                    L.side[rp] = 1;
+                    if (lp & (lp - 1)) L.slot[lp] = (int8_t)(t + 1);
+                    if (rp & (rp - 1)) L.slot[rp] = (int8_t)(t + __popc(lp));
                 }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = (int)lane + 64 * h;
+                if (q == 0 || q > full) continue;
+                const int t = L.slot[q], u = L.up[q];
+                if (t < 0 && u < 0) continue;   // not in the rebuilt treelet
+                const int me = t >= 0 ? L.inner[t] : L.ref[__builtin_ctz(q)];
+                if (t >= 0) {
+                    a.cost[me] = L.cost[q];
+                    a.height[me] = L.hh[q];
+                }
+                if (u < 0) continue;   // the root
+                const int par = L.inner[u], sd = L.side[q];
+                nodes[par].child[sd] = me;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) nodes[par].box[6 * sd + e] = L.box[q][e];
+                a.parent[me >= 0 ? me : (a.n - 1) + ~me] = par;
             }
         }
     }
@@ -663,15 +805,33 @@ __device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, Tree
     }
 }
 
-__global__ __launch_bounds__(256) void k_tl_process(TreeletArgs a, uint32_t d) {
+// Wavefront w of the grid climbs from the w-th start node.  Between a node and its parent: release fence, one relaxed
+// counter increment by lane 0, acquire fence on the arrival that carries on, so the next
+// restructuring sees every write below it whichever XCD made them.
+__global__ __launch_bounds__(256) void k_tl_climb(TreeletArgs a) {
     __shared__ TreeletLds s_tl[4];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t start = a.bins[2 * kDepthBins + d], count = a.bins[d];
-    for (uint32_t i = blockIdx.x * 4u + w; i < count; i += gridDim.x * 4u) {
-        const int n = __builtin_amdgcn_readfirstlane((int)a.order[start + i]);
-        tl_process_wave(a, n, lane, s_tl[w]);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+    const uint32_t i = blockIdx.x * 4u + w;
+    const uint32_t count = __builtin_amdgcn_readfirstlane(a.starts[0]);
+    if (i < min(count, a.start_cap)) {
+        int subs[6];
+#pragma unroll
+        for (int sz = 0; sz < 6; ++sz) subs[sz] = kTreeletBySize.v[sz][lane];
+        int n = __builtin_amdgcn_readfirstlane((int)a.starts[1 + i]);
+        for (;;) {
+            tl_process_wave(a, n, lane, s_tl[w], subs);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (n == 0) break;
+            const int p = __builtin_amdgcn_readfirstlane(a.parent[n]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            uint32_t old = 0;
+            if (lane == 0) old = __hip_atomic_fetch_add(&a.arrivals[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            old = __builtin_amdgcn_readfirstlane(old);
+            if (old != 1u) break;   // a restructured child of p is still to come; its wavefront carries on
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            n = p;
+        }
     }
 }
 
@@ -745,21 +905,21 @@ void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
                        a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.tl_cost, a.tl_height, a.error);
     const int ni = n > 1 ? (int)n - 1 : 1;
     if (a.treelet_rounds > 0 && n >= 3) {
-        TreeletArgs t{a.nodes, a.parent, ni, (int)n, a.tl_cost, a.tl_height, a.tl_order, a.tl_bins, a.error};
+        // the refit's arrival counters are free again: they count arrivals of the climb
+        const uint32_t cap = treelet_start_cap(n);
+        TreeletArgs t{a.nodes, a.parent, ni, (int)n, a.tl_cost, a.tl_height, a.tl_snap, a.refit_cnt,
+                      a.tl_starts, cap, a.error};
         const uint32_t gi = (uint32_t)(ni + 255) / 256u;
         for (uint32_t r = 0; r < a.treelet_rounds; ++r) {
-            launch_fill_u32(a.tl_bins, 0u, 3 * kDepthBins, s);
-            hipLaunchKernelGGL(k_tl_hist, dim3(gi), dim3(256), 0, s, t);
-            hipLaunchKernelGGL(k_tl_scan, dim3(1), dim3(64), 0, s, a.tl_bins);
-            hipLaunchKernelGGL(k_tl_scatter, dim3(gi), dim3(256), 0, s, t);
-            // lowest first; heights the tree does not reach find no node (fewer nodes
-            // stand at each height above the lowest ones: smaller grids there)
-            for (int h = kTreeletMinHeight; h < kDepthBins; ++h)
-                hipLaunchKernelGGL(k_tl_process, dim3(h < 16 ? 1024 : 64), dim3(256), 0, s, t, (uint32_t)h);
+            launch_fill_u32(a.tl_starts, 0u, 1u, s);
+            hipLaunchKernelGGL(k_tl_prep, dim3(gi), dim3(256), 0, s, t);
+            hipLaunchKernelGGL(k_tl_climb, dim3((cap + 3) / 4), dim3(256), 0, s, t);
         }
     }
     hipLaunchKernelGGL(k_collapse4, dim3((ni + 255) / 256), dim3(256), 0, s, a.nodes, ni, a.nodes4);
 }
+
+uint32_t treelet_start_cap(uint32_t n) { return n / 6u + 1u; }
 
 uint32_t sort_hist_entries(uint32_t n) { return 256u * ((n + kSortTile - 1) / kSortTile); }
 

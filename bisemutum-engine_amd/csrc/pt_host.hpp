@@ -155,8 +155,8 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<bipt::LeafGeo> leaf_geo;
     bipt::DevBuf<bipt::LeafShade> leaf_shade;
     bipt::DevBuf<float> tl_cost;
-    bipt::DevBuf<int32_t> tl_height;
-    bipt::DevBuf<uint32_t> tl_order, tl_bins;
+    bipt::DevBuf<int32_t> tl_height, tl_snap;
+    bipt::DevBuf<uint32_t> tl_starts;
     uint32_t treelet_rounds = 2;    // bi_pt_set_build_options
     bool bvh_built = false;
     // accumulation

@@ -47,11 +47,12 @@ struct BuildArgs {
     uint32_t treelet_rounds;
     float* tl_cost;
     int32_t* tl_height;
-    uint32_t* tl_order;
-    uint32_t* tl_bins;   // 192 entries
+    int32_t* tl_snap;    // per internal node: height at the start of a round
+    uint32_t* tl_starts; // 1 + treelet_start_cap(num_tris) entries
 };
 void launch_build_lbvh(const BuildArgs& a, hipStream_t s);
 uint32_t sort_hist_entries(uint32_t n);
+uint32_t treelet_start_cap(uint32_t n);
 
 // Jobs of a wave: records, path bases (num_jobs + 1 entries, the last = paths in the
 // wave) and pixel bases (num_jobs + 1 entries) for the resolve pass.

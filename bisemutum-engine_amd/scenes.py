@@ -321,6 +321,21 @@ def sized_room_plan(target_tris: int):
     return s, max(1, int(round(body / (20 * 4 ** s))))
 
 
+def scene_random_triangles(n: int, seed: int):
+    """n triangles of edge ~1 scattered in a 10 m cube (one instance, c0's first
+    material, sky and sun): small trees whose treelets reach 6 and 7 leaves."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-5.0, 5.0, (n, 1, 3)).astype(np.float32)
+    v = (c + rng.uniform(-0.5, 0.5, (n, 3, 3))).astype(np.float32).reshape(-1, 3)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    nrm = np.tile(np.array([0.0, 1.0, 0.0], np.float32), (3 * n, 1))
+    base = scene_c0()
+    inst = np.zeros(1, INSTANCE_DTYPE)
+    inst["xform"][0] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    return dataclasses.replace(base, name=f"random_triangles_{n}", meshes=[(v, nrm, idx)], instances=inst,
+                               materials=base.materials[:1])
+
+
 def scene_sized(target_tris: int, seed: int):
     """c0-style room (24 triangles) with icospheres adding up to about target_tris.
 

@@ -75,6 +75,22 @@ def test_bvh_bit_exact_c0_rounds(oracle_lib, c0_scene, rounds):
     d.close()
 
 
+@pytest.mark.parametrize("n,seed", [(12, 1), (20, 2), (40, 3), (64, 4), (100, 5), (200, 6)])
+def test_treelets_small_trees_bit_exact(oracle_lib, n, seed):
+    """Trees of 12-200 triangles: the root's treelet (and most others) has 6 or 7 leaves,
+    so the split search over the full leaf set decides the result."""
+    print(f"case: {n} random triangles, seed {seed}", flush=True)
+    sc = scenes.scene_random_triangles(n, seed)
+    for rounds in (1, 2):
+        g = bi_pt.PathTracer()
+        g.set_scene(sc)
+        g.set_build_options(treelet_rounds=rounds)
+        o = oracle_lib.OracleScene(sc)
+        o.set_treelet_rounds(rounds)
+        compare_bvh(g.bvh(), o.bvh())
+        g.close()
+
+
 def random_rays(rng, n, centre, radius, tmax=1000.0):
     o = centre + rng.uniform(-radius, radius, size=(n, 3))
     d = rng.normal(size=(n, 3))
