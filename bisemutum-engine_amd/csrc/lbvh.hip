@@ -417,13 +417,19 @@ __device__ __forceinline__ float half_area(const float* b) {
 // every children-first order gives the same tree -- the CPU restatement's recursion.
 // One launch per round schedules them bottom-up like the refit: a wavefront starts at
 // each node of exactly that height (its subtree holds no other such node), restructures
-// it, then counts itself in at the parent; the arrival that completes the parent's count
-// (one per child that is restructured this round) carries on there, the other ends.  No
+// it, then moves to the parent -- directly if it is the parent's only restructured
+// child, else it counts itself in and the second arrival carries on, the first ends.  No
 // wavefront waits for another.  The refit publishes every subtree's cost and height for
 // the first round; a round leaves them current for the next.
 constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletMaxHeight = 48;
 constexpr int kTreeletMinHeight = 5;   // lower subtrees are left as built (their share of the cost is small)
+constexpr uint32_t kTreeletSoleChild = 0x100u;   // arrival count of a parent with one restructured child
+// Subtrees whose root stands at most this high (and whose parent stands higher) are
+// restructured by one wavefront alone, children first: at most 2^(8 - 4) - 1 = 15 nodes
+// of height >= kTreeletMinHeight each.
+constexpr int kTreeletSerialHeight = 8;
+constexpr int kTreeletSerialMax = 1 << (kTreeletSerialHeight - 4);
 
 __device__ __forceinline__ float half_area6(const float* b) {
     const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
@@ -437,7 +443,8 @@ struct TreeletArgs {
     float* cost;        // per internal node: surface-area cost of its subtree
     int32_t* height;    // per internal node: levels below it
     int32_t* snap;      // per internal node: its height at the start of the round
-    uint32_t* arrivals; // per internal node: 2 - children restructured this round, + arrivals
+    uint32_t* arrivals; // per internal node: kTreeletSoleChild if one child is restructured
+                        // this round, else 0 + arrivals
     uint32_t* starts;   // [0]: count, then the nodes a climb starts from (at most n / 6)
     uint32_t start_cap;
     uint32_t* error;
@@ -446,17 +453,20 @@ struct TreeletArgs {
 __global__ __launch_bounds__(256) void k_tl_prep(TreeletArgs a) {
     const int k = (int)(blockIdx.x * 256u + threadIdx.x);
     if (k >= a.ni) return;
-    a.snap[k] = a.height[k];
+    const int h = a.height[k];
+    a.snap[k] = h;
     uint32_t t = 0;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int ch = a.nodes[k].child[c];
         t += (ch >= 0 && a.height[ch] >= kTreeletMinHeight) ? 1u : 0u;
     }
-    a.arrivals[k] = 2u - t;
-    // a climb starts at every node of height exactly kTreeletMinHeight (their subtrees
-    // are disjoint and hold at least 6 triangles each); one atomic per wavefront
-    const bool start = a.snap[k] == kTreeletMinHeight;
+    a.arrivals[k] = t == 1u ? kTreeletSoleChild : 0u;
+    // a wavefront starts at every node of height kTreeletMinHeight..kTreeletSerialHeight
+    // whose parent stands higher (or that is the root): these subtrees are disjoint and
+    // hold at least 6 triangles each.  One atomic per wavefront.
+    const bool start = h >= kTreeletMinHeight && h <= kTreeletSerialHeight &&
+                       (k == 0 || a.height[a.parent[k]] > kTreeletSerialHeight);
     const uint64_t m = __ballot(start);
     if (!start) return;
     const uint32_t lane = threadIdx.x & 63u;
@@ -485,6 +495,7 @@ struct TreeletLds {
     int8_t side[1 << kTreeletLeaves];   // rebuild: 0 left part, 1 right part
     int ref[8];     // treelet leaves: internal node index, or ~triangle
     int inner[8];   // the treelet's internal nodes, root first
+    int list[kTreeletSerialMax];   // the start node's subtree: its nodes to restructure, level by level
 };
 
 // Subsets of 7 leaves by size (1..5): entry [s][l] is the l-th subset of s leaves in
@@ -805,33 +816,77 @@ __device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, Tree
     }
 }
 
-// Wavefront w of the grid climbs from the w-th start node.  Between a node and its parent: release fence, one relaxed
-// counter increment by lane 0, acquire fence on the arrival that carries on, so the next
-// restructuring sees every write below it whichever XCD made them.
-__global__ __launch_bounds__(256) void k_tl_climb(TreeletArgs a) {
-    __shared__ TreeletLds s_tl[4];
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t i = blockIdx.x * 4u + w;
+// Workgroup w (one wavefront) restructures the subtree of the w-th start node, then
+// climbs from there.  A parent whose only
+// restructured child is this one is taken over directly (a workgroup-scope fence makes
+// this wavefront's own stores visible to all its lanes).  Otherwise: release fence, one
+// relaxed counter increment by lane 0, acquire fence on the arrival that carries on, so
+// the next restructuring sees every write below it whichever XCD made them.
+__global__ __launch_bounds__(64) void k_tl_climb(TreeletArgs a) {
+    __shared__ TreeletLds s_tl;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x;
     const uint32_t count = __builtin_amdgcn_readfirstlane(a.starts[0]);
-    if (i < min(count, a.start_cap)) {
-        int subs[6];
+    if (i >= min(count, a.start_cap)) return;
+    int subs[6];
 #pragma unroll
-        for (int sz = 0; sz < 6; ++sz) subs[sz] = kTreeletBySize.v[sz][lane];
-        int n = __builtin_amdgcn_readfirstlane((int)a.starts[1 + i]);
-        for (;;) {
-            tl_process_wave(a, n, lane, s_tl[w], subs);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (n == 0) break;
-            const int p = __builtin_amdgcn_readfirstlane(a.parent[n]);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            uint32_t old = 0;
-            if (lane == 0) old = __hip_atomic_fetch_add(&a.arrivals[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            old = __builtin_amdgcn_readfirstlane(old);
-            if (old != 1u) break;   // a restructured child of p is still to come; its wavefront carries on
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            n = p;
+    for (int sz = 0; sz < 6; ++sz) subs[sz] = kTreeletBySize.v[sz][lane];
+    int n = __builtin_amdgcn_readfirstlane((int)a.starts[1 + i]);
+    // the start node's subtree, level by level (lane j expands entry j of a level; the
+    // nodes it adds keep lane order), then restructured in reverse: children first
+    if (lane == 0) s_tl.list[0] = n;
+    int lo = 0, hi = 1;
+    while (lo < hi) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int c0 = -1, c1 = -1;
+        bool t0 = false, t1 = false;
+        if (lo + (int)lane < hi) {
+            const int x = s_tl.list[lo + lane];
+            c0 = a.nodes[x].child[0];
+            c1 = a.nodes[x].child[1];
+            t0 = c0 >= 0 && a.snap[c0] >= kTreeletMinHeight;
+            t1 = c1 >= 0 && a.snap[c1] >= kTreeletMinHeight;
         }
+        const uint64_t b0 = __ballot(t0), b1 = __ballot(t1);
+        const int added = __popcll(b0) + __popcll(b1);
+        if (hi + added > kTreeletSerialMax) {   // cannot happen for a start node (see above)
+            if (lane == 0) atomicOr(a.error, 16u);
+            return;
+        }
+        const uint64_t below = (1ull << lane) - 1ull;
+        const int pos = hi + __popcll(b0 & below) + __popcll(b1 & below);
+        if (t0) s_tl.list[pos] = c0;
+        if (t1) s_tl.list[pos + (t0 ? 1 : 0)] = c1;
+        lo = hi;
+        hi += added;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int j = hi - 1; j > 0; --j) {
+        tl_process_wave(a, __builtin_amdgcn_readfirstlane(s_tl.list[j]), lane, s_tl, subs);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (;;) {
+        tl_process_wave(a, n, lane, s_tl, subs);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (n == 0) break;
+        const int p = __builtin_amdgcn_readfirstlane(a.parent[n]);
+        const uint32_t pre = __builtin_amdgcn_readfirstlane(a.arrivals[p]);
+        if (pre == kTreeletSoleChild) {   // the only restructured child of p (no count is kept)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            n = p;
+            continue;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        uint32_t old = 0;
+        if (lane == 0) old = __hip_atomic_fetch_add(&a.arrivals[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        old = __builtin_amdgcn_readfirstlane(old);
+        if (old != 1u) break;   // the other restructured child of p is still to come; its wavefront carries on
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        n = p;
     }
 }
 
@@ -913,7 +968,7 @@ void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
         for (uint32_t r = 0; r < a.treelet_rounds; ++r) {
             launch_fill_u32(a.tl_starts, 0u, 1u, s);
             hipLaunchKernelGGL(k_tl_prep, dim3(gi), dim3(256), 0, s, t);
-            hipLaunchKernelGGL(k_tl_climb, dim3((cap + 3) / 4), dim3(256), 0, s, t);
+            hipLaunchKernelGGL(k_tl_climb, dim3(cap), dim3(64), 0, s, t);
         }
     }
     hipLaunchKernelGGL(k_collapse4, dim3((ni + 255) / 256), dim3(256), 0, s, a.nodes, ni, a.nodes4);
