@@ -51,10 +51,10 @@ bi_pt_status engine_init(Owner* own, Engine& eng) {
 void engine_free(Owner* own, Engine& eng) {
     for (int k = 0; k < 6; ++k) { dfree(own, eng.qa[k]); dfree(own, eng.qb[k]); }
     for (int k = 0; k < 4; ++k) dfree(own, eng.ps_f[k]);
-    for (int k = 0; k < 9; ++k) { dfree(own, eng.sq_f[k]); dfree(own, eng.st_f[k]); }
-    dfree(own, eng.acc); dfree(own, eng.st_c);
-    dfree(own, eng.qa_path); dfree(own, eng.qb_path); dfree(own, eng.sq_pk); dfree(own, eng.st_path);
-    dfree(own, eng.st_mask); dfree(own, eng.tile_sky); dfree(own, eng.hit_rec); dfree(own, eng.tile_sums);
+    for (int k = 0; k < 9; ++k) dfree(own, eng.sq_f[k]);
+    dfree(own, eng.acc);
+    dfree(own, eng.qa_path); dfree(own, eng.qb_path); dfree(own, eng.sq_pk);
+    dfree(own, eng.hit_rec); dfree(own, eng.tile_state);
     dfree(own, eng.counters); dfree(own, eng.plan); dfree(own, eng.empty_tree);
     if (eng.ready) {
         for (auto& ev : eng.ev) (void)hipEventDestroy(ev);
@@ -81,7 +81,8 @@ bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out) {
                     std::string("device check failed: ") + ((hc[kCntError] & 1u) ? "traversal stack overflow " : "") +
                         ((hc[kCntError] & 4u) ? "refit wait limit " : "") +
                         ((hc[kCntError] & 8u) ? "traversal step bound exceeded " : "") +
-                        ((hc[kCntError] & 16u) ? "treelet start list overflow" : ""));
+                        ((hc[kCntError] & 16u) ? "treelet start list overflow " : "") +
+                        ((hc[kCntError] & 32u) ? "shade look-back wait limit" : ""));
     return BI_PT_OK;
 }
 
@@ -93,8 +94,7 @@ uint64_t engine_bytes(uint32_t cap, uint32_t slots, uint32_t sh_mult) {
     b += 4 * c * 4;                              // path state
     b += (uint64_t)slots * 3 * c * 4;            // radiance accumulators
     b += (uint64_t)sh_mult * c * (9 * 4 + 4);    // shadow queue
-    b += 9 * c * 4 + 3 * (uint64_t)(slots - 1) * c * 4 + 2 * c * 4;   // shade staging
-    b += tiles * (8 + 4);                        // tile totals
+    b += tiles * 8;                              // look-back words of the shade tiles
     return b;
 }
 
@@ -117,12 +117,7 @@ bi_pt_status ensure_wave_buffers(Owner* own, Engine& eng, uint32_t cap, uint32_t
     const size_t shcap = (size_t)cap * sh_mult;
     for (int k = 0; k < 9; ++k) BI_TRY(dalloc(own, eng.sq_f[k], shcap));
     BI_TRY(dalloc(own, eng.sq_pk, shcap));
-    for (int k = 0; k < 9; ++k) BI_TRY(dalloc(own, eng.st_f[k], cap));
-    BI_TRY(dalloc(own, eng.st_c, (size_t)3 * (slots - 1u) * cap));
-    BI_TRY(dalloc(own, eng.st_path, cap));
-    BI_TRY(dalloc(own, eng.st_mask, cap));
-    BI_TRY(dalloc(own, eng.tile_sums, (cap + 255u) / 256u + 1u));
-    BI_TRY(dalloc(own, eng.tile_sky, (cap + 255u) / 256u + 1u));
+    BI_TRY(dalloc(own, eng.tile_state, (cap + 255u) / 256u + 1u));
     eng.wave_cap = cap;
     eng.wave_slots = slots;
     eng.wave_sh_mult = sh_mult;
@@ -438,13 +433,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         RayQueue qA = queue_of(eng.qa, eng.qa_path), qB = queue_of(eng.qb, eng.qb_path);
         ShadowQueue sq{eng.sq_f[0].p, eng.sq_f[1].p, eng.sq_f[2].p, eng.sq_f[3].p, eng.sq_f[4].p,
                        eng.sq_f[5].p, eng.sq_f[6].p, eng.sq_f[7].p, eng.sq_f[8].p, eng.sq_pk.p};
-        const size_t wc = eng.wave_cap;
-        const size_t cslots = eng.wave_slots - 1u;
-        ShadeStage stage{eng.st_f[0].p, eng.st_f[1].p, eng.st_f[2].p, eng.st_path.p,
-                         eng.st_f[3].p, eng.st_f[4].p, eng.st_f[5].p,
-                         eng.st_f[6].p, eng.st_f[7].p, eng.st_f[8].p,
-                         eng.st_c.p, eng.st_c.p + cslots * wc, eng.st_c.p + 2 * cslots * wc,
-                         eng.st_mask.p, eng.tile_sums.p, eng.tile_sky.p};
+
         if (record) { eng.rec_ext.clear(); eng.rec_sh.clear(); }
 
         // the launches of one kind are timed together: events around each class of kernel
@@ -497,13 +486,21 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 ++launches_closest;
                 BI_TRY(mark(1));
 
+                launch_shade_reset(eng.tile_state.p, primary ? nullptr : cnt_in, J.num_paths, cnt_out, &cnt[kCntShadow],
+                                   J.num_paths, s);
                 ShadeArgs sa{};
                 sa.jobs = J;
                 sa.qin = qin;
                 sa.count_in = primary ? nullptr : cnt_in;
                 sa.hits = hits;
-                sa.st = stage;
                 sa.ps = ps;
+                sa.qout = qout;
+                sa.sq = sq;
+                sa.count_ext = cnt_out;
+                sa.count_shadow = &cnt[kCntShadow];
+                sa.ray_counters = rays;
+                sa.tile_state = eng.tile_state.p;
+                sa.error = &cnt[kCntError];
                 sa.trace_work_reset = &cnt[kCntTraceWork];
                 sa.shadow_work_reset = &cnt[kCntShadowWork];
                 sa.bounce = b;
@@ -522,25 +519,6 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                     launch_nrc_infer(na, s);
                 }
 
-                ScanArgs sc{};
-                sc.tile_sums = stage.tile_sums;
-                sc.tile_sky = stage.tile_sky;
-                sc.count_in = primary ? nullptr : cnt_in;
-                sc.num_entries = J.num_paths;
-                sc.count_ext = cnt_out;
-                sc.count_shadow = &cnt[kCntShadow];
-                sc.ray_counters = rays;
-                launch_scan_tiles(sc, s);
-
-                CompactArgs ca{};
-                ca.jobs = J;
-                ca.st = stage;
-                ca.count_in = primary ? nullptr : cnt_in;
-                ca.num_entries = J.num_paths;
-                ca.cap = wc;
-                ca.qout = qout;
-                ca.sq = sq;
-                launch_compact(ca, J.num_paths, s);
                 BI_TRY(mark(2));
 
                 ShadowArgs sha{};
@@ -548,6 +526,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sha.q = sq;
                 sha.count_in = &cnt[kCntShadow];
                 sha.ps = ps;
+                sha.ray_counters = rays;
                 sha.work = &cnt[kCntShadowWork];
                 sha.parts = parts;
                 sha.refill = refill;

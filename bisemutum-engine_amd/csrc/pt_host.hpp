@@ -93,10 +93,10 @@ struct Knobs {
 // Queues and per-wave buffers, sized for the largest wave rendered so far.
 struct Engine {
     uint32_t wave_cap = 0, wave_slots = 0, wave_sh_mult = 0;
-    DevBuf<float> qa[6], qb[6], ps_f[4], acc, sq_f[9], st_f[9], st_c;
-    DevBuf<uint32_t> qa_path, qb_path, sq_pk, st_path, st_mask, tile_sky;
+    DevBuf<float> qa[6], qb[6], ps_f[4], acc, sq_f[9];
+    DevBuf<uint32_t> qa_path, qb_path, sq_pk;
     DevBuf<HitRec> hit_rec;
-    DevBuf<uint64_t> tile_sums;
+    DevBuf<uint64_t> tile_state;           // look-back words of the shade launch's tiles
     DevBuf<uint32_t> counters;             // kCntSlots (ray counters at kCntRays as 4 x u64)
     DevBuf<unsigned char> plan;            // job records and bases of every wave of a call
     unsigned char* plan_host = nullptr;    // pinned staging of the plan (copied asynchronously)

@@ -115,6 +115,7 @@ struct ShadowArgs {
     ShadowQueue q;
     const uint32_t* count_in;
     PathState ps;
+    uint64_t* ray_counters;     // sky shadow rays move from [2] (all shadow rays) to [3]
     uint32_t* work;             // part counters of the persistent lanes (zero at launch)
     uint32_t parts;             // queue parts (1..kWorkPartsMax)
     uint32_t refill;            // idle lanes before a refill (1..64)
@@ -122,54 +123,33 @@ struct ShadowArgs {
 };
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, uint32_t block, hipStream_t s);
 
-// Per-entry staging written by the shade kernel and read by the compaction kernel.
-struct ShadeStage {
-    float *px, *py, *pz;    // hit point (entries with any output)
-    uint32_t* path;
-    float *ex, *ey, *ez;    // extension direction
-    float *sx, *sy, *sz;    // sky shadow-ray direction
-    float *cx, *cy, *cz;    // shadow contribution of kind k (lights 0..n-1, sky n) at k * cap + entry
-    uint32_t* mask;         // bit 0 extension ray, bit 1 + l light l, bit 1 + kMaxDirLights sky
-    uint64_t* tile_sums;    // per 256-entry tile: extension << 32 | shadow (exclusive prefix after the scan)
-    uint32_t* tile_sky;     // per tile: sky shadow rays
-};
-
+// Shading of one segment writes its outputs straight into the next queues, in entry
+// order: each 256-entry tile publishes its totals and finds its offset by decoupled
+// look-back over the tiles before it.
 struct ShadeArgs {
     WaveJobs jobs;
     RayQueue qin;
     const uint32_t* count_in;   // null at bounce 0
     HitBuf hits;
-    ShadeStage st;
     PathState ps;
+    RayQueue qout;              // extension rays of the next segment
+    ShadowQueue sq;             // shadow rays of this segment (kinds: lights, then the sky)
+    uint32_t* count_ext;        // written by the last tile (zeroed by launch_shade_reset)
+    uint32_t* count_shadow;
+    uint64_t* ray_counters;     // [camera, extension, light shadow, sky shadow]
+    uint64_t* tile_state;       // per tile: look-back word (zero at launch)
+    uint32_t* error;            // 32: a look-back wait ran out
     uint32_t* trace_work_reset;   // zeroed here for the next closest-hit launch
     uint32_t* shadow_work_reset;  // zeroed here for this segment's shadow launch
     uint32_t bounce;
     NrcRecord* nrc_query;         // non-null: the cache kernel runs after this segment's
                                   // shading; entry i's query (or kNoPath) goes to [i]
 };
+// Clears the output counts and the look-back words of the tiles the next shade launch
+// covers (count_in entries, or num_entries if count_in is null).
+void launch_shade_reset(uint64_t* tile_state, const uint32_t* count_in, uint32_t num_entries, uint32_t* count_ext,
+                        uint32_t* count_shadow, uint32_t capacity, hipStream_t s);
 void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s);
-
-struct ScanArgs {
-    uint64_t* tile_sums;
-    const uint32_t* tile_sky;
-    const uint32_t* count_in;   // null: num_entries
-    uint32_t num_entries;
-    uint32_t* count_ext;
-    uint32_t* count_shadow;
-    uint64_t* ray_counters;     // [camera, extension, light shadow, sky shadow]
-};
-void launch_scan_tiles(const ScanArgs& a, hipStream_t s);
-
-struct CompactArgs {
-    WaveJobs jobs;
-    ShadeStage st;
-    const uint32_t* count_in;   // null: num_entries
-    uint32_t num_entries;
-    size_t cap;
-    RayQueue qout;
-    ShadowQueue sq;
-};
-void launch_compact(const CompactArgs& a, uint32_t capacity, hipStream_t s);
 
 struct ResolveArgs {
     WaveJobs jobs;

@@ -206,7 +206,8 @@ enum CounterSlot : uint32_t {
     kCntExtA = 0,       // entries in extension queue A
     kCntExtB = 1,       // entries in extension queue B
     kCntShadow = 2,     // entries in the shadow queue
-    kCntError = 4,      // nonzero: a kernel hit an internal limit (1 stack, 4 refit wait, 8 step bound)
+    kCntError = 4,      // nonzero: a kernel hit an internal limit (1 stack, 4 refit wait, 8 step bound,
+                        // 16 treelet start list, 32 shade look-back wait)
     kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
     kCntTraceWork = 32,                                       // part counters of the closest-hit lanes
     kCntShadowWork = kCntTraceWork + kWorkPartsMax * kWorkStride, // part counters of the shadow lanes

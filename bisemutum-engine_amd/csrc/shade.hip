@@ -8,15 +8,17 @@
 //             (exact visibility replaces the shadow map); one sky NEE shadow ray;
 //             VNDF-sampled extension ray while max(throughput) >= 0.001
 // Instead of full-screen passes with dead lanes, live rays are compacted into the
-// next queues in three steps:
-//   k_shade        one lane per queue entry; outputs go to fixed staging slots (entry
-//                  i, shadow kind k at k*cap + i), an emission mask per entry and the
-//                  (extension, shadow) totals of each 256-entry tile
-//   k_scan_tiles   exclusive scan of the tile totals (one workgroup)
-//   k_compact      wavefront prefix of the masks inside each tile plus the tile's
-//                  offset: copies survivors, in entry order, into the compact queues
-// so queue order equals input order (deterministic contents), and no lane holds its
-// outputs while waiting for other tiles.
+// next queues by the shading kernel itself, in entry order (deterministic contents):
+// one lane per queue entry keeps its outputs in registers; workgroup t shades tile t
+// (256 entries), scans its entries' emissions (extension ray, shadow rays) and
+// publishes the tile total, then finds its offset by decoupled look-back over the
+// tiles before it (one wavefront reads 64 tile words at a time) and publishes its
+// inclusive prefix; the lanes then write their rays at their final places.  A tile
+// waits only for lower tiles, and publishes its total before it waits.  Workgroups
+// reach each XCD in index order, so the lowest unfinished tile is always running and
+// every wait ends; the wait is bounded all the same (error flag 32).  (A ticket taken
+// with one atomic per workgroup instead of the index costs more than the compaction
+// saves: ~130k same-address atomics per launch.)
 #include "pt_kernels.hpp"
 
 namespace bipt {
@@ -25,15 +27,26 @@ namespace {
 constexpr uint32_t kTile = 256;
 constexpr uint32_t kMaskExt = 1u;   // bit 0: extension ray; bit 1 + k: shadow slot k (lights 0..3, sky 4)
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
+// Look-back word of a tile: status in bits 62-63 (1 tile total, 2 inclusive prefix),
+// shadow rays in bits 31-61, extension rays in bits 0-30 (at most 5 * 2^28 and 2^28:
+// fields never carry into each other when words are added).
+constexpr uint64_t kLbTotal = 1ull << 62, kLbPrefix = 2ull << 62, kLbFields = (1ull << 62) - 1ull;
+constexpr uint32_t kLbSpinMax = 1u << 22;
+__device__ __forceinline__ uint64_t lb_word(uint32_t ext, uint32_t sh) { return ((uint64_t)sh << 31) | ext; }
+__device__ __forceinline__ uint32_t lb_ext(uint64_t w) { return (uint32_t)(w & 0x7fffffffull); }
+__device__ __forceinline__ uint32_t lb_sh(uint64_t w) { return (uint32_t)((w >> 31) & 0x7fffffffull); }
 
-// Shades queue entry i (path `path`, the job's local path `local`) of job J: writes the
-// entry's staging slots and returns its emission mask.
-__device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec& J, uint32_t i, uint32_t path,
-                                                uint32_t local, bool primary) {
+// Outputs of one queue entry, held until the entry's place in the queues is known.
+struct ShadeOut {
+    uint32_t mask = 0, path = 0;
+    V3 P, ext, sky;
+    V3 c[kMaxDirLights + 1];   // shadow contribution of light l at [l], of the sky at [kMaxDirLights]
+};
+
+// Shades queue entry i (path `path`, the job's local path `local`) of job J: its rays
+// go to `out` (mask: which of them exist).
+__device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J, uint32_t i, uint32_t path,
+                                            uint32_t local, bool primary, ShadeOut& out) {
     const size_t cap = a.ps.cap;
     uint32_t mask = 0;
     const PathKey key = path_key(J, local);
@@ -104,7 +117,8 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
                 // for this vertex's lighting and everything after it
                 r.tag = path;
                 a.nrc_query[i] = r;
-                return 0u;
+                out.mask = 0u;
+                return;
             }
             // training path: the radiance gathered from here on, per unit throughput,
             // is the target of this sample
@@ -121,7 +135,9 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
         Frame fr = make_frame(N);
         Pcg rng = pcg_keyed(J.seed, pixel, sample, a.bounce);
         float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
-        for (uint32_t l = 0; l < nl; ++l) {
+#pragma unroll
+        for (uint32_t l = 0; l < kMaxDirLights; ++l) {
+            if (l >= nl) break;
             const LightRec& L = J.lights[l];
             V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
             V3 ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
@@ -135,8 +151,7 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
                 acc[2 * cap + path] = acc[2 * cap + path] + c.z;
                 continue;
             }
-            const size_t s = (size_t)l * cap + i;
-            a.st.cx[s] = c.x; a.st.cy[s] = c.y; a.st.cz[s] = c.z;
+            out.c[l] = c;
             mask |= 2u << l;
         }
         // sky NEE only where the next segment is sampled too (the reference sees the
@@ -153,9 +168,8 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
                     float w = power_heuristic(pl, pb);
                     V3 c = ((sky_eval(J.sky, wi) * f) * (w / pl)) * beta;
                     if (any_pos(c) && finite3(c)) {
-                        const size_t s = (size_t)nl * cap + i;
-                        a.st.cx[s] = c.x; a.st.cy[s] = c.y; a.st.cz[s] = c.z;
-                        a.st.sx[i] = wi.x; a.st.sy[i] = wi.y; a.st.sz[i] = wi.z;
+                        out.c[kMaxDirLights] = c;
+                        out.sky = wi;
                         mask |= 2u << kMaxDirLights;
                     }
                 }
@@ -173,7 +187,7 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
             if (!finite3(w)) w = v3(0.0f, 0.0f, 0.0f);
             V3 nb = beta * w;
             if (max3(nb) > 0.0f) {
-                a.st.ex[i] = Lext.x; a.st.ey[i] = Lext.y; a.st.ez[i] = Lext.z;
+                out.ext = Lext;
                 a.ps.bx[path] = nb.x; a.ps.by[path] = nb.y; a.ps.bz[path] = nb.z;
                 a.ps.pdf[path] = pdf;
                 mask |= kMaskExt;
@@ -186,15 +200,73 @@ __device__ __forceinline__ uint32_t shade_entry(const ShadeArgs& a, const JobRec
             J.depth_out[pixel] = z / w;
         }
     }
-    if (mask) {
-        a.st.px[i] = P.x; a.st.py[i] = P.y; a.st.pz[i] = P.z;
-        a.st.path[i] = path;
+    out.mask = mask;
+    out.P = P;
+    out.path = path;
+}
+
+// Writes an entry's rays: the extension ray at queue place e, its shadow rays from
+// place o on in kind order (lights 0..3, then the sky).
+__device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, const ShadeOut& out, uint32_t e,
+                                           uint32_t o) {
+    const uint32_t mask = out.mask;
+    if (mask & kMaskExt) {
+        a.qout.ox[e] = out.P.x; a.qout.oy[e] = out.P.y; a.qout.oz[e] = out.P.z;
+        a.qout.dx[e] = out.ext.x; a.qout.dy[e] = out.ext.y; a.qout.dz[e] = out.ext.z;
+        a.qout.path[e] = out.path;
     }
-    return mask;
+#pragma unroll
+    for (uint32_t k = 0; k <= kMaxDirLights; ++k) {
+        const bool sky = k == kMaxDirLights;
+        if (!(mask & (2u << k))) continue;
+        a.sq.ox[o] = out.P.x; a.sq.oy[o] = out.P.y; a.sq.oz[o] = out.P.z;
+        if (sky) {
+            a.sq.dx[o] = out.sky.x; a.sq.dy[o] = out.sky.y; a.sq.dz[o] = out.sky.z;
+        } else {
+            const float* ld = J.lights[k].direction;
+            a.sq.dx[o] = ld[0]; a.sq.dy[o] = ld[1]; a.sq.dz[o] = ld[2];
+        }
+        a.sq.cx[o] = out.c[k].x; a.sq.cy[o] = out.c[k].y; a.sq.cz[o] = out.c[k].z;
+        a.sq.pk[o] = (out.path << 3) | (sky ? kKindSky : k);
+        ++o;
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Offset of tile t (> 0) in the queues: the sum of the words of tiles t-1, t-2, ...
+// down to the nearest published inclusive prefix, 64 tiles per step (lane j reads
+// tile p - j; tiles before 0 count as an empty prefix).  Runs in one wavefront.
+__device__ __forceinline__ uint64_t look_back(const uint64_t* state, uint32_t t, uint32_t lane, uint32_t* error) {
+    uint64_t sum = 0;
+    int p = (int)t - 1;
+    for (;;) {
+        const int q = p - (int)lane;
+        uint64_t w = q >= 0 ? __hip_atomic_load(state + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kLbPrefix;
+        uint32_t spins = 0;
+        while (__ballot((w >> 62) == 0ull) != 0ull) {
+            if (++spins > kLbSpinMax) {   // a tile never published: report, count it as empty
+                if (lane == 0) atomicOr(error, 32u);
+                if ((w >> 62) == 0ull) w = kLbPrefix;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            if ((w >> 62) == 0ull) w = __hip_atomic_load(state + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const uint64_t pm = __ballot((w >> 62) == 2ull);
+        const uint32_t j = pm ? (uint32_t)__builtin_ctzll(pm) : 64u;
+        sum += wave_sum64(lane <= j ? (w & kLbFields) : 0ull);
+        if (j < 64u) return sum;
+        p -= 64;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    __shared__ uint32_t s_wsum[4], s_wsky[4];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint64_t s_base;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const bool primary = a.bounce == 0;
     const uint32_t count = primary ? a.jobs.num_paths : *a.count_in;
@@ -204,136 +276,76 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         a.trace_work_reset[tid * kWorkStride] = 0u;
         a.shadow_work_reset[tid * kWorkStride] = 0u;
     }
+    const uint32_t ntiles = (count + kTile - 1u) / kTile;
     const uint32_t tile = blockIdx.x;
-    if (tile * kTile >= count) return;
+    if (tile >= ntiles) return;
     const uint32_t i = tile * kTile + tid;
-    uint32_t mask = 0;
+    ShadeOut out;
+    const JobRec* J = a.jobs.rec;
     if (i < count) {
         const uint32_t path = primary ? i : a.qin.path[i];
         if (a.jobs.num_jobs == 1u) {
-            mask = shade_entry(a, a.jobs.rec[0], i, path, path, primary);   // uniform record: scalar loads
+            shade_entry(a, a.jobs.rec[0], i, path, path, primary, out);   // uniform record: scalar loads
         } else {
             const uint32_t jid = job_of(a.jobs.path_base, a.jobs.num_jobs, path);
-            mask = shade_entry(a, a.jobs.rec[jid], i, path, path - a.jobs.path_base[jid], primary);
+            J = a.jobs.rec + jid;
+            shade_entry(a, *J, i, path, path - a.jobs.path_base[jid], primary, out);
         }
-        a.st.mask[i] = mask;
     }
-    // tile totals (extension, shadow) and the tile's sky shadow rays (ray statistics)
-    const uint32_t packed = ((mask & kMaskExt) << 16) | (uint32_t)__popc(mask >> 1);
-    const uint32_t sum = wave_sum(packed);
-    const uint32_t sky = wave_sum((mask >> (1 + kMaxDirLights)) & 1u);
-    if (lane == 0) {
-        s_wsum[wave] = sum;
-        s_wsky[wave] = sky;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t agg = (s_wsum[0] + s_wsum[1]) + (s_wsum[2] + s_wsum[3]);
-        a.st.tile_sums[tile] = ((uint64_t)(agg >> 16) << 32) | (agg & 0xffffu);
-        a.st.tile_sky[tile] = (s_wsky[0] + s_wsky[1]) + (s_wsky[2] + s_wsky[3]);
-    }
-}
-
-// Exclusive scan of the per-tile (extension << 32 | shadow) totals, one workgroup.
-__global__ __launch_bounds__(1024) void k_scan_tiles(ScanArgs a) {
-    __shared__ uint64_t wsum[16];
-    __shared__ uint64_t carry_s;
-    __shared__ uint32_t wsky[16];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t count = a.count_in ? *a.count_in : a.num_entries;
-    const uint32_t ntiles = (count + kTile - 1) / kTile;
-    if (threadIdx.x == 0) carry_s = 0;
-    __syncthreads();
-    uint32_t sky = 0;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += 1024u) sky += a.tile_sky[t];
-    for (uint32_t start = 0; start < ntiles; start += 1024u) {
-        const uint32_t t = start + threadIdx.x;
-        const uint64_t v = t < ntiles ? a.tile_sums[t] : 0ull;
-        uint64_t inc = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint64_t y = __shfl_up(inc, off);
-            if (lane >= (uint32_t)off) inc += y;
-        }
-        if (lane == 63) wsum[wave] = inc;
-        __syncthreads();
-        uint64_t wbase = 0, total = 0;
-        for (uint32_t w = 0; w < 16; ++w) {
-            const uint64_t s = wsum[w];
-            if (w < wave) wbase += s;
-            total += s;
-        }
-        const uint64_t carry = carry_s;
-        if (t < ntiles) a.tile_sums[t] = carry + wbase + inc - v;
-        __syncthreads();
-        if (threadIdx.x == 0) carry_s = carry + total;
-        __syncthreads();
-    }
-    sky = (uint32_t)wave_sum(sky);
-    if (lane == 0) wsky[wave] = sky;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t nsky = 0;
-        for (uint32_t w = 0; w < 16; ++w) nsky += wsky[w];
-        const uint64_t tot = carry_s;
-        const uint32_t ne = (uint32_t)(tot >> 32), ns = (uint32_t)(tot & 0xffffffffull);
-        *a.count_ext = ne;
-        *a.count_shadow = ns;
-        // ray statistics: extension, light shadow and sky shadow rays of this segment
-        a.ray_counters[1] += ne;
-        a.ray_counters[2] += ns - nsky;
-        a.ray_counters[3] += nsky;
-    }
-}
-
-// Copies the staged outputs of one tile into the compact queues, in entry order.
-__global__ __launch_bounds__(256) void k_compact(CompactArgs a) {
-    __shared__ uint32_t s_wsum[4];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t count = a.count_in ? *a.count_in : a.num_entries;
-    const uint32_t tile = blockIdx.x;
-    if (tile * kTile >= count) return;
-    const uint32_t i = tile * kTile + tid;
-    const uint32_t mask = i < count ? a.st.mask[i] : 0u;
-    const uint32_t packed = ((mask & kMaskExt) << 16) | (uint32_t)__popc(mask >> 1);
+    // the tile's emissions: (extension << 16 | shadow) per entry, scanned in entry order
+    const uint32_t packed = ((out.mask & kMaskExt) << 16) | (uint32_t)__popc(out.mask >> 1);
     uint32_t inc = packed;
     for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(inc, off);
+        const uint32_t y = __shfl_up(inc, off);
         if (lane >= (uint32_t)off) inc += y;
     }
     if (lane == 63) s_wsum[wave] = inc;
     __syncthreads();
     uint32_t wbase = 0;
-    for (uint32_t w = 0; w < wave; ++w) wbase += s_wsum[w];
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) wbase += w < wave ? s_wsum[w] : 0u;
     const uint32_t excl = wbase + inc - packed;
-    const uint64_t base = a.st.tile_sums[tile];
-    if (mask) {
-        const float px = a.st.px[i], py = a.st.py[i], pz = a.st.pz[i];
-        const uint32_t path = a.st.path[i];
-        if (mask & kMaskExt) {
-            const uint32_t e = (uint32_t)(base >> 32) + (excl >> 16);
-            a.qout.ox[e] = px; a.qout.oy[e] = py; a.qout.oz[e] = pz;
-            a.qout.dx[e] = a.st.ex[i]; a.qout.dy[e] = a.st.ey[i]; a.qout.dz[e] = a.st.ez[i];
-            a.qout.path[e] = path;
+    if (wave == 0) {
+        const uint32_t agg = (s_wsum[0] + s_wsum[1]) + (s_wsum[2] + s_wsum[3]);
+        const uint64_t own = lb_word(agg >> 16, agg & 0xffffu);
+        uint64_t base = 0;
+        if (tile == 0) {
+            if (lane == 0) __hip_atomic_store(a.tile_state, kLbPrefix | own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(a.tile_state + tile, kLbTotal | own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            base = look_back(a.tile_state, tile, lane, a.error);
+            if (lane == 0)
+                __hip_atomic_store(a.tile_state + tile, kLbPrefix | (base + own), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        // shadow rays in kind order: lights 0..3, then the sky
-        const JobRec& J = a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, path)];
-        const uint32_t nl = J.num_lights;
-        uint32_t o = (uint32_t)(base & 0xffffffffull) + (excl & 0xffffu);
-        for (uint32_t k = 0; k <= nl; ++k) {
-            const bool sky = k == nl;
-            if (!(mask & (2u << (sky ? (uint32_t)kMaxDirLights : k)))) continue;
-            const size_t s = (size_t)k * a.cap + i;
-            a.sq.ox[o] = px; a.sq.oy[o] = py; a.sq.oz[o] = pz;
-            if (sky) {
-                a.sq.dx[o] = a.st.sx[i]; a.sq.dy[o] = a.st.sy[i]; a.sq.dz[o] = a.st.sz[i];
-            } else {
-                const float* ld = J.lights[k].direction;
-                a.sq.dx[o] = ld[0]; a.sq.dy[o] = ld[1]; a.sq.dz[o] = ld[2];
+        if (lane == 0) {
+            s_base = base;
+            if (tile == ntiles - 1u) {   // the last tile: totals of the segment
+                const uint64_t tot = base + own;
+                *a.count_ext = lb_ext(tot);
+                *a.count_shadow = lb_sh(tot);
+                a.ray_counters[1] += lb_ext(tot);
+                a.ray_counters[2] += lb_sh(tot);
             }
-            a.sq.cx[o] = a.st.cx[s]; a.sq.cy[o] = a.st.cy[s]; a.sq.cz[o] = a.st.cz[s];
-            a.sq.pk[o] = (path << 3) | (sky ? kKindSky : k);
-            ++o;
         }
+    }
+    __syncthreads();
+    if (out.mask) {
+        const uint64_t base = s_base;
+        emit_entry(a, *J, out, lb_ext(base) + (excl >> 16), lb_sh(base) + (excl & 0xffffu));
+    }
+}
+
+// Output counts and look-back words of the next shade launch.
+__global__ __launch_bounds__(256) void k_shade_reset(uint64_t* state, const uint32_t* count_in, uint32_t num_entries,
+                                                     uint32_t* count_ext, uint32_t* count_shadow) {
+    const uint32_t count = count_in ? *count_in : num_entries;
+    const uint32_t ntiles = (count + kTile - 1u) / kTile;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < ntiles) state[t] = 0ull;
+    if (t == 0) {
+        *count_ext = 0u;
+        *count_shadow = 0u;
     }
 }
 
@@ -394,18 +406,16 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
 
 }  // namespace
 
+void launch_shade_reset(uint64_t* tile_state, const uint32_t* count_in, uint32_t num_entries, uint32_t* count_ext,
+                        uint32_t* count_shadow, uint32_t capacity, hipStream_t s) {
+    const uint32_t tiles = (capacity + kTile - 1u) / kTile;
+    hipLaunchKernelGGL(k_shade_reset, dim3((tiles + 255u) / 256u + 1u), dim3(256), 0, s, tile_state, count_in,
+                       num_entries, count_ext, count_shadow);
+}
+
 void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s) {
     const uint32_t tiles = (capacity + kTile - 1) / kTile;
     hipLaunchKernelGGL(k_shade, dim3(tiles ? tiles : 1), dim3(256), 0, s, a);
-}
-
-void launch_scan_tiles(const ScanArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, a);
-}
-
-void launch_compact(const CompactArgs& a, uint32_t capacity, hipStream_t s) {
-    const uint32_t tiles = (capacity + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_compact, dim3(tiles ? tiles : 1), dim3(256), 0, s, a);
 }
 
 void launch_resolve(const ResolveArgs& a, hipStream_t s) {

@@ -469,7 +469,8 @@ __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3
     }
 }
 
-__device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bool occ) {
+// Adds shadow ray i's contribution unless it is occluded; returns whether it is a sky ray.
+__device__ __forceinline__ bool apply_shadow(const ShadowArgs& a, uint32_t i, bool occ) {
     const uint32_t pk = a.q.pk[i];
     const uint32_t path = pk >> 3, kind = pk & 7u;
     float keep = 1.0f;
@@ -484,8 +485,8 @@ __device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bo
     }
     V3 c = v3(a.q.cx[i], a.q.cy[i], a.q.cz[i]);
     if (occ) {
-        if (kind == kKindSky) return;
-        if (!(keep > 0.0f)) return;
+        if (kind == kKindSky) return true;
+        if (!(keep > 0.0f)) return false;
         c = c * keep;
     }
     const uint32_t slot = kind == kKindSky ? nl + 1u : kind + 1u;
@@ -494,6 +495,7 @@ __device__ __forceinline__ void apply_shadow(const ShadowArgs& a, uint32_t i, bo
     acc[path] = acc[path] + c.x;
     acc[cap + path] = acc[cap + path] + c.y;
     acc[2 * cap + path] = acc[2 * cap + path] + c.z;
+    return kind == kKindSky;
 }
 
 template <int L, int B>
@@ -525,6 +527,7 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
     ShortStack<L, B> st;
     int sp = 0;
     uint32_t steps = 0;
+    uint32_t sky_rays = 0;   // ray statistics: sky shadow rays this lane finished
     Node4 cur = load_node(nodes, 0);
     for (;;) {
         int next = kDone;
@@ -588,7 +591,7 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
             if (occ) occf[lane] = 0u;
         }
         if (ray != kNoRay && (occ || (parked && qn == 0u))) {
-            apply_shadow(a, ray, occ);
+            sky_rays += apply_shadow(a, ray, occ) ? 1u : 0u;
             ray = kNoRay;
             parked = false;
         }
@@ -618,6 +621,12 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
             steps = 0;
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
+    }
+    // the shade launch counted every shadow ray as a light shadow ray
+    for (int off = 32; off >= 1; off >>= 1) sky_rays += __shfl_xor(sky_rays, off);
+    if (lane == 0 && sky_rays) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 2), (unsigned long long)(0ull - sky_rays));
+        atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 3), (unsigned long long)sky_rays);
     }
 }
 
