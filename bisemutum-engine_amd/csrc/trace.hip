@@ -622,11 +622,19 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
-    // the shade launch counted every shadow ray as a light shadow ray
+    // the shade launch counted every shadow ray as a light shadow ray: move the sky
+    // rays over, one atomic pair per workgroup
     for (int off = 32; off >= 1; off >>= 1) sky_rays += __shfl_xor(sky_rays, off);
-    if (lane == 0 && sky_rays) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 2), (unsigned long long)(0ull - sky_rays));
-        atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 3), (unsigned long long)sky_rays);
+    __shared__ uint32_t s_sky[B / 64];
+    if (lane == 0) s_sky[w] = sky_rays;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int k = 0; k < B / 64; ++k) tot += s_sky[k];
+        if (tot) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 2), (unsigned long long)(0ull - tot));
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 3), (unsigned long long)tot);
+        }
     }
 }
 
