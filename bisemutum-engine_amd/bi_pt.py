@@ -323,7 +323,7 @@ class PathTracer:
     def reset(self):
         self._check(lib().bi_pt_reset_accumulation(self.h))
 
-    def set_build_options(self, treelet_rounds=2):
+    def set_build_options(self, treelet_rounds=3):
         """Rounds of treelet restructuring after the LBVH (0: the Morton/Karras tree)."""
         opt = (C.c_uint32 * 4)(treelet_rounds, 0, 0, 0)
         self._check(lib().bi_pt_set_build_options(self.h, C.byref(opt)))

@@ -14,7 +14,7 @@
 //   5. leaves     leaf intersection/shading records in sorted order, then the
 //                 bottom-up AABB refit (second arrival at each node continues)
 //   6. treelets   rounds of treelet restructuring of the refit tree (surface-area
-//                 cost; bi_pt_set_build_options, default 2 rounds)
+//                 cost; bi_pt_set_build_options, default 3 rounds)
 //   7. collapse   BVH4 traversal nodes
 // All steps are integer or exact min/max except the Morton quantisation and the
 // surface-area costs, whose float operations are written in a fixed order.

@@ -157,7 +157,7 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<float> tl_cost;
     bipt::DevBuf<int32_t> tl_height, tl_snap;
     bipt::DevBuf<uint32_t> tl_starts;
-    uint32_t treelet_rounds = 2;    // bi_pt_set_build_options
+    uint32_t treelet_rounds = 3;    // bi_pt_set_build_options
     bool bvh_built = false;
     // accumulation
     bipt::DevBuf<float> sums;

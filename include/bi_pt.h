@@ -198,7 +198,7 @@ bi_pt_status bi_pt_reset_accumulation(bi_pt_ctx* ctx);
  * topology by dynamic programming; Karras & Aila 2013).  Closest hits do not depend on
  * the tree, so images are the same for any setting; only the speed changes. */
 typedef struct bi_pt_build_options {
-    uint32_t treelet_rounds;   /* 0 = the Morton/Karras tree as built; default 2; at most 16 */
+    uint32_t treelet_rounds;   /* 0 = the Morton/Karras tree as built; default 3; at most 16 */
     uint32_t _reserved[3];
 } bi_pt_build_options;
 bi_pt_status bi_pt_set_build_options(bi_pt_ctx* ctx, const bi_pt_build_options* options);

@@ -46,7 +46,7 @@ struct Scene {
     std::vector<Node> nodes;
     std::vector<Node4> nodes4;
     std::vector<int32_t> parent;
-    uint32_t treelet_rounds = 2;   // restructuring rounds after the refit (as the GPU's default)
+    uint32_t treelet_rounds = 3;   // restructuring rounds after the refit (as the GPU's default)
     std::vector<float> leaf_v0, leaf_e1, leaf_e2;
     // FP64 oracle BVH (built lazily by ref64.cpp)
     mutable void* bvh64 = nullptr;
