@@ -60,6 +60,14 @@ void engine_free(Owner* own, Engine& eng) {
         for (auto& ev : eng.ev) (void)hipEventDestroy(ev);
         (void)hipEventDestroy(eng.plan_ev);
     }
+    if (eng.build_fork) {
+        (void)hipEventDestroy(eng.build_fork);
+        for (int k = 0; k < Engine::kBuildStreams; ++k) {
+            (void)hipEventDestroy(eng.build_join[k]);
+            (void)hipStreamDestroy(eng.build_s[k]);
+        }
+        eng.build_fork = nullptr;
+    }
     if (eng.plan_host) (void)hipHostFree(eng.plan_host);
     eng.plan_host = nullptr;
     eng.plan_host_n = 0;
@@ -380,6 +388,58 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     BI_HIP_TRY(own, hipEventRecord(eng.plan_ev, s));
     eng.plan_pending = true;
 
+    // streams for concurrent LBVH builds: made once, before any capture, by the first call
+    // that rebuilds two or more scenes
+    if (!eng.build_fork) {
+        uint32_t nbuild = 0;
+        for (uint32_t j = 0; j < n; ++j) nbuild += prep[j].rebuild ? 1u : 0u;
+        if (nbuild > 1) {
+            BI_HIP_TRY(own, hipEventCreateWithFlags(&eng.build_fork, hipEventDisableTiming));
+            for (int k = 0; k < Engine::kBuildStreams; ++k) {
+                BI_HIP_TRY(own, hipStreamCreateWithFlags(&eng.build_s[k], hipStreamNonBlocking));
+                BI_HIP_TRY(own, hipEventCreateWithFlags(&eng.build_join[k], hipEventDisableTiming));
+            }
+        }
+    }
+
+    // every call, outside any graph: the counters and the LBVH builds.  Queue lengths and
+    // error flag (the ray statistics accumulate) and the persistent lanes' part counters
+    // start at zero.  The scenes' LBVHs are independent: build b goes to s (b % lanes == 0)
+    // or to build stream b % lanes - 1, forked from s and joined back into s before
+    // anything reads them.
+    {
+        uint32_t* cnt = eng.counters.p;
+        launch_fill_u32(cnt, 0u, kCntRays, s);
+        launch_fill_u32(cnt + kCntTraceWork, 0u, kCntSlots - kCntTraceWork, s);
+        if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[0], s));
+        std::vector<uint32_t> builds;
+        for (uint32_t j = 0; j < n; ++j)
+            if (prep[j].rebuild) builds.push_back(j);
+        const uint32_t lanes = std::min<uint32_t>((uint32_t)builds.size(), 1u + Engine::kBuildStreams);
+        bi_pt_status berr = BI_PT_OK;
+        std::string bmsg;
+        if (lanes > 1) {
+            berr = hipEventRecord(eng.build_fork, s) == hipSuccess ? BI_PT_OK : BI_PT_ERROR_HIP;
+            for (uint32_t k = 1; k < lanes && berr == BI_PT_OK; ++k)
+                if (hipStreamWaitEvent(eng.build_s[k - 1], eng.build_fork, 0) != hipSuccess) berr = BI_PT_ERROR_HIP;
+            if (berr != BI_PT_OK) bmsg = "build stream fork failed";
+        }
+        for (size_t b = 0; b < builds.size() && berr == BI_PT_OK; ++b) {
+            const uint32_t k = (uint32_t)(b % (lanes ? lanes : 1u));
+            bi_pt_ctx* sc = prep[builds[b]].scene;
+            berr = build_bvh(sc, k == 0 ? s : eng.build_s[k - 1], &cnt[kCntError]);
+            if (berr != BI_PT_OK) bmsg = sc->err;
+        }
+        for (uint32_t k = 1; k < lanes; ++k) {   // join even after a failure
+            if (hipEventRecord(eng.build_join[k - 1], eng.build_s[k - 1]) != hipSuccess ||
+                hipStreamWaitEvent(s, eng.build_join[k - 1], 0) != hipSuccess) {
+                if (berr == BI_PT_OK) { berr = BI_PT_ERROR_HIP; bmsg = "build stream join failed"; }
+            }
+        }
+        if (berr != BI_PT_OK) return fail(own, berr, bmsg);
+        if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[1], s));
+    }
+
     // graph replay: the same launch sequence with the same buffers (the plan's contents,
     // uploaded above, carry everything that changes between calls)
     const bool graph = graph_owner && graph_owner->use_graph && !timing && !record;
@@ -406,27 +466,15 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     {
         uint32_t* cnt = eng.counters.p;
         uint64_t* rays = reinterpret_cast<uint64_t*>(cnt + kCntRays);
-        // per call: queue lengths and error flag (the ray statistics accumulate), the
-        // persistent lanes' part counters, and the images of jobs that start over
-        launch_fill_u32(cnt, 0u, kCntRays, s);
-        launch_fill_u32(cnt + kCntTraceWork, 0u, kCntSlots - kCntTraceWork, s);
+        // the images of jobs that start over
         for (uint32_t j = 0; j < n; ++j) {
             if (prep[j].reset) launch_fill_u32(reinterpret_cast<uint32_t*>(prep[j].sums), 0u, 3 * prep[j].npix, s);
             if (prep[j].role == kRoleTrain)   // training samples start unrecorded
                 launch_fill_u32(reinterpret_cast<uint32_t*>(nrc->train_rec.p), 0u, 16 * prep[j].npix, s);
         }
-        if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[0], s));
-        for (uint32_t j = 0; j < n; ++j) {
-            if (!prep[j].rebuild) continue;
-            if (bi_pt_status e = build_bvh(prep[j].scene, s, &cnt[kCntError]); e != BI_PT_OK) {
-                if (graph) { hipGraph_t g; (void)hipStreamEndCapture(s, &g); if (g) (void)hipGraphDestroy(g); }
-                return fail(own, e, prep[j].scene->err);
-            }
-        }
         if ((any_cached || any_train) && prep[0].scene->cb_out.p)   // the cache's position box
             BI_HIP_TRY(own, hipMemcpyAsync(nrc->bounds.p, prep[0].scene->cb_out.p, 6 * sizeof(float),
                                            hipMemcpyDeviceToDevice, s));
-        if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[1], s));
 
         PathState ps{eng.ps_f[0].p, eng.ps_f[1].p, eng.ps_f[2].p, eng.ps_f[3].p, eng.acc.p, eng.wave_cap};
         HitBuf hits{eng.hit_rec.p};

@@ -105,6 +105,11 @@ struct Engine {
     DevBuf<Bvh4Node> empty_tree;           // one node of empty slots (jobs without triangles)
     hipEvent_t ev[3] = {};
     hipEvent_t plan_ev = nullptr;
+    // LBVH builds of independent scenes run concurrently: forked from the call's stream
+    // onto these streams (default priority) and joined back by events
+    static constexpr int kBuildStreams = 3;
+    hipStream_t build_s[kBuildStreams] = {};
+    hipEvent_t build_fork = nullptr, build_join[kBuildStreams] = {};
     bool ready = false;
     // statistics
     bi_pt_stats stats{};
