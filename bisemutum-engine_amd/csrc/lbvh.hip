@@ -421,6 +421,8 @@ __device__ __forceinline__ float half_area(const float* b) {
 // child, else it counts itself in and the second arrival carries on, the first ends.  No
 // wavefront waits for another.  The refit publishes every subtree's cost and height for
 // the first round; a round leaves them current for the next.
+// [host-wave begin] From here to the end marker: the treelet code of one wavefront,
+// which tests/test_treelet_host.py also compiles for the CPU (64 threads as the lanes).
 constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletMaxHeight = 48;
 constexpr int kTreeletMinHeight = 5;   // lower subtrees are left as built (their share of the cost is small)
@@ -449,35 +451,6 @@ struct TreeletArgs {
     uint32_t start_cap;
     uint32_t* error;
 };
-
-__global__ __launch_bounds__(256) void k_tl_prep(TreeletArgs a) {
-    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
-    if (k >= a.ni) return;
-    const int h = a.height[k];
-    a.snap[k] = h;
-    uint32_t t = 0;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int ch = a.nodes[k].child[c];
-        t += (ch >= 0 && a.height[ch] >= kTreeletMinHeight) ? 1u : 0u;
-    }
-    a.arrivals[k] = t == 1u ? kTreeletSoleChild : 0u;
-    // a wavefront starts at every node of height kTreeletMinHeight..kTreeletSerialHeight
-    // whose parent stands higher (or that is the root): these subtrees are disjoint and
-    // hold at least 6 triangles each.  One atomic per wavefront.
-    const bool start = h >= kTreeletMinHeight && h <= kTreeletSerialHeight &&
-                       (k == 0 || a.height[a.parent[k]] > kTreeletSerialHeight);
-    const uint64_t m = __ballot(start);
-    if (!start) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t first = (uint32_t)__builtin_ctzll(m);
-    uint32_t base = 0;
-    if (lane == first) base = atomicAdd(a.starts, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)first);
-    const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (idx < a.start_cap) a.starts[1 + idx] = (uint32_t)k;
-    else atomicOr(a.error, 16u);
-}
 
 // One treelet per wavefront.  Every lane forms the same treelet (uniform values); the
 // dynamic programme runs over subset sizes, lane l owning subsets l + 1 and l + 65, each
@@ -814,6 +787,37 @@ __device__ void tl_process_wave(const TreeletArgs& a, int n, uint32_t lane, Tree
         a.cost[n] = cur;
         a.height[n] = curh;
     }
+}
+
+// [host-wave end]
+
+__global__ __launch_bounds__(256) void k_tl_prep(TreeletArgs a) {
+    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (k >= a.ni) return;
+    const int h = a.height[k];
+    a.snap[k] = h;
+    uint32_t t = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int ch = a.nodes[k].child[c];
+        t += (ch >= 0 && a.height[ch] >= kTreeletMinHeight) ? 1u : 0u;
+    }
+    a.arrivals[k] = t == 1u ? kTreeletSoleChild : 0u;
+    // a wavefront starts at every node of height kTreeletMinHeight..kTreeletSerialHeight
+    // whose parent stands higher (or that is the root): these subtrees are disjoint and
+    // hold at least 6 triangles each.  One atomic per wavefront.
+    const bool start = h >= kTreeletMinHeight && h <= kTreeletSerialHeight &&
+                       (k == 0 || a.height[a.parent[k]] > kTreeletSerialHeight);
+    const uint64_t m = __ballot(start);
+    if (!start) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t first = (uint32_t)__builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(a.starts, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)first);
+    const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (idx < a.start_cap) a.starts[1 + idx] = (uint32_t)k;
+    else atomicOr(a.error, 16u);
 }
 
 // Workgroup w (one wavefront) restructures the subtree of the w-th start node, then
