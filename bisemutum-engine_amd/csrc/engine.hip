@@ -500,10 +500,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
 
         for (size_t w = 0; w < waves.size(); ++w) {
             const WaveJobs& J = wj[w];
-            uint32_t D = 0;
+            uint32_t D = 0, max_nl = 0;
             bool wave_cached = false;
             for (const Item& it : waves[w]) {
                 D = std::max(D, prep[it.job].D);
+                max_nl = std::max(max_nl, prep[it.job].nl);
                 wave_cached |= prep[it.job].role == kRoleCached;
             }
             if (record) {
@@ -554,7 +555,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sa.bounce = b;
                 const bool cache_now = wave_cached && b == nrc->cache_bounce;
                 sa.nrc_query = cache_now ? nrc->query.p : nullptr;
-                launch_shade(sa, J.num_paths, s);
+                launch_shade(sa, J.num_paths, max_nl, s);
                 if (cache_now) {
                     // paths that ended in the cache at this segment: their radiance estimate
                     NrcInferArgs na{};

@@ -36,17 +36,21 @@ __device__ __forceinline__ uint64_t lb_word(uint32_t ext, uint32_t sh) { return 
 __device__ __forceinline__ uint32_t lb_ext(uint64_t w) { return (uint32_t)(w & 0x7fffffffull); }
 __device__ __forceinline__ uint32_t lb_sh(uint64_t w) { return (uint32_t)((w >> 31) & 0x7fffffffull); }
 
-// Outputs of one queue entry, held until the entry's place in the queues is known.
+// Outputs of one queue entry, held until the entry's place in the queues is known.  NL:
+// the most lights of any job in the launch (1, 2 or 4), so only that many contributions
+// take registers.
+template <uint32_t NL>
 struct ShadeOut {
     uint32_t mask = 0, path = 0;
     V3 P, ext, sky;
-    V3 c[kMaxDirLights + 1];   // shadow contribution of light l at [l], of the sky at [kMaxDirLights]
+    V3 c[NL + 1];   // shadow contribution of light l at [l], of the sky at [NL]
 };
 
 // Shades queue entry i (path `path`, the job's local path `local`) of job J: its rays
 // go to `out` (mask: which of them exist).
+template <uint32_t NL>
 __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J, uint32_t i, uint32_t path,
-                                            uint32_t local, bool primary, ShadeOut& out) {
+                                            uint32_t local, bool primary, ShadeOut<NL>& out) {
     const size_t cap = a.ps.cap;
     uint32_t mask = 0;
     const PathKey key = path_key(J, local);
@@ -136,7 +140,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
         Pcg rng = pcg_keyed(J.seed, pixel, sample, a.bounce);
         float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
 #pragma unroll
-        for (uint32_t l = 0; l < kMaxDirLights; ++l) {
+        for (uint32_t l = 0; l < NL; ++l) {
             if (l >= nl) break;
             const LightRec& L = J.lights[l];
             V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
@@ -168,7 +172,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
                     float w = power_heuristic(pl, pb);
                     V3 c = ((sky_eval(J.sky, wi) * f) * (w / pl)) * beta;
                     if (any_pos(c) && finite3(c)) {
-                        out.c[kMaxDirLights] = c;
+                        out.c[NL] = c;
                         out.sky = wi;
                         mask |= 2u << kMaxDirLights;
                     }
@@ -207,7 +211,8 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
 
 // Writes an entry's rays: the extension ray at queue place e, its shadow rays from
 // place o on in kind order (lights 0..3, then the sky).
-__device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, const ShadeOut& out, uint32_t e,
+template <uint32_t NL>
+__device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, const ShadeOut<NL>& out, uint32_t e,
                                            uint32_t o) {
     const uint32_t mask = out.mask;
     if (mask & kMaskExt) {
@@ -216,9 +221,9 @@ __device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, 
         a.qout.path[e] = out.path;
     }
 #pragma unroll
-    for (uint32_t k = 0; k <= kMaxDirLights; ++k) {
-        const bool sky = k == kMaxDirLights;
-        if (!(mask & (2u << k))) continue;
+    for (uint32_t k = 0; k <= NL; ++k) {
+        const bool sky = k == NL;
+        if (!(mask & (2u << (sky ? (uint32_t)kMaxDirLights : k)))) continue;
         a.sq.ox[o] = out.P.x; a.sq.oy[o] = out.P.y; a.sq.oz[o] = out.P.z;
         if (sky) {
             a.sq.dx[o] = out.sky.x; a.sq.dy[o] = out.sky.y; a.sq.dz[o] = out.sky.z;
@@ -264,7 +269,8 @@ __device__ __forceinline__ uint64_t look_back(const uint64_t* state, uint32_t t,
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
+template <uint32_t NL>
+__device__ __forceinline__ void shade_tile(const ShadeArgs& a) {
     __shared__ uint32_t s_wsum[4];
     __shared__ uint64_t s_base;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -280,7 +286,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const uint32_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     const uint32_t i = tile * kTile + tid;
-    ShadeOut out;
+    ShadeOut<NL> out;
     const JobRec* J = a.jobs.rec;
     if (i < count) {
         const uint32_t path = primary ? i : a.qin.path[i];
@@ -335,6 +341,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         emit_entry(a, *J, out, lb_ext(base) + (excl >> 16), lb_sh(base) + (excl & 0xffffu));
     }
 }
+
+// One kernel per light bound; the one-light form (the common case) fits 80 registers,
+// 6 wavefronts per SIMD, without spilling.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_shade_l1(ShadeArgs a) {
+    shade_tile<1>(a);
+}
+__global__ __launch_bounds__(256) void k_shade_l2(ShadeArgs a) { shade_tile<2>(a); }
+__global__ __launch_bounds__(256) void k_shade_l4(ShadeArgs a) { shade_tile<kMaxDirLights>(a); }
 
 // Output counts and look-back words of the next shade launch.
 __global__ __launch_bounds__(256) void k_shade_reset(uint64_t* state, const uint32_t* count_in, uint32_t num_entries,
@@ -413,9 +427,12 @@ void launch_shade_reset(uint64_t* tile_state, const uint32_t* count_in, uint32_t
                        num_entries, count_ext, count_shadow);
 }
 
-void launch_shade(const ShadeArgs& a, uint32_t capacity, hipStream_t s) {
+void launch_shade(const ShadeArgs& a, uint32_t capacity, uint32_t max_lights, hipStream_t s) {
     const uint32_t tiles = (capacity + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_shade, dim3(tiles ? tiles : 1), dim3(256), 0, s, a);
+    const dim3 grid(tiles ? tiles : 1), block(256);
+    if (max_lights <= 1) hipLaunchKernelGGL(k_shade_l1, grid, block, 0, s, a);
+    else if (max_lights <= 2) hipLaunchKernelGGL(k_shade_l2, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_shade_l4, grid, block, 0, s, a);
 }
 
 void launch_resolve(const ResolveArgs& a, hipStream_t s) {

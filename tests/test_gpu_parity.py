@@ -160,6 +160,28 @@ def test_radiance_c0_matches_oracle(c0_gpu, c0_oracle):
     assert np.all(img[:, 3] == 1.0)
 
 
+@pytest.mark.parametrize("nl", [2, 4])
+def test_radiance_c0_several_lights(oracle_lib, c0_scene, nl):
+    """Shading keeps only as many light contributions in registers as the launch's jobs
+    have lights (1, 2 or 4): two and four lights, with and without shadows and with a
+    partial shadow strength, against the restatement."""
+    print(f"case: c0 radiance 96x72 2spp, {nl} lights", flush=True)
+    import dataclasses
+    lights = [scenes.dir_light((2.6, 2.5, 2.3), (0.35, 0.85, 0.4)),
+              scenes.dir_light((0.8, 0.9, 1.2), (-0.6, 0.5, 0.2), cast_shadow=False),
+              scenes.dir_light((1.5, 0.4, 0.3), (0.1, 0.7, -0.7), shadow_strength=0.5),
+              scenes.dir_light((0.3, 1.1, 0.5), (-0.2, 0.9, -0.4))][:nl]
+    sc = dataclasses.replace(c0_scene, dir_lights=np.stack(lights))
+    cam = scenes.camera_c0(96, 72)
+    g = bi_pt.PathTracer()
+    g.set_scene(sc)
+    g.render(cam, spp=2, max_bounces=5, seed=0x5EED0042)
+    gs = g.sums(cam.width * cam.height)
+    os_ = oracle_lib.OracleScene(sc).render32(cam, 5, 0x5EED0042, spp=2)["sums"]
+    assert radiance_check(gs, os_)
+    g.close()
+
+
 def expected_queues(segments, bits, D, nl=1):
     n = len(segments)
     ext, sh = [], []
