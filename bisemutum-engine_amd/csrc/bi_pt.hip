@@ -83,7 +83,8 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
 
 // Enqueues the LBVH build of the context's scene on s (buffers must exist: see
 // ensure_build_buffers); error flags of the refit go to `error`.
-bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error) {
+bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error, uint32_t rounds) {
+    ctx->built_rounds = rounds;
     if (ctx->num_tris == 0) return BI_PT_OK;
     BuildArgs a{};
     a.positions = ctx->pos.p;
@@ -109,7 +110,7 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error) {
     a.leaf_geo = ctx->leaf_geo.p;
     a.leaf_shade = ctx->leaf_shade.p;
     a.error = error;
-    a.treelet_rounds = ctx->treelet_rounds;
+    a.treelet_rounds = rounds;
     a.tl_cost = ctx->tl_cost.p;
     a.tl_height = ctx->tl_height.p;
     a.tl_snap = ctx->tl_snap.p;
@@ -127,7 +128,7 @@ namespace {
 bi_pt_status build_bvh_now(bi_pt_ctx* ctx) {
     BI_TRY(engine_init(ctx, ctx->eng));
     BI_TRY(ensure_build_buffers(ctx));
-    BI_TRY(build_bvh(ctx, nullptr, &ctx->eng.counters.p[kCntError]));
+    BI_TRY(build_bvh(ctx, nullptr, &ctx->eng.counters.p[kCntError], ctx->treelet_rounds));
     ctx->bvh_built = true;
     return BI_PT_OK;
 }
@@ -405,7 +406,7 @@ bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* ord
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (!ctx->bvh_built) {
+    if (!ctx->bvh_built || ctx->built_rounds != ctx->treelet_rounds) {   // the tree as configured
         BI_TRY(build_bvh_now(ctx));
     }
     BI_HIP_TRY(ctx, hipDeviceSynchronize());
@@ -437,7 +438,7 @@ bi_pt_status bi_pt_debug_get_bvh4(bi_pt_ctx* ctx, int32_t* child4, float* box4) 
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (!ctx->bvh_built) BI_TRY(build_bvh_now(ctx));
+    if (!ctx->bvh_built || ctx->built_rounds != ctx->treelet_rounds) BI_TRY(build_bvh_now(ctx));
     BI_HIP_TRY(ctx, hipDeviceSynchronize());
     const uint32_t n = ctx->num_tris;
     if (n == 0) return BI_PT_OK;

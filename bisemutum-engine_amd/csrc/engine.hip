@@ -142,6 +142,7 @@ struct Prep {
     bi_pt_ctx* scene;     // scene, lights, sky, BVH
     CameraParams cam;
     uint32_t npix, M, D, nl, spp, sh, role;
+    uint32_t rounds;      // treelet rounds if this job's call builds its scene
     bool sky_nee, reset, rebuild;
     float* sums;          // running sums (training paths: the cache's targets)
 };
@@ -278,8 +279,25 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
             if ((e = dalloc(ctx, ctx->sums, (size_t)3 * prep[j].npix)) != BI_PT_OK) return fail(own, e, ctx->err);
             prep[j].sums = ctx->sums.p;
         }
-        if (prep[j].rebuild && (e = ensure_build_buffers(prep[j].scene)) != BI_PT_OK)
-            return fail(own, e, prep[j].scene->err);
+    }
+    // treelet rounds of each scene's build: the configured number when the call traces at
+    // least max(4 n, 2^20) paths of the scene's n triangles, else none (a round costs
+    // ~0.1-0.5 ms and saves ~5% of the trace time, ~0.1 ns per path); a kept BVH with
+    // fewer rounds than the call wants is rebuilt
+    for (uint32_t j = 0; j < n; ++j) {
+        Prep& p = prep[j];
+        bool first = true;
+        for (uint32_t k = 0; k < j; ++k) first = first && prep[k].scene != p.scene;
+        p.rounds = 0;
+        if (!first) continue;
+        uint64_t paths = 0;
+        for (uint32_t k = j; k < n; ++k)
+            if (prep[k].scene == p.scene) paths += (uint64_t)prep[k].npix * prep[k].spp;
+        const uint64_t need = std::max<uint64_t>(4ull * p.scene->num_tris, 1ull << 20);
+        p.rounds = paths >= need ? p.scene->treelet_rounds : 0u;
+        if (!p.rebuild && p.scene->built_rounds < p.rounds) p.rebuild = true;
+        if (p.rebuild)
+            if (bi_pt_status e = ensure_build_buffers(p.scene); e != BI_PT_OK) return fail(own, e, p.scene->err);
     }
     BI_TRY(ensure_wave_buffers(own, eng, wave_max, slots, sh_mult));
     if (any_cached) BI_TRY(nrc_ensure_query(nrc, eng.wave_cap));
@@ -427,7 +445,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         for (size_t b = 0; b < builds.size() && berr == BI_PT_OK; ++b) {
             const uint32_t k = (uint32_t)(b % (lanes ? lanes : 1u));
             bi_pt_ctx* sc = prep[builds[b]].scene;
-            berr = build_bvh(sc, k == 0 ? s : eng.build_s[k - 1], &cnt[kCntError]);
+            berr = build_bvh(sc, k == 0 ? s : eng.build_s[k - 1], &cnt[kCntError], prep[builds[b]].rounds);
             if (berr != BI_PT_OK) bmsg = sc->err;
         }
         for (uint32_t k = 1; k < lanes; ++k) {   // join even after a failure

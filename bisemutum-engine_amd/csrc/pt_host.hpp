@@ -164,6 +164,7 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<uint32_t> tl_starts;
     uint32_t treelet_rounds = 3;    // bi_pt_set_build_options
     bool bvh_built = false;
+    uint32_t built_rounds = 0;      // treelet rounds of the BVH built last
     // accumulation
     bipt::DevBuf<float> sums;
     uint32_t acc_w = 0, acc_h = 0;
@@ -192,7 +193,9 @@ namespace bipt {
 inline bi_pt_ctx* scene_of(bi_pt_ctx* c) { return c->scene_src ? c->scene_src : c; }
 
 bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx);
-bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error);
+// rounds: treelet rounds of this build (the context's setting, or fewer for a call that
+// traces too few paths to repay them)
+bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error, uint32_t rounds);
 BvhView bvh_view(const bi_pt_ctx* ctx);
 
 // One job of a call.  Roles: a plain image; an image whose paths end in the call's

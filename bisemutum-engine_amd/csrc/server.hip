@@ -125,7 +125,8 @@ bi_pt_status bi_pt_server_add_scene(bi_pt_server* s, bi_pt_ctx* scene, uint32_t*
     if (!scene->bvh_built) {
         BI_TRY(engine_init(scene, scene->eng));
         if (bi_pt_status e = ensure_build_buffers(scene); e != BI_PT_OK) return fail(s, e, scene->err);
-        if (bi_pt_status e = build_bvh(scene, nullptr, &scene->eng.counters.p[kCntError]); e != BI_PT_OK)
+        if (bi_pt_status e = build_bvh(scene, nullptr, &scene->eng.counters.p[kCntError], scene->treelet_rounds);
+            e != BI_PT_OK)
             return fail(s, e, scene->err);
         BI_HIP_TRY(s, hipDeviceSynchronize());
         scene->bvh_built = true;

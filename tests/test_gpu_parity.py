@@ -75,6 +75,20 @@ def test_bvh_bit_exact_c0_rounds(oracle_lib, c0_scene, rounds):
     d.close()
 
 
+def test_small_render_builds_plain_tree_inspection_shows_configured(oracle_lib, c0_scene):
+    """A call tracing fewer than max(4 n, 2^20) paths of a scene builds its LBVH without
+    treelet rounds; the inspection hooks rebuild the configured tree."""
+    print("case: c0 small render, then the BVH as configured", flush=True)
+    g = bi_pt.PathTracer()
+    g.set_scene(c0_scene)
+    cam = scenes.camera_c0(48, 32)   # 1,536 paths << 2^20
+    g.render(cam, spp=1, max_bounces=5, seed=5)
+    o = oracle_lib.OracleScene(c0_scene)
+    compare_bvh(g.bvh(), o.bvh())
+    compare_bvh4(g, o)
+    g.close()
+
+
 @pytest.mark.parametrize("n,seed", [(12, 1), (20, 2), (40, 3), (64, 4), (100, 5), (200, 6)])
 def test_treelets_small_trees_bit_exact(oracle_lib, n, seed):
     """Trees of 12-200 triangles: the root's treelet (and most others) has 6 or 7 leaves,
