@@ -415,12 +415,13 @@ __device__ __forceinline__ float half_area(const float* b) {
 // round are restructured, children first: restructuring a node changes nothing outside
 // its subtree and subtrees of nodes where neither contains the other are disjoint, so
 // every children-first order gives the same tree -- the CPU restatement's recursion.
-// One launch per round schedules them bottom-up like the refit: a wavefront starts at
-// each node of exactly that height (its subtree holds no other such node), restructures
-// it, then moves to the parent -- directly if it is the parent's only restructured
-// child, else it counts itself in and the second arrival carries on, the first ends.  No
-// wavefront waits for another.  The refit publishes every subtree's cost and height for
-// the first round; a round leaves them current for the next.
+// One launch per round schedules them bottom-up like the refit: a wavefront takes each
+// node of height kTreeletMinHeight..kTreeletSerialHeight whose parent stands higher,
+// restructures that node's subtree alone (children first), then climbs to the parent --
+// directly if it is the parent's only restructured child, else it counts itself in and
+// the second arrival carries on, the first ends.  No wavefront waits for another.  The
+// refit publishes every subtree's cost and height for the first round; a round leaves
+// them current for the next.
 // [host-wave begin] From here to the end marker: the treelet code of one wavefront,
 // which tests/test_treelet_host.py also compiles for the CPU (64 threads as the lanes).
 constexpr int kTreeletLeaves = 7;
@@ -452,11 +453,12 @@ struct TreeletArgs {
     uint32_t* error;
 };
 
-// One treelet per wavefront.  Every lane forms the same treelet (uniform values); the
-// dynamic programme runs over subset sizes, lane l owning subsets l + 1 and l + 65, each
-// subset's cheapest split found by its owning lane in the enumeration order above, the
-// table in LDS; lane 0 rebuilds an accepted treelet.  Register arrays are indexed with
-// compile-time indices only (selects), so nothing goes to scratch memory.
+// One treelet per wavefront.  Every lane forms the same treelet (uniform values, leaf
+// boxes in registers with compile-time indices, leaf and internal node indices in LDS);
+// lane l computes the unions of subsets l + 1 and l + 65; the dynamic programme runs
+// over subset sizes (one lane per subset of up to 5 leaves, 8 lanes per subset of 6, the
+// wavefront for the full set), the table in LDS; an accepted treelet is rebuilt with
+// every link written by its own lane.
 struct TreeletLds {
     float box[1 << kTreeletLeaves][6];   // union of every subset's boxes
     float area[1 << kTreeletLeaves];     // its half surface
