@@ -174,6 +174,23 @@ def test_radiance_c0_matches_oracle(c0_gpu, c0_oracle):
     assert np.all(img[:, 3] == 1.0)
 
 
+def test_radiance_all_camera_rays_miss(oracle_lib):
+    """Every camera ray escapes: the segments after the first have no entries (zero shading
+    tiles, zero queue lengths); the image is the sky, as in the restatement."""
+    print("case: random triangles, camera facing away, 40x24 2spp", flush=True)
+    sc = scenes.scene_random_triangles(50, 3)
+    cam = scenes.make_camera((0.0, 0.0, 20.0), (0.0, 0.0, 40.0), 40, 24, 40.0)
+    g = bi_pt.PathTracer()
+    g.set_scene(sc)
+    g.render(cam, spp=2, max_bounces=5, seed=0x5EED0077)
+    gs = g.sums(cam.width * cam.height)
+    os_ = oracle_lib.OracleScene(sc).render32(cam, 5, 0x5EED0077, spp=2)["sums"]
+    assert radiance_check(gs, os_)
+    st = g.stats()
+    assert st["rays_extension"] == 0 and st["rays_light_shadow"] == 0 and st["rays_sky_shadow"] == 0
+    g.close()
+
+
 @pytest.mark.parametrize("nl", [2, 4])
 def test_radiance_c0_several_lights(oracle_lib, c0_scene, nl):
     """Shading keeps only as many light contributions in registers as the launch's jobs
