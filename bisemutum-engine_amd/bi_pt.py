@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("BI_PT_LIBRARY") or os.path.join(HERE, "libbi_pt.so")
 
 FLAG_JITTER = 1 << 0
 FLAG_NO_SKY_NEE = 1 << 1
+FLAG_OUTPUT_RGBA16F = 1 << 2
 FLAG_RECORD_QUEUES = 1 << 8
 FLAG_TIMING = 1 << 9
 FLAG_REUSE_BVH = 1 << 10
@@ -29,7 +30,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
            "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
-           "bi_pt_debug_trace", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
+           "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
            "bi_pt_batch_autotune", "bi_pt_create_view", "bi_pt_server_create", "bi_pt_server_destroy",
            "bi_pt_server_last_error", "bi_pt_server_clock", "bi_pt_server_add_scene", "bi_pt_server_submit",
@@ -194,12 +195,20 @@ def lib():
         L.bi_pt_nrc_train.argtypes = [vp, vp, vp, u32, vp]
         L.bi_pt_nrc_debug_gradient.argtypes = [vp, f32p, u32]
         L.bi_pt_nrc_debug_samples.argtypes = [vp, f32p, f32p, u32]
+        L.bi_pt_trace_closest.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
+        L.bi_pt_trace_occlusion.argtypes = [vp, u32, vp, C.c_float, vp, vp]
         for name in EXPORTS:
             if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris", "bi_pt_batch_last_error",
                             "bi_pt_server_last_error", "bi_pt_server_clock", "bi_pt_nrc_last_error"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
+
+
+class Rays(C.Structure):
+    """bi_pt_rays: six device arrays (origin xyz, direction xyz)."""
+    _fields_ = [("ox", C.c_void_p), ("oy", C.c_void_p), ("oz", C.c_void_p),
+                ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p)]
 
 
 def _p(a, t=C.c_float):
@@ -298,8 +307,9 @@ class PathTracer:
         torch = self.torch
         npix = cam.width * cam.height
         dev = torch.device("cuda", self.device)
-        if self._out is None or self._out.numel() != npix * 4:
-            self._out = torch.empty(npix * 4, dtype=torch.float32, device=dev)
+        dtype = torch.float16 if flags & FLAG_OUTPUT_RGBA16F else torch.float32
+        if self._out is None or self._out.numel() != npix * 4 or self._out.dtype != dtype:
+            self._out = torch.empty(npix * 4, dtype=dtype, device=dev)
         dptr = None
         if depth:
             if self._depth is None or self._depth.numel() != npix:
@@ -371,6 +381,44 @@ class PathTracer:
         self._check(lib().bi_pt_debug_get_sums(self.h, _p(out)))
         return out
 
+    def _query_rays(self, origins, dirs):
+        torch = self.torch
+        dev = torch.device("cuda", self.device)
+        o = torch.as_tensor(np.ascontiguousarray(origins, np.float32)).to(dev)
+        d = torch.as_tensor(np.ascontiguousarray(dirs, np.float32)).to(dev)
+        cols = [o[:, k].contiguous() for k in range(3)] + [d[:, k].contiguous() for k in range(3)]
+        r = Rays(*[C.c_void_p(c.data_ptr()) for c in cols])
+        return cols, r, dev
+
+    def trace_closest(self, origins, dirs, tmax=1000.0, stream=None):
+        """Closest hits of n rays (origins, dirs: (n, 3)) through bi_pt_trace_closest on
+        device arrays: (t, triangle, u, v) as numpy arrays (t = inf, triangle = 0xffffffff
+        on a miss)."""
+        torch = self.torch
+        cols, r, dev = self._query_rays(origins, dirs)
+        n = cols[0].numel()
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        tri = torch.empty(n, dtype=torch.int32, device=dev)
+        u = torch.empty(n, dtype=torch.float32, device=dev)
+        v = torch.empty(n, dtype=torch.float32, device=dev)
+        s = C.c_void_p(stream) if stream else None
+        self._check(lib().bi_pt_trace_closest(self.h, n, C.byref(r), tmax, C.c_void_p(t.data_ptr()),
+                                              C.c_void_p(tri.data_ptr()), C.c_void_p(u.data_ptr()),
+                                              C.c_void_p(v.data_ptr()), s))
+        torch.cuda.synchronize()
+        return t.cpu().numpy(), tri.cpu().numpy().view(np.uint32), u.cpu().numpy(), v.cpu().numpy()
+
+    def trace_occlusion(self, origins, dirs, tmax=1000.0, stream=None):
+        """1 where a ray hits any triangle in [0.001, tmax], else 0 (bi_pt_trace_occlusion)."""
+        torch = self.torch
+        cols, r, dev = self._query_rays(origins, dirs)
+        n = cols[0].numel()
+        occ = torch.empty(n, dtype=torch.int32, device=dev)
+        s = C.c_void_p(stream) if stream else None
+        self._check(lib().bi_pt_trace_occlusion(self.h, n, C.byref(r), tmax, C.c_void_p(occ.data_ptr()), s))
+        torch.cuda.synchronize()
+        return occ.cpu().numpy().view(np.uint32)
+
     def trace(self, rays, any_hit=False):
         rays = np.ascontiguousarray(rays, np.float32)
         n = len(rays)
@@ -399,8 +447,9 @@ class Job:
         torch = t.torch
         npix = self.cam.width * self.cam.height
         dev = torch.device("cuda", t.device)
-        if t._out is None or t._out.numel() != npix * 4:
-            t._out = torch.empty(npix * 4, dtype=torch.float32, device=dev)
+        dtype = torch.float16 if self.settings.flags & FLAG_OUTPUT_RGBA16F else torch.float32
+        if t._out is None or t._out.numel() != npix * 4 or t._out.dtype != dtype:
+            t._out = torch.empty(npix * 4, dtype=dtype, device=dev)
         if self.depth and (t._depth is None or t._depth.numel() != npix):
             t._depth = torch.empty(npix, dtype=torch.float32, device=dev)
         return t._out, (t._depth if self.depth else None)

@@ -135,6 +135,56 @@ bi_pt_status build_bvh_now(bi_pt_ctx* ctx) {
 
 }  // namespace
 
+namespace {
+
+// Ray queries: the job record of the context's scene and the path bases {0, n} (one job),
+// written by a kernel so the record travels as a launch argument.
+__global__ void k_put_query_job(bipt::JobRec rec, uint32_t n, unsigned char* dst) {
+    if (threadIdx.x != 0) return;
+    *reinterpret_cast<bipt::JobRec*>(dst) = rec;
+    uint32_t* bases = reinterpret_cast<uint32_t*>(dst + sizeof(bipt::JobRec));
+    bases[0] = 0u;
+    bases[1] = n;
+    bases[2] = 0u;
+    bases[3] = 0u;
+}
+
+// Closest-hit records -> caller's arrays: t (+inf on a miss), the triangle's index in the
+// scene's instance order (leaf -> sorted order), barycentrics.
+__global__ __launch_bounds__(256) void k_query_hits(const bipt::HitRec* __restrict__ hits,
+                                                    const uint32_t* __restrict__ order, uint32_t n, float* t,
+                                                    uint32_t* tri, float* u, float* v) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const bipt::HitRec h = hits[i];
+    const bool hit = h.leaf != bipt::kLeafMiss;
+    if (t) t[i] = hit ? h.t : __builtin_inff();
+    tri[i] = hit ? order[h.leaf] : bipt::kLeafMiss;
+    if (u) u[i] = hit ? h.u : 0.0f;
+    if (v) v[i] = hit ? h.v : 0.0f;
+}
+
+// Checks a query and makes sure the context's BVH is current (built on s).
+bi_pt_status query_prepare(bi_pt_ctx*& ctx, uint32_t n, const bi_pt_rays* rays, float tmax, hipStream_t s) {
+    if (ctx && ctx->scene_src) ctx = ctx->scene_src;   // a view traces its scene
+    if (!ctx || (n && !rays)) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (n && (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null ray array");
+    if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
+    if (!(tmax > kTMin) || !(tmax <= kShadowTMax))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "ray queries need 0.001 < tmax <= 1.7e38");
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    BI_TRY(engine_init(ctx, ctx->eng));
+    if (!ctx->bvh_built) {
+        BI_TRY(ensure_build_buffers(ctx));
+        BI_TRY(build_bvh(ctx, s, &ctx->eng.counters.p[kCntError], ctx->treelet_rounds));
+        ctx->bvh_built = true;
+    }
+    return BI_PT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 bi_pt_status bi_pt_create(const bi_pt_device_desc* desc, bi_pt_ctx** out_ctx) {
@@ -173,6 +223,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes);
     dfree(ctx, ctx->nodes4); dfree(ctx, ctx->parent); dfree(ctx, ctx->leaf_geo); dfree(ctx, ctx->leaf_shade);
     dfree(ctx, ctx->tl_cost); dfree(ctx, ctx->tl_height); dfree(ctx, ctx->tl_snap); dfree(ctx, ctx->tl_starts);
+    dfree(ctx, ctx->rq_job); dfree(ctx, ctx->rq_cnt); dfree(ctx, ctx->rq_path); dfree(ctx, ctx->rq_hits);
     dfree(ctx, ctx->sums);
     engine_free(ctx, ctx->eng);
     delete ctx;
@@ -562,6 +613,63 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
     dfree(ctx, cnt);
     dfree(ctx, out);
     if (status != BI_PT_OK) return fail(ctx, status, "debug trace failed");
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_trace_closest(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax, float* hit_t,
+                                 uint32_t* hit_triangle, float* hit_u, float* hit_v, bi_pt_stream stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BI_TRY(query_prepare(ctx, n, rays, tmax, s));
+    if (n == 0) return BI_PT_OK;
+    if (!hit_triangle) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null hit_triangle");
+    BI_TRY(dalloc(ctx, ctx->rq_job, sizeof(JobRec) + 4 * sizeof(uint32_t)));
+    BI_TRY(dalloc(ctx, ctx->rq_cnt, 4 + kWorkParts * kWorkStride));
+    BI_TRY(dalloc(ctx, ctx->rq_path, n));
+    BI_TRY(dalloc(ctx, ctx->rq_hits, n));
+    JobRec rec{};
+    rec.nodes = ctx->num_tris ? ctx->nodes4.p : ctx->eng.empty_tree.p;
+    rec.leaf_geo = ctx->leaf_geo.p;
+    rec.num_tris = ctx->num_tris;
+    rec.ray_length = tmax;
+    hipLaunchKernelGGL(k_put_query_job, dim3(1), dim3(64), 0, s, rec, n, ctx->rq_job.p);
+    launch_fill_u32(ctx->rq_cnt.p, 0u, 4 + kWorkParts * kWorkStride, s);   // count, then zeroed part counters
+    launch_fill_u32(ctx->rq_cnt.p, n, 1, s);
+    launch_fill_u32(ctx->rq_path.p, 0u, n, s);
+    TraceArgs ta{};
+    ta.jobs = WaveJobs{reinterpret_cast<const JobRec*>(ctx->rq_job.p),
+                       reinterpret_cast<const uint32_t*>(ctx->rq_job.p + sizeof(JobRec)),
+                       reinterpret_cast<const uint32_t*>(ctx->rq_job.p + sizeof(JobRec)) + 2, 1u, n};
+    ta.q = RayQueue{const_cast<float*>(rays->ox), const_cast<float*>(rays->oy), const_cast<float*>(rays->oz),
+                    const_cast<float*>(rays->dx), const_cast<float*>(rays->dy), const_cast<float*>(rays->dz),
+                    ctx->rq_path.p};
+    ta.count_in = ctx->rq_cnt.p;
+    ta.work = ctx->rq_cnt.p + 4;
+    ta.parts = kWorkParts;
+    ta.refill = 32;
+    ta.hits = HitBuf{ctx->rq_hits.p};
+    ta.error = &ctx->eng.counters.p[kCntError];
+    launch_trace_closest(ta, n, false, 16, 256, s);
+    hipLaunchKernelGGL(k_query_hits, dim3((n + 255u) / 256u), dim3(256), 0, s, ctx->rq_hits.p,
+                       ctx->num_tris ? ctx->vals0.p : ctx->rq_path.p, n, hit_t, hit_triangle, hit_u, hit_v);
+    BI_HIP_TRY(ctx, hipGetLastError());
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_trace_occlusion(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax,
+                                   uint32_t* occluded, bi_pt_stream stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BI_TRY(query_prepare(ctx, n, rays, tmax, s));
+    if (n == 0) return BI_PT_OK;
+    if (!occluded) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null occluded");
+    if (ctx->num_tris == 0) {
+        launch_fill_u32(occluded, 0u, n, s);
+        BI_HIP_TRY(ctx, hipGetLastError());
+        return BI_PT_OK;
+    }
+    RayQueue q{const_cast<float*>(rays->ox), const_cast<float*>(rays->oy), const_cast<float*>(rays->oz),
+               const_cast<float*>(rays->dx), const_cast<float*>(rays->dy), const_cast<float*>(rays->dz), nullptr};
+    launch_debug_anyhit(bvh_view(ctx), q, n, tmax, occluded, &ctx->eng.counters.p[kCntError], s);
+    BI_HIP_TRY(ctx, hipGetLastError());
     return BI_PT_OK;
 }
 

@@ -346,7 +346,8 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         r.samples = samples;
         r.seed = jobs[j].st->seed;
         r.flags = (jobs[j].st->flags & 3u) | (p.role == kRoleCached ? kJobCache : 0u) |
-                  (p.role == kRoleTrain ? kJobTrain : 0u);
+                  (p.role == kRoleTrain ? kJobTrain : 0u) |
+                  ((jobs[j].st->flags & BI_PT_FLAG_OUTPUT_RGBA16F) ? kJobHalfOut : 0u);
         r.cache_bounce = p.role != kRoleImage ? nrc->cache_bounce : 0xffffffffu;
         r.cache_rec = p.role == kRoleTrain ? nrc->train_rec.p : nullptr;
         r.ray_length = jobs[j].st->ray_length;

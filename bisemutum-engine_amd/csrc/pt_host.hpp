@@ -165,6 +165,11 @@ struct bi_pt_ctx : bipt::Owner {
     uint32_t treelet_rounds = 3;    // bi_pt_set_build_options
     bool bvh_built = false;
     uint32_t built_rounds = 0;      // treelet rounds of the BVH built last
+    // ray queries (bi_pt_trace_closest / bi_pt_trace_occlusion): job record + bases,
+    // queue-part counters, path ids, hit records
+    bipt::DevBuf<unsigned char> rq_job;
+    bipt::DevBuf<uint32_t> rq_cnt, rq_path;
+    bipt::DevBuf<bipt::HitRec> rq_hits;
     // accumulation
     bipt::DevBuf<float> sums;
     uint32_t acc_w = 0, acc_h = 0;

@@ -117,6 +117,7 @@ struct BvhView {
 
 // Job kinds beside plain images (JobRec::flags).
 constexpr uint32_t kJobCache = 1u << 2;   // image paths end in the radiance cache at cache_bounce
+constexpr uint32_t kJobHalfOut = 1u << 4;   // the colour output is RGBA16F
 constexpr uint32_t kJobTrain = 1u << 3;   // the cache's training paths: one per slot, random pixels,
                                           // radiance after cache_bounce goes to the slot's target
 constexpr uint32_t kNoPath = 0xffffffffu;
@@ -157,7 +158,7 @@ struct JobRec {
     uint32_t sample0;       // sample index of the job's first sample in this wave
     uint32_t samples;       // samples of the job in this wave
     uint32_t seed;
-    uint32_t flags;         // bit 0: pixel jitter; kJobCache, kJobTrain
+    uint32_t flags;         // bit 0: pixel jitter; kJobCache, kJobTrain, kJobHalfOut
     uint32_t cache_bounce;  // segment at which the radiance cache takes over (kJobCache) or
                             // at which a training path records its sample (kJobTrain)
     NrcRecord* cache_rec;   // kJobTrain: one record per slot
@@ -166,7 +167,7 @@ struct JobRec {
     float* sums;            // running per-pixel sums (3 floats per pixel)
     float* depth_out;       // nullable: reverse-Z depth of the primary hit
     uint32_t depth_sample;  // the sample whose primary hit gives the depth
-    float* out_rgba;        // output mean (RGBA32F)
+    float* out_rgba;        // output mean (RGBA32F, or RGBA16F with kJobHalfOut)
     float out_den;          // samples accumulated after this call
 };
 

@@ -410,7 +410,14 @@ __global__ __launch_bounds__(256) void k_output(WaveJobs jobs, uint32_t num_pix)
     o.y = sums[3 * (size_t)p + 1] / den;
     o.z = sums[3 * (size_t)p + 2] / den;
     o.w = 1.0f;
-    reinterpret_cast<float4*>(J.out_rgba)[p] = o;
+    if (J.flags & kJobHalfOut) {   // RGBA16F, round to nearest even (v_cvt_f16_f32)
+        const _Float16 h[4] = {(_Float16)o.x, (_Float16)o.y, (_Float16)o.z, (_Float16)o.w};
+        ushort4 u;
+        __builtin_memcpy(&u, h, sizeof u);
+        reinterpret_cast<ushort4*>(J.out_rgba)[p] = u;
+    } else {
+        reinterpret_cast<float4*>(J.out_rgba)[p] = o;
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {

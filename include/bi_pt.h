@@ -116,6 +116,9 @@ typedef struct bi_pt_camera {
 enum {
     BI_PT_FLAG_JITTER = 1u << 0,         /* jitter primary rays inside the pixel (reference: pixel centre) */
     BI_PT_FLAG_NO_SKY_NEE = 1u << 1,     /* sky only on BSDF-sampled misses, as the reference */
+    BI_PT_FLAG_OUTPUT_RGBA16F = 1u << 2, /* the colour output is RGBA16F (4 IEEE halves per pixel, rounded
+                                          * to nearest even), the reference's OutputData.color format
+                                          * (path_tracing.hpp:19-23); else RGBA32F */
     BI_PT_FLAG_RECORD_QUEUES = 1u << 8,  /* keep every bounce's queue contents for bi_pt_debug_queue */
     BI_PT_FLAG_TIMING = 1u << 9,         /* time each kernel class with HIP events (bi_pt_get_stats) */
     BI_PT_FLAG_REUSE_BVH = 1u << 10      /* keep the built LBVH while the scene and transforms are unchanged */
@@ -185,7 +188,8 @@ bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba32f, uint32_t width,
 
 /* ---- the pass (PathTracingPass::render, path_tracing.cpp:224-488) ----
  * Rebuilds the LBVH, traces settings->spp samples per pixel and writes the
- * running mean to out_rgba32f (device, width*height*4 floats).  out_depth
+ * running mean to out_rgba32f (device, width*height*4 floats; with
+ * BI_PT_FLAG_OUTPUT_RGBA16F width*height*4 halves at the same pointer).  out_depth
  * (device, width*height floats, nullable) receives the primary hit's reverse-Z
  * depth (0 on miss).  Stream-ordered; returns once the work is enqueued
  * (unless statistics or queue recording require a synchronisation). */
@@ -352,6 +356,32 @@ bi_pt_status bi_pt_nrc_debug_gradient(bi_pt_nrc* nrc, float* grad, uint32_t coun
 /* The last frame's first `count` training samples: records (16 floats each) and
  * targets (3 floats each), to host memory. */
 bi_pt_status bi_pt_nrc_debug_samples(bi_pt_nrc* nrc, float* records, float* targets, uint32_t count);
+
+/* ---- ray queries for the engine's other ray-tracing consumers ----
+ * The pass's traversal kernels with rays made by the caller: RT reflections
+ * (src/renderer/pass/reflection.cpp:161-170,384-404), DDGI probe tracing
+ * (src/renderer/pass/ddgi_update.cpp:229-256, shaders/renderer/ddgi/trace_gbuffer.hlsl:10-51)
+ * and RTAO (shaders/renderer/ambient_occlusion/ambient_occlusion_rt.hlsl:43-63) each build
+ * rays and call TraceRay against the scene's TLAS; here they pass device arrays instead.
+ * Rays: n origins and directions as six device arrays (directions need not be unit
+ * length; t is in units of the direction), TMin = 0.001 as in the reference's passes, one
+ * TMax for all rays (0.001 < tmax <= 1.7e38).  The context's LBVH is built on `stream` if
+ * the scene changed since the last build (order queries after other streams' renders of
+ * the same context).  Closest hits are those of the path tracer (independent of the tree
+ * and of traversal order).  Errors of the traversal (stack bound) are reported by
+ * bi_pt_get_stats like a render's.  Stream-ordered, no host synchronisation. */
+typedef struct bi_pt_rays {
+    const float *ox, *oy, *oz;   /* device, n each */
+    const float *dx, *dy, *dz;   /* device, n each */
+} bi_pt_rays;
+/* Closest hit: hit_t (+inf on a miss, nullable), hit_triangle (the triangle's index in the
+ * scene's instance order, as bi_pt_set_scene lays them out; 0xffffffff on a miss),
+ * barycentrics hit_u, hit_v of the second and third vertex (nullable).  All device, n each. */
+bi_pt_status bi_pt_trace_closest(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax, float* hit_t,
+                                 uint32_t* hit_triangle, float* hit_u, float* hit_v, bi_pt_stream stream);
+/* Occlusion (any hit in [0.001, tmax]): occluded[i] = 1 or 0 (device, n). */
+bi_pt_status bi_pt_trace_occlusion(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax,
+                                   uint32_t* occluded, bi_pt_stream stream);
 
 /* ---- inspection hooks for parity tests (no reference counterpart) ----
  * Copy the last-built LBVH to host memory: morton[n] (input triangle order),

@@ -376,3 +376,55 @@ def test_error_paths(pt, c0_scene):
     assert e.value.status == 2
     small.close()
     pt.render(cam, spp=1)   # the context still works after the failures
+
+
+def test_rgba16f_output_is_rounded_rgba32f():
+    """BI_PT_FLAG_OUTPUT_RGBA16F: the same running mean, each channel rounded to the
+    nearest even IEEE half (numpy's float32 -> float16 conversion)."""
+    print("case: c0 64x48 2spp, RGBA16F output", flush=True)
+    sc = scenes.scene_c0()
+    cam = scenes.camera_c0(64, 48)
+    a = bi_pt.PathTracer()
+    a.set_scene(sc)
+    a.render(cam, spp=2, max_bounces=5, seed=21)
+    f32 = a.image().copy()
+    b = bi_pt.PathTracer()
+    b.set_scene(sc)
+    b.render(cam, spp=2, max_bounces=5, seed=21, flags=bi_pt.FLAG_OUTPUT_RGBA16F)
+    f16 = b.image()
+    assert f16.dtype == np.float16
+    assert np.array_equal(f16.view(np.uint16), f32.astype(np.float16).view(np.uint16))
+    a.close()
+    b.close()
+
+
+def test_ray_queries_match_restatement(c0_oracle, c0_scene):
+    """bi_pt_trace_closest / bi_pt_trace_occlusion (rays in device memory, the pass's
+    traversal kernels): hits, triangles (in the scene's order), barycentrics and occlusion
+    equal the restatement's traversal bit for bit; a far tmax and unnormalised directions."""
+    print("case: c0 ray queries, 30000 rays", flush=True)
+    rng = np.random.default_rng(9)
+    rays = random_rays(rng, 30000, np.array([0.0, 2.0, 0.0]), 6.0)
+    g = bi_pt.PathTracer()
+    g.set_scene(c0_scene)
+    t, tri, u, v = g.trace_closest(rays[:, 0:3], rays[:, 4:7], tmax=1000.0)
+    ot, oleaf, ou, ov = c0_oracle.trace(rays)
+    hit = oleaf != 0xFFFFFFFF
+    order = c0_oracle.bvh()["order"]
+    assert np.array_equal(tri[hit], order[oleaf[hit]]) and np.all(tri[~hit] == 0xFFFFFFFF)
+    assert np.array_equal(t[hit].view(np.uint32), ot[hit].view(np.uint32)) and np.all(np.isinf(t[~hit]))
+    assert np.array_equal(u[hit].view(np.uint32), ou[hit].view(np.uint32))
+    assert np.array_equal(v[hit].view(np.uint32), ov[hit].view(np.uint32))
+    assert 0.3 < hit.mean() < 1.0
+    occ = g.trace_occlusion(rays[:, 0:3], rays[:, 4:7], tmax=1000.0)
+    assert np.array_equal(occ, c0_oracle.trace(rays, any_hit=True)[0])
+    # directions scaled by 2: t in units of the direction halves (float rounding of the
+    # slab and triangle tests, and TMin = 0.001 now in the doubled unit, may change a few
+    # grazing or near-origin rays)
+    t2, tri2, _, _ = g.trace_closest(rays[:, 0:3], 2.0 * rays[:, 4:7], tmax=500.0)
+    same = (tri2 == tri) & hit
+    assert same.sum() >= 0.999 * hit.sum()
+    assert np.allclose(t2[same], 0.5 * t[same], rtol=1e-5)
+    with pytest.raises(bi_pt.PtError):
+        g.trace_closest(rays[:4, 0:3], rays[:4, 4:7], tmax=3.0e38)
+    g.close()
