@@ -279,6 +279,15 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
     for (uint32_t m = 0; m < num_materials; ++m)
         if (!(materials[m].roughness >= 0.0f) || !(materials[m].anisotropy >= 0.0f && materials[m].anisotropy <= 1.0f))
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "material roughness/anisotropy out of range");
+    for (uint32_t m = 0; m < num_materials; ++m) {
+        const bi_pt_material& M = materials[m];
+        if (M.pattern > BI_PT_PATTERN_CHECKER_XZ)
+            return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "unknown material pattern");
+        if (M.pattern == BI_PT_PATTERN_CHECKER_XZ &&
+            (M.pattern_material >= num_materials || materials[M.pattern_material].pattern != BI_PT_PATTERN_NONE))
+            return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT,
+                        "checkerboard pattern_material must name a material without a pattern");
+    }
     std::vector<float> hp(3 * nv), hn(3 * nv);
     std::vector<uint32_t> hi(ni);
     for (uint32_t m = 0; m < num_meshes; ++m) {

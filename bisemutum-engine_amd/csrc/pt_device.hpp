@@ -187,6 +187,22 @@ PT_HD Mat load_mat(const MaterialRec& m) {
     return r;
 }
 
+// Checkerboard cell of a world coordinate: int(floor(x)), floor clamped to
+// [-2^31, 2^31 - 128] (NaN to the low end) so the conversion is always defined.
+PT_HD int checker_cell(float x) {
+    float f = __builtin_floorf(x);
+    if (!(f >= -2147483648.0f)) f = -2147483648.0f;
+    if (f > 2147483520.0f) f = 2147483520.0f;
+    return (int)f;
+}
+// The record that shades hit point P: a checkerboard material hands even cells of
+// (int(floor(P.x)) ^ int(floor(P.z))) to its pattern_material.
+PT_HD uint32_t resolve_material(const MaterialRec* mats, uint32_t id, V3 P) {
+    if (mats[id].pattern == kPatternCheckerXZ && ((checker_cell(P.x) ^ checker_cell(P.z)) & 1) == 0)
+        return mats[id].pattern_material;
+    return id;
+}
+
 struct Frame { V3 t, b, n; };
 PT_HD Frame make_frame(V3 n) {
     float sign = n.z > 0.0f ? 1.0f : -1.0f;

@@ -66,8 +66,11 @@ struct MaterialRec {   // = bi_pt_material
     float f90_color[3];
     float ior;
     uint32_t two_sided;
-    uint32_t pad[3];
+    uint32_t pattern;           // kPatternCheckerXZ: odd cells this record, even cells pattern_material
+    uint32_t pattern_material;
+    uint32_t pad;
 };
+constexpr uint32_t kPatternCheckerXZ = 1u;
 static_assert(sizeof(MaterialRec) == 64, "material record is 64 bytes");
 
 struct LightRec {      // = bi_pt_dir_light (DirLightData)

@@ -102,7 +102,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
         const float4* lsp = reinterpret_cast<const float4*>(J.leaf_shade + leaf);
         const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
         V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
-        const uint32_t mat_id = __float_as_uint(s2.y);
+        const uint32_t mat_id = resolve_material(J.materials, __float_as_uint(s2.y), P);
         V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
         Mat m = load_mat(J.materials[mat_id]);
         if (m.two_sided && dot(d, N) > 0.0f) N = -N;

@@ -75,7 +75,16 @@ typedef struct bi_pt_instance {
 
 /* Lit material (SurfaceData of shaders/core/material/utils.hlsl:5-16 restricted
  * to parameter materials).  64 bytes.  anisotropy follows
- * get_anisotropic_roughness; ior is only used for cos < 0 (never for reflection). */
+ * get_anisotropic_roughness; ior is only used for cos < 0 (never for reflection).
+ * pattern BI_PT_PATTERN_CHECKER_XZ: the world-space checkerboard of the example
+ * project's material (examples/scene_basic/materials/checkerboard.toml): at a hit
+ * point P, cells with (int(floor(P.x)) ^ int(floor(P.z))) odd use this record and
+ * even cells use record pattern_material (which must have no pattern itself);
+ * floor(P) is clamped to [-2^31, 2^31 - 128] before the conversion. */
+enum {
+    BI_PT_PATTERN_NONE = 0,
+    BI_PT_PATTERN_CHECKER_XZ = 1
+};
 typedef struct bi_pt_material {
     float base_color[3];
     float roughness;
@@ -84,7 +93,9 @@ typedef struct bi_pt_material {
     float f90_color[3];
     float ior;
     uint32_t two_sided;
-    uint32_t _pad[3];
+    uint32_t pattern;            /* BI_PT_PATTERN_* (0 for a parameter material) */
+    uint32_t pattern_material;   /* the record of even cells */
+    uint32_t _pad;
 } bi_pt_material;
 
 /* Byte-identical to DirLightData (shaders/renderer/lights_struct.hlsl:3-14,

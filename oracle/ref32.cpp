@@ -716,6 +716,24 @@ void camera_ray(const ora_camera& c, uint32_t px, uint32_t py, float jx, float j
 }
 
 // ------------------------------------------------------------------ material
+// Checkerboard materials (word 13 = 1, the example project's checkerboard.toml
+// select): cells of (int(floor(P.x)) ^ int(floor(P.z))) that are even shade with the
+// record in word 14, odd cells with the material itself.  floor is clamped to
+// [-2^31, 2^31 - 128] (NaN to the low end) before the conversion.
+inline int checker_cell(float x) {
+    float f = std::floor(x);
+    if (!(f >= -2147483648.0f)) f = -2147483648.0f;
+    if (f > 2147483520.0f) f = 2147483520.0f;
+    return (int)f;
+}
+inline uint32_t resolve_mat(const std::vector<float>& mats, uint32_t id, F3 P) {
+    uint32_t pattern, alt;
+    std::memcpy(&pattern, &mats[16 * (size_t)id + 13], 4);
+    std::memcpy(&alt, &mats[16 * (size_t)id + 14], 4);
+    if (pattern == 1u && ((checker_cell(P.x) ^ checker_cell(P.z)) & 1) == 0) return alt;
+    return id;
+}
+
 struct Mat {
     F3 base, f0, f90;
     float rx, ry;
@@ -986,7 +1004,8 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         const float* tn = &s.tri_n[9 * tri];
         F3 n0 = f3(tn[0], tn[1], tn[2]), n1 = f3(tn[3], tn[4], tn[5]), n2 = f3(tn[6], tn[7], tn[8]);
         F3 N = normalize((n0 + (n1 - n0) * h.u) + (n2 - n0) * h.v);
-        Mat m = load_mat(&s.materials[16 * s.tri_mat[tri]]);
+        const uint32_t mid = resolve_mat(s.materials, s.tri_mat[tri], P);
+        Mat m = load_mat(&s.materials[16 * mid]);
         if (m.two_sided && dot(ray.d, N) > 0.0f) N = -N;
         F3 V = normalize(ray.o - P);
         if (cm && (int)b == cm->bounce) {
@@ -996,7 +1015,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
                 acc_miss = finite3(c) ? c : f3(0, 0, 0);
                 break;
             }
-            const float* mr = &s.materials[16 * s.tri_mat[tri]];
+            const float* mr = &s.materials[16 * mid];
             float* r = cm->rec;
             r[0] = P.x; r[1] = P.y; r[2] = P.z;
             r[3] = V.x; r[4] = V.y; r[5] = V.z;

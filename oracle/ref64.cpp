@@ -202,6 +202,22 @@ bool any64(const Bvh64& b, D3 o, D3 d, double tmin, double tmax) {
 }
 
 // ------------------------------------------------------------------ shading
+// Checkerboard materials (word 13 = 1): even cells of floor(P.x) ^ floor(P.z) shade
+// with the record in word 14, evaluated at the FP64 hit point.
+uint32_t resolve_mat64(const std::vector<float>& mats, uint32_t id, D3 P) {
+    uint32_t pattern, alt;
+    std::memcpy(&pattern, &mats[16 * (size_t)id + 13], 4);
+    std::memcpy(&alt, &mats[16 * (size_t)id + 14], 4);
+    if (pattern != 1u) return id;
+    auto cell = [](double x) {
+        double f = std::floor(x);
+        if (!(f >= -2147483648.0)) f = -2147483648.0;
+        if (f > 2147483520.0) f = 2147483520.0;
+        return (long long)f;
+    };
+    return ((cell(P.x) ^ cell(P.z)) & 1) == 0 ? alt : id;
+}
+
 struct Mat64 { D3 base, f0, f90; double rx, ry; bool two_sided; };
 Mat64 load_mat64(const float* m) {
     Mat64 r;
@@ -419,7 +435,7 @@ D3 path64(const Scene& s, const Bvh64& b, const Sky64& k, const ora_camera& cam,
         const float* tn = &s.tri_n[9 * (size_t)h.tri];
         D3 n0 = d3(tn[0], tn[1], tn[2]), n1 = d3(tn[3], tn[4], tn[5]), n2 = d3(tn[6], tn[7], tn[8]);
         D3 N = normalize(n0 + (n1 - n0) * h.u + (n2 - n0) * h.v);
-        Mat64 m = load_mat64(&s.materials[16 * (size_t)s.tri_mat[h.tri]]);
+        Mat64 m = load_mat64(&s.materials[16 * (size_t)resolve_mat64(s.materials, s.tri_mat[h.tri], P)]);
         if (m.two_sided && dot(dir, N) > 0) N = -N;
         D3 V = normalize(o - P);
         Frame64 fr = frame64(N);

@@ -1,0 +1,59 @@
+"""Writes the scene_basic fixtures from the engine's example project.
+
+Run in a checkout that has the example project (default
+/root/reference/examples/scene_basic).  Output under tests/golden/scene_basic/:
+  meshes/*.static_mesh.biasset  the three StaticMesh assets, byte for byte (binary data)
+  scene.json                    scene.toml as parsed data (objects, components, values)
+  assets.json                   asset id -> {type, file} of asset_metadata.toml (meshes and materials)
+  materials.json                asset id -> material data: blend_mode, surface_model, params and,
+                                in place of the HLSL material function, its parsed statements
+                                or the reason it is outside the parameter subset
+Textures and the prefab are not written: the pass reads parameter materials only.
+"""
+import json
+import os
+import shutil
+import sys
+
+import tomli
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "bisemutum-engine_amd"))
+import assets  # noqa: E402
+
+
+def main(src="/root/reference/examples/scene_basic"):
+    out = os.path.join(HERE, "scene_basic")
+    os.makedirs(os.path.join(out, "meshes"), exist_ok=True)
+
+    def toml(rel):
+        with open(os.path.join(src, rel), "rb") as f:
+            return tomli.load(f)
+
+    meta = toml("asset_metadata.toml")
+    table, mats = {}, {}
+    for a in meta["assets"]:
+        rel = a["path"][len("/project/"):]
+        if a["type"] == "StaticMesh":
+            name = os.path.basename(rel)
+            shutil.copyfile(os.path.join(src, rel), os.path.join(out, "meshes", name))
+            table[a["id"]] = {"type": "StaticMesh", "file": "meshes/" + name}
+        elif a["type"] == "Material":
+            m = toml(rel)
+            rec = {k: m[k] for k in ("blend_mode", "surface_model", "params") if k in m}
+            try:
+                rec["statements"] = assets.parse_material_function(m.get("material_function", ""))
+            except assets.UnsupportedMaterial as e:
+                rec["outside_subset"] = str(e).split(":")[0]
+            mats[a["id"]] = rec
+            table[a["id"]] = {"type": "Material", "file": os.path.basename(rel)}
+    with open(os.path.join(out, "assets.json"), "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    with open(os.path.join(out, "materials.json"), "w") as f:
+        json.dump(mats, f, indent=1, sort_keys=True)
+    with open(os.path.join(out, "scene.json"), "w") as f:
+        json.dump(toml("scene.toml"), f, indent=1)
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])

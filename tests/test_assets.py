@@ -1,0 +1,249 @@
+"""Scene ingestion: StaticMesh assets, transforms, parameter materials and the example
+project (fixtures under tests/golden/scene_basic, written by make_scene_basic.py)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import assets
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scene_basic")
+REF_PROJECT = "/root/reference/examples/scene_basic"
+
+
+def mesh_bytes(name):
+    with open(os.path.join(GOLD, "meshes", f"{name}.static_mesh.biasset"), "rb") as f:
+        return f.read()
+
+
+def fixture_scene(width=None, height=None, on_unsupported="default"):
+    with open(os.path.join(GOLD, "scene.json")) as f:
+        scene = json.load(f)
+    with open(os.path.join(GOLD, "assets.json")) as f:
+        table = {int(k): v for k, v in json.load(f).items()}
+    with open(os.path.join(GOLD, "materials.json")) as f:
+        mats = {int(k): v for k, v in json.load(f).items()}
+
+    def fetch(i):
+        if table[i]["type"] == "StaticMesh":
+            with open(os.path.join(GOLD, table[i]["file"]), "rb") as f:
+                return f.read()
+        return mats[i]
+    return assets.scene_from_description(scene, fetch, "scene_basic", width, height, on_unsupported)
+
+
+# ---------------------------------------------------------------- StaticMesh
+
+def winding_agrees(p, n, idx):
+    a, b, c = p[idx[:, 0]], p[idx[:, 1]], p[idx[:, 2]]
+    g = np.cross(b - a, c - a)
+    return np.all((g * (n[idx[:, 0]] + n[idx[:, 1]] + n[idx[:, 2]])).sum(1) > 0)
+
+
+def test_cube_asset_is_the_unit_cube():
+    m = assets.read_static_mesh(mesh_bytes("cube"))
+    assert len(m.positions) == 24 and len(m.indices) == 36 and len(m.submeshes) == 1
+    assert len(m.normals) == 24 and len(m.tangents) == 24 and len(m.texcoords) == 24
+    assert len(m.colors) == 0 and len(m.texcoords2) == 0
+    assert int(m.submeshes[0]["num_indices"]) == 0xFFFFFFFF and int(m.submeshes[0]["topology"]) == 3
+    p, n, idx = assets.submesh_triangles(m, 0)
+    assert idx.shape == (12, 3)
+    assert np.array_equal(np.abs(p), np.ones_like(p))               # corners of [-1,1]^3
+    assert np.array_equal(np.abs(n).sum(1), np.ones(24, np.float32))  # axis-aligned face normals
+    assert np.array_equal((n * p).sum(1), np.ones(24, np.float32))    # each vertex on its face plane
+    assert winding_agrees(p, n, idx)
+    # every face is a quad of 4 vertices with one normal, 6 faces
+    assert len({tuple(v) for v in n}) == 6
+
+
+def test_plane_and_sphere_assets():
+    pl = assets.read_static_mesh(mesh_bytes("plane"))
+    p, n, idx = assets.submesh_triangles(pl, 0)
+    assert idx.shape == (2, 3) and np.all(p[:, 1] == 0) and np.all(n == [0, 1, 0])
+    assert np.array_equal(np.sort(np.abs(p[:, [0, 2]]).ravel()), np.ones(8, np.float32))
+    assert winding_agrees(p, n, idx)
+    sp = assets.read_static_mesh(mesh_bytes("sphere"))
+    p, n, idx = assets.submesh_triangles(sp, 0)
+    assert idx.shape == (960, 3) and len(p) == 559
+    r = np.linalg.norm(p.astype(np.float64), axis=1)
+    assert np.all(np.abs(r - 1.0) < 1e-5)                          # unit sphere
+    assert np.all(np.abs(np.linalg.norm(n.astype(np.float64), axis=1) - 1.0) < 1e-3)
+    assert np.all((n * p).sum(1) > 0.99)                           # smooth normals point outwards
+    assert winding_agrees(p, n, idx)
+    # closed mesh: every edge is shared by exactly two triangles (up to welded seams)
+    assert np.all(np.unique(idx) == np.arange(len(p))) or len(np.unique(idx)) <= len(p)
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_static_mesh_round_trip(version):
+    m = assets.read_static_mesh(mesh_bytes("sphere"))
+    data = assets.encode_static_mesh(m, version)
+    m2 = assets.read_static_mesh(data)
+    for f in ("positions", "normals", "tangents", "colors", "texcoords", "texcoords2", "indices"):
+        assert np.array_equal(getattr(m, f), getattr(m2, f)), f
+    assert m.submeshes.tobytes() == m2.submeshes.tobytes()
+    if version == 2:   # same layout as the asset written by the engine, up to the zlib stream
+        assert data[:26] == mesh_bytes("sphere")[:26]
+
+
+def test_static_mesh_errors():
+    good = mesh_bytes("cube")
+    with pytest.raises(assets.AssetError, match="not a StaticMesh"):
+        assets.read_static_mesh(b"\0\0\0\0" + good[4:])
+    with pytest.raises(assets.AssetError, match="truncated"):
+        assets.read_static_mesh(good[:20])
+    bad_version = bytearray(good)
+    bad_version[22:26] = (7).to_bytes(4, "little")
+    with pytest.raises(assets.AssetError, match="version 7"):
+        assets.read_static_mesh(bytes(bad_version))
+    m = assets.read_static_mesh(good)
+    m.submeshes = m.submeshes.copy()
+    m.submeshes["topology"] = 4   # triangle strip
+    with pytest.raises(assets.AssetError, match="triangle list"):
+        assets.submesh_triangles(m, 0)
+
+
+def test_submesh_ranges():
+    m = assets.read_static_mesh(mesh_bytes("cube"))
+    sub = np.zeros(2, assets.SUBMESH_DTYPE)
+    sub[0] = (0, 0, 18, 3, 0)
+    sub[1] = (0, 18, 0xFFFFFFFF, 3, 0)
+    m.submeshes = sub
+    _, _, a = assets.submesh_triangles(m, 0)
+    _, _, b = assets.submesh_triangles(m, 1)
+    assert np.array_equal(np.concatenate([a, b]).ravel(), m.indices)
+    m.indices = np.zeros(0, np.uint32)           # no index data: vertices in order
+    sub[0] = (3, 0, 6, 3, 0)
+    _, _, c = assets.submesh_triangles(m, 0)
+    assert np.array_equal(c.ravel(), np.arange(3, 9))
+
+
+# ---------------------------------------------------------------- transforms
+
+def test_euler_zxy_single_axes_and_order():
+    r = assets.euler_zxy_degrees
+    assert np.allclose(r((90, 0, 0)) @ [0, 1, 0], [0, 0, 1])      # about x: y -> z
+    assert np.allclose(r((0, 90, 0)) @ [0, 0, 1], [1, 0, 0])      # about y: z -> x
+    assert np.allclose(r((0, 0, 90)) @ [1, 0, 0], [0, 1, 0])      # about z: x -> y
+    # glm::eulerAngleZXY(t1 = z, t2 = x, t3 = y) in closed form, column by column
+    x, y, z = 0.3, -1.1, 0.7
+    c1, s1, c2, s2, c3, s3 = math.cos(z), math.sin(z), math.cos(x), math.sin(x), math.cos(y), math.sin(y)
+    cols = np.array([[c1 * c3 - s1 * s2 * s3, c3 * s1 + c1 * s2 * s3, -c2 * s3],
+                     [-c2 * s1, c1 * c2, s2],
+                     [c1 * s3 + c3 * s1 * s2, s1 * s3 - c1 * c3 * s2, c2 * c3]])
+    got = r(tuple(math.degrees(a) for a in (x, y, z)))
+    assert np.allclose(got, cols.T, atol=1e-12)
+
+
+def test_transform_matrix_is_trs():
+    v = {"translation": [1.0, 2.0, 3.0], "scaling": [2.0, 1.0, 0.5], "rotation": [0.0, 90.0, 0.0]}
+    m = assets.transform_matrix(v)
+    # scale, then rotate z -> x, then translate
+    assert np.allclose(m @ [0, 0, 1, 1], [1.5, 2.0, 3.0, 1.0])
+    assert np.allclose(m @ [1, 0, 0, 1], [1.0, 2.0, 1.0, 1.0])
+    assert np.allclose(assets.transform_matrix({}), np.eye(4))
+
+
+# ---------------------------------------------------------------- materials
+
+def test_parse_material_function_subset():
+    s = assets.parse_material_function(
+        "surface.base_color = PARAM_albedo;\n surface.roughness = 0.25f;\n"
+        "surface.two_sided = true;\nsurface.f0_color = float3(0.1, 0.2, 0.3);\n")
+    assert s == [("base_color", ("param", "albedo")), ("roughness", ("const", 0.25)),
+                 ("two_sided", ("const", True)), ("f0_color", ("const", (0.1, 0.2, 0.3)))]
+    for bad, why in (("surface.base_color = PARAM_t.Sample(PARAM_s, vertex.texcoord).xyz;", "texture"),
+                     ("surface.roughness = vertex.position_world.x > 0 ? 1.0 : 0.5;", "expression"),
+                     ("float a = 1.0;", "statement"),
+                     ("surface.sheen = 1.0;", "unknown surface field")):
+        with pytest.raises(assets.UnsupportedMaterial, match=why):
+            assets.parse_material_function(bad)
+
+
+def test_parse_checker_select():
+    text = ("int cell = int(floor(vertex.position_world.z)) ^ int(floor(vertex.position_world.x));\n"
+            "surface.base_color = (cell & 1) == 0 ? PARAM_even : float3(1, 0, 0);\n"
+            "surface.roughness = (cell & 1) == 1 ? 0.25 : PARAM_r;")
+    s = assets.parse_material_function(text)
+    assert s == [("$cell", ("checker_xz",)),
+                 ("base_color", ("checker", "cell", ("const", (1.0, 0.0, 0.0)), ("param", "even"))),
+                 ("roughness", ("checker", "cell", ("const", 0.25), ("param", "r")))]
+    surf = assets.evaluate_material({"material_function": text, "params": [
+        {"name": "even", "value": [0.0, 0.0, 1.0]}, {"name": "r", "value": 0.75}]})
+    assert surf["base_color"] == (1.0, 0.0, 0.0) and surf["roughness"] == 0.25
+    assert surf["pattern_even"]["base_color"] == (0.0, 0.0, 1.0) and surf["pattern_even"]["roughness"] == 0.75
+    for bad in ("int c = int(floor(vertex.position_world.x)) ^ int(floor(vertex.position_world.y));",
+                "surface.roughness = (other & 1) == 1 ? 0.5 : 1.0;"):
+        with pytest.raises(assets.UnsupportedMaterial):
+            assets.parse_material_function(bad)
+
+
+def test_evaluate_material_defaults_and_params():
+    surf = assets.evaluate_material({"blend_mode": "opaque", "material_function": "surface.roughness = PARAM_r;",
+                                     "params": [{"name": "r", "value": 0.3}]})
+    assert surf["roughness"] == 0.3 and surf["base_color"] == (0.5, 0.5, 0.5) and surf["f0_color"] == (0.04,) * 3
+    rec = assets.material_record(surf)
+    assert np.allclose(rec[0:12], [0.5, 0.5, 0.5, 0.3, 0.04, 0.04, 0.04, 0.0, 1, 1, 1, 1.5])
+    assert rec.view(np.uint32)[12] == 0
+    scalar = assets.evaluate_material({"material_function": "surface.base_color = PARAM_c;",
+                                       "params": [{"name": "c", "value": 0.25}]})
+    assert scalar["base_color"] == (0.25, 0.25, 0.25)                   # HLSL scalar broadcast
+    with pytest.raises(assets.UnsupportedMaterial, match="stochastic"):
+        assets.evaluate_material({"blend_mode": "translucent", "material_function": "surface.opacity = 0.5;"})
+    with pytest.raises(assets.UnsupportedMaterial, match="not declared"):
+        assets.evaluate_material({"material_function": "surface.roughness = PARAM_missing;"})
+    with pytest.raises(assets.UnsupportedMaterial, match="texture parameter"):
+        assets.evaluate_material({"material_function": "surface.base_color = PARAM_t;",
+                                  "params": [{"name": "t", "value": {"asset_id": 6}}]})
+
+
+# ---------------------------------------------------------------- the example project
+
+def test_scene_basic_from_fixtures():
+    ps = fixture_scene()
+    sc = ps.scene
+    assert [o[0] for o in ps.objects] == ["Sphere", "Cube", "Transparent Cube", "Cage Cube", "Plane"]
+    assert len(sc.meshes) == 3 and sc.num_tris == 960 + 3 * 12 + 2
+    assert set(ps.unsupported) == {4, 9, 11}
+    assert "texture" in ps.unsupported[4] and "texture" in ps.unsupported[11]
+    assert "stochastic" in ps.unsupported[9]
+    assert ps.settings == {"max_bounces": 3, "ray_length": 100.0, "accumulate": True}
+    white = sc.materials[sc.instances["material"][1]]
+    assert np.array_equal(white[:3], [1, 1, 1]) and white[3] == np.float32(0.5)
+    # checkerboard floor: odd cells 0.8 grey / roughness 1, even cells 0.1 / 0.1
+    k = int(sc.instances["material"][4])
+    odd, even = sc.materials[k], sc.materials[sc.materials[k].view(np.uint32)[14]]
+    assert odd.view(np.uint32)[13] == assets.PATTERN_CHECKER_XZ and even.view(np.uint32)[13] == 0
+    assert np.allclose(odd[:4], [0.8, 0.8, 0.8, 1.0]) and np.allclose(even[:4], [0.1, 0.1, 0.1, 0.1])
+    assert np.allclose(odd[4:7], 0.04) and np.allclose(even[4:7], 0.04)
+    # plane: scaled 5 x 1 x 5 and lowered by one
+    plane = sc.instances[4]["xform"].reshape(3, 4)
+    assert np.allclose(plane, [[5, 0, 0, 0], [0, 1, 0, -1], [0, 0, 5, 0]])
+    # light: colour * strength, direction R(10, 0, -30) (0, 1, 0)
+    li = sc.dir_lights[0]
+    assert np.allclose(li[0:3], [4.0, 3.6, 3.2])
+    d = assets.euler_zxy_degrees((10.000005722045898, 0.0, -30.000001907348633)) @ [0, 1, 0]
+    assert np.allclose(li[4:7], d, atol=1e-6) and li.view(np.int32)[3] == 0 and li[7] == 1.0
+    # camera at its translation, looking down its -z
+    cam = ps.camera
+    assert (cam.width, cam.height) == (1298, 635)
+    assert np.allclose(cam.inv_view[:3, 3], [10.5, 4.5, 6.0], atol=1e-5)
+    front = assets.euler_zxy_degrees((-9.3324127197265625, 63.346549987792969, 17.90446662902832)) @ [0, 0, -1]
+    assert np.allclose(cam.inv_view[:3, :3] @ [0, 0, -1], front, atol=1e-5)
+    with pytest.raises(assets.UnsupportedMaterial, match="material asset 4: texture"):
+        fixture_scene(on_unsupported="raise")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_PROJECT), reason="the engine's example project is not present")
+def test_load_project_matches_fixtures():
+    a = assets.load_project(REF_PROJECT, 130, 64, on_unsupported="default")
+    b = fixture_scene(130, 64)
+    assert a.objects == b.objects and a.settings == b.settings and set(a.unsupported) == set(b.unsupported)
+    assert a.scene.instances.tobytes() == b.scene.instances.tobytes()
+    assert a.scene.materials.tobytes() == b.scene.materials.tobytes()
+    assert a.scene.dir_lights.tobytes() == b.scene.dir_lights.tobytes()
+    for (p, n, i), (q, m, j) in zip(a.scene.meshes, b.scene.meshes):
+        assert p.tobytes() == q.tobytes() and n.tobytes() == m.tobytes() and i.tobytes() == j.tobytes()
+    assert a.camera.inv_view.tobytes() == b.camera.inv_view.tobytes()

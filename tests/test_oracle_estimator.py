@@ -118,3 +118,60 @@ def test_fp32_converges_to_fp64_c0(c0_oracle):
     # aggregate over the subset: the summed difference is within 4 combined standard errors
     zsum = (s32[pix] - m64).sum(axis=0) / np.sqrt((se ** 2).sum(axis=0))
     assert np.all(np.abs(zsum) < 4), zsum
+
+
+RHO_ODD = np.array([0.8, 0.7, 0.6])
+RHO_EVEN = np.array([0.1, 0.2, 0.3])
+
+
+def checker_scene():
+    """A Lambertian floor whose material is the world checkerboard (odd cells RHO_ODD,
+    even cells RHO_EVEN) under a vertical sun; black sky."""
+    p, n, i = scenes.grid_quad((-100, 0, -100), (200, 0, 0), (0, 0, 200), 1, 1, (0, 1, 0))
+    mats = []
+    for rho in (RHO_ODD, RHO_EVEN):
+        m = scenes.material(rho, 1.0, 0.0)
+        m[4:7] = 0.0
+        m[8:11] = 0.0
+        mats.append(m)
+    mats[0].view(np.uint32)[13] = 1   # checkerboard, even cells take material 1
+    mats[0].view(np.uint32)[14] = 1
+    inst = np.array([(scenes.translate_scale((0, 0, 0)), 0, 0)], scenes.INSTANCE_DTYPE)
+    sky, col = scenes.constant_sky((0.0, 0.0, 0.0))
+    light = scenes.dir_light(E_SUN, (0.0, 1.0, 0.0), cast_shadow=False)
+    return scenes.Scene("checker", [(p, n, i)], inst, np.stack(mats), light[None], sky, col,
+                        np.eye(3, dtype=np.float32).reshape(-1))
+
+
+def pixel_floor_points(cam):
+    """FP64 hit points of the pixel-centre camera rays on the plane y = 0."""
+    w, h = cam.width, cam.height
+    x, y = np.meshgrid((np.arange(w) + 0.5) / w, (np.arange(h) + 0.5) / h)
+    ndc = np.stack([2 * x - 1, 1 - 2 * y, np.ones_like(x), np.ones_like(x)], -1).reshape(-1, 4)
+    v = ndc @ cam.inv_proj.astype(np.float64).T
+    v = v[:, :3] / np.linalg.norm(v[:, :3], axis=1, keepdims=True)
+    d = v @ cam.inv_view[:3, :3].astype(np.float64).T
+    o = cam.inv_view[:3, 3].astype(np.float64)
+    t = -o[1] / d[:, 1]
+    return o + d * t[:, None]
+
+
+def test_checker_floor_closed_form(oracle_lib):
+    """Each pixel shows rho/pi * E of its cell: odd (int(floor(x)) ^ int(floor(z))) the
+    material itself, even its pattern material; pixels near a cell edge are skipped."""
+    sc = oracle_lib.OracleScene(checker_scene())
+    cam = scenes.make_camera((0.3, 6.0, 0.2), (0.0, 0.0, -1.5), 96, 64, 60.0)
+    out = sc.render32(cam, max_bounces=2, seed=1, spp=1)["sums"]
+    P = pixel_floor_points(cam)
+    fx, fz = np.floor(P[:, 0]), np.floor(P[:, 2])
+    odd = ((fx.astype(np.int64) ^ fz.astype(np.int64)) & 1) == 1
+    clear = (np.minimum(P[:, 0] - fx, fx + 1 - P[:, 0]) > 1e-3) & (np.minimum(P[:, 2] - fz, fz + 1 - P[:, 2]) > 1e-3)
+    want = np.where(odd[:, None], RHO_ODD, RHO_EVEN) / math.pi * E_SUN
+    assert clear.mean() > 0.95 and 0.3 < odd[clear].mean() < 0.7      # both kinds of cell in view
+    np.testing.assert_allclose(out[clear], want[clear], rtol=2e-6)
+    # the FP64 oracle shades the same cells
+    pix = np.flatnonzero(clear)[::17].astype(np.uint32)
+    mean, _ = sc.render64(cam, 2, pix, 1)
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)   # the scene stores float32
+    want64 = np.where(odd[:, None], f32(RHO_ODD), f32(RHO_EVEN)) / math.pi * f32(E_SUN)
+    np.testing.assert_allclose(mean, want64[pix], rtol=1e-12)
