@@ -102,9 +102,15 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
         const float4* lsp = reinterpret_cast<const float4*>(J.leaf_shade + leaf);
         const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
         V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
-        const uint32_t mat_id = resolve_material(J.materials, __float_as_uint(s2.y), P);
+        uint32_t mat_id = __float_as_uint(s2.y);
         V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
+        // the triangle's record is loaded at once (its pattern word with it); a checkerboard's
+        // even cells then reload from the pattern material, so plain materials wait for one load
         Mat m = load_mat(J.materials[mat_id]);
+        if (const uint32_t cell_id = resolve_material(J.materials, mat_id, P); cell_id != mat_id) {
+            mat_id = cell_id;
+            m = load_mat(J.materials[mat_id]);
+        }
         if (m.two_sided && dot(d, N) > 0.0f) N = -N;
         V3 V = normalize(o - P);
         if (a.bounce == J.cache_bounce && (J.flags & (kJobCache | kJobTrain))) {
