@@ -68,3 +68,16 @@ def test_no_gpu_means_loud_failure():
         pytest.skip("a GPU is visible")
     with pytest.raises(RuntimeError):
         bi_pt.PathTracer()
+
+
+def test_assets_library_exports_its_header():
+    """libbi_assets.so (host-only StaticMesh loader) exports every bi_static_mesh_* in include/bi_assets.h."""
+    path = os.path.join(ROOT, "bisemutum-engine_amd", "libbi_assets.so")
+    if not os.path.exists(path):
+        pytest.skip("libbi_assets.so not built")
+    text = open(os.path.join(ROOT, "include", "bi_assets.h")).read()
+    names = set(re.findall(r"^\s*(?:[a-z_0-9]+\s*\*?\s+)+\**(bi_static_mesh_[a-z_0-9]+)\s*\(", text, re.M))
+    assert names == {"bi_static_mesh_load", "bi_static_mesh_destroy", "bi_static_mesh_counts", "bi_static_mesh_submesh"}
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r" T (bi_static_mesh_[a-z_0-9]+)$", out.stdout, re.M))
+    assert names <= exported

@@ -247,3 +247,85 @@ def test_load_project_matches_fixtures():
     for (p, n, i), (q, m, j) in zip(a.scene.meshes, b.scene.meshes):
         assert p.tobytes() == q.tobytes() and n.tobytes() == m.tobytes() and i.tobytes() == j.tobytes()
     assert a.camera.inv_view.tobytes() == b.camera.inv_view.tobytes()
+
+
+# ---------------------------------------------------------------- the C++ loader (bi_assets.h)
+
+ASSETS_LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "bisemutum-engine_amd", "libbi_assets.so")
+
+
+class _PtMesh(__import__("ctypes").Structure):
+    import ctypes as C
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32)]
+
+
+def native_lib():
+    import ctypes as C
+    if not os.path.exists(ASSETS_LIB):
+        pytest.skip("libbi_assets.so not built")
+    lib = C.CDLL(ASSETS_LIB)
+    lib.bi_static_mesh_load.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
+    lib.bi_static_mesh_destroy.argtypes = [C.c_void_p]
+    lib.bi_static_mesh_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    lib.bi_static_mesh_submesh.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(_PtMesh), C.c_char_p, C.c_size_t]
+    return lib
+
+
+def native_load(lib, data):
+    import ctypes as C
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    buf = C.create_string_buffer(data, len(data))
+    rc = lib.bi_static_mesh_load(buf, len(data), C.byref(h), err, 256)
+    return rc, h, err.value.decode()
+
+
+@pytest.mark.parametrize("name", ["cube", "plane", "sphere"])
+@pytest.mark.parametrize("version", [1, 2])
+def test_native_loader_matches(name, version):
+    import ctypes as C
+    lib = native_lib()
+    m = assets.read_static_mesh(mesh_bytes(name))
+    data = mesh_bytes(name) if version == 2 else assets.encode_static_mesh(m, 1)
+    rc, h, err = native_load(lib, data)
+    assert rc == 0, err
+    counts = (C.c_uint64 * 8)()
+    lib.bi_static_mesh_counts(h, counts)
+    assert list(counts) == [len(m.positions), len(m.normals), len(m.tangents), len(m.colors), len(m.texcoords),
+                            len(m.texcoords2), len(m.indices), len(m.submeshes)]
+    out = _PtMesh()
+    assert lib.bi_static_mesh_submesh(h, 0, C.byref(out), None, 0) == 0
+    p, n, idx = assets.submesh_triangles(m, 0)
+    got_p = np.ctypeslib.as_array(out.positions, (out.num_vertices * 3,)).reshape(-1, 3)
+    got_n = np.ctypeslib.as_array(out.normals, (out.num_vertices * 3,)).reshape(-1, 3)
+    got_i = np.ctypeslib.as_array(out.indices, (out.num_indices,)).reshape(-1, 3)
+    assert np.array_equal(got_p.view(np.uint32), p.view(np.uint32))
+    assert np.array_equal(got_n.view(np.uint32), n.view(np.uint32))
+    assert np.array_equal(got_i, idx)
+    lib.bi_static_mesh_destroy(h)
+
+
+def test_native_loader_errors():
+    import ctypes as C
+    lib = native_lib()
+    good = mesh_bytes("cube")
+    bad_version = bytearray(good)
+    bad_version[22:26] = (7).to_bytes(4, "little")
+    bad_z = bytearray(good)
+    bad_z[50] ^= 0xFF
+    for data, why in ((b"\0\0\0\0" + good[4:], "not a StaticMesh"), (good[:20], "truncated"),
+                      (bytes(bad_version), "version 7"), (bytes(bad_z), "inflate")):
+        rc, h, err = native_load(lib, data)
+        assert rc != 0 and why in err and not h.value, (why, err)
+    m = assets.read_static_mesh(good)
+    m.submeshes = m.submeshes.copy()
+    m.submeshes["topology"] = 4
+    rc, h, err = native_load(lib, assets.encode_static_mesh(m, 2))
+    assert rc == 0
+    msg = C.create_string_buffer(128)
+    out = _PtMesh()
+    assert lib.bi_static_mesh_submesh(h, 0, C.byref(out), msg, 128) != 0 and b"triangle list" in msg.value
+    assert lib.bi_static_mesh_submesh(h, 5, C.byref(out), msg, 128) != 0 and b"out of range" in msg.value
+    lib.bi_static_mesh_destroy(h)
