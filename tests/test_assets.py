@@ -329,3 +329,35 @@ def test_native_loader_errors():
     assert lib.bi_static_mesh_submesh(h, 0, C.byref(out), msg, 128) != 0 and b"triangle list" in msg.value
     assert lib.bi_static_mesh_submesh(h, 5, C.byref(out), msg, 128) != 0 and b"out of range" in msg.value
     lib.bi_static_mesh_destroy(h)
+
+
+def test_parent_transforms_compose():
+    """An object under a parent is placed by parent_world @ local (matrix product of TRS
+    transforms); the camera and light follow their parents too."""
+    def obj(name, comps, parent=None):
+        o = {"name": name, "components": [{"type": t, "value": v} for t, v in comps]}
+        if parent is not None:
+            o["parent"] = parent
+        return o
+    root = {"translation": [1.0, 0.0, 0.0], "rotation": [0.0, 90.0, 0.0], "scaling": [2.0, 2.0, 2.0]}
+    child = {"translation": [0.0, 0.0, 1.0]}
+    mesh = [("StaticMeshComponent", {"static_mesh": {"asset_id": 1}}),
+            ("MeshRendererComponent", {"materials": [{"asset_id": 5}]})]
+    desc = {"objects": [
+        obj("root", [("Transform", root)]),
+        obj("cube", [("Transform", child)] + mesh, parent=0),
+        obj("cam", [("Transform", {"translation": [0.0, 0.0, 5.0]}),
+                    ("CameraComponent", {"yfov": 40.0, "render_target_size": [64, 32]})], parent=0),
+        obj("sun", [("Transform", {"rotation": [90.0, 0.0, 0.0]}), ("DirectionalLightComponent", {"strength": 2.0})],
+            parent=0),
+    ]}
+    white = {"blend_mode": "opaque", "statements": [], "params": []}
+    ps = assets.scene_from_description(desc, lambda i: mesh_bytes("cube") if i == 1 else white)
+    x = ps.scene.instances[0]["xform"].reshape(3, 4).astype(np.float64)
+    # child origin: parent maps (0,0,1) -> R_y(90) (0,0,2) + (1,0,0) = (3,0,0); scale 2 carried
+    assert np.allclose(x[:, 3], [3.0, 0.0, 0.0], atol=1e-6)
+    assert np.allclose(x[:, :3], assets.euler_zxy_degrees((0, 90, 0)) * 2.0, atol=1e-6)
+    assert np.allclose(ps.camera.inv_view[:3, 3], [11.0, 0.0, 0.0], atol=1e-5)
+    # light: R_parent R_local (0,1,0) with the scale removed; R_x(90) takes y to z, R_y(90) z to x
+    assert np.allclose(ps.scene.dir_lights[0][4:7], [1.0, 0.0, 0.0], atol=1e-6)
+    assert np.allclose(ps.scene.dir_lights[0][0:3], [2.0, 2.0, 2.0])
