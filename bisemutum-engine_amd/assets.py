@@ -435,7 +435,10 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
             world[i] = world_of(p, depth + 1) @ local if 0 <= p < len(objects) else local
         return world[i]
 
-    meshes, mesh_key, mats, mat_key, inst, placed = [], {}, [], {}, [], []
+    meshes, mats, inst, placed = [], [], [], []
+    parsed = {}        # mesh asset id -> MeshData
+    mesh_index = {}    # (mesh asset id, submesh) -> index into meshes
+    mat_index = {}     # material asset id -> index into mats
     unsupported: dict = {}
     lights = []
     camera = None
@@ -472,9 +475,9 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
             mesh_id = int(c["StaticMeshComponent"]["static_mesh"]["asset_id"])
             r = c["MeshRendererComponent"]
             start = int(r.get("submesh_start_index", 0))
-            if mesh_id not in mesh_key:
-                mesh_key[mesh_id] = read_static_mesh(fetch(mesh_id), f"asset {mesh_id}")
-            md = mesh_key[mesh_id]
+            if mesh_id not in parsed:
+                parsed[mesh_id] = read_static_mesh(fetch(mesh_id), f"asset {mesh_id}")
+            md = parsed[mesh_id]
             count = len(md.submeshes) - start
             mat_ids = [int(m["asset_id"]) for m in r.get("materials", [])]
             if count > len(mat_ids):
@@ -483,11 +486,11 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
             for k in range(count):
                 sub = start + k
                 key = (mesh_id, sub)
-                if key not in mat_key:
-                    mat_key[key] = len(meshes)
+                if key not in mesh_index:
+                    mesh_index[key] = len(meshes)
                     meshes.append(submesh_triangles(md, sub))
                 mid = mat_ids[k]
-                if ("m", mid) not in mat_key:
+                if mid not in mat_index:
                     try:
                         surf = evaluate_material(fetch(mid))
                     except UnsupportedMaterial as e:
@@ -495,7 +498,7 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
                             raise UnsupportedMaterial(f"material asset {mid}: {e}") from None
                         unsupported[mid] = str(e)
                         surf = surface_default()
-                    mat_key[("m", mid)] = len(mats)
+                    mat_index[mid] = len(mats)
                     rec = material_record(surf)
                     if surf.get("pattern_even") is not None:
                         rec.view(np.uint32)[13] = PATTERN_CHECKER_XZ
@@ -503,7 +506,7 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
                         mats.append(rec)
                         rec = material_record(surf["pattern_even"])
                     mats.append(rec)
-                inst.append((xform, mat_key[key], mat_key[("m", mid)]))
+                inst.append((xform, mesh_index[key], mat_index[mid]))
                 placed.append((obj.get("name", f"object {i}"), mesh_id, sub, mid))
     if camera is None:
         raise AssetError("the scene has no camera")
