@@ -37,7 +37,9 @@ void bi_static_mesh_destroy(bi_static_mesh* mesh);
  * texcoords, texcoords2, indices, submeshes (8 values). */
 void bi_static_mesh_counts(const bi_static_mesh* mesh, uint64_t counts[8]);
 
-/* Submesh s as the pass's mesh input (pointers owned by `mesh`): triangles
+/* Submesh s as the pass's mesh input.  The pointers are owned by `mesh` and stay valid
+ * until bi_static_mesh_destroy or the next call for the same submesh; calls for one
+ * mesh from several threads must not overlap.  Triangles
  * indices[index_offset .. +n] + base_vertex with n = num_indices (0xFFFFFFFF: every
  * index from index_offset on), clamped to the index data and cut to whole triangles
  * as the bottom-level build counts them (src/graphics/graphics_manager.cpp:634);
