@@ -4,7 +4,7 @@ Every random draw comes from one PCG32 stream (O'Neill 2014) of the scene seed, 
 the order documented per generator, so the GPU path and the CPU restatement receive
 identical inputs.  Geometry is computed in float64 and stored as float32.
 
-Scene layout follows the pass's inputs (SURVEY inputs of PathTracingPass):
+Scene layout follows the inputs of PathTracingPass (src/renderer/pass/path_tracing.hpp:13-23):
   meshes     object-space positions/normals (float32 xyz) + uint32 triangle indices
   instances  3x4 row-major transforms (AccelerationStructureInstanceDesc.transform,
              include/bisemutum/rhi/accel.hpp:48-55), mesh id, material id
