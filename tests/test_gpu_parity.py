@@ -312,7 +312,7 @@ def test_gpu_converges_to_fp64_rough_c0(c0_scene):
 
 
 def test_gpu_converges_to_fp64_c0(c0_gpu, c0_oracle):
-    """The config-0 materials include near-mirror metals (roughness 0.05): a few pixels
+    """The box-room materials include near-mirror metals (roughness 0.05): a few pixels
     carry heavy-tailed (firefly) estimates, so the per-pixel bound is looser while the
     aggregate over the subset stays within 4 standard errors."""
     print("case: c0 convergence 1024 spp vs fp64 4096 spp", flush=True)

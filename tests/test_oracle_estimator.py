@@ -2,7 +2,7 @@
 
 Closed forms (a Lambertian plane under a constant sky and a directional light),
 sample-order invariances, and convergence of the FP32 transcription to the FP64
-oracle on the config-0 scene.
+oracle on the box-room scene (room + 64 icospheres).
 """
 import math
 
