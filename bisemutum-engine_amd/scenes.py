@@ -403,6 +403,27 @@ def templates_c3(count: int = 8, seed: int = 0xB15E0003):
     return [scene_sized(log_uniform_tris(rng, 1e4, 1e6), seed + 1 + i) for i in range(count)]
 
 
+def arrivals_c3(seconds: float = 20.0, rate: float = 200.0, templates: int = 8, seed: int = 0xA11C0000):
+    """Serving workload: Poisson arrivals at `rate` jobs/s over [0, seconds).  Per job
+    (draw order as written): exponential inter-arrival time, template index, resolution
+    (C3_RESOLUTIONS), spp (C3_SPP), orbit camera.  Returns (arrival_s, template, camera, spp)."""
+    rng = Pcg32(seed, 0)
+    jobs = []
+    t = 0.0
+    while True:
+        t += -math.log(1.0 - rng.uniform()) / rate
+        if t >= seconds:
+            return jobs
+        k = min(int(rng.uniform() * templates), templates - 1)
+        w, h = C3_RESOLUTIONS[min(int(rng.uniform() * 3), 2)]
+        spp = C3_SPP[min(int(rng.uniform() * 3), 2)]
+        jobs.append((t, k, camera_orbit(rng, w, h), spp))
+
+
+C3_MAX_BOUNCES = 7     # depth 6
+C3_SEED = 0x5EED0003
+
+
 def scene_c1(seed: int = 0xB15E0001):
     """Atrium of exactly 262,144 triangles.
 

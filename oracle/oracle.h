@@ -84,7 +84,7 @@ typedef struct ora_settings {
 
 ora_ctx* ora_create(const ora_scene_desc* desc);
 /* Rebuilds the BVH with `rounds` rounds of treelet restructuring after the LBVH (the
- * default is 2, as bi_pt_set_build_options; 0 keeps the Morton/Karras tree). */
+ * default is 3, as bi_pt_set_build_options; 0 keeps the Morton/Karras tree). */
 void ora_set_treelet_rounds(ora_ctx* ctx, uint32_t rounds);
 void ora_destroy(ora_ctx* ctx);
 uint32_t ora_num_tris(const ora_ctx* ctx);
@@ -127,6 +127,12 @@ int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
                  uint32_t sample_base, uint32_t spp, float* sum_rgb,
                  uint8_t* path_segments, uint8_t* shadow_bits,
                  uint64_t* ray_counts, int num_threads);
+
+/* ora_render32 restricted to a list of pixels (image indices, row-major): sum_rgb[3*i..]
+ * is pixel pixels[i]'s running sum, accumulated over samples [sample_base,
+ * sample_base+spp) in sample order, exactly as ora_render32 sums that pixel. */
+int ora_render32_pixels(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample_base,
+                        uint32_t spp, uint32_t n, const uint32_t* pixels, float* sum_rgb, int num_threads);
 
 /* Radiance-cache modes of bi_pt_render_nrc (src: the build's cache; no reference
  * counterpart), for a cache whose network output is the constant cache_rgb:

@@ -71,6 +71,9 @@ def lib():
         L.ora_render32.restype = C.c_int
         L.ora_render32.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, C.c_uint32,
                                    _f32p, _u8p, _u8p, _u64p, C.c_int]
+        L.ora_render32_pixels.restype = C.c_int
+        L.ora_render32_pixels.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, C.c_uint32,
+                                          C.c_uint32, _u32p, _f32p, C.c_int]
         L.ora_render32_cached.restype = C.c_int
         L.ora_render32_cached.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, _f32p,
                                           C.c_uint32, C.c_uint32, _f32p, C.c_int]
@@ -242,6 +245,18 @@ class OracleScene:
             out["segments"] = segs
             out["shadow_bits"] = bits
         return out
+
+    def render32_pixels(self, cam, max_bounces, seed, spp, pixels, sample_base=0, ray_length=1000.0, flags=0,
+                        threads=0):
+        """Running sums [len(pixels), 3] of the listed pixels (row-major image indices) over
+        samples [sample_base, sample_base + spp), as render32 sums them."""
+        pixels = np.ascontiguousarray(pixels, np.uint32)
+        sums = np.zeros((len(pixels), 3), np.float32)
+        rc = lib().ora_render32_pixels(self.h, C.byref(camera_struct(cam)),
+                                       C.byref(settings_struct(max_bounces, seed, ray_length, flags)), sample_base,
+                                       spp, len(pixels), ptr(pixels, _u32p), ptr(sums, _f32p), threads)
+        assert rc == 0
+        return sums
 
     def render32_cached(self, cam, max_bounces, seed, spp, cache_bounce, cache_rgb, flags=0, ray_length=1000.0,
                         threads=0):

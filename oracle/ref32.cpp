@@ -1233,6 +1233,34 @@ int render32_impl(const ora_ctx* ctx, const ora_camera* cam, const ora_settings*
 }
 }  // namespace
 
+int ora_render32_pixels(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample_base,
+                        uint32_t spp, uint32_t n, const uint32_t* pixels, float* sum_rgb, int num_threads) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    const uint64_t npix = (uint64_t)cam->width * cam->height;
+    if (npix == 0) return -1;
+    for (uint32_t i = 0; i < n; ++i)
+        if (pixels[i] >= npix) return -1;
+    if (num_threads <= 0) num_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const uint32_t p = pixels[i];
+            F3 sum = f3(sum_rgb[3 * (size_t)i], sum_rgb[3 * (size_t)i + 1], sum_rgb[3 * (size_t)i + 2]);
+            for (uint32_t sl = 0; sl < spp; ++sl) sum = sum + trace_path(s, *cam, *st, p, sample_base + sl).radiance;
+            sum_rgb[3 * (size_t)i] = sum.x;
+            sum_rgb[3 * (size_t)i + 1] = sum.y;
+            sum_rgb[3 * (size_t)i + 2] = sum.z;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < num_threads; ++k) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    return 0;
+}
+
 void ora_trace(const ora_ctx* ctx, uint32_t n, const float* rays, int any_hit, float* hit_t,
                uint32_t* hit_leaf, float* hit_u, float* hit_v) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);

@@ -1,8 +1,8 @@
 """Batched rendering (bi_pt_render_batch): several contexts' jobs share one set of waves.
 
 Each job must come out bit for bit as the same job rendered alone with bi_pt_render
-(which the parity tests pin against the CPU restatement), whatever else is in the
-batch, with or without the hipGraph replay, and under any tuning the autotuner picks.
+and as the CPU restatement renders it, whatever else is in the batch, with or without
+the hipGraph replay, and under any tuning the autotuner picks.
 """
 import numpy as np
 import pytest
@@ -62,8 +62,22 @@ def assert_same(a, b):
         assert np.array_equal(ia.view(np.uint32), ib.view(np.uint32))
 
 
-def test_batch_equals_solo(c0_scene):
-    print("case: batch of 4 jobs (c0 64x48, plane, c0 33x17 depth 8, empty scene) vs solo", flush=True)
+def assert_oracle(specs, got, oracle_lib):
+    """Each job's running sums equal the CPU restatement's (the scene-less job sees the
+    constant sky on every sample)."""
+    for (sc, cam, kw), (sums, _) in zip(specs, got):
+        if sc.num_tris == 0:
+            sky = np.asarray(sc.sky_rgba[0, 0, :3], np.float32) * np.asarray(sc.sky_color, np.float32)
+            want = np.tile(kw["spp"] * sky, (cam.width * cam.height, 1))
+            np.testing.assert_allclose(sums, want, rtol=1e-6)
+            continue
+        want = oracle_lib.OracleScene(sc).render32(cam, kw["max_bounces"], kw["seed"], spp=kw["spp"],
+                                                   flags=kw.get("flags", 0))["sums"]
+        assert np.array_equal(sums.view(np.uint32), want.view(np.uint32)), sc.name
+
+
+def test_batch_equals_solo(c0_scene, oracle_lib):
+    print("case: batch of 4 jobs (c0 64x48, plane, c0 33x17 depth 8, empty scene) vs solo and oracle", flush=True)
     specs = job_specs(c0_scene)
     solo = solo_sums(specs)
     assert np.all(np.isfinite(solo[0][0])) and solo[0][0].max() > 0
@@ -71,6 +85,7 @@ def test_batch_equals_solo(c0_scene):
     ts = tracers(specs)
     got = batch_sums(specs, b, ts)
     assert_same(got, solo)
+    assert_oracle(specs, got, oracle_lib)
     # small waves: the jobs' samples split over many waves and mix differently
     b.set_tuning(wave_paths=5000)
     ts2 = tracers(specs)

@@ -48,7 +48,7 @@ def compare_bvh4(gpu, ora):
 
 
 def test_bvh_bit_exact_c0(c0_gpu, c0_oracle):
-    print("case: c0 bvh (LBVH + 2 rounds of treelet restructuring)", flush=True)
+    print("case: c0 bvh (LBVH + the default 3 rounds of treelet restructuring)", flush=True)
     compare_bvh(c0_gpu.bvh(), c0_oracle.bvh())
     compare_bvh4(c0_gpu, c0_oracle)
 
