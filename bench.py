@@ -83,8 +83,12 @@ def main():
     ap.add_argument("--nrc-warmup", type=int, default=48, help="frames of online training before the cache's RMSE")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-calls", type=int, default=3)
-    ap.add_argument("--serve-seconds", type=float, default=2.0,
-                    help="arrival window of the serving leg (0 skips it; 20 = the full 4,000-job run)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of the serving leg (0 skips it; 20 s = about 4,100 jobs)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after one warm-up)")
+    ap.add_argument("--profile-nrc", action="store_true",
+                    help="only run the radiance-cache frames that profiles/kernels_nrc_c1_1080p.md profiles "
+                         "(8 frames at 1 spp of online training, then 4 at 16 spp) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy first (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -99,6 +103,9 @@ def main():
     import bi_pt
     import scenes
 
+    if args.profile_nrc:
+        profile_nrc()
+        return
     cfg = scenes.CONFIGS[args.config]
     t = time.perf_counter()
     scene = cfg["scene"]()
@@ -217,7 +224,7 @@ def main():
     if args.serve_seconds > 0:
         result["serving"] = serve_leg(args.serve_seconds)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_leg(scene, cfg)
+        result["cpu_baseline"] = cpu_leg(scene, cfg, args.cpu_runs)
     pt.close()
     print(json.dumps(result), flush=True)
 
@@ -316,19 +323,8 @@ def serve_leg(seconds, rate=200.0):
         tr = bi_pt.PathTracer()
         tr.set_scene(sc)
         tracers.append(tr)
-    rng = scenes.Pcg32(0xA11C0000, 0)
-    jobs = []
-    t_arr = 0.0
-    while True:
-        t_arr += -math.log(1.0 - rng.uniform()) / rate
-        if t_arr >= seconds:
-            break
-        k = min(int(rng.uniform() * len(templates)), len(templates) - 1)
-        w, h = scenes.C3_RESOLUTIONS[min(int(rng.uniform() * 3), 2)]
-        spp = scenes.C3_SPP[min(int(rng.uniform() * 3), 2)]
-        cam = scenes.camera_orbit(rng, w, h)
-        jobs.append((t_arr, k, cam, spp))
-    kw = dict(max_bounces=7, seed=0x5EED0003)
+    jobs = scenes.arrivals_c3(seconds, rate, len(templates))
+    kw = dict(max_bounces=scenes.C3_MAX_BOUNCES, seed=scenes.C3_SEED)
     log(f"serve leg: {len(jobs)} jobs over {seconds}s, templates {[s.num_tris for s in templates]}")
 
     srv = bi_pt.Server(hbm_budget_bytes=96 << 30, spp_per_round=4)
@@ -499,27 +495,84 @@ def nrc_leg(pt, cam, cfg, ref, plain, frames, warmup):
     return out
 
 
-def cpu_leg(scene, cfg):
-    """The FP32 CPU restatement on this host, a bounded sample of the same workload."""
+def host_threads():
+    """Worker threads for the CPU baseline: the CPUs this process may run on, at most 16
+    (the host share that goes with one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(scene, cfg, runs):
+    """The FP32 CPU restatement on this host, a bounded sample of the same workload: the
+    full frame at 4 of the 16 spp, one warm-up run, then the median of `runs` runs; the
+    restatement's scene set-up (flattening and LBVH build) is timed on its own."""
+    import numpy as np
+
     import oracle
-    import scenes
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     w, h = cfg["width"], cfg["height"]
     cam = cfg["camera"](w, h)
     t = time.perf_counter()
     ors = oracle.OracleScene(scene)
     build_s = time.perf_counter() - t
-    spp = max(1, cfg["spp"] // 2)
-    t = time.perf_counter()
-    out = ors.render32(cam, cfg["max_bounces"], cfg["seed"], spp=spp, threads=threads)
-    secs = time.perf_counter() - t
-    rays = int(out["counts"].sum())
+    spp = 4
+    secs, rays = [], 0
+    for k in range(runs + 1):
+        t = time.perf_counter()
+        out = ors.render32(cam, cfg["max_bounces"], cfg["seed"] + k, spp=spp, threads=threads)
+        dt = time.perf_counter() - t
+        if k > 0:
+            secs.append(dt)
+            rays += int(out["counts"].sum())
+        log(f"cpu leg: run {k} {'(warm-up) ' if k == 0 else ''}{dt:.2f}s on {threads} threads")
     ors.close()
-    log(f"cpu leg: {rays} rays in {secs:.2f}s on {threads} threads")
-    return {"value": round(rays / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"same scene and camera, {w}x{h}, {spp} of the {cfg['spp']} spp, depth "
-                      f"{cfg['max_bounces'] - 1}; render only, LBVH build {build_s:.2f}s excluded",
-            "seconds": round(secs, 2)}
+    med = float(np.median(secs))
+    rate = rays / sum(secs) / 1e6   # rays per run vary by a few per mille; the rate of the median run
+    per_run = rays / len(secs)
+    value = per_run / med / 1e6
+    log(f"cpu leg: median {med:.2f}s, {value:.2f} Mrays/s (mean {rate:.2f})")
+    return {"value": round(value, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"same scene and camera, {w}x{h}, {spp} of the {cfg['spp']} spp per run, depth "
+                      f"{cfg['max_bounces'] - 1}; median of {runs} runs after 1 warm-up; render only (the "
+                      f"restatement's flattening + LBVH build, {build_s:.2f}s, timed apart)",
+            "cpu_model": cpu_model(), "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(),
+            "median_run_s": round(med, 3), "runs_s": [round(x, 3) for x in secs], "build_s": round(build_s, 3)}
+
+
+def profile_nrc():
+    """Radiance-cache frames on the 1920x1080 atrium (262,144 triangles, depth 8):
+    8 frames of bi_pt_render_nrc at 1 spp (online training), then 4 at 16 spp."""
+    import torch
+
+    import bi_pt
+    import scenes
+    cfg = scenes.CONFIGS["c1"]
+    cam = cfg["camera"]()
+    pt = bi_pt.PathTracer()
+    pt.set_scene(cfg["scene"]())
+    nrc = bi_pt.Nrc(pt, train_samples=65536, train_steps=4, cache_bounce=2, seed=1)
+    for k in range(8):
+        nrc.render(cam, spp=1, max_bounces=cfg["max_bounces"], seed=100 + k)
+    for k in range(4):
+        nrc.render(cam, spp=16, max_bounces=cfg["max_bounces"], seed=200 + k)
+    torch.cuda.synchronize()
+    log(f"cache frames done: {nrc.stats()}")
+    nrc.close()
+    pt.close()
 
 
 if __name__ == "__main__":

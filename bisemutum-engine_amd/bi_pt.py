@@ -38,7 +38,7 @@ EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene
            "bi_pt_nrc_create", "bi_pt_nrc_destroy", "bi_pt_nrc_last_error", "bi_pt_render_nrc",
            "bi_pt_nrc_get_stats", "bi_pt_nrc_get_params", "bi_pt_nrc_set_params", "bi_pt_nrc_set_bounds",
            "bi_pt_nrc_infer", "bi_pt_nrc_train", "bi_pt_nrc_debug_gradient", "bi_pt_nrc_debug_samples",
-           "bi_pt_set_build_options"]
+           "bi_pt_set_build_options", "bi_pt_debug_inject_error", "bi_pt_server_debug_inject_error"]
 NRC_PARAMS = 17408
 NRC_FREEZE = 1
 
@@ -197,6 +197,8 @@ def lib():
         L.bi_pt_nrc_debug_samples.argtypes = [vp, f32p, f32p, u32]
         L.bi_pt_trace_closest.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
         L.bi_pt_trace_occlusion.argtypes = [vp, u32, vp, C.c_float, vp, vp]
+        L.bi_pt_debug_inject_error.argtypes = [vp, u32]
+        L.bi_pt_server_debug_inject_error.argtypes = [vp, u32]
         for name in EXPORTS:
             if name not in ("bi_pt_last_error", "bi_pt_debug_num_tris", "bi_pt_batch_last_error",
                             "bi_pt_server_last_error", "bi_pt_server_clock", "bi_pt_nrc_last_error"):
@@ -332,6 +334,10 @@ class PathTracer:
 
     def reset(self):
         self._check(lib().bi_pt_reset_accumulation(self.h))
+
+    def inject_error(self, flags):
+        """Raise device error flags from the host (reported by the next stats())."""
+        self._check(lib().bi_pt_debug_inject_error(self.h, flags))
 
     def set_build_options(self, treelet_rounds=3):
         """Rounds of treelet restructuring after the LBVH (0: the Morton/Karras tree)."""
@@ -585,6 +591,10 @@ class Server:
         out = np.zeros((npix, 4), np.float32)
         self._check(lib().bi_pt_server_take_result(self.h, job_id, _p(out), out.size))
         return out
+
+    def inject_error(self, flags):
+        """Raise device error flags from the host: the next round fails its jobs."""
+        self._check(lib().bi_pt_server_debug_inject_error(self.h, flags))
 
     def stats(self):
         s = ServerStats()

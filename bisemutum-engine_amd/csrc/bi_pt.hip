@@ -460,6 +460,12 @@ bi_pt_status bi_pt_get_stats(const bi_pt_ctx* cctx, bi_pt_stats* out) {
     return st;
 }
 
+bi_pt_status bi_pt_debug_inject_error(bi_pt_ctx* ctx, uint32_t flags) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return engine_inject_error(ctx, ctx->eng, flags);
+}
+
 bi_pt_status bi_pt_debug_get_bvh(bi_pt_ctx* ctx, uint32_t* morton, uint32_t* order, int32_t* children,
                                  float* child_bounds, int32_t* parents, float* centroid_bounds) {
     if (ctx && ctx->scene_src) ctx = ctx->scene_src;   // a view is inspected through its scene

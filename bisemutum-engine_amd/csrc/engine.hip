@@ -74,6 +74,19 @@ void engine_free(Owner* own, Engine& eng) {
     eng.ready = false;
 }
 
+std::string device_error_text(uint32_t flags) {
+    std::string m = "device check failed:";
+    if (flags & 1u) m += " traversal stack overflow;";
+    if (flags & 4u) m += " refit wait limit;";
+    if (flags & 8u) m += " traversal step bound exceeded;";
+    if (flags & 16u) m += " treelet start list overflow;";
+    if (flags & 32u) m += " shade look-back wait limit;";
+    if (flags & ~(1u | 4u | 8u | 16u | 32u)) m += " flags " + std::to_string(flags) + ";";
+    return m;
+}
+
+// The error word is sticky: render calls never clear it, so a flag raised by any call is
+// reported by the next check, which clears it.
 bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out) {
     BI_HIP_TRY(own, hipDeviceSynchronize());
     uint32_t hc[kCntSlots];
@@ -84,13 +97,42 @@ bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out) {
     eng.stats.rays_light_shadow = r[2];
     eng.stats.rays_sky_shadow = r[3];
     if (out) *out = eng.stats;
-    if (hc[kCntError])
-        return fail(own, BI_PT_ERROR_DEVICE_CHECK,
-                    std::string("device check failed: ") + ((hc[kCntError] & 1u) ? "traversal stack overflow " : "") +
-                        ((hc[kCntError] & 4u) ? "refit wait limit " : "") +
-                        ((hc[kCntError] & 8u) ? "traversal step bound exceeded " : "") +
-                        ((hc[kCntError] & 16u) ? "treelet start list overflow " : "") +
-                        ((hc[kCntError] & 32u) ? "shade look-back wait limit" : ""));
+    if (hc[kCntError]) {
+        BI_HIP_TRY(own, hipMemset(eng.counters.p + kCntError, 0, sizeof(uint32_t)));
+        return fail(own, BI_PT_ERROR_DEVICE_CHECK, device_error_text(hc[kCntError]));
+    }
+    return BI_PT_OK;
+}
+
+bi_pt_status engine_take_errors(Owner* own, Engine& eng, hipStream_t s) {
+    if (!eng.ready) return BI_PT_OK;
+    uint32_t flags = 0;
+    BI_HIP_TRY(own, hipMemcpyAsync(&flags, eng.counters.p + kCntError, sizeof flags, hipMemcpyDeviceToHost, s));
+    BI_HIP_TRY(own, hipStreamSynchronize(s));
+    if (flags == 0) return BI_PT_OK;
+    launch_fill_u32(eng.counters.p + kCntError, 0u, 1u, s);
+    BI_HIP_TRY(own, hipStreamSynchronize(s));
+    return fail(own, BI_PT_ERROR_DEVICE_CHECK, device_error_text(flags));
+}
+
+bi_pt_status engine_inject_error(Owner* own, Engine& eng, uint32_t flags) {
+    BI_TRY(engine_init(own, eng));
+    BI_HIP_TRY(own, hipDeviceSynchronize());
+    uint32_t w = 0;
+    BI_HIP_TRY(own, hipMemcpy(&w, eng.counters.p + kCntError, sizeof w, hipMemcpyDeviceToHost));
+    w |= flags;
+    BI_HIP_TRY(own, hipMemcpy(eng.counters.p + kCntError, &w, sizeof w, hipMemcpyHostToDevice));
+    return BI_PT_OK;
+}
+
+bi_pt_status validate_job(Owner* own, const bi_pt_camera* c, const bi_pt_settings* st) {
+    if (!c || !st) return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "null camera or settings");
+    if (c->width == 0 || c->height == 0 || st->spp == 0)
+        return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "empty image or zero samples");
+    if ((uint64_t)c->width * c->height > (1ull << 26))
+        return fail(own, BI_PT_ERROR_UNSUPPORTED, "image larger than 2^26 pixels");
+    if (!(st->ray_length > 0.0f) || !(st->ray_length <= kShadowTMax))
+        return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "ray_length must be positive and at most 1.7e38");
     return BI_PT_OK;
 }
 
@@ -183,12 +225,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         if (!S->has_scene) return fail(own, BI_PT_ERROR_NO_SCENE, "no scene set (job " + std::to_string(j) + ")");
         const bi_pt_camera* c = J.camera;
         const bi_pt_settings* st = J.st;
-        if (c->width == 0 || c->height == 0 || st->spp == 0)
-            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "empty image or zero samples");
+        if (bi_pt_status e = validate_job(own, c, st); e != BI_PT_OK)
+            return fail(own, e, own->err + " (job " + std::to_string(j) + ")");
         const uint64_t npix64 = (uint64_t)c->width * c->height;
-        if (npix64 > (1ull << 26)) return fail(own, BI_PT_ERROR_UNSUPPORTED, "image larger than 2^26 pixels");
-        if (!(st->ray_length > 0.0f) || !(st->ray_length <= kShadowTMax))
-            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "ray_length must be positive and at most 1.7e38");
         if (!train && ctx->acc_count + st->spp > 0xffffffffull)
             return fail(own, BI_PT_ERROR_UNSUPPORTED, "more than 2^32 accumulated samples");
         Prep& p = prep[j];
@@ -422,13 +461,14 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     }
 
     // every call, outside any graph: the counters and the LBVH builds.  Queue lengths and
-    // error flag (the ray statistics accumulate) and the persistent lanes' part counters
-    // start at zero.  The scenes' LBVHs are independent: build b goes to s (b % lanes == 0)
+    // the persistent lanes' part counters start at zero (the ray statistics accumulate and
+    // the error word is sticky until a check reads it).  The scenes' LBVHs are independent: build b goes to s (b % lanes == 0)
     // or to build stream b % lanes - 1, forked from s and joined back into s before
     // anything reads them.
     {
         uint32_t* cnt = eng.counters.p;
-        launch_fill_u32(cnt, 0u, kCntRays, s);
+        launch_fill_u32(cnt, 0u, kCntError, s);
+        launch_fill_u32(cnt + kCntError + 1, 0u, kCntRays - kCntError - 1, s);
         launch_fill_u32(cnt + kCntTraceWork, 0u, kCntSlots - kCntTraceWork, s);
         if (timing) BI_HIP_TRY(own, hipEventRecord(eng.ev[0], s));
         std::vector<uint32_t> builds;
@@ -462,6 +502,17 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     // graph replay: the same launch sequence with the same buffers (the plan's contents,
     // uploaded above, carry everything that changes between calls)
     const bool graph = graph_owner && graph_owner->use_graph && !timing && !record;
+    // a capture that an error leaves open is ended and its graph dropped, so the caller's
+    // stream is usable again
+    struct CaptureGuard {
+        hipStream_t s = nullptr;
+        ~CaptureGuard() {
+            if (!s) return;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } capture;
     std::vector<uint64_t> key;
     if (graph) {
         key = {n, cap, lds_stack, refill, parts, block, (uint64_t)(uintptr_t)eng.plan.p, eng.wave_cap, eng.wave_slots,
@@ -481,6 +532,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         if (graph_owner->graph) { (void)hipGraphDestroy(graph_owner->graph); graph_owner->graph = nullptr; }
         graph_owner->graph_key.clear();
         BI_HIP_TRY(own, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        capture.s = s;
     }
     {
         uint32_t* cnt = eng.counters.p;
@@ -660,6 +712,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     }
     if (graph) {
         hipGraph_t g = nullptr;
+        capture.s = nullptr;
         BI_HIP_TRY(own, hipStreamEndCapture(s, &g));
         graph_owner->graph = g;
         BI_HIP_TRY(own, hipGraphInstantiate(&graph_owner->graph_exec, g, nullptr, nullptr, 0));

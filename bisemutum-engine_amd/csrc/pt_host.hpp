@@ -123,8 +123,18 @@ bi_pt_status engine_init(Owner* own, Engine& eng);
 // `sh_mult` shadow rays per path (what ensure_wave_buffers allocates).
 uint64_t engine_bytes(uint32_t cap, uint32_t slots, uint32_t sh_mult);
 void engine_free(Owner* own, Engine& eng);
-// Error flags of the engine's last calls (synchronises the device).
+// Statistics and the error flags raised since the last check (synchronises the device;
+// clears the flags it reports).
 bi_pt_status engine_check(Owner* own, Engine& eng, bi_pt_stats* out);
+// The error flags raised since the last check, after the work on stream s (synchronises
+// s; clears the flags it reports).
+bi_pt_status engine_take_errors(Owner* own, Engine& eng, hipStream_t s);
+std::string device_error_text(uint32_t flags);
+// ORs flags into the error word from the host (error-path tests).
+bi_pt_status engine_inject_error(Owner* own, Engine& eng, uint32_t flags);
+// Rejects a camera / settings pair that a render call would reject (image size, samples,
+// ray length).
+bi_pt_status validate_job(Owner* own, const bi_pt_camera* c, const bi_pt_settings* st);
 
 }  // namespace bipt
 

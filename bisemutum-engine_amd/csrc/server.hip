@@ -32,6 +32,8 @@ struct SrvJob {
     uint32_t state = BI_PT_JOB_WAITING;
     uint32_t spp_done = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
+    bi_pt_status failed = BI_PT_OK;   // BI_PT_JOB_FAILED: the status of the round that failed it
+    std::string why;
     bi_pt_ctx* view = nullptr;
     DevBuf<float> out;            // RGBA32F running mean
     uint64_t bytes = 0;           // charged to the budget while admitted
@@ -156,8 +158,10 @@ bi_pt_status bi_pt_server_add_scene(bi_pt_server* s, bi_pt_ctx* scene, uint32_t*
 bi_pt_status bi_pt_server_submit(bi_pt_server* s, const bi_pt_request* r, uint64_t* job_id) {
     if (!s || !r || !job_id) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (r->scene_id >= s->scenes.size()) return fail(s, BI_PT_ERROR_INVALID_ARGUMENT, "unknown scene id");
-    if (r->camera.width == 0 || r->camera.height == 0 || r->settings.spp == 0)
-        return fail(s, BI_PT_ERROR_INVALID_ARGUMENT, "empty image or zero samples");
+    // everything a round would reject is rejected here, so one job cannot fail the others' rounds
+    BI_TRY(validate_job(s, &r->camera, &r->settings));
+    if (r->settings.flags & BI_PT_FLAG_OUTPUT_RGBA16F)
+        return fail(s, BI_PT_ERROR_UNSUPPORTED, "the server returns RGBA32F images (BI_PT_FLAG_OUTPUT_RGBA16F)");
     SrvJob j;
     j.scene = r->scene_id;
     j.cam = r->camera;
@@ -211,9 +215,34 @@ bi_pt_status bi_pt_server_step(bi_pt_server* s, bi_pt_stream stream, uint32_t* a
             st[k].accumulate = j.spp_done > 0 ? 1u : 0u;
             bj[k] = bi_pt_job{j.view, &j.cam, &st[k], j.out.p, nullptr};
         }
-        if (bi_pt_status e = bi_pt_render_batch(s->batch, bj.data(), (uint32_t)bj.size(), stream); e != BI_PT_OK)
-            return fail(s, e, std::string("round: ") + bi_pt_batch_last_error(s->batch));
-        BI_HIP_TRY(s, hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+        // a round that fails (a HIP error, or a kernel's internal-limit flag) fails its jobs:
+        // they leave with state BI_PT_JOB_FAILED and their memory is freed; waiting jobs and
+        // later rounds go on
+        const hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+        bi_pt_status e = bi_pt_render_batch(s->batch, bj.data(), (uint32_t)bj.size(), stream);
+        std::string why = e != BI_PT_OK ? std::string("round: ") + bi_pt_batch_last_error(s->batch) : "";
+        if (e == BI_PT_OK && hipStreamSynchronize(hs) != hipSuccess) {
+            e = BI_PT_ERROR_HIP;
+            why = "round: stream synchronisation failed";
+        }
+        if (e == BI_PT_OK && (e = engine_take_errors(s->batch, s->batch->eng, hs)) != BI_PT_OK)
+            why = "round: " + s->batch->err;
+        if (e != BI_PT_OK) {
+            (void)hipStreamSynchronize(hs);
+            const double t = s->now();
+            for (uint64_t id : s->running) {
+                SrvJob& j = s->jobs[id];
+                release(s, j);
+                j.state = BI_PT_JOB_FAILED;
+                j.t_done = t;
+                j.failed = e;
+                j.why = why;
+            }
+            s->running.clear();
+            ++s->rounds;
+            if (active) *active = (uint32_t)s->waiting.size();
+            return fail(s, e, why);
+        }
         const double t = s->now();
         ++s->rounds;
         std::vector<uint64_t> still;
@@ -251,6 +280,12 @@ bi_pt_status bi_pt_server_take_result(bi_pt_server* s, uint64_t id, float* host_
     auto it = s->jobs.find(id);
     if (it == s->jobs.end()) return fail(s, BI_PT_ERROR_INVALID_ARGUMENT, "unknown job id");
     SrvJob& j = it->second;
+    if (j.state == BI_PT_JOB_FAILED) {   // nothing to copy: report why and forget the job
+        const bi_pt_status e = j.failed;
+        const std::string why = j.why;
+        s->jobs.erase(it);
+        return fail(s, e, "job failed: " + why);
+    }
     if (j.state != BI_PT_JOB_DONE) return fail(s, BI_PT_ERROR_INVALID_ARGUMENT, "job not finished");
     const uint64_t n = (uint64_t)4 * j.cam.width * j.cam.height;
     if (host_rgba) {
@@ -260,6 +295,14 @@ bi_pt_status bi_pt_server_take_result(bi_pt_server* s, uint64_t id, float* host_
     }
     release(s, j);
     s->jobs.erase(it);
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_server_debug_inject_error(bi_pt_server* s, uint32_t flags) {
+    if (!s) return BI_PT_ERROR_INVALID_ARGUMENT;
+    BI_HIP_TRY(s, hipSetDevice(s->device));
+    if (bi_pt_status e = engine_inject_error(s->batch, s->batch->eng, flags); e != BI_PT_OK)
+        return fail(s, e, s->batch->err);
     return BI_PT_OK;
 }
 

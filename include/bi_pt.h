@@ -217,6 +217,9 @@ typedef struct bi_pt_build_options {
     uint32_t _reserved[3];
 } bi_pt_build_options;
 bi_pt_status bi_pt_set_build_options(bi_pt_ctx* ctx, const bi_pt_build_options* options);
+/* Statistics; returns BI_PT_ERROR_DEVICE_CHECK when a kernel of any call on this context
+ * since the last bi_pt_get_stats raised an internal-limit flag (the flags are kept until
+ * this call reports them, then cleared). */
 bi_pt_status bi_pt_get_stats(const bi_pt_ctx* ctx, bi_pt_stats* out);
 
 /* ---- batched rendering (no reference counterpart: the reference renders one view per
@@ -252,7 +255,8 @@ bi_pt_status bi_pt_batch_get_tuning(const bi_pt_batch* batch, bi_pt_tuning* tuni
 /* Renders every job; stream-ordered.  BI_PT_FLAG_REUSE_BVH in a job's flags keeps that
  * context's built LBVH; BI_PT_FLAG_TIMING / RECORD_QUEUES are ignored here. */
 bi_pt_status bi_pt_render_batch(bi_pt_batch* batch, const bi_pt_job* jobs, uint32_t num_jobs, bi_pt_stream stream);
-/* Ray statistics of the batch (all its calls); checks the device error flags. */
+/* Ray statistics of the batch (all its calls) and the device error flags raised by any of
+ * its calls since the last check (which clears them). */
 bi_pt_status bi_pt_batch_get_stats(bi_pt_batch* batch, bi_pt_stats* out);
 /* Picks the batch's tuning for these jobs by timing candidate knob settings (wave
  * capacity, traversal block shape, LDS stack depth, refill threshold) on them, `rounds` calls each; writes the
@@ -281,9 +285,13 @@ typedef struct bi_pt_request {
     uint32_t _pad;
     double arrival_s;              /* arrival time on the server clock; < 0 = now */
     bi_pt_camera camera;
-    bi_pt_settings settings;       /* spp = the job's total samples; accumulate and queue knobs ignored */
+    bi_pt_settings settings;       /* spp = the job's total samples; accumulate and queue knobs ignored;
+                                      checked at submission as bi_pt_render checks them, and
+                                      BI_PT_FLAG_OUTPUT_RGBA16F is rejected (results are RGBA32F) */
 } bi_pt_request;
-enum { BI_PT_JOB_WAITING = 0, BI_PT_JOB_RUNNING = 1, BI_PT_JOB_DONE = 2 };
+/* A job whose round failed (a HIP error or a kernel's internal-limit flag) leaves with
+ * BI_PT_JOB_FAILED; bi_pt_server_step returns that round's error, the other jobs go on. */
+enum { BI_PT_JOB_WAITING = 0, BI_PT_JOB_RUNNING = 1, BI_PT_JOB_DONE = 2, BI_PT_JOB_FAILED = 3 };
 typedef struct bi_pt_job_status {
     uint32_t state;
     uint32_t spp_done;
@@ -307,7 +315,8 @@ bi_pt_status bi_pt_server_submit(bi_pt_server* server, const bi_pt_request* requ
  * *active (nullable) = jobs still waiting or running. */
 bi_pt_status bi_pt_server_step(bi_pt_server* server, bi_pt_stream stream, uint32_t* active);
 bi_pt_status bi_pt_server_poll(bi_pt_server* server, uint64_t job_id, bi_pt_job_status* status);
-/* Copies a finished job's RGBA32F image to host memory (nullable: discard) and frees the job. */
+/* Copies a finished job's RGBA32F image to host memory (nullable: discard) and frees the job.
+ * For a failed job: returns the status of the round that failed it and frees the job. */
 bi_pt_status bi_pt_server_take_result(bi_pt_server* server, uint64_t job_id, float* host_rgba,
                                       uint64_t capacity_floats);
 bi_pt_status bi_pt_server_get_stats(bi_pt_server* server, bi_pt_server_stats* out);
@@ -419,6 +428,11 @@ bi_pt_status bi_pt_debug_get_sums(bi_pt_ctx* ctx, float* sums_rgb);
  * (o.xyz, tmin, d.xyz, tmax); closest -> (t, leaf, u, v), any-hit -> leaf = occluded. */
 bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, int any_hit, float* hit_t,
                                uint32_t* hit_leaf, float* hit_u, float* hit_v);
+/* Raises device error flags from the host (tests of the error paths: the flags a kernel
+ * would raise on an internal-limit violation, without making any kernel violate one).
+ * The context's next check (bi_pt_get_stats) or the server's next round reports them. */
+bi_pt_status bi_pt_debug_inject_error(bi_pt_ctx* ctx, uint32_t flags);
+bi_pt_status bi_pt_server_debug_inject_error(bi_pt_server* server, uint32_t flags);
 
 #ifdef __cplusplus
 }

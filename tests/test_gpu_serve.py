@@ -1,9 +1,10 @@
 """Serving loop (bi_pt_server) and scene views (bi_pt_create_view).
 
 Each job finished by the server must equal, bit for bit, the same camera and settings
-rendered alone by bi_pt_render on a context holding its scene -- whatever other jobs it
-shared its rounds with, however its samples were split over rounds, and whenever it was
-admitted.  Admission must respect the HBM budget.
+rendered alone by bi_pt_render on a context holding its scene, and the CPU
+restatement's image -- whatever other jobs it shared its rounds with, however its
+samples were split over rounds, and whenever it was admitted.  Admission must respect
+the HBM budget.
 """
 import numpy as np
 import pytest
@@ -45,8 +46,8 @@ def test_view_renders_like_its_scene(c0_scene):
     owner.close()
 
 
-def test_server_results_equal_solo(c0_scene):
-    print("case: server, 9 jobs over 2 scenes, 3 spp per round", flush=True)
+def test_server_results_equal_solo(c0_scene, oracle_lib):
+    print("case: server, 9 jobs over 2 scenes, 3 spp per round, vs solo and oracle", flush=True)
     plane = plane_scene()
     owners = []
     for sc in (c0_scene, plane):
@@ -55,6 +56,7 @@ def test_server_results_equal_solo(c0_scene):
         owners.append(t)
     srv = bi_pt.Server(spp_per_round=3, max_jobs_per_round=4)
     sids = [srv.add_scene(t) for t in owners]
+    oracles = [oracle_lib.OracleScene(sc) for sc in (c0_scene, plane)]
     specs = []
     rng = np.random.default_rng(5)
     for k in range(9):
@@ -81,6 +83,10 @@ def test_server_results_equal_solo(c0_scene):
         got = srv.take_result(jid, cam.width * cam.height)
         want = solo_image((c0_scene, plane)[which], cam, **kw)
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (jid, kw)
+        # the restatement's running mean: sums / spp, both correctly rounded
+        sums = oracles[which].render32(cam, kw["max_bounces"], kw["seed"], spp=kw["spp"], flags=kw["flags"])["sums"]
+        mean = sums / np.float32(kw["spp"])
+        assert np.array_equal(got[:, :3].view(np.uint32), mean.view(np.uint32)), (jid, kw)
     srv.close()
     for t in owners:
         t.close()
@@ -126,3 +132,87 @@ def test_server_admission_by_budget(c0_scene):
         srv.submit(sid, scenes.camera_c0(8192, 8192), spp=1)   # can never fit
     srv.close()
     owner.close()
+
+
+def test_server_rejects_what_a_round_would_reject(c0_scene):
+    """Jobs a render would refuse are refused at submission (a zero-initialised ray length,
+    an oversized image, the RGBA16F output flag), so they never fail the other jobs'
+    rounds; the valid job next to them completes and equals its solo render."""
+    print("case: server submission checks next to a valid job", flush=True)
+    owner = bi_pt.PathTracer()
+    owner.set_scene(c0_scene)
+    srv = bi_pt.Server(spp_per_round=2)
+    sid = srv.add_scene(owner)
+    cam = scenes.camera_c0(32, 24)
+    good = srv.submit(sid, cam, spp=3, max_bounces=4, seed=77)
+    with pytest.raises(bi_pt.PtError) as e:
+        srv.submit(sid, cam, spp=2, max_bounces=4, seed=78, ray_length=0.0)
+    assert e.value.status == 1
+    with pytest.raises(bi_pt.PtError) as e:
+        srv.submit(sid, scenes.camera_c0(8192, 8193), spp=1)
+    assert e.value.status == 6
+    with pytest.raises(bi_pt.PtError) as e:
+        srv.submit(sid, cam, spp=2, flags=bi_pt.FLAG_OUTPUT_RGBA16F)
+    assert e.value.status == 6
+    while srv.step():
+        pass
+    assert srv.poll(good)["state"] == 2
+    got = srv.take_result(good, cam.width * cam.height)
+    want = solo_image(c0_scene, cam, spp=3, max_bounces=4, seed=77)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    srv.close()
+    owner.close()
+
+
+def test_server_failed_round_fails_its_jobs_only(c0_scene):
+    """A round whose kernels raise an internal-limit flag (raised here from the host) fails
+    the jobs of that round with BI_PT_JOB_FAILED; jobs still waiting and later ones go on."""
+    print("case: server round failure (flag raised from the host)", flush=True)
+    owner = bi_pt.PathTracer()
+    owner.set_scene(c0_scene)
+    srv = bi_pt.Server(spp_per_round=1, max_jobs_per_round=2)
+    sid = srv.add_scene(owner)
+    cam = scenes.camera_c0(24, 16)
+    a = srv.submit(sid, cam, spp=3, max_bounces=4, seed=1)
+    b = srv.submit(sid, cam, spp=3, max_bounces=4, seed=2)
+    c = srv.submit(sid, cam, spp=2, max_bounces=4, seed=3)   # waits: two jobs per round
+    assert srv.step() == 3
+    srv.inject_error(1)
+    with pytest.raises(bi_pt.PtError) as e:
+        srv.step()
+    assert e.value.status == 5 and "stack overflow" in str(e.value)
+    assert srv.poll(a)["state"] == 3 and srv.poll(b)["state"] == 3
+    assert srv.poll(c)["state"] == 0
+    with pytest.raises(bi_pt.PtError) as e:
+        srv.take_result(a, cam.width * cam.height)
+    assert e.value.status == 5
+    d = srv.submit(sid, cam, spp=2, max_bounces=4, seed=4)
+    while srv.step():
+        pass
+    for jid, seed in ((c, 3), (d, 4)):
+        assert srv.poll(jid)["state"] == 2
+        got = srv.take_result(jid, cam.width * cam.height)
+        want = solo_image(c0_scene, cam, spp=2, max_bounces=4, seed=seed)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    srv.close()
+    owner.close()
+
+
+def test_error_flags_are_sticky_until_checked(c0_scene):
+    """A flag raised before or during one render is still reported after later renders,
+    by the next statistics call, which clears it."""
+    print("case: sticky device error flags", flush=True)
+    t = bi_pt.PathTracer()
+    t.set_scene(c0_scene)
+    cam = scenes.camera_c0(24, 16)
+    t.render(cam, spp=1, max_bounces=3, seed=1)
+    t.stats()
+    t.inject_error(8)
+    t.render(cam, spp=1, max_bounces=3, seed=2)
+    t.render(cam, spp=1, max_bounces=3, seed=3)
+    with pytest.raises(bi_pt.PtError) as e:
+        t.stats()
+    assert e.value.status == 5 and "step bound" in str(e.value)
+    t.render(cam, spp=1, max_bounces=3, seed=4)
+    t.stats()   # cleared by the report
+    t.close()
