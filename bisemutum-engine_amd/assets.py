@@ -200,6 +200,113 @@ def submesh_triangles(mesh: MeshData, s: int):
             idx.astype(np.uint32).reshape(-1, 3))
 
 
+def submesh_attributes(mesh: MeshData):
+    """(texcoords [V,2] or None, tangents [V,4] or None): the asset's per-vertex texture
+    coordinates and tangents when it has one per vertex (texcoords_buffer, tangents_buffer)."""
+    nv = len(mesh.positions)
+    uv = np.ascontiguousarray(mesh.texcoords, np.float32) if len(mesh.texcoords) == nv and nv else None
+    tan = np.ascontiguousarray(mesh.tangents, np.float32) if len(mesh.tangents) == nv and nv else None
+    return uv, tan
+
+
+# ---------------------------------------------------------------- textures
+
+TEXTURE_TYPE = "Texture"
+# rhi::ResourceFormat values of the formats the pass samples (include/bisemutum/rhi/defines.hpp:44-)
+FORMAT_RGBA8_UNORM, FORMAT_RGBA8_SRGB, FORMAT_RGBA32_SFLOAT = 37, 43, 109
+_FILTER_LINEAR = 1
+_ADDRESS = {0: 0, 1: 1, 2: 2}   # repeat, mirrored repeat, clamp to edge (clamp to border / mirror clamp: no)
+
+
+@dataclasses.dataclass
+class TextureData:
+    width: int
+    height: int
+    levels: int
+    format: int          # rhi::ResourceFormat
+    sampler: dict        # mag/min filter, mipmap mode, address modes, lod range
+    texels: np.ndarray   # level 0: [H, W, 4] uint8 or float32
+
+
+def read_texture(data: bytes, what: str = "texture") -> TextureData:
+    """Parse a Texture asset (TextureAsset::load, src/scene_basic/texture.cpp:92-136):
+    magic, type name, version, rhi::SamplerDesc (28 bytes: 9 one-byte enums and flags,
+    then anisotropy, lod bias, lod min, lod max), rhi::TextureDesc (20 bytes: extent
+    width/height/layers, levels, one-byte format, dimension, usages), then level 0's texel
+    bytes as a vector, zlib-compressed (version 2) or raw (version 1, storage type 0;
+    storage type 1, PNG layers, is not read)."""
+    r = _Reader(data, what)
+    magic = r.u32()
+    type_name = r.string()
+    version = r.u32()
+    if magic != ASSET_MAGIC or type_name != TEXTURE_TYPE:
+        raise AssetError(f"{what}: not a Texture asset (magic {magic:#010x}, type {type_name!r})")
+    sb = bytes(r.take(28))
+    sampler = {"mag_filter": sb[0], "min_filter": sb[1], "mipmap_mode": sb[2], "address_u": sb[3],
+               "address_v": sb[4], "address_w": sb[5], "border_color": sb[6], "compare_enabled": sb[7],
+               "compare_op": sb[8]}
+    sampler.update(zip(("anisotropy", "lod_bias", "lod_min", "lod_max"), struct.unpack("<4f", sb[12:28])))
+    width, height, layers, levels = struct.unpack("<4I", bytes(r.take(16)))
+    fmt, dim, _usages, _pad = bytes(r.take(4))
+    if version == 1:
+        storage = r.u32()
+        if storage != 0:
+            raise AssetError(f"{what}: texture storage type {storage} (PNG layers) is not read")
+        body = r
+    elif version == 2:
+        length = r.u64()
+        compressed = r.u64()
+        raw = zlib.decompress(bytes(r.take(compressed)))
+        if len(raw) != length:
+            raise AssetError(f"{what}: compressed part holds {len(raw)} bytes, header says {length}")
+        body = _Reader(raw, what)
+    else:
+        raise AssetError(f"{what}: unknown Texture version {version}")
+    if dim != 1 or layers != 1:
+        raise AssetError(f"{what}: only 2D textures of one layer are read (dimension {dim}, {layers} layers)")
+    if width == 0 or height == 0 or width > 16384 or height > 16384:
+        raise AssetError(f"{what}: texture size {width}x{height}")
+    if fmt in (FORMAT_RGBA8_UNORM, FORMAT_RGBA8_SRGB):
+        texels = body.vector("u1")
+        if len(texels) < 4 * width * height:
+            raise AssetError(f"{what}: {len(texels)} texel bytes for {width}x{height} RGBA8")
+        texels = texels[:4 * width * height].reshape(height, width, 4)
+    elif fmt == FORMAT_RGBA32_SFLOAT:
+        raw = body.vector("u1")
+        if len(raw) < 16 * width * height:
+            raise AssetError(f"{what}: {len(raw)} texel bytes for {width}x{height} RGBA32F")
+        texels = np.frombuffer(raw[:16 * width * height].tobytes(), "<f4").reshape(height, width, 4).copy()
+    else:
+        raise AssetError(f"{what}: texture format {fmt} is not RGBA8 or RGBA32F")
+    return TextureData(width, height, levels, fmt, sampler, texels)
+
+
+def encode_texture(t: TextureData) -> bytes:
+    """A Texture asset (version 2, as TextureAsset::save writes) of level 0 of t."""
+    name = TEXTURE_TYPE.encode()
+    sm = t.sampler
+    sb = bytes([sm["mag_filter"], sm["min_filter"], sm["mipmap_mode"], sm["address_u"], sm["address_v"],
+                sm["address_w"], sm["border_color"], sm["compare_enabled"], sm["compare_op"], 0, 0, 0])
+    sb += struct.pack("<4f", sm["anisotropy"], sm["lod_bias"], sm["lod_min"], sm["lod_max"])
+    td = struct.pack("<4I", t.width, t.height, 1, t.levels) + bytes([t.format, 1, 7, 0])
+    raw = np.ascontiguousarray(t.texels).tobytes()
+    body = struct.pack("<Q", len(raw)) + raw
+    z = zlib.compress(body)
+    return struct.pack("<IQ", ASSET_MAGIC, len(name)) + name + struct.pack("<I", 2) + sb + td + \
+        struct.pack("<QQ", len(body), len(z)) + z
+
+
+def texture_record(t: TextureData) -> "scenes.Texture":
+    """The pass's texture of an asset: level 0 (the reference's ray-tracing materials sample
+    level 0), the sampler's magnification filter and u / v addressing."""
+    if t.sampler["address_u"] not in _ADDRESS or t.sampler["address_v"] not in _ADDRESS:
+        raise AssetError("texture addressing clamp-to-border / mirror-clamp-to-edge is not supported")
+    fmt = {FORMAT_RGBA8_UNORM: scenes.TEXEL_RGBA8_UNORM, FORMAT_RGBA8_SRGB: scenes.TEXEL_RGBA8_SRGB,
+           FORMAT_RGBA32_SFLOAT: scenes.TEXEL_RGBA32F}[t.format]
+    return scenes.Texture(np.ascontiguousarray(t.texels), fmt, 1 if t.sampler["mag_filter"] == _FILTER_LINEAR else 0,
+                          _ADDRESS[t.sampler["address_u"]], _ADDRESS[t.sampler["address_v"]])
+
+
 # ---------------------------------------------------------------- transforms
 
 def euler_zxy_degrees(rot_deg) -> np.ndarray:
@@ -250,6 +357,13 @@ _STMT = re.compile(r"^surface\.(\w+)\s*=\s*(.+?)\s*$")
 _VEC = re.compile(r"^float3\s*\(\s*(" + _NUM + r")\s*,\s*(" + _NUM + r")\s*,\s*(" + _NUM + r")\s*\)$")
 
 
+_SAMPLE = r"PARAM_(\w+)\s*\.\s*Sample\s*\(\s*PARAM_\w+\s*,\s*vertex\.texcoord\s*\)"
+_SWZ = r"(xyz|rgb|[xyzwrgba])"
+_TEX_DECL = re.compile(r"^float4\s+(\w+)\s*=\s*" + _SAMPLE + r"$")
+_TEX_USE = re.compile(r"^(?:" + _SAMPLE + r"|(\w+))\s*\.\s*" + _SWZ + r"$")
+_TEX_STEP = re.compile(r"^(?:" + _SAMPLE + r"|(\w+))\s*\.\s*" + _SWZ + r"\s*<\s*(" + r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?f?" +
+                       r")\s*\?\s*(.+?)\s*:\s*(.+)$")
+_CHANNEL = {"x": 0, "r": 0, "y": 1, "g": 1, "z": 2, "b": 2, "w": 3, "a": 3}
 _CELL = r"int\s*\(\s*floor\s*\(\s*vertex\.position_world\.([xz])\s*\)\s*\)"
 _DECL = re.compile(r"^int\s+(\w+)\s*=\s*" + _CELL + r"\s*\^\s*" + _CELL + r"$")
 _SEL = re.compile(r"^\(\s*(\w+)\s*&\s*1\s*\)\s*==\s*([01])\s*\?\s*(.+?)\s*:\s*(.+)$")
@@ -276,16 +390,24 @@ def parse_material_function(text: str) -> list:
     project is read too: `int g = int(floor(vertex.position_world.x)) ^
     int(floor(vertex.position_world.z));` gives ("$g", ("checker_xz",)), and
     `surface.<field> = (g & 1) == 1 ? A : B;` gives (field, ("checker", "g", A, B)) with
-    A for odd cells ("== 0" swaps them).  Anything else raises UnsupportedMaterial
+    A for odd cells ("== 0" swaps them).  Textures sampled at vertex.texcoord:
+    `surface.<field> = PARAM_<t>.Sample(PARAM_<t>_sampler, vertex.texcoord).<swizzle>;`, or
+    through `float4 v = PARAM_<t>.Sample(...);` then `surface.<field> = v.<swizzle>;`, give
+    (field, ("texture", t, swizzle)); `surface.<field> = v.<c> < a ? b : c;` with constants
+    gives (field, ("step", t, c, a, b, c)).  Anything else raises UnsupportedMaterial
     naming the reason."""
     out = []
     cells = set()
+    samples = {}   # float4 variable -> texture parameter
     for raw in text.split(";"):
         stmt = " ".join(raw.split())
         if not stmt:
             continue
-        if ".Sample" in stmt or "Texture" in stmt:
-            raise UnsupportedMaterial(f"texture sampling: {stmt!r}")
+        td = _TEX_DECL.match(stmt)
+        if td:
+            samples[td.group(1)] = td.group(2)
+            out.append(("$" + td.group(1), ("sample", td.group(2))))
+            continue
         d = _DECL.match(stmt)
         if d and {d.group(2), d.group(3)} == {"x", "z"}:
             cells.add(d.group(1))
@@ -301,6 +423,19 @@ def parse_material_function(text: str) -> list:
         if op is not None:
             out.append((field, op))
             continue
+        tu = _TEX_USE.match(rhs)
+        if tu and (tu.group(1) or tu.group(2) in samples):
+            out.append((field, ("texture", tu.group(1) or samples[tu.group(2)], tu.group(3))))
+            continue
+        ts = _TEX_STEP.match(rhs)
+        if ts and (ts.group(1) or ts.group(2) in samples) and len(ts.group(3)) == 1:
+            below, above = _operand(ts.group(5)), _operand(ts.group(6))
+            if below is not None and above is not None and below[0] == above[0] == "const":
+                out.append((field, ("step", ts.group(1) or samples[ts.group(2)], ts.group(3),
+                                    float(ts.group(4).rstrip("f")), float(below[1]), float(above[1]))))
+                continue
+        if ".Sample" in stmt or "Texture" in stmt:
+            raise UnsupportedMaterial(f"texture sampling outside the supported forms: {stmt!r}")
         sel = _SEL.match(rhs)
         if sel and sel.group(1) in cells:
             a, b = _operand(sel.group(3)), _operand(sel.group(4))
@@ -353,9 +488,30 @@ def evaluate_material(desc: dict) -> dict:
 
     odd, even = surface_default(), surface_default()
     cells, pattern = set(), False
+    textured = {}
+
+    def texture_asset(name):
+        v = params.get(name)
+        if not isinstance(v, dict) or "asset_id" not in v:
+            raise UnsupportedMaterial(f"parameter {name!r} is not a texture")
+        return int(v["asset_id"])
+
     for field, rhs in stmts:
         if field.startswith("$"):
-            cells.add(field[1:])
+            if rhs[0] == "checker_xz":
+                cells.add(field[1:])
+        elif rhs[0] == "texture":
+            tex, swz = texture_asset(rhs[1]), rhs[2]
+            if field in ("base_color", "f0_color", "normal_map_value") and swz in ("xyz", "rgb"):
+                textured[field] = (tex,)
+            elif field in ("roughness", "opacity") and len(swz) == 1:
+                textured[field] = (tex, _CHANNEL[swz])
+            else:
+                raise UnsupportedMaterial(f"{field} from texture channels {swz!r}")
+        elif rhs[0] == "step":
+            if field != "opacity":
+                raise UnsupportedMaterial(f"a stepped texture channel for {field}")
+            textured[field] = (texture_asset(rhs[1]), _CHANNEL[rhs[2]], rhs[3], rhs[4], rhs[5])
         elif rhs[0] == "checker":
             if rhs[1] not in cells:
                 raise UnsupportedMaterial(f"checker variable {rhs[1]!r} is not declared")
@@ -366,13 +522,15 @@ def evaluate_material(desc: dict) -> dict:
     blend = desc.get("blend_mode", "opaque")
     if blend not in ("opaque", "alpha_test", "translucent", "additive", "modulate"):
         raise UnsupportedMaterial(f"blend mode {blend!r}")
+    if pattern and textured:
+        raise UnsupportedMaterial("a checkerboard of textured fields")
     for surf in (odd, even):
-        if blend == "alpha_test" and surf["opacity"] < 0.5:
-            raise UnsupportedMaterial("alpha-tested material with opacity below the threshold (cut out everywhere)")
-        if blend in ("translucent", "additive", "modulate") and surf["opacity"] < 1.0:
-            raise UnsupportedMaterial(f"{blend} material with opacity {surf['opacity']} (stochastic any-hit)")
-        if tuple(surf["normal_map_value"]) != (0.5, 0.5, 1.0):
+        if tuple(surf["normal_map_value"]) != (0.5, 0.5, 1.0) and "normal_map_value" not in textured:
             raise UnsupportedMaterial("constant normal map value other than (0.5, 0.5, 1)")
+        # the ray-tracing any-hit: alpha test for alpha_test, stochastic opacity for the
+        # other non-opaque modes (hits/rt_gbuffer_hit.hlsl:20-33)
+        surf["blend"] = {"opaque": 0, "alpha_test": 1}.get(blend, 2)
+        surf["textures"] = dict(textured)
     odd["pattern_even"] = even if pattern else None
     return odd
 
@@ -380,18 +538,37 @@ def evaluate_material(desc: dict) -> dict:
 PATTERN_CHECKER_XZ = 1   # bi_pt_material.pattern: odd cells this record, even cells pattern_material
 
 
-def material_record(surf: dict) -> np.ndarray:
-    """bi_pt_material record (16 floats) of an evaluated surface; emission is not used by the pass.
+def material_record(surf: dict, tex_index=None) -> np.ndarray:
+    """bi_pt_material record (32 floats) of an evaluated surface; emission is not used by the pass.
     A checkerboard surface becomes two records: this one (odd cells, pattern set) followed by
-    the even cells' record."""
-    rec = np.zeros(16, np.float32)
+    the even cells' record.  tex_index maps a texture asset id to its index in the scene's
+    textures (references are stored as index + 1)."""
+    rec = np.zeros(scenes.MATERIAL_FLOATS, np.float32)
+    w = rec.view(np.uint32)
     rec[0:3] = surf["base_color"]
     rec[3] = surf["roughness"]
     rec[4:7] = surf["f0_color"]
     rec[7] = surf["anisotropy"]
     rec[8:11] = surf["f90_color"]
     rec[11] = surf["ior"]
-    rec.view(np.uint32)[12] = 1 if surf["two_sided"] else 0
+    w[12] = 1 if surf["two_sided"] else 0
+    w[scenes.M_BLEND] = surf.get("blend", 0)
+    rec[scenes.M_OPACITY] = surf["opacity"]
+    for field, t in surf.get("textures", {}).items():
+        ref = tex_index[t[0]] + 1
+        if field == "base_color":
+            w[scenes.M_BASE_TEX] = ref
+        elif field == "f0_color":
+            w[scenes.M_F0_TEX] = ref
+        elif field == "normal_map_value":
+            w[scenes.M_NORMAL_TEX] = ref
+        elif field == "roughness":
+            w[scenes.M_ROUGH_TEX], w[scenes.M_ROUGH_CHANNEL] = ref, t[1]
+        elif field == "opacity":
+            w[scenes.M_OPACITY_TEX], w[scenes.M_OPACITY_CHANNEL] = ref, t[1]
+            if len(t) == 5:
+                w[scenes.M_OPACITY_STEP] = 1
+                rec[scenes.M_STEP_AT], rec[scenes.M_STEP_BELOW], rec[scenes.M_STEP_ABOVE] = t[2], t[3], t[4]
     return rec
 
 
@@ -417,7 +594,7 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
                            on_unsupported="raise") -> ProjectScene:
     """Build the pass's scene and camera from a parsed scene file.
 
-    fetch(asset_id) returns a StaticMesh's bytes or a material's table.  on_unsupported:
+    fetch(asset_id) returns a StaticMesh's or Texture's bytes or a material's table.  on_unsupported:
     "raise", or "default" to replace such a material by surface_data_default() and list it."""
     if on_unsupported not in ("raise", "default"):
         raise ValueError("on_unsupported must be 'raise' or 'default'")
@@ -436,6 +613,8 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
         return world[i]
 
     meshes, mats, inst, placed = [], [], [], []
+    mesh_uv, mesh_tan = [], []
+    textures, tex_index = [], {}   # pass textures; texture asset id -> index
     parsed = {}        # mesh asset id -> MeshData
     mesh_index = {}    # (mesh asset id, submesh) -> index into meshes
     mat_index = {}     # material asset id -> index into mats
@@ -489,6 +668,9 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
                 if key not in mesh_index:
                     mesh_index[key] = len(meshes)
                     meshes.append(submesh_triangles(md, sub))
+                    uv, tan = submesh_attributes(md)
+                    mesh_uv.append(uv)
+                    mesh_tan.append(tan)
                 mid = mat_ids[k]
                 if mid not in mat_index:
                     try:
@@ -498,8 +680,12 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
                             raise UnsupportedMaterial(f"material asset {mid}: {e}") from None
                         unsupported[mid] = str(e)
                         surf = surface_default()
+                    for t in surf.get("textures", {}).values():
+                        if t[0] not in tex_index:
+                            tex_index[t[0]] = len(textures)
+                            textures.append(texture_record(read_texture(fetch(t[0]), f"asset {t[0]}")))
                     mat_index[mid] = len(mats)
-                    rec = material_record(surf)
+                    rec = material_record(surf, tex_index)
                     if surf.get("pattern_even") is not None:
                         rec.view(np.uint32)[13] = PATTERN_CHECKER_XZ
                         rec.view(np.uint32)[14] = len(mats) + 1
@@ -513,9 +699,13 @@ def scene_from_description(scene_desc: dict, fetch, name="project", width=None, 
     if not inst:
         raise AssetError("the scene has no static meshes")
     sky = np.zeros((1, 1, 4), np.float32)
+    any_uv = textures and any(u is not None for u in mesh_uv)
+    any_tan = textures and any(t is not None for t in mesh_tan)
     sc = scenes.Scene(name, meshes, np.array(inst, scenes.INSTANCE_DTYPE), np.stack(mats),
                       np.stack(lights) if lights else np.zeros((0, 16), np.float32), sky,
-                      np.zeros(3, np.float32), np.eye(3, dtype=np.float32).reshape(-1))
+                      np.zeros(3, np.float32), np.eye(3, dtype=np.float32).reshape(-1),
+                      mesh_uv=mesh_uv if any_uv else None, mesh_tan=mesh_tan if any_tan else None,
+                      textures=textures)
     return ProjectScene(sc, camera, settings, unsupported, placed)
 
 
@@ -548,6 +738,9 @@ def load_project(root: str, width=None, height=None, on_unsupported="raise") -> 
                 return f.read()
         if a["type"] == "Material":
             return toml(path)
-        raise AssetError(f"asset {asset_id} has type {a['type']!r}, not a mesh or material")
+        if a["type"] == "Texture":
+            with open(path, "rb") as f:
+                return f.read()
+        raise AssetError(f"asset {asset_id} has type {a['type']!r}, not a mesh, material or texture")
 
     return scene_from_description(scene_desc, fetch, project.get("name", "project"), width, height, on_unsupported)

@@ -182,5 +182,8 @@ extern "C" int bi_static_mesh_submesh(const bi_static_mesh* mc, uint32_t s, bi_p
     out->indices = tri.data();
     out->num_vertices = static_cast<uint32_t>(nv);
     out->num_indices = static_cast<uint32_t>(n);
+    // texture coordinates and tangents when the asset has one per vertex
+    out->texcoords = m->texcoords.size() == 2 * nv && nv ? m->texcoords.data() : nullptr;
+    out->tangents = m->tangents.size() == 4 * nv && nv ? m->tangents.data() : nullptr;
     return 0;
 }

@@ -38,7 +38,8 @@ EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene
            "bi_pt_nrc_create", "bi_pt_nrc_destroy", "bi_pt_nrc_last_error", "bi_pt_render_nrc",
            "bi_pt_nrc_get_stats", "bi_pt_nrc_get_params", "bi_pt_nrc_set_params", "bi_pt_nrc_set_bounds",
            "bi_pt_nrc_infer", "bi_pt_nrc_train", "bi_pt_nrc_debug_gradient", "bi_pt_nrc_debug_samples",
-           "bi_pt_set_build_options", "bi_pt_debug_inject_error", "bi_pt_server_debug_inject_error"]
+           "bi_pt_set_build_options", "bi_pt_debug_inject_error", "bi_pt_server_debug_inject_error",
+           "bi_pt_set_textures"]
 NRC_PARAMS = 17408
 NRC_FREEZE = 1
 
@@ -55,7 +56,13 @@ class DeviceDesc(C.Structure):
 
 class Mesh(C.Structure):
     _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
-                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32)]
+                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32),
+                ("texcoords", C.POINTER(C.c_float)), ("tangents", C.POINTER(C.c_float))]
+
+
+class TextureC(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
+                ("filter", C.c_uint32), ("address_u", C.c_uint32), ("address_v", C.c_uint32)]
 
 
 class Instance(C.Structure):
@@ -197,6 +204,7 @@ def lib():
         L.bi_pt_nrc_debug_samples.argtypes = [vp, f32p, f32p, u32]
         L.bi_pt_trace_closest.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
         L.bi_pt_trace_occlusion.argtypes = [vp, u32, vp, C.c_float, vp, vp]
+        L.bi_pt_set_textures.argtypes = [vp, vp, u32]
         L.bi_pt_debug_inject_error.argtypes = [vp, u32]
         L.bi_pt_server_debug_inject_error.argtypes = [vp, u32]
         for name in EXPORTS:
@@ -274,8 +282,15 @@ class PathTracer:
             p = np.ascontiguousarray(p, np.float32)
             n = np.ascontiguousarray(n, np.float32)
             idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
-            keep += [p, n, idx]
-            meshes[i] = Mesh(_p(p), _p(n), _p(idx, C.c_uint32), len(p), len(idx))
+            uv = tan = None
+            if scene.mesh_uv is not None and scene.mesh_uv[i] is not None:
+                uv = np.ascontiguousarray(scene.mesh_uv[i], np.float32)
+            if scene.mesh_tan is not None and scene.mesh_tan[i] is not None:
+                tan = np.ascontiguousarray(scene.mesh_tan[i], np.float32)
+            keep += [p, n, idx, uv, tan]
+            meshes[i] = Mesh(_p(p), _p(n), _p(idx, C.c_uint32), len(p), len(idx),
+                             _p(uv) if uv is not None else None, _p(tan) if tan is not None else None)
+        self.set_textures(scene.textures)
         inst = (Instance * len(scene.instances))()
         for i, rec in enumerate(scene.instances):
             x = np.ascontiguousarray(rec["xform"], np.float32)
@@ -287,6 +302,16 @@ class PathTracer:
                                           len(mats)))
         self.set_lights(scene.dir_lights)
         self.set_sky(scene.sky_rgba, scene.sky_color, scene.sky_rot)
+
+    def set_textures(self, textures):
+        """bi_pt_set_textures: scenes.Texture records (texels [H, W, 4] uint8 or float32)."""
+        arr = (TextureC * max(len(textures), 1))()
+        keep = []
+        for i, t in enumerate(textures):
+            a = np.ascontiguousarray(t.texels)
+            keep.append(a)
+            arr[i] = TextureC(a.ctypes.data, a.shape[1], a.shape[0], t.format, t.filter, t.address_u, t.address_v)
+        self._check(lib().bi_pt_set_textures(self.h, arr, len(textures)))
 
     def update_transforms(self, xforms):
         x = np.ascontiguousarray(xforms, np.float32)

@@ -51,7 +51,8 @@ void build_sky_table(const float* rgba, uint32_t W, uint32_t H, const float colo
 namespace bipt {
 
 BvhView bvh_view(const bi_pt_ctx* ctx) {
-    return BvhView{ctx->nodes4.p, ctx->leaf_geo.p, ctx->num_tris};
+    return BvhView{ctx->nodes4.p, ctx->leaf_geo.p, ctx->num_tris, ctx->leaf_shade.p, ctx->mats.p,
+                   scene_leaf_uv(ctx), scene_tex(ctx), 0u};
 }
 
 bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
@@ -78,6 +79,10 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
     BI_TRY(dalloc(ctx, ctx->tl_height, ni));
     BI_TRY(dalloc(ctx, ctx->tl_snap, ni));
     BI_TRY(dalloc(ctx, ctx->tl_starts, 1 + treelet_start_cap(n)));
+    if (ctx->need_uv) BI_TRY(dalloc(ctx, ctx->leaf_uv, n));
+    else dfree(ctx, ctx->leaf_uv);
+    if (ctx->need_tan) BI_TRY(dalloc(ctx, ctx->leaf_tan, n));
+    else dfree(ctx, ctx->leaf_tan);
     return BI_PT_OK;
 }
 
@@ -109,6 +114,11 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error, uint32_t 
     a.refit_cnt = ctx->refit_cnt.p;
     a.leaf_geo = ctx->leaf_geo.p;
     a.leaf_shade = ctx->leaf_shade.p;
+    a.materials = ctx->mats.p;
+    a.uvs = ctx->need_uv ? ctx->uvs.p : nullptr;
+    a.tangents = ctx->need_tan ? ctx->tans.p : nullptr;
+    a.leaf_uv = ctx->need_uv ? ctx->leaf_uv.p : nullptr;
+    a.leaf_tan = ctx->need_tan ? ctx->leaf_tan.p : nullptr;
     a.error = error;
     a.treelet_rounds = rounds;
     a.tl_cost = ctx->tl_cost.p;
@@ -279,10 +289,22 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
     for (uint32_t m = 0; m < num_materials; ++m)
         if (!(materials[m].roughness >= 0.0f) || !(materials[m].anisotropy >= 0.0f && materials[m].anisotropy <= 1.0f))
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "material roughness/anisotropy out of range");
+    bool need_uv = false, need_tan = false;
+    uint32_t tex_refs = 0;
     for (uint32_t m = 0; m < num_materials; ++m) {
         const bi_pt_material& M = materials[m];
         if (M.pattern > BI_PT_PATTERN_CHECKER_XZ)
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "unknown material pattern");
+        if (M.blend_mode > BI_PT_BLEND_TRANSLUCENT || M.opacity_channel > 3 || M.roughness_channel > 3 ||
+            !std::isfinite(M.opacity) || !std::isfinite(M.step_at) || !std::isfinite(M.step_below) ||
+            !std::isfinite(M.step_above))
+            return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "material blend mode / opacity / channel out of range");
+        for (uint32_t t : {M.base_color_tex, M.f0_tex, M.normal_tex, M.roughness_tex, M.opacity_tex}) {
+            if (t == BI_PT_NO_TEXTURE) continue;
+            need_uv = true;
+            tex_refs = std::max(tex_refs, t);   // references are index + 1
+        }
+        need_tan = need_tan || M.normal_tex != BI_PT_NO_TEXTURE;
         if (M.pattern == BI_PT_PATTERN_CHECKER_XZ &&
             (M.pattern_material >= num_materials || materials[M.pattern_material].pattern != BI_PT_PATTERN_NONE))
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT,
@@ -298,6 +320,45 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
         if (meshes[m].num_indices)
             std::memcpy(&hi[idx_off[m]], meshes[m].indices, 4 * (size_t)meshes[m].num_indices);
     }
+    // texture coordinates (default 0) and tangents (default (1, 0, 0, 1)) of every vertex,
+    // only when a material samples a texture / a normal map
+    std::vector<float> huv(need_uv ? 2 * nv : 0, 0.0f), htan;
+    if (need_tan) {
+        htan.resize(4 * nv);
+        for (size_t v = 0; v < nv; ++v) {
+            htan[4 * v] = 1.0f; htan[4 * v + 1] = 0.0f; htan[4 * v + 2] = 0.0f; htan[4 * v + 3] = 1.0f;
+        }
+    }
+    for (uint32_t m = 0; m < num_meshes; ++m) {
+        const size_t nvm = meshes[m].num_vertices;
+        if (need_uv && meshes[m].texcoords) {
+            for (size_t k = 0; k < 2 * nvm; ++k)
+                if (!std::isfinite(meshes[m].texcoords[k]))
+                    return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite texture coordinate");
+            std::memcpy(&huv[2 * (size_t)vtx_off[m]], meshes[m].texcoords, 8 * nvm);
+        }
+        if (need_tan && meshes[m].tangents) {
+            for (size_t k = 0; k < 4 * nvm; ++k)
+                if (!std::isfinite(meshes[m].tangents[k]))
+                    return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite tangent");
+            std::memcpy(&htan[4 * (size_t)vtx_off[m]], meshes[m].tangents, 16 * nvm);
+        }
+    }
+    if (need_uv) {
+        BI_TRY(dalloc(ctx, ctx->uvs, std::max<size_t>(huv.size(), 1)));
+        if (!huv.empty()) BI_HIP_TRY(ctx, hipMemcpy(ctx->uvs.p, huv.data(), huv.size() * 4, hipMemcpyHostToDevice));
+    } else {
+        dfree(ctx, ctx->uvs);
+    }
+    if (need_tan) {
+        BI_TRY(dalloc(ctx, ctx->tans, std::max<size_t>(htan.size(), 1)));
+        if (!htan.empty()) BI_HIP_TRY(ctx, hipMemcpy(ctx->tans.p, htan.data(), htan.size() * 4, hipMemcpyHostToDevice));
+    } else {
+        dfree(ctx, ctx->tans);
+    }
+    ctx->need_uv = need_uv;
+    ctx->need_tan = need_tan;
+    ctx->tex_refs = tex_refs;
     BI_TRY(dalloc(ctx, ctx->pos, std::max<size_t>(hp.size(), 1)));
     BI_TRY(dalloc(ctx, ctx->nrm, std::max<size_t>(hn.size(), 1)));
     BI_TRY(dalloc(ctx, ctx->idx, std::max<size_t>(hi.size(), 1)));
@@ -334,6 +395,55 @@ bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* xf, uint32_t n
         BI_HIP_TRY(ctx, hipMemcpy(ctx->inst.p, ctx->inst_host.data(), num_instances * sizeof(InstanceRec),
                           hipMemcpyHostToDevice));
     ctx->bvh_built = false;
+    ctx->acc_count = 0;
+    ctx->scene_version++;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_set_textures(bi_pt_ctx* ctx, const bi_pt_texture* textures, uint32_t num_textures) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if (num_textures && !textures) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null textures");
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<TexRec> recs(num_textures);
+    size_t words = 0;
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const bi_pt_texture& T = textures[k];
+        if (!T.texels || T.width == 0 || T.height == 0 || T.width > 16384 || T.height > 16384 ||
+            T.format > BI_PT_TEXEL_RGBA32F || T.filter > 1 || T.address_u > BI_PT_ADDRESS_CLAMP_TO_EDGE ||
+            T.address_v > BI_PT_ADDRESS_CLAMP_TO_EDGE)
+            return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "texture " + std::to_string(k) + ": bad description");
+        words = (words + 3u) & ~(size_t)3u;   // RGBA32F texels are read as 16-B words
+        recs[k] = TexRec{(uint32_t)words, T.width, T.height, T.format, T.filter, T.address_u, T.address_v, 0u};
+        words += (size_t)T.width * T.height * (T.format == BI_PT_TEXEL_RGBA32F ? 4u : 1u);
+        if (words >= (1ull << 32)) return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "textures larger than 16 GiB");
+    }
+    std::vector<uint32_t> pool(words, 0u);
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const bi_pt_texture& T = textures[k];
+        const size_t n = (size_t)T.width * T.height * (T.format == BI_PT_TEXEL_RGBA32F ? 4u : 1u);
+        if (T.format == BI_PT_TEXEL_RGBA32F) {
+            const float* f = static_cast<const float*>(T.texels);
+            for (size_t i = 0; i < n; ++i)
+                if (!std::isfinite(f[i]))
+                    return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "texture " + std::to_string(k) + ": non-finite texel");
+        }
+        std::memcpy(&pool[recs[k].offset], T.texels, 4 * n);
+    }
+    // sRGB byte -> linear (IEC 61966-2-1), evaluated in double and rounded once
+    float lut[256];
+    for (int c = 0; c < 256; ++c) {
+        const double x = c / 255.0;
+        lut[c] = (float)(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+    }
+    BI_TRY(dalloc(ctx, ctx->tex_recs, std::max<size_t>(recs.size(), 1)));
+    BI_TRY(dalloc(ctx, ctx->tex_pool, std::max<size_t>(pool.size(), 4)));
+    BI_TRY(dalloc(ctx, ctx->srgb, 256));
+    if (!recs.empty())
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->tex_recs.p, recs.data(), recs.size() * sizeof(TexRec), hipMemcpyHostToDevice));
+    if (!pool.empty()) BI_HIP_TRY(ctx, hipMemcpy(ctx->tex_pool.p, pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
+    BI_HIP_TRY(ctx, hipMemcpy(ctx->srgb.p, lut, sizeof lut, hipMemcpyHostToDevice));
+    ctx->num_tex = num_textures;
     ctx->acc_count = 0;
     ctx->scene_version++;
     return BI_PT_OK;
@@ -588,6 +698,10 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             JobRec rec{};
             rec.nodes = ctx->num_tris ? ctx->nodes4.p : ctx->eng.empty_tree.p;
             rec.leaf_geo = ctx->leaf_geo.p;
+            rec.leaf_shade = ctx->leaf_shade.p;
+            rec.materials = ctx->mats.p;
+            rec.leaf_uv = scene_leaf_uv(ctx);
+            rec.tex = scene_tex(ctx);
             rec.num_tris = ctx->num_tris;
             rec.ray_length = tmax;
             const uint32_t bases[4] = {0u, n, 0u, 0u};
@@ -644,6 +758,10 @@ bi_pt_status bi_pt_trace_closest(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* r
     JobRec rec{};
     rec.nodes = ctx->num_tris ? ctx->nodes4.p : ctx->eng.empty_tree.p;
     rec.leaf_geo = ctx->leaf_geo.p;
+    rec.leaf_shade = ctx->leaf_shade.p;
+    rec.materials = ctx->mats.p;
+    rec.leaf_uv = scene_leaf_uv(ctx);
+    rec.tex = scene_tex(ctx);
     rec.num_tris = ctx->num_tris;
     rec.ray_length = tmax;
     hipLaunchKernelGGL(k_put_query_job, dim3(1), dim3(64), 0, s, rec, n, ctx->rq_job.p);

@@ -223,6 +223,10 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "a context appears twice in one call");
         bi_pt_ctx* S = scene_of(ctx);
         if (!S->has_scene) return fail(own, BI_PT_ERROR_NO_SCENE, "no scene set (job " + std::to_string(j) + ")");
+        if (S->tex_refs > S->num_tex)
+            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT,
+                        "a material names texture index " + std::to_string(S->tex_refs - 1) + " but " +
+                            std::to_string(S->num_tex) + " textures are set (job " + std::to_string(j) + ")");
         const bi_pt_camera* c = J.camera;
         const bi_pt_settings* st = J.st;
         if (bi_pt_status e = validate_job(own, c, st); e != BI_PT_OK)
@@ -370,6 +374,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         r.leaf_geo = empty ? nullptr : S->leaf_geo.p;
         r.leaf_shade = empty ? nullptr : S->leaf_shade.p;
         r.materials = S->mats.p;
+        r.leaf_uv = empty ? nullptr : scene_leaf_uv(S);
+        r.leaf_tan = empty ? nullptr : scene_leaf_tan(S);
+        r.tex = scene_tex(S);
         r.num_tris = S->num_tris;
         r.num_lights = p.nl;
         r.sky_nee = p.sky_nee ? 1u : 0u;

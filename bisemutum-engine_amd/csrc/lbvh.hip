@@ -98,6 +98,50 @@ __global__ __launch_bounds__(256) void k_flatten(const float* __restrict__ pos, 
     }
 }
 
+// Texture coordinates and world tangents of each leaf's vertices (scenes whose materials
+// sample textures or normal maps): leaf k is triangle order[k]; its instance is found by
+// binary search over the instances' first triangles; tangents take the instance's linear
+// part (ObjectToWorld3x4 * (t.xyz, 0), hit.hlsl:155-156), their w (the bitangent sign) as is.
+__global__ __launch_bounds__(256) void k_leaf_attrs(const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ idx,
+                                                    const InstanceRec* __restrict__ inst, uint32_t ninst,
+                                                    uint32_t n, const float* __restrict__ uvs,
+                                                    const float* __restrict__ tans, LeafUv* __restrict__ leaf_uv,
+                                                    LeafTan* __restrict__ leaf_tan) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = order[k];
+    uint32_t lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (inst[mid].first_tri <= t) lo = mid; else hi = mid - 1;
+    }
+    const InstanceRec& I = inst[lo];
+    const float* m = I.xform;
+    const uint32_t kk = t - I.first_tri;
+    LeafUv u;
+    LeafTan g;
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t vi = I.vtx_off + idx[I.idx_off + 3 * kk + c];
+        if (leaf_uv) {
+            u.uv[2 * c] = uvs[2 * (size_t)vi];
+            u.uv[2 * c + 1] = uvs[2 * (size_t)vi + 1];
+        }
+        if (leaf_tan) {
+            const float x = tans[4 * (size_t)vi], y = tans[4 * (size_t)vi + 1], z = tans[4 * (size_t)vi + 2];
+            g.t[4 * c + 0] = (m[0] * x + m[1] * y) + m[2] * z;
+            g.t[4 * c + 1] = (m[4] * x + m[5] * y) + m[6] * z;
+            g.t[4 * c + 2] = (m[8] * x + m[9] * y) + m[10] * z;
+            g.t[4 * c + 3] = tans[4 * (size_t)vi + 3];
+        }
+    }
+    if (leaf_uv) {
+        u.uv[6] = u.uv[7] = 0.0f;
+        leaf_uv[k] = u;
+    }
+    if (leaf_tan) leaf_tan[k] = g;
+}
+
 __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_v, uint32_t n,
                                                 const uint32_t* __restrict__ cb, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ vals, float* __restrict__ cb_out) {
@@ -289,7 +333,8 @@ __device__ __forceinline__ float half_area_box(const float* b) {
 
 __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ order, const float* __restrict__ tri_v,
                                                 const float* __restrict__ tri_n,
-                                                const uint32_t* __restrict__ tri_mat, int n,
+                                                const uint32_t* __restrict__ tri_mat,
+                                                const MaterialRec* __restrict__ mats, int n,
                                                 LeafGeo* __restrict__ geo, LeafShade* __restrict__ shade,
                                                 BvhNode* nodes, const int32_t* __restrict__ parent,
                                                 uint32_t* refit_cnt, float* sub_cost, int32_t* sub_height,
@@ -306,7 +351,10 @@ __global__ __launch_bounds__(256) void k_leaves(const uint32_t* __restrict__ ord
         g.v[3 + a] = v[3 + a] - v[a];
         g.v[6 + a] = v[6 + a] - v[a];
     }
-    g.v[9] = g.v[10] = g.v[11] = 0.0f;
+    // alpha-tested and translucent triangles are marked for the traversal's any-hit test
+    // (the instance flags force_opaque / force_non_opaque of accel.cpp:108-116, per material)
+    g.v[9] = __uint_as_float(mats[tri_mat[t]].blend_mode != kBlendOpaque ? kLeafNonOpaque : 0u);
+    g.v[10] = g.v[11] = 0.0f;
     geo[k] = g;
     LeafShade sh;
     for (int a = 0; a < 9; ++a) sh.n[a] = tri_n[9 * (size_t)t + a];
@@ -962,8 +1010,12 @@ void launch_build_lbvh(const BuildArgs& a, hipStream_t s) {
     // 4 passes: the sorted keys/values end in buffer 0
     hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, a.keys[cur], (int)n, a.nodes, a.parent, a.refit_cnt,
                        a.tl_cost, a.tl_height);
-    hipLaunchKernelGGL(k_leaves, dim3(g256), dim3(256), 0, s, a.vals[cur], a.tri_v, a.tri_n, a.tri_mat, (int)n,
-                       a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.tl_cost, a.tl_height, a.error);
+    hipLaunchKernelGGL(k_leaves, dim3(g256), dim3(256), 0, s, a.vals[cur], a.tri_v, a.tri_n, a.tri_mat,
+                       a.materials, (int)n, a.leaf_geo, a.leaf_shade, a.nodes, a.parent, a.refit_cnt, a.tl_cost,
+                       a.tl_height, a.error);
+    if (a.leaf_uv || a.leaf_tan)
+        hipLaunchKernelGGL(k_leaf_attrs, dim3(g256), dim3(256), 0, s, a.vals[cur], a.indices, a.instances,
+                           a.num_instances, n, a.uvs, a.tangents, a.leaf_uv, a.leaf_tan);
     const int ni = n > 1 ? (int)n - 1 : 1;
     if (a.treelet_rounds > 0 && n >= 3) {
         // the refit's arrival counters are free again: they count arrivals of the climb

@@ -174,17 +174,109 @@ struct Mat {
     float rx, ry;
     bool two_sided;
 };
+// get_anisotropic_roughness (material/utils.hlsl:54-59) with the 0.001 floor.
+PT_HD void mat_alpha(Mat& r, float roughness, float anisotropy) {
+    float aniso = __builtin_sqrtf(1.0f - anisotropy * 0.9f);
+    float r2 = roughness * roughness;
+    r.rx = sel_max(r2 / aniso, 0.001f);
+    r.ry = sel_max(r2 * aniso, 0.001f);
+}
 PT_HD Mat load_mat(const MaterialRec& m) {
     Mat r;
     r.base = v3(m.base_color[0], m.base_color[1], m.base_color[2]);
     r.f0 = v3(m.f0_color[0], m.f0_color[1], m.f0_color[2]);
     r.f90 = v3(m.f90_color[0], m.f90_color[1], m.f90_color[2]);
     r.two_sided = m.two_sided != 0u;
-    float aniso = __builtin_sqrtf(1.0f - m.anisotropy * 0.9f);   // get_anisotropic_roughness
-    float r2 = m.roughness * m.roughness;
-    r.rx = sel_max(r2 / aniso, 0.001f);
-    r.ry = sel_max(r2 * aniso, 0.001f);
+    mat_alpha(r, m.roughness, m.anisotropy);
     return r;
+}
+PT_HD bool mat_textured(const MaterialRec& m) {
+    return (m.base_color_tex | m.f0_tex | m.normal_tex | m.roughness_tex) != kNoTexture;
+}
+
+// ---------------------------------------------------------------- textures
+// Level-0 sampling of the material textures (SampleLevel(..., 0) with the texture's
+// sampler): texel centres at (i + 0.5) / W, bilinear weights in full FP32 (hardware
+// quantises them to 8 bits), repeat / mirrored-repeat / clamp-to-edge addressing
+// (Vulkan's rules), RGBA8 decoded as c / 255 (UNORM) or through the sRGB table.
+struct V4 { float x, y, z, w; };
+PT_HD float comp4(const V4& v, uint32_t c) { return c == 0u ? v.x : c == 1u ? v.y : c == 2u ? v.z : v.w; }
+PT_HD int tex_wrap(int i, int n, uint32_t mode) {
+    if (mode == 2u) return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    if (mode == 1u) {
+        int m = i % (2 * n);
+        if (m < 0) m += 2 * n;
+        return m >= n ? 2 * n - 1 - m : m;
+    }
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+PT_HD V4 tex_texel(const SceneTex& T, const TexRec& r, int i, int j) {
+    const size_t k = (size_t)j * r.width + (size_t)i;
+    if (r.format == kTexRgba32f) {
+        const float* f = reinterpret_cast<const float*>(T.pool + r.offset) + 4 * k;
+        return V4{f[0], f[1], f[2], f[3]};
+    }
+    const uint32_t w = T.pool[r.offset + k];
+    const uint32_t c0 = w & 255u, c1 = (w >> 8) & 255u, c2 = (w >> 16) & 255u, c3 = w >> 24;
+    if (r.format == kTexRgba8Srgb) return V4{T.srgb[c0], T.srgb[c1], T.srgb[c2], (float)c3 / 255.0f};
+    return V4{(float)c0 / 255.0f, (float)c1 / 255.0f, (float)c2 / 255.0f, (float)c3 / 255.0f};
+}
+// x in texel units; coordinates that are not finite or beyond 2^24 texels read texel 0
+PT_HD float tex_coord(float x) { return (x > -16777216.0f && x < 16777216.0f) ? x : 0.0f; }
+PT_HD V4 tex_sample(const SceneTex& T, uint32_t id, float u, float v) {
+    const TexRec r = T.recs[id];
+    const int W = (int)r.width, H = (int)r.height;
+    if (r.filter == 0u) {
+        const int i = tex_wrap((int)__builtin_floorf(tex_coord(u * (float)W)), W, r.address_u);
+        const int j = tex_wrap((int)__builtin_floorf(tex_coord(v * (float)H)), H, r.address_v);
+        return tex_texel(T, r, i, j);
+    }
+    const float x = tex_coord(u * (float)W - 0.5f), y = tex_coord(v * (float)H - 0.5f);
+    const float x0 = __builtin_floorf(x), y0 = __builtin_floorf(y);
+    const float ax = x - x0, ay = y - y0;
+    const int i0 = tex_wrap((int)x0, W, r.address_u), i1 = tex_wrap((int)x0 + 1, W, r.address_u);
+    const int j0 = tex_wrap((int)y0, H, r.address_v), j1 = tex_wrap((int)y0 + 1, H, r.address_v);
+    const V4 t00 = tex_texel(T, r, i0, j0), t10 = tex_texel(T, r, i1, j0);
+    const V4 t01 = tex_texel(T, r, i0, j1), t11 = tex_texel(T, r, i1, j1);
+    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    const V4 top{lerp(t00.x, t10.x, ax), lerp(t00.y, t10.y, ax), lerp(t00.z, t10.z, ax), lerp(t00.w, t10.w, ax)};
+    const V4 bot{lerp(t01.x, t11.x, ax), lerp(t01.y, t11.y, ax), lerp(t01.z, t11.z, ax), lerp(t01.w, t11.w, ax)};
+    return V4{lerp(top.x, bot.x, ay), lerp(top.y, bot.y, ay), lerp(top.z, bot.z, ay), lerp(top.w, bot.w, ay)};
+}
+// Texture coordinates of a leaf at barycentrics (u, v): uv0 + (uv1 - uv0) u + (uv2 - uv0) v.
+PT_HD void leaf_texcoord(const LeafUv& L, float bu, float bv, float* s, float* t) {
+    *s = (L.uv[0] + (L.uv[2] - L.uv[0]) * bu) + (L.uv[4] - L.uv[0]) * bv;
+    *t = (L.uv[1] + (L.uv[3] - L.uv[1]) * bu) + (L.uv[5] - L.uv[1]) * bv;
+}
+
+// surface.opacity of a material at texture coordinates (s, t).
+PT_HD float surface_opacity(const MaterialRec& M, const SceneTex& T, float s, float t) {
+    if (M.opacity_tex == kNoTexture) return M.opacity;
+    const float c = comp4(tex_sample(T, M.opacity_tex - 1u, s, t), M.opacity_channel);
+    return M.opacity_step ? (c < M.step_at ? M.step_below : M.step_above) : c;
+}
+// The per-ray uniform number of stochastic translucency: one draw per ray, keyed by the
+// bits of its origin and direction (as gen_opacity_random_for, rt_gbuffer.hlsl:14-19,
+// keys it by the ray; the stream is PCG32 keyed by the job seed).
+constexpr uint32_t kOpacityBounce = 253u;
+PT_HD float opacity_random(uint32_t seed, V3 o, V3 d) {
+    const uint32_t h = __builtin_bit_cast(uint32_t, o.x) ^ __builtin_bit_cast(uint32_t, o.y) ^
+                       __builtin_bit_cast(uint32_t, o.z) ^ __builtin_bit_cast(uint32_t, d.x) ^
+                       __builtin_bit_cast(uint32_t, d.y) ^ __builtin_bit_cast(uint32_t, d.z);
+    Pcg r = pcg_keyed(seed, h, 0u, kOpacityBounce);
+    return pcg_float(r);
+}
+// Whether a non-opaque triangle counts as a hit of a ray (rt_gbuffer_rahit,
+// hits/rt_gbuffer_hit.hlsl:20-33): alpha test drops it below the 0.01 threshold,
+// translucency when the ray's number is below 1 - opacity.
+PT_HD bool surface_present(const MaterialRec& M, const SceneTex& T, const LeafUv* uv, uint32_t leaf, float bu,
+                           float bv, float rnd) {
+    float s = 0.0f, t = 0.0f;
+    if (uv) leaf_texcoord(uv[leaf], bu, bv, &s, &t);
+    const float op = surface_opacity(M, T, s, t);
+    if (M.blend_mode == kBlendAlphaTest) return !(op < kAlphaTestThreshold);
+    return !(rnd < 1.0f - op);
 }
 
 // Checkerboard cell of a world coordinate: int(floor(x)), floor clamped to

@@ -153,6 +153,15 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<bipt::MaterialRec> mats;
     std::vector<bipt::InstanceRec> inst_host;
     uint32_t num_tris = 0, num_inst = 0, num_mats = 0;
+    // texture coordinates and tangents (vertex pools, present when the materials sample
+    // textures / normal maps) and the material textures
+    bool need_uv = false, need_tan = false;
+    bipt::DevBuf<float> uvs, tans;
+    bipt::DevBuf<bipt::TexRec> tex_recs;
+    bipt::DevBuf<uint32_t> tex_pool;
+    bipt::DevBuf<float> srgb;
+    uint32_t num_tex = 0;
+    uint32_t tex_refs = 0;          // 1 + the largest texture index a material names (0: none)
     // lights
     std::vector<bipt::LightRec> lights;
     // sky
@@ -169,6 +178,8 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<int32_t> parent;
     bipt::DevBuf<bipt::LeafGeo> leaf_geo;
     bipt::DevBuf<bipt::LeafShade> leaf_shade;
+    bipt::DevBuf<bipt::LeafUv> leaf_uv;
+    bipt::DevBuf<bipt::LeafTan> leaf_tan;
     bipt::DevBuf<float> tl_cost;
     bipt::DevBuf<int32_t> tl_height, tl_snap;
     bipt::DevBuf<uint32_t> tl_starts;
@@ -212,6 +223,10 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx);
 // traces too few paths to repay them)
 bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error, uint32_t rounds);
 BvhView bvh_view(const bi_pt_ctx* ctx);
+// Texture inputs of a scene context's kernels.
+inline SceneTex scene_tex(const bi_pt_ctx* c) { return SceneTex{c->tex_recs.p, c->tex_pool.p, c->srgb.p}; }
+inline const LeafUv* scene_leaf_uv(const bi_pt_ctx* c) { return c->need_uv ? c->leaf_uv.p : nullptr; }
+inline const LeafTan* scene_leaf_tan(const bi_pt_ctx* c) { return c->need_tan ? c->leaf_tan.p : nullptr; }
 
 // One job of a call.  Roles: a plain image; an image whose paths end in the call's
 // radiance cache; the cache's training paths (one sample per slot into the cache's

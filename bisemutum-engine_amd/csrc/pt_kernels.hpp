@@ -42,6 +42,13 @@ struct BuildArgs {
     uint32_t* refit_cnt;
     LeafGeo* leaf_geo;
     LeafShade* leaf_shade;
+    const MaterialRec* materials;   // blend modes: non-opaque leaves are flagged
+    // per-vertex texture coordinates (2 floats) and tangents (4), and the per-leaf copies
+    // made from them (all null unless the scene's materials need them)
+    const float* uvs;
+    const float* tangents;
+    LeafUv* leaf_uv;
+    LeafTan* leaf_tan;
     uint32_t* error;
     // treelet restructuring after the refit (0 rounds: the Morton/Karras tree as built)
     uint32_t treelet_rounds;

@@ -49,8 +49,9 @@ struct alignas(128) Bvh4Node {
 static_assert(sizeof(Bvh4Node) == 128, "BVH4 node record is 128 bytes");
 
 struct alignas(16) LeafGeo {
-    float v[12];       // v0.xyz, e1.xyz, e2.xyz, unused
+    float v[12];       // v0.xyz, e1.xyz, e2.xyz; v[9]: flag bits (kLeafNonOpaque), then unused
 };
+constexpr uint32_t kLeafNonOpaque = 1u;   // the triangle's material is alpha-tested or translucent
 struct alignas(16) LeafShade {
     float n[9];        // n0, n1, n2 (cofactor-transformed, not normalised)
     uint32_t material;
@@ -68,10 +69,54 @@ struct MaterialRec {   // = bi_pt_material
     uint32_t two_sided;
     uint32_t pattern;           // kPatternCheckerXZ: odd cells this record, even cells pattern_material
     uint32_t pattern_material;
-    uint32_t pad;
+    uint32_t blend_mode;        // kBlendOpaque / kBlendAlphaTest / kBlendTranslucent
+    float opacity;              // surface.opacity unless opacity_tex names a texture
+    uint32_t opacity_tex;       // texture index + 1 whose channel gives the opacity (kNoTexture: none)
+    uint32_t opacity_channel;   // 0..3
+    uint32_t opacity_step;      // 1: opacity = channel < step_at ? step_below : step_above
+    float step_at, step_below, step_above;
+    uint32_t base_color_tex;    // texture index + 1: surface.base_color = texel.rgb
+    uint32_t f0_tex;            // texture index + 1: surface.f0_color = texel.rgb
+    uint32_t normal_tex;        // texture index + 1: surface.normal_map_value = texel.rgb
+    uint32_t roughness_tex;     // texture index + 1: surface.roughness = texel channel roughness_channel
+    uint32_t roughness_channel;
+    uint32_t pad[4];
 };
 constexpr uint32_t kPatternCheckerXZ = 1u;
-static_assert(sizeof(MaterialRec) == 64, "material record is 64 bytes");
+constexpr uint32_t kBlendOpaque = 0u, kBlendAlphaTest = 1u, kBlendTranslucent = 2u;
+constexpr uint32_t kNoTexture = 0u;   // texture references are index + 1, so a zeroed record has none
+constexpr float kAlphaTestThreshold = 0.01f;   // ALPHA_TEST_OPACITY_THRESHOLD, material/utils.hlsl:38
+static_assert(sizeof(MaterialRec) == 128, "material record is 128 bytes");
+
+// Texture of the material table: texels in the scene's texture pool from word `offset`
+// (RGBA8: one word per texel, RGBA32F: four), level 0 only (the reference's ray-tracing
+// materials sample level 0: material.cpp rewrites Sample into SampleLevel(..., 0)).
+struct TexRec {
+    uint32_t offset;
+    uint32_t width, height;
+    uint32_t format;            // kTexRgba8Unorm / kTexRgba8Srgb / kTexRgba32f
+    uint32_t filter;            // 0 nearest, 1 linear
+    uint32_t address_u, address_v;   // 0 repeat, 1 mirrored repeat, 2 clamp to edge
+    uint32_t pad;
+};
+constexpr uint32_t kTexRgba8Unorm = 0u, kTexRgba8Srgb = 1u, kTexRgba32f = 2u;
+static_assert(sizeof(TexRec) == 32, "texture record is 32 bytes");
+
+// Textures of a scene as the shading and any-hit code read them.
+struct SceneTex {
+    const TexRec* recs;
+    const uint32_t* pool;
+    const float* srgb;          // 256 entries: sRGB-encoded byte -> linear
+};
+
+// Per-leaf texture coordinates (uv0, uv1, uv2, unused) and world tangents (xyz, w = the
+// bitangent sign) of the three vertices; present when the scene's meshes carry them.
+struct alignas(16) LeafUv {
+    float uv[8];
+};
+struct alignas(16) LeafTan {
+    float t[12];
+};
 
 struct LightRec {      // = bi_pt_dir_light (DirLightData)
     float emission[3];
@@ -116,6 +161,12 @@ struct BvhView {
     const Bvh4Node* nodes;
     const LeafGeo* leaf_geo;
     uint32_t num_tris;
+    // any-hit inputs of non-opaque leaves
+    const LeafShade* leaf_shade;
+    const MaterialRec* materials;
+    const LeafUv* leaf_uv;
+    SceneTex tex;
+    uint32_t seed;
 };
 
 // Job kinds beside plain images (JobRec::flags).
@@ -149,6 +200,9 @@ struct JobRec {
     const LeafGeo* leaf_geo;
     const LeafShade* leaf_shade;
     const MaterialRec* materials;
+    const LeafUv* leaf_uv;          // null: the scene has no texture coordinates
+    const LeafTan* leaf_tan;        // null: no tangents (normal maps are not applied)
+    SceneTex tex;
     uint32_t num_tris;
     uint32_t num_lights;
     uint32_t sky_nee;       // sky next-event estimation on

@@ -46,6 +46,40 @@ struct ShadeOut {
     V3 c[NL + 1];   // shadow contribution of light l at [l], of the sky at [NL]
 };
 
+// Texture-driven surface fields at a hit (material_function statements that sample the
+// material's textures at vertex.texcoord, hits/rt_gbuffer_hit.hlsl:8-11): base colour,
+// f0, roughness, and the normal map applied in the interpolated tangent frame,
+// N = normalize(n.x T + n.y B + n.z N) with n = 2 texel - 1, T = normalize(tangent.xyz),
+// B = normalize(cross(N, T)) * tangent.w (hit.hlsl:155-157).
+__device__ __forceinline__ void apply_textures(const JobRec& J, const MaterialRec& M, uint32_t leaf, float hu,
+                                               float hv, Mat& m, V3& N, float& rough) {
+    float s = 0.0f, t = 0.0f;
+    if (J.leaf_uv) leaf_texcoord(J.leaf_uv[leaf], hu, hv, &s, &t);
+    if (M.base_color_tex != kNoTexture) {
+        const V4 c = tex_sample(J.tex, M.base_color_tex - 1u, s, t);
+        m.base = v3(c.x, c.y, c.z);
+    }
+    if (M.f0_tex != kNoTexture) {
+        const V4 c = tex_sample(J.tex, M.f0_tex - 1u, s, t);
+        m.f0 = v3(c.x, c.y, c.z);
+    }
+    if (M.roughness_tex != kNoTexture) {
+        rough = comp4(tex_sample(J.tex, M.roughness_tex - 1u, s, t), M.roughness_channel);
+        mat_alpha(m, rough, M.anisotropy);
+    }
+    if (M.normal_tex != kNoTexture && J.leaf_tan) {
+        const V4 c = tex_sample(J.tex, M.normal_tex - 1u, s, t);
+        const V3 nt = v3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+        const float* g = J.leaf_tan[leaf].t;
+        const V3 T = normalize(v3((g[0] + (g[4] - g[0]) * hu) + (g[8] - g[0]) * hv,
+                                  (g[1] + (g[5] - g[1]) * hu) + (g[9] - g[1]) * hv,
+                                  (g[2] + (g[6] - g[2]) * hu) + (g[10] - g[2]) * hv));
+        const float tw = (g[3] + (g[7] - g[3]) * hu) + (g[11] - g[3]) * hv;
+        const V3 B = normalize(cross(N, T)) * tw;
+        N = normalize((T * nt.x + B * nt.y) + N * nt.z);
+    }
+}
+
 // Shades queue entry i (path `path`, the job's local path `local`) of job J: its rays
 // go to `out` (mask: which of them exist).
 template <uint32_t NL>
@@ -111,6 +145,8 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
             mat_id = cell_id;
             m = load_mat(J.materials[mat_id]);
         }
+        float rough = J.materials[mat_id].roughness;
+        if (mat_textured(J.materials[mat_id])) apply_textures(J, J.materials[mat_id], leaf, hu, hv, m, N, rough);
         if (m.two_sided && dot(d, N) > 0.0f) N = -N;
         V3 V = normalize(o - P);
         if (a.bounce == J.cache_bounce && (J.flags & (kJobCache | kJobTrain))) {
@@ -118,7 +154,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
             r.p[0] = P.x; r.p[1] = P.y; r.p[2] = P.z;
             r.wo[0] = V.x; r.wo[1] = V.y; r.wo[2] = V.z;
             r.n[0] = N.x; r.n[1] = N.y; r.n[2] = N.z;
-            r.roughness = J.materials[mat_id].roughness;
+            r.roughness = rough;
             r.albedo[0] = m.base.x; r.albedo[1] = m.base.y; r.albedo[2] = m.base.z;
             r.pad[0] = 0.0f; r.pad[1] = 0.0f;
             if (J.flags & kJobCache) {

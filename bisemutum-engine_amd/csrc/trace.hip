@@ -67,18 +67,28 @@ __device__ __forceinline__ bool box_k(const Node4& n, int k, V3 o, V3 inv, float
     return tn <= tf * kRobust;
 }
 
-// Leaf intersection record held in registers: v0, e1, e2.
+// Leaf intersection record held in registers: v0, e1, e2 and the flag word.
 struct LeafRegs {
     float4 g0, g1;
-    float g2;
+    float2 g2;
 };
 __device__ __forceinline__ LeafRegs load_leaf(const LeafGeo* geo, uint32_t leaf) {
     const float4* p = reinterpret_cast<const float4*>(geo + leaf);
     LeafRegs r;
     r.g0 = p[0];
     r.g1 = p[1];
-    r.g2 = reinterpret_cast<const float*>(p + 2)[0];
+    r.g2 = reinterpret_cast<const float2*>(p + 2)[0];
     return r;
+}
+__device__ __forceinline__ bool leaf_non_opaque(const LeafRegs& r) {
+    return (__float_as_uint(r.g2.y) & kLeafNonOpaque) != 0u;
+}
+// Any-hit test of a non-opaque leaf hit at barycentric numerators (un, vn) / det by the
+// ray (o, d) of a job: whether its surface is present for that ray.
+__device__ __forceinline__ bool leaf_present(const JobRec& J, uint32_t leaf, float un, float vn, float det, V3 o,
+                                             V3 d) {
+    const MaterialRec& M = J.materials[J.leaf_shade[leaf].material];
+    return surface_present(M, J.tex, J.leaf_uv, leaf, un / det, vn / det, opacity_random(J.seed, o, d));
 }
 
 // Moller-Trumbore with division-free barycentric acceptance; t = tn / det.
@@ -86,7 +96,7 @@ __device__ __forceinline__ bool tri_test(const LeafRegs& g, V3 o, V3 d, float* t
                                          float* det_out) {
     V3 v0 = v3(g.g0.x, g.g0.y, g.g0.z);
     V3 e1 = v3(g.g0.w, g.g1.x, g.g1.y);
-    V3 e2 = v3(g.g1.z, g.g1.w, g.g2);
+    V3 e2 = v3(g.g1.z, g.g1.w, g.g2.x);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
     if (det == 0.0f) return false;
@@ -270,6 +280,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
     uint32_t best = kLeafMiss;
     const Bvh4Node* nodes = a.jobs.rec[0].nodes;
     const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
+    const JobRec* jr = a.jobs.rec;   // the lane's job (materials for any-hit tests)
     uint32_t max_steps = 1;
     ShortStack<L, B> st;
     int sp = 0;
@@ -334,15 +345,20 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
                 const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
                 const float tb = __shfl(tbest, owner);
                 const LeafGeo* og = shfl_ptr(geo, owner);
-                if (valid) {
-                    // a hit counts only where the leaf's own box passes the slab test up to
-                    // the hit distance (the acceptance rule that makes the closest hit
-                    // independent of the order leaves are found in)
-                    float t, un, vn, det;
-                    if (tri_test(og, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tb &&
-                        tnl <= t * kRobust)
-                        atomicMin(&key[owner], ((unsigned long long)__float_as_uint(t) << 32) | leaf);
+                // a hit counts only where the leaf's own box passes the slab test up to the
+                // hit distance (the acceptance rule that makes the closest hit independent
+                // of the order leaves are found in), and, for a non-opaque leaf, where its
+                // surface is present for the ray (a function of the ray and the leaf alone)
+                float t = 0.0f, un = 0.0f, vn = 0.0f, det = 1.0f;
+                const LeafRegs lr = valid ? load_leaf(og, leaf) : LeafRegs{};
+                bool cand = valid && tri_test(lr, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tb &&
+                            tnl <= t * kRobust;
+                const bool nop = cand && leaf_non_opaque(lr);
+                if (__ballot(nop) != 0ull) {
+                    const JobRec* oj = shfl_ptr(jr, owner);
+                    if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
                 }
+                if (cand) atomicMin(&key[owner], ((unsigned long long)__float_as_uint(t) << 32) | leaf);
             }
             __builtin_amdgcn_wave_barrier();
             qn = 0;
@@ -382,6 +398,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
             auto setup = [&](const JobRec& J, uint32_t local) {
                 nodes = J.nodes;
                 geo = J.leaf_geo;
+                jr = &J;
                 max_steps = J.num_tris > 1u ? J.num_tris : 1u;
                 tmax = J.ray_length;
                 if (PRIMARY) {
@@ -449,7 +466,12 @@ __device__ __forceinline__ bool occluded(const BvhView& bvh, int (*lds)[256], V3
             const int c = comp(nb.ch, k);
             if (c < 0) {
                 float t, un, vn, det;
-                if (tri_test(bvh.leaf_geo, (uint32_t)~c, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
+                const uint32_t leaf = (uint32_t)~c;
+                const LeafRegs lr = load_leaf(bvh.leaf_geo, leaf);
+                if (tri_test(lr, o, d, &t, &un, &vn, &det) && t >= tmin && t <= tmax &&
+                    (!leaf_non_opaque(lr) ||
+                     surface_present(bvh.materials[bvh.leaf_shade[leaf].material], bvh.tex, bvh.leaf_uv, leaf,
+                                     un / det, vn / det, opacity_random(bvh.seed, o, d))))
                     return true;
             } else if (next == kDone) {
                 next = c;
@@ -523,6 +545,7 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f), inv = v3(0.f, 0.f, 0.f);
     const Bvh4Node* nodes = a.jobs.rec[0].nodes;   // the lane's ray's scene (see k_trace_closest)
     const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
+    const JobRec* jr = a.jobs.rec;
     uint32_t max_steps = 1;
     ShortStack<L, B> st;
     int sp = 0;
@@ -579,11 +602,15 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
                 const V3 oo = v3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
                 const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
                 const LeafGeo* og = shfl_ptr(geo, owner);
-                if (valid) {
-                    float t, un, vn, det;
-                    if (tri_test(og, leaf, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax)
-                        occf[owner] = 1u;
+                float t = 0.0f, un = 0.0f, vn = 0.0f, det = 1.0f;
+                const LeafRegs lr = valid ? load_leaf(og, leaf) : LeafRegs{};
+                bool cand = valid && tri_test(lr, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax;
+                const bool nop = cand && leaf_non_opaque(lr);
+                if (__ballot(nop) != 0ull) {
+                    const JobRec* oj = shfl_ptr(jr, owner);
+                    if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
                 }
+                if (cand) occf[owner] = 1u;
             }
             __builtin_amdgcn_wave_barrier();
             qn = 0;
@@ -603,6 +630,7 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
             auto setup = [&](const JobRec& J) {
                 nodes = J.nodes;
                 geo = J.leaf_geo;
+                jr = &J;
                 max_steps = J.num_tris > 1u ? J.num_tris : 1u;
             };
             if (a.jobs.num_jobs == 1u) setup(a.jobs.rec[0]);

@@ -61,21 +61,47 @@ class Scene:
     sky_color: np.ndarray  # f32 [3]
     sky_rot: np.ndarray    # f32 [9] row-major
     notes: dict = dataclasses.field(default_factory=dict)
+    mesh_uv: list = None    # per mesh: f32 [V, 2] texture coordinates, or None (all meshes without)
+    mesh_tan: list = None   # per mesh: f32 [V, 4] tangents (xyz, bitangent sign), or None
+    textures: list = dataclasses.field(default_factory=list)   # Texture records
 
     @property
     def num_tris(self) -> int:
         return int(sum(len(self.meshes[m][2]) for m in self.instances["mesh"]))
 
 
+@dataclasses.dataclass
+class Texture:
+    """A material texture (bi_pt_texture): level 0, texels [H, W, 4] from v = 0 (uint8
+    RGBA or float32 RGBA), format TEXEL_*, filter 0 nearest / 1 linear, addressing 0
+    repeat / 1 mirrored repeat / 2 clamp to edge."""
+    texels: np.ndarray
+    format: int = 0          # TEXEL_RGBA8_UNORM
+    filter: int = 1
+    address_u: int = 0
+    address_v: int = 0
+
+
 INSTANCE_DTYPE = np.dtype([("xform", np.float32, (12,)), ("mesh", np.uint32), ("material", np.uint32)])
 
 
+MATERIAL_FLOATS = 32   # bi_pt_material: 128 bytes
+# float/word index of the bi_pt_material fields
+M_BLEND, M_OPACITY, M_OPACITY_TEX, M_OPACITY_CHANNEL, M_OPACITY_STEP = 15, 16, 17, 18, 19
+M_STEP_AT, M_STEP_BELOW, M_STEP_ABOVE = 20, 21, 22
+M_BASE_TEX, M_F0_TEX, M_NORMAL_TEX, M_ROUGH_TEX, M_ROUGH_CHANNEL = 23, 24, 25, 26, 27
+BLEND_OPAQUE, BLEND_ALPHA_TEST, BLEND_TRANSLUCENT = 0, 1, 2
+TEXEL_RGBA8_UNORM, TEXEL_RGBA8_SRGB, TEXEL_RGBA32F = 0, 1, 2
+
+
 def material(base_color, roughness, metallic, two_sided=True, anisotropy=0.0):
-    """bi_pt_material record (16 floats) with the importer's metallic mapping."""
+    """bi_pt_material record (32 floats) with the importer's metallic mapping: opaque,
+    opacity 1, no textures."""
     base = np.asarray(base_color, np.float32)
     m = np.float32(metallic)
     f0 = (np.float32(0.04) + (base - np.float32(0.04)) * m).astype(np.float32)  # lerp(0.04, base, metallic)
-    rec = np.zeros(16, np.float32)
+    rec = np.zeros(MATERIAL_FLOATS, np.float32)
+    rec[M_OPACITY] = 1.0
     rec[0:3] = base
     rec[3] = roughness
     rec[4:7] = f0

@@ -56,13 +56,19 @@ typedef struct bi_pt_device_desc {
 
 /* One mesh (host memory, copied at bi_pt_set_scene): float xyz positions and
  * normals (flat float buffers of 3 per vertex, as the reference's
- * positions_buffer / normals_buffer), uint32 triangle-list indices. */
+ * positions_buffer / normals_buffer), uint32 triangle-list indices, and optionally
+ * texture coordinates (2 floats per vertex, texcoords_buffer) and tangents (4 floats per
+ * vertex: xyz and the bitangent sign w, tangents_buffer; gpu_scene_data.hpp:10-24).
+ * Missing texture coordinates read as 0 and missing tangents as (1, 0, 0, 1), the
+ * defaults of fetch_vertex_attributes (hit.hlsl:74, 116). */
 typedef struct bi_pt_mesh {
     const float* positions;
     const float* normals;
     const uint32_t* indices;
     uint32_t num_vertices;
     uint32_t num_indices;
+    const float* texcoords;      /* nullable */
+    const float* tangents;       /* nullable */
 } bi_pt_mesh;
 
 /* One drawable: 3x4 row-major object-to-world transform, laid out like
@@ -73,18 +79,38 @@ typedef struct bi_pt_instance {
     uint32_t material;
 } bi_pt_instance;
 
-/* Lit material (SurfaceData of shaders/core/material/utils.hlsl:5-16 restricted
- * to parameter materials).  64 bytes.  anisotropy follows
+/* Lit material (SurfaceData of shaders/core/material/utils.hlsl:5-31, as the
+ * material_function of a material asset fills it).  128 bytes.  anisotropy follows
  * get_anisotropic_roughness; ior is only used for cos < 0 (never for reflection).
  * pattern BI_PT_PATTERN_CHECKER_XZ: the world-space checkerboard of the example
  * project's material (examples/scene_basic/materials/checkerboard.toml): at a hit
  * point P, cells with (int(floor(P.x)) ^ int(floor(P.z))) odd use this record and
  * even cells use record pattern_material (which must have no pattern itself);
- * floor(P) is clamped to [-2^31, 2^31 - 128] before the conversion. */
+ * floor(P) is clamped to [-2^31, 2^31 - 128] before the conversion.
+ * Texture-driven fields (material functions that sample textures at vertex.texcoord,
+ * examples/scene_basic/materials/{textured,cage}.toml): base_color_tex / f0_tex set the
+ * colour from the texel's rgb, roughness_tex the roughness from one channel,
+ * normal_tex the tangent-space normal map value (rgb; N = normalize(n.x T + n.y B +
+ * n.z N), n = 2 rgb - 1, hits/rt_gbuffer_hit.hlsl:8-11).  Textures are sampled at level
+ * 0 with their own filter and addressing (bi_pt_set_textures).
+ * blend_mode (accel.cpp:108-116, hits/rt_gbuffer_hit.hlsl:20-33): an alpha-tested
+ * surface is absent where its opacity is below 0.01; a translucent one is absent for a
+ * ray whose uniform number (one per ray) is below 1 - opacity.  The opacity is
+ * `opacity`, or channel opacity_channel of opacity_tex, stepped to step_below /
+ * step_above at step_at when opacity_step is set (cage.toml: value.w < 0.5 ? 0 : 1).
+ * Blend mode and opacity come from the triangle's own record (a checkerboard's cells
+ * share them).  Texture references hold the texture's index + 1 (BI_PT_NO_TEXTURE = 0:
+ * none), so a zero-initialised record is an opaque parameter material. */
 enum {
     BI_PT_PATTERN_NONE = 0,
     BI_PT_PATTERN_CHECKER_XZ = 1
 };
+enum {
+    BI_PT_BLEND_OPAQUE = 0,
+    BI_PT_BLEND_ALPHA_TEST = 1,
+    BI_PT_BLEND_TRANSLUCENT = 2
+};
+#define BI_PT_NO_TEXTURE 0u
 typedef struct bi_pt_material {
     float base_color[3];
     float roughness;
@@ -95,8 +121,42 @@ typedef struct bi_pt_material {
     uint32_t two_sided;
     uint32_t pattern;            /* BI_PT_PATTERN_* (0 for a parameter material) */
     uint32_t pattern_material;   /* the record of even cells */
-    uint32_t _pad;
+    uint32_t blend_mode;         /* BI_PT_BLEND_* */
+    float opacity;
+    uint32_t opacity_tex;
+    uint32_t opacity_channel;    /* 0..3 = r, g, b, a */
+    uint32_t opacity_step;
+    float step_at, step_below, step_above;
+    uint32_t base_color_tex;
+    uint32_t f0_tex;
+    uint32_t normal_tex;
+    uint32_t roughness_tex;
+    uint32_t roughness_channel;  /* 0..3 */
+    uint32_t _pad[4];
 } bi_pt_material;
+
+/* A material texture (host memory, copied at bi_pt_set_textures): level 0 of the
+ * engine's texture asset (TextureAsset, src/scene_basic/texture.cpp:92-136) with its
+ * sampler's level-0 filter and addressing (rhi::SamplerDesc, include/bisemutum/rhi/sampler.hpp:37-53).
+ * texels: width * height, row-major from v = 0; RGBA8 as 4 bytes per texel (UNORM,
+ * or sRGB decoded to linear before filtering), RGBA32F as 4 floats. */
+enum {
+    BI_PT_TEXEL_RGBA8_UNORM = 0,
+    BI_PT_TEXEL_RGBA8_SRGB = 1,
+    BI_PT_TEXEL_RGBA32F = 2
+};
+enum {
+    BI_PT_ADDRESS_REPEAT = 0,
+    BI_PT_ADDRESS_MIRRORED_REPEAT = 1,
+    BI_PT_ADDRESS_CLAMP_TO_EDGE = 2
+};
+typedef struct bi_pt_texture {
+    const void* texels;
+    uint32_t width, height;
+    uint32_t format;             /* BI_PT_TEXEL_* */
+    uint32_t filter;             /* 0 nearest, 1 linear */
+    uint32_t address_u, address_v;   /* BI_PT_ADDRESS_* */
+} bi_pt_texture;
 
 /* Byte-identical to DirLightData (shaders/renderer/lights_struct.hlsl:3-14,
  * src/renderer/context/lights.hpp:12-22).  64 bytes.  emission = colour*strength,
@@ -190,6 +250,10 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
                              const bi_pt_material* materials, uint32_t num_materials);
 /* Replace instance transforms only (same count/meshes/materials); the LBVH is rebuilt by the next render. */
 bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* transforms_3x4, uint32_t num_instances);
+/* The material textures of the scene (the material table names them by index); replaces
+ * the previous set.  A render fails (INVALID_ARGUMENT) while a material names a texture
+ * that is not set. */
+bi_pt_status bi_pt_set_textures(bi_pt_ctx* ctx, const bi_pt_texture* textures, uint32_t num_textures);
 bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uint32_t num_lights);
 /* Equirect RGBA32F sky (host memory, row 0 = +Y pole), colour multiplier
  * (SkyboxSystem color*strength) and 3x3 row-major rotation applied to directions

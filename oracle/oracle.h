@@ -37,14 +37,24 @@ extern "C" {
 
 typedef struct ora_ctx ora_ctx;
 
+/* Material texture (host memory): level 0, width * height texels from v = 0, format
+ * 0 RGBA8 UNORM, 1 RGBA8 sRGB, 2 RGBA32F; filter 0 nearest / 1 linear; addressing 0
+ * repeat, 1 mirrored repeat, 2 clamp to edge (as bi_pt_texture). */
+typedef struct ora_texture {
+    const void* texels;
+    uint32_t width, height, format, filter, address_u, address_v;
+} ora_texture;
+
 /* Scene description, all host memory.
  * meshes are concatenated: mesh m owns vertices [vtx_off[m], vtx_off[m]+vtx_cnt[m])
  * of positions/normals (3 floats each) and indices [idx_off[m], idx_off[m]+idx_cnt[m])
  * (mesh-local vertex numbers).  Instance i places mesh inst_mesh[i] with the
  * 3x4 row-major transform inst_xform[12*i..] and material inst_mat[i].
- * materials: 16 floats each, layout of bi_pt_material (include/bi_pt.h).
+ * materials: 32 floats each, layout of bi_pt_material (include/bi_pt.h).
  * dir_lights: 16 floats each, layout of bi_pt_dir_light (= DirLightData).
- * sky: sky_w*sky_h RGBA32F texels (row 0 = theta 0, the +Y pole). */
+ * sky: sky_w*sky_h RGBA32F texels (row 0 = theta 0, the +Y pole).
+ * texcoords (2 floats per vertex) and tangents (4 per vertex) run parallel to the
+ * positions (nullable: 0 and (1, 0, 0, 1)). */
 typedef struct ora_scene_desc {
     uint32_t num_meshes;
     const float* positions;
@@ -65,6 +75,10 @@ typedef struct ora_scene_desc {
     const float* sky_rgba;
     float sky_color[3];
     float sky_rot[9];
+    const float* texcoords;
+    const float* tangents;
+    uint32_t num_textures;
+    const ora_texture* textures;
 } ora_scene_desc;
 
 /* Camera: row-major 4x4 matrices (m[row*4+col]), as bi_pt_camera. */
@@ -161,6 +175,8 @@ int ora_render64(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
                  double* mean_rgb, double* var_rgb, int num_threads);
 
 /* ---- unit-level entry points (for known-answer and invariant tests) ---- */
+/* Level-0 samples of scene texture `tex` at n (u, v) pairs: 4 floats each. */
+void ora_tex_sample32(const ora_ctx* ctx, uint32_t tex, uint32_t n, const float* uv, float* out4);
 void ora_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out);
 void ora_pcg32_keyed(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
                      uint32_t n, float* out);

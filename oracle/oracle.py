@@ -22,6 +22,11 @@ _u8p = C.POINTER(C.c_uint8)
 _u64p = C.POINTER(C.c_uint64)
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
+                ("filter", C.c_uint32), ("address_u", C.c_uint32), ("address_v", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("num_meshes", C.c_uint32), ("positions", _f32p), ("normals", _f32p), ("indices", _u32p),
@@ -31,6 +36,8 @@ class SceneDesc(C.Structure):
         ("num_dir_lights", C.c_uint32), ("dir_lights", _f32p),
         ("sky_w", C.c_uint32), ("sky_h", C.c_uint32), ("sky_rgba", _f32p),
         ("sky_color", C.c_float * 3), ("sky_rot", C.c_float * 9),
+        ("texcoords", _f32p), ("tangents", _f32p),
+        ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
     ]
 
 
@@ -86,6 +93,7 @@ def lib():
         L.ora_render64.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, _u32p,
                                    C.c_uint32, C.c_uint32, _f64p, _f64p, C.c_int]
         L.ora_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
+        L.ora_tex_sample32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, _f32p, _f32p]
         L.ora_pcg32_keyed.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
         L.ora_morton3.restype = C.c_uint32
         L.ora_morton3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
@@ -162,6 +170,25 @@ class OracleScene:
             d.sky_color[i] = float(scene.sky_color[i])
         for i in range(9):
             d.sky_rot[i] = float(scene.sky_rot[i])
+        nv = [len(p) for p in pos]
+        if scene.mesh_uv is not None:
+            k["uv"] = np.ascontiguousarray(np.concatenate(
+                [np.zeros((n, 2), np.float32) if u is None else np.asarray(u, np.float32).reshape(n, 2)
+                 for n, u in zip(nv, scene.mesh_uv)]), np.float32)
+            d.texcoords = ptr(k["uv"], _f32p)
+        if scene.mesh_tan is not None:
+            k["tan"] = np.ascontiguousarray(np.concatenate(
+                [np.tile(np.array([1, 0, 0, 1], np.float32), (n, 1)) if t is None else
+                 np.asarray(t, np.float32).reshape(n, 4) for n, t in zip(nv, scene.mesh_tan)]), np.float32)
+            d.tangents = ptr(k["tan"], _f32p)
+        if scene.textures:
+            k["tex_data"] = [np.ascontiguousarray(t.texels) for t in scene.textures]
+            k["tex"] = (TextureDesc * len(scene.textures))()
+            for i, (t, a) in enumerate(zip(scene.textures, k["tex_data"])):
+                k["tex"][i] = TextureDesc(a.ctypes.data, a.shape[1], a.shape[0], t.format, t.filter,
+                                          t.address_u, t.address_v)
+            d.num_textures = len(scene.textures)
+            d.textures = k["tex"]
         self.h = lib().ora_create(C.byref(d))
         self.n = lib().ora_num_tris(self.h)
 
@@ -290,6 +317,13 @@ class OracleScene:
                                 ptr(mean, _f64p), ptr(var, _f64p), threads)
         assert rc == 0
         return mean, var
+
+    def tex_sample(self, tex, uv):
+        """Level-0 samples [n, 4] of texture `tex` at uv [n, 2]."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 4), np.float32)
+        lib().ora_tex_sample32(self.h, tex, len(uv), ptr(uv, _f32p), ptr(out, _f32p))
+        return out
 
     def sky_eval(self, d):
         d = np.ascontiguousarray(d, np.float32)

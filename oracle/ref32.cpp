@@ -554,7 +554,93 @@ void build_bvh(Scene& s) {
 namespace ora {
 namespace {
 
-struct Ray { F3 o, d; float tmin, tmax; };
+// ---------------------------------------------------------------- textures
+// Level-0 sampling of the material textures (SampleLevel(..., 0), material.cpp:325-349):
+// texel centres at (i + 0.5) / W, bilinear weights in full FP32, Vulkan's repeat /
+// mirrored-repeat / clamp-to-edge addressing, RGBA8 as c / 255 (UNORM) or through the
+// sRGB table.  The same operations in the same order as the HIP kernels.
+struct F4 { float x, y, z, w; };
+inline float comp4(const F4& v, uint32_t c) { return c == 0u ? v.x : c == 1u ? v.y : c == 2u ? v.z : v.w; }
+inline int tex_wrap(int i, int n, uint32_t mode) {
+    if (mode == 2u) return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    if (mode == 1u) {
+        int m = i % (2 * n);
+        if (m < 0) m += 2 * n;
+        return m >= n ? 2 * n - 1 - m : m;
+    }
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+inline F4 tex_texel(const Scene& s, const Tex& t, int i, int j) {
+    const size_t k = (size_t)j * t.width + (size_t)i;
+    if (t.format == 2u) {
+        float f[4];
+        std::memcpy(f, &t.words[4 * k], sizeof f);
+        return F4{f[0], f[1], f[2], f[3]};
+    }
+    const uint32_t w = t.words[k];
+    const uint32_t c0 = w & 255u, c1 = (w >> 8) & 255u, c2 = (w >> 16) & 255u, c3 = w >> 24;
+    if (t.format == 1u) return F4{s.srgb[c0], s.srgb[c1], s.srgb[c2], (float)c3 / 255.0f};
+    return F4{(float)c0 / 255.0f, (float)c1 / 255.0f, (float)c2 / 255.0f, (float)c3 / 255.0f};
+}
+inline float tex_coord(float x) { return (x > -16777216.0f && x < 16777216.0f) ? x : 0.0f; }
+F4 tex_sample(const Scene& s, uint32_t id, float u, float v) {
+    const Tex& t = s.textures[id];
+    const int W = (int)t.width, H = (int)t.height;
+    if (t.filter == 0u) {
+        const int i = tex_wrap((int)std::floor(tex_coord(u * (float)W)), W, t.address_u);
+        const int j = tex_wrap((int)std::floor(tex_coord(v * (float)H)), H, t.address_v);
+        return tex_texel(s, t, i, j);
+    }
+    const float x = tex_coord(u * (float)W - 0.5f), y = tex_coord(v * (float)H - 0.5f);
+    const float x0 = std::floor(x), y0 = std::floor(y);
+    const float ax = x - x0, ay = y - y0;
+    const int i0 = tex_wrap((int)x0, W, t.address_u), i1 = tex_wrap((int)x0 + 1, W, t.address_u);
+    const int j0 = tex_wrap((int)y0, H, t.address_v), j1 = tex_wrap((int)y0 + 1, H, t.address_v);
+    const F4 t00 = tex_texel(s, t, i0, j0), t10 = tex_texel(s, t, i1, j0);
+    const F4 t01 = tex_texel(s, t, i0, j1), t11 = tex_texel(s, t, i1, j1);
+    auto lerp = [](float a, float b, float w) { return a + (b - a) * w; };
+    const F4 top{lerp(t00.x, t10.x, ax), lerp(t00.y, t10.y, ax), lerp(t00.z, t10.z, ax), lerp(t00.w, t10.w, ax)};
+    const F4 bot{lerp(t01.x, t11.x, ax), lerp(t01.y, t11.y, ax), lerp(t01.z, t11.z, ax), lerp(t01.w, t11.w, ax)};
+    return F4{lerp(top.x, bot.x, ay), lerp(top.y, bot.y, ay), lerp(top.z, bot.z, ay), lerp(top.w, bot.w, ay)};
+}
+inline uint32_t mat_word(const Scene& s, uint32_t mat, int k) {
+    uint32_t w;
+    std::memcpy(&w, &s.materials[kMatFloats * mat + k], 4);
+    return w;
+}
+// Texture coordinates of triangle `tri` at barycentrics (u, v).
+inline void tri_texcoord(const Scene& s, uint32_t tri, float bu, float bv, float* su, float* sv) {
+    if (!s.need_uv) { *su = 0.0f; *sv = 0.0f; return; }
+    const float* q = &s.tri_uv[6 * (size_t)tri];
+    *su = (q[0] + (q[2] - q[0]) * bu) + (q[4] - q[0]) * bv;
+    *sv = (q[1] + (q[3] - q[1]) * bu) + (q[5] - q[1]) * bv;
+}
+// Whether a non-opaque triangle counts as a hit of a ray (rt_gbuffer_rahit,
+// hits/rt_gbuffer_hit.hlsl:20-33): the alpha test drops it below opacity 0.01,
+// translucency when the ray's number (PCG32 keyed by the job seed and the xor of the
+// ray's origin and direction bits, stream 253) is below 1 - opacity.
+constexpr uint32_t kOpacityBounce = 253u;
+inline uint32_t fbits(float x) { uint32_t u; std::memcpy(&u, &x, 4); return u; }
+bool surface_present(const Scene& s, uint32_t tri, float bu, float bv, uint32_t seed, F3 o, F3 d) {
+    const uint32_t mat = s.tri_mat[tri];
+    const float* M = &s.materials[kMatFloats * mat];
+    float op = M[16];
+    const uint32_t otex = mat_word(s, mat, 17);
+    if (otex) {
+        float su, sv;
+        tri_texcoord(s, tri, bu, bv, &su, &sv);
+        const float c = comp4(tex_sample(s, otex - 1u, su, sv), mat_word(s, mat, 18));
+        op = mat_word(s, mat, 19) ? (c < M[20] ? M[21] : M[22]) : c;
+    }
+    if (mat_word(s, mat, 15) == 1u) return !(op < 0.01f);
+    Pcg r = pcg_keyed(seed, fbits(o.x) ^ fbits(o.y) ^ fbits(o.z) ^ fbits(d.x) ^ fbits(d.y) ^ fbits(d.z), 0u,
+                      kOpacityBounce);
+    return !(pcg_float(r) < 1.0f - op);
+}
+inline bool tri_opaque(const Scene& s, uint32_t tri) { return mat_word(s, s.tri_mat[tri], 15) == 0u; }
+
+struct Ray { F3 o, d; float tmin, tmax; uint32_t seed = 0; };
 struct Hit { float t; uint32_t leaf; float u, v; };
 constexpr uint32_t kMiss = 0xffffffffu;
 constexpr int kStackMax = 192;   // 3 pushes per BVH4 level over at most 63 levels
@@ -648,7 +734,9 @@ Hit trace_closest(const Scene& s, const Ray& r) {
             const uint32_t leaf = (uint32_t)~nd.child[j];
             float t, un, vn, det;
             if (tri_test(s, leaf, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin && tn[j] <= t * kRobust &&
-                (t < h.t || (t == h.t && leaf < h.leaf))) {
+                (t < h.t || (t == h.t && leaf < h.leaf)) &&
+                (tri_opaque(s, s.leaf_tri[leaf]) ||
+                 surface_present(s, s.leaf_tri[leaf], un / det, vn / det, r.seed, r.o, r.d))) {
                 h.t = t; h.leaf = leaf; best_un = un; best_vn = vn; best_det = det;
             }
         }
@@ -687,7 +775,10 @@ bool trace_any(const Scene& s, const Ray& r) {
         for (int j = 0; j < 4; ++j) {
             if (!hit[j] || nd.child[j] >= 0) continue;
             float t, un, vn, det;
-            if (tri_test(s, (uint32_t)~nd.child[j], r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin && t <= r.tmax)
+            const uint32_t leaf = (uint32_t)~nd.child[j];
+            if (tri_test(s, leaf, r.o, r.d, &t, &un, &vn, &det) && t >= r.tmin && t <= r.tmax &&
+                (tri_opaque(s, s.leaf_tri[leaf]) ||
+                 surface_present(s, s.leaf_tri[leaf], un / det, vn / det, r.seed, r.o, r.d)))
                 return true;
         }
         if (!ok || steps > s.num_tris) return false;
@@ -728,8 +819,8 @@ inline int checker_cell(float x) {
 }
 inline uint32_t resolve_mat(const std::vector<float>& mats, uint32_t id, F3 P) {
     uint32_t pattern, alt;
-    std::memcpy(&pattern, &mats[16 * (size_t)id + 13], 4);
-    std::memcpy(&alt, &mats[16 * (size_t)id + 14], 4);
+    std::memcpy(&pattern, &mats[kMatFloats * (size_t)id + 13], 4);
+    std::memcpy(&alt, &mats[kMatFloats * (size_t)id + 14], 4);
     if (pattern == 1u && ((checker_cell(P.x) ^ checker_cell(P.z)) & 1) == 0) return alt;
     return id;
 }
@@ -739,21 +830,22 @@ struct Mat {
     float rx, ry;
     bool two_sided;
 };
-inline Mat load_mat(const float* m) {
-    Mat r;
-    r.base = f3(m[0], m[1], m[2]);
-    float rough = m[3];
-    r.f0 = f3(m[4], m[5], m[6]);
-    float aniso_p = m[7];
-    r.f90 = f3(m[8], m[9], m[10]);
-    uint32_t ts;
-    std::memcpy(&ts, &m[12], sizeof ts);
-    r.two_sided = ts != 0u;
-    // get_anisotropic_roughness (utils.hlsl:54-59)
+// get_anisotropic_roughness (utils.hlsl:54-59)
+inline void mat_alpha(Mat& r, float rough, float aniso_p) {
     float aniso = std::sqrt(1.0f - aniso_p * 0.9f);
     float r2 = rough * rough;
     r.rx = fmax_(r2 / aniso, 0.001f);
     r.ry = fmax_(r2 * aniso, 0.001f);
+}
+inline Mat load_mat(const float* m) {
+    Mat r;
+    r.base = f3(m[0], m[1], m[2]);
+    r.f0 = f3(m[4], m[5], m[6]);
+    r.f90 = f3(m[8], m[9], m[10]);
+    uint32_t ts;
+    std::memcpy(&ts, &m[12], sizeof ts);
+    r.two_sided = ts != 0u;
+    mat_alpha(r, m[3], m[7]);
     return r;
 }
 
@@ -984,6 +1076,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
     camera_ray(cam, px, py, jx, jy, &ray.o, &ray.d);
     ray.tmin = kTMin;
     ray.tmax = st.ray_length;
+    ray.seed = st.seed;
     F3 beta = f3(1.0f, 1.0f, 1.0f);
     float pdf_prev = 0.0f;
     F3 acc_miss = f3(0, 0, 0), acc_sky = f3(0, 0, 0);
@@ -1005,7 +1098,36 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         F3 n0 = f3(tn[0], tn[1], tn[2]), n1 = f3(tn[3], tn[4], tn[5]), n2 = f3(tn[6], tn[7], tn[8]);
         F3 N = normalize((n0 + (n1 - n0) * h.u) + (n2 - n0) * h.v);
         const uint32_t mid = resolve_mat(s.materials, s.tri_mat[tri], P);
-        Mat m = load_mat(&s.materials[16 * mid]);
+        Mat m = load_mat(&s.materials[kMatFloats * mid]);
+        float rough = s.materials[kMatFloats * mid + 3];
+        {
+            // texture-driven fields (material_function samples at vertex.texcoord) and the
+            // normal map in the interpolated tangent frame (hits/rt_gbuffer_hit.hlsl:8-11,
+            // hit.hlsl:155-157)
+            const uint32_t tb = mat_word(s, mid, 23), tf = mat_word(s, mid, 24), tn = mat_word(s, mid, 25),
+                           tr = mat_word(s, mid, 26);
+            if (tb | tf | tn | tr) {
+                float su, sv;
+                tri_texcoord(s, tri, h.u, h.v, &su, &sv);
+                if (tb) { F4 c = tex_sample(s, tb - 1u, su, sv); m.base = f3(c.x, c.y, c.z); }
+                if (tf) { F4 c = tex_sample(s, tf - 1u, su, sv); m.f0 = f3(c.x, c.y, c.z); }
+                if (tr) {
+                    rough = comp4(tex_sample(s, tr - 1u, su, sv), mat_word(s, mid, 27));
+                    mat_alpha(m, rough, s.materials[kMatFloats * mid + 7]);
+                }
+                if (tn && s.need_tan) {
+                    F4 c = tex_sample(s, tn - 1u, su, sv);
+                    F3 nt = f3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+                    const float* g = &s.tri_tan[12 * (size_t)tri];
+                    F3 T = normalize(f3((g[0] + (g[4] - g[0]) * h.u) + (g[8] - g[0]) * h.v,
+                                        (g[1] + (g[5] - g[1]) * h.u) + (g[9] - g[1]) * h.v,
+                                        (g[2] + (g[6] - g[2]) * h.u) + (g[10] - g[2]) * h.v));
+                    float tw = (g[3] + (g[7] - g[3]) * h.u) + (g[11] - g[3]) * h.v;
+                    F3 B = normalize(cross(N, T)) * tw;
+                    N = normalize((T * nt.x + B * nt.y) + N * nt.z);
+                }
+            }
+        }
         if (m.two_sided && dot(ray.d, N) > 0.0f) N = -N;
         F3 V = normalize(ray.o - P);
         if (cm && (int)b == cm->bounce) {
@@ -1015,13 +1137,12 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
                 acc_miss = finite3(c) ? c : f3(0, 0, 0);
                 break;
             }
-            const float* mr = &s.materials[16 * mid];
             float* r = cm->rec;
             r[0] = P.x; r[1] = P.y; r[2] = P.z;
             r[3] = V.x; r[4] = V.y; r[5] = V.z;
             r[6] = N.x; r[7] = N.y; r[8] = N.z;
-            r[9] = mr[3];
-            r[10] = mr[0]; r[11] = mr[1]; r[12] = mr[2];
+            r[9] = rough;
+            r[10] = m.base.x; r[11] = m.base.y; r[12] = m.base.z;
             const uint32_t tag = 1u;
             std::memcpy(&r[13], &tag, 4);
             r[14] = r[15] = 0.0f;
@@ -1046,7 +1167,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
             if (sm_index < 0) { acc_dir[l] = acc_dir[l] + c; continue; }
             out.shadow_bits[b] |= (uint8_t)(1u << l);
             out.n_light_shadow++;
-            Ray sr{P, ldir, kTMin, kShadowTMax};
+            Ray sr{P, ldir, kTMin, kShadowTMax, st.seed};
             if (!trace_any(s, sr)) {
                 acc_dir[l] = acc_dir[l] + c;
             } else {
@@ -1072,7 +1193,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
                     if (any_pos(c) && finite3(c)) {
                         out.shadow_bits[b] |= 0x80u;
                         out.n_sky_shadow++;
-                        Ray sr{P, wi, kTMin, kShadowTMax};
+                        Ray sr{P, wi, kTMin, kShadowTMax, st.seed};
                         if (!trace_any(s, sr)) acc_sky = acc_sky + c;
                     }
                 }
@@ -1273,6 +1394,14 @@ void ora_trace(const ora_ctx* ctx, uint32_t n, const float* rays, int any_hit, f
             Hit h = trace_closest(s, ray);
             hit_t[i] = h.t; hit_leaf[i] = h.leaf; hit_u[i] = h.u; hit_v[i] = h.v;
         }
+    }
+}
+
+void ora_tex_sample32(const ora_ctx* ctx, uint32_t tex, uint32_t n, const float* uv, float* out4) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    for (uint32_t i = 0; i < n; ++i) {
+        const F4 c = tex_sample(s, tex, uv[2 * i], uv[2 * i + 1]);
+        out4[4 * i] = c.x; out4[4 * i + 1] = c.y; out4[4 * i + 2] = c.z; out4[4 * i + 3] = c.w;
     }
 }
 

@@ -206,8 +206,8 @@ bool any64(const Bvh64& b, D3 o, D3 d, double tmin, double tmax) {
 // with the record in word 14, evaluated at the FP64 hit point.
 uint32_t resolve_mat64(const std::vector<float>& mats, uint32_t id, D3 P) {
     uint32_t pattern, alt;
-    std::memcpy(&pattern, &mats[16 * (size_t)id + 13], 4);
-    std::memcpy(&alt, &mats[16 * (size_t)id + 14], 4);
+    std::memcpy(&pattern, &mats[kMatFloats * (size_t)id + 13], 4);
+    std::memcpy(&alt, &mats[kMatFloats * (size_t)id + 14], 4);
     if (pattern != 1u) return id;
     auto cell = [](double x) {
         double f = std::floor(x);
@@ -435,7 +435,7 @@ D3 path64(const Scene& s, const Bvh64& b, const Sky64& k, const ora_camera& cam,
         const float* tn = &s.tri_n[9 * (size_t)h.tri];
         D3 n0 = d3(tn[0], tn[1], tn[2]), n1 = d3(tn[3], tn[4], tn[5]), n2 = d3(tn[6], tn[7], tn[8]);
         D3 N = normalize(n0 + (n1 - n0) * h.u + (n2 - n0) * h.v);
-        Mat64 m = load_mat64(&s.materials[16 * (size_t)resolve_mat64(s.materials, s.tri_mat[h.tri], P)]);
+        Mat64 m = load_mat64(&s.materials[kMatFloats * (size_t)resolve_mat64(s.materials, s.tri_mat[h.tri], P)]);
         if (m.two_sided && dot(dir, N) > 0) N = -N;
         D3 V = normalize(o - P);
         Frame64 fr = frame64(N);
@@ -520,6 +520,13 @@ int ora_render64(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
                  const uint32_t* pixels, uint32_t spp, uint32_t seed64, double* mean_rgb, double* var_rgb,
                  int num_threads) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    // the FP64 oracle covers opaque parameter materials (textured and non-opaque
+    // materials are pinned through the FP32 restatement's closed-form tests)
+    for (size_t m = 0; m < s.materials.size() / kMatFloats; ++m) {
+        uint32_t w[32];
+        std::memcpy(w, &s.materials[kMatFloats * m], sizeof w);
+        if (w[15] || w[17] || w[23] || w[24] || w[25] || w[26]) return -2;
+    }
     const Bvh64& b = bvh64(s);
     Sky64 k = build_sky64(s);
     if (num_threads <= 0) num_threads = (int)std::max(1u, std::thread::hardware_concurrency());

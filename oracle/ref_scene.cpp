@@ -5,6 +5,7 @@
 // 3x4 row-major transforms, include/bisemutum/rhi/accel.hpp:48-55) as one world-space
 // triangle list; normals use the cofactor of the 3x3 part, which is proportional to the
 // inverse transpose that hit.hlsl:155 applies (WorldToObject4x3 * n).
+#include <cmath>
 #include <cstring>
 
 #include "oracle.h"
@@ -22,6 +23,8 @@ void flatten(const ora_scene_desc& d, Scene& s) {
     s.tri_v.resize((size_t)9 * total);
     s.tri_n.resize((size_t)9 * total);
     s.tri_mat.resize(total);
+    if (s.need_uv) s.tri_uv.assign((size_t)6 * total, 0.0f);
+    if (s.need_tan) s.tri_tan.assign((size_t)12 * total, 0.0f);
     uint32_t t = 0;
     for (uint32_t i = 0; i < d.num_instances; ++i) {
         const float* m = &d.inst_xform[12 * i];
@@ -39,6 +42,23 @@ void flatten(const ora_scene_desc& d, Scene& s) {
         for (uint32_t k = 0; k < ntri; ++k, ++t) {
             for (int c = 0; c < 3; ++c) {
                 uint32_t vi = idx[3 * k + c];
+                const size_t gv = (size_t)d.vtx_off[mesh] + vi;   // vertex in the concatenated pool
+                if (s.need_uv && d.texcoords) {
+                    s.tri_uv[6 * (size_t)t + 2 * c] = d.texcoords[2 * gv];
+                    s.tri_uv[6 * (size_t)t + 2 * c + 1] = d.texcoords[2 * gv + 1];
+                }
+                if (s.need_tan) {
+                    // ObjectToWorld3x4 * (tangent.xyz, 0) (hit.hlsl:156); w = the bitangent sign
+                    const float tx = d.tangents ? d.tangents[4 * gv] : 1.0f;
+                    const float ty = d.tangents ? d.tangents[4 * gv + 1] : 0.0f;
+                    const float tz = d.tangents ? d.tangents[4 * gv + 2] : 0.0f;
+                    const float tw = d.tangents ? d.tangents[4 * gv + 3] : 1.0f;
+                    float* g = &s.tri_tan[12 * (size_t)t + 4 * c];
+                    g[0] = (m[0] * tx + m[1] * ty) + m[2] * tz;
+                    g[1] = (m[4] * tx + m[5] * ty) + m[6] * tz;
+                    g[2] = (m[8] * tx + m[9] * ty) + m[10] * tz;
+                    g[3] = tw;
+                }
                 float x = P[3 * vi], y = P[3 * vi + 1], z = P[3 * vi + 2];
                 s.tri_v[9 * t + 3 * c + 0] = ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
                 s.tri_v[9 * t + 3 * c + 1] = ((m[4] * x + m[5] * y) + m[6] * z) + m[7];
@@ -92,8 +112,30 @@ extern "C" {
 
 ora_ctx* ora_create(const ora_scene_desc* d) {
     Scene* s = new Scene();
+    s->materials.assign(d->materials, d->materials + kMatFloats * (size_t)d->num_materials);
+    // texture references (index + 1) in words 17 (opacity) and 23-26 (colour, f0, normal, roughness)
+    for (uint32_t m = 0; m < d->num_materials; ++m) {
+        uint32_t w[32];
+        std::memcpy(w, &d->materials[kMatFloats * m], sizeof w);
+        s->need_uv = s->need_uv || w[17] || w[23] || w[24] || w[25] || w[26];
+        s->need_tan = s->need_tan || w[25];
+    }
     flatten(*d, *s);
-    s->materials.assign(d->materials, d->materials + 16 * (size_t)d->num_materials);
+    for (uint32_t k = 0; k < d->num_textures; ++k) {
+        const ora_texture& T = d->textures[k];
+        Tex t;
+        t.width = T.width; t.height = T.height; t.format = T.format; t.filter = T.filter;
+        t.address_u = T.address_u; t.address_v = T.address_v;
+        const size_t words = (size_t)T.width * T.height * (T.format == 2u ? 4u : 1u);
+        t.words.resize(words);
+        std::memcpy(t.words.data(), T.texels, 4 * words);
+        s->textures.push_back(std::move(t));
+    }
+    // sRGB byte -> linear (IEC 61966-2-1), evaluated in double and rounded once
+    for (int c = 0; c < 256; ++c) {
+        const double x = c / 255.0;
+        s->srgb[c] = (float)(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+    }
     s->num_dir_lights = d->num_dir_lights;
     s->dir_lights.assign(d->dir_lights, d->dir_lights + 16 * (size_t)d->num_dir_lights);
     s->sky_w = d->sky_rgba ? d->sky_w : 0;

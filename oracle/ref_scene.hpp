@@ -21,12 +21,26 @@ struct Node4 {
     int32_t child[4];
 };
 
+constexpr size_t kMatFloats = 32;   // floats per material record (bi_pt_material)
+
+// Level-0 material texture: one word per RGBA8 texel, four per RGBA32F texel.
+struct Tex {
+    uint32_t width = 0, height = 0, format = 0, filter = 0, address_u = 0, address_v = 0;
+    std::vector<uint32_t> words;
+};
+
 struct Scene {
     uint32_t num_tris = 0;
     std::vector<float> tri_v;      // 9 per triangle, world space, instance order
     std::vector<float> tri_n;      // 9 per triangle, cofactor-transformed normals
     std::vector<uint32_t> tri_mat;
-    std::vector<float> materials;  // 16 per material
+    std::vector<float> materials;  // kMatFloats per material
+    // texture coordinates (6 per triangle) and world tangents (12 per triangle: xyz +
+    // bitangent sign per vertex), kept when the materials sample textures / normal maps
+    bool need_uv = false, need_tan = false;
+    std::vector<float> tri_uv, tri_tan;
+    std::vector<Tex> textures;
+    float srgb[256] = {};
     uint32_t num_dir_lights = 0;
     std::vector<float> dir_lights; // 16 per light
     // sky

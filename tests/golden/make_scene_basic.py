@@ -7,8 +7,10 @@ Run in a checkout that has the example project (default
   assets.json                   asset id -> {type, file} of asset_metadata.toml (meshes and materials)
   materials.json                asset id -> material data: blend_mode, surface_model, params and,
                                 in place of the HLSL material function, its parsed statements
-                                or the reason it is outside the parameter subset
-Textures and the prefab are not written: the pass reads parameter materials only.
+                                or the reason it is outside the supported subset
+  textures/*.texture.biasset    the Texture assets at 1/8 of their size (every 8th texel of level
+                                0, sampler and format kept), re-encoded in the asset layout
+The prefab is not written.
 """
 import json
 import os
@@ -38,6 +40,17 @@ def main(src="/root/reference/examples/scene_basic"):
             name = os.path.basename(rel)
             shutil.copyfile(os.path.join(src, rel), os.path.join(out, "meshes", name))
             table[a["id"]] = {"type": "StaticMesh", "file": "meshes/" + name}
+        elif a["type"] == "Texture":
+            name = os.path.basename(rel)
+            with open(os.path.join(src, rel), "rb") as f:
+                t = assets.read_texture(f.read(), name)
+            small = assets.TextureData(len(t.texels[::8, 0]), len(t.texels[0, ::8]), 1, t.format, t.sampler,
+                                       t.texels[::8, ::8].copy())
+            small.width, small.height = small.texels.shape[1], small.texels.shape[0]
+            os.makedirs(os.path.join(out, "textures"), exist_ok=True)
+            with open(os.path.join(out, "textures", name), "wb") as f:
+                f.write(assets.encode_texture(small))
+            table[a["id"]] = {"type": "Texture", "file": "textures/" + name}
         elif a["type"] == "Material":
             m = toml(rel)
             rec = {k: m[k] for k in ("blend_mode", "surface_model", "params") if k in m}

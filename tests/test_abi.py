@@ -42,9 +42,10 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "bi_pt.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(bi_pt_device_desc), sizeof(bi_pt_mesh),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(bi_pt_device_desc), sizeof(bi_pt_mesh),
          sizeof(bi_pt_instance), sizeof(bi_pt_material), sizeof(bi_pt_dir_light), sizeof(bi_pt_camera),
-         sizeof(bi_pt_settings), sizeof(bi_pt_stats), offsetof(bi_pt_stats, ms_build));
+         sizeof(bi_pt_settings), sizeof(bi_pt_stats), offsetof(bi_pt_stats, ms_build), sizeof(bi_pt_texture),
+         offsetof(bi_pt_material, base_color_tex));
   return 0;
 }
 """
@@ -54,11 +55,13 @@ int main(void) {
         exe = os.path.join(d, "t")
         subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()))
-    mirror = [ctypes.sizeof(bi_pt.DeviceDesc), ctypes.sizeof(bi_pt.Mesh), ctypes.sizeof(bi_pt.Instance), 64, 64,
-              ctypes.sizeof(bi_pt.CameraC), ctypes.sizeof(bi_pt.Settings), ctypes.sizeof(bi_pt.Stats),
-              bi_pt.Stats.ms_build.offset]
+    import scenes
+    mirror = [ctypes.sizeof(bi_pt.DeviceDesc), ctypes.sizeof(bi_pt.Mesh), ctypes.sizeof(bi_pt.Instance),
+              4 * scenes.MATERIAL_FLOATS, 64, ctypes.sizeof(bi_pt.CameraC), ctypes.sizeof(bi_pt.Settings),
+              ctypes.sizeof(bi_pt.Stats), bi_pt.Stats.ms_build.offset, ctypes.sizeof(bi_pt.TextureC),
+              4 * scenes.M_BASE_TEX]
     assert sizes == mirror
-    assert sizes[3] == 64 and sizes[4] == 64   # bi_pt_material, bi_pt_dir_light (= DirLightData)
+    assert sizes[3] == 128 and sizes[4] == 64   # bi_pt_material, bi_pt_dir_light (= DirLightData)
 
 
 def test_no_gpu_means_loud_failure():

@@ -27,7 +27,7 @@ def fixture_scene(width=None, height=None, on_unsupported="default"):
         mats = {int(k): v for k, v in json.load(f).items()}
 
     def fetch(i):
-        if table[i]["type"] == "StaticMesh":
+        if table[i]["type"] in ("StaticMesh", "Texture"):
             with open(os.path.join(GOLD, table[i]["file"]), "rb") as f:
                 return f.read()
         return mats[i]
@@ -154,7 +154,8 @@ def test_parse_material_function_subset():
         "surface.two_sided = true;\nsurface.f0_color = float3(0.1, 0.2, 0.3);\n")
     assert s == [("base_color", ("param", "albedo")), ("roughness", ("const", 0.25)),
                  ("two_sided", ("const", True)), ("f0_color", ("const", (0.1, 0.2, 0.3)))]
-    for bad, why in (("surface.base_color = PARAM_t.Sample(PARAM_s, vertex.texcoord).xyz;", "texture"),
+    for bad, why in (("surface.base_color = PARAM_t.Sample(PARAM_s, vertex.texcoord2).xyz;", "texture"),
+                     ("surface.base_color = PARAM_t.Sample(PARAM_s, vertex.texcoord).xyz * 2.0;", "texture"),
                      ("surface.roughness = vertex.position_world.x > 0 ? 1.0 : 0.5;", "expression"),
                      ("float a = 1.0;", "statement"),
                      ("surface.sheen = 1.0;", "unknown surface field")):
@@ -190,13 +191,86 @@ def test_evaluate_material_defaults_and_params():
     scalar = assets.evaluate_material({"material_function": "surface.base_color = PARAM_c;",
                                        "params": [{"name": "c", "value": 0.25}]})
     assert scalar["base_color"] == (0.25, 0.25, 0.25)                   # HLSL scalar broadcast
-    with pytest.raises(assets.UnsupportedMaterial, match="stochastic"):
-        assets.evaluate_material({"blend_mode": "translucent", "material_function": "surface.opacity = 0.5;"})
+    tr = assets.evaluate_material({"blend_mode": "translucent", "material_function": "surface.opacity = 0.5;"})
+    rec = assets.material_record(tr)
+    assert rec.view(np.uint32)[15] == 2 and rec[16] == np.float32(0.5)          # stochastic opacity
+    for mode, word in (("alpha_test", 1), ("additive", 2), ("modulate", 2), ("opaque", 0)):
+        assert assets.material_record(assets.evaluate_material({"blend_mode": mode})).view(np.uint32)[15] == word
     with pytest.raises(assets.UnsupportedMaterial, match="not declared"):
         assets.evaluate_material({"material_function": "surface.roughness = PARAM_missing;"})
     with pytest.raises(assets.UnsupportedMaterial, match="texture parameter"):
         assets.evaluate_material({"material_function": "surface.base_color = PARAM_t;",
                                   "params": [{"name": "t", "value": {"asset_id": 6}}]})
+
+
+def test_textured_material_statements_and_record():
+    text = ("float4 value = PARAM_cage_tex.Sample(PARAM_cage_tex_sampler, vertex.texcoord);\n"
+            "surface.base_color = value.xyz;\nsurface.f0_color = value.rgb;\n"
+            "surface.opacity = value.w < 0.5 ? 0.0 : 1.0;\n"
+            "surface.normal_map_value = PARAM_n.Sample(PARAM_n_sampler, vertex.texcoord).xyz;\n"
+            "surface.roughness = PARAM_r.Sample(PARAM_r_sampler, vertex.texcoord).g;\nsurface.two_sided = true;")
+    s = assets.parse_material_function(text)
+    assert s == [("$value", ("sample", "cage_tex")), ("base_color", ("texture", "cage_tex", "xyz")),
+                 ("f0_color", ("texture", "cage_tex", "rgb")), ("opacity", ("step", "cage_tex", "w", 0.5, 0.0, 1.0)),
+                 ("normal_map_value", ("texture", "n", "xyz")), ("roughness", ("texture", "r", "g")),
+                 ("two_sided", ("const", True))]
+    surf = assets.evaluate_material({"blend_mode": "alpha_test", "material_function": text, "params": [
+        {"name": "cage_tex", "value": {"asset_id": 10}}, {"name": "n", "value": {"asset_id": 7}},
+        {"name": "r", "value": {"asset_id": 3}}]})
+    rec = assets.material_record(surf, {10: 0, 7: 1, 3: 2})
+    w = rec.view(np.uint32)
+    assert w[15] == 1 and w[23] == 1 and w[24] == 1 and w[25] == 2 and w[26] == 3 and w[27] == 1
+    assert w[17] == 1 and w[18] == 3 and w[19] == 1 and (rec[20], rec[21], rec[22]) == (0.5, 0.0, 1.0)
+    with pytest.raises(assets.UnsupportedMaterial, match="texture channels"):
+        assets.evaluate_material({"material_function": "surface.f90_color = PARAM_t.Sample(PARAM_s, vertex.texcoord).xyz;",
+                                  "params": [{"name": "t", "value": {"asset_id": 1}}]})
+
+
+def test_texture_asset_round_trip_and_layout():
+    rng = np.random.default_rng(3)
+    texels = rng.integers(0, 256, size=(5, 7, 4), dtype=np.uint8)
+    sampler = {"mag_filter": 1, "min_filter": 0, "mipmap_mode": 1, "address_u": 2, "address_v": 1, "address_w": 0,
+               "border_color": 0, "compare_enabled": 0, "compare_op": 0, "anisotropy": 0.0, "lod_bias": 0.0,
+               "lod_min": 0.0, "lod_max": 1000.0}
+    t = assets.TextureData(7, 5, 3, assets.FORMAT_RGBA8_SRGB, sampler, texels)
+    data = assets.encode_texture(t)
+    # layout: magic, type name, version 2, 28-byte SamplerDesc, 20-byte TextureDesc
+    assert data[:4] == (0x0B1A55E7).to_bytes(4, "little") and data[12:19] == b"Texture"
+    off = 4 + 8 + 7 + 4
+    assert list(data[off:off + 9]) == [1, 0, 1, 2, 1, 0, 0, 0, 0]
+    back = assets.read_texture(data)
+    assert (back.width, back.height, back.format) == (7, 5, assets.FORMAT_RGBA8_SRGB)
+    assert np.array_equal(back.texels, texels) and back.sampler == sampler
+    rec = assets.texture_record(back)
+    assert (rec.format, rec.filter, rec.address_u, rec.address_v) == (1, 1, 2, 1)
+    with pytest.raises(assets.AssetError):
+        assets.read_texture(data[:40])
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_PROJECT), reason="the engine's example project is not present")
+def test_reference_texture_assets():
+    """The example project's three textures: level 0 of the sizes and formats their
+    descriptions give, linear filtering with repeat addressing, and the fixtures are every
+    8th texel of them."""
+    want = {"earth_diffuse": (2048, 1024, assets.FORMAT_RGBA8_SRGB), "earth_normal": (2048, 1024, assets.FORMAT_RGBA8_UNORM),
+            "cage": (1299, 1300, assets.FORMAT_RGBA8_UNORM)}
+    for name, (w, h, fmt) in want.items():
+        with open(os.path.join(REF_PROJECT, "textures", f"{name}.texture.biasset"), "rb") as f:
+            t = assets.read_texture(f.read(), name)
+        assert (t.width, t.height, t.format) == (w, h, fmt) and t.texels.shape == (h, w, 4)
+        assert t.sampler["mag_filter"] == 1 and t.sampler["address_u"] == 0 and t.sampler["address_v"] == 0
+        with open(os.path.join(GOLD, "textures", f"{name}.texture.biasset"), "rb") as f:
+            small = assets.read_texture(f.read())
+        assert np.array_equal(small.texels, t.texels[::8, ::8])
+    # the normal map's texels encode tangent-space normals (the shader normalises them):
+    # |2 t - 1| near 1 for most texels, all pointing away from the surface
+    with open(os.path.join(REF_PROJECT, "textures", "earth_normal.texture.biasset"), "rb") as f:
+        n = assets.read_texture(f.read()).texels[::37, ::37, :3].astype(np.float64) / 255.0 * 2.0 - 1.0
+    assert np.mean(np.abs(np.linalg.norm(n, axis=-1) - 1.0) < 0.1) > 0.9 and np.all(n[..., 2] > 0.5)
+    # the cage's alpha is a cut-out mask: almost all texels fully opaque or fully clear
+    with open(os.path.join(REF_PROJECT, "textures", "cage.texture.biasset"), "rb") as f:
+        a = assets.read_texture(f.read()).texels[..., 3]
+    assert np.mean((a == 0) | (a == 255)) > 0.9 and 0.05 < np.mean(a >= 128) < 0.95
 
 
 # ---------------------------------------------------------------- the example project
@@ -206,9 +280,15 @@ def test_scene_basic_from_fixtures():
     sc = ps.scene
     assert [o[0] for o in ps.objects] == ["Sphere", "Cube", "Transparent Cube", "Cage Cube", "Plane"]
     assert len(sc.meshes) == 3 and sc.num_tris == 960 + 3 * 12 + 2
-    assert set(ps.unsupported) == {4, 9, 11}
-    assert "texture" in ps.unsupported[4] and "texture" in ps.unsupported[11]
-    assert "stochastic" in ps.unsupported[9]
+    assert ps.unsupported == {}
+    assert [t.texels.shape for t in sc.textures] == [(128, 256, 4), (128, 256, 4), (163, 163, 4)]
+    assert [t.format for t in sc.textures] == [1, 0, 0]          # earth colour is sRGB
+    m_earth, m_tr, m_cage = (sc.materials[sc.instances["material"][k]] for k in (0, 2, 3))
+    assert m_earth.view(np.uint32)[23] == 1 and m_earth.view(np.uint32)[25] == 2   # base colour, normal map
+    assert m_tr.view(np.uint32)[15] == 2 and m_tr[16] == np.float32(0.5)         # translucent, opacity 0.5
+    assert m_cage.view(np.uint32)[15] == 1 and m_cage.view(np.uint32)[17] == 3   # alpha test from the cage's w
+    assert sc.mesh_uv is not None and all(u is not None for u in sc.mesh_uv)
+    assert sc.mesh_tan is not None
     assert ps.settings == {"max_bounces": 3, "ray_length": 100.0, "accumulate": True}
     white = sc.materials[sc.instances["material"][1]]
     assert np.array_equal(white[:3], [1, 1, 1]) and white[3] == np.float32(0.5)
@@ -232,8 +312,7 @@ def test_scene_basic_from_fixtures():
     assert np.allclose(cam.inv_view[:3, 3], [10.5, 4.5, 6.0], atol=1e-5)
     front = assets.euler_zxy_degrees((-9.3324127197265625, 63.346549987792969, 17.90446662902832)) @ [0, 0, -1]
     assert np.allclose(cam.inv_view[:3, :3] @ [0, 0, -1], front, atol=1e-5)
-    with pytest.raises(assets.UnsupportedMaterial, match="material asset 4: texture"):
-        fixture_scene(on_unsupported="raise")
+    fixture_scene(on_unsupported="raise")   # every material of the project is in the subset
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_PROJECT), reason="the engine's example project is not present")
@@ -258,7 +337,8 @@ ASSETS_LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file
 class _PtMesh(__import__("ctypes").Structure):
     import ctypes as C
     _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
-                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32)]
+                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32),
+                ("texcoords", C.POINTER(C.c_float)), ("tangents", C.POINTER(C.c_float))]
 
 
 def native_lib():
@@ -304,6 +384,13 @@ def test_native_loader_matches(name, version):
     assert np.array_equal(got_p.view(np.uint32), p.view(np.uint32))
     assert np.array_equal(got_n.view(np.uint32), n.view(np.uint32))
     assert np.array_equal(got_i, idx)
+    uv, tan = assets.submesh_attributes(m)
+    for got, want, w in ((out.texcoords, uv, 2), (out.tangents, tan, 4)):
+        if want is None:
+            assert not got
+        else:
+            arr = np.ctypeslib.as_array(got, (out.num_vertices * w,)).reshape(-1, w)
+            assert np.array_equal(arr.view(np.uint32), want.view(np.uint32))
     lib.bi_static_mesh_destroy(h)
 
 
