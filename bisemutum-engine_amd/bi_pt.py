@@ -39,7 +39,8 @@ EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene
            "bi_pt_nrc_get_stats", "bi_pt_nrc_get_params", "bi_pt_nrc_set_params", "bi_pt_nrc_set_bounds",
            "bi_pt_nrc_infer", "bi_pt_nrc_train", "bi_pt_nrc_debug_gradient", "bi_pt_nrc_debug_samples",
            "bi_pt_set_build_options", "bi_pt_debug_inject_error", "bi_pt_server_debug_inject_error",
-           "bi_pt_set_textures"]
+           "bi_pt_trace_closest_noalloc", "bi_pt_set_textures"]
+NOALLOC_MAX_RAYS = 1 << 18
 NRC_PARAMS = 17408
 NRC_FREEZE = 1
 
@@ -205,6 +206,7 @@ def lib():
         L.bi_pt_trace_closest.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
         L.bi_pt_trace_occlusion.argtypes = [vp, u32, vp, C.c_float, vp, vp]
         L.bi_pt_set_textures.argtypes = [vp, vp, u32]
+        L.bi_pt_trace_closest_noalloc.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
         L.bi_pt_debug_inject_error.argtypes = [vp, u32]
         L.bi_pt_server_debug_inject_error.argtypes = [vp, u32]
         for name in EXPORTS:
@@ -436,6 +438,23 @@ class PathTracer:
         self._check(lib().bi_pt_trace_closest(self.h, n, C.byref(r), tmax, C.c_void_p(t.data_ptr()),
                                               C.c_void_p(tri.data_ptr()), C.c_void_p(u.data_ptr()),
                                               C.c_void_p(v.data_ptr()), s))
+        torch.cuda.synchronize()
+        return t.cpu().numpy(), tri.cpu().numpy().view(np.uint32), u.cpu().numpy(), v.cpu().numpy()
+
+    def trace_closest_noalloc(self, origins, dirs, tmax=1000.0, stream=None):
+        """trace_closest through bi_pt_trace_closest_noalloc (the library's static scratch;
+        the scene's LBVH must be current)."""
+        torch = self.torch
+        cols, r, dev = self._query_rays(origins, dirs)
+        n = cols[0].numel()
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        tri = torch.empty(n, dtype=torch.int32, device=dev)
+        u = torch.empty(n, dtype=torch.float32, device=dev)
+        v = torch.empty(n, dtype=torch.float32, device=dev)
+        s = C.c_void_p(stream) if stream else None
+        self._check(lib().bi_pt_trace_closest_noalloc(self.h, n, C.byref(r), tmax, C.c_void_p(t.data_ptr()),
+                                                      C.c_void_p(tri.data_ptr()), C.c_void_p(u.data_ptr()),
+                                                      C.c_void_p(v.data_ptr()), s))
         torch.cuda.synchronize()
         return t.cpu().numpy(), tri.cpu().numpy().view(np.uint32), u.cpu().numpy(), v.cpu().numpy()
 

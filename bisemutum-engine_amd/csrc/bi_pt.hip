@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -784,6 +785,112 @@ bi_pt_status bi_pt_trace_closest(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* r
     launch_trace_closest(ta, n, false, 16, 256, s);
     hipLaunchKernelGGL(k_query_hits, dim3((n + 255u) / 256u), dim3(256), 0, s, ctx->rq_hits.p,
                        ctx->num_tris ? ctx->vals0.p : ctx->rq_path.p, n, hit_t, hit_triangle, hit_u, hit_v);
+    BI_HIP_TRY(ctx, hipGetLastError());
+    return BI_PT_OK;
+}
+
+}  // extern "C"
+
+// ---- closest-hit queries without device allocation ------------------------------
+// The scratch of bi_pt_trace_closest (the job record and its path bases, the ray count
+// and the persistent lanes' part counters, one hit record per ray) lives in this static
+// device array instead of context buffers.  Each GPU holds its own copy for the life of
+// the process; every call on a GPU uses that GPU's copy, so calls that overlap on one GPU
+// race (callers serialise them, e.g. on one stream).
+namespace {
+
+constexpr size_t kNoAllocJobBytes = 1024;     // JobRec + 4 path/pixel bases
+constexpr size_t kNoAllocCntBytes = 2048;     // count + part counters
+constexpr size_t kNoAllocBytes =
+    kNoAllocJobBytes + kNoAllocCntBytes + (size_t)BI_PT_NOALLOC_MAX_RAYS * sizeof(HitRec);
+static_assert(sizeof(JobRec) + 4 * sizeof(uint32_t) <= kNoAllocJobBytes, "job record fits its scratch slot");
+static_assert((4 + kWorkParts * kWorkStride) * sizeof(uint32_t) <= kNoAllocCntBytes, "counters fit");
+static_assert(kNoAllocJobBytes % 256 == 0 && kNoAllocCntBytes % 256 == 0, "slots keep 256-B alignment");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch (looked up once per device, with the device
+// current; host code uses this address, never the array's name).
+bi_pt_status noalloc_scratch(Owner* own, int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(own, BI_PT_ERROR_UNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(own, BI_PT_ERROR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return BI_PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bi_pt_status bi_pt_trace_closest_noalloc(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax,
+                                         float* hit_t, uint32_t* hit_triangle, float* hit_u, float* hit_v,
+                                         bi_pt_stream stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (ctx && ctx->scene_src) ctx = ctx->scene_src;   // a view traces its scene
+    if (!ctx || (n && !rays)) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (n > BI_PT_NOALLOC_MAX_RAYS)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT,
+                    "bi_pt_trace_closest_noalloc takes at most " + std::to_string(BI_PT_NOALLOC_MAX_RAYS) +
+                        " rays per call (got " + std::to_string(n) + ")");
+    if (n && (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null ray array");
+    if (!ctx->has_scene) return fail(ctx, BI_PT_ERROR_NO_SCENE, "no scene set");
+    if (!(tmax > kTMin) || !(tmax <= kShadowTMax))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "ray queries need 0.001 < tmax <= 1.7e38");
+    // nothing is built or allocated here: the LBVH must be current
+    if (!ctx->eng.ready || !ctx->bvh_built)
+        return fail(ctx, BI_PT_ERROR_UNSUPPORTED,
+                    "bi_pt_trace_closest_noalloc needs the scene's LBVH built (render or query the scene once "
+                    "after its last change)");
+    if (n == 0) return BI_PT_OK;
+    if (!hit_triangle) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null hit_triangle");
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    unsigned char* base = nullptr;
+    BI_TRY(noalloc_scratch(ctx, ctx->device, &base));
+    unsigned char* job = base;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(base + kNoAllocJobBytes);
+    HitRec* hits = reinterpret_cast<HitRec*>(base + kNoAllocJobBytes + kNoAllocCntBytes);
+    JobRec rec{};
+    rec.nodes = ctx->num_tris ? ctx->nodes4.p : ctx->eng.empty_tree.p;
+    rec.leaf_geo = ctx->leaf_geo.p;
+    rec.leaf_shade = ctx->leaf_shade.p;
+    rec.materials = ctx->mats.p;
+    rec.leaf_uv = scene_leaf_uv(ctx);
+    rec.tex = scene_tex(ctx);
+    rec.num_tris = ctx->num_tris;
+    rec.ray_length = tmax;
+    // the previous call's contents are stale: the record, the count and the part
+    // counters are written afresh (fill kernels, no memset nodes under capture)
+    hipLaunchKernelGGL(k_put_query_job, dim3(1), dim3(64), 0, s, rec, n, job);
+    launch_fill_u32(cnt, 0u, 4 + kWorkParts * kWorkStride, s);
+    launch_fill_u32(cnt, n, 1, s);
+    TraceArgs ta{};
+    ta.jobs = WaveJobs{reinterpret_cast<const JobRec*>(job), reinterpret_cast<const uint32_t*>(job + sizeof(JobRec)),
+                       reinterpret_cast<const uint32_t*>(job + sizeof(JobRec)) + 2, 1u, n};
+    // one job: the kernel never reads path ids
+    ta.q = RayQueue{const_cast<float*>(rays->ox), const_cast<float*>(rays->oy), const_cast<float*>(rays->oz),
+                    const_cast<float*>(rays->dx), const_cast<float*>(rays->dy), const_cast<float*>(rays->dz), nullptr};
+    ta.count_in = cnt;
+    ta.work = cnt + 4;
+    ta.parts = kWorkParts;
+    ta.refill = 32;
+    ta.hits = HitBuf{hits};
+    ta.error = &ctx->eng.counters.p[kCntError];
+    launch_trace_closest(ta, n, false, 16, 256, s);
+    // a miss never reads the order; an empty scene has no order, any valid address serves
+    hipLaunchKernelGGL(k_query_hits, dim3((n + 255u) / 256u), dim3(256), 0, s, hits,
+                       ctx->num_tris ? ctx->vals0.p : cnt, n, hit_t, hit_triangle, hit_u, hit_v);
     BI_HIP_TRY(ctx, hipGetLastError());
     return BI_PT_OK;
 }

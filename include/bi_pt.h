@@ -463,6 +463,19 @@ typedef struct bi_pt_rays {
  * barycentrics hit_u, hit_v of the second and third vertex (nullable).  All device, n each. */
 bi_pt_status bi_pt_trace_closest(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax, float* hit_t,
                                  uint32_t* hit_triangle, float* hit_u, float* hit_v, bi_pt_stream stream);
+/* bi_pt_trace_closest for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  Same arguments and results; its scratch (job record, part counters,
+ * one 16-B hit record per ray) is a static device array of the library (4 MiB + 3 KiB,
+ * held by each GPU the library runs on until the process ends), so at most
+ * BI_PT_NOALLOC_MAX_RAYS rays per call (more: INVALID_ARGUMENT), and it builds nothing:
+ * the scene's LBVH must be current (rendered or queried since the scene last changed;
+ * else UNSUPPORTED).  Calls on one GPU share the scratch: calls that overlap on one GPU
+ * race, so serialise them (for example on one stream). */
+#define BI_PT_NOALLOC_MAX_RAYS (1u << 18)
+bi_pt_status bi_pt_trace_closest_noalloc(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax,
+                                         float* hit_t, uint32_t* hit_triangle, float* hit_u, float* hit_v,
+                                         bi_pt_stream stream);
 /* Occlusion (any hit in [0.001, tmax]): occluded[i] = 1 or 0 (device, n). */
 bi_pt_status bi_pt_trace_occlusion(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* rays, float tmax,
                                    uint32_t* occluded, bi_pt_stream stream);
