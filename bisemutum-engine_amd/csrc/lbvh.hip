@@ -476,6 +476,9 @@ constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletMaxHeight = 48;
 constexpr int kTreeletMinHeight = 5;   // lower subtrees are left as built (their share of the cost is small)
 constexpr uint32_t kTreeletSoleChild = 0x100u;   // arrival count of a parent with one restructured child
+// s_waitcnt operand (gfx9 encoding) that waits for vmcnt 0 only: expcnt and lgkmcnt at
+// their maxima (bits 6:4 and 11:8), vmcnt 0 (bits 3:0 and 15:14)
+constexpr unsigned kWaitVmcnt0 = 0x0F70u;
 // Subtrees whose root stands at most this high (and whose parent stands higher) are
 // restructured by one wavefront alone, children first: at most 2^(8 - 4) - 1 = 15 nodes
 // of height >= kTreeletMinHeight each.
@@ -934,12 +937,18 @@ __global__ __launch_bounds__(64) void k_tl_climb(TreeletArgs a) {
             n = p;
             continue;
         }
+        // release: the fence writes this XCD's L2 back (buffer_wbl2), and the write-back
+        // must be complete before the counter moves; hipcc of ROCm 7.2 drops the
+        // vmcnt wait after buffer_wbl2 when no other vector-memory operation is
+        // outstanding, so the wait is stated here
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         uint32_t old = 0;
         if (lane == 0) old = __hip_atomic_fetch_add(&a.arrivals[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         old = __builtin_amdgcn_readfirstlane(old);
         if (old != 1u) break;   // the other restructured child of p is still to come; its wavefront carries on
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // buffer_inv: later loads miss stale lines
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         n = p;
     }
 }
