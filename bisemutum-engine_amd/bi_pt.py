@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
-           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
@@ -157,6 +157,7 @@ def lib():
         L.bi_pt_update_transforms.argtypes = [vp, f32p, u32]
         L.bi_pt_set_lights.argtypes = [vp, f32p, u32]
         L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
+        L.bi_pt_set_sky_cube.argtypes = [vp, f32p, u32, f32p, f32p]
         L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
         L.bi_pt_reset_accumulation.argtypes = [vp]
         L.bi_pt_set_build_options.argtypes = [vp, C.POINTER(C.c_uint32 * 4)]
@@ -303,7 +304,10 @@ class PathTracer:
         self._check(lib().bi_pt_set_scene(self.h, meshes, len(scene.meshes), inst, len(scene.instances), _p(mats),
                                           len(mats)))
         self.set_lights(scene.dir_lights)
-        self.set_sky(scene.sky_rgba, scene.sky_color, scene.sky_rot)
+        if getattr(scene, "sky_layout", 0) == 1:
+            self.set_sky_cube(scene.sky_rgba, scene.sky_color, scene.sky_rot)
+        else:
+            self.set_sky(scene.sky_rgba, scene.sky_color, scene.sky_rot)
 
     def set_textures(self, textures):
         """bi_pt_set_textures: scenes.Texture records (texels [H, W, 4] uint8 or float32)."""
@@ -329,6 +333,17 @@ class PathTracer:
         rot = np.ascontiguousarray(rot, np.float32)
         h, w = (rgba.shape[0], rgba.shape[1]) if rgba.size else (0, 0)
         self._check(lib().bi_pt_set_sky(self.h, _p(rgba) if rgba.size else None, w, h, _p(col), _p(rot)))
+
+    def set_sky_cube(self, faces, color, rot):
+        """bi_pt_set_sky_cube: faces f32 [6, S, S, 4] in the order +X, -X, +Y, -Y, +Z, -Z."""
+        faces = np.ascontiguousarray(faces, np.float32)
+        if faces.size and (faces.ndim != 4 or faces.shape[0] != 6 or faces.shape[1] != faces.shape[2]
+                           or faces.shape[3] != 4):
+            raise ValueError(f"cube faces must be [6, S, S, 4], got {faces.shape}")
+        col = np.ascontiguousarray(color, np.float32)
+        rot = np.ascontiguousarray(rot, np.float32)
+        s = faces.shape[1] if faces.size else 0
+        self._check(lib().bi_pt_set_sky_cube(self.h, _p(faces) if faces.size else None, s, _p(col), _p(rot)))
 
     # ---------------------------------------------------------------- render
     def render(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,

@@ -18,23 +18,35 @@ using namespace bipt;
 
 namespace {
 
-// Importance table of the sky (luminance * sin(theta) per texel), built in FP32
-// in a fixed order on the host.
-void build_sky_table(const float* rgba, uint32_t W, uint32_t H, const float color[3], std::vector<float>& w,
-                     std::vector<float>& ccdf, std::vector<float>& rows, std::vector<float>& marg, float& total) {
+// Importance table of the sky, built in FP32 in a fixed order on the host: per texel
+// luminance(texel * colour) times sin(theta) of its row (equirect) or 1/r^3 of its
+// centre (cube face texel at face coordinates (a, b), r^2 = 1 + a^2 + b^2: the
+// texel's solid angle up to a constant), then running sums per row and over rows.
+void build_sky_table(const float* rgba, uint32_t W, uint32_t H, uint32_t layout, const float color[3],
+                     std::vector<float>& w, std::vector<float>& ccdf, std::vector<float>& rows,
+                     std::vector<float>& marg, float& total) {
     w.assign((size_t)W * H, 0.0f);
     ccdf.assign((size_t)(W + 1) * H, 0.0f);
     rows.assign(H, 0.0f);
     marg.assign(H + 1, 0.0f);
     const float pi_over_h = kPi / (float)H;
     for (uint32_t i = 0; i < H; ++i) {
-        float st = pt_sin(((float)i + 0.5f) * pi_over_h);
+        const uint32_t r = layout == kSkyCube ? i % W : i;
+        const float b = (float)(2 * r + 1) / (float)W - 1.0f;
+        const float st = layout == kSkyCube ? 0.0f : pt_sin(((float)i + 0.5f) * pi_over_h);
         float run = 0.0f;
         for (uint32_t j = 0; j < W; ++j) {
             const float* p = &rgba[4 * ((size_t)i * W + j)];
-            float r = p[0] * color[0], g = p[1] * color[1], b = p[2] * color[2];
-            float lum = (0.2126f * r + 0.7152f * g) + 0.0722f * b;
-            float x = lum * st;
+            float rr = p[0] * color[0], g = p[1] * color[1], bb = p[2] * color[2];
+            float lum = (0.2126f * rr + 0.7152f * g) + 0.0722f * bb;
+            float x;
+            if (layout == kSkyCube) {
+                const float a = (float)(2 * j + 1) / (float)W - 1.0f;
+                const float q = (1.0f + a * a) + b * b;
+                x = lum * (1.0f / (q * std::sqrt(q)));
+            } else {
+                x = lum * st;
+            }
             if (!(x > 0.0f)) x = 0.0f;
             w[(size_t)i * W + j] = x;
             ccdf[(size_t)i * (W + 1) + j] = run;
@@ -468,15 +480,15 @@ bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uin
     return BI_PT_OK;
 }
 
-bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, const float color[3],
-                           const float rot[9]) {
-    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
-    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
-    if ((w == 0) != (h == 0) || ((w || h) && !rgba) || !color || !rot)
-        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "bad sky arguments");
+namespace {
+
+// The sky of a context: texels (equirect w x h, or cube faces w x w with h = 6 w) and
+// their importance table.
+bi_pt_status set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, uint32_t layout,
+                     const float color[3], const float rot[9]) {
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const size_t ntex = (size_t)w * h * 4;
-    if (ctx->sky_set && ctx->sky.width == w && ctx->sky.height == h &&
+    if (ctx->sky_set && ctx->sky.width == w && ctx->sky.height == h && ctx->sky.layout == layout &&
         std::memcmp(ctx->sky.color, color, sizeof ctx->sky.color) == 0 &&
         std::memcmp(ctx->sky.rot, rot, sizeof ctx->sky.rot) == 0 && ctx->sky_host.size() == ntex &&
         (ntex == 0 || std::memcmp(ctx->sky_host.data(), rgba, ntex * sizeof(float)) == 0))
@@ -486,12 +498,13 @@ bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32
     SkyParams sp{};
     sp.width = w;
     sp.height = h;
+    sp.layout = layout;
     std::memcpy(sp.color, color, sizeof sp.color);
     std::memcpy(sp.rot, rot, sizeof sp.rot);
     if (w) {
         std::vector<float> wt, cc, rows, marg;
         float total = 0.0f;
-        build_sky_table(rgba, w, h, color, wt, cc, rows, marg, total);
+        build_sky_table(rgba, w, h, layout, color, wt, cc, rows, marg, total);
         BI_TRY(dalloc(ctx, ctx->sky_tex, (size_t)w * h));
         BI_TRY(dalloc(ctx, ctx->sky_w, wt.size()));
         BI_TRY(dalloc(ctx, ctx->sky_ccdf, cc.size()));
@@ -508,12 +521,35 @@ bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32
         sp.rows = ctx->sky_rows.p;
         sp.marg = ctx->sky_marg.p;
         sp.total = total;
-        sp.wh_over_total = total > 0.0f ? (float)(w * h) / total : 0.0f;
+        if (total > 0.0f)
+            sp.pdf_scale = layout == kSkyCube ? ((float)w * (float)w / total) * 0.25f : (float)(w * h) / total;
     }
     ctx->sky = sp;
     ctx->acc_count = 0;
     ctx->scene_version++;
     return BI_PT_OK;
+}
+
+}  // namespace
+
+bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, const float color[3],
+                           const float rot[9]) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if ((w == 0) != (h == 0) || ((w || h) && !rgba) || !color || !rot)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "bad sky arguments");
+    return set_sky(ctx, rgba, w, h, kSkyEquirect, color, rot);
+}
+
+bi_pt_status bi_pt_set_sky_cube(bi_pt_ctx* ctx, const float* rgba_faces, uint32_t face_size, const float color[3],
+                                const float rot[9]) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if ((face_size && !rgba_faces) || !color || !rot)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "bad sky arguments");
+    if (face_size > (1u << 13))
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "cube face larger than 8192 texels");
+    return set_sky(ctx, rgba_faces, face_size, 6u * face_size, face_size ? kSkyCube : kSkyEquirect, color, rot);
 }
 
 bi_pt_status bi_pt_create_view(bi_pt_ctx* scene, bi_pt_ctx** out_view) {

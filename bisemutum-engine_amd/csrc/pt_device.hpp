@@ -12,6 +12,7 @@
 //   shading frame           shaders/core/utils/frame.hlsl:9-20 (Duff et al. 2017)
 //   camera ray              shaders/renderer/raytracing/direction_sample/generate_camera_ray.hlsl:9-15
 //   equirect mapping        shaders/core/equitangular_to_cubemap.hlsl:28-33
+//   cube map faces          shaders/core/utils/cubemap.hlsl:3-21 (face order and orientation)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -397,9 +398,113 @@ __device__ __forceinline__ V3 sky_texel(const SkyParams& s, int i, int j) {
     float4 t = s.texels[(size_t)i * s.width + j];
     return v3(t.x, t.y, t.z);
 }
-// Bilinear lookup (texel centres at (i+0.5)/W, wrap in u, clamp in v) times the sky colour.
+// ---- cube map (kSkyCube)
+// Face of a direction and its in-face coordinates sc, tc over |ma| (Vulkan's cube face
+// selection; ties go to X, then Y).  The face layout of cubemap.hlsl:3-21: face f,
+// texel column from a = 2u - 1, row from b = 2v - 1, direction cube_point(f, a, b).
+PT_HD uint32_t cube_face(V3 d, float* sc, float* tc, float* ma) {
+    const float ax = d.x < 0.0f ? -d.x : d.x, ay = d.y < 0.0f ? -d.y : d.y, az = d.z < 0.0f ? -d.z : d.z;
+    if (ax >= ay && ax >= az) {
+        *ma = ax;
+        *tc = -d.y;
+        if (d.x >= 0.0f) { *sc = -d.z; return 0u; }
+        *sc = d.z;
+        return 1u;
+    }
+    if (ay >= az) {
+        *ma = ay;
+        *sc = d.x;
+        if (d.y >= 0.0f) { *tc = d.z; return 2u; }
+        *tc = -d.z;
+        return 3u;
+    }
+    *ma = az;
+    *tc = -d.y;
+    if (d.z >= 0.0f) { *sc = d.x; return 4u; }
+    *sc = -d.x;
+    return 5u;
+}
+PT_HD V3 cube_point(uint32_t f, float a, float b) {
+    switch (f) {
+        case 0u: return v3(1.0f, -b, -a);
+        case 1u: return v3(-1.0f, -b, a);
+        case 2u: return v3(a, 1.0f, b);
+        case 3u: return v3(a, -1.0f, -b);
+        case 4u: return v3(a, -b, 1.0f);
+        default: return v3(-a, -b, -1.0f);
+    }
+}
+// Texel index of in-face coordinate c (= sc / ma) on a face of S texels.
+PT_HD int cube_index(float c, int S) {
+    const int k = (int)__builtin_floorf(((c + 1.0f) * 0.5f) * (float)S);
+    return k < 0 ? 0 : (k > S - 1 ? S - 1 : k);
+}
+// Texel (row i, column j) of face f whose centre lies beyond one edge of the face (one
+// of i, j is -1 or S): the texel of the adjacent face across that edge (Vulkan's
+// seamless cube filtering).  The centre, on the face's extended plane, is one half
+// texel outside; projected through the cube it lands in the adjacent face's edge row or
+// column, at the same position along the edge.
+PT_HD void cube_across(int S, uint32_t f, int i, int j, uint32_t* g, int* gi, int* gj) {
+    const float a = (float)(2 * j + 1) / (float)S - 1.0f, b = (float)(2 * i + 1) / (float)S - 1.0f;
+    float sc, tc, ma;
+    *g = cube_face(cube_point(f, a, b), &sc, &tc, &ma);
+    *gj = cube_index(sc / ma, S);
+    *gi = cube_index(tc / ma, S);
+}
+__device__ __forceinline__ V3 cube_texel(const SkyParams& s, uint32_t f, int i, int j) {
+    const float4 t = s.texels[((size_t)f * s.width + (size_t)i) * s.width + (size_t)j];
+    return v3(t.x, t.y, t.z);
+}
+// A bilinear tap of face f: inside the face its texel; across one edge the adjacent
+// face's texel; beyond a corner the mean of the three texels that meet there.
+__device__ __forceinline__ V3 cube_tap(const SkyParams& s, uint32_t f, int i, int j) {
+    const int S = (int)s.width;
+    const bool oi = i < 0 || i >= S, oj = j < 0 || j >= S;
+    if (!oi && !oj) return cube_texel(s, f, i, j);
+    uint32_t g;
+    int gi, gj;
+    if (oi != oj) {
+        cube_across(S, f, i, j, &g, &gi, &gj);
+        return cube_texel(s, g, gi, gj);
+    }
+    const int ci = i < 0 ? 0 : S - 1, cj = j < 0 ? 0 : S - 1;
+    const V3 c0 = cube_texel(s, f, ci, cj);
+    cube_across(S, f, ci, j, &g, &gi, &gj);
+    const V3 c1 = cube_texel(s, g, gi, gj);
+    cube_across(S, f, i, cj, &g, &gi, &gj);
+    const V3 c2 = cube_texel(s, g, gi, gj);
+    const V3 m = (c0 + c1) + c2;
+    return v3(m.x / 3.0f, m.y / 3.0f, m.z / 3.0f);
+}
+// Bilinear lookup in the face the direction selects (texel centres at (k+0.5)/S).
+__device__ __forceinline__ V3 cube_eval(const SkyParams& s, V3 ds) {
+    float sc, tc, ma;
+    const uint32_t f = cube_face(ds, &sc, &tc, &ma);
+    if (!(ma > 0.0f)) return v3(0.0f, 0.0f, 0.0f);
+    const float S = (float)s.width;
+    const float x = ((sc / ma + 1.0f) * 0.5f) * S - 0.5f, y = ((tc / ma + 1.0f) * 0.5f) * S - 0.5f;
+    const float fx0 = __builtin_floorf(x), fy0 = __builtin_floorf(y);
+    const float fx = x - fx0, fy = y - fy0;
+    const int j0 = (int)fx0, i0 = (int)fy0;
+    const V3 c00 = cube_tap(s, f, i0, j0), c10 = cube_tap(s, f, i0, j0 + 1);
+    const V3 c01 = cube_tap(s, f, i0 + 1, j0), c11 = cube_tap(s, f, i0 + 1, j0 + 1);
+    const V3 top = c00 + (c10 - c00) * fx;
+    const V3 bot = c01 + (c11 - c01) * fx;
+    return top + (bot - top) * fy;
+}
+// Solid-angle density of the cube's piecewise-constant texel sampling at face
+// coordinates (a, b): the texel's share w / total, uniform over its S^-2 of the face's
+// uv area, and dA_uv / d(omega) = r^3 / 4 with r^2 = 1 + a^2 + b^2.
+PT_HD float cube_density(float w, float scale, float a, float b) {
+    const float q = (1.0f + a * a) + b * b;
+    return (w * scale) * (q * __builtin_sqrtf(q));
+}
+
+// Bilinear lookup times the sky colour: equirect (texel centres at (i+0.5)/W, wrap in
+// u, clamp in v) or cube map.
 __device__ __forceinline__ V3 sky_eval(const SkyParams& s, V3 d) {
     if (s.width == 0) return v3(0.0f, 0.0f, 0.0f);
+    if (s.layout == kSkyCube) return cube_eval(s, sky_to_local(s, d)) * v3(s.color[0], s.color[1], s.color[2]);
     float u, v, th;
     sky_uv(sky_to_local(s, d), &u, &v, &th);
     const int W = (int)s.width, H = (int)s.height;
@@ -421,6 +526,16 @@ __device__ __forceinline__ V3 sky_eval(const SkyParams& s, V3 d) {
 }
 __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
     if (!(s.total > 0.0f)) return 0.0f;
+    if (s.layout == kSkyCube) {
+        float sc, tc, ma;
+        const uint32_t f = cube_face(sky_to_local(s, d), &sc, &tc, &ma);
+        if (!(ma > 0.0f)) return 0.0f;
+        const int S = (int)s.width;
+        const float a = sc / ma, b = tc / ma;
+        const float w = s.weight[((size_t)f * S + cube_index(b, S)) * S + cube_index(a, S)];
+        if (!(w > 0.0f)) return 0.0f;
+        return cube_density(w, s.pdf_scale, a, b);
+    }
     float u, v, th;
     sky_uv(sky_to_local(s, d), &u, &v, &th);
     const int W = (int)s.width, H = (int)s.height;
@@ -430,9 +545,10 @@ __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
     float w = s.weight[(size_t)i * W + j];
     float st = pt_sin(th);
     if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
-    return (w * s.wh_over_total) / (kTwoPiSq * st);
+    return (w * s.pdf_scale) / (kTwoPiSq * st);
 }
-// Piecewise-constant importance sampling of luminance * sin(theta) over the equirect.
+// Piecewise-constant importance sampling of the texel weights: a row from the running
+// row sums, a texel from the row's running sums, a uniform point inside the texel.
 __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u1, V3* dir, float* pdf) {
     *pdf = 0.0f;
     if (!(s.total > 0.0f)) return false;
@@ -460,18 +576,28 @@ __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u
     if (!(w > 0.0f)) return false;
     float du = (t1 - cc[j]) / w;
     du = du < 0.0f ? 0.0f : (du >= 1.0f ? 0x1.fffffep-1f : du);
-    float u = ((float)j + du) / (float)W;
-    float v = ((float)i + dv) / (float)H;
-    float st, ct, sp, cp;
-    pt_sincos(v * kPi, &st, &ct);
-    pt_sincos(u * kTwoPi, &sp, &cp);
-    if (!(st > 0.0f)) return false;
-    V3 ds = v3(st * sp, ct, st * cp);
+    V3 ds;
+    if (s.layout == kSkyCube) {
+        const uint32_t f = (uint32_t)i / (uint32_t)W;
+        const int r = i - (int)f * W;
+        const float a = (((float)j + du) / (float)W) * 2.0f - 1.0f;
+        const float b = (((float)r + dv) / (float)W) * 2.0f - 1.0f;
+        ds = normalize(cube_point(f, a, b));
+        *pdf = cube_density(w, s.pdf_scale, a, b);
+    } else {
+        float u = ((float)j + du) / (float)W;
+        float v = ((float)i + dv) / (float)H;
+        float st, ct, sp, cp;
+        pt_sincos(v * kPi, &st, &ct);
+        pt_sincos(u * kTwoPi, &sp, &cp);
+        if (!(st > 0.0f)) return false;
+        ds = v3(st * sp, ct, st * cp);
+        *pdf = (w * s.pdf_scale) / (kTwoPiSq * st);
+    }
     const float* R = s.rot;
     *dir = v3((R[0] * ds.x + R[3] * ds.y) + R[6] * ds.z,
               (R[1] * ds.x + R[4] * ds.y) + R[7] * ds.z,
               (R[2] * ds.x + R[5] * ds.y) + R[8] * ds.z);
-    *pdf = (w * s.wh_over_total) / (kTwoPiSq * st);
     return true;
 }
 

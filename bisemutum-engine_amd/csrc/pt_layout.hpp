@@ -145,14 +145,20 @@ struct CameraParams {
     uint32_t width, height;
 };
 
+// Sky texel layouts: an equirect (width x height, row 0 = +Y pole) or a cube map of six
+// width x width faces stored one after the other (+X, -X, +Y, -Y, +Z, -Z; height =
+// 6 * width), the layers of the engine's skybox texture.
+constexpr uint32_t kSkyEquirect = 0u, kSkyCube = 1u;
 struct SkyParams {
     const float4* texels;
-    const float* weight;   // luminance * sin(theta) per texel
+    const float* weight;   // per texel: luminance * sin(theta) (equirect) or * 1/r^3 (cube)
     const float* ccdf;     // per row: W+1 running sums
     const float* rows;     // per row sum
     const float* marg;     // H+1 running sums of rows
     uint32_t width, height;
-    float total, wh_over_total;
+    uint32_t layout;
+    float total;
+    float pdf_scale;       // W*H/total (equirect), S*S/(4*total) (cube)
     float color[3];
     float rot[9];
 };

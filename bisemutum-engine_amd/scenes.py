@@ -64,6 +64,7 @@ class Scene:
     mesh_uv: list = None    # per mesh: f32 [V, 2] texture coordinates, or None (all meshes without)
     mesh_tan: list = None   # per mesh: f32 [V, 4] tangents (xyz, bitangent sign), or None
     textures: list = dataclasses.field(default_factory=list)   # Texture records
+    sky_layout: int = 0     # SKY_EQUIRECT: sky_rgba [H, W, 4]; SKY_CUBE: [6, S, S, 4] (+X, -X, +Y, -Y, +Z, -Z)
 
     @property
     def num_tris(self) -> int:
@@ -258,6 +259,42 @@ def unit_arch(arc_segments=14, tube_segments=14, tube_radius=0.12, rise=0.5):
 
 
 # ---------------------------------------------------------------- sky
+
+SKY_EQUIRECT, SKY_CUBE = 0, 1
+
+
+def cube_face_directions(size):
+    """Unit directions of the texel centres of a cube map's six faces, f32 [6, S, S, 3]:
+    face f (+X, -X, +Y, -Y, +Z, -Z), row from v, column from u, with the orientation of
+    the engine's layered-cube mapping (shaders/core/utils/cubemap.hlsl:3-21)."""
+    c = (np.arange(size, dtype=np.float64) + 0.5) / size * 2.0 - 1.0
+    b, a = np.meshgrid(c, c, indexing="ij")
+    one = np.ones_like(a)
+    faces = [(one, -b, -a), (-one, -b, a), (a, one, b), (a, -one, -b), (a, -b, one), (-a, -b, -one)]
+    d = np.stack([np.stack(f, -1) for f in faces])
+    return (d / np.linalg.norm(d, axis=-1, keepdims=True)).astype(np.float32)
+
+
+def equirect_to_cube(rgba, size):
+    """A cube map made from an equirect as the engine imports a skybox
+    (shaders/core/equitangular_to_cubemap.hlsl:20-34): each face texel takes the
+    equirect's bilinear value (wrap in u, clamp in v) in its centre direction."""
+    d = cube_face_directions(size).astype(np.float64)
+    H, W = rgba.shape[:2]
+    th = np.arccos(np.clip(d[..., 1], -1.0, 1.0))
+    ph = np.where(d[..., 2] == 0.0, 0.0, np.arctan2(d[..., 0], d[..., 2]))
+    ph = np.where(ph < 0.0, ph + 2 * math.pi, ph)
+    x = ph / (2 * math.pi) * W - 0.5
+    y = th / math.pi * H - 0.5
+    x0, y0 = np.floor(x), np.floor(y)
+    fx, fy = (x - x0)[..., None], (y - y0)[..., None]
+    i0, j0 = y0.astype(np.int64), x0.astype(np.int64)
+    ja, jb = j0 % W, (j0 + 1) % W
+    ia, ib = np.clip(i0, 0, H - 1), np.clip(i0 + 1, 0, H - 1)
+    src = rgba.astype(np.float64)
+    top = src[ia, ja] + (src[ia, jb] - src[ia, ja]) * fx
+    bot = src[ib, ja] + (src[ib, jb] - src[ib, ja]) * fx
+    return (top + (bot - top) * fy).astype(np.float32)
 
 def constant_sky(color=(0.5, 0.6, 0.8), w=64, h=32):
     rgba = np.ones((h, w, 4), np.float32)

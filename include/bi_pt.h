@@ -260,6 +260,19 @@ bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uin
  * before lookup (deferred_lighting_secondary.hlsl:25). */
 bi_pt_status bi_pt_set_sky(bi_pt_ctx* ctx, const float* rgba32f, uint32_t width, uint32_t height,
                            const float color[3], const float rotation_3x3[9]);
+/* Cube-map sky: the engine's skybox texture (SkyboxSystem::current_skybox,
+ * src/scene_basic/skybox.cpp:56-67: a 2D texture of width == height with six RGBA32F
+ * layers, made by shaders/core/equitangular_to_cubemap.hlsl:20-34), host memory, six
+ * face_size x face_size faces one after the other in the layer order +X, -X, +Y, -Y,
+ * +Z, -Z, rows from v = 0, orientation of shaders/core/utils/cubemap.hlsl:3-21.
+ * Looked up as deferred_lighting_secondary.hlsl:25-26 samples its TextureCube with the
+ * linear sampler of src/renderer/context/skybox.cpp:31-37: level 0, bilinear, across
+ * face edges seamlessly as Vulkan filters cube maps (an out-of-face tap reads the
+ * adjacent face's texel; beyond a corner the mean of the three corner texels).  Importance sampling
+ * uses a per-texel table (luminance times solid angle).  face_size 0 clears the sky;
+ * at most 8192.  Replaces any equirect sky. */
+bi_pt_status bi_pt_set_sky_cube(bi_pt_ctx* ctx, const float* rgba32f_faces, uint32_t face_size,
+                                const float color[3], const float rotation_3x3[9]);
 
 /* ---- the pass (PathTracingPass::render, path_tracing.cpp:224-488) ----
  * Rebuilds the LBVH, traces settings->spp samples per pixel and writes the

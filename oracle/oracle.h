@@ -52,7 +52,8 @@ typedef struct ora_texture {
  * 3x4 row-major transform inst_xform[12*i..] and material inst_mat[i].
  * materials: 32 floats each, layout of bi_pt_material (include/bi_pt.h).
  * dir_lights: 16 floats each, layout of bi_pt_dir_light (= DirLightData).
- * sky: sky_w*sky_h RGBA32F texels (row 0 = theta 0, the +Y pole).
+ * sky: sky_w*sky_h RGBA32F texels (row 0 = theta 0, the +Y pole), or with sky_layout 1
+ * six sky_w x sky_w cube faces (+X, -X, +Y, -Y, +Z, -Z, rows from v = 0).
  * texcoords (2 floats per vertex) and tangents (4 per vertex) run parallel to the
  * positions (nullable: 0 and (1, 0, 0, 1)). */
 typedef struct ora_scene_desc {
@@ -79,6 +80,7 @@ typedef struct ora_scene_desc {
     const float* tangents;
     uint32_t num_textures;
     const ora_texture* textures;
+    uint32_t sky_layout;    /* 0 equirect; 1 cube map: sky_w x sky_w faces, sky_h = 6 sky_w */
 } ora_scene_desc;
 
 /* Camera: row-major 4x4 matrices (m[row*4+col]), as bi_pt_camera. */

@@ -38,6 +38,7 @@ class SceneDesc(C.Structure):
         ("sky_color", C.c_float * 3), ("sky_rot", C.c_float * 9),
         ("texcoords", _f32p), ("tangents", _f32p),
         ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
+        ("sky_layout", C.c_uint32),
     ]
 
 
@@ -164,8 +165,16 @@ class OracleScene:
         d.materials = ptr(k["materials"], _f32p)
         d.num_dir_lights = len(scene.dir_lights)
         d.dir_lights = ptr(k["dir_lights"], _f32p)
-        d.sky_h, d.sky_w = scene.sky_rgba.shape[0], scene.sky_rgba.shape[1]
-        d.sky_rgba = ptr(k["sky"], _f32p)
+        layout = getattr(scene, "sky_layout", 0)
+        if layout == 1:   # cube faces [6, S, S, 4]: rows face-major
+            if k["sky"].size and (k["sky"].ndim != 4 or k["sky"].shape[0] != 6 or k["sky"].shape[1] != k["sky"].shape[2]):
+                raise ValueError(f"cube sky must be [6, S, S, 4], got {k['sky'].shape}")
+            d.sky_w = k["sky"].shape[1] if k["sky"].size else 0
+            d.sky_h = 6 * d.sky_w
+            d.sky_layout = 1 if d.sky_w else 0
+        else:
+            d.sky_h, d.sky_w = scene.sky_rgba.shape[0], scene.sky_rgba.shape[1]
+        d.sky_rgba = ptr(k["sky"], _f32p) if k["sky"].size else None
         for i in range(3):
             d.sky_color[i] = float(scene.sky_color[i])
         for i in range(9):

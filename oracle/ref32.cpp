@@ -14,7 +14,8 @@
 //                     (interpolated normal, two-sided flip), lit.hlsl:5-35 (BSDF),
 //                     material/utils.hlsl:40-121 (Fresnel, GGX, VNDF), frame.hlsl:9-20
 //   light loop        deferred_lighting_secondary.hlsl:45-60, lights.hlsl:10-13,69-101
-//   sky               deferred_lighting_secondary.hlsl:24-29, equitangular_to_cubemap.hlsl:28-33
+//   sky               deferred_lighting_secondary.hlsl:24-29, equitangular_to_cubemap.hlsl:28-33,
+//                     cube map faces utils/cubemap.hlsl:3-21
 //   bounce loop       path_tracing.cpp:289-460, sample_secondary_ray.hlsl:17-68
 //   accumulation      pt_accumulate.hlsl:9 as a running FP32 sum over samples
 #include "oracle.h"
@@ -946,8 +947,90 @@ inline void sky_uv(F3 ds, float* u, float* v, float* theta) {
     *v = th * kInvPi;
     *theta = th;
 }
+// ---- cube map sky (Scene::sky_layout 1): six W x W faces, face-major rows.
+// Face selection of Vulkan's cube sampling (largest |component|, ties to X then Y) with
+// the face orientation of cubemap.hlsl:3-21: face f holds the direction
+// cube_dir(f, a, b) at column coordinate a = 2u - 1 and row coordinate b = 2v - 1.
+int cube_select(F3 d, float* sc, float* tc, float* ma) {
+    float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    if (ax >= ay && ax >= az) {
+        *ma = ax; *tc = -d.y;
+        if (d.x >= 0.0f) { *sc = -d.z; return 0; }
+        *sc = d.z; return 1;
+    }
+    if (ay >= az) {
+        *ma = ay; *sc = d.x;
+        if (d.y >= 0.0f) { *tc = d.z; return 2; }
+        *tc = -d.z; return 3;
+    }
+    *ma = az; *tc = -d.y;
+    if (d.z >= 0.0f) { *sc = d.x; return 4; }
+    *sc = -d.x; return 5;
+}
+F3 cube_dir(int f, float a, float b) {
+    static const float sign_tab[6][3] = {{1, -1, -1}, {-1, -1, 1}, {1, 1, 1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, -1}};
+    const float* g = sign_tab[f];
+    if (f < 2) return f3(g[0], g[1] * b, g[2] * a);
+    if (f < 4) return f3(g[0] * a, g[1], g[2] * b);
+    return f3(g[0] * a, g[1] * b, g[2]);
+}
+// texel number of a face coordinate c in [-1, 1] on S texels
+int cube_texel_index(float c, int S) {
+    int k = (int)std::floor(((c + 1.0f) * 0.5f) * (float)S);
+    return k < 0 ? 0 : (k > S - 1 ? S - 1 : k);
+}
+F3 cube_fetch(const Scene& s, int f, int i, int j) {
+    const int S = (int)s.sky_w;
+    const float* p = &s.sky_rgba[4 * (((size_t)f * S + i) * S + j)];
+    return f3(p[0], p[1], p[2]);
+}
+// A texel one step outside face f across an edge (row i or column j is -1 or S) is the
+// adjacent face's texel that contains the projection of its centre through the cube.
+F3 cube_fetch_outside(const Scene& s, int f, int i, int j) {
+    const int S = (int)s.sky_w;
+    float a = (float)(2 * j + 1) / (float)S - 1.0f, b = (float)(2 * i + 1) / (float)S - 1.0f;
+    float sc, tc, ma;
+    int g = cube_select(cube_dir(f, a, b), &sc, &tc, &ma);
+    return cube_fetch(s, g, cube_texel_index(tc / ma, S), cube_texel_index(sc / ma, S));
+}
+// Bilinear tap with seamless edges: beyond a corner, the mean of the face's corner
+// texel and the two adjacent faces' texels at that corner.
+F3 cube_tap_value(const Scene& s, int f, int i, int j) {
+    const int S = (int)s.sky_w;
+    bool out_i = i < 0 || i >= S, out_j = j < 0 || j >= S;
+    if (!out_i && !out_j) return cube_fetch(s, f, i, j);
+    if (out_i != out_j) return cube_fetch_outside(s, f, i, j);
+    int ci = i < 0 ? 0 : S - 1, cj = j < 0 ? 0 : S - 1;
+    F3 sum = (cube_fetch(s, f, ci, cj) + cube_fetch_outside(s, f, ci, j)) + cube_fetch_outside(s, f, i, cj);
+    return f3(sum.x / 3.0f, sum.y / 3.0f, sum.z / 3.0f);
+}
+F3 cube_lookup(const Scene& s, F3 dl) {
+    float sc, tc, ma;
+    int f = cube_select(dl, &sc, &tc, &ma);
+    if (!(ma > 0.0f)) return f3(0, 0, 0);
+    const float S = (float)s.sky_w;
+    float x = ((sc / ma + 1.0f) * 0.5f) * S - 0.5f;
+    float y = ((tc / ma + 1.0f) * 0.5f) * S - 0.5f;
+    float x0 = std::floor(x), y0 = std::floor(y);
+    float fx = x - x0, fy = y - y0;
+    int j = (int)x0, i = (int)y0;
+    F3 c00 = cube_tap_value(s, f, i, j), c10 = cube_tap_value(s, f, i, j + 1);
+    F3 c01 = cube_tap_value(s, f, i + 1, j), c11 = cube_tap_value(s, f, i + 1, j + 1);
+    F3 top = c00 + (c10 - c00) * fx;
+    F3 bot = c01 + (c11 - c01) * fx;
+    return top + (bot - top) * fy;
+}
+// Solid-angle density of a point (a, b) of the texel with weight w: (w / total) * S^2
+// per unit face uv area, times dA_uv / d(omega) = (1 + a^2 + b^2)^(3/2) / 4.
+float cube_pdf_at(const Scene& s, float w, float a, float b) {
+    float q = (1.0f + a * a) + b * b;
+    return (w * s.sky_pdf_scale) * (q * std::sqrt(q));
+}
+
 F3 sky_eval(const Scene& s, F3 d) {
     if (s.sky_w == 0) return f3(0, 0, 0);
+    if (s.sky_layout == 1u)
+        return cube_lookup(s, sky_to_local(s, d)) * f3(s.sky_color[0], s.sky_color[1], s.sky_color[2]);
     float u, v, th;
     sky_uv(sky_to_local(s, d), &u, &v, &th);
     const int W = (int)s.sky_w, H = (int)s.sky_h;
@@ -972,6 +1055,16 @@ F3 sky_eval(const Scene& s, F3 d) {
 }
 float sky_pdf(const Scene& s, F3 d) {
     if (!(s.sky_total > 0.0f)) return 0.0f;
+    if (s.sky_layout == 1u) {
+        float sc, tc, ma;
+        int f = cube_select(sky_to_local(s, d), &sc, &tc, &ma);
+        if (!(ma > 0.0f)) return 0.0f;
+        const int S = (int)s.sky_w;
+        float a = sc / ma, b = tc / ma;
+        float w = s.sky_w_tex[((size_t)f * S + cube_texel_index(b, S)) * S + cube_texel_index(a, S)];
+        if (!(w > 0.0f)) return 0.0f;
+        return cube_pdf_at(s, w, a, b);
+    }
     float u, v, th;
     sky_uv(sky_to_local(s, d), &u, &v, &th);
     const int W = (int)s.sky_w, H = (int)s.sky_h;
@@ -981,9 +1074,10 @@ float sky_pdf(const Scene& s, F3 d) {
     float w = s.sky_w_tex[(size_t)i * W + j];
     float st = sin32(th);
     if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
-    return (w * s.sky_wh_over_total) / (kTwoPiSq * st);
+    return (w * s.sky_pdf_scale) / (kTwoPiSq * st);
 }
-// Piecewise-constant (luminance * sin(theta)) importance sampling of the equirect.
+// Piecewise-constant importance sampling of the texel weights (a row by the running
+// row sums, a texel by the row's running sums, a uniform point in the texel).
 bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
     *pdf = 0.0f;
     if (!(s.sky_total > 0.0f)) return false;
@@ -1011,18 +1105,27 @@ bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
     if (!(w > 0.0f)) return false;
     float du = (t1 - cc[j]) / w;
     du = du < 0.0f ? 0.0f : (du >= 1.0f ? 0x1.fffffep-1f : du);
-    float u = ((float)j + du) / (float)W;
-    float v = ((float)i + dv) / (float)H;
-    float st, ct, sp, cp;
-    sincos32(v * kPiF, &st, &ct);
-    sincos32(u * kTwoPi, &sp, &cp);
-    if (!(st > 0.0f)) return false;
-    F3 ds = f3(st * sp, ct, st * cp);
+    F3 ds;
+    if (s.sky_layout == 1u) {
+        int f = i / W, row = i - f * W;
+        float a = (((float)j + du) / (float)W) * 2.0f - 1.0f;
+        float b = (((float)row + dv) / (float)W) * 2.0f - 1.0f;
+        ds = normalize(cube_dir(f, a, b));
+        *pdf = cube_pdf_at(s, w, a, b);
+    } else {
+        float u = ((float)j + du) / (float)W;
+        float v = ((float)i + dv) / (float)H;
+        float st, ct, sp, cp;
+        sincos32(v * kPiF, &st, &ct);
+        sincos32(u * kTwoPi, &sp, &cp);
+        if (!(st > 0.0f)) return false;
+        ds = f3(st * sp, ct, st * cp);
+        *pdf = (w * s.sky_pdf_scale) / (kTwoPiSq * st);
+    }
     const float* R = s.sky_rot;
     *dir = f3((R[0] * ds.x + R[3] * ds.y) + R[6] * ds.z,
               (R[1] * ds.x + R[4] * ds.y) + R[7] * ds.z,
               (R[2] * ds.x + R[5] * ds.y) + R[8] * ds.z);
-    *pdf = (w * s.sky_wh_over_total) / (kTwoPiSq * st);
     return true;
 }
 

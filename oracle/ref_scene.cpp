@@ -73,23 +73,37 @@ void flatten(const ora_scene_desc& d, Scene& s) {
     }
 }
 
-// Importance table over the equirect: w = luminance(texel * sky_color) * sin(theta_row).
+// Importance table over the sky texels: w = luminance(texel * sky_color) times
+// sin(theta_row) (equirect) or 1/r^3 at the texel centre (cube face texel at face
+// coordinates a = (2j+1)/S - 1, b = (2i+1)/S - 1, r^2 = 1 + a^2 + b^2, proportional to
+// the texel's solid angle); running sums per row and over rows (cube rows: face-major).
 void build_sky_table(Scene& s) {
     const uint32_t W = s.sky_w, H = s.sky_h;
+    const bool cube = s.sky_layout == 1u;
     s.sky_w_tex.assign((size_t)W * H, 0.0f);
     s.sky_ccdf.assign((size_t)(W + 1) * H, 0.0f);
     s.sky_rows.assign(H, 0.0f);
     s.sky_marg.assign(H + 1, 0.0f);
+    s.sky_pdf_scale = 0.0f;
     if (W == 0 || H == 0) { s.sky_total = 0.0f; return; }
     const float pi_over_h = kPiF / (float)H;
     for (uint32_t i = 0; i < H; ++i) {
-        float st = sin32(((float)i + 0.5f) * pi_over_h);
+        float st = 0.0f, b = 0.0f;
+        if (cube) b = (float)(2 * (i % W) + 1) / (float)W - 1.0f;
+        else st = sin32(((float)i + 0.5f) * pi_over_h);
         float run = 0.0f;
         for (uint32_t j = 0; j < W; ++j) {
             const float* p = &s.sky_rgba[4 * ((size_t)i * W + j)];
-            float r = p[0] * s.sky_color[0], g = p[1] * s.sky_color[1], b = p[2] * s.sky_color[2];
-            float lum = (0.2126f * r + 0.7152f * g) + 0.0722f * b;
-            float w = lum * st;
+            float r = p[0] * s.sky_color[0], g = p[1] * s.sky_color[1], bl = p[2] * s.sky_color[2];
+            float lum = (0.2126f * r + 0.7152f * g) + 0.0722f * bl;
+            float w;
+            if (cube) {
+                float a = (float)(2 * j + 1) / (float)W - 1.0f;
+                float q = (1.0f + a * a) + b * b;
+                w = lum * (1.0f / (q * std::sqrt(q)));
+            } else {
+                w = lum * st;
+            }
             if (!(w > 0.0f)) w = 0.0f;
             s.sky_w_tex[(size_t)i * W + j] = w;
             s.sky_ccdf[(size_t)i * (W + 1) + j] = run;
@@ -100,7 +114,8 @@ void build_sky_table(Scene& s) {
         s.sky_marg[i + 1] = s.sky_marg[i] + run;
     }
     s.sky_total = s.sky_marg[H];
-    s.sky_wh_over_total = s.sky_total > 0.0f ? (float)(W * H) / s.sky_total : 0.0f;
+    if (s.sky_total > 0.0f)
+        s.sky_pdf_scale = cube ? ((float)W * (float)W / s.sky_total) * 0.25f : (float)(W * H) / s.sky_total;
 }
 
 }  // namespace
@@ -140,6 +155,7 @@ ora_ctx* ora_create(const ora_scene_desc* d) {
     s->dir_lights.assign(d->dir_lights, d->dir_lights + 16 * (size_t)d->num_dir_lights);
     s->sky_w = d->sky_rgba ? d->sky_w : 0;
     s->sky_h = d->sky_rgba ? d->sky_h : 0;
+    s->sky_layout = d->sky_rgba ? d->sky_layout : 0;
     if (d->sky_rgba) s->sky_rgba.assign(d->sky_rgba, d->sky_rgba + 4 * (size_t)d->sky_w * d->sky_h);
     std::memcpy(s->sky_color, d->sky_color, sizeof s->sky_color);
     std::memcpy(s->sky_rot, d->sky_rot, sizeof s->sky_rot);

@@ -45,15 +45,16 @@ struct Scene {
     std::vector<float> dir_lights; // 16 per light
     // sky
     uint32_t sky_w = 0, sky_h = 0;
+    uint32_t sky_layout = 0;       // 0 equirect (W x H), 1 cube map (six W x W faces, H = 6 W)
     std::vector<float> sky_rgba;
     float sky_color[3] = {0, 0, 0};
     float sky_rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    std::vector<float> sky_w_tex;  // per texel luminance * sin(theta_row)
+    std::vector<float> sky_w_tex;  // per texel luminance * sin(theta_row) (cube: * 1/r^3)
     std::vector<float> sky_ccdf;   // per row W+1 running sums
     std::vector<float> sky_rows;   // per row sum
     std::vector<float> sky_marg;   // H+1 running sums of rows
     float sky_total = 0.0f;
-    float sky_wh_over_total = 0.0f;
+    float sky_pdf_scale = 0.0f;    // W*H/total (equirect), W*W/(4*total) (cube)
     // LBVH (FP32 transcription)
     float cbounds[6] = {};
     std::vector<uint32_t> morton, order, leaf_tri;
