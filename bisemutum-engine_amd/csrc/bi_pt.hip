@@ -59,6 +59,19 @@ void build_sky_table(const float* rgba, uint32_t W, uint32_t H, uint32_t layout,
     total = marg[H];
 }
 
+// Guide tables that bracket the two searches of the sampler: for k in [0, n), the first
+// index i in [0, n - 1) with cum[i + 1] > ((float)k / n) * sum, else n - 1.  The sampler
+// searches between the entries for k - 1 and k + 2, k = (int)(u * n), and so ends at the
+// index of a binary search over the whole range.
+void build_guide(const float* cum, uint32_t n, float sum, uint32_t* guide) {
+    uint32_t i = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const float thr = ((float)k / (float)n) * sum;
+        while (i < n - 1 && !(cum[i + 1] > thr)) ++i;
+        guide[k] = i;
+    }
+}
+
 }  // namespace
 
 namespace bipt {
@@ -240,7 +253,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     (void)hipDeviceSynchronize();
     dfree(ctx, ctx->pos); dfree(ctx, ctx->nrm); dfree(ctx, ctx->idx); dfree(ctx, ctx->inst); dfree(ctx, ctx->mats);
     dfree(ctx, ctx->sky_tex); dfree(ctx, ctx->sky_w); dfree(ctx, ctx->sky_ccdf); dfree(ctx, ctx->sky_rows);
-    dfree(ctx, ctx->sky_marg);
+    dfree(ctx, ctx->sky_marg); dfree(ctx, ctx->sky_mguide); dfree(ctx, ctx->sky_rguide);
     dfree(ctx, ctx->tri_v); dfree(ctx, ctx->tri_n); dfree(ctx, ctx->cb_out); dfree(ctx, ctx->tri_mat);
     dfree(ctx, ctx->cb_ord); dfree(ctx, ctx->keys0); dfree(ctx, ctx->keys1); dfree(ctx, ctx->vals0);
     dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes);
@@ -302,7 +315,7 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
     for (uint32_t m = 0; m < num_materials; ++m)
         if (!(materials[m].roughness >= 0.0f) || !(materials[m].anisotropy >= 0.0f && materials[m].anisotropy <= 1.0f))
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "material roughness/anisotropy out of range");
-    bool need_uv = false, need_tan = false;
+    bool need_uv = false, need_tan = false, non_opaque = false;
     uint32_t tex_refs = 0;
     for (uint32_t m = 0; m < num_materials; ++m) {
         const bi_pt_material& M = materials[m];
@@ -318,6 +331,7 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
             tex_refs = std::max(tex_refs, t);   // references are index + 1
         }
         need_tan = need_tan || M.normal_tex != BI_PT_NO_TEXTURE;
+        non_opaque = non_opaque || M.blend_mode != BI_PT_BLEND_OPAQUE;
         if (M.pattern == BI_PT_PATTERN_CHECKER_XZ &&
             (M.pattern_material >= num_materials || materials[M.pattern_material].pattern != BI_PT_PATTERN_NONE))
             return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT,
@@ -371,6 +385,7 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
     }
     ctx->need_uv = need_uv;
     ctx->need_tan = need_tan;
+    ctx->non_opaque = non_opaque;
     ctx->tex_refs = tex_refs;
     BI_TRY(dalloc(ctx, ctx->pos, std::max<size_t>(hp.size(), 1)));
     BI_TRY(dalloc(ctx, ctx->nrm, std::max<size_t>(hn.size(), 1)));
@@ -505,6 +520,13 @@ bi_pt_status set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, 
         std::vector<float> wt, cc, rows, marg;
         float total = 0.0f;
         build_sky_table(rgba, w, h, layout, color, wt, cc, rows, marg, total);
+        std::vector<uint32_t> mg(h), rg((size_t)w * h);
+        build_guide(marg.data(), h, total, mg.data());
+        for (uint32_t i = 0; i < h; ++i) build_guide(&cc[(size_t)i * (w + 1)], w, rows[i], &rg[(size_t)i * w]);
+        BI_TRY(dalloc(ctx, ctx->sky_mguide, mg.size()));
+        BI_TRY(dalloc(ctx, ctx->sky_rguide, rg.size()));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_mguide.p, mg.data(), mg.size() * 4, hipMemcpyHostToDevice));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_rguide.p, rg.data(), rg.size() * 4, hipMemcpyHostToDevice));
         BI_TRY(dalloc(ctx, ctx->sky_tex, (size_t)w * h));
         BI_TRY(dalloc(ctx, ctx->sky_w, wt.size()));
         BI_TRY(dalloc(ctx, ctx->sky_ccdf, cc.size()));
@@ -520,6 +542,8 @@ bi_pt_status set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, 
         sp.ccdf = ctx->sky_ccdf.p;
         sp.rows = ctx->sky_rows.p;
         sp.marg = ctx->sky_marg.p;
+        sp.mguide = ctx->sky_mguide.p;
+        sp.rguide = ctx->sky_rguide.p;
         sp.total = total;
         if (total > 0.0f)
             sp.pdf_scale = layout == kSkyCube ? ((float)w * (float)w / total) * 0.25f : (float)(w * h) / total;
@@ -756,6 +780,7 @@ bi_pt_status bi_pt_debug_trace(bi_pt_ctx* ctx, uint32_t n, const float* rays, in
             ta.work = cnt.p + 4;   // zeroed part counters
             ta.parts = kWorkParts;
             ta.refill = 32;
+            ta.any_hit = ctx->non_opaque ? 1u : 0u;
             ta.hits = HitBuf{hrec.p};
             ta.error = &ctx->eng.counters.p[kCntError];
             if (status == BI_PT_OK) launch_trace_closest(ta, n, false, 16, 256, nullptr);
@@ -816,6 +841,7 @@ bi_pt_status bi_pt_trace_closest(bi_pt_ctx* ctx, uint32_t n, const bi_pt_rays* r
     ta.work = ctx->rq_cnt.p + 4;
     ta.parts = kWorkParts;
     ta.refill = 32;
+    ta.any_hit = ctx->non_opaque ? 1u : 0u;
     ta.hits = HitBuf{ctx->rq_hits.p};
     ta.error = &ctx->eng.counters.p[kCntError];
     launch_trace_closest(ta, n, false, 16, 256, s);
@@ -921,6 +947,7 @@ bi_pt_status bi_pt_trace_closest_noalloc(bi_pt_ctx* ctx, uint32_t n, const bi_pt
     ta.work = cnt + 4;
     ta.parts = kWorkParts;
     ta.refill = 32;
+    ta.any_hit = ctx->non_opaque ? 1u : 0u;
     ta.hits = HitBuf{hits};
     ta.error = &ctx->eng.counters.p[kCntError];
     launch_trace_closest(ta, n, false, 16, 256, s);

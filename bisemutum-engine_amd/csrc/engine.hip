@@ -311,6 +311,10 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     const uint32_t parts = kn.work_parts ? std::min(kn.work_parts, kWorkPartsMax) : kWorkParts;
     const uint32_t block = kn.block_threads == 128 || kn.block_threads == 512 ? kn.block_threads : 256u;
 
+    // non-opaque triangles anywhere in the call select the traversal kernels with any-hit tests
+    bool any_hit = false;
+    for (uint32_t j = 0; j < n; ++j) any_hit = any_hit || prep[j].scene->non_opaque;
+
     // device memory: per-job images, BVH builds, wave buffers, plan
     bool any_cached = false, any_train = false;
     for (uint32_t j = 0; j < n; ++j) {
@@ -607,6 +611,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 ta.work = &cnt[kCntTraceWork];
                 ta.parts = parts;
                 ta.refill = refill;
+                ta.any_hit = any_hit ? 1u : 0u;
                 ta.error = &cnt[kCntError];
                 BI_TRY(mark(0));
                 launch_trace_closest(ta, J.num_paths, primary, lds_stack, block, s);
@@ -657,6 +662,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sha.work = &cnt[kCntShadowWork];
                 sha.parts = parts;
                 sha.refill = refill;
+                sha.any_hit = any_hit ? 1u : 0u;
                 sha.error = &cnt[kCntError];
                 launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, block, s);
                 ++launches_shadow;

@@ -547,6 +547,20 @@ __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
     if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
     return (w * s.pdf_scale) / (kTwoPiSq * st);
 }
+// The first index i in [0, n - 1) with cum[i + 1] > t (else n - 1) for t = u * sum: the
+// index a binary search over the running sums finds, searched only between the guide
+// table's entries for k - 1 and k + 2, k = (int)(u * n) (rounding moves k by at most one,
+// so their thresholds stay at or below and above t: the answer lies between them).
+__device__ __forceinline__ int guided_search(const float* cum, const uint32_t* guide, int n, float u, float t) {
+    const int k = (int)(u * (float)n);
+    int lo = (int)guide[k - 1 < 0 ? 0 : (k - 1 > n - 1 ? n - 1 : k - 1)];
+    int hi = k + 2 > n - 1 ? n - 1 : (int)guide[k + 2];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cum[mid + 1] > t) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
 // Piecewise-constant importance sampling of the texel weights: a row from the running
 // row sums, a texel from the row's running sums, a uniform point inside the texel.
 __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u1, V3* dir, float* pdf) {
@@ -554,24 +568,14 @@ __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u
     if (!(s.total > 0.0f)) return false;
     const int W = (int)s.width, H = (int)s.height;
     float t0 = u0 * s.total;
-    int lo = 0, hi = H - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (s.marg[mid + 1] > t0) hi = mid; else lo = mid + 1;
-    }
-    int i = lo;
+    int i = guided_search(s.marg, s.mguide, H, u0, t0);
     float rs = s.rows[i];
     if (!(rs > 0.0f)) return false;
     float dv = (t0 - s.marg[i]) / rs;
     dv = dv < 0.0f ? 0.0f : (dv >= 1.0f ? 0x1.fffffep-1f : dv);
     float t1 = u1 * rs;
     const float* cc = s.ccdf + (size_t)i * (W + 1);
-    lo = 0; hi = W - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (cc[mid + 1] > t1) hi = mid; else lo = mid + 1;
-    }
-    int j = lo;
+    int j = guided_search(cc, s.rguide + (size_t)i * W, W, u1, t1);
     float w = s.weight[(size_t)i * W + j];
     if (!(w > 0.0f)) return false;
     float du = (t1 - cc[j]) / w;

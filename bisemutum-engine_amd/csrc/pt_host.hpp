@@ -156,6 +156,7 @@ struct bi_pt_ctx : bipt::Owner {
     // texture coordinates and tangents (vertex pools, present when the materials sample
     // textures / normal maps) and the material textures
     bool need_uv = false, need_tan = false;
+    bool non_opaque = false;        // some material is alpha-tested or translucent (any-hit tests)
     bipt::DevBuf<float> uvs, tans;
     bipt::DevBuf<bipt::TexRec> tex_recs;
     bipt::DevBuf<uint32_t> tex_pool;
@@ -167,6 +168,7 @@ struct bi_pt_ctx : bipt::Owner {
     // sky
     bipt::DevBuf<float4> sky_tex;
     bipt::DevBuf<float> sky_w, sky_ccdf, sky_rows, sky_marg;
+    bipt::DevBuf<uint32_t> sky_mguide, sky_rguide;
     bipt::SkyParams sky{};
     std::vector<float> sky_host;
     bool sky_set = false;

@@ -1,15 +1,18 @@
 // pt_layout.hpp -- device data layout of the path-tracing pass (host and device).
 //
 // HBM layout (one context):
-//   scene      mesh pool (float xyz positions/normals, uint32 indices), instance records,
-//              material table (64 B each), equirect sky + importance tables
+//   scene      mesh pool (float xyz positions/normals, uint32 indices; texture
+//              coordinates and tangents where materials sample textures), instance
+//              records, material table (128 B each), texture records + texel pool + sRGB
+//              table, sky texels (equirect or six cube faces) + importance and guide tables
 //   per frame  world triangles (36 B) + normals (36 B) in instance order, Morton keys,
 //              sorted order, BVH2 nodes (64 B: both child boxes + child links; the build
 //              structure), BVH4 nodes (128 B, one 128-B line: four child boxes as six
 //              4-wide planes + four child links; the traversal structure, stored at the
 //              index of the BVH2 node it collapses), leaf intersection records (48 B:
-//              v0, e1, e2) and leaf shading records (48 B: n0, n1, n2, material) in sorted
-//              leaf order
+//              v0, e1, e2, a non-opaque mark), leaf shading records (48 B: n0, n1, n2,
+//              material) and, where needed, leaf texture coordinates (32 B) and tangents
+//              (48 B) in sorted leaf order
 //   per wave   SoA ray queues (o.xyz, d.xyz, path id), 16-B hit records (t, leaf, u, v),
 //              SoA path state (throughput, last BSDF pdf), per-kind radiance accumulators,
 //              SoA shadow queue (o.xyz, d.xyz, contribution.xyz, path<<3|kind)
@@ -155,6 +158,11 @@ struct SkyParams {
     const float* ccdf;     // per row: W+1 running sums
     const float* rows;     // per row sum
     const float* marg;     // H+1 running sums of rows
+    // guide tables of the two searches: mguide[k] = the first row i with marg[i+1] above
+    // (k / H) * total; rguide[i*W + k] = the first texel j of row i with ccdf[j+1] above
+    // (k / W) * rows[i] (the last index where none is)
+    const uint32_t* mguide;
+    const uint32_t* rguide;
     uint32_t width, height;
     uint32_t layout;
     float total;

@@ -246,7 +246,9 @@ __device__ __forceinline__ uint32_t append_leaves(uint32_t* qleaf, uint8_t* qown
     return qn;
 }
 
-template <bool PRIMARY, int L, int B>
+// AH: the call's scenes hold non-opaque (alpha-tested or translucent) triangles, whose
+// hits pass the any-hit test; without them the test and its registers are compiled out.
+template <bool PRIMARY, bool AH, int L, int B>
 __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
     __shared__ int lds[L][B];
     __shared__ uint32_t s_qleaf[B / 64][kLeafQueue];
@@ -353,10 +355,12 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
                 const LeafRegs lr = valid ? load_leaf(og, leaf) : LeafRegs{};
                 bool cand = valid && tri_test(lr, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tb &&
                             tnl <= t * kRobust;
-                const bool nop = cand && leaf_non_opaque(lr);
-                if (__ballot(nop) != 0ull) {
-                    const JobRec* oj = shfl_ptr(jr, owner);
-                    if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
+                if constexpr (AH) {
+                    const bool nop = cand && leaf_non_opaque(lr);
+                    if (__ballot(nop) != 0ull) {
+                        const JobRec* oj = shfl_ptr(jr, owner);
+                        if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
+                    }
                 }
                 if (cand) atomicMin(&key[owner], ((unsigned long long)__float_as_uint(t) << 32) | leaf);
             }
@@ -520,7 +524,7 @@ __device__ __forceinline__ bool apply_shadow(const ShadowArgs& a, uint32_t i, bo
     return kind == kKindSky;
 }
 
-template <int L, int B>
+template <bool AH, int L, int B>
 __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
     __shared__ int lds[L][B];
     __shared__ uint32_t s_qleaf[B / 64][kLeafQueue];
@@ -605,10 +609,12 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
                 float t = 0.0f, un = 0.0f, vn = 0.0f, det = 1.0f;
                 const LeafRegs lr = valid ? load_leaf(og, leaf) : LeafRegs{};
                 bool cand = valid && tri_test(lr, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax;
-                const bool nop = cand && leaf_non_opaque(lr);
-                if (__ballot(nop) != 0ull) {
-                    const JobRec* oj = shfl_ptr(jr, owner);
-                    if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
+                if constexpr (AH) {
+                    const bool nop = cand && leaf_non_opaque(lr);
+                    if (__ballot(nop) != 0ull) {
+                        const JobRec* oj = shfl_ptr(jr, owner);
+                        if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
+                    }
                 }
                 if (cand) occf[owner] = 1u;
             }
@@ -692,48 +698,55 @@ uint32_t resident_groups(K kernel, int block) {
     return (uint32_t)(cus * per_cu);
 }
 
-template <bool P, int L, int B>
+template <bool P, bool AH, int L, int B>
 void closest_lb(const TraceArgs& a, uint32_t capacity, hipStream_t s) {
     uint32_t groups = (capacity + B - 1u) / B;
     static uint32_t resident = 0;
-    if (resident == 0) resident = resident_groups(k_trace_closest<P, L, B>, B);
+    if (resident == 0) resident = resident_groups(k_trace_closest<P, AH, L, B>, B);
     if (groups > resident) groups = resident;
     if (groups == 0) groups = 1;
-    hipLaunchKernelGGL((k_trace_closest<P, L, B>), dim3(groups), dim3(B), 0, s, a);
+    hipLaunchKernelGGL((k_trace_closest<P, AH, L, B>), dim3(groups), dim3(B), 0, s, a);
 }
 
-template <int L, int B>
+template <bool AH, int L, int B>
 void shadow_lb(const ShadowArgs& a, uint32_t capacity, hipStream_t s) {
     uint32_t groups = (capacity + B - 1u) / B;
     static uint32_t resident = 0;
-    if (resident == 0) resident = resident_groups(k_trace_shadow<L, B>, B);
+    if (resident == 0) resident = resident_groups(k_trace_shadow<AH, L, B>, B);
     if (groups > resident) groups = resident;
     if (groups == 0) groups = 1;
-    hipLaunchKernelGGL((k_trace_shadow<L, B>), dim3(groups), dim3(B), 0, s, a);
+    hipLaunchKernelGGL((k_trace_shadow<AH, L, B>), dim3(groups), dim3(B), 0, s, a);
 }
 
 // Block shapes: 128, 256 (default) or 512 threads; LDS stack depths 8, 16 (default), 24, 32.
-template <bool P, int B>
+template <bool P, bool AH, int B>
 void closest_b(const TraceArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
-    if (lds <= 8) closest_lb<P, 8, B>(a, capacity, s);
-    else if (lds <= 16) closest_lb<P, 16, B>(a, capacity, s);
-    else if (lds <= 24) closest_lb<P, 24, B>(a, capacity, s);
-    else closest_lb<P, 32, B>(a, capacity, s);
+    if (lds <= 8) closest_lb<P, AH, 8, B>(a, capacity, s);
+    else if (lds <= 16) closest_lb<P, AH, 16, B>(a, capacity, s);
+    else if (lds <= 24) closest_lb<P, AH, 24, B>(a, capacity, s);
+    else closest_lb<P, AH, 32, B>(a, capacity, s);
 }
 
-template <bool P>
+template <bool P, bool AH>
 void dispatch_closest(const TraceArgs& a, uint32_t capacity, uint32_t lds, uint32_t block, hipStream_t s) {
-    if (block == 128) closest_b<P, 128>(a, capacity, lds, s);
-    else if (block == 512) closest_b<P, 512>(a, capacity, lds, s);
-    else closest_b<P, 256>(a, capacity, lds, s);
+    if (block == 128) closest_b<P, AH, 128>(a, capacity, lds, s);
+    else if (block == 512) closest_b<P, AH, 512>(a, capacity, lds, s);
+    else closest_b<P, AH, 256>(a, capacity, lds, s);
 }
 
-template <int B>
+template <bool AH, int B>
 void shadow_b(const ShadowArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
-    if (lds <= 8) shadow_lb<8, B>(a, capacity, s);
-    else if (lds <= 16) shadow_lb<16, B>(a, capacity, s);
-    else if (lds <= 24) shadow_lb<24, B>(a, capacity, s);
-    else shadow_lb<32, B>(a, capacity, s);
+    if (lds <= 8) shadow_lb<AH, 8, B>(a, capacity, s);
+    else if (lds <= 16) shadow_lb<AH, 16, B>(a, capacity, s);
+    else if (lds <= 24) shadow_lb<AH, 24, B>(a, capacity, s);
+    else shadow_lb<AH, 32, B>(a, capacity, s);
+}
+
+template <bool AH>
+void dispatch_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds, uint32_t block, hipStream_t s) {
+    if (block == 128) shadow_b<AH, 128>(a, capacity, lds, s);
+    else if (block == 512) shadow_b<AH, 512>(a, capacity, lds, s);
+    else shadow_b<AH, 256>(a, capacity, lds, s);
 }
 
 }  // namespace
@@ -741,8 +754,13 @@ void shadow_b(const ShadowArgs& a, uint32_t capacity, uint32_t lds, hipStream_t 
 void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, uint32_t block,
                           hipStream_t s) {
     if (capacity == 0) return;
-    if (primary) dispatch_closest<true>(a, capacity, lds_stack, block, s);
-    else dispatch_closest<false>(a, capacity, lds_stack, block, s);
+    if (primary) {
+        if (a.any_hit) dispatch_closest<true, true>(a, capacity, lds_stack, block, s);
+        else dispatch_closest<true, false>(a, capacity, lds_stack, block, s);
+    } else {
+        if (a.any_hit) dispatch_closest<false, true>(a, capacity, lds_stack, block, s);
+        else dispatch_closest<false, false>(a, capacity, lds_stack, block, s);
+    }
 }
 
 void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, float tmax, uint32_t* out,
@@ -753,9 +771,8 @@ void launch_debug_anyhit(const BvhView& bvh, const RayQueue& q, uint32_t n, floa
 
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds, uint32_t block, hipStream_t s) {
     if (capacity == 0) return;
-    if (block == 128) shadow_b<128>(a, capacity, lds, s);
-    else if (block == 512) shadow_b<512>(a, capacity, lds, s);
-    else shadow_b<256>(a, capacity, lds, s);
+    if (a.any_hit) dispatch_shadow<true>(a, capacity, lds, block, s);
+    else dispatch_shadow<false>(a, capacity, lds, block, s);
 }
 
 }  // namespace bipt

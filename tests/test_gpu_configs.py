@@ -72,6 +72,43 @@ def test_batch_c2_graph_replay_equals_solo_and_oracle(oracle_lib):
         t.close()
 
 
+def test_batch_c2_trees_equal_oracle(oracle_lib):
+    """After one batched call of the 64 scenes (their LBVHs built concurrently on four
+    streams, treelet rounds where the call traces enough paths), every scene's BVH4 is the
+    restatement's, node for node and bit for bit."""
+    import torch
+    specs = scenes.batch_c2()
+    print("case: 64-scene batch, eager call, every BVH4 against the oracle's", flush=True)
+    ts, rounds = [], []
+    for sc, cam, kw in specs:
+        paths = cam.width * cam.height * kw["spp"]
+        r = 3 if paths >= max(4 * sc.num_tris, 1 << 20) else 0   # the call's rule for treelet rounds
+        t = bi_pt.PathTracer()
+        t.set_scene(sc)
+        t.set_build_options(treelet_rounds=r)
+        ts.append(t)
+        rounds.append(r)
+    b = bi_pt.Batch()
+    stream = torch.cuda.Stream()
+    jobs = [bi_pt.Job(t, cam, **kw) for (sc, cam, kw), t in zip(specs, ts)]
+    with torch.cuda.stream(stream):
+        b.render(jobs, stream=stream.cuda_stream)
+    stream.synchronize()
+    b.stats()   # raises on any device flag
+    assert sum(1 for r in rounds if r) > 8
+    for k, ((sc, _, _), t, r) in enumerate(zip(specs, ts, rounds)):
+        o = oracle_lib.OracleScene(sc)
+        o.set_treelet_rounds(r)
+        oc, ob = o.bvh4()
+        o.close()
+        gc, gb = t.bvh4()
+        assert np.array_equal(gc, oc) and np.array_equal(gb.view(np.uint32), ob.view(np.uint32)), (k, sc.num_tris, r)
+    print(f"64 trees equal ({sum(1 for r in rounds if r)} restructured)", flush=True)
+    b.close()
+    for t in ts:
+        t.close()
+
+
 def test_serving_c3_first_arrivals(oracle_lib):
     templates = scenes.templates_c3()
     arrivals = scenes.arrivals_c3()[:100]
