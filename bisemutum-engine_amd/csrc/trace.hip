@@ -265,8 +265,10 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
     // nearest (a lane with nothing left is parked until its queued leaves are tested);
     // (2) the leaf queue is flushed when full, before a refill, or when no lane is busy,
     // and parked lanes with no queued leaves write their hit; (3) idle lanes take new
-    // rays (starting at the root) once enough of them are idle; (4) the next node is
-    // loaded, at this single place, so the loop keeps it in the same registers.
+    // rays (starting at the root) once enough of them are idle.  The node a lane visits
+    // is loaded where its boxes are tested: its 28 registers are then free during the
+    // leaf flush, which keeps the kernel at 79 VGPRs (6 wavefronts per SIMD when the LDS
+    // stack is 12 deep or less).
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     uint32_t* qleaf = s_qleaf[w];
     uint8_t* qown = s_qown[w];
@@ -283,18 +285,17 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
     const Bvh4Node* nodes = a.jobs.rec[0].nodes;
     const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
     const JobRec* jr = a.jobs.rec;   // the lane's job (materials for any-hit tests)
-    uint32_t max_steps = 1;
     ShortStack<L, B> st;
     int sp = 0;
-    uint32_t steps = 0;
-    Node4 cur = load_node(nodes, 0);
+    uint32_t steps_left = 0;   // node visits left before the ray is taken to loop
+    int cur_i = 0;             // the node the lane visits next
     for (;;) {
         int next = kDone;
         uint32_t lm = 0;
         int4 ch = make_int4(0, 0, 0, 0);
         float4 tnv = make_float4(0.f, 0.f, 0.f, 0.f);
         if (ray != kNoRay && !parked) {
-            const Node4 nb = cur;
+            const Node4 nb = load_node(nodes, cur_i);
             ch = nb.ch;
             float tn[4];
             bool h[4];
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
             else if (sp > 0) next = st.pop(lds, sp);
             // a ray enters each node at most once: more steps than nodes means corrupt
             // links, and the ray is ended instead of looping
-            if (++steps > max_steps) {
+            if (steps_left-- == 0u) {
                 atomicOr(a.error, 8u);
                 next = kDone;
             }
@@ -396,6 +397,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
         const bool fresh = got != kNoRay;
         V3 no = o, nd = d;
         float tmax = 0.0f;
+        uint32_t setup_steps = 0;
         if (fresh) {
             // one job: its record sits at a uniform address (scalar loads); several: the
             // path's job is looked up per lane
@@ -403,7 +405,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
                 nodes = J.nodes;
                 geo = J.leaf_geo;
                 jr = &J;
-                max_steps = J.num_tris > 1u ? J.num_tris : 1u;
+                setup_steps = J.num_tris > 1u ? J.num_tris : 1u;
                 tmax = J.ray_length;
                 if (PRIMARY) {
                     const PathKey k = path_key(J, local);
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
             }
             next = 0;
         }
-        cur = load_node(nodes, next != kDone ? next : 0);
+        cur_i = next != kDone ? next : 0;
         if (fresh) {
             ray = got;
             o = no;
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
             tbest = tmax;
             best = kLeafMiss;
             sp = 0;
-            steps = 0;
+            steps_left = setup_steps;
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
@@ -550,18 +552,17 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
     const Bvh4Node* nodes = a.jobs.rec[0].nodes;   // the lane's ray's scene (see k_trace_closest)
     const LeafGeo* geo = a.jobs.rec[0].leaf_geo;
     const JobRec* jr = a.jobs.rec;
-    uint32_t max_steps = 1;
     ShortStack<L, B> st;
     int sp = 0;
-    uint32_t steps = 0;
+    uint32_t steps_left = 0;   // node visits left before the ray is taken to loop
     uint32_t sky_rays = 0;   // ray statistics: sky shadow rays this lane finished
-    Node4 cur = load_node(nodes, 0);
+    int cur_i = 0;             // the node the lane visits next
     for (;;) {
         int next = kDone;
         uint32_t lm = 0;
         int4 ch = make_int4(0, 0, 0, 0);
         if (ray != kNoRay && !parked) {
-            const Node4 nb = cur;
+            const Node4 nb = load_node(nodes, cur_i);
             ch = nb.ch;
             bool h[4];
 #pragma unroll
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
             } else if (next == kDone && sp > 0) {
                 next = st.pop(lds, sp);
             }
-            if (++steps > max_steps) {   // corrupt links: end the ray instead of looping
+            if (steps_left-- == 0u) {   // corrupt links: end the ray instead of looping
                 atomicOr(a.error, 8u);
                 next = kDone;
             }
@@ -632,12 +633,13 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
         if (refill_now) got = fetch_ray(a.work, count, ray == kNoRay && !exhausted, &exhausted, wc);
         const bool fresh = got != kNoRay;
         V3 no = o, nd = d;
+        uint32_t setup_steps = 0;
         if (fresh) {
             auto setup = [&](const JobRec& J) {
                 nodes = J.nodes;
                 geo = J.leaf_geo;
                 jr = &J;
-                max_steps = J.num_tris > 1u ? J.num_tris : 1u;
+                setup_steps = J.num_tris > 1u ? J.num_tris : 1u;
             };
             if (a.jobs.num_jobs == 1u) setup(a.jobs.rec[0]);
             else setup(a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, a.q.pk[got] >> 3)]);
@@ -645,14 +647,14 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
             nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
             next = 0;
         }
-        cur = load_node(nodes, next != kDone ? next : 0);
+        cur_i = next != kDone ? next : 0;
         if (fresh) {
             ray = got;
             o = no;
             d = nd;
             inv = safe_inv(nd);
             sp = 0;
-            steps = 0;
+            steps_left = setup_steps;
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
@@ -718,10 +720,11 @@ void shadow_lb(const ShadowArgs& a, uint32_t capacity, hipStream_t s) {
     hipLaunchKernelGGL((k_trace_shadow<AH, L, B>), dim3(groups), dim3(B), 0, s, a);
 }
 
-// Block shapes: 128, 256 (default) or 512 threads; LDS stack depths 8, 16 (default), 24, 32.
+// Block shapes: 128, 256 (default) or 512 threads; LDS stack depths 8, 12, 16 (default), 24, 32.
 template <bool P, bool AH, int B>
 void closest_b(const TraceArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
     if (lds <= 8) closest_lb<P, AH, 8, B>(a, capacity, s);
+    else if (lds <= 12) closest_lb<P, AH, 12, B>(a, capacity, s);
     else if (lds <= 16) closest_lb<P, AH, 16, B>(a, capacity, s);
     else if (lds <= 24) closest_lb<P, AH, 24, B>(a, capacity, s);
     else closest_lb<P, AH, 32, B>(a, capacity, s);
@@ -737,6 +740,7 @@ void dispatch_closest(const TraceArgs& a, uint32_t capacity, uint32_t lds, uint3
 template <bool AH, int B>
 void shadow_b(const ShadowArgs& a, uint32_t capacity, uint32_t lds, hipStream_t s) {
     if (lds <= 8) shadow_lb<AH, 8, B>(a, capacity, s);
+    else if (lds <= 12) shadow_lb<AH, 12, B>(a, capacity, s);
     else if (lds <= 16) shadow_lb<AH, 16, B>(a, capacity, s);
     else if (lds <= 24) shadow_lb<AH, 24, B>(a, capacity, s);
     else shadow_lb<AH, 32, B>(a, capacity, s);
