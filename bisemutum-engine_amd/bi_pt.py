@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
-           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
@@ -73,6 +73,12 @@ class Instance(C.Structure):
 class CameraC(C.Structure):
     _fields_ = [("matrix_inv_view", C.c_float * 16), ("matrix_inv_proj", C.c_float * 16),
                 ("matrix_proj_view", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class GBufferC(C.Structure):
+    """bi_pt_gbuffer: device pointers of the four GBuffer images (nullable)."""
+    _fields_ = [("base_color", C.c_void_p), ("normal_roughness", C.c_void_p), ("fresnel", C.c_void_p),
+                ("material_0", C.c_void_p)]
 
 
 class Settings(C.Structure):
@@ -159,6 +165,7 @@ def lib():
         L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
         L.bi_pt_set_sky_cube.argtypes = [vp, f32p, u32, f32p, f32p]
         L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
+        L.bi_pt_render_gbuffer.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, C.POINTER(GBufferC), vp]
         L.bi_pt_reset_accumulation.argtypes = [vp]
         L.bi_pt_set_build_options.argtypes = [vp, C.POINTER(C.c_uint32 * 4)]
         L.bi_pt_get_stats.argtypes = [vp, C.POINTER(Stats)]
@@ -365,6 +372,34 @@ class PathTracer:
         self._check(lib().bi_pt_render(self.h, C.byref(camera_struct(cam)), C.byref(st),
                                        C.c_void_p(self._out.data_ptr()), dptr, s))
         return self._out
+
+    def render_gbuffer(self, cam, spp=1, max_bounces=5, seed=0, flags=0, ray_length=1000.0, accumulate=False,
+                       parts=("base_color", "normal_roughness", "fresnel", "material_0"), stream=None):
+        """bi_pt_render_gbuffer: render as render() and write the primary-hit GBuffer of the
+        call's first sample.  Returns {part: numpy array [npix, 4]} (uint16 half bits for
+        base_color / normal_roughness, uint16 unorm for fresnel, uint8 for material_0)."""
+        torch = self.torch
+        npix = cam.width * cam.height
+        dev = torch.device("cuda", self.device)
+        dtype = torch.float16 if flags & FLAG_OUTPUT_RGBA16F else torch.float32
+        if self._out is None or self._out.numel() != npix * 4 or self._out.dtype != dtype:
+            self._out = torch.empty(npix * 4, dtype=dtype, device=dev)
+        bufs = {}
+        for name in parts:
+            bufs[name] = torch.full((npix * 4,), 0x5A, dtype=torch.uint8 if name == "material_0" else torch.int16,
+                                    device=dev)
+        g = GBufferC(*[C.c_void_p(bufs[k].data_ptr()) if k in bufs else None
+                       for k in ("base_color", "normal_roughness", "fresnel", "material_0")])
+        st = Settings(ray_length, max_bounces, 1 if accumulate else 0, spp, seed, flags, 0, 0, 0, 0, 0)
+        s = C.c_void_p(stream) if stream else None
+        self._check(lib().bi_pt_render_gbuffer(self.h, C.byref(camera_struct(cam)), C.byref(st),
+                                               C.c_void_p(self._out.data_ptr()), None, C.byref(g), s))
+        torch.cuda.synchronize()
+        out = {}
+        for k, t in bufs.items():
+            a = t.cpu().numpy().reshape(npix, 4)
+            out[k] = a if k == "material_0" else a.view(np.uint16)
+        return out
 
     def image(self):
         self.torch.cuda.synchronize()

@@ -107,7 +107,7 @@ bi_pt_status ensure_build_buffers(bi_pt_ctx* ctx) {
     BI_TRY(dalloc(ctx, ctx->tl_starts, 1 + treelet_start_cap(n)));
     if (ctx->need_uv) BI_TRY(dalloc(ctx, ctx->leaf_uv, n));
     else dfree(ctx, ctx->leaf_uv);
-    if (ctx->need_tan) BI_TRY(dalloc(ctx, ctx->leaf_tan, n));
+    if (scene_wants_tan(ctx)) BI_TRY(dalloc(ctx, ctx->leaf_tan, n));
     else dfree(ctx, ctx->leaf_tan);
     return BI_PT_OK;
 }
@@ -142,9 +142,9 @@ bi_pt_status build_bvh(bi_pt_ctx* ctx, hipStream_t s, uint32_t* error, uint32_t 
     a.leaf_shade = ctx->leaf_shade.p;
     a.materials = ctx->mats.p;
     a.uvs = ctx->need_uv ? ctx->uvs.p : nullptr;
-    a.tangents = ctx->need_tan ? ctx->tans.p : nullptr;
+    a.tangents = scene_wants_tan(ctx) ? ctx->tans.p : nullptr;
     a.leaf_uv = ctx->need_uv ? ctx->leaf_uv.p : nullptr;
-    a.leaf_tan = ctx->need_tan ? ctx->leaf_tan.p : nullptr;
+    a.leaf_tan = scene_wants_tan(ctx) ? ctx->leaf_tan.p : nullptr;
     a.error = error;
     a.treelet_rounds = rounds;
     a.tl_cost = ctx->tl_cost.p;
@@ -347,10 +347,10 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
         if (meshes[m].num_indices)
             std::memcpy(&hi[idx_off[m]], meshes[m].indices, 4 * (size_t)meshes[m].num_indices);
     }
-    // texture coordinates (default 0) and tangents (default (1, 0, 0, 1)) of every vertex,
-    // only when a material samples a texture / a normal map
+    // texture coordinates (default 0) of every vertex when a material samples a texture;
+    // tangents (default (1, 0, 0, 1)) always: normal maps and the GBuffer read them
     std::vector<float> huv(need_uv ? 2 * nv : 0, 0.0f), htan;
-    if (need_tan) {
+    {
         htan.resize(4 * nv);
         for (size_t v = 0; v < nv; ++v) {
             htan[4 * v] = 1.0f; htan[4 * v + 1] = 0.0f; htan[4 * v + 2] = 0.0f; htan[4 * v + 3] = 1.0f;
@@ -364,7 +364,7 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
                     return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite texture coordinate");
             std::memcpy(&huv[2 * (size_t)vtx_off[m]], meshes[m].texcoords, 8 * nvm);
         }
-        if (need_tan && meshes[m].tangents) {
+        if (meshes[m].tangents) {
             for (size_t k = 0; k < 4 * nvm; ++k)
                 if (!std::isfinite(meshes[m].tangents[k]))
                     return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite tangent");
@@ -377,12 +377,8 @@ bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t 
     } else {
         dfree(ctx, ctx->uvs);
     }
-    if (need_tan) {
-        BI_TRY(dalloc(ctx, ctx->tans, std::max<size_t>(htan.size(), 1)));
-        if (!htan.empty()) BI_HIP_TRY(ctx, hipMemcpy(ctx->tans.p, htan.data(), htan.size() * 4, hipMemcpyHostToDevice));
-    } else {
-        dfree(ctx, ctx->tans);
-    }
+    BI_TRY(dalloc(ctx, ctx->tans, std::max<size_t>(htan.size(), 1)));
+    if (!htan.empty()) BI_HIP_TRY(ctx, hipMemcpy(ctx->tans.p, htan.data(), htan.size() * 4, hipMemcpyHostToDevice));
     ctx->need_uv = need_uv;
     ctx->need_tan = need_tan;
     ctx->non_opaque = non_opaque;
@@ -618,6 +614,24 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     opt.knobs = Knobs{st->wave_paths, st->stack_lds, st->refill_lanes, st->work_parts, st->block_threads};
     opt.timing = (st->flags & BI_PT_FLAG_TIMING) != 0;
     opt.record = (st->flags & BI_PT_FLAG_RECORD_QUEUES) != 0;
+    opt.reuse_bvh = (st->flags & BI_PT_FLAG_REUSE_BVH) != 0;
+    return render_jobs(ctx, ctx->eng, &job, 1, reinterpret_cast<hipStream_t>(stream), opt, nullptr);
+}
+
+bi_pt_status bi_pt_render_gbuffer(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_pt_settings* st,
+                                  float* out_rgba, float* out_depth, const bi_pt_gbuffer* gbuffer, bi_pt_stream stream) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (!camera || !st || !out_rgba || !gbuffer)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null camera/settings/output/gbuffer");
+    if (st->flags & BI_PT_FLAG_RECORD_QUEUES)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "queue recording and the GBuffer are not combined");
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    JobIn job{ctx, camera, st, out_rgba, out_depth};
+    const bool any = gbuffer->base_color || gbuffer->normal_roughness || gbuffer->fresnel || gbuffer->material_0;
+    job.gbuf = any ? gbuffer : nullptr;
+    RenderOpts opt;
+    opt.knobs = Knobs{st->wave_paths, st->stack_lds, st->refill_lanes, st->work_parts, st->block_threads};
+    opt.timing = (st->flags & BI_PT_FLAG_TIMING) != 0;
     opt.reuse_bvh = (st->flags & BI_PT_FLAG_REUSE_BVH) != 0;
     return render_jobs(ctx, ctx->eng, &job, 1, reinterpret_cast<hipStream_t>(stream), opt, nullptr);
 }

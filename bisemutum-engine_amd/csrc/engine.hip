@@ -264,6 +264,10 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
             p.spp = 1;
             p.sums = nrc->targets.p;
         }
+        if (J.gbuf && !S->gbuf_tan) {   // the GBuffer reads leaf tangents: build them from now on
+            S->gbuf_tan = true;
+            S->bvh_built = false;
+        }
         p.rebuild = !(opt.reuse_bvh && S->bvh_built);
         for (uint32_t k = 0; k < j; ++k)
             if (prep[k].scene == S) p.rebuild = false;   // one build per scene and call
@@ -409,6 +413,12 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         r.depth_sample = (uint32_t)(p.reset ? 0 : ctx->acc_count);
         r.out_rgba = jobs[j].out_rgba;
         r.out_den = (float)((p.reset ? 0 : ctx->acc_count) + p.spp);
+        if (const bi_pt_gbuffer* g = jobs[j].gbuf) {
+            r.gb_base = g->base_color;
+            r.gb_nr = g->normal_roughness;
+            r.gb_fres = g->fresnel;
+            r.gb_m0 = g->material_0;
+        }
     };
     std::vector<WaveJobs> wj(waves.size());
     for (size_t w = 0; w < waves.size(); ++w) {
@@ -535,7 +545,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
             key.insert(key.end(), {(uint64_t)(uintptr_t)p.ctx, (uint64_t)(uintptr_t)p.scene,
                                    (uint64_t)(uintptr_t)p.scene->nodes4.p, (uint64_t)(uintptr_t)p.ctx->sums.p,
                                    p.npix, p.spp, p.M, p.nl, p.sh, (uint64_t)p.reset, (uint64_t)p.rebuild,
-                                   p.scene->num_tris});
+                                   p.scene->num_tris, (uint64_t)(jobs[j].gbuf != nullptr)});
         }
         if (graph_owner->graph_exec && key == graph_owner->graph_key) {
             BI_HIP_TRY(own, hipGraphLaunch(graph_owner->graph_exec, s));
@@ -618,6 +628,15 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 BI_TRY(mark(0));
                 launch_trace_closest(ta, J.num_paths, primary, lds_stack, block, s);
                 ++launches_closest;
+                if (primary) {   // GBuffers of the jobs whose first sample of the call is in this wave
+                    const size_t k = waves[w].size();
+                    const uint32_t* pbh = reinterpret_cast<const uint32_t*>(host + wave_off[w] + k * sizeof(JobRec));
+                    for (size_t i = 0; i < k; ++i) {
+                        const Item& it = waves[w][i];
+                        if (jobs[it.job].gbuf && it.sample_off == 0)
+                            launch_gbuffer(J.rec + i, hits.rec, pbh[i], prep[it.job].npix, s);
+                    }
+                }
                 BI_TRY(mark(1));
 
                 launch_shade_reset(eng.tile_state.p, primary ? nullptr : cnt_in, J.num_paths, cnt_out, &cnt[kCntShadow],

@@ -605,4 +605,45 @@ __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u
     return true;
 }
 
+// --------------------------------------------------------------------- GBuffer
+// The shaders' float -> uint conversion (saturating, NaN -> 0).
+PT_HD uint32_t cvt_u32_sat(float x) {
+    return x > 0.0f ? (x >= 4294967296.0f ? 0xffffffffu : (uint32_t)x) : 0u;
+}
+// pack_color_rg11b10 (pack.hlsl:14-16)
+PT_HD uint32_t pack_rg11b10(V3 c) {
+    return cvt_u32_sat(c.x * 2047.0f) | (cvt_u32_sat(c.y * 2047.0f) << 11) | (cvt_u32_sat(c.z * 1023.0f) << 22);
+}
+// A 16-bit half of pack_u32_to_unorm16x2 (pack.hlsl:6-8), h / 65536, as an rgba16 unorm
+// store holds it: the exact round(h * 65535 / 65536).
+PT_HD uint32_t unorm16_of_half_word(uint32_t h) { return (h * 65535u + 32768u) >> 16; }
+// An rgba8 unorm store of v: round(clamp(v, 0, 1) * 255) in FP32, ties to even.
+PT_HD uint32_t unorm8_of(float v) {
+    const float c = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
+    return (uint32_t)__builtin_rintf(c * 255.0f);
+}
+// pack_normal_and_tangent (pack.hlsl:89-110): octahedral N (oct_encode) and the tangent's
+// direction in N's frame (frame.hlsl:9-20), L1-normalised, its x mapped to [0, 1] and
+// signed by its y.
+PT_HD void pack_normal_tangent(V3 N, V3 T, float out[3]) {
+    const float s = (__builtin_fabsf(N.x) + __builtin_fabsf(N.y)) + __builtin_fabsf(N.z);
+    float x = N.x / s, y = N.y / s;
+    const float z = N.z / s;
+    if (!(z >= 0.0f)) {
+        const float wx = (1.0f - __builtin_fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        const float wy = (1.0f - __builtin_fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = wx;
+        y = wy;
+    }
+    const Frame f = make_frame(N);
+    float px = dot(T, f.t), py = dot(T, f.b);
+    const float l = __builtin_fabsf(px) + __builtin_fabsf(py);
+    px = px / l;
+    py = py / l;
+    const float pxx = px * 0.5f + 0.5f;
+    out[0] = x;
+    out[1] = y;
+    out[2] = py < 0.0f ? -pxx : pxx;
+}
+
 }  // namespace bipt

@@ -157,6 +157,7 @@ struct bi_pt_ctx : bipt::Owner {
     // textures / normal maps) and the material textures
     bool need_uv = false, need_tan = false;
     bool non_opaque = false;        // some material is alpha-tested or translucent (any-hit tests)
+    bool gbuf_tan = false;          // a GBuffer was asked for: leaf tangents are built from then on
     bipt::DevBuf<float> uvs, tans;
     bipt::DevBuf<bipt::TexRec> tex_recs;
     bipt::DevBuf<uint32_t> tex_pool;
@@ -228,7 +229,9 @@ BvhView bvh_view(const bi_pt_ctx* ctx);
 // Texture inputs of a scene context's kernels.
 inline SceneTex scene_tex(const bi_pt_ctx* c) { return SceneTex{c->tex_recs.p, c->tex_pool.p, c->srgb.p}; }
 inline const LeafUv* scene_leaf_uv(const bi_pt_ctx* c) { return c->need_uv ? c->leaf_uv.p : nullptr; }
-inline const LeafTan* scene_leaf_tan(const bi_pt_ctx* c) { return c->need_tan ? c->leaf_tan.p : nullptr; }
+// Leaf tangents are built for normal maps and, once one is asked for, for the GBuffer.
+inline bool scene_wants_tan(const bi_pt_ctx* c) { return c->need_tan || c->gbuf_tan; }
+inline const LeafTan* scene_leaf_tan(const bi_pt_ctx* c) { return scene_wants_tan(c) ? c->leaf_tan.p : nullptr; }
 
 // One job of a call.  Roles: a plain image; an image whose paths end in the call's
 // radiance cache; the cache's training paths (one sample per slot into the cache's
@@ -241,6 +244,7 @@ struct JobIn {
     float* out_rgba;
     float* out_depth;
     uint32_t role = kRoleImage;
+    const bi_pt_gbuffer* gbuf = nullptr;   // primary-hit GBuffer of the call's first sample
 };
 
 struct RenderOpts {

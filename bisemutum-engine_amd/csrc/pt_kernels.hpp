@@ -169,6 +169,9 @@ struct ResolveArgs {
 void launch_resolve(const ResolveArgs& a, hipStream_t s);
 // Running mean of every job of the wave -> its RGBA32F output.
 void launch_output(const WaveJobs& jobs, uint32_t num_pix, hipStream_t s);
+// Primary-hit GBuffer of the job record `job` (device) from the segment-0 hit records of its
+// paths [path0, path0 + npix): its first sample in the wave, which must be the call's first.
+void launch_gbuffer(const JobRec* job, const HitRec* hits, uint32_t path0, uint32_t npix, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
 
 }  // namespace bipt

@@ -88,6 +88,7 @@ struct MaterialRec {   // = bi_pt_material
 constexpr uint32_t kPatternCheckerXZ = 1u;
 constexpr uint32_t kBlendOpaque = 0u, kBlendAlphaTest = 1u, kBlendTranslucent = 2u;
 constexpr uint32_t kNoTexture = 0u;   // texture references are index + 1, so a zeroed record has none
+constexpr uint32_t kSurfaceModelLit = 1u;   // SurfaceModel::lit (include/bisemutum/graphics/material.hpp:13-16)
 constexpr float kAlphaTestThreshold = 0.01f;   // ALPHA_TEST_OPACITY_THRESHOLD, material/utils.hlsl:38
 static_assert(sizeof(MaterialRec) == 128, "material record is 128 bytes");
 
@@ -240,6 +241,12 @@ struct JobRec {
     uint32_t depth_sample;  // the sample whose primary hit gives the depth
     float* out_rgba;        // output mean (RGBA32F, or RGBA16F with kJobHalfOut)
     float out_den;          // samples accumulated after this call
+    // primary-hit GBuffer of sample depth_sample (each nullable): rgba16f base colour,
+    // rgba16f normal / tangent / roughness, rgba16 unorm fresnel, rgba8 unorm material_0
+    uint16_t* gb_base;
+    uint16_t* gb_nr;
+    uint16_t* gb_fres;
+    uint8_t* gb_m0;
 };
 
 // Ray queue in SoA form.

@@ -80,6 +80,29 @@ __device__ __forceinline__ void apply_textures(const JobRec& J, const MaterialRe
     }
 }
 
+// The surface a ray (direction d) sees at a hit of leaf `leaf` at (hu, hv), point P: the
+// interpolated normal, the material (a checkerboard's cell by P), textures and normal
+// map, then the two-sided flip against d.
+__device__ __forceinline__ void hit_surface(const JobRec& J, uint32_t leaf, float hu, float hv, V3 P, V3 d, Mat& m,
+                                            V3& N, float& rough, uint32_t& mat_out) {
+    const float4* lsp = reinterpret_cast<const float4*>(J.leaf_shade + leaf);
+    const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
+    V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
+    uint32_t mat_id = __float_as_uint(s2.y);
+    N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
+    // the triangle's record is loaded at once (its pattern word with it); a checkerboard's
+    // even cells then reload from the pattern material, so plain materials wait for one load
+    m = load_mat(J.materials[mat_id]);
+    if (const uint32_t cell_id = resolve_material(J.materials, mat_id, P); cell_id != mat_id) {
+        mat_id = cell_id;
+        m = load_mat(J.materials[mat_id]);
+    }
+    rough = J.materials[mat_id].roughness;
+    if (mat_textured(J.materials[mat_id])) apply_textures(J, J.materials[mat_id], leaf, hu, hv, m, N, rough);
+    if (m.two_sided && dot(d, N) > 0.0f) N = -N;
+    mat_out = mat_id;
+}
+
 // Shades queue entry i (path `path`, the job's local path `local`) of job J: its rays
 // go to `out` (mask: which of them exist).
 template <uint32_t NL>
@@ -133,21 +156,11 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
     } else {
         const float t = hit.t, hu = hit.u, hv = hit.v;
         P = o + d * t;
-        const float4* lsp = reinterpret_cast<const float4*>(J.leaf_shade + leaf);
-        const float4 s0 = lsp[0], s1 = lsp[1], s2 = lsp[2];
-        V3 n0 = v3(s0.x, s0.y, s0.z), n1 = v3(s0.w, s1.x, s1.y), n2 = v3(s1.z, s1.w, s2.x);
-        uint32_t mat_id = __float_as_uint(s2.y);
-        V3 N = normalize((n0 + (n1 - n0) * hu) + (n2 - n0) * hv);
-        // the triangle's record is loaded at once (its pattern word with it); a checkerboard's
-        // even cells then reload from the pattern material, so plain materials wait for one load
-        Mat m = load_mat(J.materials[mat_id]);
-        if (const uint32_t cell_id = resolve_material(J.materials, mat_id, P); cell_id != mat_id) {
-            mat_id = cell_id;
-            m = load_mat(J.materials[mat_id]);
-        }
-        float rough = J.materials[mat_id].roughness;
-        if (mat_textured(J.materials[mat_id])) apply_textures(J, J.materials[mat_id], leaf, hu, hv, m, N, rough);
-        if (m.two_sided && dot(d, N) > 0.0f) N = -N;
+        Mat m;
+        V3 N;
+        float rough;
+        uint32_t mat_id;
+        hit_surface(J, leaf, hu, hv, P, d, m, N, rough, mat_id);
         V3 V = normalize(o - P);
         if (a.bounce == J.cache_bounce && (J.flags & (kJobCache | kJobTrain))) {
             NrcRecord r;
@@ -462,6 +475,65 @@ __global__ __launch_bounds__(256) void k_output(WaveJobs jobs, uint32_t num_pix)
     }
 }
 
+// Primary-hit GBuffer (hits/rt_gbuffer_hit.hlsl:7-17, gbuffer.hlsl:18-34) of job J's
+// first sample in this call: thread q reads the segment-0 hit record of the job's path q
+// (the sample's pixel q in path order), rebuilds the camera ray and the surface as the
+// shading does, and stores the four packed texels (zeros on a miss).
+__global__ __launch_bounds__(256) void k_gbuffer(const JobRec* jrec, const HitRec* hits, uint32_t path0) {
+    const JobRec& J = *jrec;
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= J.npix) return;
+    const PathKey key = path_key(J, q);
+    const uint32_t pixel = key.pixel;
+    float jx = 0.5f, jy = 0.5f;
+    if (J.flags & 1u) {
+        Pcg jr = pcg_keyed(J.seed, pixel, key.sample, kJitterBounce);
+        jx = pcg_float(jr);
+        jy = pcg_float(jr);
+    }
+    V3 o, d;
+    camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, jx, jy, &o, &d);
+    const HitRec hit = hits[path0 + q];
+    ushort4 base = make_ushort4(0, 0, 0, 0), nr = make_ushort4(0, 0, 0, 0), fr = make_ushort4(0, 0, 0, 0);
+    uchar4 m0 = make_uchar4(0, 0, 0, 0);
+    if (hit.leaf != kLeafMiss) {
+        const V3 P = o + d * hit.t;
+        Mat m;
+        V3 N;
+        float rough;
+        uint32_t mat_id;
+        hit_surface(J, hit.leaf, hit.u, hit.v, P, d, m, N, rough, mat_id);
+        V3 T = v3(1.0f, 0.0f, 0.0f);
+        if (J.leaf_tan) {
+            const float* g = J.leaf_tan[hit.leaf].t;
+            T = normalize(v3((g[0] + (g[4] - g[0]) * hit.u) + (g[8] - g[0]) * hit.v,
+                             (g[1] + (g[5] - g[1]) * hit.u) + (g[9] - g[1]) * hit.v,
+                             (g[2] + (g[6] - g[2]) * hit.u) + (g[10] - g[2]) * hit.v));
+        }
+        float pk[3];
+        pack_normal_tangent(N, T, pk);
+        auto h = [](float x) {
+            const _Float16 v = (_Float16)x;   // round to nearest even (v_cvt_f16_f32)
+            unsigned short u;
+            __builtin_memcpy(&u, &v, sizeof u);
+            return u;
+        };
+        base = make_ushort4(h(m.base.x), h(m.base.y), h(m.base.z), h(1.0f));
+        nr = make_ushort4(h(pk[0]), h(pk[1]), h(pk[2]), h(rough));
+        const uint32_t f0 = pack_rg11b10(m.f0), f90 = pack_rg11b10(m.f90);
+        fr = make_ushort4((unsigned short)unorm16_of_half_word(f0 & 0xffffu), (unsigned short)unorm16_of_half_word(f0 >> 16),
+                          (unsigned short)unorm16_of_half_word(f90 & 0xffffu),
+                          (unsigned short)unorm16_of_half_word(f90 >> 16));
+        const MaterialRec& M = J.materials[mat_id];
+        m0 = make_uchar4((unsigned char)unorm8_of(M.anisotropy), (unsigned char)unorm8_of(1.0f / M.ior), 0,
+                         (unsigned char)unorm8_of(kSurfaceModelLit / 256.0f));
+    }
+    if (J.gb_base) reinterpret_cast<ushort4*>(J.gb_base)[pixel] = base;
+    if (J.gb_nr) reinterpret_cast<ushort4*>(J.gb_nr)[pixel] = nr;
+    if (J.gb_fres) reinterpret_cast<ushort4*>(J.gb_fres)[pixel] = fr;
+    if (J.gb_m0) reinterpret_cast<uchar4*>(J.gb_m0)[pixel] = m0;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) p[i] = v;
@@ -492,6 +564,11 @@ void launch_resolve(const ResolveArgs& a, hipStream_t s) {
 void launch_output(const WaveJobs& jobs, uint32_t num_pix, hipStream_t s) {
     if (num_pix == 0) return;
     hipLaunchKernelGGL(k_output, dim3((num_pix + 255u) / 256u), dim3(256), 0, s, jobs, num_pix);
+}
+
+void launch_gbuffer(const JobRec* job, const HitRec* hits, uint32_t path0, uint32_t npix, hipStream_t s) {
+    if (npix == 0) return;
+    hipLaunchKernelGGL(k_gbuffer, dim3((npix + 255u) / 256u), dim3(256), 0, s, job, hits, path0);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {

@@ -283,6 +283,34 @@ bi_pt_status bi_pt_set_sky_cube(bi_pt_ctx* ctx, const float* rgba32f_faces, uint
  * (unless statistics or queue recording require a synchronisation). */
 bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_pt_settings* settings,
                           float* out_rgba32f, float* out_depth, bi_pt_stream stream);
+
+/* Primary-hit GBuffer of a render (OutputData.gbuffer, path_tracing.cpp:264-265,353-358,
+ * 482-487): for the first sample the call traces per pixel (the sample whose depth
+ * out_depth holds), the surface its camera ray hits, packed as pack_surface_to_gbuffer
+ * (shaders/renderer/gbuffer.hlsl:18-34) packs it after the closest-hit shader's normal
+ * map and two-sided flip (hits/rt_gbuffer_hit.hlsl:7-17), in the formats of
+ * GBufferTextures (src/renderer/context/gbuffer.hpp:14-17).  Device images of
+ * width*height texels, row-major; each pointer nullable.  A pixel whose ray misses holds
+ * zeros.
+ *   base_color        rgba16f (4 half words): base colour, 1
+ *   normal_roughness  rgba16f: octahedral normal (x, y), the tangent's direction in the
+ *                     normal's frame (frame.hlsl:9-20) as pack_normal_and_tangent's third
+ *                     component (pack.hlsl:102-110), roughness
+ *   fresnel           rgba16 unorm (4 uint16): f0 then f90, each an rg11b10 word split into
+ *                     two unorm16 halves (pack.hlsl:6-16; the word's halves over 65536,
+ *                     stored as the exact unorm16 value)
+ *   material_0        rgba8 unorm (4 bytes): anisotropy, 1/ior, 0, surface model (lit) / 256
+ * Tangents are the meshes' (default (1, 0, 0, 1) in object space), so the first call that
+ * asks for a GBuffer builds the scene's per-leaf tangents and keeps building them. */
+typedef struct bi_pt_gbuffer {
+    uint16_t* base_color;
+    uint16_t* normal_roughness;
+    uint16_t* fresnel;
+    uint8_t* material_0;
+} bi_pt_gbuffer;
+bi_pt_status bi_pt_render_gbuffer(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_pt_settings* settings,
+                                  float* out_rgba32f, float* out_depth, const bi_pt_gbuffer* gbuffer,
+                                  bi_pt_stream stream);
 bi_pt_status bi_pt_reset_accumulation(bi_pt_ctx* ctx);
 /* How the per-frame BVH is built (no reference counterpart: the reference's BVH is the
  * driver's).  After the LBVH (Morton codes, radix sort, Karras hierarchy, refit), rounds

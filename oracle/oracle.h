@@ -147,6 +147,13 @@ int ora_render32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
 /* ora_render32 restricted to a list of pixels (image indices, row-major): sum_rgb[3*i..]
  * is pixel pixels[i]'s running sum, accumulated over samples [sample_base,
  * sample_base+spp) in sample order, exactly as ora_render32 sums that pixel. */
+/* Primary-hit GBuffer of sample `sample` of the listed pixels (bi_pt_render_gbuffer), by
+ * list position: base_color, normal_roughness 4 half words each, fresnel 4 uint16 (unorm),
+ * material_0 4 bytes (unorm); each nullable; zeros where the camera ray misses. */
+int ora_gbuffer32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample, uint32_t n,
+                  const uint32_t* pixels, uint16_t* base, uint16_t* nr, uint16_t* fres, uint8_t* m0);
+/* float -> IEEE half bits, round to nearest even. */
+uint16_t ora_half32(float f);
 int ora_render32_pixels(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample_base,
                         uint32_t spp, uint32_t n, const uint32_t* pixels, float* sum_rgb, int num_threads);
 

@@ -105,6 +105,11 @@ def lib():
         L.ora_vndf_pdf64.restype = C.c_double
         L.ora_vndf_pdf64.argtypes = [_f64p, _f64p, C.c_double]
         L.ora_sky_eval32.argtypes = [C.c_void_p, _f32p, _f32p]
+        L.ora_gbuffer32.restype = C.c_int
+        L.ora_gbuffer32.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(Settings), C.c_uint32, C.c_uint32, _u32p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ora_half32.restype = C.c_uint16
+        L.ora_half32.argtypes = [C.c_float]
         L.ora_sky_sample32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p]
         L.ora_sky_pdf32.restype = C.c_float
         L.ora_sky_pdf32.argtypes = [C.c_void_p, _f32p]
@@ -334,6 +339,21 @@ class OracleScene:
         lib().ora_tex_sample32(self.h, tex, len(uv), ptr(uv, _f32p), ptr(out, _f32p))
         return out
 
+    def gbuffer32(self, cam, pixels, sample=0, seed=0, flags=0, ray_length=1000.0, max_bounces=5):
+        """Primary-hit GBuffer of `sample` at the listed pixels: dict of [n, 4] arrays (uint16
+        half bits base_color / normal_roughness, uint16 fresnel, uint8 material_0)."""
+        pixels = np.ascontiguousarray(pixels, np.uint32)
+        n = len(pixels)
+        out = dict(base_color=np.zeros((n, 4), np.uint16), normal_roughness=np.zeros((n, 4), np.uint16),
+                   fresnel=np.zeros((n, 4), np.uint16), material_0=np.zeros((n, 4), np.uint8))
+        rc = lib().ora_gbuffer32(self.h, C.byref(camera_struct(cam)),
+                                 C.byref(settings_struct(max_bounces, seed, ray_length, flags)), sample, n,
+                                 ptr(pixels, _u32p), out["base_color"].ctypes.data, out["normal_roughness"].ctypes.data,
+                                 out["fresnel"].ctypes.data, out["material_0"].ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"ora_gbuffer32 failed ({rc})")
+        return out
+
     def sky_eval(self, d):
         d = np.ascontiguousarray(d, np.float32)
         out = np.zeros(3, np.float32)
@@ -420,6 +440,11 @@ def vndf_pdf64_batch(v_local, ls, alpha):
     out = np.zeros(len(ls), np.float64)
     lib().ora_vndf_pdf64_batch(ptr(v, _f64p), len(ls), ptr(ls, _f64p), alpha, ptr(out, _f64p))
     return out
+
+
+def half32(x):
+    """float -> IEEE half bits (round to nearest even), the restatement's conversion."""
+    return int(lib().ora_half32(float(np.float32(x))))
 
 
 def sky_batch32(scene, dirs):

@@ -1161,6 +1161,52 @@ struct CacheMode {
     uint32_t suffix_key = 0;
 };
 
+// The surface a ray of direction d sees at hit h of triangle tri, point P
+// (hits/rt_gbuffer_hit.hlsl:6-17): interpolated normal, the material (a checkerboard's
+// cell by P), texture-driven fields, the normal map, and the two-sided flip.
+void hit_surface(const Scene& s, uint32_t tri, const Hit& h, F3 P, F3 d, F3* N_out, Mat* m_out, float* rough_out,
+                 uint32_t* mid_out) {
+    const float* tn = &s.tri_n[9 * tri];
+    F3 n0 = f3(tn[0], tn[1], tn[2]), n1 = f3(tn[3], tn[4], tn[5]), n2 = f3(tn[6], tn[7], tn[8]);
+    F3 N = normalize((n0 + (n1 - n0) * h.u) + (n2 - n0) * h.v);
+    const uint32_t mid = resolve_mat(s.materials, s.tri_mat[tri], P);
+    Mat m = load_mat(&s.materials[kMatFloats * mid]);
+    float rough = s.materials[kMatFloats * mid + 3];
+    {
+        // texture-driven fields (material_function samples at vertex.texcoord) and the
+        // normal map in the interpolated tangent frame (hits/rt_gbuffer_hit.hlsl:8-11,
+        // hit.hlsl:155-157)
+        const uint32_t tb = mat_word(s, mid, 23), tf = mat_word(s, mid, 24), tn = mat_word(s, mid, 25),
+                       tr = mat_word(s, mid, 26);
+        if (tb | tf | tn | tr) {
+            float su, sv;
+            tri_texcoord(s, tri, h.u, h.v, &su, &sv);
+            if (tb) { F4 c = tex_sample(s, tb - 1u, su, sv); m.base = f3(c.x, c.y, c.z); }
+            if (tf) { F4 c = tex_sample(s, tf - 1u, su, sv); m.f0 = f3(c.x, c.y, c.z); }
+            if (tr) {
+                rough = comp4(tex_sample(s, tr - 1u, su, sv), mat_word(s, mid, 27));
+                mat_alpha(m, rough, s.materials[kMatFloats * mid + 7]);
+            }
+            if (tn && s.need_tan) {
+                F4 c = tex_sample(s, tn - 1u, su, sv);
+                F3 nt = f3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+                const float* g = &s.tri_tan[12 * (size_t)tri];
+                F3 T = normalize(f3((g[0] + (g[4] - g[0]) * h.u) + (g[8] - g[0]) * h.v,
+                                    (g[1] + (g[5] - g[1]) * h.u) + (g[9] - g[1]) * h.v,
+                                    (g[2] + (g[6] - g[2]) * h.u) + (g[10] - g[2]) * h.v));
+                float tw = (g[3] + (g[7] - g[3]) * h.u) + (g[11] - g[3]) * h.v;
+                F3 B = normalize(cross(N, T)) * tw;
+                N = normalize((T * nt.x + B * nt.y) + N * nt.z);
+            }
+        }
+    }
+    if (m.two_sided && dot(d, N) > 0.0f) N = -N;
+    *N_out = N;
+    *m_out = m;
+    *rough_out = rough;
+    *mid_out = mid;
+}
+
 PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st,
                    uint32_t pixel, uint32_t sample, const CacheMode* cm = nullptr) {
     PathOut out{};
@@ -1197,41 +1243,11 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         }
         uint32_t tri = s.leaf_tri[h.leaf];
         F3 P = ray.o + ray.d * h.t;
-        const float* tn = &s.tri_n[9 * tri];
-        F3 n0 = f3(tn[0], tn[1], tn[2]), n1 = f3(tn[3], tn[4], tn[5]), n2 = f3(tn[6], tn[7], tn[8]);
-        F3 N = normalize((n0 + (n1 - n0) * h.u) + (n2 - n0) * h.v);
-        const uint32_t mid = resolve_mat(s.materials, s.tri_mat[tri], P);
-        Mat m = load_mat(&s.materials[kMatFloats * mid]);
-        float rough = s.materials[kMatFloats * mid + 3];
-        {
-            // texture-driven fields (material_function samples at vertex.texcoord) and the
-            // normal map in the interpolated tangent frame (hits/rt_gbuffer_hit.hlsl:8-11,
-            // hit.hlsl:155-157)
-            const uint32_t tb = mat_word(s, mid, 23), tf = mat_word(s, mid, 24), tn = mat_word(s, mid, 25),
-                           tr = mat_word(s, mid, 26);
-            if (tb | tf | tn | tr) {
-                float su, sv;
-                tri_texcoord(s, tri, h.u, h.v, &su, &sv);
-                if (tb) { F4 c = tex_sample(s, tb - 1u, su, sv); m.base = f3(c.x, c.y, c.z); }
-                if (tf) { F4 c = tex_sample(s, tf - 1u, su, sv); m.f0 = f3(c.x, c.y, c.z); }
-                if (tr) {
-                    rough = comp4(tex_sample(s, tr - 1u, su, sv), mat_word(s, mid, 27));
-                    mat_alpha(m, rough, s.materials[kMatFloats * mid + 7]);
-                }
-                if (tn && s.need_tan) {
-                    F4 c = tex_sample(s, tn - 1u, su, sv);
-                    F3 nt = f3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
-                    const float* g = &s.tri_tan[12 * (size_t)tri];
-                    F3 T = normalize(f3((g[0] + (g[4] - g[0]) * h.u) + (g[8] - g[0]) * h.v,
-                                        (g[1] + (g[5] - g[1]) * h.u) + (g[9] - g[1]) * h.v,
-                                        (g[2] + (g[6] - g[2]) * h.u) + (g[10] - g[2]) * h.v));
-                    float tw = (g[3] + (g[7] - g[3]) * h.u) + (g[11] - g[3]) * h.v;
-                    F3 B = normalize(cross(N, T)) * tw;
-                    N = normalize((T * nt.x + B * nt.y) + N * nt.z);
-                }
-            }
-        }
-        if (m.two_sided && dot(ray.d, N) > 0.0f) N = -N;
+        F3 N;
+        Mat m;
+        float rough;
+        uint32_t mid;
+        hit_surface(s, tri, h, P, ray.d, &N, &m, &rough, &mid);
         F3 V = normalize(ray.o - P);
         if (cm && (int)b == cm->bounce) {
             if (!cm->train) {
@@ -1337,6 +1353,64 @@ uint32_t pixel_of(uint32_t q, uint32_t W, uint32_t H) {
         return ((ty << 3) + (within >> 3)) * W + (tx << 3) + (within & 7u);
     }
     return q;
+}
+
+// float -> IEEE half, round to nearest even (the conversion of an rgba16f store); NaN
+// keeps a quiet payload, overflow goes to infinity, small values to half subnormals.
+uint16_t half_rne(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u | ((ax >> 13) & 0x3ffu) : 0u));
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);   // |f| >= 65520 rounds past 65504
+    const uint32_t e = ax >> 23;
+    if (e >= 113u) {   // normal half: 2^-14 <= |f|
+        uint32_t r = ax >> 13;                               // exponent and top 10 mantissa bits
+        const uint32_t rem = ax & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+        return (uint16_t)(sign | (r - (112u << 10)));
+    }
+    // half subnormal: units of 2^-24; |f| = M * 2^(e - 150) with M the significand
+    const uint32_t M = e ? ((ax & 0x7fffffu) | 0x800000u) : (ax & 0x7fffffu);
+    const uint32_t sh = (e ? 126u - e : 125u);
+    if (sh > 31u) return (uint16_t)sign;
+    uint32_t r = M >> sh;
+    const uint32_t rem = M & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+    if (rem > half || (rem == half && (r & 1u))) ++r;
+    return (uint16_t)(sign | r);
+}
+uint32_t cvt_u32_sat(float x) { return x > 0.0f ? (x >= 4294967296.0f ? 0xffffffffu : (uint32_t)x) : 0u; }
+// pack_color_rg11b10 (pack.hlsl:14-16)
+uint32_t pack_rg11b10(F3 c) {
+    return cvt_u32_sat(c.x * 2047.0f) | (cvt_u32_sat(c.y * 2047.0f) << 11) | (cvt_u32_sat(c.z * 1023.0f) << 22);
+}
+// pack_u32_to_unorm16x2's halves h / 65536 (pack.hlsl:6-8) as an rgba16 unorm texel holds
+// them: round(h * 65535 / 65536), exactly.
+uint16_t unorm16_half_word(uint32_t h) { return (uint16_t)((h * 65535u + 32768u) >> 16); }
+uint8_t unorm8(float v) {
+    const float c = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
+    return (uint8_t)std::nearbyint(c * 255.0f);   // round to nearest even (default mode)
+}
+// pack_normal_and_tangent (pack.hlsl:89-110)
+void pack_normal_tangent(F3 N, F3 T, float out[3]) {
+    const float s = (std::fabs(N.x) + std::fabs(N.y)) + std::fabs(N.z);
+    float x = N.x / s, y = N.y / s;
+    const float z = N.z / s;
+    if (!(z >= 0.0f)) {   // oct_wrap
+        const float wx = (1.0f - std::fabs(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        const float wy = (1.0f - std::fabs(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = wx;
+        y = wy;
+    }
+    const Frame f = make_frame(N);
+    float px = dot(T, f.t), py = dot(T, f.b);
+    const float l = std::fabs(px) + std::fabs(py);
+    px = px / l;
+    py = py / l;
+    const float pxx = px * 0.5f + 0.5f;
+    out[0] = x;
+    out[1] = y;
+    out[2] = py < 0.0f ? -pxx : pxx;
 }
 
 }  // namespace ora
@@ -1456,6 +1530,63 @@ int render32_impl(const ora_ctx* ctx, const ora_camera* cam, const ora_settings*
     return 0;
 }
 }  // namespace
+
+/* Primary-hit GBuffer of sample `sample` of the listed pixels (bi_pt_render_gbuffer),
+ * indexed by list position: base_color and normal_roughness 4 half words each, fresnel 4
+ * uint16, material_0 4 bytes (each nullable); zeros where the camera ray misses. */
+int ora_gbuffer32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample, uint32_t n,
+                  const uint32_t* pixels, uint16_t* base, uint16_t* nr, uint16_t* fres, uint8_t* m0) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t pixel = pixels[i];
+        float jx = 0.5f, jy = 0.5f;
+        if (st->flags & 1u) {
+            Pcg jr = pcg_keyed(st->seed, pixel, sample, kJitterBounce);
+            jx = pcg_float(jr);
+            jy = pcg_float(jr);
+        }
+        Ray ray;
+        camera_ray(*cam, pixel % cam->width, pixel / cam->width, jx, jy, &ray.o, &ray.d);
+        ray.tmin = kTMin;
+        ray.tmax = st->ray_length;
+        ray.seed = st->seed;
+        uint16_t b4[4] = {0, 0, 0, 0}, n4[4] = {0, 0, 0, 0}, f4[4] = {0, 0, 0, 0};
+        uint8_t m4[4] = {0, 0, 0, 0};
+        const Hit h = trace_closest(s, ray);
+        if (h.leaf != kMiss) {
+            const uint32_t tri = s.leaf_tri[h.leaf];
+            const F3 P = ray.o + ray.d * h.t;
+            F3 N;
+            Mat m;
+            float rough;
+            uint32_t mid;
+            hit_surface(s, tri, h, P, ray.d, &N, &m, &rough, &mid);
+            const float* g = &s.tri_tan[12 * (size_t)tri];
+            const F3 T = normalize(f3((g[0] + (g[4] - g[0]) * h.u) + (g[8] - g[0]) * h.v,
+                                      (g[1] + (g[5] - g[1]) * h.u) + (g[9] - g[1]) * h.v,
+                                      (g[2] + (g[6] - g[2]) * h.u) + (g[10] - g[2]) * h.v));
+            float pk[3];
+            pack_normal_tangent(N, T, pk);
+            b4[0] = half_rne(m.base.x); b4[1] = half_rne(m.base.y); b4[2] = half_rne(m.base.z);
+            b4[3] = half_rne(1.0f);
+            n4[0] = half_rne(pk[0]); n4[1] = half_rne(pk[1]); n4[2] = half_rne(pk[2]); n4[3] = half_rne(rough);
+            const uint32_t p0 = pack_rg11b10(m.f0), p1 = pack_rg11b10(m.f90);
+            f4[0] = unorm16_half_word(p0 & 0xffffu); f4[1] = unorm16_half_word(p0 >> 16);
+            f4[2] = unorm16_half_word(p1 & 0xffffu); f4[3] = unorm16_half_word(p1 >> 16);
+            const float* M = &s.materials[kMatFloats * mid];
+            m4[0] = unorm8(M[7]);            // anisotropy
+            m4[1] = unorm8(1.0f / M[11]);    // 1 / ior
+            m4[3] = unorm8(1.0f / 256.0f);   // SurfaceModel::lit / 256
+        }
+        if (base) std::memcpy(&base[4 * (size_t)i], b4, 8);
+        if (nr) std::memcpy(&nr[4 * (size_t)i], n4, 8);
+        if (fres) std::memcpy(&fres[4 * (size_t)i], f4, 8);
+        if (m0) std::memcpy(&m0[4 * (size_t)i], m4, 4);
+    }
+    return 0;
+}
+
+uint16_t ora_half32(float f) { return half_rne(f); }
 
 int ora_render32_pixels(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample_base,
                         uint32_t spp, uint32_t n, const uint32_t* pixels, float* sum_rgb, int num_threads) {

@@ -24,7 +24,7 @@ void flatten(const ora_scene_desc& d, Scene& s) {
     s.tri_n.resize((size_t)9 * total);
     s.tri_mat.resize(total);
     if (s.need_uv) s.tri_uv.assign((size_t)6 * total, 0.0f);
-    if (s.need_tan) s.tri_tan.assign((size_t)12 * total, 0.0f);
+    s.tri_tan.assign((size_t)12 * total, 0.0f);   // normal maps and the GBuffer read them
     uint32_t t = 0;
     for (uint32_t i = 0; i < d.num_instances; ++i) {
         const float* m = &d.inst_xform[12 * i];
@@ -47,7 +47,7 @@ void flatten(const ora_scene_desc& d, Scene& s) {
                     s.tri_uv[6 * (size_t)t + 2 * c] = d.texcoords[2 * gv];
                     s.tri_uv[6 * (size_t)t + 2 * c + 1] = d.texcoords[2 * gv + 1];
                 }
-                if (s.need_tan) {
+                {
                     // ObjectToWorld3x4 * (tangent.xyz, 0) (hit.hlsl:156); w = the bitangent sign
                     const float tx = d.tangents ? d.tangents[4 * gv] : 1.0f;
                     const float ty = d.tangents ? d.tangents[4 * gv + 1] : 0.0f;
