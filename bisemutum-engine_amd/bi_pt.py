@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
-           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
@@ -59,6 +59,65 @@ class Mesh(C.Structure):
     _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
                 ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32), ("num_indices", C.c_uint32),
                 ("texcoords", C.POINTER(C.c_float)), ("tangents", C.POINTER(C.c_float))]
+
+
+VA_POSITION, VA_NORMAL, VA_TANGENT, VA_TEXCOORD = 0x1, 0x4, 0x8, 0x20
+
+
+class GpuSceneC(C.Structure):
+    """bi_pt_gpu_scene: the flat vertex/index buffers (host) and their element counts."""
+    _fields_ = [("positions", C.c_void_p), ("num_positions", C.c_uint64), ("normals", C.c_void_p),
+                ("num_normals", C.c_uint64), ("tangents", C.c_void_p), ("num_tangents", C.c_uint64),
+                ("texcoords", C.c_void_p), ("num_texcoords", C.c_uint64), ("indices", C.c_void_p),
+                ("num_indices", C.c_uint64)]
+
+
+class DrawableC(C.Structure):
+    """bi_pt_drawable: DrawableSbtData + index count, attributes, material, glm matrix."""
+    _fields_ = [(k, C.c_uint32) for k in ("drawable_index", "position_offset", "normal_offset", "tangent_offset",
+                                          "color_offset", "texcoord_offset", "texcoord2_offset", "index_offset",
+                                          "material_offset", "index_count", "attributes", "material")] + \
+               [("matrix", C.c_float * 16)]
+
+
+class SubmeshRangeC(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("position_offset", "normal_offset", "tangent_offset", "texcoord_offset",
+                                          "index_offset")] + [("index_count", C.c_uint32), ("num_vertices", C.c_uint32)]
+
+
+class GpuScene:
+    """The engine's GPU-scene view of a scene: flat buffers (numpy, kept alive here) and
+    drawable records, for bi_pt_drawables_layout / bi_pt_set_scene_drawables."""
+
+    def __init__(self, positions, normals, tangents, texcoords, indices, drawables):
+        self.arrays = [np.ascontiguousarray(a, t) if a is not None else None
+                       for a, t in ((positions, np.float32), (normals, np.float32), (tangents, np.float32),
+                                    (texcoords, np.float32), (indices, np.uint32))]
+        p, n, t, uv, i = self.arrays
+        ptr = lambda a: a.ctypes.data if a is not None else None
+        size = lambda a: a.size if a is not None else 0
+        self.c = GpuSceneC(ptr(p), size(p), ptr(n), size(n), ptr(t), size(t), ptr(uv), size(uv), ptr(i), size(i))
+        self.drawables = (DrawableC * max(len(drawables), 1))()
+        for k, dr in enumerate(drawables):
+            for key, v in dr.items():
+                if key == "matrix":
+                    self.drawables[k].matrix[:] = [float(x) for x in np.asarray(v, np.float32).reshape(-1)]
+                else:
+                    setattr(self.drawables[k], key, int(v))
+        self.n = len(drawables)
+
+    def layout(self):
+        """bi_pt_drawables_layout (host only): ([range dicts], instance transforms [n, 12],
+        instance mesh indices, instance materials)."""
+        ranges = (SubmeshRangeC * max(self.n, 1))()
+        inst = (Instance * max(self.n, 1))()
+        nr = C.c_uint32()
+        st = lib().bi_pt_drawables_layout(C.byref(self.c), self.drawables, self.n, ranges, C.byref(nr), inst)
+        if st != 0:
+            raise PtError(st, "drawable records do not fit the GPU-scene buffers")
+        out = [{k: getattr(ranges[m], k) for k, _ in SubmeshRangeC._fields_} for m in range(nr.value)]
+        xf = np.array([list(inst[k].transform) for k in range(self.n)], np.float32).reshape(self.n, 12)
+        return out, xf, np.array([inst[k].mesh for k in range(self.n)]), np.array([inst[k].material for k in range(self.n)])
 
 
 class TextureC(C.Structure):
@@ -214,6 +273,9 @@ def lib():
         L.bi_pt_trace_closest.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
         L.bi_pt_trace_occlusion.argtypes = [vp, u32, vp, C.c_float, vp, vp]
         L.bi_pt_set_textures.argtypes = [vp, vp, u32]
+        L.bi_pt_drawables_layout.argtypes = [C.POINTER(GpuSceneC), C.POINTER(DrawableC), u32, C.POINTER(SubmeshRangeC),
+                                             C.POINTER(C.c_uint32), vp]
+        L.bi_pt_set_scene_drawables.argtypes = [vp, C.POINTER(GpuSceneC), C.POINTER(DrawableC), u32, vp, u32]
         L.bi_pt_trace_closest_noalloc.argtypes = [vp, u32, vp, C.c_float, vp, vp, vp, vp, vp]
         L.bi_pt_debug_inject_error.argtypes = [vp, u32]
         L.bi_pt_server_debug_inject_error.argtypes = [vp, u32]
@@ -315,6 +377,15 @@ class PathTracer:
             self.set_sky_cube(scene.sky_rgba, scene.sky_color, scene.sky_rot)
         else:
             self.set_sky(scene.sky_rgba, scene.sky_color, scene.sky_rot)
+
+    def set_scene_drawables(self, gpu_scene, materials, lights, sky_rgba, sky_color, sky_rot, textures=()):
+        """bi_pt_set_scene_drawables from a GpuScene, then lights, sky and textures."""
+        self.set_textures(list(textures))
+        mats = np.ascontiguousarray(materials, np.float32)
+        self._check(lib().bi_pt_set_scene_drawables(self.h, C.byref(gpu_scene.c), gpu_scene.drawables, gpu_scene.n,
+                                                    _p(mats), len(mats)))
+        self.set_lights(lights)
+        self.set_sky(sky_rgba, sky_color, sky_rot)
 
     def set_textures(self, textures):
         """bi_pt_set_textures: scenes.Texture records (texels [H, W, 4] uint8 or float32)."""

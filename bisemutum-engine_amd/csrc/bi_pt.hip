@@ -618,6 +618,81 @@ bi_pt_status bi_pt_render(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_p
     return render_jobs(ctx, ctx->eng, &job, 1, reinterpret_cast<hipStream_t>(stream), opt, nullptr);
 }
 
+bi_pt_status bi_pt_drawables_layout(const bi_pt_gpu_scene* gs, const bi_pt_drawable* d, uint32_t n,
+                                    bi_pt_submesh_range* ranges, uint32_t* num_ranges, bi_pt_instance* inst) {
+    if (!gs || (n && (!d || !ranges || !num_ranges || !inst)) || !gs->positions || !gs->indices)
+        return BI_PT_ERROR_INVALID_ARGUMENT;
+    constexpr uint64_t kAbsent = ~0ull;
+    uint32_t nr = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const bi_pt_drawable& D = d[k];
+        if (!(D.attributes & BI_PT_VA_POSITION) || D.index_count % 3 != 0) return BI_PT_ERROR_INVALID_ARGUMENT;
+        if ((uint64_t)D.index_offset + D.index_count > gs->num_indices) return BI_PT_ERROR_INVALID_ARGUMENT;
+        bi_pt_submesh_range r{};
+        r.position_offset = D.position_offset;
+        r.normal_offset = (D.attributes & BI_PT_VA_NORMAL) ? D.normal_offset : kAbsent;
+        r.tangent_offset = (D.attributes & BI_PT_VA_TANGENT) ? D.tangent_offset : kAbsent;
+        r.texcoord_offset = (D.attributes & BI_PT_VA_TEXCOORD) ? D.texcoord_offset : kAbsent;
+        r.index_offset = D.index_offset;
+        r.index_count = D.index_count;
+        uint32_t nv = 0;
+        for (uint32_t i = 0; i < D.index_count; ++i) nv = std::max(nv, gs->indices[D.index_offset + i] + 1u);
+        if (D.index_count && nv == 0) return BI_PT_ERROR_INVALID_ARGUMENT;   // index 0xffffffff
+        r.num_vertices = nv;
+        auto fits = [&](uint64_t off, uint32_t width, const float* buf, uint64_t len) {
+            return off == kAbsent || (buf && off + (uint64_t)width * nv <= len);
+        };
+        if (!fits(r.position_offset, 3, gs->positions, gs->num_positions) ||
+            !fits(r.normal_offset, 3, gs->normals, gs->num_normals) ||
+            !fits(r.tangent_offset, 4, gs->tangents, gs->num_tangents) ||
+            !fits(r.texcoord_offset, 2, gs->texcoords, gs->num_texcoords))
+            return BI_PT_ERROR_INVALID_ARGUMENT;
+        uint32_t m = 0;
+        while (m < nr && std::memcmp(&ranges[m], &r, sizeof r) != 0) ++m;   // drawables that share a submesh
+        if (m == nr) ranges[nr++] = r;
+        // the TLAS instance's 3x4 rows are the matrix's rows: element [c][r] of glm's
+        // column-major storage goes to row r, column c (accel.cpp:119-131)
+        for (int row = 0; row < 3; ++row)
+            for (int col = 0; col < 4; ++col) inst[k].transform[4 * row + col] = D.matrix[4 * col + row];
+        inst[k].mesh = m;
+        inst[k].material = D.material;
+    }
+    *num_ranges = nr;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_set_scene_drawables(bi_pt_ctx* ctx, const bi_pt_gpu_scene* gs, const bi_pt_drawable* d,
+                                       uint32_t n, const bi_pt_material* materials, uint32_t num_materials) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    std::vector<bi_pt_submesh_range> ranges(n);
+    std::vector<bi_pt_instance> inst(n);
+    uint32_t nr = 0;
+    if (bi_pt_drawables_layout(gs, d, n, ranges.data(), &nr, inst.data()) != BI_PT_OK)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT,
+                    "drawable records do not fit the GPU-scene buffers (offsets, index counts or attributes)");
+    // the meshes point into the flat buffers; missing normals read (0, 0, 1) (hit.hlsl:54)
+    std::vector<bi_pt_mesh> meshes(nr);
+    std::vector<std::vector<float>> flat_normals;
+    for (uint32_t m = 0; m < nr; ++m) {
+        const bi_pt_submesh_range& r = ranges[m];
+        bi_pt_mesh& M = meshes[m];
+        M.positions = gs->positions + r.position_offset;
+        if (r.normal_offset != ~0ull) {
+            M.normals = gs->normals + r.normal_offset;
+        } else {
+            flat_normals.emplace_back((size_t)3 * r.num_vertices, 0.0f);
+            for (uint32_t v = 0; v < r.num_vertices; ++v) flat_normals.back()[3 * (size_t)v + 2] = 1.0f;
+            M.normals = flat_normals.back().data();
+        }
+        M.indices = gs->indices + r.index_offset;
+        M.num_vertices = r.num_vertices;
+        M.num_indices = r.index_count;
+        M.texcoords = r.texcoord_offset != ~0ull ? gs->texcoords + r.texcoord_offset : nullptr;
+        M.tangents = r.tangent_offset != ~0ull ? gs->tangents + r.tangent_offset : nullptr;
+    }
+    return bi_pt_set_scene(ctx, meshes.data(), nr, inst.data(), n, materials, num_materials);
+}
+
 bi_pt_status bi_pt_render_gbuffer(bi_pt_ctx* ctx, const bi_pt_camera* camera, const bi_pt_settings* st,
                                   float* out_rgba, float* out_depth, const bi_pt_gbuffer* gbuffer, bi_pt_stream stream) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;

@@ -248,6 +248,76 @@ bi_pt_status bi_pt_create_view(bi_pt_ctx* scene, bi_pt_ctx** out_view);
 bi_pt_status bi_pt_set_scene(bi_pt_ctx* ctx, const bi_pt_mesh* meshes, uint32_t num_meshes,
                              const bi_pt_instance* instances, uint32_t num_instances,
                              const bi_pt_material* materials, uint32_t num_materials);
+/* The engine's GPU scene as its ray-tracing shaders read it (GpuSceneData,
+ * src/graphics/gpu_scene_data.hpp:10-24): vertex attributes of every mesh in shared flat
+ * buffers (positions and normals 3 floats per vertex, tangents 4, texture coordinates 2;
+ * uint32 indices), here host copies, with their lengths in elements.  Nullable but
+ * positions and indices. */
+typedef struct bi_pt_gpu_scene {
+    const float* positions;
+    uint64_t num_positions;
+    const float* normals;
+    uint64_t num_normals;
+    const float* tangents;
+    uint64_t num_tangents;
+    const float* texcoords;
+    uint64_t num_texcoords;
+    const uint32_t* indices;
+    uint64_t num_indices;
+} bi_pt_gpu_scene;
+/* Vertex attributes a mesh has (VertexAttributesType, include/bisemutum/graphics/
+ * vertex_attributes_type.hpp:7-19); the hit shader reads only those (hit.hlsl:55-136). */
+enum {
+    BI_PT_VA_POSITION = 0x1,
+    BI_PT_VA_NORMAL = 0x4,
+    BI_PT_VA_TANGENT = 0x8,
+    BI_PT_VA_TEXCOORD = 0x20
+};
+/* One drawable: its hit-group shader record (DrawableSbtData, src/graphics/
+ * drawable_stb_data.hpp:7-17, as graphics_manager.cpp:1314-1330 fills it: element offsets
+ * into the flat buffers with the submesh's base vertex included, the submesh's first
+ * index), the submesh's index count (its BLAS build input, graphics_manager.cpp:620-640),
+ * its mesh's vertex attributes, its material's record in the table passed alongside
+ * (material_offset locates the engine's parameter struct, which this pass reads as a
+ * bi_pt_material), and its object-to-world matrix as glm holds it (column-major,
+ * drawable->transform.matrix()), transposed into the 3x4 instance transform as the TLAS
+ * build does (accel.cpp:119-131).  A mesh without normals reads (0, 0, 1), without
+ * tangents (1, 0, 0, 1), without texture coordinates (0, 0) (hit.hlsl:54, 74, 117). */
+typedef struct bi_pt_drawable {
+    uint32_t drawable_index;
+    uint32_t position_offset;
+    uint32_t normal_offset;
+    uint32_t tangent_offset;
+    uint32_t color_offset;
+    uint32_t texcoord_offset;
+    uint32_t texcoord2_offset;
+    uint32_t index_offset;
+    uint32_t material_offset;
+    uint32_t index_count;
+    uint32_t attributes;         /* BI_PT_VA_* */
+    uint32_t material;           /* index into the material table */
+    float matrix[16];            /* glm::mat4, column-major */
+} bi_pt_drawable;
+/* Where one mesh of the converted scene sits in the flat buffers (element offsets;
+ * UINT64_MAX: attribute absent) and its index and vertex counts (vertices: one past the
+ * largest index). */
+typedef struct bi_pt_submesh_range {
+    uint64_t position_offset, normal_offset, tangent_offset, texcoord_offset, index_offset;
+    uint32_t index_count, num_vertices;
+} bi_pt_submesh_range;
+/* The scene layout of drawables (host only, no device work): one range per distinct
+ * submesh record (drawables that share one share it) and one instance per drawable
+ * (transposed transform, range index, material).  ranges and instances hold n entries;
+ * *num_ranges receives the count used.  INVALID_ARGUMENT when a record reaches past its
+ * buffer, an index count is not a multiple of 3, or an attribute the drawable names has no
+ * buffer. */
+bi_pt_status bi_pt_drawables_layout(const bi_pt_gpu_scene* scene, const bi_pt_drawable* drawables, uint32_t n,
+                                    bi_pt_submesh_range* ranges, uint32_t* num_ranges, bi_pt_instance* instances);
+/* bi_pt_set_scene from the engine's GPU-scene buffers and drawables (the inputs the
+ * reference's TLAS and hit shaders read, gpu_scene_data.hpp:10-24, accel.cpp:103-132). */
+bi_pt_status bi_pt_set_scene_drawables(bi_pt_ctx* ctx, const bi_pt_gpu_scene* scene, const bi_pt_drawable* drawables,
+                                       uint32_t num_drawables, const bi_pt_material* materials,
+                                       uint32_t num_materials);
 /* Replace instance transforms only (same count/meshes/materials); the LBVH is rebuilt by the next render. */
 bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* transforms_3x4, uint32_t num_instances);
 /* The material textures of the scene (the material table names them by index); replaces
