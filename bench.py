@@ -79,8 +79,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rmse", action="store_true", help="skip the FP64 oracle RMSE leg")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache leg")
-    ap.add_argument("--rmse-frames", type=int, default=6, help="frames averaged in each RMSE figure")
-    ap.add_argument("--nrc-warmup", type=int, default=48, help="frames of online training before the cache's RMSE")
+    ap.add_argument("--rmse-frames", type=int, default=16, help="frames averaged in each RMSE figure")
+    ap.add_argument("--nrc-warmup", type=int, default=96, help="frames of online training before the cache's RMSE")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-calls", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -446,51 +446,89 @@ def rmse_leg(pt, cam, cfg, ref, frames):
         rel.append(q)
     out = {"relative_rmse": round(float(np.sqrt(np.mean(mse))), 4), "relative_mse": round(float(np.mean(rel)), 4),
            "frames": frames, "ms_per_frame": round(float(np.mean(ms)), 2), "spp": cfg["spp"],
-           "pixels": int(len(ref["pixels"])), "oracle_spp": int(ref["spp"])}
+           "pixels": int(len(ref["pixels"])), "oracle_spp": int(ref["spp"]),
+           "relative_mse_per_frame": [round(float(x), 4) for x in rel]}
     log(f"rmse leg: plain {out}")
     return out
+
+
+def bootstrap_ci(values, n=2000, seed=0x5EED):
+    """Mean and 95% percentile bootstrap interval of the mean over frames."""
+    import numpy as np
+    v = np.asarray(values, np.float64)
+    rng = np.random.default_rng(seed)
+    means = v[rng.integers(0, len(v), (n, len(v)))].mean(1)
+    return float(v.mean()), float(np.percentile(means, 2.5)), float(np.percentile(means, 97.5))
+
+
+def constant_cache_params(rgb):
+    """Network weights whose output is the constant rgb for every input: the constant input
+    (59) carried through one hidden unit of each layer, the output row scaled by rgb."""
+    import numpy as np
+    p = np.zeros(4 * 4096 + 64 * 16, np.float32)
+    p[59 * 64 + 0] = 1.0                       # layer 1: input 59 -> unit 0
+    for layer in (1, 2, 3):
+        p[layer * 4096 + 0] = 1.0              # unit 0 -> unit 0
+    p[16384:16384 + 3] = np.asarray(rgb, np.float32)   # output layer row 0
+    return p
 
 
 def nrc_leg(pt, cam, cfg, ref, plain, frames, warmup):
     """Neural radiance cache form at equal time: paths end in the cache after 2 bounces,
     65,536 training paths and 4 Adam steps per frame (online).  After `warmup` frames of
     online training, the cached frame's spp is set so that its time matches the plain
-    16-spp frame; RMSE against the FP64 reference, averaged over `frames` frames."""
+    16-spp frame; per-frame relative MSE against the FP64 reference over `frames` frames
+    (at least 16), with 95% bootstrap intervals.  Control: a cache whose output is one
+    constant (the mean of the last frame's training targets), frozen, at the same spp."""
     import numpy as np
 
     import bi_pt
     mb = cfg["max_bounces"]
+    frames = max(frames, 16)
     nrc = bi_pt.Nrc(pt, train_samples=65536, train_steps=4, cache_bounce=2, seed=1)
     t0 = time.perf_counter()
     for k in range(warmup):
         nrc.render(cam, spp=1, max_bounces=mb, seed=cfg["seed"] + 9000 + k)
     nrc.stats()
     warm_s = time.perf_counter() - t0
+    rec, tgt = nrc.samples(65536)
+    valid = rec[:, 13].view(np.uint32) == 1
+    const_rgb = tgt[valid].astype(np.float64).mean(0) if valid.any() else np.zeros(3)
     t16 = timed(lambda: nrc.render(cam, spp=16, max_bounces=mb, seed=cfg["seed"] + 9500))
     spp_eq = max(1, int(cfg["spp"] * plain["ms_per_frame"] / t16))
 
-    def run(spp, base):
-        ms, mse, rel = [], [], []
+    def run(cache, spp, base):
+        ms, rel = [], []
         for k in range(frames):
-            ms.append(timed(lambda: nrc.render(cam, spp=spp, max_bounces=mb, seed=base + k)))
-            r, q = image_errors(pt.image(), ref)
-            mse.append(r * r)
-            rel.append(q)
-        return {"spp": spp, "ms_per_frame": round(float(np.mean(ms)), 2),
-                "relative_rmse": round(float(np.sqrt(np.mean(mse))), 4), "relative_mse": round(float(np.mean(rel)), 4)}
+            ms.append(timed(lambda: cache.render(cam, spp=spp, max_bounces=mb, seed=base + k)))
+            rel.append(image_errors(pt.image(), ref)[1])
+        m, lo, hi = bootstrap_ci(rel)
+        return {"spp": spp, "frames": frames, "ms_per_frame": round(float(np.mean(ms)), 2),
+                "relative_mse": round(m, 4), "relative_mse_ci95": [round(lo, 4), round(hi, 4)],
+                "relative_mse_per_frame": [round(float(x), 4) for x in rel]}
 
-    eq = run(spp_eq, cfg["seed"] + 10000)
-    same_spp = run(cfg["spp"], cfg["seed"] + 11000)
+    eq = run(nrc, spp_eq, cfg["seed"] + 10000)
+    same_spp = run(nrc, cfg["spp"], cfg["seed"] + 11000)
     st = nrc.stats()
     nrc.close()
+    ctl = bi_pt.Nrc(pt, train_samples=65536, train_steps=4, cache_bounce=2, seed=1, flags=bi_pt.NRC_FREEZE)
+    ctl.set_params(constant_cache_params(const_rgb))
+    control = run(ctl, spp_eq, cfg["seed"] + 10000)
+    ctl.close()
+    plain_rel = plain.get("relative_mse_per_frame")
     out = {"workload": f"cache after 2 bounces, 65,536 training paths and 4 Adam steps per frame, "
                        f"{warmup} warm-up frames of online training at 1 spp",
-           "equal_time": eq, "equal_spp": same_spp, "plain": {k: plain[k] for k in ("spp", "ms_per_frame",
-                                                                                   "relative_rmse", "relative_mse")},
-           "rmse_ratio_equal_time": round(eq["relative_rmse"] / plain["relative_rmse"], 4),
+           "equal_time": eq, "equal_spp": same_spp,
+           "constant_cache_equal_time": dict(control, constant_rgb=[round(float(x), 5) for x in const_rgb]),
+           "plain": {k: plain[k] for k in ("spp", "ms_per_frame", "relative_rmse", "relative_mse")},
            "relmse_ratio_equal_time": round(eq["relative_mse"] / plain["relative_mse"], 4),
+           "relmse_ratio_to_constant_cache": round(eq["relative_mse"] / control["relative_mse"], 4),
+           "network_beats_constant_beyond_ci": bool(eq["relative_mse_ci95"][1] < control["relative_mse_ci95"][0]),
            "warmup_s": round(warm_s, 2), "train_steps": int(st["steps"]), "last_loss": round(st["last_loss"], 4),
            "valid_training_samples": int(st["last_valid_samples"])}
+    if plain_rel:
+        m, lo, hi = bootstrap_ci(plain_rel)
+        out["plain"]["relative_mse_ci95"] = [round(lo, 4), round(hi, 4)]
     log(f"nrc leg: {out}")
     return out
 

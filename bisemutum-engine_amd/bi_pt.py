@@ -183,7 +183,7 @@ class ServerStats(C.Structure):
 class NrcDesc(C.Structure):
     _fields_ = [("train_samples", C.c_uint32), ("train_steps", C.c_uint32), ("cache_bounce", C.c_uint32),
                 ("learning_rate", C.c_float), ("use_after_steps", C.c_uint32), ("seed", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("target_clamp", C.c_float)]
 
 
 class NrcStats(C.Structure):
@@ -771,12 +771,13 @@ class Nrc:
     """bi_pt_nrc: the neural radiance cache of one PathTracer's scene."""
 
     def __init__(self, tracer, train_samples=0, train_steps=0, cache_bounce=0, learning_rate=0.0,
-                 use_after_steps=0, seed=0, flags=0):
+                 use_after_steps=0, seed=0, flags=0, target_clamp=0.0):
         self.tracer = tracer
         self.torch = tracer.torch
         h = C.c_void_p()
         st = lib().bi_pt_nrc_create(tracer.h, C.byref(NrcDesc(train_samples, train_steps, cache_bounce,
-                                                                learning_rate, use_after_steps, seed, flags)),
+                                                                learning_rate, use_after_steps, seed, flags,
+                                                                target_clamp)),
                                     C.byref(h))
         if st != 0:
             raise PtError(st, "bi_pt_nrc_create failed")

@@ -217,6 +217,57 @@ __device__ __forceinline__ bf16x8 gather_col(const __bf16* M, int row0, int col)
     return v;
 }
 
+// The bound of a frame's training targets: factor x the mean luminance of the valid
+// targets.  Per 256-slot block the luminance sum and count (a fixed-order tree in LDS),
+// then one block adds the block partials in order, so the bound is reproducible.
+__global__ __launch_bounds__(256) void k_target_lum(const NrcRecord* rec, const float* target, uint32_t n,
+                                                    float* part) {
+    __shared__ float s_sum[256], s_cnt[256];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    float l = 0.0f, c = 0.0f;
+    if (i < n && rec[i].tag == 1u) {
+        const float r = target[3 * (size_t)i], g = target[3 * (size_t)i + 1], b = target[3 * (size_t)i + 2];
+        const float y = (0.2126f * r + 0.7152f * g) + 0.0722f * b;
+        if (isfinite(y)) {
+            l = y > 0.0f ? y : 0.0f;
+            c = 1.0f;
+        }
+    }
+    s_sum[threadIdx.x] = l;
+    s_cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t h = 128u; h > 0u; h >>= 1) {
+        if (threadIdx.x < h) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + h];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (size_t)blockIdx.x] = s_sum[0];
+        part[2 * (size_t)blockIdx.x + 1] = s_cnt[0];
+    }
+}
+__global__ __launch_bounds__(256) void k_target_bound(float* part, uint32_t blocks, float factor, float* bound) {
+    __shared__ float s_sum[256], s_cnt[256];
+    float l = 0.0f, c = 0.0f;
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256u) {
+        l += part[2 * (size_t)b];
+        c += part[2 * (size_t)b + 1];
+    }
+    s_sum[threadIdx.x] = l;
+    s_cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t h = 128u; h > 0u; h >>= 1) {
+        if (threadIdx.x < h) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + h];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *bound = s_cnt[0] > 0.0f ? factor * (s_sum[0] / s_cnt[0]) : __builtin_inff();
+}
+
 // Training on one 64-sample chunk (one wave): forward keeping every layer's input,
 // relative-L2 loss gradient, backward; the chunk's weight gradients go to its partial.
 __global__ __launch_bounds__(64) void k_nrc_train(NrcTrainArgs a) {
@@ -240,6 +291,10 @@ __global__ __launch_bounds__(64) void k_nrc_train(NrcTrainArgs a) {
             tg[1] = a.target[3 * (size_t)i + 1];
             tg[2] = a.target[3 * (size_t)i + 2];
             valid = isfinite(tg[0]) && isfinite(tg[1]) && isfinite(tg[2]);
+            if (a.clamp) {
+                const float c = *a.clamp;
+                for (int k = 0; k < 3; ++k) tg[k] = tg[k] < c ? tg[k] : c;
+            }
         }
     }
     if (!valid)
@@ -462,7 +517,7 @@ bi_pt_status ensure_grad(bi_pt_nrc* nrc, uint32_t chunks) {
 
 // One Adam step on rows [row0, row0 + rows) of rec / target; loss partials at loss_off chunks.
 bi_pt_status train_step(bi_pt_nrc* nrc, const NrcRecord* rec, const float* target, uint32_t row0, uint32_t rows,
-                        uint32_t loss_off, hipStream_t s) {
+                        uint32_t loss_off, const float* clamp, hipStream_t s) {
     const uint32_t chunks = (rows + 63u) / 64u;
     NrcTrainArgs ta{};
     ta.rec = rec;
@@ -475,6 +530,7 @@ bi_pt_status train_step(bi_pt_nrc* nrc, const NrcRecord* rec, const float* targe
     ta.grad_part = nrc->grad_part.p;
     ta.loss_part = nrc->loss_part.p + 2 * (size_t)loss_off;
     ta.scale = 1.0f / (3.0f * (float)rows);
+    ta.clamp = clamp;
     launch_nrc_train(ta, s);
     nrc->steps++;
     NrcAdamArgs aa{};
@@ -516,8 +572,19 @@ bi_pt_status nrc_train_frame(bi_pt_nrc* nrc, hipStream_t s) {
     const uint32_t per = nrc->train_samples / nrc->train_steps;
     const uint32_t chunks = (per + 63u) / 64u;
     BI_TRY(ensure_grad(nrc, chunks));
+    const float* clamp = nullptr;
+    if (nrc->clamp_factor > 0.0f) {   // the frame's target bound, before its steps
+        const uint32_t blocks = (nrc->train_samples + 255u) / 256u;
+        BI_TRY(dalloc(nrc, nrc->clamp_part, (size_t)2 * blocks + 1));
+        float* bound = nrc->clamp_part.p + 2 * (size_t)blocks;
+        hipLaunchKernelGGL(k_target_lum, dim3(blocks), dim3(256), 0, s, nrc->train_rec.p, nrc->targets.p,
+                           nrc->train_samples, nrc->clamp_part.p);
+        hipLaunchKernelGGL(k_target_bound, dim3(1), dim3(256), 0, s, nrc->clamp_part.p, blocks, nrc->clamp_factor,
+                           bound);
+        clamp = bound;
+    }
     for (uint32_t k = 0; k < nrc->train_steps; ++k)
-        BI_TRY(train_step(nrc, nrc->train_rec.p, nrc->targets.p, k * per, per, k * chunks, s));
+        BI_TRY(train_step(nrc, nrc->train_rec.p, nrc->targets.p, k * per, per, k * chunks, clamp, s));
     nrc->last_chunks = chunks * nrc->train_steps;
     nrc->last_grad_chunks = chunks;
     return BI_PT_OK;
@@ -543,6 +610,7 @@ bi_pt_status bi_pt_nrc_create(bi_pt_ctx* ctx, const bi_pt_nrc_desc* desc, bi_pt_
         if (desc->cache_bounce) n->cache_bounce = desc->cache_bounce;
         if (desc->learning_rate > 0.0f) n->lr = desc->learning_rate;
         n->use_after = desc->use_after_steps;
+        if (desc->target_clamp != 0.0f) n->clamp_factor = desc->target_clamp;
         n->flags = desc->flags;
         seed = desc->seed;
     }
@@ -552,7 +620,7 @@ bi_pt_status bi_pt_nrc_create(bi_pt_ctx* ctx, const bi_pt_nrc_desc* desc, bi_pt_
     }
     auto bad = [&](bi_pt_status st) {
         dfree(n, n->w); dfree(n, n->m); dfree(n, n->v); dfree(n, n->targets); dfree(n, n->grad_part);
-        dfree(n, n->loss_part); dfree(n, n->bounds); dfree(n, n->fwd); dfree(n, n->bwd); dfree(n, n->train_rec);
+        dfree(n, n->loss_part); dfree(n, n->clamp_part); dfree(n, n->bounds); dfree(n, n->fwd); dfree(n, n->bwd); dfree(n, n->train_rec);
         dfree(n, n->query);
         delete n;
         return st;
@@ -591,7 +659,7 @@ bi_pt_status bi_pt_nrc_destroy(bi_pt_nrc* n) {
     (void)hipSetDevice(n->device);
     (void)hipDeviceSynchronize();
     dfree(n, n->w); dfree(n, n->m); dfree(n, n->v); dfree(n, n->targets); dfree(n, n->grad_part);
-    dfree(n, n->loss_part); dfree(n, n->bounds); dfree(n, n->fwd); dfree(n, n->bwd); dfree(n, n->train_rec);
+    dfree(n, n->loss_part); dfree(n, n->clamp_part); dfree(n, n->bounds); dfree(n, n->fwd); dfree(n, n->bwd); dfree(n, n->train_rec);
     dfree(n, n->query);
     delete n;
     return BI_PT_OK;
@@ -697,7 +765,7 @@ bi_pt_status bi_pt_nrc_train(bi_pt_nrc* n, const void* records, const float* tar
     BI_HIP_TRY(n, hipSetDevice(n->device));
     const uint32_t chunks = (count + 63u) / 64u;
     BI_TRY(ensure_grad(n, chunks));
-    BI_TRY(train_step(n, static_cast<const NrcRecord*>(records), targets, 0, count, 0,
+    BI_TRY(train_step(n, static_cast<const NrcRecord*>(records), targets, 0, count, 0, nullptr,
                       reinterpret_cast<hipStream_t>(stream)));
     n->last_chunks = chunks;
     n->last_grad_chunks = chunks;

@@ -47,6 +47,7 @@ struct NrcTrainArgs {
     float* grad_part;           // kNrcParams per 64-sample chunk
     float* loss_part;           // per chunk: loss sum, valid samples
     float scale;                // gradient of the mean loss per sample term
+    const float* clamp;         // device scalar: upper bound of every target channel (null: none)
 };
 void launch_nrc_train(const NrcTrainArgs& a, hipStream_t s);
 
@@ -69,7 +70,9 @@ void launch_nrc_adam(const NrcAdamArgs& a, hipStream_t s);
 struct bi_pt_nrc : bipt::Owner {
     bi_pt_ctx* ctx = nullptr;
     uint32_t train_samples = 65536, train_steps = 4, cache_bounce = 2, use_after = 0, flags = 0;
-    float lr = 1e-2f, b1 = 0.9f, b2 = 0.99f, eps = 1e-15f;
+    float lr = 3e-3f, b1 = 0.9f, b2 = 0.99f, eps = 1e-15f;
+    float clamp_factor = 16.0f;   // targets clamped to this multiple of their mean luminance (<= 0: none)
+    bipt::DevBuf<float> clamp_part;   // per-block luminance sums and counts, then the bound
     bipt::DevBuf<float> w, m, v, targets, grad_part, loss_part, bounds;
     bipt::DevBuf<uint16_t> fwd, bwd;
     bipt::DevBuf<bipt::NrcRecord> train_rec, query;

@@ -37,16 +37,26 @@ struct Node4 {
     float4 lx, ly, lz, hx, hy, hz;   // child k's box in component k
     int4 ch;
 };
+// Node and leaf records are read through global-address-space pointers (the integer
+// round trip keeps the compiler from folding the cast away): global loads, not flat ones,
+// so the seven 16-B loads of a node issue back to back and the waits count only vector
+// memory (flat loads, which might address LDS, made the compiler wait between them).
+typedef float VF4 __attribute__((ext_vector_type(4)));
+typedef float VF2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const VF4 GlobalF4;
+typedef __attribute__((address_space(1))) const VF2 GlobalF2;
+__device__ __forceinline__ float4 f4(VF4 v) { return make_float4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ Node4 load_node(const Bvh4Node* nodes, int i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    const GlobalF4* p = (const GlobalF4*)(uintptr_t)(nodes + i);
     Node4 n;
-    n.lx = p[0];
-    n.ly = p[1];
-    n.lz = p[2];
-    n.hx = p[3];
-    n.hy = p[4];
-    n.hz = p[5];
-    n.ch = reinterpret_cast<const int4*>(p)[6];
+    n.lx = f4(p[0]);
+    n.ly = f4(p[1]);
+    n.lz = f4(p[2]);
+    n.hx = f4(p[3]);
+    n.hy = f4(p[4]);
+    n.hz = f4(p[5]);
+    const VF4 c = p[6];
+    n.ch = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z), __float_as_int(c.w));
     return n;
 }
 
@@ -73,11 +83,12 @@ struct LeafRegs {
     float2 g2;
 };
 __device__ __forceinline__ LeafRegs load_leaf(const LeafGeo* geo, uint32_t leaf) {
-    const float4* p = reinterpret_cast<const float4*>(geo + leaf);
+    const GlobalF4* p = (const GlobalF4*)(uintptr_t)(geo + leaf);
     LeafRegs r;
-    r.g0 = p[0];
-    r.g1 = p[1];
-    r.g2 = reinterpret_cast<const float2*>(p + 2)[0];
+    r.g0 = f4(p[0]);
+    r.g1 = f4(p[1]);
+    const VF2 g2 = ((const GlobalF2*)(p + 2))[0];
+    r.g2 = make_float2(g2.x, g2.y);
     return r;
 }
 __device__ __forceinline__ bool leaf_non_opaque(const LeafRegs& r) {

@@ -511,10 +511,13 @@ typedef struct bi_pt_nrc_desc {
     uint32_t train_samples;       /* training paths per frame; 0 = 65,536 (a multiple of 64 * train_steps) */
     uint32_t train_steps;         /* Adam steps per frame; 0 = 4 */
     uint32_t cache_bounce;        /* segment at which paths end in the cache; 0 = 2 */
-    float learning_rate;          /* 0 = 1e-2 (beta1 0.9, beta2 0.99, eps 1e-15) */
+    float learning_rate;          /* 0 = 3e-3 (beta1 0.9, beta2 0.99, eps 1e-15) */
     uint32_t use_after_steps;     /* plain rendering before this many training steps */
     uint32_t seed;                /* initial weights */
     uint32_t flags;               /* BI_PT_NRC_FREEZE: render with the cache, no training */
+    float target_clamp;           /* a frame's training targets are clamped, per channel, to this
+                                     multiple of their mean luminance, so that a single firefly
+                                     does not dominate the loss; 0 = 16; < 0: no clamp */
 } bi_pt_nrc_desc;
 enum { BI_PT_NRC_FREEZE = 1u << 0 };
 typedef struct bi_pt_nrc_stats {

@@ -46,7 +46,7 @@ def random_records(rng, n, lo, hi, valid_frac=1.0):
 def nrc_pair(c0_scene):
     t = bi_pt.PathTracer()
     t.set_scene(c0_scene)
-    n = bi_pt.Nrc(t, train_samples=4096, train_steps=4)
+    n = bi_pt.Nrc(t, train_samples=4096, train_steps=4, learning_rate=1e-2)   # the Adam tests' step size
     yield t, n
     n.close()
     t.close()
@@ -214,22 +214,27 @@ def test_training_samples_match_oracle(c0_scene, c0_oracle):
 
 
 def test_online_training_learns_suffix_radiance(c0_scene, c0_oracle):
+    """128 frames of online training on C0 (65,536 training paths, 4 Adam steps per frame,
+    targets clamped to 16x their mean luminance).  Held out: 8,192 vertices of other paths,
+    each target the mean of 1,024 continuations (the restatement's), so the targets' own
+    noise stays well under the spread the cache has to explain.  The trained cache's
+    relative squared error must be at most 0.7x that of the best constant."""
     import torch
-    print("case: c0 160x120, 32 frames of online training; held-out vertices, 64-suffix targets", flush=True)
+    print("case: c0 160x120, 128 frames of online training; held-out vertices, 1024-suffix targets", flush=True)
     cam = scenes.camera_c0(160, 120)
     t = bi_pt.PathTracer()
     t.set_scene(c0_scene)
-    nrc = bi_pt.Nrc(t, train_samples=16384, train_steps=4, cache_bounce=2, seed=7)
+    nrc = bi_pt.Nrc(t, train_samples=65536, train_steps=4, cache_bounce=2, seed=7)
     fresh = bi_pt.Nrc(t, train_samples=4096, train_steps=4, cache_bounce=2, seed=7)   # the initial weights
-    # held-out vertices from other pixels, each target the mean of 64 continuations
-    orec, otgt = c0_oracle.train_samples(cam, 5, 0xABCD, 2, 8192, suffixes=64)
+    orec, otgt = c0_oracle.train_samples(cam, 5, 0xABCD, 2, 8192, suffixes=1024)
     v = orec[:, 13].view(np.uint32) == 1
     rec = torch.as_tensor(orec[v]).cuda()
     tgt = otgt[v].astype(np.float64)
     losses = []
-    for f in range(32):
+    for f in range(128):
         nrc.render(cam, spp=1, max_bounces=5, seed=100 + f)
-        losses.append(nrc.stats()["last_loss"])
+        if f % 16 == 15:
+            losses.append(nrc.stats()["last_loss"])
     cb = t.bvh()["centroid_bounds"]
     fresh.set_bounds(cb[:3], cb[3:])
 
@@ -241,12 +246,11 @@ def test_online_training_learns_suffix_radiance(c0_scene, c0_oracle):
     const = ((tgt - tgt.mean(0)) ** 2).sum() / (tgt ** 2).sum()
     img = t.image()
     print(f"held-out relative squared error: initial {before:.4f}, trained {after:.4f}, best constant {const:.4f} "
-          f"({int(v.sum())} vertices); training loss per frame {' '.join(f'{x:.2f}' for x in losses)}")
+          f"(ratio {after / const:.3f}, {int(v.sum())} vertices); training loss every 16 frames "
+          f"{' '.join(f'{x:.2f}' for x in losses)}", flush=True)
     assert np.all(np.isfinite(img)) and np.all(img[:, :3] >= 0)
-    # the initial network is close to the zero predictor; the trained cache must explain
-    # part of how the suffix radiance varies over the vertices: beat the best constant
     assert after < 0.9 * before
-    assert after < const
+    assert after <= 0.7 * const
     fresh.close()
     nrc.close()
     t.close()
