@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
-           "bi_pt_set_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_set_lights", "bi_pt_set_point_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
@@ -222,6 +222,7 @@ def lib():
         L.bi_pt_update_transforms.argtypes = [vp, f32p, u32]
         L.bi_pt_set_lights.argtypes = [vp, f32p, u32]
         L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
+        L.bi_pt_set_point_lights.argtypes = [vp, f32p, u32]
         L.bi_pt_set_sky_cube.argtypes = [vp, f32p, u32, f32p, f32p]
         L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
         L.bi_pt_render_gbuffer.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, C.POINTER(GBufferC), vp]
@@ -330,6 +331,7 @@ class PathTracer:
         self.h = h
         self._out = None
         self._depth = None
+        self._n_point_lights = 0
 
     def _check(self, st):
         if st != 0:
@@ -372,7 +374,10 @@ class PathTracer:
         mats = np.ascontiguousarray(scene.materials, np.float32)
         self._check(lib().bi_pt_set_scene(self.h, meshes, len(scene.meshes), inst, len(scene.instances), _p(mats),
                                           len(mats)))
+        if len(scene.dir_lights) + self._n_point_lights > 4:
+            self.set_point_lights(None)   # the light list holds four lights, directional and point together
         self.set_lights(scene.dir_lights)
+        self.set_point_lights(getattr(scene, "point_lights", None))
         if getattr(scene, "sky_layout", 0) == 1:
             self.set_sky_cube(scene.sky_rgba, scene.sky_color, scene.sky_rot)
         else:
@@ -400,6 +405,12 @@ class PathTracer:
     def update_transforms(self, xforms):
         x = np.ascontiguousarray(xforms, np.float32)
         self._check(lib().bi_pt_update_transforms(self.h, _p(x), len(x)))
+
+    def set_point_lights(self, lights):
+        """bi_pt_set_point_lights: records [P, 16] (scenes.point_light), or None."""
+        lights = np.ascontiguousarray(lights if lights is not None else np.zeros((0, 16)), np.float32).reshape(-1, 16)
+        self._check(lib().bi_pt_set_point_lights(self.h, _p(lights) if len(lights) else None, len(lights)))
+        self._n_point_lights = len(lights)
 
     def set_lights(self, lights):
         lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)

@@ -473,22 +473,76 @@ bi_pt_status bi_pt_set_textures(bi_pt_ctx* ctx, const bi_pt_texture* textures, u
     return BI_PT_OK;
 }
 
+namespace {
+
+// The kernels' light records: the directional lights, then the point lights.  A change
+// restarts the running image (unchanged lights, set every frame, keep it).
+bi_pt_status commit_lights(bi_pt_ctx* ctx, std::vector<LightRec>& dir, std::vector<LightRec>& point) {
+    if (dir.size() + point.size() > (size_t)kMaxDirLights)
+        return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 4 lights (directional and point together)");
+    std::vector<LightRec> all(dir);
+    all.insert(all.end(), point.begin(), point.end());
+    const bool changed = all.size() != ctx->lights.size() ||
+                         (!all.empty() && std::memcmp(all.data(), ctx->lights.data(), all.size() * sizeof(LightRec)) != 0);
+    ctx->dir_lights.swap(dir);
+    ctx->point_lights.swap(point);
+    ctx->lights.swap(all);
+    if (changed) {
+        ctx->acc_count = 0;
+        ctx->scene_version++;
+    }
+    return BI_PT_OK;
+}
+
+}  // namespace
+
 bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uint32_t num_lights) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
     if (num_lights > (uint32_t)kMaxDirLights)
         return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 4 directional lights");
     if (num_lights && !lights) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null lights");
-    std::vector<LightRec> next(num_lights);
-    if (num_lights) std::memcpy(next.data(), lights, num_lights * sizeof(LightRec));
-    const bool changed = next.size() != ctx->lights.size() ||
-                         (num_lights && std::memcmp(next.data(), ctx->lights.data(), num_lights * sizeof(LightRec)) != 0);
-    ctx->lights.swap(next);
-    if (changed) {   // unchanged lights (set every frame) keep the running image
-        ctx->acc_count = 0;
-        ctx->scene_version++;
+    std::vector<LightRec> dir(num_lights);
+    for (uint32_t l = 0; l < num_lights; ++l) {
+        const bi_pt_dir_light& D = lights[l];
+        LightRec& r = dir[l];
+        r = LightRec{};
+        std::memcpy(r.emission, D.emission, sizeof r.emission);
+        r.sm_index = D.sm_index;
+        std::memcpy(r.direction, D.direction, sizeof r.direction);
+        r.shadow_strength = D.shadow_strength;
+        r.type = kLightDirectional;
     }
-    return BI_PT_OK;
+    std::vector<LightRec> point(ctx->point_lights);
+    return commit_lights(ctx, dir, point);
+}
+
+bi_pt_status bi_pt_set_point_lights(bi_pt_ctx* ctx, const bi_pt_point_light* lights, uint32_t num_lights) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if (num_lights > (uint32_t)kMaxDirLights)
+        return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 4 lights (directional and point together)");
+    if (num_lights && !lights) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null lights");
+    std::vector<LightRec> point(num_lights);
+    for (uint32_t l = 0; l < num_lights; ++l) {
+        const bi_pt_point_light& P = lights[l];
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(P.position[k]) || !std::isfinite(P.emission[k]) || !std::isfinite(P.direction[k]))
+                return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite point light");
+        LightRec& r = point[l];
+        r = LightRec{};
+        std::memcpy(r.emission, P.emission, sizeof r.emission);
+        r.sm_index = P.sm_index;
+        std::memcpy(r.direction, P.direction, sizeof r.direction);
+        r.shadow_strength = P.shadow_strength;
+        std::memcpy(r.position, P.position, sizeof r.position);
+        r.range_sqr_inv = P.range_sqr_inv;
+        r.cos_inner = P.cos_inner;
+        r.cos_outer = P.cos_outer;
+        r.type = kLightPoint;
+    }
+    std::vector<LightRec> dir(ctx->dir_lights);
+    return commit_lights(ctx, dir, point);
 }
 
 namespace {

@@ -318,8 +318,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     const uint32_t block = kn.block_threads == 128 || kn.block_threads == 512 ? kn.block_threads : 256u;
 
     // non-opaque triangles anywhere in the call select the traversal kernels with any-hit tests
-    bool any_hit = false;
-    for (uint32_t j = 0; j < n; ++j) any_hit = any_hit || prep[j].scene->non_opaque;
+    bool any_hit = false, any_point = false;   // point lights: shadow rays of finite extent
+    for (uint32_t j = 0; j < n; ++j) {
+        any_hit = any_hit || prep[j].scene->non_opaque;
+        for (const LightRec& L : prep[j].scene->lights) any_point = any_point || L.type == kLightPoint;
+    }
 
     // device memory: per-job images, BVH builds, wave buffers, plan
     bool any_cached = false, any_train = false;
@@ -539,7 +542,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     std::vector<uint64_t> key;
     if (graph) {
         key = {n, cap, lds_stack, refill, parts, block, (uint64_t)(uintptr_t)eng.plan.p, eng.wave_cap, eng.wave_slots,
-               eng.wave_sh_mult, plan_bytes};
+               eng.wave_sh_mult, plan_bytes, (uint64_t)any_hit, (uint64_t)any_point};
         for (uint32_t j = 0; j < n; ++j) {
             const Prep& p = prep[j];
             key.insert(key.end(), {(uint64_t)(uintptr_t)p.ctx, (uint64_t)(uintptr_t)p.scene,
@@ -683,7 +686,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sha.work = &cnt[kCntShadowWork];
                 sha.parts = parts;
                 sha.refill = refill;
-                sha.any_hit = any_hit ? 1u : 0u;
+                sha.any_hit = (any_hit || any_point) ? 1u : 0u;
                 sha.error = &cnt[kCntError];
                 launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, block, s);
                 ++launches_shadow;

@@ -378,6 +378,39 @@ PT_HD float power_heuristic(float a, float b) {
     return a2 / (a2 + b * b);
 }
 
+// -------------------------------------------------------------------- lights
+// Radiance arriving at P from light L and the unit direction towards it: a directional
+// light's emission along its direction; a point light's emission attenuated by distance
+// and spot cone (point_light_eval, lights.hlsl:15-25).
+PT_HD V3 light_eval(const LightRec& L, V3 P, V3* ldir) {
+    const V3 e = v3(L.emission[0], L.emission[1], L.emission[2]);
+    if (L.type != kLightPoint) {
+        *ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
+        return e;
+    }
+    const V3 lv = v3(L.position[0], L.position[1], L.position[2]) - P;
+    const float d2 = dot(lv, lv);
+    const float dist = __builtin_sqrtf(d2);
+    *ldir = v3(lv.x / dist, lv.y / dist, lv.z / dist);
+    const float q = d2 * L.range_sqr_inv;
+    float fall = 1.0f - q * q;
+    fall = fall < 0.0f ? 0.0f : (fall > 1.0f ? 1.0f : fall);
+    float att = fall / sel_max(d2, 0.001f);
+    if (L.cos_inner > L.cos_outer) {
+        float ct = dot(*ldir, v3(L.direction[0], L.direction[1], L.direction[2]));
+        ct = ct < L.cos_outer ? L.cos_outer : (ct > L.cos_inner ? L.cos_inner : ct);
+        att = att * ((ct - L.cos_outer) / sel_max(L.cos_inner - L.cos_outer, 0.001f));
+    }
+    return e * att;
+}
+// Extent of a shadow ray from P towards light L: unbounded for a directional light, the
+// distance to a point light.
+PT_HD float light_shadow_tmax(const LightRec& L, V3 P) {
+    if (L.type != kLightPoint) return kShadowTMax;
+    const V3 lv = v3(L.position[0], L.position[1], L.position[2]) - P;
+    return __builtin_sqrtf(dot(lv, lv));
+}
+
 // --------------------------------------------------------------------- sky
 PT_HD V3 sky_to_local(const SkyParams& s, V3 d) {
     const float* R = s.rot;

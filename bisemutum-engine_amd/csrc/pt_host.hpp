@@ -164,8 +164,8 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<float> srgb;
     uint32_t num_tex = 0;
     uint32_t tex_refs = 0;          // 1 + the largest texture index a material names (0: none)
-    // lights
-    std::vector<bipt::LightRec> lights;
+    // lights: the directional ones, then the point ones (the records the kernels read)
+    std::vector<bipt::LightRec> lights, dir_lights, point_lights;
     // sky
     bipt::DevBuf<float4> sky_tex;
     bipt::DevBuf<float> sky_w, sky_ccdf, sky_rows, sky_marg;

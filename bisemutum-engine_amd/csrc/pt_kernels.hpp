@@ -127,7 +127,7 @@ struct ShadowArgs {
     uint32_t* work;             // part counters of the persistent lanes (zero at launch)
     uint32_t parts;             // queue parts (1..kWorkPartsMax)
     uint32_t refill;            // idle lanes before a refill (1..64)
-    uint32_t any_hit;           // nonzero: some scene of the call has non-opaque triangles
+    uint32_t any_hit;           // nonzero: some scene of the call has non-opaque triangles or point lights
     uint32_t* error;
 };
 void launch_trace_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds_stack, uint32_t block, hipStream_t s);

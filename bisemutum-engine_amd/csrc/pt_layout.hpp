@@ -122,14 +122,20 @@ struct alignas(16) LeafTan {
     float t[12];
 };
 
-struct LightRec {      // = bi_pt_dir_light (DirLightData)
+// A light of the job: directional (bi_pt_dir_light = DirLightData) or point / spot
+// (bi_pt_point_light = PointLightData), the fields the shading reads.  Directional
+// lights come first, then point lights; at most kMaxDirLights in all.
+constexpr uint32_t kLightDirectional = 0u, kLightPoint = 1u;
+struct LightRec {
     float emission[3];
-    int32_t sm_index;
-    float direction[3];
+    int32_t sm_index;       // < 0: no shadow ray
+    float direction[3];     // directional: towards the light; point: the spot axis, towards the light
     float shadow_strength;
-    float cascade[4];
-    float depth_bias, normal_bias;
-    float pad[2];
+    float position[3];      // point lights
+    float range_sqr_inv;
+    float cos_inner, cos_outer;
+    uint32_t type;          // kLightDirectional / kLightPoint
+    float pad;
 };
 static_assert(sizeof(LightRec) == 64, "light record is 64 bytes");
 

@@ -198,8 +198,8 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
         for (uint32_t l = 0; l < NL; ++l) {
             if (l >= nl) break;
             const LightRec& L = J.lights[l];
-            V3 le = v3(L.emission[0], L.emission[1], L.emission[2]);
-            V3 ldir = v3(L.direction[0], L.direction[1], L.direction[2]);
+            V3 ldir;
+            const V3 le = light_eval(L, P, &ldir);
             V3 f = bsdf_eval(fr, m, V, ldir);
             V3 c = (le * f) * beta;
             if (!any_pos(c)) continue;
@@ -282,9 +282,10 @@ __device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, 
         a.sq.ox[o] = out.P.x; a.sq.oy[o] = out.P.y; a.sq.oz[o] = out.P.z;
         if (sky) {
             a.sq.dx[o] = out.sky.x; a.sq.dy[o] = out.sky.y; a.sq.dz[o] = out.sky.z;
-        } else {
-            const float* ld = J.lights[k].direction;
-            a.sq.dx[o] = ld[0]; a.sq.dy[o] = ld[1]; a.sq.dz[o] = ld[2];
+        } else {   // the light's direction, recomputed from the light as the shading computed it
+            V3 ld;
+            (void)light_eval(J.lights[k], out.P, &ld);
+            a.sq.dx[o] = ld.x; a.sq.dy[o] = ld.y; a.sq.dz[o] = ld.z;
         }
         a.sq.cx[o] = out.c[k].x; a.sq.cy[o] = out.c[k].y; a.sq.cz[o] = out.c[k].z;
         a.sq.pk[o] = (out.path << 3) | (sky ? kKindSky : k);

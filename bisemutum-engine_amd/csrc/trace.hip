@@ -537,6 +537,8 @@ __device__ __forceinline__ bool apply_shadow(const ShadowArgs& a, uint32_t i, bo
     return kind == kKindSky;
 }
 
+// AH (the general variant): any-hit tests of non-opaque leaves and rays that end at a
+// point light; launched when a scene of the call has either.
 template <bool AH, int L, int B>
 __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
     __shared__ int lds[L][B];
@@ -544,7 +546,10 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
     __shared__ uint8_t s_qown[B / 64][kLeafQueue];
     __shared__ uint32_t s_occ[B / 64][64];
     const uint32_t count = *a.count_in;
-    const float tmin = kTMin, tmax = kShadowTMax;
+    const float tmin = kTMin;
+    // a ray's extent: unbounded towards the sky and directional lights, the distance to a
+    // point light (the general variant only; the other has no point lights)
+    float tmax = kShadowTMax;
     // Same iteration as the closest-hit kernel: box steps queue their leaves, the queue
     // is flushed when full, before a refill or when no lane is busy, and parked lanes
     // finish once their leaves are tested.  The first hit internal child is entered and
@@ -620,7 +625,8 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
                 const LeafGeo* og = shfl_ptr(geo, owner);
                 float t = 0.0f, un = 0.0f, vn = 0.0f, det = 1.0f;
                 const LeafRegs lr = valid ? load_leaf(og, leaf) : LeafRegs{};
-                bool cand = valid && tri_test(lr, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tmax;
+                const float tm = AH ? __shfl(tmax, owner) : kShadowTMax;
+                bool cand = valid && tri_test(lr, oo, dd, &t, &un, &vn, &det) && t >= tmin && t <= tm;
                 if constexpr (AH) {
                     const bool nop = cand && leaf_non_opaque(lr);
                     if (__ballot(nop) != 0ull) {
@@ -652,10 +658,16 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
                 jr = &J;
                 setup_steps = J.num_tris > 1u ? J.num_tris : 1u;
             };
-            if (a.jobs.num_jobs == 1u) setup(a.jobs.rec[0]);
-            else setup(a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, a.q.pk[got] >> 3)]);
+            const uint32_t pk = a.q.pk[got];
+            const JobRec& J = a.jobs.num_jobs == 1u ? a.jobs.rec[0]
+                                                    : a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, pk >> 3)];
+            setup(J);
             no = v3(a.q.ox[got], a.q.oy[got], a.q.oz[got]);
             nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
+            if constexpr (AH) {
+                const uint32_t kind = pk & 7u;
+                tmax = kind != kKindSky ? light_shadow_tmax(J.lights[kind], no) : kShadowTMax;
+            }
             next = 0;
         }
         cur_i = next != kDone ? next : 0;

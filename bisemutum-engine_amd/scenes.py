@@ -65,6 +65,7 @@ class Scene:
     mesh_tan: list = None   # per mesh: f32 [V, 4] tangents (xyz, bitangent sign), or None
     textures: list = dataclasses.field(default_factory=list)   # Texture records
     sky_layout: int = 0     # SKY_EQUIRECT: sky_rgba [H, W, 4]; SKY_CUBE: [6, S, S, 4] (+X, -X, +Y, -Y, +Z, -Z)
+    point_lights: np.ndarray = None   # f32 [P, 16] (bi_pt_point_light records), after the directional lights
 
     @property
     def num_tris(self) -> int:
@@ -122,6 +123,25 @@ def dir_light(emission, direction, cast_shadow=True, shadow_strength=1.0):
     rec[7] = shadow_strength
     rec.view(np.int32)[3] = 0 if cast_shadow else -1
     return rec
+
+
+def point_light(emission, position, radius=20.0, spot_axis=None, inner_deg=0.0, outer_deg=0.0, cast_shadow=True,
+                shadow_strength=1.0):
+    """A bi_pt_point_light record (PointLightData layout): range_sqr_inv = 1 / radius^2;
+    a spot when spot_axis (pointing back to the light) and inner < outer angles are given
+    (cos_inner > cos_outer), as LightsContext fills it (lights.cpp:130-142)."""
+    r = np.zeros(16, np.float32)
+    r[0:3] = emission
+    r[3] = 1.0 / (radius * radius)
+    r[4:7] = position
+    if spot_axis is not None:
+        a = np.asarray(spot_axis, np.float64)
+        r[8:11] = a / np.linalg.norm(a)
+        r[7] = math.cos(math.radians(inner_deg))
+        r[11] = math.cos(math.radians(outer_deg))
+    r[12:13].view(np.int32)[0] = 0 if cast_shadow else -1
+    r[13] = shadow_strength
+    return r
 
 
 def random_material(rng: Pcg32, two_sided=True):

@@ -325,6 +325,29 @@ bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* transforms_3x4
  * that is not set. */
 bi_pt_status bi_pt_set_textures(bi_pt_ctx* ctx, const bi_pt_texture* textures, uint32_t num_textures);
 bi_pt_status bi_pt_set_lights(bi_pt_ctx* ctx, const bi_pt_dir_light* lights, uint32_t num_lights);
+/* Byte-identical to PointLightData (shaders/renderer/lights_struct.hlsl:16-28, filled by
+ * LightsContext::collect_all_lights, src/renderer/context/lights.cpp:120-150).  64 bytes.
+ * Radiance at a point at distance d: emission * saturate(1 - (d^2 range_sqr_inv)^2) /
+ * max(d^2, 0.001), times the spot factor (clamp(dot(l, direction), cos_outer, cos_inner) -
+ * cos_outer) / max(cos_inner - cos_outer, 0.001) when cos_inner > cos_outer (l: unit
+ * vector to the light; direction: the spot axis pointing to the light) (point_light_eval,
+ * lights.hlsl:15-25).  sm_index < 0: no shadow; otherwise an exact shadow ray to the
+ * light's position replaces the cube shadow map, and an occluded sample keeps
+ * (1 - shadow_strength). */
+typedef struct bi_pt_point_light {
+    float emission[3];
+    float range_sqr_inv;
+    float position[3];
+    float cos_inner;
+    float direction[3];
+    float cos_outer;
+    int32_t sm_index;
+    float shadow_strength;
+    float shadow_depth_bias;
+    float shadow_normal_bias;
+} bi_pt_point_light;
+/* Point and spot lights; directional and point lights together are at most 4. */
+bi_pt_status bi_pt_set_point_lights(bi_pt_ctx* ctx, const bi_pt_point_light* lights, uint32_t num_lights);
 /* Equirect RGBA32F sky (host memory, row 0 = +Y pole), colour multiplier
  * (SkyboxSystem color*strength) and 3x3 row-major rotation applied to directions
  * before lookup (deferred_lighting_secondary.hlsl:25). */

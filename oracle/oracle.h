@@ -81,6 +81,9 @@ typedef struct ora_scene_desc {
     uint32_t num_textures;
     const ora_texture* textures;
     uint32_t sky_layout;    /* 0 equirect; 1 cube map: sky_w x sky_w faces, sky_h = 6 sky_w */
+    uint32_t num_point_lights;
+    const float* point_lights;   /* 16 floats each, layout of bi_pt_point_light; lights are the
+                                    directional ones then these, at most 4 in all */
 } ora_scene_desc;
 
 /* Camera: row-major 4x4 matrices (m[row*4+col]), as bi_pt_camera. */

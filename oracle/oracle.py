@@ -39,6 +39,7 @@ class SceneDesc(C.Structure):
         ("texcoords", _f32p), ("tangents", _f32p),
         ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
         ("sky_layout", C.c_uint32),
+        ("num_point_lights", C.c_uint32), ("point_lights", _f32p),
     ]
 
 
@@ -157,6 +158,8 @@ class OracleScene:
             inst_mat=np.ascontiguousarray(scene.instances["material"], np.uint32),
             materials=np.ascontiguousarray(scene.materials, np.float32),
             dir_lights=np.ascontiguousarray(scene.dir_lights, np.float32),
+            point_lights=np.ascontiguousarray(getattr(scene, "point_lights", None) if getattr(scene, "point_lights", None)
+                                              is not None else np.zeros((0, 16), np.float32), np.float32).reshape(-1, 16),
             sky=np.ascontiguousarray(scene.sky_rgba, np.float32),
         )
         k = self._keep
@@ -170,6 +173,8 @@ class OracleScene:
         d.materials = ptr(k["materials"], _f32p)
         d.num_dir_lights = len(scene.dir_lights)
         d.dir_lights = ptr(k["dir_lights"], _f32p)
+        d.num_point_lights = len(k["point_lights"])
+        d.point_lights = ptr(k["point_lights"], _f32p) if len(k["point_lights"]) else None
         layout = getattr(scene, "sky_layout", 0)
         if layout == 1:   # cube faces [6, S, S, 4]: rows face-major
             if k["sky"].size and (k["sky"].ndim != 4 or k["sky"].shape[0] != 6 or k["sky"].shape[1] != k["sky"].shape[2]):

@@ -1161,6 +1161,27 @@ struct CacheMode {
     uint32_t suffix_key = 0;
 };
 
+// point_light_eval (lights.hlsl:15-25) of a bi_pt_point_light record L at P: the radiance
+// arriving, the unit direction to the light and the distance to it (the shadow ray's extent).
+F3 point_light_eval(const float* L, F3 P, F3* ldir, float* dist_out) {
+    const F3 lv = f3(L[4] - P.x, L[5] - P.y, L[6] - P.z);
+    const float d2 = dot(lv, lv);
+    const float dist = std::sqrt(d2);
+    *ldir = f3(lv.x / dist, lv.y / dist, lv.z / dist);
+    *dist_out = dist;
+    const float q = d2 * L[3];
+    float fall = 1.0f - q * q;
+    fall = fall < 0.0f ? 0.0f : (fall > 1.0f ? 1.0f : fall);
+    float att = fall / fmax_(d2, 0.001f);
+    const float cos_inner = L[7], cos_outer = L[11];
+    if (cos_inner > cos_outer) {
+        float ct = dot(*ldir, f3(L[8], L[9], L[10]));
+        ct = ct < cos_outer ? cos_outer : (ct > cos_inner ? cos_inner : ct);
+        att = att * ((ct - cos_outer) / fmax_(cos_inner - cos_outer, 0.001f));
+    }
+    return f3(L[0] * att, L[1] * att, L[2] * att);
+}
+
 // The surface a ray of direction d sees at hit h of triangle tri, point P
 // (hits/rt_gbuffer_hit.hlsl:6-17): interpolated normal, the material (a checkerboard's
 // cell by P), texture-driven fields, the normal map, and the two-sided flip.
@@ -1213,7 +1234,8 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
     const uint32_t M = st.max_bounces < 2 ? 2 : (st.max_bounces > 16 ? 16 : st.max_bounces);
     const uint32_t D = M - 1;
     const bool sky_nee = (st.flags & 2u) == 0 && s.sky_total > 0.0f;
-    const uint32_t nl = s.num_dir_lights < (uint32_t)kMaxDirLights ? s.num_dir_lights : kMaxDirLights;
+    const uint32_t ndir = s.num_dir_lights < (uint32_t)kMaxDirLights ? s.num_dir_lights : kMaxDirLights;
+    const uint32_t nl = ndir + s.num_point_lights < (uint32_t)kMaxDirLights ? ndir + s.num_point_lights : kMaxDirLights;
     uint32_t px = pixel % cam.width, py = pixel / cam.width;
     float jx = 0.5f, jy = 0.5f;
     if (st.flags & 1u) {
@@ -1273,24 +1295,37 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         const bool suffix = cm && cm->train && (int)b >= cm->bounce;
         Pcg rng = pcg_keyed(suffix ? st.seed ^ cm->suffix_key : st.seed, pixel, sample, b);
         float u0 = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng), u3 = pcg_float(rng);
-        // directional lights (deferred_lighting_secondary.hlsl:45-60) with shadow rays
+        // directional lights (deferred_lighting_secondary.hlsl:45-60), then point lights
+        // (:61-70), with shadow rays: unbounded towards a directional light, up to the
+        // light's position for a point light
         for (uint32_t l = 0; l < nl; ++l) {
-            const float* L = &s.dir_lights[16 * l];
-            F3 le = f3(L[0], L[1], L[2]);
-            F3 ldir = f3(L[4], L[5], L[6]);
+            F3 ldir;
+            float tmax = kShadowTMax, strength;
+            int sm_index;
+            F3 le;
+            if (l < ndir) {
+                const float* L = &s.dir_lights[16 * l];
+                le = f3(L[0], L[1], L[2]);
+                ldir = f3(L[4], L[5], L[6]);
+                std::memcpy(&sm_index, &L[3], sizeof sm_index);
+                strength = L[7];
+            } else {
+                const float* L = &s.point_lights[16 * (l - ndir)];
+                le = point_light_eval(L, P, &ldir, &tmax);
+                std::memcpy(&sm_index, &L[12], sizeof sm_index);
+                strength = L[13];
+            }
             F3 f = bsdf_eval(fr, m, V, ldir);
             F3 c = (le * f) * beta;
             if (!any_pos(c)) continue;
-            int sm_index;
-            std::memcpy(&sm_index, &L[3], sizeof sm_index);
             if (sm_index < 0) { acc_dir[l] = acc_dir[l] + c; continue; }
             out.shadow_bits[b] |= (uint8_t)(1u << l);
             out.n_light_shadow++;
-            Ray sr{P, ldir, kTMin, kShadowTMax, st.seed};
+            Ray sr{P, ldir, kTMin, tmax, st.seed};
             if (!trace_any(s, sr)) {
                 acc_dir[l] = acc_dir[l] + c;
             } else {
-                float keep = 1.0f - L[7];
+                float keep = 1.0f - strength;
                 if (keep > 0.0f) acc_dir[l] = acc_dir[l] + c * keep;
             }
         }

@@ -153,6 +153,9 @@ ora_ctx* ora_create(const ora_scene_desc* d) {
     }
     s->num_dir_lights = d->num_dir_lights;
     s->dir_lights.assign(d->dir_lights, d->dir_lights + 16 * (size_t)d->num_dir_lights);
+    s->num_point_lights = d->point_lights ? d->num_point_lights : 0;
+    if (s->num_point_lights)
+        s->point_lights.assign(d->point_lights, d->point_lights + 16 * (size_t)d->num_point_lights);
     s->sky_w = d->sky_rgba ? d->sky_w : 0;
     s->sky_h = d->sky_rgba ? d->sky_h : 0;
     s->sky_layout = d->sky_rgba ? d->sky_layout : 0;

@@ -43,6 +43,8 @@ struct Scene {
     float srgb[256] = {};
     uint32_t num_dir_lights = 0;
     std::vector<float> dir_lights; // 16 per light
+    uint32_t num_point_lights = 0;
+    std::vector<float> point_lights;   // 16 per light (layout of bi_pt_point_light); after the directional ones
     // sky
     uint32_t sky_w = 0, sky_h = 0;
     uint32_t sky_layout = 0;       // 0 equirect (W x H), 1 cube map (six W x W faces, H = 6 W)
