@@ -215,6 +215,9 @@ def main():
     }
 
     ref = fp64_reference(scene) if not args.no_rmse else None
+    if ref is None and not args.no_rmse:
+        log("rmse and nrc legs skipped: the FP64 reference fixture is missing or was made for another scene "
+            "(python tests/golden/make_c1_fp64.py writes it)")
     if ref is not None:
         result["rmse_vs_fp64"] = rmse_leg(pt, cam, cfg, ref, args.rmse_frames)
         if not args.no_nrc:

@@ -171,7 +171,7 @@ bi_pt_status bi_pt_batch_autotune(bi_pt_batch* b, const bi_pt_job* jobs, uint32_
     bi_pt_status status = time_knobs(best, &best_ms);
     const uint32_t waves[] = {1u << 23, 1u << 24, 1u << 25};
     const uint32_t stacks[] = {8u, 12u, 16u};
-    const uint32_t refills[] = {24u, 32u, 48u};
+    const uint32_t refills[] = {32u, 44u, 52u};
     const uint32_t blocks[] = {128u, 256u, 512u};
     for (int axis = 0; axis < 4 && status == BI_PT_OK; ++axis) {
         for (int c = 0; c < 3 && status == BI_PT_OK; ++c) {

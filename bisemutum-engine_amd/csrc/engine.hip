@@ -313,7 +313,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
     // 12 LDS entries: the traversal kernels reach 6 (closest hit) and 7 (shadow) wavefronts per
     // SIMD, against 5 and 6 with 16 (C1: 71.6 -> 70.3 ms per frame)
     const uint32_t lds_stack = kn.stack_lds ? std::min(kn.stack_lds, 32u) : 12u;
-    const uint32_t refill = kn.refill_lanes ? std::min(kn.refill_lanes, 64u) : 32u;
+    const uint32_t refill = kn.refill_lanes ? std::min(kn.refill_lanes, 64u) : 44u;
     const uint32_t parts = kn.work_parts ? std::min(kn.work_parts, kWorkPartsMax) : kWorkParts;
     const uint32_t block = kn.block_threads == 128 || kn.block_threads == 512 ? kn.block_threads : 256u;
 
