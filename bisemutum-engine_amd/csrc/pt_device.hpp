@@ -403,12 +403,18 @@ PT_HD V3 light_eval(const LightRec& L, V3 P, V3* ldir) {
     }
     return e * att;
 }
-// Extent of a shadow ray from P towards light L: unbounded for a directional light, the
-// distance to a point light.
-PT_HD float light_shadow_tmax(const LightRec& L, V3 P) {
-    if (L.type != kLightPoint) return kShadowTMax;
+// Shadow ray from P towards light L: the direction light_eval gives, and the extent:
+// unbounded for a directional light, the distance to a point light.
+PT_HD void light_shadow_ray(const LightRec& L, V3 P, V3* dir, float* tmax) {
+    if (L.type != kLightPoint) {
+        *dir = v3(L.direction[0], L.direction[1], L.direction[2]);
+        *tmax = kShadowTMax;
+        return;
+    }
     const V3 lv = v3(L.position[0], L.position[1], L.position[2]) - P;
-    return __builtin_sqrtf(dot(lv, lv));
+    const float dist = __builtin_sqrtf(dot(lv, lv));
+    *dir = v3(lv.x / dist, lv.y / dist, lv.z / dist);
+    *tmax = dist;
 }
 
 // --------------------------------------------------------------------- sky

@@ -271,6 +271,8 @@ struct alignas(16) HitRec {
 struct HitBuf {
     HitRec* rec;
 };
+// A shadow ray's direction is stored for sky rays only; a light's ray takes it from the
+// light (a directional light's direction, or from the origin towards a point light).
 struct ShadowQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz, *cx, *cy, *cz;
     uint32_t* pk;   // path << 3 | kind

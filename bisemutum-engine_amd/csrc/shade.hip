@@ -267,8 +267,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
 // Writes an entry's rays: the extension ray at queue place e, its shadow rays from
 // place o on in kind order (lights 0..3, then the sky).
 template <uint32_t NL>
-__device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, const ShadeOut<NL>& out, uint32_t e,
-                                           uint32_t o) {
+__device__ __forceinline__ void emit_entry(const ShadeArgs& a, const ShadeOut<NL>& out, uint32_t e, uint32_t o) {
     const uint32_t mask = out.mask;
     if (mask & kMaskExt) {
         a.qout.ox[e] = out.P.x; a.qout.oy[e] = out.P.y; a.qout.oz[e] = out.P.z;
@@ -280,12 +279,8 @@ __device__ __forceinline__ void emit_entry(const ShadeArgs& a, const JobRec& J, 
         const bool sky = k == NL;
         if (!(mask & (2u << (sky ? (uint32_t)kMaxDirLights : k)))) continue;
         a.sq.ox[o] = out.P.x; a.sq.oy[o] = out.P.y; a.sq.oz[o] = out.P.z;
-        if (sky) {
+        if (sky) {   // a light's ray takes its direction from the light in the shadow kernel
             a.sq.dx[o] = out.sky.x; a.sq.dy[o] = out.sky.y; a.sq.dz[o] = out.sky.z;
-        } else {   // the light's direction, recomputed from the light as the shading computed it
-            V3 ld;
-            (void)light_eval(J.lights[k], out.P, &ld);
-            a.sq.dx[o] = ld.x; a.sq.dy[o] = ld.y; a.sq.dz[o] = ld.z;
         }
         a.sq.cx[o] = out.c[k].x; a.sq.cy[o] = out.c[k].y; a.sq.cz[o] = out.c[k].z;
         a.sq.pk[o] = (out.path << 3) | (sky ? kKindSky : k);
@@ -394,7 +389,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a) {
     __syncthreads();
     if (out.mask) {
         const uint64_t base = s_base;
-        emit_entry(a, *J, out, lb_ext(base) + (excl >> 16), lb_sh(base) + (excl & 0xffffu));
+        emit_entry(a, out, lb_ext(base) + (excl >> 16), lb_sh(base) + (excl & 0xffffu));
     }
 }
 

@@ -663,10 +663,18 @@ __global__ __launch_bounds__(B) void k_trace_shadow(ShadowArgs a) {
                                                     : a.jobs.rec[job_of(a.jobs.path_base, a.jobs.num_jobs, pk >> 3)];
             setup(J);
             no = v3(a.q.ox[got], a.q.oy[got], a.q.oz[got]);
-            nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
-            if constexpr (AH) {
-                const uint32_t kind = pk & 7u;
-                tmax = kind != kKindSky ? light_shadow_tmax(J.lights[kind], no) : kShadowTMax;
+            // a sky ray's direction is in the queue; a light's ray is rebuilt from the
+            // light as the shading built it (without point lights, every light is
+            // directional and every extent unbounded)
+            const uint32_t kind = pk & 7u;
+            if (kind == kKindSky) {
+                nd = v3(a.q.dx[got], a.q.dy[got], a.q.dz[got]);
+                if constexpr (AH) tmax = kShadowTMax;
+            } else if constexpr (AH) {
+                light_shadow_ray(J.lights[kind], no, &nd, &tmax);
+            } else {
+                const LightRec& Lk = J.lights[kind];
+                nd = v3(Lk.direction[0], Lk.direction[1], Lk.direction[2]);
             }
             next = 0;
         }
