@@ -259,8 +259,10 @@ __device__ __forceinline__ uint32_t append_leaves(uint32_t* qleaf, uint8_t* qown
 
 // AH: the call's scenes hold non-opaque (alpha-tested or translucent) triangles, whose
 // hits pass the any-hit test; without them the test and its registers are compiled out.
-template <bool PRIMARY, bool AH, int L, int B>
-__global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
+// ONE: every ray of the launch belongs to job 0, so the scene pointers are uniform
+// (scalar registers) instead of per lane.
+template <bool PRIMARY, bool AH, int L, int B, bool ONE>
+__device__ __forceinline__ void trace_closest_body(const TraceArgs& a) {
     __shared__ int lds[L][B];
     __shared__ uint32_t s_qleaf[B / 64][kLeafQueue];
     __shared__ uint8_t s_qown[B / 64][kLeafQueue];
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
                 const V3 oo = v3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
                 const V3 dd = v3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
                 const float tb = __shfl(tbest, owner);
-                const LeafGeo* og = shfl_ptr(geo, owner);
+                const LeafGeo* og = ONE ? geo : shfl_ptr(geo, owner);
                 // a hit counts only where the leaf's own box passes the slab test up to the
                 // hit distance (the acceptance rule that makes the closest hit independent
                 // of the order leaves are found in), and, for a non-opaque leaf, where its
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
                 if constexpr (AH) {
                     const bool nop = cand && leaf_non_opaque(lr);
                     if (__ballot(nop) != 0ull) {
-                        const JobRec* oj = shfl_ptr(jr, owner);
+                        const JobRec* oj = ONE ? jr : shfl_ptr(jr, owner);
                         if (nop) cand = leaf_present(*oj, leaf, un, vn, det, oo, dd);
                     }
                 }
@@ -413,9 +415,11 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
             // one job: its record sits at a uniform address (scalar loads); several: the
             // path's job is looked up per lane
             auto setup = [&](const JobRec& J, uint32_t local) {
-                nodes = J.nodes;
-                geo = J.leaf_geo;
-                jr = &J;
+                if constexpr (!ONE) {
+                    nodes = J.nodes;
+                    geo = J.leaf_geo;
+                    jr = &J;
+                }
                 setup_steps = J.num_tris > 1u ? J.num_tris : 1u;
                 tmax = J.ray_length;
                 if (PRIMARY) {
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
                     camera_ray(J.cam, k.pixel % J.cam.width, k.pixel / J.cam.width, jx, jy, &no, &nd);
                 }
             };
-            if (a.jobs.num_jobs == 1u) {
+            if (ONE || a.jobs.num_jobs == 1u) {
                 setup(a.jobs.rec[0], got);
             } else {
                 const uint32_t path = PRIMARY ? got : a.q.path[got];
@@ -455,6 +459,17 @@ __global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
         }
         if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
     }
+}
+
+template <bool PRIMARY, bool AH, int L, int B>
+__global__ __launch_bounds__(B) void k_trace_closest(TraceArgs a) {
+    trace_closest_body<PRIMARY, AH, L, B, false>(a);
+}
+// One job, opaque scene, 256-thread blocks and a 12-deep LDS stack (the defaults): the
+// scene pointers sit in scalar registers, which frees six vector registers per lane.
+template <bool PRIMARY>
+__global__ __launch_bounds__(256) void k_trace_closest_one(TraceArgs a) {
+    trace_closest_body<PRIMARY, false, 12, 256, true>(a);
 }
 
 // Any-hit traversal of one ray (inspection kernel).  The answer does not depend on the
@@ -741,6 +756,16 @@ void closest_lb(const TraceArgs& a, uint32_t capacity, hipStream_t s) {
     hipLaunchKernelGGL((k_trace_closest<P, AH, L, B>), dim3(groups), dim3(B), 0, s, a);
 }
 
+template <bool P>
+void closest_one(const TraceArgs& a, uint32_t capacity, hipStream_t s) {
+    uint32_t groups = (capacity + 255u) / 256u;
+    static uint32_t resident = 0;
+    if (resident == 0) resident = resident_groups(k_trace_closest_one<P>, 256);
+    if (groups > resident) groups = resident;
+    if (groups == 0) groups = 1;
+    hipLaunchKernelGGL((k_trace_closest_one<P>), dim3(groups), dim3(256), 0, s, a);
+}
+
 template <bool AH, int L, int B>
 void shadow_lb(const ShadowArgs& a, uint32_t capacity, hipStream_t s) {
     uint32_t groups = (capacity + B - 1u) / B;
@@ -789,6 +814,11 @@ void dispatch_shadow(const ShadowArgs& a, uint32_t capacity, uint32_t lds, uint3
 void launch_trace_closest(const TraceArgs& a, uint32_t capacity, bool primary, uint32_t lds_stack, uint32_t block,
                           hipStream_t s) {
     if (capacity == 0) return;
+    if (a.jobs.num_jobs == 1u && !a.any_hit && block == 256 && lds_stack > 8 && lds_stack <= 12) {
+        if (primary) closest_one<true>(a, capacity, s);
+        else closest_one<false>(a, capacity, s);
+        return;
+    }
     if (primary) {
         if (a.any_hit) dispatch_closest<true, true>(a, capacity, lds_stack, block, s);
         else dispatch_closest<true, false>(a, capacity, lds_stack, block, s);
