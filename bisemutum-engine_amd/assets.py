@@ -214,6 +214,8 @@ def submesh_attributes(mesh: MeshData):
 TEXTURE_TYPE = "Texture"
 # rhi::ResourceFormat values of the formats the pass samples (include/bisemutum/rhi/defines.hpp:44-)
 FORMAT_RGBA8_UNORM, FORMAT_RGBA8_SRGB, FORMAT_RGBA32_SFLOAT = 37, 43, 109
+FORMAT_RG32_SFLOAT = 103
+_DIM_2D, _DIM_3D = 1, 2
 _FILTER_LINEAR = 1
 _ADDRESS = {0: 0, 1: 1, 2: 2}   # repeat, mirrored repeat, clamp to edge (clamp to border / mirror clamp: no)
 
@@ -279,6 +281,57 @@ def read_texture(data: bytes, what: str = "texture") -> TextureData:
     else:
         raise AssetError(f"{what}: texture format {fmt} is not RGBA8 or RGBA32F")
     return TextureData(width, height, levels, fmt, sampler, texels)
+
+
+def read_texture3d(data: bytes, what: str = "texture") -> np.ndarray:
+    """Level 0 of a 3D Texture asset (the engine's LTC tables, lights.cpp:26-37): float32
+    texels [depth, height, width, channels] of an RGBA32F or RG32F texture (the header as
+    read_texture; extent.depth_or_layers is the depth)."""
+    r = _Reader(data, what)
+    magic = r.u32()
+    type_name = r.string()
+    version = r.u32()
+    if magic != ASSET_MAGIC or type_name != TEXTURE_TYPE:
+        raise AssetError(f"{what}: not a Texture asset (magic {magic:#010x}, type {type_name!r})")
+    r.take(28)
+    width, height, depth, _levels = struct.unpack("<4I", bytes(r.take(16)))
+    fmt, dim, _usages, _pad = bytes(r.take(4))
+    if dim != _DIM_3D:
+        raise AssetError(f"{what}: not a 3D texture (dimension {dim})")
+    if version == 1:
+        if r.u32() != 0:
+            raise AssetError(f"{what}: texture storage type is not raw")
+        body = r
+    elif version == 2:
+        length = r.u64()
+        compressed = r.u64()
+        raw = zlib.decompress(bytes(r.take(compressed)))
+        if len(raw) != length:
+            raise AssetError(f"{what}: compressed part holds {len(raw)} bytes, header says {length}")
+        body = _Reader(raw, what)
+    else:
+        raise AssetError(f"{what}: unknown Texture version {version}")
+    channels = {FORMAT_RGBA32_SFLOAT: 4, FORMAT_RG32_SFLOAT: 2}.get(fmt)
+    if channels is None:
+        raise AssetError(f"{what}: 3D texture format {fmt} is not RGBA32F or RG32F")
+    n = width * height * depth * channels
+    raw = body.vector("u1")
+    if len(raw) < 4 * n or n == 0:
+        raise AssetError(f"{what}: {len(raw)} texel bytes for {width}x{height}x{depth}x{channels} float32")
+    return np.frombuffer(raw[:4 * n].tobytes(), "<f4").reshape(depth, height, width, channels).copy()
+
+
+def read_ltc_tables(asset_dir: str):
+    """The engine's LTC tables (ltc_matrix_lut0..2 and ltc_norm_lut texture assets in
+    asset_dir), as bi_pt_set_ltc_luts takes them: three [4096, 4] and one [4096, 2]."""
+    out = []
+    for name in ("ltc_matrix_lut0", "ltc_matrix_lut1", "ltc_matrix_lut2", "ltc_norm_lut"):
+        with open(os.path.join(asset_dir, name + ".texture.biasset"), "rb") as f:
+            t = read_texture3d(f.read(), name)
+        if t.shape[:3] != (64, 8, 8):
+            raise AssetError(f"{name}: LTC tables are 8 x 8 x 64 texels, not {t.shape[2]}x{t.shape[1]}x{t.shape[0]}")
+        out.append(t.reshape(4096, t.shape[3]))
+    return tuple(out)
 
 
 def encode_texture(t: TextureData) -> bytes:

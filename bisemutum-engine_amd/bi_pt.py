@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
-           "bi_pt_set_lights", "bi_pt_set_point_lights", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_set_lights", "bi_pt_set_point_lights", "bi_pt_set_rect_lights", "bi_pt_set_ltc_luts", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
@@ -223,6 +223,8 @@ def lib():
         L.bi_pt_set_lights.argtypes = [vp, f32p, u32]
         L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
         L.bi_pt_set_point_lights.argtypes = [vp, f32p, u32]
+        L.bi_pt_set_rect_lights.argtypes = [vp, f32p, u32]
+        L.bi_pt_set_ltc_luts.argtypes = [vp, f32p, f32p, f32p, f32p]
         L.bi_pt_set_sky_cube.argtypes = [vp, f32p, u32, f32p, f32p]
         L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
         L.bi_pt_render_gbuffer.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, C.POINTER(GBufferC), vp]
@@ -378,6 +380,9 @@ class PathTracer:
             self.set_point_lights(None)   # the light list holds four lights, directional and point together
         self.set_lights(scene.dir_lights)
         self.set_point_lights(getattr(scene, "point_lights", None))
+        if getattr(scene, "ltc", None) is not None:
+            self.set_ltc_luts(*scene.ltc)
+        self.set_rect_lights(getattr(scene, "rect_lights", None))
         if getattr(scene, "sky_layout", 0) == 1:
             self.set_sky_cube(scene.sky_rgba, scene.sky_color, scene.sky_rot)
         else:
@@ -411,6 +416,18 @@ class PathTracer:
         lights = np.ascontiguousarray(lights if lights is not None else np.zeros((0, 16)), np.float32).reshape(-1, 16)
         self._check(lib().bi_pt_set_point_lights(self.h, _p(lights) if len(lights) else None, len(lights)))
         self._n_point_lights = len(lights)
+
+    def set_rect_lights(self, lights):
+        """bi_pt_set_rect_lights: records [R, 28] (scenes.rect_light), or None."""
+        lights = np.ascontiguousarray(lights if lights is not None else np.zeros((0, 28)), np.float32).reshape(-1, 28)
+        self._check(lib().bi_pt_set_rect_lights(self.h, _p(lights) if len(lights) else None, len(lights)))
+
+    def set_ltc_luts(self, m0, m1, m2, norm):
+        """bi_pt_set_ltc_luts: three [4096, 4] matrix tables and the [4096, 2] norm table."""
+        t = [np.ascontiguousarray(x, np.float32) for x in (m0, m1, m2, norm)]
+        if any(x.size != 4096 * 4 for x in t[:3]) or t[3].size != 4096 * 2:
+            raise ValueError("LTC tables are 8 x 8 x 64 texels")
+        self._check(lib().bi_pt_set_ltc_luts(self.h, *[_p(x) for x in t]))
 
     def set_lights(self, lights):
         lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)

@@ -252,6 +252,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     dfree(ctx, ctx->pos); dfree(ctx, ctx->nrm); dfree(ctx, ctx->idx); dfree(ctx, ctx->inst); dfree(ctx, ctx->mats);
+    dfree(ctx, ctx->rects); dfree(ctx, ctx->ltc);
     dfree(ctx, ctx->sky_tex); dfree(ctx, ctx->sky_w); dfree(ctx, ctx->sky_ccdf); dfree(ctx, ctx->sky_rows);
     dfree(ctx, ctx->sky_marg); dfree(ctx, ctx->sky_mguide); dfree(ctx, ctx->sky_rguide);
     dfree(ctx, ctx->tri_v); dfree(ctx, ctx->tri_n); dfree(ctx, ctx->cb_out); dfree(ctx, ctx->tri_mat);
@@ -543,6 +544,65 @@ bi_pt_status bi_pt_set_point_lights(bi_pt_ctx* ctx, const bi_pt_point_light* lig
     }
     std::vector<LightRec> dir(ctx->dir_lights);
     return commit_lights(ctx, dir, point);
+}
+
+bi_pt_status bi_pt_set_rect_lights(bi_pt_ctx* ctx, const bi_pt_rect_light* lights, uint32_t num_lights) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if (num_lights > kMaxRectLights) return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 16 rect lights");
+    if (num_lights && !lights) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null rect lights");
+    static_assert(sizeof(bi_pt_rect_light) == sizeof(RectRec), "bi_pt_rect_light is the kernels' record");
+    std::vector<RectRec> recs(num_lights);
+    for (uint32_t l = 0; l < num_lights; ++l) {
+        std::memcpy(&recs[l], &lights[l], sizeof(RectRec));
+        const RectRec& r = recs[l];
+        if (r.texture_index >= 0)
+            return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "textured rect lights are not supported (texture_index >= 0)");
+        const float* f[] = {r.emission, r.p0, r.p1, r.p2, r.p3};
+        for (const float* v : f)
+            for (int k = 0; k < 3; ++k)
+                if (!std::isfinite(v[k])) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite rect light");
+    }
+    const bool changed = recs.size() != ctx->rects_host.size() ||
+                         (!recs.empty() && std::memcmp(recs.data(), ctx->rects_host.data(), recs.size() * sizeof(RectRec)) != 0);
+    if (!changed) return BI_PT_OK;
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (num_lights) {
+        BI_TRY(dalloc(ctx, ctx->rects, num_lights));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->rects.p, recs.data(), num_lights * sizeof(RectRec), hipMemcpyHostToDevice));
+    }
+    ctx->rects_host.swap(recs);
+    ctx->acc_count = 0;
+    ctx->scene_version++;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_set_ltc_luts(bi_pt_ctx* ctx, const float* matrix_lut0, const float* matrix_lut1,
+                                const float* matrix_lut2, const float* norm_lut) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if (!matrix_lut0 || !matrix_lut1 || !matrix_lut2 || !norm_lut)
+        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null LTC table");
+    // four float4 tables: the matrix rows as given (RGBA32F), the norm table's RG32F pairs padded
+    std::vector<float> t((size_t)4 * kLtcTexels * 4, 0.0f);
+    const float* m[3] = {matrix_lut0, matrix_lut1, matrix_lut2};
+    for (int k = 0; k < 3; ++k) std::memcpy(&t[(size_t)k * kLtcTexels * 4], m[k], (size_t)kLtcTexels * 4 * sizeof(float));
+    for (uint32_t i = 0; i < kLtcTexels; ++i) {
+        t[((size_t)3 * kLtcTexels + i) * 4] = norm_lut[2 * i];
+        t[((size_t)3 * kLtcTexels + i) * 4 + 1] = norm_lut[2 * i + 1];
+    }
+    for (float v : t)
+        if (!std::isfinite(v)) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "non-finite LTC table entry");
+    if (t == ctx->ltc_host) return BI_PT_OK;
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    BI_TRY(dalloc(ctx, ctx->ltc, (size_t)4 * kLtcTexels));
+    BI_HIP_TRY(ctx, hipMemcpy(ctx->ltc.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->ltc_host.swap(t);
+    if (!ctx->rects_host.empty()) {
+        ctx->acc_count = 0;
+        ctx->scene_version++;
+    }
+    return BI_PT_OK;
 }
 
 namespace {

@@ -184,6 +184,7 @@ struct Prep {
     bi_pt_ctx* scene;     // scene, lights, sky, BVH
     CameraParams cam;
     uint32_t npix, M, D, nl, spp, sh, role;
+    uint32_t nrect;       // rect lights (their accumulator follows the sky's)
     uint32_t rounds;      // treelet rounds if this job's call builds its scene
     bool sky_nee, reset, rebuild;
     float* sums;          // running sums (training paths: the cache's targets)
@@ -247,6 +248,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         p.D = p.M - 1;
         p.scene = S;
         p.nl = (uint32_t)S->lights.size();
+        p.nrect = (uint32_t)S->rects_host.size();
+        if (p.nrect && S->ltc_host.empty())
+            return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "rect lights need the LTC tables (bi_pt_set_ltc_luts)");
         p.spp = st->spp;
         p.sky_nee = !(st->flags & BI_PT_FLAG_NO_SKY_NEE) && S->sky.total > 0.0f;
         p.sh = 1;
@@ -272,7 +276,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         for (uint32_t k = 0; k < j; ++k)
             if (prep[k].scene == S) p.rebuild = false;   // one build per scene and call
         max_npix = std::max(max_npix, p.npix);
-        slots = std::max(slots, p.nl + 2u);
+        slots = std::max(slots, p.nl + 2u + (p.nrect ? 1u : 0u));
         sh_mult = std::max(sh_mult, p.sh);
     }
 
@@ -399,6 +403,9 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
             r.keep[l] = 1.0f - S->lights[l].shadow_strength;
         }
         r.sky = S->sky;
+        r.rects = p.nrect ? S->rects.p : nullptr;
+        r.num_rects = p.nrect;
+        r.ltc = p.nrect ? S->ltc.p : nullptr;
         r.cam = p.cam;
         r.npix = p.npix;
         r.sample0 = (uint32_t)(p.reset ? 0 : ctx->acc_count) + sample_off;
@@ -598,10 +605,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         for (size_t w = 0; w < waves.size(); ++w) {
             const WaveJobs& J = wj[w];
             uint32_t D = 0, max_nl = 0;
-            bool wave_cached = false;
+            bool wave_cached = false, wave_rect = false;
             for (const Item& it : waves[w]) {
                 D = std::max(D, prep[it.job].D);
                 max_nl = std::max(max_nl, prep[it.job].nl);
+                wave_rect |= prep[it.job].nrect > 0;
                 wave_cached |= prep[it.job].role == kRoleCached;
             }
             if (record) {
@@ -662,7 +670,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sa.bounce = b;
                 const bool cache_now = wave_cached && b == nrc->cache_bounce;
                 sa.nrc_query = cache_now ? nrc->query.p : nullptr;
-                launch_shade(sa, J.num_paths, max_nl, s);
+                launch_shade(sa, J.num_paths, max_nl, wave_rect, s);
                 if (cache_now) {
                     // paths that ended in the cache at this segment: their radiance estimate
                     NrcInferArgs na{};

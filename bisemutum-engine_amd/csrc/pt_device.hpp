@@ -417,6 +417,195 @@ PT_HD void light_shadow_ray(const LightRec& L, V3 P, V3* dir, float* tmax) {
     *tmax = dist;
 }
 
+// --------------------------------------------------------------------- rect lights
+// Linearly transformed cosines over a quad (lights.hlsl:161-470, after Heitz et al. 2016,
+// "Real-Time Polygonal-Light Shading with Linearly Transformed Cosines", with anisotropic
+// 4D tables): the diffuse integral of the clipped quad under the plain cosine, the
+// specular one under the table's transform, both in the shading frame (t, b, n).
+struct M3 { V3 r0, r1, r2; };   // rows
+PT_HD V3 m3_mul(const M3& m, V3 v) { return v3(dot(m.r0, v), dot(m.r1, v), dot(m.r2, v)); }
+// Inverse by the adjugate: inv[i][j] = cofactor[j][i] / det, det expanded along row 0.
+PT_HD M3 m3_inverse(const M3& m) {
+    const float a = m.r0.x, b = m.r0.y, c = m.r0.z, d = m.r1.x, e = m.r1.y, f = m.r1.z, g = m.r2.x, h = m.r2.y,
+                k = m.r2.z;
+    const float A = e * k - f * h, B = f * g - d * k, Cc = d * h - e * g;
+    const float det = (a * A + b * B) + c * Cc;
+    const float s = 1.0f / det;
+    M3 r;
+    r.r0 = v3(A * s, (c * h - b * k) * s, (b * f - c * e) * s);
+    r.r1 = v3(B * s, (a * k - c * g) * s, (c * d - a * f) * s);
+    r.r2 = v3(Cc * s, (b * g - a * h) * s, (a * e - b * d) * s);
+    return r;
+}
+
+// Texel coordinate of a normalised coordinate on an axis of n texels (linear filter,
+// clamp to edge): the two texels and the weight of the second.
+PT_HD void ltc_axis(float u, int n, int* i0, int* i1, float* f) {
+    const float t = u * (float)n - 0.5f;
+    const float fl = __builtin_floorf(t);
+    *f = t - fl;
+    const int a = (int)fl, b = a + 1;
+    *i0 = a < 0 ? 0 : (a > n - 1 ? n - 1 : a);
+    *i1 = b < 0 ? 0 : (b > n - 1 ? n - 1 : b);
+}
+PT_HD float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+PT_HD V4 lerp4(V4 a, V4 b, float t) {
+    return V4{lerpf(a.x, b.x, t), lerpf(a.y, b.y, t), lerpf(a.z, b.z, t), lerpf(a.w, b.w, t)};
+}
+PT_HD V4 ld4(const float4* t, int i) { const float4 v = t[i]; return V4{v.x, v.y, v.z, v.w}; }
+// Trilinear sample of an 8 x 8 x 64 table (x fastest) at normalised (x, y, z).
+PT_HD V4 ltc_sample(const float4* t, float x, float y, float z) {
+    int x0, x1, y0, y1, z0, z1;
+    float fx, fy, fz;
+    ltc_axis(x, (int)kLtcSize, &x0, &x1, &fx);
+    ltc_axis(y, (int)kLtcSize, &y0, &y1, &fy);
+    ltc_axis(z, (int)kLtcDepth, &z0, &z1, &fz);
+    const int s = (int)kLtcSize, s2 = s * s;
+    V4 e[2];
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {   // the two depth slices, each bilinear in x and y
+        const int zo = s2 * (k ? z1 : z0);
+        const V4 c0 = lerp4(ld4(t, x0 + s * y0 + zo), ld4(t, x1 + s * y0 + zo), fx);
+        const V4 c1 = lerp4(ld4(t, x0 + s * y1 + zo), ld4(t, x1 + s * y1 + zo), fx);
+        e[k] = lerp4(c0, c1, fy);
+    }
+    return lerp4(e[0], e[1], fz);
+}
+// The tables' transform and (norm, Fresnel) pair at table coordinate (u3, u2, u1, u0):
+// u0 (the major roughness) selects two groups of 8 depth slices, blended by its
+// fraction (get_ltc_tex3d_coord, fetch_ltc_matrix, fetch_ltc_brdf: lights.hlsl:164-201).
+PT_HD void ltc_fetch(const float4* ltc, float u3, float u2, float u1, float u0, M3* m, float* bx, float* by) {
+    const float ws = u0 * 7.0f;
+    const float ws_f = __builtin_floorf(ws);
+    const float wc = __builtin_floorf(ws + 1.0f);
+    const float ws_c = wc < 7.0f ? wc : 7.0f;
+    const float w = ws - ws_f;
+    const float x = (u3 * 7.0f + 0.5f) / 8.0f, y = (u2 * 7.0f + 0.5f) / 8.0f;
+    const float z1 = ((u1 * 7.0f + 8.0f * ws_f) + 0.5f) / 64.0f, z2 = ((u1 * 7.0f + 8.0f * ws_c) + 0.5f) / 64.0f;
+    V3 r[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const V4 a = ltc_sample(ltc + k * (int)kLtcTexels, x, y, z1), b = ltc_sample(ltc + k * (int)kLtcTexels, x, y, z2);
+        r[k] = v3(lerpf(a.x, b.x, w), lerpf(a.y, b.y, w), lerpf(a.z, b.z, w));
+    }
+    m->r0 = r[0]; m->r1 = r[1]; m->r2 = r[2];
+    const V4 n1 = ltc_sample(ltc + 3 * (int)kLtcTexels, x, y, z1), n2 = ltc_sample(ltc + 3 * (int)kLtcTexels, x, y, z2);
+    *bx = lerpf(n1.x, n2.x, w);
+    *by = lerpf(n1.y, n2.y, w);
+}
+PT_HD void quad_reverse(V3 q[4]) {
+    const V3 a = q[0], b = q[1];
+    q[0] = q[3]; q[1] = q[2]; q[2] = b; q[3] = a;
+}
+// get_ltc_matrix_and_brdf (lights.hlsl:203-255): the table coordinate of the view
+// direction and roughness, folded into the first quadrant of azimuth and the larger
+// roughness first, the transform unfolded by row flips; reflections reverse the quad.
+PT_HD void ltc_matrix_brdf(const float4* ltc, V3 lv, float rx, float ry, V3 q[4], M3* m, float* bx, float* by) {
+    const float theta = pt_acos(lv.z);
+    const bool flip_r = ry > rx;
+    float phi = pt_atan2(lv.y, lv.x);
+    phi = flip_r ? kPi * 0.5f - phi : phi;
+    phi = phi >= 0.0f ? phi : phi + 2.0f * kPi;
+    const float u0 = sel_max((flip_r ? ry : rx) - 0.001f, 0.0f) / (1.0f - 0.001f);
+    const float u1 = flip_r ? rx / ry : ry / rx;
+    const float u2 = theta / (kPi * 0.5f);
+    if (phi < kPi * 0.5f) {
+        ltc_fetch(ltc, phi / (kPi * 0.5f), u2, u1, u0, m, bx, by);
+    } else if (phi < kPi) {
+        ltc_fetch(ltc, (kPi - phi) / (kPi * 0.5f), u2, u1, u0, m, bx, by);
+        quad_reverse(q);
+        m->r0 = -m->r0;
+    } else if (phi < 1.5f * kPi) {
+        ltc_fetch(ltc, (phi - kPi) / (kPi * 0.5f), u2, u1, u0, m, bx, by);
+        m->r0 = -m->r0;
+        m->r1 = -m->r1;
+    } else {
+        ltc_fetch(ltc, (2.0f * kPi - phi) / (kPi * 0.5f), u2, u1, u0, m, bx, by);
+        quad_reverse(q);
+        m->r1 = -m->r1;
+    }
+    if (flip_r) {
+        quad_reverse(q);
+        const V3 t = m->r0;
+        m->r0 = m->r1;
+        m->r1 = t;
+    }
+}
+// Edge term of the polygon integral (ltc_integrate_edge, lights.hlsl:347-361): the z of
+// cross(v1, v2) * theta / sin(theta), theta / sin(theta) by its rational fit.
+PT_HD float ltc_edge(V3 v1, V3 v2) {
+    const float x = dot(v1, v2);
+    const float y = x < 0.0f ? -x : x;
+    const float a = 5.42031f + (3.12829f + 0.0902326f * y) * y;
+    const float b = 3.45068f + (4.18814f + y) * y;
+    float t = a / b;
+    if (x < 0.0f) t = kPi / __builtin_sqrtf(1.0f - x * x) - t;
+    return (v1.x * v2.y - v1.y * v2.x) * t;
+}
+// ltc_integrate (lights.hlsl:362-400) without the light texture's direction: the quad q
+// relative to P in the frame (t, b, n), transformed by minv (none when null), clipped to
+// z > 0 (a vertex pair across the plane contributes the point in.z * out - out.z * in),
+// projected onto the sphere and integrated edge by edge; one-sided lights count only
+// positive integrals.
+PT_HD float ltc_integrate(V3 P, const Frame& fr, const M3* minv, const V3 q[4], bool two_sided) {
+    V3 l[4];
+    uint32_t inside = 0;
+    for (int k = 0; k < 4; ++k) {
+        const V3 r = q[k] - P;
+        V3 v = v3(dot(fr.t, r), dot(fr.b, r), dot(fr.n, r));
+        if (minv) v = m3_mul(*minv, v);
+        l[k] = v;
+        inside |= (v.z > 0.0f ? 1u : 0u) << k;
+    }
+    if (inside == 0u || inside == 5u || inside == 10u) return 0.0f;   // below, or a diagonal pair
+    V3 c[5];
+    int n = 0;
+    for (int k = 0; k < 4; ++k) {
+        const V3 a = l[k], b = l[(k + 1) & 3];
+        const bool ai = (inside >> k) & 1u, bi = (inside >> ((k + 1) & 3)) & 1u;
+        if (ai) c[n++] = a;
+        if (ai != bi) {
+            const V3 in = ai ? a : b, out = ai ? b : a;
+            c[n++] = v3(in.z * out.x - out.z * in.x, in.z * out.y - out.z * in.y, in.z * out.z - out.z * in.z);
+        }
+    }
+    for (int k = 0; k < n; ++k) c[k] = normalize(c[k]);
+    float sum = 0.0f;
+    for (int k = 0; k < n; ++k) sum = sum + ltc_edge(c[k], c[k + 1 < n ? k + 1 : 0]);
+    float r = two_sided ? (sum < 0.0f ? -sum : sum) : sel_max(sum, 0.0f);
+    if (!(r - r == 0.0f)) r = 0.0f;   // NaN or infinite
+    return r;
+}
+// A rect light's contribution at a lit surface (rect_light_eval_ltc, lights.hlsl:
+// 471-513, then surface_eval_lut_lit, material/lit.hlsl:37-58): the diffuse integral
+// times (1 - F) base / pi plus the specular one times f0 * norm + f90 * Fresnel term.
+// (Called, not inlined: its table reads and two polygon integrals would otherwise share
+// the shading kernel's registers.)
+static __device__ __noinline__ V3 rect_light_lit(const RectRec& R, const float4* ltc, V3 P, const Frame& fr, V3 V,
+                                                 const Mat& m) {
+    const V3 lv = v3(dot(V, fr.t), dot(V, fr.b), dot(V, fr.n));
+    if (lv.z <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    V3 q[4] = {v3(R.p3[0], R.p3[1], R.p3[2]), v3(R.p2[0], R.p2[1], R.p2[2]), v3(R.p1[0], R.p1[1], R.p1[2]),
+               v3(R.p0[0], R.p0[1], R.p0[2])};
+    const bool two = R.two_sided != 0u;
+    const V3 e = v3(R.emission[0], R.emission[1], R.emission[2]);
+    const float i_d = ltc_integrate(P, fr, nullptr, q, two);
+    M3 mt;
+    float bx, by;
+    ltc_matrix_brdf(ltc, lv, m.rx, m.ry, q, &mt, &bx, &by);
+    const M3 mi = m3_inverse(mt);
+    const float i_s = ltc_integrate(P, fr, &mi, q, two);
+    const float ndotv = dot(fr.n, V);
+    if (ndotv <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    const float x = 1.0f - ndotv;
+    const float p5 = ((x * x) * (x * x)) * x;
+    const V3 fres = m.f0 + (m.f90 - m.f0) * p5;
+    const V3 one = v3(1.0f, 1.0f, 1.0f);
+    const V3 diffuse = ((one - fres) * m.base) * kInvPi;
+    const V3 specular = m.f0 * bx + m.f90 * by;
+    return diffuse * (e * i_d) + specular * (e * i_s);
+}
+
 // --------------------------------------------------------------------- sky
 PT_HD V3 sky_to_local(const SkyParams& s, V3 d) {
     const float* R = s.rot;

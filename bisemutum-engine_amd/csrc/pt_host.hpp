@@ -166,6 +166,11 @@ struct bi_pt_ctx : bipt::Owner {
     uint32_t tex_refs = 0;          // 1 + the largest texture index a material names (0: none)
     // lights: the directional ones, then the point ones (the records the kernels read)
     std::vector<bipt::LightRec> lights, dir_lights, point_lights;
+    // rect lights and the LTC tables they are integrated with (4 * kLtcTexels texels)
+    std::vector<bipt::RectRec> rects_host;
+    bipt::DevBuf<bipt::RectRec> rects;
+    bipt::DevBuf<float4> ltc;
+    std::vector<float> ltc_host;
     // sky
     bipt::DevBuf<float4> sky_tex;
     bipt::DevBuf<float> sky_w, sky_ccdf, sky_rows, sky_marg;

@@ -158,8 +158,9 @@ struct ShadeArgs {
 // covers (count_in entries, or num_entries if count_in is null).
 void launch_shade_reset(uint64_t* tile_state, const uint32_t* count_in, uint32_t num_entries, uint32_t* count_ext,
                         uint32_t* count_shadow, uint32_t capacity, hipStream_t s);
-// max_lights: the most directional lights of any job in the launch (at most kMaxDirLights).
-void launch_shade(const ShadeArgs& a, uint32_t capacity, uint32_t max_lights, hipStream_t s);
+// max_lights: the most lights of any job in the launch (at most kMaxDirLights); rect: some
+// job of the launch has rect lights.
+void launch_shade(const ShadeArgs& a, uint32_t capacity, uint32_t max_lights, bool rect, hipStream_t s);
 
 struct ResolveArgs {
     WaveJobs jobs;

@@ -26,7 +26,7 @@
 namespace bipt {
 
 constexpr int kMaxDirLights = 4;
-constexpr int kMaxKinds = kMaxDirLights + 2;   // miss, lights..., sky
+constexpr int kMaxKinds = kMaxDirLights + 3;   // miss, lights..., sky, rect lights
 // Traversal stack bound: a Karras tree over 30-bit Morton codes with index tie-breaks
 // is at most 63 levels deep, a BVH4 level spans at least one of them, and a step pushes
 // at most 3 siblings.
@@ -138,6 +138,32 @@ struct LightRec {
     float pad;
 };
 static_assert(sizeof(LightRec) == 64, "light record is 64 bytes");
+
+// A rect light (bi_pt_rect_light = RectLightData, lights_struct.hlsl:38-53): corners
+// position0..3 (centre + (w, h), (-w, h), (-w, -h), (w, -h) in the light's frame),
+// emission, one- or two-sided.  Rect lights cast no shadows; each shaded vertex adds
+// their linearly-transformed-cosine integrals.
+constexpr uint32_t kMaxRectLights = 16;
+struct RectRec {
+    float emission[3];
+    int32_t texture_index;  // < 0 (textured rect lights are not supported)
+    float center[3];
+    uint32_t two_sided;
+    float p0[3];
+    float inv_width_sqr;
+    float p1[3];
+    float inv_height_sqr;
+    float p2[3];
+    float inv_texel_size;
+    float p3[3];
+    float pad1;
+    float normal[3];
+    float pad2;
+};
+static_assert(sizeof(RectRec) == 112, "rect light record is 112 bytes");
+// The LTC tables (lights.cpp:26-37): 8 x 8 x 64 texels each, x fastest; three matrix rows
+// (xyz of RGBA32F) and the norm / Fresnel pair (xy of RG32F), held as four float4 tables.
+constexpr uint32_t kLtcSize = 8, kLtcDepth = 64, kLtcTexels = kLtcSize * kLtcSize * kLtcDepth;
 
 struct InstanceRec {
     float xform[12];
@@ -253,6 +279,9 @@ struct JobRec {
     uint16_t* gb_nr;
     uint16_t* gb_fres;
     uint8_t* gb_m0;
+    const RectRec* rects;   // num_rects rect lights (their slot follows the sky's)
+    uint32_t num_rects;
+    const float4* ltc;      // 4 * kLtcTexels: matrix rows 0, 1, 2, then the norm table
 };
 
 // Ray queue in SoA form.

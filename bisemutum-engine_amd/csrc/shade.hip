@@ -105,7 +105,7 @@ __device__ __forceinline__ void hit_surface(const JobRec& J, uint32_t leaf, floa
 
 // Shades queue entry i (path `path`, the job's local path `local`) of job J: its rays
 // go to `out` (mask: which of them exist).
-template <uint32_t NL>
+template <uint32_t NL, bool RECT>
 __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J, uint32_t i, uint32_t path,
                                             uint32_t local, bool primary, ShadeOut<NL>& out) {
     const size_t cap = a.ps.cap;
@@ -115,7 +115,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
     const uint32_t sample = key.sample;
     const uint32_t nl = J.num_lights;
     if (a.nrc_query) a.nrc_query[i].tag = kNoPath;   // entries without a cache query
-    const uint32_t nslots = nl + 2u;
+    const uint32_t nslots = nl + 2u + (RECT && J.num_rects ? 1u : 0u);
     V3 o, d, P = v3(0.0f, 0.0f, 0.0f);
     V3 beta;
     float pdf_prev;
@@ -212,6 +212,19 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
             }
             out.c[l] = c;
             mask |= 2u << l;
+        }
+        // rect lights (deferred_lighting_secondary.hlsl:73-96): no shadow rays, each light's
+        // lighting times the throughput added in light order to the slot after the sky's
+        if constexpr (RECT) {
+            if (J.num_rects) {
+                float* acc = a.ps.acc + (size_t)(nl + 2u) * 3u * cap;
+                V3 sum = v3(acc[path], acc[cap + path], acc[2 * cap + path]);
+                for (uint32_t r = 0; r < J.num_rects; ++r)
+                    sum = sum + rect_light_lit(J.rects[r], J.ltc, P, fr, V, m) * beta;
+                acc[path] = sum.x;
+                acc[cap + path] = sum.y;
+                acc[2 * cap + path] = sum.z;
+            }
         }
         // sky NEE only where the next segment is sampled too (the reference sees the
         // sky only through a traced segment that escapes)
@@ -320,7 +333,7 @@ __device__ __forceinline__ uint64_t look_back(const uint64_t* state, uint32_t t,
     }
 }
 
-template <uint32_t NL>
+template <uint32_t NL, bool RECT>
 __device__ __forceinline__ void shade_tile(const ShadeArgs& a) {
     __shared__ uint32_t s_wsum[4];
     __shared__ uint64_t s_base;
@@ -342,11 +355,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a) {
     if (i < count) {
         const uint32_t path = primary ? i : a.qin.path[i];
         if (a.jobs.num_jobs == 1u) {
-            shade_entry(a, a.jobs.rec[0], i, path, path, primary, out);   // uniform record: scalar loads
+            shade_entry<NL, RECT>(a, a.jobs.rec[0], i, path, path, primary, out);   // uniform record: scalar loads
         } else {
             const uint32_t jid = job_of(a.jobs.path_base, a.jobs.num_jobs, path);
             J = a.jobs.rec + jid;
-            shade_entry(a, *J, i, path, path - a.jobs.path_base[jid], primary, out);
+            shade_entry<NL, RECT>(a, *J, i, path, path - a.jobs.path_base[jid], primary, out);
         }
     }
     // the tile's emissions: (extension << 16 | shadow) per entry, scanned in entry order
@@ -396,10 +409,14 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a) {
 // One kernel per light bound; the one-light form (the common case) fits 80 registers,
 // 6 wavefronts per SIMD, without spilling.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_shade_l1(ShadeArgs a) {
-    shade_tile<1>(a);
+    shade_tile<1, false>(a);
 }
-__global__ __launch_bounds__(256) void k_shade_l2(ShadeArgs a) { shade_tile<2>(a); }
-__global__ __launch_bounds__(256) void k_shade_l4(ShadeArgs a) { shade_tile<kMaxDirLights>(a); }
+__global__ __launch_bounds__(256) void k_shade_l2(ShadeArgs a) { shade_tile<2, false>(a); }
+__global__ __launch_bounds__(256) void k_shade_l4(ShadeArgs a) { shade_tile<kMaxDirLights, false>(a); }
+// With rect lights (launches where some job has them): the LTC integrals inline.
+__global__ __launch_bounds__(256) void k_shade_rect_l1(ShadeArgs a) { shade_tile<1, true>(a); }
+__global__ __launch_bounds__(256) void k_shade_rect_l2(ShadeArgs a) { shade_tile<2, true>(a); }
+__global__ __launch_bounds__(256) void k_shade_rect_l4(ShadeArgs a) { shade_tile<kMaxDirLights, true>(a); }
 
 // Output counts and look-back words of the next shade launch.
 __global__ __launch_bounds__(256) void k_shade_reset(uint64_t* state, const uint32_t* count_in, uint32_t num_entries,
@@ -426,7 +443,7 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
     const uint32_t npix = J.npix;
     // a training job's slot q sums into target q
     const uint32_t pixel = (J.flags & kJobTrain) ? q : pixel_of(q, J.cam.width, J.cam.height);
-    const uint32_t nslots = J.num_lights + 2u;
+    const uint32_t nslots = J.num_lights + 2u + (J.num_rects ? 1u : 0u);
     const uint32_t samples = J.samples;
     const uint32_t pbase = a.jobs.path_base[jid];
     float* sums = J.sums;
@@ -544,12 +561,20 @@ void launch_shade_reset(uint64_t* tile_state, const uint32_t* count_in, uint32_t
                        num_entries, count_ext, count_shadow);
 }
 
-void launch_shade(const ShadeArgs& a, uint32_t capacity, uint32_t max_lights, hipStream_t s) {
+void launch_shade(const ShadeArgs& a, uint32_t capacity, uint32_t max_lights, bool rect, hipStream_t s) {
     const uint32_t tiles = (capacity + kTile - 1) / kTile;
     const dim3 grid(tiles ? tiles : 1), block(256);
-    if (max_lights <= 1) hipLaunchKernelGGL(k_shade_l1, grid, block, 0, s, a);
-    else if (max_lights <= 2) hipLaunchKernelGGL(k_shade_l2, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(k_shade_l4, grid, block, 0, s, a);
+    if (rect) {
+        if (max_lights <= 1) hipLaunchKernelGGL(k_shade_rect_l1, grid, block, 0, s, a);
+        else if (max_lights <= 2) hipLaunchKernelGGL(k_shade_rect_l2, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(k_shade_rect_l4, grid, block, 0, s, a);
+    } else if (max_lights <= 1) {
+        hipLaunchKernelGGL(k_shade_l1, grid, block, 0, s, a);
+    } else if (max_lights <= 2) {
+        hipLaunchKernelGGL(k_shade_l2, grid, block, 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_shade_l4, grid, block, 0, s, a);
+    }
 }
 
 void launch_resolve(const ResolveArgs& a, hipStream_t s) {

@@ -66,6 +66,8 @@ class Scene:
     textures: list = dataclasses.field(default_factory=list)   # Texture records
     sky_layout: int = 0     # SKY_EQUIRECT: sky_rgba [H, W, 4]; SKY_CUBE: [6, S, S, 4] (+X, -X, +Y, -Y, +Z, -Z)
     point_lights: np.ndarray = None   # f32 [P, 16] (bi_pt_point_light records), after the directional lights
+    rect_lights: np.ndarray = None    # f32 [R, 28] (bi_pt_rect_light records)
+    ltc: tuple = None                 # LTC tables (matrix rows 0, 1, 2: [4096, 4]; norm: [4096, 2])
 
     @property
     def num_tris(self) -> int:
@@ -142,6 +144,54 @@ def point_light(emission, position, radius=20.0, spot_axis=None, inner_deg=0.0, 
     r[12:13].view(np.int32)[0] = 0 if cast_shadow else -1
     r[13] = shadow_strength
     return r
+
+
+def rect_light(emission, center, width, height, x_axis=(1, 0, 0), y_axis=(0, 1, 0), two_sided=False):
+    """A bi_pt_rect_light record as LightsContext fills it (lights.cpp:211-227): corners
+    centre + (w, h), (-w, h), (-w, -h), (w, -h) along the light's x and y axes (half sizes),
+    normal x cross y, untextured."""
+    x = np.asarray(x_axis, np.float64); x = x / np.linalg.norm(x)
+    y = np.asarray(y_axis, np.float64); y = y / np.linalg.norm(y)
+    c = np.asarray(center, np.float64)
+    w, h = 0.5 * width, 0.5 * height
+    r = np.zeros(28, np.float32)
+    r[0:3] = emission
+    r[3:4].view(np.int32)[0] = -1
+    r[4:7] = c
+    r[7:8].view(np.uint32)[0] = 1 if two_sided else 0
+    for k, (sx, sy) in enumerate(((w, h), (-w, h), (-w, -h), (w, -h))):
+        r[8 + 4 * k:11 + 4 * k] = c + sx * x + sy * y
+    r[11] = 1.0 / (width * width)
+    r[15] = 1.0 / (height * height)
+    r[24:27] = np.cross(x, y)
+    return r
+
+
+def identity_ltc(norm=(1.0, 0.0)):
+    """LTC tables whose transform is the identity everywhere (the specular integral is then the
+    cosine integral) and whose (norm, Fresnel) pair is constant."""
+    rows = [np.zeros((4096, 4), np.float32) for _ in range(3)]
+    for i in range(3):
+        rows[i][:, i] = 1.0
+    return (rows[0], rows[1], rows[2], np.tile(np.asarray(norm, np.float32), (4096, 1)))
+
+
+def smooth_ltc(seed=0):
+    """Synthetic LTC tables: transforms near diag(a, a, 1) varying smoothly over the table
+    (invertible everywhere) and a (norm, Fresnel) pair in (0, 1) (a table for parity tests,
+    not a fit)."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(4096)
+    x, y, z = i % 8, (i // 8) % 8, i // 64
+    a = 0.3 + 0.09 * x + 0.04 * y + 0.005 * z
+    rows = [np.zeros((4096, 4), np.float32) for _ in range(3)]
+    rows[0][:, 0] = a
+    rows[1][:, 1] = a * (0.8 + 0.02 * y)
+    rows[2][:, 2] = 1.0
+    for r in rows:
+        r[:, :3] += (0.02 * rng.standard_normal((4096, 3))).astype(np.float32)
+    norm = np.stack([0.9 - 0.05 * x, 0.02 + 0.01 * y], axis=1).astype(np.float32)
+    return (rows[0], rows[1], rows[2], norm)
 
 
 def random_material(rng: Pcg32, two_sided=True):

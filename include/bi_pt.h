@@ -348,6 +348,42 @@ typedef struct bi_pt_point_light {
 } bi_pt_point_light;
 /* Point and spot lights; directional and point lights together are at most 4. */
 bi_pt_status bi_pt_set_point_lights(bi_pt_ctx* ctx, const bi_pt_point_light* lights, uint32_t num_lights);
+
+/* RectLightData (shaders/renderer/lights_struct.hlsl:38-53), 112 bytes, as
+ * LightsContext::collect_all_lights fills it (src/renderer/context/lights.cpp:204-242):
+ * corners position0..3 = centre + (w, h), (-w, h), (-w, -h), (w, -h) along the light's x and
+ * y axes (w, h the half width and height), normal its z axis, emission colour * strength.
+ * Rect lights cast no shadows (deferred_lighting_secondary.hlsl:73-96 has no shadow term);
+ * each shaded vertex adds their linearly-transformed-cosine lighting. */
+typedef struct bi_pt_rect_light {
+    float emission[3];
+    int32_t texture_index;    /* must be < 0: textured rect lights are not supported */
+    float center_position[3];
+    uint32_t two_sided;
+    float position0[3];
+    float inv_width_sqr;
+    float position1[3];
+    float inv_height_sqr;
+    float position2[3];
+    float inv_texel_size;
+    float position3[3];
+    float pad1;
+    float normal[3];
+    float pad2;
+} bi_pt_rect_light;
+
+/* Replaces LightsContext's rect_lights buffer (lights.cpp:204-243): at most 16; rendering
+ * with rect lights needs the LTC tables (bi_pt_set_ltc_luts).  BI_PT_ERROR_UNSUPPORTED for
+ * a textured light or more than 16 lights. */
+bi_pt_status bi_pt_set_rect_lights(bi_pt_ctx* ctx, const bi_pt_rect_light* lights, uint32_t num_lights);
+
+/* The LTC look-up tables LightsContext loads (lights.cpp:26-37, the engine's
+ * ltc_matrix_lut0..2 and ltc_norm_lut texture assets): 8 x 8 x 64 texels each, x fastest,
+ * then y, then z; the three matrix tables RGBA32F (4 floats a texel, rows of the transform
+ * in xyz), the norm table RG32F (2 floats a texel).  Sampled with linear filtering and
+ * clamp to edge, as the engine's sampler. */
+bi_pt_status bi_pt_set_ltc_luts(bi_pt_ctx* ctx, const float* matrix_lut0, const float* matrix_lut1,
+                                const float* matrix_lut2, const float* norm_lut);
 /* Equirect RGBA32F sky (host memory, row 0 = +Y pole), colour multiplier
  * (SkyboxSystem color*strength) and 3x3 row-major rotation applied to directions
  * before lookup (deferred_lighting_secondary.hlsl:25). */

@@ -84,6 +84,10 @@ typedef struct ora_scene_desc {
     uint32_t num_point_lights;
     const float* point_lights;   /* 16 floats each, layout of bi_pt_point_light; lights are the
                                     directional ones then these, at most 4 in all */
+    uint32_t num_rect_lights;
+    const float* rect_lights;    /* 28 floats each, layout of bi_pt_rect_light (untextured) */
+    const float* ltc_matrix[3];  /* LTC tables as bi_pt_set_ltc_luts: 8 x 8 x 64 texels, 4 floats each */
+    const float* ltc_norm;       /* 2 floats a texel */
 } ora_scene_desc;
 
 /* Camera: row-major 4x4 matrices (m[row*4+col]), as bi_pt_camera. */
@@ -157,6 +161,15 @@ int ora_gbuffer32(const ora_ctx* ctx, const ora_camera* cam, const ora_settings*
                   const uint32_t* pixels, uint16_t* base, uint16_t* nr, uint16_t* fres, uint8_t* m0);
 /* float -> IEEE half bits, round to nearest even. */
 uint16_t ora_half32(float f);
+/* One rect light's lighting at a lit surface (the kernels' rect_light_lit): light record
+   rect (28 floats), LTC tables ltc (4 x 4096 texels x 4 floats: matrix rows 0-2, norm),
+   point P, shading normal N (frame from N), direction V towards the viewer, roughness
+   rx, ry, base colour, f0, f90.  Writes rgb and terms = (diffuse integral, specular
+   integral, norm, Fresnel term). */
+void ora_rect_light32(const float* rect, const float* ltc, const float* P, const float* N, const float* V, float rx,
+                      float ry, const float* base, const float* f0, const float* f90, float* rgb, float* terms);
+/* The LTC tables' transform (9 floats, rows) and pair (2 floats) at (u3, u2, u1, u0). */
+void ora_ltc_fetch32(const float* ltc, float u3, float u2, float u1, float u0, float* m, float* b);
 int ora_render32_pixels(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample_base,
                         uint32_t spp, uint32_t n, const uint32_t* pixels, float* sum_rgb, int num_threads);
 

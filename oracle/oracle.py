@@ -40,6 +40,7 @@ class SceneDesc(C.Structure):
         ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
         ("sky_layout", C.c_uint32),
         ("num_point_lights", C.c_uint32), ("point_lights", _f32p),
+        ("num_rect_lights", C.c_uint32), ("rect_lights", _f32p), ("ltc_matrix", _f32p * 3), ("ltc_norm", _f32p),
     ]
 
 
@@ -111,6 +112,9 @@ def lib():
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ora_half32.restype = C.c_uint16
         L.ora_half32.argtypes = [C.c_float]
+        L.ora_rect_light32.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, _f32p, _f32p, _f32p,
+                                       _f32p, _f32p]
+        L.ora_ltc_fetch32.argtypes = [_f32p, C.c_float, C.c_float, C.c_float, C.c_float, _f32p, _f32p]
         L.ora_sky_sample32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p]
         L.ora_sky_pdf32.restype = C.c_float
         L.ora_sky_pdf32.argtypes = [C.c_void_p, _f32p]
@@ -175,6 +179,16 @@ class OracleScene:
         d.dir_lights = ptr(k["dir_lights"], _f32p)
         d.num_point_lights = len(k["point_lights"])
         d.point_lights = ptr(k["point_lights"], _f32p) if len(k["point_lights"]) else None
+        rects = getattr(scene, "rect_lights", None)
+        if rects is not None and len(rects):
+            k["rect_lights"] = np.ascontiguousarray(rects, np.float32).reshape(-1, 28)
+            luts = [np.ascontiguousarray(t, np.float32).reshape(-1) for t in scene.ltc]
+            k["ltc"] = luts
+            d.num_rect_lights = len(k["rect_lights"])
+            d.rect_lights = ptr(k["rect_lights"], _f32p)
+            for i in range(3):
+                d.ltc_matrix[i] = ptr(luts[i], _f32p)
+            d.ltc_norm = ptr(luts[3], _f32p)
         layout = getattr(scene, "sky_layout", 0)
         if layout == 1:   # cube faces [6, S, S, 4]: rows face-major
             if k["sky"].size and (k["sky"].ndim != 4 or k["sky"].shape[0] != 6 or k["sky"].shape[1] != k["sky"].shape[2]):
@@ -445,6 +459,37 @@ def vndf_pdf64_batch(v_local, ls, alpha):
     out = np.zeros(len(ls), np.float64)
     lib().ora_vndf_pdf64_batch(ptr(v, _f64p), len(ls), ptr(ls, _f64p), alpha, ptr(out, _f64p))
     return out
+
+
+def ltc_tables(m0, m1, m2, norm):
+    """The four LTC tables in the kernels' layout: [4 * 4096 * 4] float32 (matrix rows, then
+    the norm pair padded to four floats)."""
+    t = np.zeros((4, 4096, 4), np.float32)
+    for i, m in enumerate((m0, m1, m2)):
+        t[i] = np.asarray(m, np.float32).reshape(4096, 4)
+    t[3, :, :2] = np.asarray(norm, np.float32).reshape(4096, 2)
+    return t.reshape(-1)
+
+
+def rect_light32(rect, tables, P, N, V, rx, ry, base, f0, f90):
+    """(rgb, (diffuse integral, specular integral, norm, Fresnel term)) of one rect light at
+    a lit surface; tables from ltc_tables."""
+    a = [np.ascontiguousarray(x, np.float32) for x in (rect, tables, P, N, V, base, f0, f90)]
+    rgb = np.zeros(3, np.float32)
+    terms = np.zeros(4, np.float32)
+    lib().ora_rect_light32(ptr(a[0], _f32p), ptr(a[1], _f32p), ptr(a[2], _f32p), ptr(a[3], _f32p), ptr(a[4], _f32p),
+                           float(rx), float(ry), ptr(a[5], _f32p), ptr(a[6], _f32p), ptr(a[7], _f32p),
+                           ptr(rgb, _f32p), ptr(terms, _f32p))
+    return rgb, terms
+
+
+def ltc_fetch32(tables, u3, u2, u1, u0):
+    """(transform [3, 3] rows, (norm, Fresnel term)) of the LTC tables at (u3, u2, u1, u0)."""
+    t = np.ascontiguousarray(tables, np.float32)
+    m = np.zeros(9, np.float32)
+    b = np.zeros(2, np.float32)
+    lib().ora_ltc_fetch32(ptr(t, _f32p), float(u3), float(u2), float(u1), float(u0), ptr(m, _f32p), ptr(b, _f32p))
+    return m.reshape(3, 3), b
 
 
 def half32(x):

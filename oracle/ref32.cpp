@@ -1182,6 +1182,181 @@ F3 point_light_eval(const float* L, F3 P, F3* ldir, float* dist_out) {
     return f3(L[0] * att, L[1] * att, L[2] * att);
 }
 
+// ---------------------------------------------------------------- rect lights (LTC)
+// Linearly transformed cosines over a quad light (lights.hlsl:161-513; Heitz et al. 2016
+// with anisotropic 4D tables): rect_light_eval_ltc and surface_eval_lut_lit
+// (material/lit.hlsl:37-58), in the shading frame of make_frame(N).
+struct M3 { F3 r0, r1, r2; };
+inline F3 m3_mul(const M3& m, F3 v) { return f3(dot(m.r0, v), dot(m.r1, v), dot(m.r2, v)); }
+// adjugate / determinant (determinant along row 0)
+inline M3 m3_inverse(const M3& m) {
+    const float a = m.r0.x, b = m.r0.y, c = m.r0.z, d = m.r1.x, e = m.r1.y, f = m.r1.z, g = m.r2.x, h = m.r2.y,
+                k = m.r2.z;
+    const float A = e * k - f * h, B = f * g - d * k, Cc = d * h - e * g;
+    const float det = (a * A + b * B) + c * Cc;
+    const float s = 1.0f / det;
+    M3 r;
+    r.r0 = f3(A * s, (c * h - b * k) * s, (b * f - c * e) * s);
+    r.r1 = f3(B * s, (a * k - c * g) * s, (c * d - a * f) * s);
+    r.r2 = f3(Cc * s, (b * g - a * h) * s, (a * e - b * d) * s);
+    return r;
+}
+// linear filter, clamp to edge, on an axis of n texels
+inline void ltc_axis(float u, int n, int* i0, int* i1, float* f) {
+    const float t = u * (float)n - 0.5f;
+    const float fl = std::floor(t);
+    *f = t - fl;
+    const int a = (int)fl, b = a + 1;
+    *i0 = a < 0 ? 0 : (a > n - 1 ? n - 1 : a);
+    *i1 = b < 0 ? 0 : (b > n - 1 ? n - 1 : b);
+}
+inline float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+inline F4 lerp4(F4 a, F4 b, float t) {
+    return F4{lerpf(a.x, b.x, t), lerpf(a.y, b.y, t), lerpf(a.z, b.z, t), lerpf(a.w, b.w, t)};
+}
+inline F4 ltc_texel(const float* t, int i) { return F4{t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]}; }
+// trilinear sample of one 8 x 8 x 64 table (x fastest) at normalised (x, y, z)
+F4 ltc_sample(const float* t, float x, float y, float z) {
+    int x0, x1, y0, y1, z0, z1;
+    float fx, fy, fz;
+    ltc_axis(x, 8, &x0, &x1, &fx);
+    ltc_axis(y, 8, &y0, &y1, &fy);
+    ltc_axis(z, 64, &z0, &z1, &fz);
+    F4 e[2];
+    for (int k = 0; k < 2; ++k) {
+        const int zo = 64 * (k ? z1 : z0);
+        const F4 c0 = lerp4(ltc_texel(t, x0 + 8 * y0 + zo), ltc_texel(t, x1 + 8 * y0 + zo), fx);
+        const F4 c1 = lerp4(ltc_texel(t, x0 + 8 * y1 + zo), ltc_texel(t, x1 + 8 * y1 + zo), fx);
+        e[k] = lerp4(c0, c1, fy);
+    }
+    return lerp4(e[0], e[1], fz);
+}
+// get_ltc_tex3d_coord + fetch_ltc_matrix + fetch_ltc_brdf (lights.hlsl:164-201)
+void ltc_fetch(const float* ltc, float u3, float u2, float u1, float u0, M3* m, float* bx, float* by) {
+    const float ws = u0 * 7.0f;
+    const float ws_f = std::floor(ws);
+    const float wc = std::floor(ws + 1.0f);
+    const float ws_c = wc < 7.0f ? wc : 7.0f;
+    const float w = ws - ws_f;
+    const float x = (u3 * 7.0f + 0.5f) / 8.0f, y = (u2 * 7.0f + 0.5f) / 8.0f;
+    const float z1 = ((u1 * 7.0f + 8.0f * ws_f) + 0.5f) / 64.0f, z2 = ((u1 * 7.0f + 8.0f * ws_c) + 0.5f) / 64.0f;
+    F3 r[3];
+    for (int k = 0; k < 3; ++k) {
+        const F4 a = ltc_sample(ltc + (size_t)k * 4096 * 4, x, y, z1), b = ltc_sample(ltc + (size_t)k * 4096 * 4, x, y, z2);
+        r[k] = f3(lerpf(a.x, b.x, w), lerpf(a.y, b.y, w), lerpf(a.z, b.z, w));
+    }
+    m->r0 = r[0]; m->r1 = r[1]; m->r2 = r[2];
+    const F4 n1 = ltc_sample(ltc + (size_t)3 * 4096 * 4, x, y, z1), n2 = ltc_sample(ltc + (size_t)3 * 4096 * 4, x, y, z2);
+    *bx = lerpf(n1.x, n2.x, w);
+    *by = lerpf(n1.y, n2.y, w);
+}
+inline void quad_reverse(F3 q[4]) {
+    const F3 a = q[0], b = q[1];
+    q[0] = q[3]; q[1] = q[2]; q[2] = b; q[3] = a;
+}
+// get_ltc_matrix_and_brdf (lights.hlsl:203-255)
+void ltc_matrix_brdf(const float* ltc, F3 lv, float rx, float ry, F3 q[4], M3* m, float* bx, float* by) {
+    const float theta = acos32(lv.z);
+    const bool flip_r = ry > rx;
+    float phi = atan2_32(lv.y, lv.x);
+    phi = flip_r ? kPiF * 0.5f - phi : phi;
+    phi = phi >= 0.0f ? phi : phi + 2.0f * kPiF;
+    const float u0 = fmax_((flip_r ? ry : rx) - 0.001f, 0.0f) / (1.0f - 0.001f);
+    const float u1 = flip_r ? rx / ry : ry / rx;
+    const float u2 = theta / (kPiF * 0.5f);
+    if (phi < kPiF * 0.5f) {
+        ltc_fetch(ltc, phi / (kPiF * 0.5f), u2, u1, u0, m, bx, by);
+    } else if (phi < kPiF) {
+        ltc_fetch(ltc, (kPiF - phi) / (kPiF * 0.5f), u2, u1, u0, m, bx, by);
+        quad_reverse(q);
+        m->r0 = -m->r0;
+    } else if (phi < 1.5f * kPiF) {
+        ltc_fetch(ltc, (phi - kPiF) / (kPiF * 0.5f), u2, u1, u0, m, bx, by);
+        m->r0 = -m->r0;
+        m->r1 = -m->r1;
+    } else {
+        ltc_fetch(ltc, (2.0f * kPiF - phi) / (kPiF * 0.5f), u2, u1, u0, m, bx, by);
+        quad_reverse(q);
+        m->r1 = -m->r1;
+    }
+    if (flip_r) {
+        quad_reverse(q);
+        const F3 t = m->r0;
+        m->r0 = m->r1;
+        m->r1 = t;
+    }
+}
+// ltc_integrate_edge (lights.hlsl:347-361): z of cross(v1, v2) theta / sin(theta), fitted
+inline float ltc_edge(F3 v1, F3 v2) {
+    const float x = dot(v1, v2);
+    const float y = x < 0.0f ? -x : x;
+    const float a = 5.42031f + (3.12829f + 0.0902326f * y) * y;
+    const float b = 3.45068f + (4.18814f + y) * y;
+    float t = a / b;
+    if (x < 0.0f) t = kPiF / std::sqrt(1.0f - x * x) - t;
+    return (v1.x * v2.y - v1.y * v2.x) * t;
+}
+// ltc_integrate (lights.hlsl:362-400): the quad relative to P in the shading frame,
+// transformed by minv (none: the cosine itself), clipped to z > 0 (in.z * out - out.z *
+// in for a vertex pair across the plane; a diagonal pair is the reference's empty case),
+// projected on the sphere, edges summed in polygon order.
+float ltc_integrate(F3 P, const Frame& fr, const M3* minv, const F3 q[4], bool two_sided) {
+    F3 l[4];
+    uint32_t inside = 0;
+    for (int k = 0; k < 4; ++k) {
+        const F3 r = q[k] - P;
+        F3 v = f3(dot(fr.t, r), dot(fr.b, r), dot(fr.n, r));
+        if (minv) v = m3_mul(*minv, v);
+        l[k] = v;
+        inside |= (v.z > 0.0f ? 1u : 0u) << k;
+    }
+    if (inside == 0u || inside == 5u || inside == 10u) return 0.0f;
+    F3 c[5];
+    int n = 0;
+    for (int k = 0; k < 4; ++k) {
+        const F3 a = l[k], b = l[(k + 1) & 3];
+        const bool ai = (inside >> k) & 1u, bi = (inside >> ((k + 1) & 3)) & 1u;
+        if (ai) c[n++] = a;
+        if (ai != bi) {
+            const F3 in = ai ? a : b, out = ai ? b : a;
+            c[n++] = f3(in.z * out.x - out.z * in.x, in.z * out.y - out.z * in.y, in.z * out.z - out.z * in.z);
+        }
+    }
+    for (int k = 0; k < n; ++k) c[k] = normalize(c[k]);
+    float sum = 0.0f;
+    for (int k = 0; k < n; ++k) sum = sum + ltc_edge(c[k], c[k + 1 < n ? k + 1 : 0]);
+    float r = two_sided ? (sum < 0.0f ? -sum : sum) : fmax_(sum, 0.0f);
+    if (!std::isfinite(r)) r = 0.0f;
+    return r;
+}
+// The lighting of rect light R (28 floats, bi_pt_rect_light) at P; terms (nullable):
+// diffuse integral, specular integral, norm, Fresnel term.
+F3 rect_light_lit(const float* R, const float* ltc, F3 P, const Frame& fr, F3 V, const Mat& m, float* terms) {
+    const F3 lv = f3(dot(V, fr.t), dot(V, fr.b), dot(V, fr.n));
+    if (terms) terms[0] = terms[1] = terms[2] = terms[3] = 0.0f;
+    if (lv.z <= 0.0f) return f3(0, 0, 0);
+    F3 q[4] = {f3(R[20], R[21], R[22]), f3(R[16], R[17], R[18]), f3(R[12], R[13], R[14]), f3(R[8], R[9], R[10])};
+    uint32_t two;
+    std::memcpy(&two, &R[7], 4);
+    const F3 e = f3(R[0], R[1], R[2]);
+    const float i_d = ltc_integrate(P, fr, nullptr, q, two != 0u);
+    M3 mt;
+    float bx, by;
+    ltc_matrix_brdf(ltc, lv, m.rx, m.ry, q, &mt, &bx, &by);
+    const M3 mi = m3_inverse(mt);
+    const float i_s = ltc_integrate(P, fr, &mi, q, two != 0u);
+    if (terms) { terms[0] = i_d; terms[1] = i_s; terms[2] = bx; terms[3] = by; }
+    const float ndotv = dot(fr.n, V);
+    if (ndotv <= 0.0f) return f3(0, 0, 0);
+    const float x = 1.0f - ndotv;
+    const float p5 = ((x * x) * (x * x)) * x;
+    const F3 fres = m.f0 + (m.f90 - m.f0) * p5;
+    const F3 one = f3(1.0f, 1.0f, 1.0f);
+    const F3 diffuse = ((one - fres) * m.base) * kInvPi;
+    const F3 specular = m.f0 * bx + m.f90 * by;
+    return diffuse * (e * i_d) + specular * (e * i_s);
+}
+
 // The surface a ray of direction d sees at hit h of triangle tri, point P
 // (hits/rt_gbuffer_hit.hlsl:6-17): interpolated normal, the material (a checkerboard's
 // cell by P), texture-driven fields, the normal map, and the two-sided flip.
@@ -1250,7 +1425,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
     ray.seed = st.seed;
     F3 beta = f3(1.0f, 1.0f, 1.0f);
     float pdf_prev = 0.0f;
-    F3 acc_miss = f3(0, 0, 0), acc_sky = f3(0, 0, 0);
+    F3 acc_miss = f3(0, 0, 0), acc_sky = f3(0, 0, 0), acc_rect = f3(0, 0, 0);
     F3 acc_dir[kMaxDirLights];
     for (int l = 0; l < kMaxDirLights; ++l) acc_dir[l] = f3(0, 0, 0);
     for (uint32_t b = 0; b < D; ++b) {
@@ -1288,7 +1463,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
             std::memcpy(&r[13], &tag, 4);
             r[14] = r[15] = 0.0f;
             beta = f3(1.0f, 1.0f, 1.0f);
-            acc_miss = acc_sky = f3(0, 0, 0);
+            acc_miss = acc_sky = acc_rect = f3(0, 0, 0);
             for (int l = 0; l < kMaxDirLights; ++l) acc_dir[l] = f3(0, 0, 0);
         }
         Frame fr = make_frame(N);
@@ -1329,6 +1504,9 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
                 if (keep > 0.0f) acc_dir[l] = acc_dir[l] + c * keep;
             }
         }
+        // rect lights (deferred_lighting_secondary.hlsl:73-96): unshadowed, in light order
+        for (uint32_t r = 0; r < s.num_rect_lights; ++r)
+            acc_rect = acc_rect + rect_light_lit(&s.rect_lights[28 * r], s.ltc.data(), P, fr, V, m, nullptr) * beta;
         // Sky next-event estimation with power-heuristic MIS against BSDF sampling, only
         // where the path also samples the next segment: the reference receives sky light
         // only through a traced segment that escapes (deferred_lighting_secondary.hlsl:24-29),
@@ -1374,6 +1552,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
     F3 r = acc_miss;
     for (uint32_t l = 0; l < nl; ++l) r = r + acc_dir[l];
     r = r + acc_sky;
+    if (s.num_rect_lights) r = r + acc_rect;
     out.radiance = r;
     return out;
 }
@@ -1725,6 +1904,26 @@ float ora_sky_pdf32(const ora_ctx* ctx, const float* dir) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);
     return sky_pdf(s, f3(dir[0], dir[1], dir[2]));
 }
+void ora_rect_light32(const float* rect, const float* ltc, const float* P, const float* N, const float* V, float rx,
+                      float ry, const float* base, const float* f0, const float* f90, float* rgb, float* terms) {
+    Mat m{};
+    m.base = f3(base[0], base[1], base[2]);
+    m.f0 = f3(f0[0], f0[1], f0[2]);
+    m.f90 = f3(f90[0], f90[1], f90[2]);
+    m.rx = rx;
+    m.ry = ry;
+    const Frame fr = make_frame(f3(N[0], N[1], N[2]));
+    const F3 c = rect_light_lit(rect, ltc, f3(P[0], P[1], P[2]), fr, f3(V[0], V[1], V[2]), m, terms);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+
+void ora_ltc_fetch32(const float* ltc, float u3, float u2, float u1, float u0, float* m, float* b) {
+    M3 mt;
+    ltc_fetch(ltc, u3, u2, u1, u0, &mt, &b[0], &b[1]);
+    const F3 r[3] = {mt.r0, mt.r1, mt.r2};
+    for (int k = 0; k < 3; ++k) { m[3 * k] = r[k].x; m[3 * k + 1] = r[k].y; m[3 * k + 2] = r[k].z; }
+}
+
 void ora_camera_ray32(const ora_camera* cam, uint32_t px, uint32_t py, float jx, float jy, float* o,
                       float* d) {
     F3 oo, dd;

@@ -521,9 +521,9 @@ int ora_render64(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* 
                  int num_threads) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);
     // the FP64 oracle covers opaque parameter materials, directional lights and the
-    // equirect sky (textured and non-opaque materials, point lights and the cube-map sky
-    // are pinned through the FP32 restatement's closed-form tests)
-    if (s.sky_layout != 0u || s.num_point_lights != 0u) return -2;
+    // equirect sky (textured and non-opaque materials, point and rect lights and the
+    // cube-map sky are pinned through the FP32 restatement's closed-form tests)
+    if (s.sky_layout != 0u || s.num_point_lights != 0u || s.num_rect_lights != 0u) return -2;
     for (size_t m = 0; m < s.materials.size() / kMatFloats; ++m) {
         uint32_t w[32];
         std::memcpy(w, &s.materials[kMatFloats * m], sizeof w);

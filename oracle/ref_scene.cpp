@@ -5,6 +5,7 @@
 // 3x4 row-major transforms, include/bisemutum/rhi/accel.hpp:48-55) as one world-space
 // triangle list; normals use the cofactor of the 3x3 part, which is proportional to the
 // inverse transpose that hit.hlsl:155 applies (WorldToObject4x3 * n).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -153,6 +154,18 @@ ora_ctx* ora_create(const ora_scene_desc* d) {
     }
     s->num_dir_lights = d->num_dir_lights;
     s->dir_lights.assign(d->dir_lights, d->dir_lights + 16 * (size_t)d->num_dir_lights);
+    s->num_rect_lights = d->rect_lights ? d->num_rect_lights : 0;
+    if (s->num_rect_lights) {
+        s->rect_lights.assign(d->rect_lights, d->rect_lights + 28 * (size_t)d->num_rect_lights);
+        s->ltc.assign((size_t)4 * 4096 * 4, 0.0f);
+        for (int k = 0; k < 3; ++k)
+            if (d->ltc_matrix[k]) std::copy(d->ltc_matrix[k], d->ltc_matrix[k] + 4096 * 4, s->ltc.begin() + (size_t)k * 4096 * 4);
+        if (d->ltc_norm)
+            for (size_t i = 0; i < 4096; ++i) {
+                s->ltc[((size_t)3 * 4096 + i) * 4] = d->ltc_norm[2 * i];
+                s->ltc[((size_t)3 * 4096 + i) * 4 + 1] = d->ltc_norm[2 * i + 1];
+            }
+    }
     s->num_point_lights = d->point_lights ? d->num_point_lights : 0;
     if (s->num_point_lights)
         s->point_lights.assign(d->point_lights, d->point_lights + 16 * (size_t)d->num_point_lights);

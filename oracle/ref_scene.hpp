@@ -45,6 +45,9 @@ struct Scene {
     std::vector<float> dir_lights; // 16 per light
     uint32_t num_point_lights = 0;
     std::vector<float> point_lights;   // 16 per light (layout of bi_pt_point_light); after the directional ones
+    uint32_t num_rect_lights = 0;
+    std::vector<float> rect_lights;    // 28 per light (layout of bi_pt_rect_light)
+    std::vector<float> ltc;            // 4 tables x 4096 texels x 4 floats: matrix rows 0-2 (xyz), norm (xy)
     // sky
     uint32_t sky_w = 0, sky_h = 0;
     uint32_t sky_layout = 0;       // 0 equirect (W x H), 1 cube map (six W x W faces, H = 6 W)
