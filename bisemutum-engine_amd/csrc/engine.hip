@@ -44,6 +44,9 @@ bi_pt_status engine_init(Owner* own, Engine& eng) {
     BI_HIP_TRY(own, hipMemcpy(eng.empty_tree.p, &e, sizeof e, hipMemcpyHostToDevice));
     for (auto& ev : eng.ev) BI_HIP_TRY(own, hipEventCreate(&ev));
     BI_HIP_TRY(own, hipEventCreate(&eng.plan_ev));
+    BI_HIP_TRY(own, hipStreamCreateWithFlags(&eng.aux_s, hipStreamNonBlocking));
+    BI_HIP_TRY(own, hipEventCreateWithFlags(&eng.ev_shade, hipEventDisableTiming));
+    for (auto& ev : eng.ev_shadow) BI_HIP_TRY(own, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     eng.ready = true;
     return BI_PT_OK;
 }
@@ -51,15 +54,27 @@ bi_pt_status engine_init(Owner* own, Engine& eng) {
 void engine_free(Owner* own, Engine& eng) {
     for (int k = 0; k < 6; ++k) { dfree(own, eng.qa[k]); dfree(own, eng.qb[k]); }
     for (int k = 0; k < 4; ++k) dfree(own, eng.ps_f[k]);
-    for (int k = 0; k < 9; ++k) dfree(own, eng.sq_f[k]);
+    for (int t = 0; t < 2; ++t) {
+        for (int k = 0; k < 9; ++k) dfree(own, eng.sq_f[t][k]);
+        dfree(own, eng.sq_pk[t]);
+    }
     dfree(own, eng.acc);
-    dfree(own, eng.qa_path); dfree(own, eng.qb_path); dfree(own, eng.sq_pk);
+    dfree(own, eng.qa_path); dfree(own, eng.qb_path);
     dfree(own, eng.hit_rec); dfree(own, eng.tile_state);
     dfree(own, eng.counters); dfree(own, eng.plan); dfree(own, eng.empty_tree);
     if (eng.ready) {
         for (auto& ev : eng.ev) (void)hipEventDestroy(ev);
         (void)hipEventDestroy(eng.plan_ev);
     }
+    if (eng.aux_s) {
+        (void)hipStreamDestroy(eng.aux_s);
+        eng.aux_s = nullptr;
+    }
+    if (eng.ev_shade) (void)hipEventDestroy(eng.ev_shade);
+    for (auto& ev : eng.ev_shadow)
+        if (ev) (void)hipEventDestroy(ev);
+    eng.ev_shade = nullptr;
+    eng.ev_shadow[0] = eng.ev_shadow[1] = nullptr;
     if (eng.build_fork) {
         (void)hipEventDestroy(eng.build_fork);
         for (int k = 0; k < Engine::kBuildStreams; ++k) {
@@ -143,7 +158,7 @@ uint64_t engine_bytes(uint32_t cap, uint32_t slots, uint32_t sh_mult) {
     b += c * sizeof(HitRec);                     // hit records
     b += 4 * c * 4;                              // path state
     b += (uint64_t)slots * 3 * c * 4;            // radiance accumulators
-    b += (uint64_t)sh_mult * c * (9 * 4 + 4);    // shadow queue
+    b += 2 * (uint64_t)sh_mult * c * (9 * 4 + 4);   // two shadow queue sets
     b += tiles * 8;                              // look-back words of the shade tiles
     return b;
 }
@@ -165,8 +180,10 @@ bi_pt_status ensure_wave_buffers(Owner* own, Engine& eng, uint32_t cap, uint32_t
     for (int k = 0; k < 4; ++k) BI_TRY(dalloc(own, eng.ps_f[k], cap));
     BI_TRY(dalloc(own, eng.acc, (size_t)slots * 3u * cap));
     const size_t shcap = (size_t)cap * sh_mult;
-    for (int k = 0; k < 9; ++k) BI_TRY(dalloc(own, eng.sq_f[k], shcap));
-    BI_TRY(dalloc(own, eng.sq_pk, shcap));
+    for (int t = 0; t < 2; ++t) {
+        for (int k = 0; k < 9; ++k) BI_TRY(dalloc(own, eng.sq_f[t][k], shcap));
+        BI_TRY(dalloc(own, eng.sq_pk[t], shcap));
+    }
     BI_TRY(dalloc(own, eng.tile_state, (cap + 255u) / 256u + 1u));
     eng.wave_cap = cap;
     eng.wave_slots = slots;
@@ -583,8 +600,14 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         PathState ps{eng.ps_f[0].p, eng.ps_f[1].p, eng.ps_f[2].p, eng.ps_f[3].p, eng.acc.p, eng.wave_cap};
         HitBuf hits{eng.hit_rec.p};
         RayQueue qA = queue_of(eng.qa, eng.qa_path), qB = queue_of(eng.qb, eng.qb_path);
-        ShadowQueue sq{eng.sq_f[0].p, eng.sq_f[1].p, eng.sq_f[2].p, eng.sq_f[3].p, eng.sq_f[4].p,
-                       eng.sq_f[5].p, eng.sq_f[6].p, eng.sq_f[7].p, eng.sq_f[8].p, eng.sq_pk.p};
+        ShadowQueue sqs[2];
+        for (int t = 0; t < 2; ++t)
+            sqs[t] = ShadowQueue{eng.sq_f[t][0].p, eng.sq_f[t][1].p, eng.sq_f[t][2].p, eng.sq_f[t][3].p, eng.sq_f[t][4].p,
+                                 eng.sq_f[t][5].p, eng.sq_f[t][6].p, eng.sq_f[t][7].p, eng.sq_f[t][8].p, eng.sq_pk[t].p};
+        // shadow launches overlap the next segment's closest hit, except when timing kernel
+        // classes, recording queues, or training the cache (whose reset of a training path's
+        // accumulators at the cache segment must follow the earlier shadow additions)
+        const bool overlap = !timing && !record && !any_train;
 
         if (record) { eng.rec_ext.clear(); eng.rec_sh.clear(); }
 
@@ -619,8 +642,13 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 for (uint32_t k = 0; k < J.num_paths; ++k) all[k] = k;
                 eng.rec_ext.back()[0] = all;
             }
+            bool set_used[2] = {false, false};
             for (uint32_t b = 0; b < D; ++b) {
                 const bool primary = b == 0;
+                const uint32_t set = overlap ? (b & 1u) : 0u;   // shadow queue set of this segment
+                const ShadowQueue& sq = sqs[set];
+                uint32_t* cnt_sh = &cnt[kCntShadow + set];
+                uint32_t* shadow_work = &cnt[set ? kCntShadowWork1 : kCntShadowWork];
                 const RayQueue& qin = (b & 1u) ? qA : qB;      // segment b reads what segment b-1 wrote
                 const RayQueue& qout = (b & 1u) ? qB : qA;
                 uint32_t* cnt_in = (b & 1u) ? &cnt[kCntExtA] : &cnt[kCntExtB];
@@ -650,7 +678,10 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 }
                 BI_TRY(mark(1));
 
-                launch_shade_reset(eng.tile_state.p, primary ? nullptr : cnt_in, J.num_paths, cnt_out, &cnt[kCntShadow],
+                // the set's previous shadow launch (two segments back) has ended before its
+                // queue and counters are written again
+                if (overlap && set_used[set]) BI_HIP_TRY(own, hipStreamWaitEvent(s, eng.ev_shadow[set], 0));
+                launch_shade_reset(eng.tile_state.p, primary ? nullptr : cnt_in, J.num_paths, cnt_out, cnt_sh,
                                    J.num_paths, s);
                 ShadeArgs sa{};
                 sa.jobs = J;
@@ -661,12 +692,12 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 sa.qout = qout;
                 sa.sq = sq;
                 sa.count_ext = cnt_out;
-                sa.count_shadow = &cnt[kCntShadow];
+                sa.count_shadow = cnt_sh;
                 sa.ray_counters = rays;
                 sa.tile_state = eng.tile_state.p;
                 sa.error = &cnt[kCntError];
                 sa.trace_work_reset = &cnt[kCntTraceWork];
-                sa.shadow_work_reset = &cnt[kCntShadowWork];
+                sa.shadow_work_reset = shadow_work;
                 sa.bounce = b;
                 const bool cache_now = wave_cached && b == nrc->cache_bounce;
                 sa.nrc_query = cache_now ? nrc->query.p : nullptr;
@@ -688,15 +719,23 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                 ShadowArgs sha{};
                 sha.jobs = J;
                 sha.q = sq;
-                sha.count_in = &cnt[kCntShadow];
+                sha.count_in = cnt_sh;
                 sha.ps = ps;
                 sha.ray_counters = rays;
-                sha.work = &cnt[kCntShadowWork];
+                sha.work = shadow_work;
                 sha.parts = parts;
                 sha.refill = refill;
                 sha.any_hit = (any_hit || any_point) ? 1u : 0u;
                 sha.error = &cnt[kCntError];
-                launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, block, s);
+                if (overlap) {   // fork: the shadow rays trace beside the next closest-hit launch
+                    BI_HIP_TRY(own, hipEventRecord(eng.ev_shade, s));
+                    BI_HIP_TRY(own, hipStreamWaitEvent(eng.aux_s, eng.ev_shade, 0));
+                    launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, block, eng.aux_s);
+                    BI_HIP_TRY(own, hipEventRecord(eng.ev_shadow[set], eng.aux_s));
+                    set_used[set] = true;
+                } else {
+                    launch_trace_shadow(sha, J.num_paths * sh_mult, lds_stack, block, s);
+                }
                 ++launches_shadow;
                 BI_TRY(mark(3));
                 BI_HIP_TRY(own, hipGetLastError());
@@ -708,7 +747,7 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                     auto& shq = eng.rec_sh.back()[b];
                     shq.resize(hc[kCntShadow]);
                     if (!shq.empty())
-                        BI_HIP_TRY(own, hipMemcpy(shq.data(), eng.sq_pk.p, shq.size() * 4, hipMemcpyDeviceToHost));
+                        BI_HIP_TRY(own, hipMemcpy(shq.data(), eng.sq_pk[0].p, shq.size() * 4, hipMemcpyDeviceToHost));
                     if (b + 1 < D) {
                         auto& ex = eng.rec_ext.back()[b + 1];
                         ex.resize(next);
@@ -716,6 +755,8 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
                     }
                 }
             }
+            for (int t = 0; t < 2; ++t)   // join: the wave's shadow launches have ended
+                if (set_used[t]) BI_HIP_TRY(own, hipStreamWaitEvent(s, eng.ev_shadow[t], 0));
             uint32_t wave_pix = 0;
             for (const Item& it : waves[w]) wave_pix += prep[it.job].npix;
             ResolveArgs ra{};

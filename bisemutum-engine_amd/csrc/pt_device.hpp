@@ -581,7 +581,7 @@ PT_HD float ltc_integrate(V3 P, const Frame& fr, const M3* minv, const V3 q[4], 
 // times (1 - F) base / pi plus the specular one times f0 * norm + f90 * Fresnel term.
 // (Called, not inlined: its table reads and two polygon integrals would otherwise share
 // the shading kernel's registers.)
-static __device__ __noinline__ V3 rect_light_lit(const RectRec& R, const float4* ltc, V3 P, const Frame& fr, V3 V,
+inline __device__ __noinline__ V3 rect_light_lit(const RectRec& R, const float4* ltc, V3 P, const Frame& fr, V3 V,
                                                  const Mat& m) {
     const V3 lv = v3(dot(V, fr.t), dot(V, fr.b), dot(V, fr.n));
     if (lv.z <= 0.0f) return v3(0.0f, 0.0f, 0.0f);

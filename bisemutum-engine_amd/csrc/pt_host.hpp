@@ -93,8 +93,8 @@ struct Knobs {
 // Queues and per-wave buffers, sized for the largest wave rendered so far.
 struct Engine {
     uint32_t wave_cap = 0, wave_slots = 0, wave_sh_mult = 0;
-    DevBuf<float> qa[6], qb[6], ps_f[4], acc, sq_f[9];
-    DevBuf<uint32_t> qa_path, qb_path, sq_pk;
+    DevBuf<float> qa[6], qb[6], ps_f[4], acc, sq_f[2][9];
+    DevBuf<uint32_t> qa_path, qb_path, sq_pk[2];
     DevBuf<HitRec> hit_rec;
     DevBuf<uint64_t> tile_state;           // look-back words of the shade launch's tiles
     DevBuf<uint32_t> counters;             // kCntSlots (ray counters at kCntRays as 4 x u64)
@@ -105,6 +105,12 @@ struct Engine {
     DevBuf<Bvh4Node> empty_tree;           // one node of empty slots (jobs without triangles)
     hipEvent_t ev[3] = {};
     hipEvent_t plan_ev = nullptr;
+    // A segment's shadow launch runs on aux_s, forked after its shading (ev_shade) and
+    // overlapping the next segment's closest-hit launch; segments alternate between two
+    // shadow queue sets, and a set is reused only after its last shadow launch has ended
+    // (ev_shadow[set]); the wave joins aux_s before its resolve.
+    hipStream_t aux_s = nullptr;
+    hipEvent_t ev_shade = nullptr, ev_shadow[2] = {};
     // LBVH builds of independent scenes run concurrently: forked from the call's stream
     // onto these streams (default priority) and joined back by events
     static constexpr int kBuildStreams = 3;

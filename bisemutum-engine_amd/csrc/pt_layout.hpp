@@ -321,13 +321,14 @@ constexpr uint32_t kWorkStride = 32;   // uint32 between part counters: one 128-
 enum CounterSlot : uint32_t {
     kCntExtA = 0,       // entries in extension queue A
     kCntExtB = 1,       // entries in extension queue B
-    kCntShadow = 2,     // entries in the shadow queue
+    kCntShadow = 2,     // entries in shadow queue set 0 (set 1: kCntShadow + 1)
     kCntError = 4,      // nonzero: a kernel hit an internal limit (1 stack, 4 refit wait, 8 step bound,
                         // 16 treelet start list, 32 shade look-back wait)
     kCntRays = 8,       // 4 x 64-bit ray counters (camera, extension, light shadow, sky shadow)
     kCntTraceWork = 32,                                       // part counters of the closest-hit lanes
-    kCntShadowWork = kCntTraceWork + kWorkPartsMax * kWorkStride, // part counters of the shadow lanes
-    kCntSlots = kCntShadowWork + kWorkPartsMax * kWorkStride
+    kCntShadowWork = kCntTraceWork + kWorkPartsMax * kWorkStride, // part counters of the shadow lanes (set 0)
+    kCntShadowWork1 = kCntShadowWork + kWorkPartsMax * kWorkStride, // ... of shadow queue set 1
+    kCntSlots = kCntShadowWork1 + kWorkPartsMax * kWorkStride
 };
 
 }  // namespace bipt

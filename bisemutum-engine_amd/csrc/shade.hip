@@ -394,8 +394,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a) {
                 const uint64_t tot = base + own;
                 *a.count_ext = lb_ext(tot);
                 *a.count_shadow = lb_sh(tot);
-                a.ray_counters[1] += lb_ext(tot);
-                a.ray_counters[2] += lb_sh(tot);
+                // atomic: a shadow launch of the previous segment may be moving its
+                // sky rays between these counters meanwhile
+                atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 1), (unsigned long long)lb_ext(tot));
+                atomicAdd(reinterpret_cast<unsigned long long*>(a.ray_counters + 2), (unsigned long long)lb_sh(tot));
             }
         }
     }
