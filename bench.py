@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--batch-calls", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the serving leg (0 skips it; 20 s = about 4,100 jobs)")
+    ap.add_argument("--serve-spp-per-round", type=int, default=16,
+                    help="samples of each admitted job per scheduling round of the serving leg")
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after one warm-up)")
     ap.add_argument("--profile-nrc", action="store_true",
                     help="only run the radiance-cache frames that profiles/kernels_nrc_c1_1080p.md profiles "
@@ -225,7 +227,7 @@ def main():
     if not args.no_batch:
         result["batched"] = batch_leg(args.batch_calls)
     if args.serve_seconds > 0:
-        result["serving"] = serve_leg(args.serve_seconds)
+        result["serving"] = serve_leg(args.serve_seconds, spp_per_round=args.serve_spp_per_round)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_leg(scene, cfg, args.cpu_runs)
     pt.close()
@@ -310,7 +312,7 @@ def batch_leg(steps):
             "autotune_s": round(tune_s, 1)}
 
 
-def serve_leg(seconds, rate=200.0):
+def serve_leg(seconds, rate=200.0, spp_per_round=16):
     """Serving form: Poisson arrivals at `rate` jobs/s for `seconds` over 8 resident
     templates (10k-1M tris); orbit cameras, 256x256 / 512x512 / 1280x720, spp 1/4/16,
     depth 6, 96 GB HBM budget.  The plain form serves the same arrivals first-come
@@ -330,7 +332,7 @@ def serve_leg(seconds, rate=200.0):
     kw = dict(max_bounces=scenes.C3_MAX_BOUNCES, seed=scenes.C3_SEED)
     log(f"serve leg: {len(jobs)} jobs over {seconds}s, templates {[s.num_tris for s in templates]}")
 
-    srv = bi_pt.Server(hbm_budget_bytes=96 << 30, spp_per_round=4)
+    srv = bi_pt.Server(hbm_budget_bytes=96 << 30, spp_per_round=spp_per_round)
     sids = [srv.add_scene(tr) for tr in tracers]
     torch.cuda.synchronize()
     ids = []
@@ -385,7 +387,7 @@ def serve_leg(seconds, rate=200.0):
     return {"workload": f"Poisson arrivals {rate:g} jobs/s for {seconds:g}s ({len(jobs)} jobs) over 8 resident "
                         f"c0-style templates ({min(s.num_tris for s in templates)}-"
                         f"{max(s.num_tris for s in templates)} tris), orbit cameras, 256x256 / 512x512 / "
-                        f"1280x720, spp 1/4/16, depth 6, 96 GB HBM budget, 4 spp per job per round",
+                        f"1280x720, spp 1/4/16, depth 6, 96 GB HBM budget, {spp_per_round} spp per job per round",
             "jobs": len(jobs), "served_s": round(t_end, 3), "rays": int(st["rays"]),
             "mrays_s": round(st["rays"] / t_end / 1e6, 1),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),

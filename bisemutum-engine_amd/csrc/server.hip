@@ -47,7 +47,7 @@ struct bi_pt_server : bipt::Owner {
     uint64_t scene_bytes = 0;      // resident scenes and their BVHs
     uint64_t job_bytes = 0;        // admitted jobs' images and outputs
     uint32_t max_jobs = 64;        // admitted jobs per round
-    uint32_t spp_per_round = 4;    // samples of each job per round
+    uint32_t spp_per_round = 16;   // samples of each job per round
     std::deque<uint64_t> waiting;
     std::vector<uint64_t> running;
     std::unordered_map<uint64_t, SrvJob> jobs;

@@ -511,7 +511,7 @@ typedef struct bi_pt_server_desc {
     int32_t device;
     uint32_t max_jobs_per_round;   /* 0 = 64 */
     uint64_t hbm_budget_bytes;     /* 0 = unlimited */
-    uint32_t spp_per_round;        /* samples of each job per round; 0 = 4 */
+    uint32_t spp_per_round;        /* samples of each job per round; 0 = 16 */
     uint32_t wave_paths;           /* paths in flight per wave; 0 = 2^25 (the budget reserves its queues) */
 } bi_pt_server_desc;
 typedef struct bi_pt_request {

@@ -157,3 +157,32 @@ def test_batch_errors(c0_scene):
     assert np.all(np.isfinite(t.image()))
     t.close()
     b.close()
+
+
+def test_batch_mixed_shadow_casting_lights_match_oracle(oracle_lib):
+    """Two jobs in one wave whose light 0 differs: casting shadows (its radiance slot written
+    by the shadow launch, which overlaps the next segment) and not (the same slot written by
+    the shading): both bit for bit as the oracle renders them, over several calls."""
+    print("case: batch of c0 with a shadow-casting and a shadowless sun, 64x48 2spp depth 6, 3 calls", flush=True)
+    import dataclasses
+    base = scenes.scene_c0()
+    lit = dataclasses.replace(base, dir_lights=scenes.dir_light((2.6, 2.5, 2.3), (0.35, 0.85, 0.4))[None])
+    bare = dataclasses.replace(base, dir_lights=scenes.dir_light((2.6, 2.5, 2.3), (0.35, 0.85, 0.4),
+                                                                 cast_shadow=False)[None])
+    cam = scenes.camera_c0(64, 48)
+    b = bi_pt.Batch()
+    ts = []
+    for sc in (lit, bare):
+        t = bi_pt.PathTracer()
+        t.set_scene(sc)
+        ts.append(t)
+    ok = []
+    for call in range(3):
+        b.render([bi_pt.Job(t, cam, spp=2, max_bounces=6, seed=40 + call) for t in ts])
+        for sc, t in zip((lit, bare), ts):
+            want = oracle_lib.OracleScene(sc).render32(cam, 6, 40 + call, spp=2)["sums"]
+            ok.append(np.array_equal(t.sums(cam.width * cam.height).view(np.uint32), want.view(np.uint32)))
+    for t in ts:
+        t.close()
+    b.close()
+    assert all(ok), ok
