@@ -466,6 +466,13 @@ def bootstrap_ci(values, n=2000, seed=0x5EED):
     return float(v.mean()), float(np.percentile(means, 2.5)), float(np.percentile(means, 97.5))
 
 
+def paired_gain(base, other):
+    """Mean of base - other over frames and its 95% percentile bootstrap interval."""
+    import numpy as np
+    m, lo, hi = bootstrap_ci(np.asarray(base, np.float64) - np.asarray(other, np.float64))
+    return [round(m, 4), round(lo, 4), round(hi, 4)]
+
+
 def constant_cache_params(rgb):
     """Network weights whose output is the constant rgb for every input: the constant input
     (59) carried through one hidden unit of each layer, the output row scaled by rgb."""
@@ -529,6 +536,11 @@ def nrc_leg(pt, cam, cfg, ref, plain, frames, warmup):
            "relmse_ratio_equal_time": round(eq["relative_mse"] / plain["relative_mse"], 4),
            "relmse_ratio_to_constant_cache": round(eq["relative_mse"] / control["relative_mse"], 4),
            "network_beats_constant_beyond_ci": bool(eq["relative_mse_ci95"][1] < control["relative_mse_ci95"][0]),
+           # paired over frames (the network and the constant cache render the same paths):
+           # the constant's relative MSE minus the network's, mean and 95% bootstrap interval
+           "paired_relmse_gain_ci95": paired_gain(control["relative_mse_per_frame"], eq["relative_mse_per_frame"]),
+           "frames_network_better": int(sum(c > a for c, a in zip(control["relative_mse_per_frame"],
+                                                                    eq["relative_mse_per_frame"]))),
            "warmup_s": round(warm_s, 2), "train_steps": int(st["steps"]), "last_loss": round(st["last_loss"], 4),
            "valid_training_samples": int(st["last_valid_samples"])}
     if plain_rel:
