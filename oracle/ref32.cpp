@@ -750,9 +750,10 @@ Hit trace_closest(const Scene& s, const Ray& r) {
     return h;
 }
 
-// Any-hit BVH4 traversal in the shadow kernel's order: the first hit internal child
-// (lowest slot) is entered and the other hit internal children pushed in slot order,
-// the hit leaves intersected in slot order.  The answer does not depend on the order.
+// Any-hit BVH4 traversal in the shadow kernel's order: the hit internal child whose box
+// the ray enters farthest away (the lowest slot among equals) is entered and the other hit
+// internal children pushed in slot order, the hit leaves intersected in slot order.  The
+// answer does not depend on the order.
 bool trace_any(const Scene& s, const Ray& r) {
     if (s.num_tris == 0) return false;
     F3 inv = safe_inv(r.d);
@@ -764,11 +765,15 @@ bool trace_any(const Scene& s, const Ray& r) {
         int next = -1;
         bool ok = true;
         bool hit[4];
+        float tn[4];
+        int far = -1;
         for (int j = 0; j < 4; ++j) {
-            float tn;
-            hit[j] = box4_hit(nd, j, r.o, inv, r.tmin, r.tmax, &tn);
+            hit[j] = box4_hit(nd, j, r.o, inv, r.tmin, r.tmax, &tn[j]);
+            if (hit[j] && nd.child[j] >= 0 && (far < 0 || tn[j] > tn[far])) far = j;
+        }
+        for (int j = 0; j < 4; ++j) {
             if (!hit[j] || nd.child[j] < 0) continue;
-            if (next < 0) next = nd.child[j];
+            if (j == far) next = nd.child[j];
             else if (!ok) continue;
             else if (sp >= kStackMax) ok = false;
             else stack[sp++] = nd.child[j];
