@@ -59,14 +59,15 @@ void build_sky_table(const float* rgba, uint32_t W, uint32_t H, uint32_t layout,
     total = marg[H];
 }
 
-// Guide tables that bracket the two searches of the sampler: for k in [0, n), the first
-// index i in [0, n - 1) with cum[i + 1] > ((float)k / n) * sum, else n - 1.  The sampler
-// searches between the entries for k - 1 and k + 2, k = (int)(u * n), and so ends at the
-// index of a binary search over the whole range.
+// Guide tables that bracket the two searches of the sampler: N = kSkyGuide n entries, for
+// k in [0, N) the first index i in [0, n - 1) with cum[i + 1] > ((float)k / N) * sum, else
+// n - 1.  The sampler searches between the entries for k - 1 and k + 2, k = (int)(u * N),
+// and so ends at the index of a binary search over the whole range.
 void build_guide(const float* cum, uint32_t n, float sum, uint32_t* guide) {
+    const uint32_t N = n * kSkyGuide;
     uint32_t i = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        const float thr = ((float)k / (float)n) * sum;
+    for (uint32_t k = 0; k < N; ++k) {
+        const float thr = ((float)k / (float)N) * sum;
         while (i < n - 1 && !(cum[i + 1] > thr)) ++i;
         guide[k] = i;
     }
@@ -630,9 +631,10 @@ bi_pt_status set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, 
         std::vector<float> wt, cc, rows, marg;
         float total = 0.0f;
         build_sky_table(rgba, w, h, layout, color, wt, cc, rows, marg, total);
-        std::vector<uint32_t> mg(h), rg((size_t)w * h);
+        std::vector<uint32_t> mg((size_t)h * kSkyGuide), rg((size_t)w * h * kSkyGuide);
         build_guide(marg.data(), h, total, mg.data());
-        for (uint32_t i = 0; i < h; ++i) build_guide(&cc[(size_t)i * (w + 1)], w, rows[i], &rg[(size_t)i * w]);
+        for (uint32_t i = 0; i < h; ++i)
+            build_guide(&cc[(size_t)i * (w + 1)], w, rows[i], &rg[(size_t)i * w * kSkyGuide]);
         BI_TRY(dalloc(ctx, ctx->sky_mguide, mg.size()));
         BI_TRY(dalloc(ctx, ctx->sky_rguide, rg.size()));
         BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_mguide.p, mg.data(), mg.size() * 4, hipMemcpyHostToDevice));

@@ -777,12 +777,14 @@ __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
 }
 // The first index i in [0, n - 1) with cum[i + 1] > t (else n - 1) for t = u * sum: the
 // index a binary search over the running sums finds, searched only between the guide
-// table's entries for k - 1 and k + 2, k = (int)(u * n) (rounding moves k by at most one,
-// so their thresholds stay at or below and above t: the answer lies between them).
+// table's entries for k - 1 and k + 2, k = (int)(u * N) over its N = kSkyGuide n entries
+// (rounding moves k by at most one, so their thresholds stay at or below and above t: the
+// answer lies between them).
 __device__ __forceinline__ int guided_search(const float* cum, const uint32_t* guide, int n, float u, float t) {
-    const int k = (int)(u * (float)n);
-    int lo = (int)guide[k - 1 < 0 ? 0 : (k - 1 > n - 1 ? n - 1 : k - 1)];
-    int hi = k + 2 > n - 1 ? n - 1 : (int)guide[k + 2];
+    const int N = n * (int)kSkyGuide;
+    const int k = (int)(u * (float)N);
+    int lo = (int)guide[k - 1 < 0 ? 0 : (k - 1 > N - 1 ? N - 1 : k - 1)];
+    int hi = k + 2 > N - 1 ? n - 1 : (int)guide[k + 2];
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (cum[mid + 1] > t) hi = mid; else lo = mid + 1;
@@ -803,7 +805,7 @@ __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u
     dv = dv < 0.0f ? 0.0f : (dv >= 1.0f ? 0x1.fffffep-1f : dv);
     float t1 = u1 * rs;
     const float* cc = s.ccdf + (size_t)i * (W + 1);
-    int j = guided_search(cc, s.rguide + (size_t)i * W, W, u1, t1);
+    int j = guided_search(cc, s.rguide + (size_t)i * kSkyGuide * W, W, u1, t1);
     float w = s.weight[(size_t)i * W + j];
     if (!(w > 0.0f)) return false;
     float du = (t1 - cc[j]) / w;

@@ -143,6 +143,9 @@ static_assert(sizeof(LightRec) == 64, "light record is 64 bytes");
 // position0..3 (centre + (w, h), (-w, h), (-w, -h), (w, -h) in the light's frame),
 // emission, one- or two-sided.  Rect lights cast no shadows; each shaded vertex adds
 // their linearly-transformed-cosine integrals.
+// Guide entries per row (marginal search) and per texel (a row's search): four bracket the
+// binary searches of an atrium's sky to about one step each.
+constexpr uint32_t kSkyGuide = 4;
 constexpr uint32_t kMaxRectLights = 16;
 struct RectRec {
     float emission[3];
@@ -191,9 +194,10 @@ struct SkyParams {
     const float* ccdf;     // per row: W+1 running sums
     const float* rows;     // per row sum
     const float* marg;     // H+1 running sums of rows
-    // guide tables of the two searches: mguide[k] = the first row i with marg[i+1] above
-    // (k / H) * total; rguide[i*W + k] = the first texel j of row i with ccdf[j+1] above
-    // (k / W) * rows[i] (the last index where none is)
+    // guide tables of the two searches, kSkyGuide entries per row / texel: mguide[k] = the
+    // first row i with marg[i+1] above (k / (kSkyGuide H)) * total; rguide[i*kSkyGuide*W +
+    // k] = the first texel j of row i with ccdf[j+1] above (k / (kSkyGuide W)) * rows[i]
+    // (the last index where none is)
     const uint32_t* mguide;
     const uint32_t* rguide;
     uint32_t width, height;
