@@ -21,14 +21,12 @@ namespace {
 // Importance table of the sky, built in FP32 in a fixed order on the host: per texel
 // luminance(texel * colour) times sin(theta) of its row (equirect) or 1/r^3 of its
 // centre (cube face texel at face coordinates (a, b), r^2 = 1 + a^2 + b^2: the
-// texel's solid angle up to a constant), then running sums per row and over rows.
+// texel's solid angle up to a constant), then the sum of each row and their total.
 void build_sky_table(const float* rgba, uint32_t W, uint32_t H, uint32_t layout, const float color[3],
-                     std::vector<float>& w, std::vector<float>& ccdf, std::vector<float>& rows,
-                     std::vector<float>& marg, float& total) {
+                     std::vector<float>& w, std::vector<float>& rows, float& total) {
     w.assign((size_t)W * H, 0.0f);
-    ccdf.assign((size_t)(W + 1) * H, 0.0f);
     rows.assign(H, 0.0f);
-    marg.assign(H + 1, 0.0f);
+    total = 0.0f;
     const float pi_over_h = kPi / (float)H;
     for (uint32_t i = 0; i < H; ++i) {
         const uint32_t r = layout == kSkyCube ? i % W : i;
@@ -49,28 +47,41 @@ void build_sky_table(const float* rgba, uint32_t W, uint32_t H, uint32_t layout,
             }
             if (!(x > 0.0f)) x = 0.0f;
             w[(size_t)i * W + j] = x;
-            ccdf[(size_t)i * (W + 1) + j] = run;
             run = run + x;
         }
-        ccdf[(size_t)i * (W + 1) + W] = run;
         rows[i] = run;
-        marg[i + 1] = marg[i] + run;
+        total = total + run;
     }
-    total = marg[H];
 }
 
-// Guide tables that bracket the two searches of the sampler: N = kSkyGuide n entries, for
-// k in [0, N) the first index i in [0, n - 1) with cum[i + 1] > ((float)k / N) * sum, else
-// n - 1.  The sampler searches between the entries for k - 1 and k + 2, k = (int)(u * N),
-// and so ends at the index of a binary search over the whole range.
-void build_guide(const float* cum, uint32_t n, float sum, uint32_t* guide) {
-    const uint32_t N = n * kSkyGuide;
-    uint32_t i = 0;
-    for (uint32_t k = 0; k < N; ++k) {
-        const float thr = ((float)k / (float)N) * sum;
-        while (i < n - 1 && !(cum[i + 1] > thr)) ++i;
-        guide[k] = i;
+// Vose's alias tables (Vose 1991, "A linear algorithm for generating random numbers with
+// a given distribution") over n weights of sum `sum`: entry k keeps itself with probability
+// prob[k], else goes to alias[k].  The weights scaled by n / sum are split into entries
+// below 1 and the rest, each a stack in index order; the top small entry takes its
+// probability and the top large one as its alias, which keeps what is left over.  Entries
+// left in either stack at the end keep themselves.  The CPU restatement takes the same steps.
+void build_alias(const float* w, uint32_t n, float sum, float* prob, uint32_t* alias) {
+    std::vector<float> q(n);
+    std::vector<uint32_t> small, large;
+    small.reserve(n);
+    large.reserve(n);
+    const float scale = sum > 0.0f ? (float)n / sum : 0.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+        q[k] = w[k] * scale;
+        (q[k] < 1.0f ? small : large).push_back(k);
     }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t l = small.back();
+        small.pop_back();
+        const uint32_t g = large.back();
+        large.pop_back();
+        prob[l] = q[l];
+        alias[l] = g;
+        q[g] = (q[g] + q[l]) - 1.0f;
+        (q[g] < 1.0f ? small : large).push_back(g);
+    }
+    for (uint32_t k : large) { prob[k] = 1.0f; alias[k] = k; }
+    for (uint32_t k : small) { prob[k] = 1.0f; alias[k] = k; }
 }
 
 }  // namespace
@@ -254,8 +265,8 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     (void)hipDeviceSynchronize();
     dfree(ctx, ctx->pos); dfree(ctx, ctx->nrm); dfree(ctx, ctx->idx); dfree(ctx, ctx->inst); dfree(ctx, ctx->mats);
     dfree(ctx, ctx->rects); dfree(ctx, ctx->ltc);
-    dfree(ctx, ctx->sky_tex); dfree(ctx, ctx->sky_w); dfree(ctx, ctx->sky_ccdf); dfree(ctx, ctx->sky_rows);
-    dfree(ctx, ctx->sky_marg); dfree(ctx, ctx->sky_mguide); dfree(ctx, ctx->sky_rguide);
+    dfree(ctx, ctx->sky_tex); dfree(ctx, ctx->sky_w);
+    dfree(ctx, ctx->sky_row_alias); dfree(ctx, ctx->sky_tex_alias);
     dfree(ctx, ctx->tri_v); dfree(ctx, ctx->tri_n); dfree(ctx, ctx->cb_out); dfree(ctx, ctx->tri_mat);
     dfree(ctx, ctx->cb_ord); dfree(ctx, ctx->keys0); dfree(ctx, ctx->keys1); dfree(ctx, ctx->vals0);
     dfree(ctx, ctx->vals1); dfree(ctx, ctx->hist); dfree(ctx, ctx->refit_cnt); dfree(ctx, ctx->nodes);
@@ -628,34 +639,35 @@ bi_pt_status set_sky(bi_pt_ctx* ctx, const float* rgba, uint32_t w, uint32_t h, 
     std::memcpy(sp.color, color, sizeof sp.color);
     std::memcpy(sp.rot, rot, sizeof sp.rot);
     if (w) {
-        std::vector<float> wt, cc, rows, marg;
+        std::vector<float> wt, rows;
         float total = 0.0f;
-        build_sky_table(rgba, w, h, layout, color, wt, cc, rows, marg, total);
-        std::vector<uint32_t> mg((size_t)h * kSkyGuide), rg((size_t)w * h * kSkyGuide);
-        build_guide(marg.data(), h, total, mg.data());
-        for (uint32_t i = 0; i < h; ++i)
-            build_guide(&cc[(size_t)i * (w + 1)], w, rows[i], &rg[(size_t)i * w * kSkyGuide]);
-        BI_TRY(dalloc(ctx, ctx->sky_mguide, mg.size()));
-        BI_TRY(dalloc(ctx, ctx->sky_rguide, rg.size()));
-        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_mguide.p, mg.data(), mg.size() * 4, hipMemcpyHostToDevice));
-        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_rguide.p, rg.data(), rg.size() * 4, hipMemcpyHostToDevice));
+        build_sky_table(rgba, w, h, layout, color, wt, rows, total);
+        // alias tables: rows by their sums, each row's texels by their weights
+        std::vector<SkyRowAlias> ra(h);
+        std::vector<SkyTexAlias> ta((size_t)w * h);
+        {
+            std::vector<float> prob(std::max(w, h));
+            std::vector<uint32_t> al(std::max(w, h));
+            build_alias(rows.data(), h, total, prob.data(), al.data());
+            for (uint32_t i = 0; i < h; ++i) ra[i] = SkyRowAlias{prob[i], al[i]};
+            for (uint32_t i = 0; i < h; ++i) {
+                const float* wr = &wt[(size_t)i * w];
+                build_alias(wr, w, rows[i], prob.data(), al.data());
+                for (uint32_t j = 0; j < w; ++j) ta[(size_t)i * w + j] = SkyTexAlias{prob[j], al[j], wr[j], wr[al[j]]};
+            }
+        }
+        BI_TRY(dalloc(ctx, ctx->sky_row_alias, ra.size()));
+        BI_TRY(dalloc(ctx, ctx->sky_tex_alias, ta.size()));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_row_alias.p, ra.data(), ra.size() * sizeof(SkyRowAlias), hipMemcpyHostToDevice));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_tex_alias.p, ta.data(), ta.size() * sizeof(SkyTexAlias), hipMemcpyHostToDevice));
         BI_TRY(dalloc(ctx, ctx->sky_tex, (size_t)w * h));
         BI_TRY(dalloc(ctx, ctx->sky_w, wt.size()));
-        BI_TRY(dalloc(ctx, ctx->sky_ccdf, cc.size()));
-        BI_TRY(dalloc(ctx, ctx->sky_rows, rows.size()));
-        BI_TRY(dalloc(ctx, ctx->sky_marg, marg.size()));
         BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_tex.p, rgba, (size_t)w * h * 16, hipMemcpyHostToDevice));
         BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_w.p, wt.data(), wt.size() * 4, hipMemcpyHostToDevice));
-        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_ccdf.p, cc.data(), cc.size() * 4, hipMemcpyHostToDevice));
-        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
-        BI_HIP_TRY(ctx, hipMemcpy(ctx->sky_marg.p, marg.data(), marg.size() * 4, hipMemcpyHostToDevice));
         sp.texels = ctx->sky_tex.p;
         sp.weight = ctx->sky_w.p;
-        sp.ccdf = ctx->sky_ccdf.p;
-        sp.rows = ctx->sky_rows.p;
-        sp.marg = ctx->sky_marg.p;
-        sp.mguide = ctx->sky_mguide.p;
-        sp.rguide = ctx->sky_rguide.p;
+        sp.row_alias = ctx->sky_row_alias.p;
+        sp.tex_alias = ctx->sky_tex_alias.p;
         sp.total = total;
         if (total > 0.0f)
             sp.pdf_scale = layout == kSkyCube ? ((float)w * (float)w / total) * 0.25f : (float)(w * h) / total;

@@ -775,40 +775,45 @@ __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
     if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
     return (w * s.pdf_scale) / (kTwoPiSq * st);
 }
-// The first index i in [0, n - 1) with cum[i + 1] > t (else n - 1) for t = u * sum: the
-// index a binary search over the running sums finds, searched only between the guide
-// table's entries for k - 1 and k + 2, k = (int)(u * N) over its N = kSkyGuide n entries
-// (rounding moves k by at most one, so their thresholds stay at or below and above t: the
-// answer lies between them).
-__device__ __forceinline__ int guided_search(const float* cum, const uint32_t* guide, int n, float u, float t) {
-    const int N = n * (int)kSkyGuide;
-    const int k = (int)(u * (float)N);
-    int lo = (int)guide[k - 1 < 0 ? 0 : (k - 1 > N - 1 ? N - 1 : k - 1)];
-    int hi = k + 2 > N - 1 ? n - 1 : (int)guide[k + 2];
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (cum[mid + 1] > t) hi = mid; else lo = mid + 1;
-    }
-    return lo;
-}
-// Piecewise-constant importance sampling of the texel weights: a row from the running
-// row sums, a texel from the row's running sums, a uniform point inside the texel.
+// Piecewise-constant importance sampling of the texel weights by alias tables: u0 picks a
+// row entry and its fraction decides between the entry and its alias, the rescaled
+// fraction placing the point inside the row; u1 the same within the row's texels.
 __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u1, V3* dir, float* pdf) {
     *pdf = 0.0f;
     if (!(s.total > 0.0f)) return false;
     const int W = (int)s.width, H = (int)s.height;
-    float t0 = u0 * s.total;
-    int i = guided_search(s.marg, s.mguide, H, u0, t0);
-    float rs = s.rows[i];
-    if (!(rs > 0.0f)) return false;
-    float dv = (t0 - s.marg[i]) / rs;
+    const float fr = u0 * (float)H;
+    int rc = (int)fr;
+    rc = rc > H - 1 ? H - 1 : rc;
+    const float x = fr - (float)rc;
+    const SkyRowAlias ra = s.row_alias[rc];
+    int i;
+    float dv;
+    if (x < ra.prob) {
+        i = rc;
+        dv = x / ra.prob;
+    } else {
+        i = (int)ra.alias;
+        dv = (x - ra.prob) / (1.0f - ra.prob);
+    }
     dv = dv < 0.0f ? 0.0f : (dv >= 1.0f ? 0x1.fffffep-1f : dv);
-    float t1 = u1 * rs;
-    const float* cc = s.ccdf + (size_t)i * (W + 1);
-    int j = guided_search(cc, s.rguide + (size_t)i * kSkyGuide * W, W, u1, t1);
-    float w = s.weight[(size_t)i * W + j];
+    const float fc = u1 * (float)W;
+    int c = (int)fc;
+    c = c > W - 1 ? W - 1 : c;
+    const float y = fc - (float)c;
+    const SkyTexAlias ta = s.tex_alias[(size_t)i * W + c];
+    int j;
+    float du, w;
+    if (y < ta.prob) {
+        j = c;
+        du = y / ta.prob;
+        w = ta.w;
+    } else {
+        j = (int)ta.alias;
+        du = (y - ta.prob) / (1.0f - ta.prob);
+        w = ta.w_alias;
+    }
     if (!(w > 0.0f)) return false;
-    float du = (t1 - cc[j]) / w;
     du = du < 0.0f ? 0.0f : (du >= 1.0f ? 0x1.fffffep-1f : du);
     V3 ds;
     if (s.layout == kSkyCube) {

@@ -179,8 +179,9 @@ struct bi_pt_ctx : bipt::Owner {
     std::vector<float> ltc_host;
     // sky
     bipt::DevBuf<float4> sky_tex;
-    bipt::DevBuf<float> sky_w, sky_ccdf, sky_rows, sky_marg;
-    bipt::DevBuf<uint32_t> sky_mguide, sky_rguide;
+    bipt::DevBuf<float> sky_w;
+    bipt::DevBuf<bipt::SkyRowAlias> sky_row_alias;
+    bipt::DevBuf<bipt::SkyTexAlias> sky_tex_alias;
     bipt::SkyParams sky{};
     std::vector<float> sky_host;
     bool sky_set = false;

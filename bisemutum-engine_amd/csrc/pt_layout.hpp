@@ -143,9 +143,6 @@ static_assert(sizeof(LightRec) == 64, "light record is 64 bytes");
 // position0..3 (centre + (w, h), (-w, h), (-w, -h), (w, -h) in the light's frame),
 // emission, one- or two-sided.  Rect lights cast no shadows; each shaded vertex adds
 // their linearly-transformed-cosine integrals.
-// Guide entries per row (marginal search) and per texel (a row's search): four bracket the
-// binary searches of an atrium's sky to about one step each.
-constexpr uint32_t kSkyGuide = 4;
 constexpr uint32_t kMaxRectLights = 16;
 struct RectRec {
     float emission[3];
@@ -188,18 +185,23 @@ struct CameraParams {
 // width x width faces stored one after the other (+X, -X, +Y, -Y, +Z, -Z; height =
 // 6 * width), the layers of the engine's skybox texture.
 constexpr uint32_t kSkyEquirect = 0u, kSkyCube = 1u;
+// Alias tables of the sky's importance sampling (Vose's method): row r keeps itself with
+// probability prob, else goes to row alias; a texel entry also carries its own weight and
+// its alias's, so the sampled texel's weight comes with the entry.
+struct SkyRowAlias {
+    float prob;
+    uint32_t alias;
+};
+struct SkyTexAlias {
+    float prob;
+    uint32_t alias;
+    float w, w_alias;
+};
 struct SkyParams {
     const float4* texels;
     const float* weight;   // per texel: luminance * sin(theta) (equirect) or * 1/r^3 (cube)
-    const float* ccdf;     // per row: W+1 running sums
-    const float* rows;     // per row sum
-    const float* marg;     // H+1 running sums of rows
-    // guide tables of the two searches, kSkyGuide entries per row / texel: mguide[k] = the
-    // first row i with marg[i+1] above (k / (kSkyGuide H)) * total; rguide[i*kSkyGuide*W +
-    // k] = the first texel j of row i with ccdf[j+1] above (k / (kSkyGuide W)) * rows[i]
-    // (the last index where none is)
-    const uint32_t* mguide;
-    const uint32_t* rguide;
+    const SkyRowAlias* row_alias;   // per row (rows weighted by their sums)
+    const SkyTexAlias* tex_alias;   // per texel (each row's texels by their weights)
     uint32_t width, height;
     uint32_t layout;
     float total;

@@ -1081,34 +1081,44 @@ float sky_pdf(const Scene& s, F3 d) {
     if (!(st > 0.0f) || !(w > 0.0f)) return 0.0f;
     return (w * s.sky_pdf_scale) / (kTwoPiSq * st);
 }
-// Piecewise-constant importance sampling of the texel weights (a row by the running
-// row sums, a texel by the row's running sums, a uniform point in the texel).
+// Piecewise-constant importance sampling of the texel weights by alias tables: u0 scaled
+// by the row count picks a row entry, whose fraction chooses the entry or its alias and,
+// rescaled, places the point in the row; u1 the same over the row's texels.
 bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
     *pdf = 0.0f;
     if (!(s.sky_total > 0.0f)) return false;
     const int W = (int)s.sky_w, H = (int)s.sky_h;
-    float t0 = u0 * s.sky_total;
-    int lo = 0, hi = H - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (s.sky_marg[mid + 1] > t0) hi = mid; else lo = mid + 1;
+    const float fr = u0 * (float)H;
+    int rc = (int)fr;
+    rc = rc > H - 1 ? H - 1 : rc;
+    const float x = fr - (float)rc;
+    const float rp = s.sky_row_prob[rc];
+    int i;
+    float dv;
+    if (x < rp) {
+        i = rc;
+        dv = x / rp;
+    } else {
+        i = (int)s.sky_row_alias[rc];
+        dv = (x - rp) / (1.0f - rp);
     }
-    int i = lo;
-    float rs = s.sky_rows[i];
-    if (!(rs > 0.0f)) return false;
-    float dv = (t0 - s.sky_marg[i]) / rs;
     dv = dv < 0.0f ? 0.0f : (dv >= 1.0f ? 0x1.fffffep-1f : dv);
-    float t1 = u1 * rs;
-    const float* cc = &s.sky_ccdf[(size_t)i * (W + 1)];
-    lo = 0; hi = W - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (cc[mid + 1] > t1) hi = mid; else lo = mid + 1;
+    const float fc = u1 * (float)W;
+    int c = (int)fc;
+    c = c > W - 1 ? W - 1 : c;
+    const float y = fc - (float)c;
+    const float tp = s.sky_tex_prob[(size_t)i * W + c];
+    int j;
+    float du;
+    if (y < tp) {
+        j = c;
+        du = y / tp;
+    } else {
+        j = (int)s.sky_tex_alias[(size_t)i * W + c];
+        du = (y - tp) / (1.0f - tp);
     }
-    int j = lo;
-    float w = s.sky_w_tex[(size_t)i * W + j];
+    const float w = s.sky_w_tex[(size_t)i * W + j];
     if (!(w > 0.0f)) return false;
-    float du = (t1 - cc[j]) / w;
     du = du < 0.0f ? 0.0f : (du >= 1.0f ? 0x1.fffffep-1f : du);
     F3 ds;
     if (s.sky_layout == 1u) {

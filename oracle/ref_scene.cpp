@@ -74,6 +74,33 @@ void flatten(const ora_scene_desc& d, Scene& s) {
     }
 }
 
+// Alias tables (Vose 1991): the weights scaled by n / sum split into entries below 1 and
+// the rest, each a stack in index order; the top small entry keeps its scaled weight as its
+// probability and takes the top large one as alias, which keeps what is left over (its
+// scaled weight + the small one's - 1) and goes back to the stack it now belongs in.
+// Entries left over keep themselves (probability 1).
+void build_alias(const float* w, uint32_t n, float sum, float* prob, uint32_t* alias) {
+    std::vector<float> q(n);
+    std::vector<uint32_t> small, large;
+    const float scale = sum > 0.0f ? (float)n / sum : 0.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+        q[k] = w[k] * scale;
+        if (q[k] < 1.0f) small.push_back(k); else large.push_back(k);
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t l = small.back();
+        small.pop_back();
+        const uint32_t g = large.back();
+        large.pop_back();
+        prob[l] = q[l];
+        alias[l] = g;
+        q[g] = (q[g] + q[l]) - 1.0f;
+        if (q[g] < 1.0f) small.push_back(g); else large.push_back(g);
+    }
+    for (uint32_t k : large) { prob[k] = 1.0f; alias[k] = k; }
+    for (uint32_t k : small) { prob[k] = 1.0f; alias[k] = k; }
+}
+
 // Importance table over the sky texels: w = luminance(texel * sky_color) times
 // sin(theta_row) (equirect) or 1/r^3 at the texel centre (cube face texel at face
 // coordinates a = (2j+1)/S - 1, b = (2i+1)/S - 1, r^2 = 1 + a^2 + b^2, proportional to
@@ -115,6 +142,14 @@ void build_sky_table(Scene& s) {
         s.sky_marg[i + 1] = s.sky_marg[i] + run;
     }
     s.sky_total = s.sky_marg[H];
+    s.sky_row_prob.assign(H, 1.0f);
+    s.sky_row_alias.assign(H, 0u);
+    s.sky_tex_prob.assign((size_t)W * H, 1.0f);
+    s.sky_tex_alias.assign((size_t)W * H, 0u);
+    build_alias(s.sky_rows.data(), H, s.sky_total, s.sky_row_prob.data(), s.sky_row_alias.data());
+    for (uint32_t i = 0; i < H; ++i)
+        build_alias(&s.sky_w_tex[(size_t)i * W], W, s.sky_rows[i], &s.sky_tex_prob[(size_t)i * W],
+                    &s.sky_tex_alias[(size_t)i * W]);
     if (s.sky_total > 0.0f)
         s.sky_pdf_scale = cube ? ((float)W * (float)W / s.sky_total) * 0.25f : (float)(W * H) / s.sky_total;
 }

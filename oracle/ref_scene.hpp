@@ -55,6 +55,9 @@ struct Scene {
     float sky_color[3] = {0, 0, 0};
     float sky_rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     std::vector<float> sky_w_tex;  // per texel luminance * sin(theta_row) (cube: * 1/r^3)
+    // alias tables (Vose): rows by their sums, each row's texels by their weights
+    std::vector<float> sky_row_prob, sky_tex_prob;
+    std::vector<uint32_t> sky_row_alias, sky_tex_alias;
     std::vector<float> sky_ccdf;   // per row W+1 running sums
     std::vector<float> sky_rows;   // per row sum
     std::vector<float> sky_marg;   // H+1 running sums of rows
