@@ -704,13 +704,11 @@ __device__ __forceinline__ V3 cube_tap(const SkyParams& s, uint32_t f, int i, in
     const V3 m = (c0 + c1) + c2;
     return v3(m.x / 3.0f, m.y / 3.0f, m.z / 3.0f);
 }
-// Bilinear lookup in the face the direction selects (texel centres at (k+0.5)/S).
-__device__ __forceinline__ V3 cube_eval(const SkyParams& s, V3 ds) {
-    float sc, tc, ma;
-    const uint32_t f = cube_face(ds, &sc, &tc, &ma);
-    if (!(ma > 0.0f)) return v3(0.0f, 0.0f, 0.0f);
+// Bilinear lookup of face f at face coordinates (a, b) in [-1, 1] (texel centres at
+// (k+0.5)/S).
+__device__ __forceinline__ V3 cube_eval_face(const SkyParams& s, uint32_t f, float a, float b) {
     const float S = (float)s.width;
-    const float x = ((sc / ma + 1.0f) * 0.5f) * S - 0.5f, y = ((tc / ma + 1.0f) * 0.5f) * S - 0.5f;
+    const float x = ((a + 1.0f) * 0.5f) * S - 0.5f, y = ((b + 1.0f) * 0.5f) * S - 0.5f;
     const float fx0 = __builtin_floorf(x), fy0 = __builtin_floorf(y);
     const float fx = x - fx0, fy = y - fy0;
     const int j0 = (int)fx0, i0 = (int)fy0;
@@ -719,6 +717,13 @@ __device__ __forceinline__ V3 cube_eval(const SkyParams& s, V3 ds) {
     const V3 top = c00 + (c10 - c00) * fx;
     const V3 bot = c01 + (c11 - c01) * fx;
     return top + (bot - top) * fy;
+}
+// Bilinear lookup in the face the direction selects.
+__device__ __forceinline__ V3 cube_eval(const SkyParams& s, V3 ds) {
+    float sc, tc, ma;
+    const uint32_t f = cube_face(ds, &sc, &tc, &ma);
+    if (!(ma > 0.0f)) return v3(0.0f, 0.0f, 0.0f);
+    return cube_eval_face(s, f, sc / ma, tc / ma);
 }
 // Solid-angle density of the cube's piecewise-constant texel sampling at face
 // coordinates (a, b): the texel's share w / total, uniform over its S^-2 of the face's
@@ -730,11 +735,8 @@ PT_HD float cube_density(float w, float scale, float a, float b) {
 
 // Bilinear lookup times the sky colour: equirect (texel centres at (i+0.5)/W, wrap in
 // u, clamp in v) or cube map.
-__device__ __forceinline__ V3 sky_eval(const SkyParams& s, V3 d) {
-    if (s.width == 0) return v3(0.0f, 0.0f, 0.0f);
-    if (s.layout == kSkyCube) return cube_eval(s, sky_to_local(s, d)) * v3(s.color[0], s.color[1], s.color[2]);
-    float u, v, th;
-    sky_uv(sky_to_local(s, d), &u, &v, &th);
+// Bilinear lookup of the equirect at (u, v), times the sky colour.
+__device__ __forceinline__ V3 equirect_eval_uv(const SkyParams& s, float u, float v) {
     const int W = (int)s.width, H = (int)s.height;
     float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
     float fx0 = floorf(x), fy0 = floorf(y);
@@ -751,6 +753,13 @@ __device__ __forceinline__ V3 sky_eval(const SkyParams& s, V3 d) {
     V3 bot = c01 + (c11 - c01) * fx;
     V3 c = top + (bot - top) * fy;
     return c * v3(s.color[0], s.color[1], s.color[2]);
+}
+__device__ __forceinline__ V3 sky_eval(const SkyParams& s, V3 d) {
+    if (s.width == 0) return v3(0.0f, 0.0f, 0.0f);
+    if (s.layout == kSkyCube) return cube_eval(s, sky_to_local(s, d)) * v3(s.color[0], s.color[1], s.color[2]);
+    float u, v, th;
+    sky_uv(sky_to_local(s, d), &u, &v, &th);
+    return equirect_eval_uv(s, u, v);
 }
 __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
     if (!(s.total > 0.0f)) return 0.0f;
@@ -777,8 +786,9 @@ __device__ __forceinline__ float sky_pdf(const SkyParams& s, V3 d) {
 }
 // Piecewise-constant importance sampling of the texel weights by alias tables: u0 picks a
 // row entry and its fraction decides between the entry and its alias, the rescaled
-// fraction placing the point inside the row; u1 the same within the row's texels.
-__device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u1, V3* dir, float* pdf) {
+// fraction placing the point inside the row; u1 the same within the row's texels.  The
+// sky's radiance there (le) is looked up at the sampled texture coordinates.
+__device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u1, V3* dir, float* pdf, V3* le) {
     *pdf = 0.0f;
     if (!(s.total > 0.0f)) return false;
     const int W = (int)s.width, H = (int)s.height;
@@ -823,6 +833,7 @@ __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u
         const float b = (((float)r + dv) / (float)W) * 2.0f - 1.0f;
         ds = normalize(cube_point(f, a, b));
         *pdf = cube_density(w, s.pdf_scale, a, b);
+        *le = cube_eval_face(s, f, a, b) * v3(s.color[0], s.color[1], s.color[2]);
     } else {
         float u = ((float)j + du) / (float)W;
         float v = ((float)i + dv) / (float)H;
@@ -832,6 +843,7 @@ __device__ __forceinline__ bool sky_sample(const SkyParams& s, float u0, float u
         if (!(st > 0.0f)) return false;
         ds = v3(st * sp, ct, st * cp);
         *pdf = (w * s.pdf_scale) / (kTwoPiSq * st);
+        *le = equirect_eval_uv(s, u, v);
     }
     const float* R = s.rot;
     *dir = v3((R[0] * ds.x + R[3] * ds.y) + R[6] * ds.z,

@@ -230,15 +230,15 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
         // sky only through a traced segment that escapes)
         const bool extend = a.bounce < J.last_bounce && !(max3(beta) < kThroughputMin);
         if (J.sky_nee && extend) {
-            V3 wi;
+            V3 wi, ls;
             float pl;
-            if (sky_sample(J.sky, u0, u1, &wi, &pl)) {
+            if (sky_sample(J.sky, u0, u1, &wi, &pl, &ls)) {
                 V3 f = bsdf_eval(fr, m, V, wi);
                 if (any_pos(f)) {
                     V3 hl = to_local(fr, normalize(V + wi));
                     float pb = vndf_pdf_l(hl, to_local(fr, V), m.rx, m.ry);
                     float w = power_heuristic(pl, pb);
-                    V3 c = ((sky_eval(J.sky, wi) * f) * (w / pl)) * beta;
+                    V3 c = ((ls * f) * (w / pl)) * beta;
                     if (any_pos(c) && finite3(c)) {
                         out.c[NL] = c;
                         out.sky = wi;

@@ -216,6 +216,9 @@ double ora_vndf_pdf64(const double* v_local, const double* l_local, double alpha
 /* Sky: radiance lookup, sample and pdf (FP32 transcription). */
 void ora_sky_eval32(const ora_ctx* ctx, const float* dir, float* rgb);
 void ora_sky_sample32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf);
+/* As ora_sky_sample32, plus the sky's radiance at the sample (looked up at the sampled
+   texture coordinates, as the next-event estimation uses it). */
+void ora_sky_sample_le32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf, float* le);
 float ora_sky_pdf32(const ora_ctx* ctx, const float* dir);
 /* Batched forms for quadrature tests. */
 void ora_bsdf64_batch(const float* material, const double* n, const double* v, uint32_t count, const double* l,

@@ -116,6 +116,7 @@ def lib():
                                        _f32p, _f32p]
         L.ora_ltc_fetch32.argtypes = [_f32p, C.c_float, C.c_float, C.c_float, C.c_float, _f32p, _f32p]
         L.ora_sky_sample32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p]
+        L.ora_sky_sample_le32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p, _f32p]
         L.ora_sky_pdf32.restype = C.c_float
         L.ora_sky_pdf32.argtypes = [C.c_void_p, _f32p]
         L.ora_camera_ray32.argtypes = [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_float,
@@ -384,6 +385,14 @@ class OracleScene:
         p = np.zeros(1, np.float32)
         lib().ora_sky_sample32(self.h, u0, u1, ptr(d, _f32p), ptr(p, _f32p))
         return d, float(p[0])
+
+    def sky_sample_le(self, u0, u1):
+        """(direction, pdf, radiance at the sample) of the sky sampler."""
+        d = np.zeros(3, np.float32)
+        p = np.zeros(1, np.float32)
+        le = np.zeros(3, np.float32)
+        lib().ora_sky_sample_le32(self.h, u0, u1, ptr(d, _f32p), ptr(p, _f32p), ptr(le, _f32p))
+        return d, float(p[0]), le
 
     def sky_pdf(self, d):
         d = np.ascontiguousarray(d, np.float32)

@@ -1009,13 +1009,11 @@ F3 cube_tap_value(const Scene& s, int f, int i, int j) {
     F3 sum = (cube_fetch(s, f, ci, cj) + cube_fetch_outside(s, f, ci, j)) + cube_fetch_outside(s, f, i, cj);
     return f3(sum.x / 3.0f, sum.y / 3.0f, sum.z / 3.0f);
 }
-F3 cube_lookup(const Scene& s, F3 dl) {
-    float sc, tc, ma;
-    int f = cube_select(dl, &sc, &tc, &ma);
-    if (!(ma > 0.0f)) return f3(0, 0, 0);
+// Bilinear lookup of face f at face coordinates (a, b).
+F3 cube_lookup_face(const Scene& s, int f, float a, float b) {
     const float S = (float)s.sky_w;
-    float x = ((sc / ma + 1.0f) * 0.5f) * S - 0.5f;
-    float y = ((tc / ma + 1.0f) * 0.5f) * S - 0.5f;
+    float x = ((a + 1.0f) * 0.5f) * S - 0.5f;
+    float y = ((b + 1.0f) * 0.5f) * S - 0.5f;
     float x0 = std::floor(x), y0 = std::floor(y);
     float fx = x - x0, fy = y - y0;
     int j = (int)x0, i = (int)y0;
@@ -1025,6 +1023,12 @@ F3 cube_lookup(const Scene& s, F3 dl) {
     F3 bot = c01 + (c11 - c01) * fx;
     return top + (bot - top) * fy;
 }
+F3 cube_lookup(const Scene& s, F3 dl) {
+    float sc, tc, ma;
+    int f = cube_select(dl, &sc, &tc, &ma);
+    if (!(ma > 0.0f)) return f3(0, 0, 0);
+    return cube_lookup_face(s, f, sc / ma, tc / ma);
+}
 // Solid-angle density of a point (a, b) of the texel with weight w: (w / total) * S^2
 // per unit face uv area, times dA_uv / d(omega) = (1 + a^2 + b^2)^(3/2) / 4.
 float cube_pdf_at(const Scene& s, float w, float a, float b) {
@@ -1032,12 +1036,8 @@ float cube_pdf_at(const Scene& s, float w, float a, float b) {
     return (w * s.sky_pdf_scale) * (q * std::sqrt(q));
 }
 
-F3 sky_eval(const Scene& s, F3 d) {
-    if (s.sky_w == 0) return f3(0, 0, 0);
-    if (s.sky_layout == 1u)
-        return cube_lookup(s, sky_to_local(s, d)) * f3(s.sky_color[0], s.sky_color[1], s.sky_color[2]);
-    float u, v, th;
-    sky_uv(sky_to_local(s, d), &u, &v, &th);
+// Bilinear equirect lookup at (u, v), times the sky colour.
+F3 equirect_eval_uv(const Scene& s, float u, float v) {
     const int W = (int)s.sky_w, H = (int)s.sky_h;
     float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
     float fx0 = std::floor(x), fy0 = std::floor(y);
@@ -1057,6 +1057,14 @@ F3 sky_eval(const Scene& s, F3 d) {
     F3 bot = c01 + (c11 - c01) * fx;
     F3 c = top + (bot - top) * fy;
     return c * f3(s.sky_color[0], s.sky_color[1], s.sky_color[2]);
+}
+F3 sky_eval(const Scene& s, F3 d) {
+    if (s.sky_w == 0) return f3(0, 0, 0);
+    if (s.sky_layout == 1u)
+        return cube_lookup(s, sky_to_local(s, d)) * f3(s.sky_color[0], s.sky_color[1], s.sky_color[2]);
+    float u, v, th;
+    sky_uv(sky_to_local(s, d), &u, &v, &th);
+    return equirect_eval_uv(s, u, v);
 }
 float sky_pdf(const Scene& s, F3 d) {
     if (!(s.sky_total > 0.0f)) return 0.0f;
@@ -1083,8 +1091,9 @@ float sky_pdf(const Scene& s, F3 d) {
 }
 // Piecewise-constant importance sampling of the texel weights by alias tables: u0 scaled
 // by the row count picks a row entry, whose fraction chooses the entry or its alias and,
-// rescaled, places the point in the row; u1 the same over the row's texels.
-bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
+// rescaled, places the point in the row; u1 the same over the row's texels.  le: the sky's
+// radiance at the sampled texture coordinates.
+bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf, F3* le) {
     *pdf = 0.0f;
     if (!(s.sky_total > 0.0f)) return false;
     const int W = (int)s.sky_w, H = (int)s.sky_h;
@@ -1127,6 +1136,7 @@ bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
         float b = (((float)row + dv) / (float)W) * 2.0f - 1.0f;
         ds = normalize(cube_dir(f, a, b));
         *pdf = cube_pdf_at(s, w, a, b);
+        *le = cube_lookup_face(s, f, a, b) * f3(s.sky_color[0], s.sky_color[1], s.sky_color[2]);
     } else {
         float u = ((float)j + du) / (float)W;
         float v = ((float)i + dv) / (float)H;
@@ -1136,6 +1146,7 @@ bool sky_sample(const Scene& s, float u0, float u1, F3* dir, float* pdf) {
         if (!(st > 0.0f)) return false;
         ds = f3(st * sp, ct, st * cp);
         *pdf = (w * s.sky_pdf_scale) / (kTwoPiSq * st);
+        *le = equirect_eval_uv(s, u, v);
     }
     const float* R = s.sky_rot;
     *dir = f3((R[0] * ds.x + R[3] * ds.y) + R[6] * ds.z,
@@ -1528,15 +1539,15 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         // so the last vertex and terminated paths get none.
         const bool extend = b + 1 < D && !(max3(beta) < kThroughputMin);
         if (sky_nee && extend) {
-            F3 wi;
+            F3 wi, ls;
             float pl;
-            if (sky_sample(s, u0, u1, &wi, &pl)) {
+            if (sky_sample(s, u0, u1, &wi, &pl, &ls)) {
                 F3 f = bsdf_eval(fr, m, V, wi);
                 if (any_pos(f)) {
                     F3 hl = to_local(fr, normalize(V + wi));
                     float pb = vndf_pdf_l(hl, to_local(fr, V), m.rx, m.ry);
                     float w = power_heuristic(pl, pb);
-                    F3 c = ((sky_eval(s, wi) * f) * (w / pl)) * beta;
+                    F3 c = ((ls * f) * (w / pl)) * beta;
                     if (any_pos(c) && finite3(c)) {
                         out.shadow_bits[b] |= 0x80u;
                         out.n_sky_shadow++;
@@ -1911,9 +1922,16 @@ void ora_sky_eval32(const ora_ctx* ctx, const float* dir, float* rgb) {
 }
 void ora_sky_sample32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);
-    F3 d = f3(0, 0, 0);
-    sky_sample(s, u0, u1, &d, pdf);
+    F3 d = f3(0, 0, 0), le = f3(0, 0, 0);
+    sky_sample(s, u0, u1, &d, pdf, &le);
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+void ora_sky_sample_le32(const ora_ctx* ctx, float u0, float u1, float* dir, float* pdf, float* le) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    F3 d = f3(0, 0, 0), l = f3(0, 0, 0);
+    sky_sample(s, u0, u1, &d, pdf, &l);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    le[0] = l.x; le[1] = l.y; le[2] = l.z;
 }
 float ora_sky_pdf32(const ora_ctx* ctx, const float* dir) {
     const Scene& s = *reinterpret_cast<const Scene*>(ctx);

@@ -3,6 +3,7 @@
 lit BSDF: shaders/core/material/lit.hlsl:5-35; VNDF: material/utils.hlsl:98-121;
 camera ray: generate_camera_ray.hlsl:9-15; sky mapping: equitangular_to_cubemap.hlsl:28-33.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -208,3 +209,22 @@ def test_camera_ray_closed_form(oracle_lib):
     view = np.linalg.inv(cam.inv_view.astype(np.float64))
     dv = view[:3, :3] @ d
     assert dv[0] < 0 and dv[1] > 0   # left and up in view space
+
+
+@pytest.mark.parametrize("cube", [False, True])
+def test_sky_sample_radiance_is_the_sky_at_the_sample(oracle_lib, cube):
+    """The next-event estimation takes the sky's radiance at the sampled texture coordinates:
+    it equals the lookup of the sampled direction (the same bilinear filter), up to the
+    rounding of the direction round trip."""
+    sky, _ = scenes.preetham_like_sky((0.45, 0.8, 0.35), 128, 64)
+    col = np.array([1.0, 0.9, 0.8], np.float32)
+    sc = tiny_sky_scene(sky, col)
+    if cube:
+        sc = dataclasses.replace(sc, sky_rgba=scenes.equirect_to_cube(sky, 16), sky_layout=scenes.SKY_CUBE)
+    s = oracle_lib.OracleScene(sc)
+    rng = np.random.default_rng(9)
+    for u0, u1 in rng.uniform(0, 1, (400, 2)).astype(np.float32):
+        d, p, le = s.sky_sample_le(float(u0), float(u1))
+        if p <= 0:
+            continue
+        np.testing.assert_allclose(le, s.sky_eval(d), rtol=2e-3, atol=1e-6)
