@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NO_S
 
 # every entry point declared in include/bi_pt.h
 EXPORTS = ["bi_pt_create", "bi_pt_destroy", "bi_pt_last_error", "bi_pt_set_scene", "bi_pt_update_transforms",
-           "bi_pt_set_lights", "bi_pt_set_point_lights", "bi_pt_set_rect_lights", "bi_pt_set_ltc_luts", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
+           "bi_pt_set_lights", "bi_pt_set_point_lights", "bi_pt_set_rect_lights", "bi_pt_set_rect_light_textures", "bi_pt_rect_texture_mips", "bi_pt_set_ltc_luts", "bi_pt_set_sky", "bi_pt_set_sky_cube", "bi_pt_render", "bi_pt_render_gbuffer", "bi_pt_drawables_layout", "bi_pt_set_scene_drawables", "bi_pt_reset_accumulation", "bi_pt_get_stats",
            "bi_pt_debug_get_bvh", "bi_pt_debug_get_bvh4", "bi_pt_debug_num_tris", "bi_pt_debug_queue", "bi_pt_debug_get_sums",
            "bi_pt_debug_trace", "bi_pt_trace_closest", "bi_pt_trace_occlusion", "bi_pt_batch_create", "bi_pt_batch_destroy", "bi_pt_batch_last_error",
            "bi_pt_batch_set_tuning", "bi_pt_batch_get_tuning", "bi_pt_render_batch", "bi_pt_batch_get_stats",
@@ -123,6 +123,13 @@ class GpuScene:
 class TextureC(C.Structure):
     _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
                 ("filter", C.c_uint32), ("address_u", C.c_uint32), ("address_v", C.c_uint32)]
+
+
+class RectTextureC(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("num_levels", C.c_uint32),
+                ("format", C.c_uint32), ("mag_filter", C.c_uint32), ("min_filter", C.c_uint32),
+                ("mipmap_mode", C.c_uint32), ("address_u", C.c_uint32), ("address_v", C.c_uint32),
+                ("lod_bias", C.c_float), ("lod_min", C.c_float), ("lod_max", C.c_float)]
 
 
 class Instance(C.Structure):
@@ -224,6 +231,8 @@ def lib():
         L.bi_pt_set_sky.argtypes = [vp, f32p, u32, u32, f32p, f32p]
         L.bi_pt_set_point_lights.argtypes = [vp, f32p, u32]
         L.bi_pt_set_rect_lights.argtypes = [vp, f32p, u32]
+        L.bi_pt_set_rect_light_textures.argtypes = [vp, C.POINTER(RectTextureC), u32]
+        L.bi_pt_rect_texture_mips.argtypes = [u32, u32, u32, u32, vp, vp]
         L.bi_pt_set_ltc_luts.argtypes = [vp, f32p, f32p, f32p, f32p]
         L.bi_pt_set_sky_cube.argtypes = [vp, f32p, u32, f32p, f32p]
         L.bi_pt_render.argtypes = [vp, C.POINTER(CameraC), C.POINTER(Settings), vp, vp, vp]
@@ -382,6 +391,7 @@ class PathTracer:
         self.set_point_lights(getattr(scene, "point_lights", None))
         if getattr(scene, "ltc", None) is not None:
             self.set_ltc_luts(*scene.ltc)
+        self.set_rect_light_textures(getattr(scene, "rect_textures", None) or [])
         self.set_rect_lights(getattr(scene, "rect_lights", None))
         if getattr(scene, "sky_layout", 0) == 1:
             self.set_sky_cube(scene.sky_rgba, scene.sky_color, scene.sky_rot)
@@ -421,6 +431,18 @@ class PathTracer:
         """bi_pt_set_rect_lights: records [R, 28] (scenes.rect_light), or None."""
         lights = np.ascontiguousarray(lights if lights is not None else np.zeros((0, 28)), np.float32).reshape(-1, 28)
         self._check(lib().bi_pt_set_rect_lights(self.h, _p(lights) if len(lights) else None, len(lights)))
+
+    def set_rect_light_textures(self, textures):
+        """bi_pt_set_rect_light_textures: scenes.RectTexture records."""
+        arr = (RectTextureC * max(len(textures), 1))()
+        keep = []
+        for i, t in enumerate(textures):
+            a = t.chain()
+            keep.append(a)
+            h, w = t.levels[0].shape[:2]
+            arr[i] = RectTextureC(a.ctypes.data, w, h, len(t.levels), t.format, t.mag_filter, t.min_filter,
+                                  t.mipmap_mode, t.address_u, t.address_v, t.lod_bias, t.lod_min, t.lod_max)
+        self._check(lib().bi_pt_set_rect_light_textures(self.h, arr, len(textures)))
 
     def set_ltc_luts(self, m0, m1, m2, norm):
         """bi_pt_set_ltc_luts: three [4096, 4] matrix tables and the [4096, 2] norm table."""
@@ -889,3 +911,23 @@ class Nrc:
 def nrc_train_seed(seed, frame):
     """Seed of the training paths of frame `frame` of bi_pt_render_nrc with render seed `seed`."""
     return ((seed * 0x9E3779B1) & 0xFFFFFFFF) ^ ((frame * 0x85EBCA77 + 0x7A1B) & 0xFFFFFFFF)
+
+
+def rect_texture_mips(level0, num_levels, fmt):
+    """bi_pt_rect_texture_mips (host only): the engine's mip chain of level0 ([h, w, 4], uint8
+    for the RGBA8 formats, float32 for RGBA32F) as a list of num_levels [h_l, w_l, 4] arrays."""
+    a = np.ascontiguousarray(level0)
+    h, w = a.shape[:2]
+    sizes, lw, lh = [], w, h
+    for _ in range(num_levels):
+        sizes.append((lh, lw))
+        lw, lh = max(lw // 2, 1), max(lh // 2, 1)
+    out = np.zeros(sum(y * x * 4 for y, x in sizes), a.dtype)
+    st = lib().bi_pt_rect_texture_mips(fmt, w, h, num_levels, a.ctypes.data, out.ctypes.data)
+    if st != 0:
+        raise PtError(st, "bi_pt_rect_texture_mips: bad format, size or level count")
+    levels, o = [], 0
+    for y, x in sizes:
+        levels.append(out[o:o + y * x * 4].reshape(y, x, 4))
+        o += y * x * 4
+    return levels

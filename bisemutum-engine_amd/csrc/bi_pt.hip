@@ -265,6 +265,7 @@ bi_pt_status bi_pt_destroy(bi_pt_ctx* ctx) {
     (void)hipDeviceSynchronize();
     dfree(ctx, ctx->pos); dfree(ctx, ctx->nrm); dfree(ctx, ctx->idx); dfree(ctx, ctx->inst); dfree(ctx, ctx->mats);
     dfree(ctx, ctx->rects); dfree(ctx, ctx->ltc);
+    dfree(ctx, ctx->rect_tex); dfree(ctx, ctx->rect_tex_levels); dfree(ctx, ctx->rect_tex_pool);
     dfree(ctx, ctx->sky_tex); dfree(ctx, ctx->sky_w);
     dfree(ctx, ctx->sky_row_alias); dfree(ctx, ctx->sky_tex_alias);
     dfree(ctx, ctx->tri_v); dfree(ctx, ctx->tri_n); dfree(ctx, ctx->cb_out); dfree(ctx, ctx->tri_mat);
@@ -437,6 +438,29 @@ bi_pt_status bi_pt_update_transforms(bi_pt_ctx* ctx, const float* xf, uint32_t n
     return BI_PT_OK;
 }
 
+namespace {
+// sRGB byte -> linear (IEC 61966-2-1), evaluated in double and rounded once; shared by the
+// material and rect-light textures.
+double srgb_decode(int c) {
+    const double x = c / 255.0;
+    return x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4);
+}
+bi_pt_status upload_srgb_table(bi_pt_ctx* ctx) {
+    if (ctx->srgb.p) return BI_PT_OK;
+    float lut[256];
+    for (int c = 0; c < 256; ++c) lut[c] = (float)srgb_decode(c);
+    BI_TRY(dalloc(ctx, ctx->srgb, 256));
+    BI_HIP_TRY(ctx, hipMemcpy(ctx->srgb.p, lut, sizeof lut, hipMemcpyHostToDevice));
+    return BI_PT_OK;
+}
+uint32_t texel_words(uint32_t format) { return format == BI_PT_TEXEL_RGBA32F ? 4u : 1u; }
+uint32_t max_mip_levels(uint32_t w, uint32_t h) {
+    uint32_t n = 1, m = std::max(w, h);
+    while (m > 1) { m >>= 1; ++n; }
+    return n;
+}
+}  // namespace
+
 bi_pt_status bi_pt_set_textures(bi_pt_ctx* ctx, const bi_pt_texture* textures, uint32_t num_textures) {
     if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
     if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
@@ -467,19 +491,12 @@ bi_pt_status bi_pt_set_textures(bi_pt_ctx* ctx, const bi_pt_texture* textures, u
         }
         std::memcpy(&pool[recs[k].offset], T.texels, 4 * n);
     }
-    // sRGB byte -> linear (IEC 61966-2-1), evaluated in double and rounded once
-    float lut[256];
-    for (int c = 0; c < 256; ++c) {
-        const double x = c / 255.0;
-        lut[c] = (float)(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
-    }
     BI_TRY(dalloc(ctx, ctx->tex_recs, std::max<size_t>(recs.size(), 1)));
     BI_TRY(dalloc(ctx, ctx->tex_pool, std::max<size_t>(pool.size(), 4)));
-    BI_TRY(dalloc(ctx, ctx->srgb, 256));
+    BI_TRY(upload_srgb_table(ctx));
     if (!recs.empty())
         BI_HIP_TRY(ctx, hipMemcpy(ctx->tex_recs.p, recs.data(), recs.size() * sizeof(TexRec), hipMemcpyHostToDevice));
     if (!pool.empty()) BI_HIP_TRY(ctx, hipMemcpy(ctx->tex_pool.p, pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
-    BI_HIP_TRY(ctx, hipMemcpy(ctx->srgb.p, lut, sizeof lut, hipMemcpyHostToDevice));
     ctx->num_tex = num_textures;
     ctx->acc_count = 0;
     ctx->scene_version++;
@@ -568,8 +585,11 @@ bi_pt_status bi_pt_set_rect_lights(bi_pt_ctx* ctx, const bi_pt_rect_light* light
     for (uint32_t l = 0; l < num_lights; ++l) {
         std::memcpy(&recs[l], &lights[l], sizeof(RectRec));
         const RectRec& r = recs[l];
-        if (r.texture_index >= 0)
-            return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "textured rect lights are not supported (texture_index >= 0)");
+        if (r.texture_index >= (int32_t)kMaxRectTextures)
+            return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "rect light texture_index beyond 16 textures");
+        if (r.texture_index >= 0 && !(std::isfinite(r.inv_texel_size) && r.inv_texel_size >= 0.0f &&
+                                      std::isfinite(r.inv_width_sqr) && std::isfinite(r.inv_height_sqr)))
+            return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "textured rect light with non-finite texel or size terms");
         const float* f[] = {r.emission, r.p0, r.p1, r.p2, r.p3};
         for (const float* v : f)
             for (int k = 0; k < 3; ++k)
@@ -586,6 +606,144 @@ bi_pt_status bi_pt_set_rect_lights(bi_pt_ctx* ctx, const bi_pt_rect_light* light
     ctx->rects_host.swap(recs);
     ctx->acc_count = 0;
     ctx->scene_version++;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_set_rect_light_textures(bi_pt_ctx* ctx, const bi_pt_rect_texture* textures, uint32_t num_textures) {
+    if (!ctx) return BI_PT_ERROR_INVALID_ARGUMENT;
+    if (ctx->scene_src) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "a view renders another context's scene");
+    if (num_textures > kMaxRectTextures) return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "at most 16 rect-light textures");
+    if (num_textures && !textures) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "null rect-light textures");
+    std::vector<RectTexRec> recs(num_textures);
+    std::vector<TexRec> levels;
+    size_t words = 0;
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const bi_pt_rect_texture& T = textures[k];
+        const bool ok = T.texels && T.width > 0 && T.height > 0 && T.width <= 16384 && T.height <= 16384 &&
+                        T.num_levels >= 1 && T.num_levels <= max_mip_levels(T.width, T.height) &&
+                        T.format <= BI_PT_TEXEL_RGBA32F && T.mag_filter <= 1 && T.min_filter <= 1 &&
+                        T.mipmap_mode <= 1 && T.address_u <= BI_PT_ADDRESS_CLAMP_TO_EDGE &&
+                        T.address_v <= BI_PT_ADDRESS_CLAMP_TO_EDGE && std::isfinite(T.lod_bias) &&
+                        std::isfinite(T.lod_min) && std::isfinite(T.lod_max) && T.lod_min <= T.lod_max;
+        if (!ok) return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT, "rect-light texture " + std::to_string(k) + ": bad description");
+        recs[k] = RectTexRec{(uint32_t)levels.size(), T.num_levels, T.mag_filter, T.min_filter, T.mipmap_mode,
+                             T.lod_bias, T.lod_min, T.lod_max};
+        uint32_t w = T.width, h = T.height;
+        for (uint32_t l = 0; l < T.num_levels; ++l) {
+            words = (words + 3u) & ~(size_t)3u;   // RGBA32F texels are read as 16-B words
+            levels.push_back(TexRec{(uint32_t)words, w, h, T.format, 0u, T.address_u, T.address_v, 0u});
+            words += (size_t)w * h * texel_words(T.format);
+            if (words >= (1ull << 32)) return fail(ctx, BI_PT_ERROR_UNSUPPORTED, "rect-light textures larger than 16 GiB");
+            w = std::max(w / 2u, 1u);
+            h = std::max(h / 2u, 1u);
+        }
+    }
+    std::vector<uint32_t> pool(words, 0u);
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const bi_pt_rect_texture& T = textures[k];
+        const uint32_t* src = static_cast<const uint32_t*>(T.texels);
+        for (uint32_t l = 0; l < T.num_levels; ++l) {
+            const TexRec& L = levels[recs[k].first_level + l];
+            const size_t n = (size_t)L.width * L.height * texel_words(T.format);
+            if (T.format == BI_PT_TEXEL_RGBA32F)
+                for (size_t i = 0; i < n; ++i) {
+                    float f;
+                    std::memcpy(&f, &src[i], 4);
+                    if (!std::isfinite(f))
+                        return fail(ctx, BI_PT_ERROR_INVALID_ARGUMENT,
+                                    "rect-light texture " + std::to_string(k) + ": non-finite texel");
+                }
+            std::memcpy(&pool[L.offset], src, 4 * n);
+            src += n;
+        }
+    }
+    BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    BI_TRY(dalloc(ctx, ctx->rect_tex, std::max<size_t>(recs.size(), 1)));
+    BI_TRY(dalloc(ctx, ctx->rect_tex_levels, std::max<size_t>(levels.size(), 1)));
+    BI_TRY(dalloc(ctx, ctx->rect_tex_pool, std::max<size_t>(pool.size(), 4)));
+    BI_TRY(upload_srgb_table(ctx));
+    if (!recs.empty()) {
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->rect_tex.p, recs.data(), recs.size() * sizeof(RectTexRec), hipMemcpyHostToDevice));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->rect_tex_levels.p, levels.data(), levels.size() * sizeof(TexRec),
+                                  hipMemcpyHostToDevice));
+        BI_HIP_TRY(ctx, hipMemcpy(ctx->rect_tex_pool.p, pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
+    }
+    ctx->num_rect_tex = num_textures;
+    ctx->acc_count = 0;
+    ctx->scene_version++;
+    return BI_PT_OK;
+}
+
+bi_pt_status bi_pt_rect_texture_mips(uint32_t format, uint32_t width, uint32_t height, uint32_t num_levels,
+                                     const void* level0, void* chain) {
+    if (!level0 || !chain || format > BI_PT_TEXEL_RGBA32F || width == 0 || height == 0 || width > 16384 ||
+        height > 16384 || num_levels == 0 || num_levels > max_mip_levels(width, height))
+        return BI_PT_ERROR_INVALID_ARGUMENT;
+    // texels as floats (RGBA8 decoded: c / 255, sRGB colour through the curve), averaged in
+    // FP32 in the shader's order, stored back (RGBA8: round to nearest; sRGB colour encoded)
+    auto decode = [format](const uint32_t* p, size_t i, float v[4]) {
+        if (format == BI_PT_TEXEL_RGBA32F) { std::memcpy(v, &p[4 * i], 16); return; }
+        const uint32_t w = p[i];
+        for (int c = 0; c < 4; ++c) {
+            const int b = (int)((w >> (8 * c)) & 255u);
+            v[c] = (format == BI_PT_TEXEL_RGBA8_SRGB && c < 3) ? (float)srgb_decode(b) : (float)b / 255.0f;
+        }
+    };
+    auto encode = [format](float v[4], uint32_t* p, size_t i) {
+        if (format == BI_PT_TEXEL_RGBA32F) { std::memcpy(&p[4 * i], v, 16); return; }
+        uint32_t w = 0;
+        for (int c = 0; c < 4; ++c) {
+            double x = std::min(std::max((double)v[c], 0.0), 1.0);
+            if (format == BI_PT_TEXEL_RGBA8_SRGB && c < 3)
+                x = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
+            w |= (uint32_t)std::lrint(x * 255.0) << (8 * c);
+        }
+        p[i] = w;
+    };
+    const size_t tw = texel_words(format);
+    const uint32_t* src = static_cast<const uint32_t*>(level0);
+    uint32_t* dst = static_cast<uint32_t*>(chain);
+    std::memmove(dst, src, (size_t)width * height * tw * 4);
+    uint32_t w = width, h = height;
+    for (uint32_t l = 1; l < num_levels; ++l) {
+        const uint32_t* s = dst;
+        uint32_t* d = dst + (size_t)w * h * tw;
+        const uint32_t dw = std::max(w / 2u, 1u), dh = std::max(h / 2u, 1u);
+        const bool odd_x = dw & 1u, odd_y = dh & 1u;   // of the destination, as mipmap.hlsl tests it
+        for (uint32_t y = 0; y < dh; ++y)
+            for (uint32_t x = 0; x < dw; ++x) {
+                // reads beyond the source level give zero (robust image access)
+                auto at = [&](uint32_t i, uint32_t j, float v[4]) {
+                    if (i >= w || j >= h) { v[0] = v[1] = v[2] = v[3] = 0.0f; return; }
+                    decode(s, (size_t)j * w + i, v);
+                };
+                float a[4], b[4], c[4], e[4], r[4];
+                const uint32_t sx = 2u * x, sy = 2u * y;
+                at(sx, sy, a); at(sx, sy + 1u, b); at(sx + 1u, sy, c); at(sx + 1u, sy + 1u, e);
+                for (int k = 0; k < 4; ++k) r[k] = (a[k] + b[k]) + (c[k] + e[k]);
+                float n = 4.0f;
+                if (odd_x) {
+                    at(sx + 2u, sy, a); at(sx + 2u, sy + 1u, b);
+                    for (int k = 0; k < 4; ++k) r[k] = r[k] + (a[k] + b[k]);
+                    n = n + 2.0f;
+                }
+                if (odd_y) {
+                    at(sx, sy + 2u, a); at(sx + 1u, sy + 2u, b);
+                    for (int k = 0; k < 4; ++k) r[k] = r[k] + (a[k] + b[k]);
+                    n = n + 2.0f;
+                }
+                if (odd_x && odd_y) {
+                    at(sx + 2u, sy + 2u, a);
+                    for (int k = 0; k < 4; ++k) r[k] = r[k] + a[k];
+                    n = n + 1.0f;
+                }
+                for (int k = 0; k < 4; ++k) r[k] = r[k] / n;
+                encode(r, d, (size_t)y * dw + x);
+            }
+        dst = d;
+        w = dw;
+        h = dh;
+    }
     return BI_PT_OK;
 }
 

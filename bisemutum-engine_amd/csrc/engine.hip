@@ -268,6 +268,11 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         p.nrect = (uint32_t)S->rects_host.size();
         if (p.nrect && S->ltc_host.empty())
             return fail(own, BI_PT_ERROR_INVALID_ARGUMENT, "rect lights need the LTC tables (bi_pt_set_ltc_luts)");
+        for (const RectRec& rr : S->rects_host)
+            if (rr.texture_index >= (int32_t)S->num_rect_tex)
+                return fail(own, BI_PT_ERROR_INVALID_ARGUMENT,
+                            "rect light texture_index " + std::to_string(rr.texture_index) + " without that texture "
+                            "(bi_pt_set_rect_light_textures)");
         p.spp = st->spp;
         p.sky_nee = !(st->flags & BI_PT_FLAG_NO_SKY_NEE) && S->sky.total > 0.0f;
         p.sh = 1;
@@ -423,6 +428,8 @@ bi_pt_status render_jobs(Owner* own, Engine& eng, const JobIn* jobs, uint32_t n,
         r.rects = p.nrect ? S->rects.p : nullptr;
         r.num_rects = p.nrect;
         r.ltc = p.nrect ? S->ltc.p : nullptr;
+        r.rect_tex = p.nrect && S->num_rect_tex ? S->rect_tex.p : nullptr;
+        r.rect_tex_levels = SceneTex{S->rect_tex_levels.p, S->rect_tex_pool.p, S->srgb.p};
         r.cam = p.cam;
         r.npix = p.npix;
         r.sample0 = (uint32_t)(p.reset ? 0 : ctx->acc_count) + sample_off;

@@ -225,10 +225,9 @@ PT_HD V4 tex_texel(const SceneTex& T, const TexRec& r, int i, int j) {
 }
 // x in texel units; coordinates that are not finite or beyond 2^24 texels read texel 0
 PT_HD float tex_coord(float x) { return (x > -16777216.0f && x < 16777216.0f) ? x : 0.0f; }
-PT_HD V4 tex_sample(const SceneTex& T, uint32_t id, float u, float v) {
-    const TexRec r = T.recs[id];
+PT_HD V4 tex_sample_rec(const SceneTex& T, const TexRec& r, uint32_t filter, float u, float v) {
     const int W = (int)r.width, H = (int)r.height;
-    if (r.filter == 0u) {
+    if (filter == 0u) {
         const int i = tex_wrap((int)__builtin_floorf(tex_coord(u * (float)W)), W, r.address_u);
         const int j = tex_wrap((int)__builtin_floorf(tex_coord(v * (float)H)), H, r.address_v);
         return tex_texel(T, r, i, j);
@@ -244,6 +243,10 @@ PT_HD V4 tex_sample(const SceneTex& T, uint32_t id, float u, float v) {
     const V4 top{lerp(t00.x, t10.x, ax), lerp(t00.y, t10.y, ax), lerp(t00.z, t10.z, ax), lerp(t00.w, t10.w, ax)};
     const V4 bot{lerp(t01.x, t11.x, ax), lerp(t01.y, t11.y, ax), lerp(t01.z, t11.z, ax), lerp(t01.w, t11.w, ax)};
     return V4{lerp(top.x, bot.x, ay), lerp(top.y, bot.y, ay), lerp(top.z, bot.z, ay), lerp(top.w, bot.w, ay)};
+}
+PT_HD V4 tex_sample(const SceneTex& T, uint32_t id, float u, float v) {
+    const TexRec r = T.recs[id];
+    return tex_sample_rec(T, r, r.filter, u, v);
 }
 // Texture coordinates of a leaf at barycentrics (u, v): uv0 + (uv1 - uv0) u + (uv2 - uv0) v.
 PT_HD void leaf_texcoord(const LeafUv& L, float bu, float bv, float* s, float* t) {
@@ -531,23 +534,25 @@ PT_HD void ltc_matrix_brdf(const float4* ltc, V3 lv, float rx, float ry, V3 q[4]
         m->r1 = t;
     }
 }
-// Edge term of the polygon integral (ltc_integrate_edge, lights.hlsl:347-361): the z of
-// cross(v1, v2) * theta / sin(theta), theta / sin(theta) by its rational fit.
-PT_HD float ltc_edge(V3 v1, V3 v2) {
+// theta / sin(theta) of the edge between unit vectors v1 and v2 by the rational fit of
+// ltc_integrate_edge (lights.hlsl:365-381); the edge's term is cross(v1, v2) times it.
+PT_HD float ltc_edge_scale(V3 v1, V3 v2) {
     const float x = dot(v1, v2);
     const float y = x < 0.0f ? -x : x;
     const float a = 5.42031f + (3.12829f + 0.0902326f * y) * y;
     const float b = 3.45068f + (4.18814f + y) * y;
     float t = a / b;
     if (x < 0.0f) t = kPi / __builtin_sqrtf(1.0f - x * x) - t;
-    return (v1.x * v2.y - v1.y * v2.x) * t;
+    return t;
 }
-// ltc_integrate (lights.hlsl:362-400) without the light texture's direction: the quad q
-// relative to P in the frame (t, b, n), transformed by minv (none when null), clipped to
-// z > 0 (a vertex pair across the plane contributes the point in.z * out - out.z * in),
-// projected onto the sphere and integrated edge by edge; one-sided lights count only
-// positive integrals.
-PT_HD float ltc_integrate(V3 P, const Frame& fr, const M3* minv, const V3 q[4], bool two_sided) {
+// ltc_integrate (lights.hlsl:382-426): the quad q relative to P in the frame (t, b, n),
+// transformed by minv (none when null), clipped to z > 0 (a vertex pair across the plane
+// contributes the point in.z * out - out.z * in), projected onto the sphere and integrated
+// edge by edge; one-sided lights count only positive integrals.  dir (nullable) receives
+// the sum of the edges' cross(v1, v2) theta / sin(theta) in the transformed space (the
+// vector whose image is the light texture's direction); it is left alone when the quad
+// lies below the plane.
+PT_HD float ltc_integrate(V3 P, const Frame& fr, const M3* minv, const V3 q[4], bool two_sided, V3* dir = nullptr) {
     V3 l[4];
     uint32_t inside = 0;
     for (int k = 0; k < 4; ++k) {
@@ -570,31 +575,132 @@ PT_HD float ltc_integrate(V3 P, const Frame& fr, const M3* minv, const V3 q[4], 
         }
     }
     for (int k = 0; k < n; ++k) c[k] = normalize(c[k]);
-    float sum = 0.0f;
-    for (int k = 0; k < n; ++k) sum = sum + ltc_edge(c[k], c[k + 1 < n ? k + 1 : 0]);
+    float sum = 0.0f, sx = 0.0f, sy = 0.0f;
+    for (int k = 0; k < n; ++k) {
+        const V3 a = c[k], b = c[k + 1 < n ? k + 1 : 0];
+        const float t = ltc_edge_scale(a, b);
+        sum = sum + (a.x * b.y - a.y * b.x) * t;
+        if (dir) {
+            sx = sx + (a.y * b.z - a.z * b.y) * t;
+            sy = sy + (a.z * b.x - a.x * b.z) * t;
+        }
+    }
+    if (dir) *dir = v3(sx, sy, sum);
     float r = two_sided ? (sum < 0.0f ? -sum : sum) : sel_max(sum, 0.0f);
     if (!(r - r == 0.0f)) r = 0.0f;   // NaN or infinite
     return r;
+}
+// log2 of x >= 1 (Moshier's Cephes log2f: x = m 2^e with m in [sqrt(1/2), sqrt(2)), a
+// degree-8 minimax polynomial for log(1 + f), scaled by log2(e) in two parts); exact at
+// powers of two; +inf gives 128.
+PT_HD float pt_log2(float x) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, x);
+    float e = (float)((int)((b >> 23) & 255u) - 126);
+    float m = __builtin_bit_cast(float, (b & 0x007fffffu) | 0x3f000000u);
+    if (m < 0.70710678118654752440f) {
+        e = e - 1.0f;
+        m = (m + m) - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    const float z = m * m;
+    float y = 7.0376836292e-2f;
+    y = y * m - 1.1514610310e-1f;
+    y = y * m + 1.1676998740e-1f;
+    y = y * m - 1.2420140846e-1f;
+    y = y * m + 1.4249322787e-1f;
+    y = y * m - 1.6668057665e-1f;
+    y = y * m + 2.0000714765e-1f;
+    y = y * m - 2.4999993993e-1f;
+    y = y * m + 3.3333331174e-1f;
+    y = (y * m) * z;
+    y = y - 0.5f * z;
+    const float log2ea = 0.44269504088896340736f;
+    float r = y * log2ea;
+    r = r + m * log2ea;
+    r = r + y;
+    r = r + m;
+    return r + e;
+}
+PT_HD float saturate_nan0(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
+// SampleLevel of a rect-light texture (Vulkan's explicit-LOD rules): lambda = level + the
+// sampler's bias, clamped to [lod_min, lod_max]; the minification filter for lambda > 0,
+// else the magnification one; the level lambda clamped to [0, levels - 1], nearest (halves
+// to the lower level) or the two levels around it blended by its fraction.
+PT_HD V4 rect_tex_sample(const RectTexRec& r, const SceneTex& T, float u, float v, float level) {
+    float lam = level + r.lod_bias;
+    lam = lam > r.lod_max ? r.lod_max : lam;
+    lam = lam < r.lod_min ? r.lod_min : lam;
+    const uint32_t filter = lam > 0.0f ? r.min_filter : r.mag_filter;
+    const float q = (float)(r.num_levels - 1u);
+    float d = lam > 0.0f ? lam : 0.0f;
+    d = d < q ? d : q;
+    if (r.mipmap_mode == 0u) {
+        const uint32_t l = (uint32_t)(__builtin_ceilf(d + 0.5f) - 1.0f);
+        return tex_sample_rec(T, T.recs[r.first_level + l], filter, u, v);
+    }
+    const float fl = __builtin_floorf(d);
+    const uint32_t l = (uint32_t)fl;
+    const float f = d - fl;
+    const V4 a = tex_sample_rec(T, T.recs[r.first_level + l], filter, u, v);
+    if (f == 0.0f) return a;
+    const V4 b = tex_sample_rec(T, T.recs[r.first_level + l + 1u], filter, u, v);
+    return V4{lerpf(a.x, b.x, f), lerpf(a.y, b.y, f), lerpf(a.z, b.z, f), lerpf(a.w, b.w, f)};
+}
+// rect_light_sample_texture (lights.hlsl:429-445): where the ray from P along direction
+// dir meets the light's plane, as (u, v) across its width (position2 -> position3) and
+// height (position2 -> position1), clamped to the light; the level log2 of the texels the
+// lobe spans there (distance * roughness * texels per unit).  Directions within 1e-4 of
+// the plane see no texture (0).
+PT_HD V3 rect_tex_lookup(const RectRec& R, const RectTexRec& rt, const SceneTex& T, V3 dir, V3 P, float roughness) {
+    const V3 n = v3(R.normal[0], R.normal[1], R.normal[2]);
+    const V3 p2 = v3(R.p2[0], R.p2[1], R.p2[2]);
+    const float st = dot(dir, n), step = st < 0.0f ? -st : st;
+    if (step < 0.0001f) return v3(0.0f, 0.0f, 0.0f);
+    const float dd = dot(P - p2, n), dist = dd < 0.0f ? -dd : dd;
+    const float t = dist / step;
+    const V3 rp = (P + dir * t) - p2;
+    const float u = saturate_nan0(dot(rp, v3(R.p3[0], R.p3[1], R.p3[2]) - p2) * R.inv_width_sqr);
+    const float v = saturate_nan0(dot(rp, v3(R.p1[0], R.p1[1], R.p1[2]) - p2) * R.inv_height_sqr);
+    const float texels = (t * roughness) * R.inv_texel_size;
+    const float level = pt_log2(texels > 1.0f ? texels : 1.0f);
+    const V4 c = rect_tex_sample(rt, T, u, v, level);
+    return v3(c.x, c.y, c.z);
+}
+// The world direction of a polygon integral's vector s (transformed space, m its forward
+// transform, none when null): normalize((m s) in the frame (t, b, n)).
+PT_HD V3 ltc_direction(const Frame& fr, const M3* m, V3 s) {
+    const V3 w = m ? m3_mul(*m, s) : s;
+    return normalize((fr.t * w.x + fr.b * w.y) + fr.n * w.z);
 }
 // A rect light's contribution at a lit surface (rect_light_eval_ltc, lights.hlsl:
 // 471-513, then surface_eval_lut_lit, material/lit.hlsl:37-58): the diffuse integral
 // times (1 - F) base / pi plus the specular one times f0 * norm + f90 * Fresnel term.
 // (Called, not inlined: its table reads and two polygon integrals would otherwise share
 // the shading kernel's registers.)
-inline __device__ __noinline__ V3 rect_light_lit(const RectRec& R, const float4* ltc, V3 P, const Frame& fr, V3 V,
-                                                 const Mat& m) {
+inline __device__ __noinline__ V3 rect_light_lit(const RectRec& R, const float4* ltc, const RectTexRec* rtex,
+                                                 const SceneTex& rtl, V3 P, const Frame& fr, V3 V, const Mat& m) {
     const V3 lv = v3(dot(V, fr.t), dot(V, fr.b), dot(V, fr.n));
     if (lv.z <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
     V3 q[4] = {v3(R.p3[0], R.p3[1], R.p3[2]), v3(R.p2[0], R.p2[1], R.p2[2]), v3(R.p1[0], R.p1[1], R.p1[2]),
                v3(R.p0[0], R.p0[1], R.p0[2])};
     const bool two = R.two_sided != 0u;
     const V3 e = v3(R.emission[0], R.emission[1], R.emission[2]);
-    const float i_d = ltc_integrate(P, fr, nullptr, q, two);
+    const bool textured = R.texture_index >= 0;
+    V3 sd, ss;
+    const float i_d = ltc_integrate(P, fr, nullptr, q, two, textured ? &sd : nullptr);
     M3 mt;
     float bx, by;
     ltc_matrix_brdf(ltc, lv, m.rx, m.ry, q, &mt, &bx, &by);
     const M3 mi = m3_inverse(mt);
-    const float i_s = ltc_integrate(P, fr, &mi, q, two);
+    const float i_s = ltc_integrate(P, fr, &mi, q, two, textured ? &ss : nullptr);
+    V3 ld = e * i_d, ls = e * i_s;
+    if (textured) {
+        // the light's texture seen along each integral's direction (lights.hlsl:495-511)
+        const RectTexRec& rt = rtex[R.texture_index];
+        if (i_d != 0.0f) ld = ld * rect_tex_lookup(R, rt, rtl, ltc_direction(fr, nullptr, sd), P, 1.0f);
+        if (i_s != 0.0f) ls = ls * rect_tex_lookup(R, rt, rtl, ltc_direction(fr, &mt, ss), P, __builtin_sqrtf(m.rx * m.ry));
+    }
     const float ndotv = dot(fr.n, V);
     if (ndotv <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
     const float x = 1.0f - ndotv;
@@ -603,7 +709,7 @@ inline __device__ __noinline__ V3 rect_light_lit(const RectRec& R, const float4*
     const V3 one = v3(1.0f, 1.0f, 1.0f);
     const V3 diffuse = ((one - fres) * m.base) * kInvPi;
     const V3 specular = m.f0 * bx + m.f90 * by;
-    return diffuse * (e * i_d) + specular * (e * i_s);
+    return diffuse * ld + specular * ls;
 }
 
 // --------------------------------------------------------------------- sky

@@ -177,6 +177,11 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<bipt::RectRec> rects;
     bipt::DevBuf<float4> ltc;
     std::vector<float> ltc_host;
+    // the rect lights' textures: one record each, their mip levels' records and texels
+    bipt::DevBuf<bipt::RectTexRec> rect_tex;
+    bipt::DevBuf<bipt::TexRec> rect_tex_levels;
+    bipt::DevBuf<uint32_t> rect_tex_pool;
+    uint32_t num_rect_tex = 0;
     // sky
     bipt::DevBuf<float4> sky_tex;
     bipt::DevBuf<float> sky_w;

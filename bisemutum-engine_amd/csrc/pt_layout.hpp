@@ -146,7 +146,7 @@ static_assert(sizeof(LightRec) == 64, "light record is 64 bytes");
 constexpr uint32_t kMaxRectLights = 16;
 struct RectRec {
     float emission[3];
-    int32_t texture_index;  // < 0 (textured rect lights are not supported)
+    int32_t texture_index;  // < 0: untextured; else the rect-light texture it samples
     float center[3];
     uint32_t two_sided;
     float p0[3];
@@ -161,6 +161,16 @@ struct RectRec {
     float pad2;
 };
 static_assert(sizeof(RectRec) == 112, "rect light record is 112 bytes");
+// A rect light's texture (LightsContext::rect_light_textures, lights.cpp:229-238, with its
+// sampler): num_levels mip levels, level l a TexRec at first_level + l of the rect-texture
+// level table; filters 0 nearest / 1 linear; the sampler's LOD bias and clamp.
+constexpr uint32_t kMaxRectTextures = 16;
+struct RectTexRec {
+    uint32_t first_level, num_levels;
+    uint32_t mag_filter, min_filter, mipmap_mode;
+    float lod_bias, lod_min, lod_max;
+};
+static_assert(sizeof(RectTexRec) == 32, "rect texture record is 32 bytes");
 // The LTC tables (lights.cpp:26-37): 8 x 8 x 64 texels each, x fastest; three matrix rows
 // (xyz of RGBA32F) and the norm / Fresnel pair (xy of RG32F), held as four float4 tables.
 constexpr uint32_t kLtcSize = 8, kLtcDepth = 64, kLtcTexels = kLtcSize * kLtcSize * kLtcDepth;
@@ -288,6 +298,8 @@ struct JobRec {
     const RectRec* rects;   // num_rects rect lights (their slot follows the sky's)
     uint32_t num_rects;
     const float4* ltc;      // 4 * kLtcTexels: matrix rows 0, 1, 2, then the norm table
+    const RectTexRec* rect_tex;   // the rect lights' textures (null: none)
+    SceneTex rect_tex_levels;     // their mip levels: level records, texel pool, sRGB table
 };
 
 // Ray queue in SoA form.

@@ -220,7 +220,7 @@ __device__ __forceinline__ void shade_entry(const ShadeArgs& a, const JobRec& J,
                 float* acc = a.ps.acc + (size_t)(nl + 2u) * 3u * cap;
                 V3 sum = v3(acc[path], acc[cap + path], acc[2 * cap + path]);
                 for (uint32_t r = 0; r < J.num_rects; ++r)
-                    sum = sum + rect_light_lit(J.rects[r], J.ltc, P, fr, V, m) * beta;
+                    sum = sum + rect_light_lit(J.rects[r], J.ltc, J.rect_tex, J.rect_tex_levels, P, fr, V, m) * beta;
                 acc[path] = sum.x;
                 acc[cap + path] = sum.y;
                 acc[2 * cap + path] = sum.z;

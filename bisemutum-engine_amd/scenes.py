@@ -68,6 +68,7 @@ class Scene:
     point_lights: np.ndarray = None   # f32 [P, 16] (bi_pt_point_light records), after the directional lights
     rect_lights: np.ndarray = None    # f32 [R, 28] (bi_pt_rect_light records)
     ltc: tuple = None                 # LTC tables (matrix rows 0, 1, 2: [4096, 4]; norm: [4096, 2])
+    rect_textures: list = dataclasses.field(default_factory=list)   # RectTexture records
 
     @property
     def num_tris(self) -> int:
@@ -96,6 +97,7 @@ M_STEP_AT, M_STEP_BELOW, M_STEP_ABOVE = 20, 21, 22
 M_BASE_TEX, M_F0_TEX, M_NORMAL_TEX, M_ROUGH_TEX, M_ROUGH_CHANNEL = 23, 24, 25, 26, 27
 BLEND_OPAQUE, BLEND_ALPHA_TEST, BLEND_TRANSLUCENT = 0, 1, 2
 TEXEL_RGBA8_UNORM, TEXEL_RGBA8_SRGB, TEXEL_RGBA32F = 0, 1, 2
+ADDRESS_REPEAT, ADDRESS_MIRRORED_REPEAT, ADDRESS_CLAMP = 0, 1, 2
 
 
 def material(base_color, roughness, metallic, two_sided=True, anisotropy=0.0):
@@ -164,6 +166,36 @@ def rect_light(emission, center, width, height, x_axis=(1, 0, 0), y_axis=(0, 1, 
     r[11] = 1.0 / (width * width)
     r[15] = 1.0 / (height * height)
     r[24:27] = np.cross(x, y)
+    return r
+
+
+@dataclasses.dataclass
+class RectTexture:
+    """A rect light's texture (bi_pt_rect_texture): levels, a list of [h_l, w_l, 4] arrays
+    (uint8 for the RGBA8 formats, float32 for RGBA32F), level l of max(1, w >> l) x
+    max(1, h >> l) texels, and the sampler."""
+    levels: list
+    format: int = TEXEL_RGBA32F
+    mag_filter: int = 1
+    min_filter: int = 1
+    mipmap_mode: int = 1
+    address_u: int = ADDRESS_CLAMP
+    address_v: int = ADDRESS_CLAMP
+    lod_bias: float = 0.0
+    lod_min: float = 0.0
+    lod_max: float = 1000.0
+
+    def chain(self):
+        """The levels one after the other, as 32-bit words."""
+        return np.concatenate([np.ascontiguousarray(l).view(np.uint32).reshape(-1) for l in self.levels])
+
+
+def textured_rect_light(record, texture_index, tex_width, tex_height, width, height):
+    """A rect light record given a texture: texture_index and inv_texel_size =
+    max(tex_width / width, tex_height / height) (lights.cpp:229-238)."""
+    r = np.array(record, np.float32, copy=True)
+    r[3:4].view(np.int32)[0] = int(texture_index)
+    r[19] = max(tex_width / width, tex_height / height)
     return r
 
 

@@ -357,7 +357,7 @@ bi_pt_status bi_pt_set_point_lights(bi_pt_ctx* ctx, const bi_pt_point_light* lig
  * each shaded vertex adds their linearly-transformed-cosine lighting. */
 typedef struct bi_pt_rect_light {
     float emission[3];
-    int32_t texture_index;    /* must be < 0: textured rect lights are not supported */
+    int32_t texture_index;    /* < 0: untextured; else a texture of bi_pt_set_rect_light_textures */
     float center_position[3];
     uint32_t two_sided;
     float position0[3];
@@ -373,9 +373,41 @@ typedef struct bi_pt_rect_light {
 } bi_pt_rect_light;
 
 /* Replaces LightsContext's rect_lights buffer (lights.cpp:204-243): at most 16; rendering
- * with rect lights needs the LTC tables (bi_pt_set_ltc_luts).  BI_PT_ERROR_UNSUPPORTED for
- * a textured light or more than 16 lights. */
+ * with rect lights needs the LTC tables (bi_pt_set_ltc_luts), and a textured light
+ * (texture_index >= 0, inv_texel_size = max(texture width / light width, texture height /
+ * light height) as lights.cpp:233-236 sets it) needs that texture at render time.
+ * BI_PT_ERROR_UNSUPPORTED for more than 16 lights. */
 bi_pt_status bi_pt_set_rect_lights(bi_pt_ctx* ctx, const bi_pt_rect_light* lights, uint32_t num_lights);
+
+/* A rect light's texture with its sampler (LightsContext's rect_light_textures and
+ * rect_light_samplers, lights.cpp:229-238,261-268; rhi::SamplerDesc,
+ * include/bisemutum/rhi/sampler.hpp:37-51): num_levels mip levels one after the other in
+ * host memory, level 0 first, level l max(1, width >> l) x max(1, height >> l) texels
+ * (num_levels <= floor(log2(max(width, height))) + 1).  Filters 0 nearest / 1 linear. */
+typedef struct bi_pt_rect_texture {
+    const void* texels;
+    uint32_t width, height, num_levels;
+    uint32_t format;                 /* BI_PT_TEXEL_* */
+    uint32_t mag_filter, min_filter, mipmap_mode;
+    uint32_t address_u, address_v;   /* BI_PT_ADDRESS_* */
+    float lod_bias, lod_min, lod_max;
+} bi_pt_rect_texture;
+/* The textures rect lights sample along their integrals' directions (rect_light_eval_ltc,
+ * shaders/renderer/lights.hlsl:495-511: the diffuse term with roughness 1, the specular one
+ * with sqrt(rx ry), at the level log2(max(1, distance * roughness * inv_texel_size))
+ * (rect_light_sample_texture, lights.hlsl:429-445)); at most 16 (max_num_rect_light_textures,
+ * src/renderer/context/lights.hpp:65). */
+bi_pt_status bi_pt_set_rect_light_textures(bi_pt_ctx* ctx, const bi_pt_rect_texture* textures, uint32_t num_textures);
+/* Host only: the mip chain the engine generates for a texture asset with levels
+ * (TextureAsset::update_gpu_data, src/scene_basic/texture.cpp:176-178 ->
+ * CommandHelpers::generate_mipmaps_2d, src/graphics/command_helpers.cpp:66-199 running
+ * shaders/core/mipmap.hlsl): each level max(1, w / 2) x max(1, h / 2) texels, the average of
+ * the 2 x 2 source texels, widened to 3 where the destination's width or height is odd,
+ * source texels beyond the level read as zero (robust image access); averaged in FP32,
+ * RGBA8 stored rounded to nearest, sRGB colour decoded before and encoded after.  chain
+ * receives all num_levels levels (level 0 copied), laid out as bi_pt_rect_texture reads. */
+bi_pt_status bi_pt_rect_texture_mips(uint32_t format, uint32_t width, uint32_t height, uint32_t num_levels,
+                                     const void* level0, void* chain);
 
 /* The LTC look-up tables LightsContext loads (lights.cpp:26-37, the engine's
  * ltc_matrix_lut0..2 and ltc_norm_lut texture assets): 8 x 8 x 64 texels each, x fastest,

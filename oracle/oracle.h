@@ -45,6 +45,14 @@ typedef struct ora_texture {
     uint32_t width, height, format, filter, address_u, address_v;
 } ora_texture;
 
+/* Rect-light texture with its mip levels and sampler (as bi_pt_rect_texture): num_levels
+ * levels one after the other, level l max(1, width >> l) x max(1, height >> l) texels. */
+typedef struct ora_rect_texture {
+    const void* texels;
+    uint32_t width, height, num_levels, format, mag_filter, min_filter, mipmap_mode, address_u, address_v;
+    float lod_bias, lod_min, lod_max;
+} ora_rect_texture;
+
 /* Scene description, all host memory.
  * meshes are concatenated: mesh m owns vertices [vtx_off[m], vtx_off[m]+vtx_cnt[m])
  * of positions/normals (3 floats each) and indices [idx_off[m], idx_off[m]+idx_cnt[m])
@@ -85,9 +93,11 @@ typedef struct ora_scene_desc {
     const float* point_lights;   /* 16 floats each, layout of bi_pt_point_light; lights are the
                                     directional ones then these, at most 4 in all */
     uint32_t num_rect_lights;
-    const float* rect_lights;    /* 28 floats each, layout of bi_pt_rect_light (untextured) */
+    const float* rect_lights;    /* 28 floats each, layout of bi_pt_rect_light */
     const float* ltc_matrix[3];  /* LTC tables as bi_pt_set_ltc_luts: 8 x 8 x 64 texels, 4 floats each */
     const float* ltc_norm;       /* 2 floats a texel */
+    uint32_t num_rect_textures;
+    const ora_rect_texture* rect_textures;   /* the textures texture_index >= 0 names */
 } ora_scene_desc;
 
 /* Camera: row-major 4x4 matrices (m[row*4+col]), as bi_pt_camera. */
@@ -168,6 +178,21 @@ uint16_t ora_half32(float f);
    integral, norm, Fresnel term). */
 void ora_rect_light32(const float* rect, const float* ltc, const float* P, const float* N, const float* V, float rx,
                       float ry, const float* base, const float* f0, const float* f90, float* rgb, float* terms);
+/* Rect light `light` of the scene at a lit surface, with the scene's LTC tables and
+   rect-light textures (the kernels' rect_light_lit): as ora_rect_light32, plus dirs (6
+   floats, nullable): the world directions the diffuse and specular integrals look up the
+   light's texture along (zero where the integral is zero or the light is untextured). */
+void ora_rect_light_scene32(const ora_ctx* ctx, uint32_t light, const float* P, const float* N, const float* V,
+                            float rx, float ry, const float* base, const float* f0, const float* f90, float* rgb,
+                            float* terms, float* dirs);
+/* rect_light_sample_texture of light `light` (its texture): the texture seen from P along
+   dir at lobe roughness r; rgb, plus uvl = (u, v, level) (nullable). */
+void ora_rect_tex_lookup32(const ora_ctx* ctx, uint32_t light, const float* dir, const float* P, float r, float* rgb,
+                           float* uvl);
+/* SampleLevel of rect-light texture `tex` at (u, v) and level (4 floats out). */
+void ora_rect_tex_sample32(const ora_ctx* ctx, uint32_t tex, float u, float v, float level, float* out4);
+/* The kernels' log2 (inputs >= 1). */
+void ora_log2_32(uint32_t n, const float* x, float* out);
 /* The LTC tables' transform (9 floats, rows) and pair (2 floats) at (u3, u2, u1, u0). */
 void ora_ltc_fetch32(const float* ltc, float u3, float u2, float u1, float u0, float* m, float* b);
 int ora_render32_pixels(const ora_ctx* ctx, const ora_camera* cam, const ora_settings* st, uint32_t sample_base,

@@ -27,6 +27,13 @@ class TextureDesc(C.Structure):
                 ("filter", C.c_uint32), ("address_u", C.c_uint32), ("address_v", C.c_uint32)]
 
 
+class RectTextureDesc(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("num_levels", C.c_uint32),
+                ("format", C.c_uint32), ("mag_filter", C.c_uint32), ("min_filter", C.c_uint32),
+                ("mipmap_mode", C.c_uint32), ("address_u", C.c_uint32), ("address_v", C.c_uint32),
+                ("lod_bias", C.c_float), ("lod_min", C.c_float), ("lod_max", C.c_float)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("num_meshes", C.c_uint32), ("positions", _f32p), ("normals", _f32p), ("indices", _u32p),
@@ -41,6 +48,7 @@ class SceneDesc(C.Structure):
         ("sky_layout", C.c_uint32),
         ("num_point_lights", C.c_uint32), ("point_lights", _f32p),
         ("num_rect_lights", C.c_uint32), ("rect_lights", _f32p), ("ltc_matrix", _f32p * 3), ("ltc_norm", _f32p),
+        ("num_rect_textures", C.c_uint32), ("rect_textures", C.POINTER(RectTextureDesc)),
     ]
 
 
@@ -117,6 +125,11 @@ def lib():
         L.ora_ltc_fetch32.argtypes = [_f32p, C.c_float, C.c_float, C.c_float, C.c_float, _f32p, _f32p]
         L.ora_sky_sample32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p]
         L.ora_sky_sample_le32.argtypes = [C.c_void_p, C.c_float, C.c_float, _f32p, _f32p, _f32p]
+        L.ora_rect_light_scene32.argtypes = [C.c_void_p, C.c_uint32, _f32p, _f32p, _f32p, C.c_float, C.c_float,
+                                             _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.ora_rect_tex_lookup32.argtypes = [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_float, _f32p, _f32p]
+        L.ora_rect_tex_sample32.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_float, _f32p]
+        L.ora_log2_32.argtypes = [C.c_uint32, _f32p, _f32p]
         L.ora_sky_pdf32.restype = C.c_float
         L.ora_sky_pdf32.argtypes = [C.c_void_p, _f32p]
         L.ora_camera_ray32.argtypes = [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_float,
@@ -190,6 +203,19 @@ class OracleScene:
             for i in range(3):
                 d.ltc_matrix[i] = ptr(luts[i], _f32p)
             d.ltc_norm = ptr(luts[3], _f32p)
+        rtex = getattr(scene, "rect_textures", None) or []
+        if rtex:
+            arr = (RectTextureDesc * len(rtex))()
+            chains = []
+            for i, t in enumerate(rtex):
+                a = t.chain()
+                chains.append(a)
+                h, w = t.levels[0].shape[:2]
+                arr[i] = RectTextureDesc(a.ctypes.data, w, h, len(t.levels), t.format, t.mag_filter, t.min_filter,
+                                         t.mipmap_mode, t.address_u, t.address_v, t.lod_bias, t.lod_min, t.lod_max)
+            k["rect_tex"] = (arr, chains)
+            d.num_rect_textures = len(rtex)
+            d.rect_textures = arr
         layout = getattr(scene, "sky_layout", 0)
         if layout == 1:   # cube faces [6, S, S, 4]: rows face-major
             if k["sky"].size and (k["sky"].ndim != 4 or k["sky"].shape[0] != 6 or k["sky"].shape[1] != k["sky"].shape[2]):
@@ -394,6 +420,30 @@ class OracleScene:
         lib().ora_sky_sample_le32(self.h, u0, u1, ptr(d, _f32p), ptr(p, _f32p), ptr(le, _f32p))
         return d, float(p[0]), le
 
+    def rect_light_scene32(self, light, P, N, V, rx, ry, base, f0, f90):
+        """(rgb, terms, dirs [2, 3]) of rect light `light` of the scene at a lit surface, with its
+        texture: dirs are the diffuse and specular lookup directions (zero when unused)."""
+        a = [np.ascontiguousarray(x, np.float32) for x in (P, N, V, base, f0, f90)]
+        rgb, terms, dirs = np.zeros(3, np.float32), np.zeros(4, np.float32), np.zeros(6, np.float32)
+        lib().ora_rect_light_scene32(self.h, light, *[ptr(x, _f32p) for x in a[:3]], float(rx), float(ry),
+                                     *[ptr(x, _f32p) for x in a[3:]], ptr(rgb, _f32p), ptr(terms, _f32p),
+                                     ptr(dirs, _f32p))
+        return rgb, terms, dirs.reshape(2, 3)
+
+    def rect_tex_lookup32(self, light, d, P, roughness):
+        """(rgb, (u, v, level)) of rect light `light`'s texture seen from P along d."""
+        d = np.ascontiguousarray(d, np.float32)
+        P = np.ascontiguousarray(P, np.float32)
+        rgb, uvl = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        lib().ora_rect_tex_lookup32(self.h, light, ptr(d, _f32p), ptr(P, _f32p), float(roughness), ptr(rgb, _f32p),
+                                    ptr(uvl, _f32p))
+        return rgb, uvl
+
+    def rect_tex_sample32(self, tex, u, v, level):
+        out = np.zeros(4, np.float32)
+        lib().ora_rect_tex_sample32(self.h, tex, float(u), float(v), float(level), ptr(out, _f32p))
+        return out
+
     def sky_pdf(self, d):
         d = np.ascontiguousarray(d, np.float32)
         return float(lib().ora_sky_pdf32(self.h, ptr(d, _f32p)))
@@ -499,6 +549,14 @@ def ltc_fetch32(tables, u3, u2, u1, u0):
     b = np.zeros(2, np.float32)
     lib().ora_ltc_fetch32(ptr(t, _f32p), float(u3), float(u2), float(u1), float(u0), ptr(m, _f32p), ptr(b, _f32p))
     return m.reshape(3, 3), b
+
+
+def log2_32(x):
+    """The kernels' log2 of x >= 1 (float32 array)."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.zeros_like(x)
+    lib().ora_log2_32(len(x), ptr(x, _f32p), ptr(out, _f32p))
+    return out
 
 
 def half32(x):

@@ -585,10 +585,9 @@ inline F4 tex_texel(const Scene& s, const Tex& t, int i, int j) {
     return F4{(float)c0 / 255.0f, (float)c1 / 255.0f, (float)c2 / 255.0f, (float)c3 / 255.0f};
 }
 inline float tex_coord(float x) { return (x > -16777216.0f && x < 16777216.0f) ? x : 0.0f; }
-F4 tex_sample(const Scene& s, uint32_t id, float u, float v) {
-    const Tex& t = s.textures[id];
+F4 tex_sample_t(const Scene& s, const Tex& t, uint32_t filter, float u, float v) {
     const int W = (int)t.width, H = (int)t.height;
-    if (t.filter == 0u) {
+    if (filter == 0u) {
         const int i = tex_wrap((int)std::floor(tex_coord(u * (float)W)), W, t.address_u);
         const int j = tex_wrap((int)std::floor(tex_coord(v * (float)H)), H, t.address_v);
         return tex_texel(s, t, i, j);
@@ -604,6 +603,10 @@ F4 tex_sample(const Scene& s, uint32_t id, float u, float v) {
     const F4 top{lerp(t00.x, t10.x, ax), lerp(t00.y, t10.y, ax), lerp(t00.z, t10.z, ax), lerp(t00.w, t10.w, ax)};
     const F4 bot{lerp(t01.x, t11.x, ax), lerp(t01.y, t11.y, ax), lerp(t01.z, t11.z, ax), lerp(t01.w, t11.w, ax)};
     return F4{lerp(top.x, bot.x, ay), lerp(top.y, bot.y, ay), lerp(top.z, bot.z, ay), lerp(top.w, bot.w, ay)};
+}
+F4 tex_sample(const Scene& s, uint32_t id, float u, float v) {
+    const Tex& t = s.textures[id];
+    return tex_sample_t(s, t, t.filter, u, v);
 }
 inline uint32_t mat_word(const Scene& s, uint32_t mat, int k) {
     uint32_t w;
@@ -1312,21 +1315,23 @@ void ltc_matrix_brdf(const float* ltc, F3 lv, float rx, float ry, F3 q[4], M3* m
         m->r1 = t;
     }
 }
-// ltc_integrate_edge (lights.hlsl:347-361): z of cross(v1, v2) theta / sin(theta), fitted
-inline float ltc_edge(F3 v1, F3 v2) {
+// ltc_integrate_edge (lights.hlsl:365-381): theta / sin(theta) by its rational fit; the
+// edge's term is cross(v1, v2) times it
+inline float ltc_edge_scale(F3 v1, F3 v2) {
     const float x = dot(v1, v2);
     const float y = x < 0.0f ? -x : x;
     const float a = 5.42031f + (3.12829f + 0.0902326f * y) * y;
     const float b = 3.45068f + (4.18814f + y) * y;
     float t = a / b;
     if (x < 0.0f) t = kPiF / std::sqrt(1.0f - x * x) - t;
-    return (v1.x * v2.y - v1.y * v2.x) * t;
+    return t;
 }
-// ltc_integrate (lights.hlsl:362-400): the quad relative to P in the shading frame,
+// ltc_integrate (lights.hlsl:382-426): the quad relative to P in the shading frame,
 // transformed by minv (none: the cosine itself), clipped to z > 0 (in.z * out - out.z *
 // in for a vertex pair across the plane; a diagonal pair is the reference's empty case),
-// projected on the sphere, edges summed in polygon order.
-float ltc_integrate(F3 P, const Frame& fr, const M3* minv, const F3 q[4], bool two_sided) {
+// projected on the sphere, edges summed in polygon order.  dir (nullable): the summed
+// cross(v1, v2) theta / sin(theta) (sum.xyz of the reference), untouched when empty.
+float ltc_integrate(F3 P, const Frame& fr, const M3* minv, const F3 q[4], bool two_sided, F3* dir = nullptr) {
     F3 l[4];
     uint32_t inside = 0;
     for (int k = 0; k < 4; ++k) {
@@ -1349,15 +1354,105 @@ float ltc_integrate(F3 P, const Frame& fr, const M3* minv, const F3 q[4], bool t
         }
     }
     for (int k = 0; k < n; ++k) c[k] = normalize(c[k]);
-    float sum = 0.0f;
-    for (int k = 0; k < n; ++k) sum = sum + ltc_edge(c[k], c[k + 1 < n ? k + 1 : 0]);
+    float sum = 0.0f, sx = 0.0f, sy = 0.0f;
+    for (int k = 0; k < n; ++k) {
+        const F3 a = c[k], b = c[k + 1 < n ? k + 1 : 0];
+        const float t = ltc_edge_scale(a, b);
+        sum = sum + (a.x * b.y - a.y * b.x) * t;
+        sx = sx + (a.y * b.z - a.z * b.y) * t;
+        sy = sy + (a.z * b.x - a.x * b.z) * t;
+    }
+    if (dir) *dir = f3(sx, sy, sum);
     float r = two_sided ? (sum < 0.0f ? -sum : sum) : fmax_(sum, 0.0f);
     if (!std::isfinite(r)) r = 0.0f;
     return r;
 }
+// log2 of x >= 1 by Moshier's Cephes log2f: x = m 2^e, m folded into [sqrt(1/2), sqrt(2)),
+// log(1 + f) by its degree-8 minimax polynomial, times log2(e) split in two.
+float log2_32(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    float e = (float)((int)((b >> 23) & 255u) - 126);
+    const uint32_t mb = (b & 0x007fffffu) | 0x3f000000u;
+    float m;
+    std::memcpy(&m, &mb, 4);
+    if (m < 0.70710678118654752440f) {
+        e = e - 1.0f;
+        m = (m + m) - 1.0f;
+    } else {
+        m = m - 1.0f;
+    }
+    const float z = m * m;
+    static const float P[9] = {7.0376836292e-2f, -1.1514610310e-1f, 1.1676998740e-1f, -1.2420140846e-1f,
+                               1.4249322787e-1f, -1.6668057665e-1f, 2.0000714765e-1f, -2.4999993993e-1f,
+                               3.3333331174e-1f};
+    float y = P[0];
+    for (int k = 1; k < 9; ++k) y = y * m + P[k];
+    y = (y * m) * z;
+    y = y - 0.5f * z;
+    const float log2ea = 0.44269504088896340736f;
+    float r = y * log2ea;
+    r = r + m * log2ea;
+    r = r + y;
+    r = r + m;
+    return r + e;
+}
+// saturate with HLSL's NaN -> 0
+inline float saturate0(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
+// SampleLevel by Vulkan's explicit-LOD rules: lambda = level + bias clamped to [lod_min,
+// lod_max]; minification filter for lambda > 0; level clamped to [0, levels - 1]; nearest
+// mip mode ceil(d + 1/2) - 1, linear the two levels around d blended by its fraction.
+F4 rect_tex_sample(const Scene& s, const RectTex& r, float u, float v, float level) {
+    float lam = level + r.lod_bias;
+    lam = lam > r.lod_max ? r.lod_max : lam;
+    lam = lam < r.lod_min ? r.lod_min : lam;
+    const uint32_t filter = lam > 0.0f ? r.min_filter : r.mag_filter;
+    const float q = (float)(r.levels.size() - 1);
+    float d = lam > 0.0f ? lam : 0.0f;
+    d = d < q ? d : q;
+    if (r.mipmap_mode == 0u) return tex_sample_t(s, r.levels[(size_t)(std::ceil(d + 0.5f) - 1.0f)], filter, u, v);
+    const float fl = std::floor(d);
+    const size_t l = (size_t)fl;
+    const float f = d - fl;
+    const F4 a = tex_sample_t(s, r.levels[l], filter, u, v);
+    if (f == 0.0f) return a;
+    const F4 b = tex_sample_t(s, r.levels[l + 1], filter, u, v);
+    auto lerp = [](float x, float y, float w) { return x + (y - x) * w; };
+    return F4{lerp(a.x, b.x, f), lerp(a.y, b.y, f), lerp(a.z, b.z, f), lerp(a.w, b.w, f)};
+}
+// rect_light_sample_texture (lights.hlsl:429-445): the light plane's point along dir from
+// P as (u, v) over position2 -> position3 and position2 -> position1, clamped; level
+// log2(max(1, t roughness inv_texel_size)).  R: 28 floats (bi_pt_rect_light).
+F3 rect_tex_lookup(const Scene& s, const float* R, F3 dir, F3 P, float roughness, float* uvl = nullptr) {
+    int32_t ti;
+    std::memcpy(&ti, &R[3], 4);
+    const F3 n = f3(R[24], R[25], R[26]);
+    const F3 p2 = f3(R[16], R[17], R[18]);
+    const float st = dot(dir, n), step = st < 0.0f ? -st : st;
+    if (step < 0.0001f) {
+        if (uvl) uvl[0] = uvl[1] = uvl[2] = 0.0f;
+        return f3(0.0f, 0.0f, 0.0f);
+    }
+    const float dd = dot(P - p2, n), dist = dd < 0.0f ? -dd : dd;
+    const float t = dist / step;
+    const F3 rp = (P + dir * t) - p2;
+    const float u = saturate0(dot(rp, f3(R[20], R[21], R[22]) - p2) * R[11]);   // inv_width_sqr
+    const float v = saturate0(dot(rp, f3(R[12], R[13], R[14]) - p2) * R[15]);   // inv_height_sqr
+    const float texels = (t * roughness) * R[19];                               // inv_texel_size
+    const float level = log2_32(texels > 1.0f ? texels : 1.0f);
+    if (uvl) { uvl[0] = u; uvl[1] = v; uvl[2] = level; }
+    const F4 c = rect_tex_sample(s, s.rect_textures[(size_t)ti], u, v, level);
+    return f3(c.x, c.y, c.z);
+}
+// world direction of an integral's vector sv: normalize((m sv) in the frame (t, b, n))
+F3 ltc_direction(const Frame& fr, const M3* m, F3 sv) {
+    const F3 w = m ? m3_mul(*m, sv) : sv;
+    return normalize((fr.t * w.x + fr.b * w.y) + fr.n * w.z);
+}
 // The lighting of rect light R (28 floats, bi_pt_rect_light) at P; terms (nullable):
 // diffuse integral, specular integral, norm, Fresnel term.
-F3 rect_light_lit(const float* R, const float* ltc, F3 P, const Frame& fr, F3 V, const Mat& m, float* terms) {
+F3 rect_light_lit(const float* R, const float* ltc, F3 P, const Frame& fr, F3 V, const Mat& m, float* terms,
+                  const Scene* s = nullptr, float* dirs = nullptr) {
     const F3 lv = f3(dot(V, fr.t), dot(V, fr.b), dot(V, fr.n));
     if (terms) terms[0] = terms[1] = terms[2] = terms[3] = 0.0f;
     if (lv.z <= 0.0f) return f3(0, 0, 0);
@@ -1365,13 +1460,32 @@ F3 rect_light_lit(const float* R, const float* ltc, F3 P, const Frame& fr, F3 V,
     uint32_t two;
     std::memcpy(&two, &R[7], 4);
     const F3 e = f3(R[0], R[1], R[2]);
-    const float i_d = ltc_integrate(P, fr, nullptr, q, two != 0u);
+    int32_t ti;
+    std::memcpy(&ti, &R[3], 4);
+    const bool textured = ti >= 0 && s;   // the scene's textures (untextured without a scene)
+    F3 sd = f3(0, 0, 0), ss = f3(0, 0, 0);
+    const float i_d = ltc_integrate(P, fr, nullptr, q, two != 0u, &sd);
     M3 mt;
     float bx, by;
     ltc_matrix_brdf(ltc, lv, m.rx, m.ry, q, &mt, &bx, &by);
     const M3 mi = m3_inverse(mt);
-    const float i_s = ltc_integrate(P, fr, &mi, q, two != 0u);
+    const float i_s = ltc_integrate(P, fr, &mi, q, two != 0u, &ss);
     if (terms) { terms[0] = i_d; terms[1] = i_s; terms[2] = bx; terms[3] = by; }
+    F3 ld = e * i_d, ls = e * i_s;
+    if (dirs) for (int k = 0; k < 6; ++k) dirs[k] = 0.0f;
+    if (textured) {
+        // the light's texture along each integral's direction (lights.hlsl:495-511)
+        if (i_d != 0.0f) {
+            const F3 w = ltc_direction(fr, nullptr, sd);
+            if (dirs) { dirs[0] = w.x; dirs[1] = w.y; dirs[2] = w.z; }
+            ld = ld * rect_tex_lookup(*s, R, w, P, 1.0f);
+        }
+        if (i_s != 0.0f) {
+            const F3 w = ltc_direction(fr, &mt, ss);
+            if (dirs) { dirs[3] = w.x; dirs[4] = w.y; dirs[5] = w.z; }
+            ls = ls * rect_tex_lookup(*s, R, w, P, std::sqrt(m.rx * m.ry));
+        }
+    }
     const float ndotv = dot(fr.n, V);
     if (ndotv <= 0.0f) return f3(0, 0, 0);
     const float x = 1.0f - ndotv;
@@ -1380,7 +1494,7 @@ F3 rect_light_lit(const float* R, const float* ltc, F3 P, const Frame& fr, F3 V,
     const F3 one = f3(1.0f, 1.0f, 1.0f);
     const F3 diffuse = ((one - fres) * m.base) * kInvPi;
     const F3 specular = m.f0 * bx + m.f90 * by;
-    return diffuse * (e * i_d) + specular * (e * i_s);
+    return diffuse * ld + specular * ls;
 }
 
 // The surface a ray of direction d sees at hit h of triangle tri, point P
@@ -1532,7 +1646,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         }
         // rect lights (deferred_lighting_secondary.hlsl:73-96): unshadowed, in light order
         for (uint32_t r = 0; r < s.num_rect_lights; ++r)
-            acc_rect = acc_rect + rect_light_lit(&s.rect_lights[28 * r], s.ltc.data(), P, fr, V, m, nullptr) * beta;
+            acc_rect = acc_rect + rect_light_lit(&s.rect_lights[28 * r], s.ltc.data(), P, fr, V, m, nullptr, &s) * beta;
         // Sky next-event estimation with power-heuristic MIS against BSDF sampling, only
         // where the path also samples the next segment: the reference receives sky light
         // only through a traced segment that escapes (deferred_lighting_secondary.hlsl:24-29),
@@ -1877,6 +1991,37 @@ void ora_tex_sample32(const ora_ctx* ctx, uint32_t tex, uint32_t n, const float*
         const F4 c = tex_sample(s, tex, uv[2 * i], uv[2 * i + 1]);
         out4[4 * i] = c.x; out4[4 * i + 1] = c.y; out4[4 * i + 2] = c.z; out4[4 * i + 3] = c.w;
     }
+}
+
+void ora_rect_light_scene32(const ora_ctx* ctx, uint32_t light, const float* P, const float* N, const float* V,
+                            float rx, float ry, const float* base, const float* f0, const float* f90, float* rgb,
+                            float* terms, float* dirs) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    Mat m{};
+    m.base = f3(base[0], base[1], base[2]);
+    m.f0 = f3(f0[0], f0[1], f0[2]);
+    m.f90 = f3(f90[0], f90[1], f90[2]);
+    m.rx = rx;
+    m.ry = ry;
+    const Frame fr = make_frame(f3(N[0], N[1], N[2]));
+    const F3 c = rect_light_lit(&s.rect_lights[28 * (size_t)light], s.ltc.data(), f3(P[0], P[1], P[2]), fr,
+                                f3(V[0], V[1], V[2]), m, terms, &s, dirs);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+void ora_rect_tex_lookup32(const ora_ctx* ctx, uint32_t light, const float* dir, const float* P, float r, float* rgb,
+                           float* uvl) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    const F3 c = rect_tex_lookup(s, &s.rect_lights[28 * (size_t)light], f3(dir[0], dir[1], dir[2]),
+                                 f3(P[0], P[1], P[2]), r, uvl);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+void ora_rect_tex_sample32(const ora_ctx* ctx, uint32_t tex, float u, float v, float level, float* out4) {
+    const Scene& s = *reinterpret_cast<const Scene*>(ctx);
+    const F4 c = rect_tex_sample(s, s.rect_textures[tex], u, v, level);
+    out4[0] = c.x; out4[1] = c.y; out4[2] = c.z; out4[3] = c.w;
+}
+void ora_log2_32(uint32_t n, const float* x, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = log2_32(x[i]);
 }
 
 void ora_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out) {

@@ -201,6 +201,25 @@ ora_ctx* ora_create(const ora_scene_desc* d) {
                 s->ltc[((size_t)3 * 4096 + i) * 4 + 1] = d->ltc_norm[2 * i + 1];
             }
     }
+    for (uint32_t k = 0; d->rect_textures && k < d->num_rect_textures; ++k) {
+        const ora_rect_texture& T = d->rect_textures[k];
+        RectTex r;
+        r.mag_filter = T.mag_filter; r.min_filter = T.min_filter; r.mipmap_mode = T.mipmap_mode;
+        r.lod_bias = T.lod_bias; r.lod_min = T.lod_min; r.lod_max = T.lod_max;
+        const uint32_t* src = static_cast<const uint32_t*>(T.texels);
+        uint32_t w = T.width, h = T.height;
+        for (uint32_t l = 0; l < T.num_levels; ++l) {
+            Tex t;
+            t.width = w; t.height = h; t.format = T.format; t.address_u = T.address_u; t.address_v = T.address_v;
+            const size_t words = (size_t)w * h * (T.format == 2u ? 4u : 1u);
+            t.words.assign(src, src + words);
+            src += words;
+            r.levels.push_back(std::move(t));
+            w = w / 2 > 1 ? w / 2 : 1;
+            h = h / 2 > 1 ? h / 2 : 1;
+        }
+        s->rect_textures.push_back(std::move(r));
+    }
     s->num_point_lights = d->point_lights ? d->num_point_lights : 0;
     if (s->num_point_lights)
         s->point_lights.assign(d->point_lights, d->point_lights + 16 * (size_t)d->num_point_lights);

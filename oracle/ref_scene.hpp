@@ -29,6 +29,13 @@ struct Tex {
     std::vector<uint32_t> words;
 };
 
+// Rect-light texture: its mip levels (Tex records, filter unused) and sampler.
+struct RectTex {
+    std::vector<Tex> levels;
+    uint32_t mag_filter = 0, min_filter = 0, mipmap_mode = 0;
+    float lod_bias = 0.0f, lod_min = 0.0f, lod_max = 0.0f;
+};
+
 struct Scene {
     uint32_t num_tris = 0;
     std::vector<float> tri_v;      // 9 per triangle, world space, instance order
@@ -48,6 +55,7 @@ struct Scene {
     uint32_t num_rect_lights = 0;
     std::vector<float> rect_lights;    // 28 per light (layout of bi_pt_rect_light)
     std::vector<float> ltc;            // 4 tables x 4096 texels x 4 floats: matrix rows 0-2 (xyz), norm (xy)
+    std::vector<RectTex> rect_textures;
     // sky
     uint32_t sky_w = 0, sky_h = 0;
     uint32_t sky_layout = 0;       // 0 equirect (W x H), 1 cube map (six W x W faces, H = 6 W)
