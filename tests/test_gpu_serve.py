@@ -216,3 +216,35 @@ def test_error_flags_are_sticky_until_checked(c0_scene):
     t.render(cam, spp=1, max_bounces=3, seed=4)
     t.stats()   # cleared by the report
     t.close()
+
+
+def test_server_with_textured_rect_lights_matches_oracle(c0_scene, oracle_lib):
+    print("case: server, 4 jobs over textured-rect c0 and c0, 2 spp per round, vs oracle", flush=True)
+    from test_gpu_rect_lights import textured_c0
+    lit = textured_c0(9)
+    owners = []
+    for sc in (lit, c0_scene):
+        t = bi_pt.PathTracer()
+        t.set_scene(sc)
+        owners.append(t)
+    srv = bi_pt.Server(spp_per_round=2, max_jobs_per_round=3)
+    sids = [srv.add_scene(t) for t in owners]
+    specs = [(0, scenes.camera_c0(40, 30), dict(spp=3, max_bounces=5, seed=71, flags=bi_pt.FLAG_JITTER)),
+             (1, scenes.camera_c0(24, 24), dict(spp=2, max_bounces=4, seed=72, flags=0)),
+             (0, scenes.camera_c0(32, 20), dict(spp=5, max_bounces=6, seed=73, flags=0)),
+             (1, scenes.camera_c0(16, 16), dict(spp=1, max_bounces=3, seed=74, flags=0))]
+    ids = [srv.submit(sids[w], cam, **kw) for w, cam, kw in specs]
+    rounds = 0
+    while srv.step() and rounds < 100:
+        rounds += 1
+    for jid, (which, cam, kw) in zip(ids, specs):
+        s = srv.poll(jid)
+        assert s["state"] == 2 and s["spp_done"] == kw["spp"]
+        got = srv.take_result(jid, cam.width * cam.height)
+        sums = oracle_lib.OracleScene((lit, c0_scene)[which]).render32(cam, kw["max_bounces"], kw["seed"],
+                                                                       spp=kw["spp"], flags=kw["flags"])["sums"]
+        mean = sums / np.float32(kw["spp"])
+        assert np.array_equal(got[:, :3].view(np.uint32), mean.view(np.uint32)), (jid, kw)
+    srv.close()
+    for t in owners:
+        t.close()
