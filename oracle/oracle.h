@@ -112,8 +112,14 @@ typedef struct ora_settings {
     float ray_length;
     uint32_t max_bounces;   /* reference semantics: clamped to [2,16]; segments = max_bounces-1 */
     uint32_t seed;
-    uint32_t flags;         /* bit0: jitter primary rays, bit1: disable sky NEE */
+    uint32_t flags;         /* bit0: jitter primary rays, bit1: disable sky NEE,
+                               ORA_FLAG_GBUFFER_QUANTIZE: drift mode (below) */
 } ora_settings;
+/* Drift mode of the FP32 render: every shaded surface goes through the reference's GBuffer
+ * packing and unpacking first (half base colour and roughness, rg11b10 f0/f90 in unorm16
+ * words, oct-encoded half normal, unorm8 anisotropy), measuring how far the reference's
+ * quantisation moves the image from the FP32 path the kernels follow. */
+#define ORA_FLAG_GBUFFER_QUANTIZE (1u << 30)
 
 ora_ctx* ora_create(const ora_scene_desc* desc);
 /* Rebuilds the BVH with `rounds` rounds of treelet restructuring after the LBVH (the

@@ -21,6 +21,8 @@ _f64p = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
 _u64p = C.POINTER(C.c_uint64)
 
+GBUFFER_QUANTIZE = 1 << 30   # render32 flag: drift mode (the reference's GBuffer round trip per surface)
+
 
 class TextureDesc(C.Structure):
     _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),

@@ -181,6 +181,7 @@ inline Pcg pcg_keyed(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bo
 }
 inline float pcg_float(Pcg& r) { return (float)(pcg_next(r) >> 8) * 0x1p-24f; }
 constexpr uint32_t kJitterBounce = 255u;
+constexpr uint32_t kOraGbufferQuantize = ORA_FLAG_GBUFFER_QUANTIZE;   // drift mode (oracle.h)
 
 }  // namespace
 }  // namespace ora
@@ -553,6 +554,11 @@ void build_bvh(Scene& s) {
 
 // ================================================================ traversal
 namespace ora {
+// GBuffer packing helpers (defined with the GBuffer export below), used by the drift mode.
+uint16_t half_rne(float f);
+uint32_t pack_rg11b10(F3 c);
+uint16_t unorm16_half_word(uint32_t h);
+uint8_t unorm8(float v);
 namespace {
 
 // ---------------------------------------------------------------- textures
@@ -1543,6 +1549,58 @@ void hit_surface(const Scene& s, uint32_t tri, const Hit& h, F3 P, F3 d, F3* N_o
     *mid_out = mid;
 }
 
+// IEEE half bits -> float (exact).
+float half_to_float(uint16_t h) {
+    const uint32_t e = (h >> 10) & 31u, m = h & 1023u;
+    float f;
+    if (e == 0u) {
+        f = (float)m * 0x1p-24f;
+    } else {
+        const uint32_t x = (e == 31u ? 0x7f800000u : (e + 112u) << 23) | (m << 13);
+        std::memcpy(&f, &x, 4);
+    }
+    return (h & 0x8000u) ? -f : f;
+}
+// Drift mode (settings flag kOraGbufferQuantize): the surface as the reference's lighting
+// reads it back from its GBuffer (pack_surface_to_gbuffer, then unpack_gbuffer_to_surface,
+// shaders/renderer/gbuffer.hlsl:18-45; pack.hlsl:6-23,86-129): base colour and roughness
+// through half floats, f0 and f90 through rg11b10 fixed point split into two unorm16
+// words (read back as uint(w / 65535 * 65535.5)), the normal oct-encoded in halves and
+// decoded, anisotropy through unorm8.  The kernels keep FP32 throughout.
+void gbuffer_round_trip(F3* N, Mat* m, float* rough, float aniso) {
+    auto qh = [](float x) { return half_to_float(half_rne(x)); };
+    m->base = f3(qh(m->base.x), qh(m->base.y), qh(m->base.z));
+    *rough = qh(*rough);
+    auto rt = [](F3 c) {
+        const uint32_t p = pack_rg11b10(c);
+        const uint32_t w0 = unorm16_half_word(p & 0xffffu), w1 = unorm16_half_word(p >> 16);
+        const uint32_t u0 = (uint32_t)(((float)w0 / 65535.0f) * 65535.5f);
+        const uint32_t u1 = (uint32_t)(((float)w1 / 65535.0f) * 65535.5f);
+        const uint32_t u = u0 | (u1 << 16);
+        return f3((float)(u & 0x7ffu) / 2047.0f, (float)((u >> 11) & 0x7ffu) / 2047.0f,
+                  (float)((u >> 22) & 0x3ffu) / 1023.0f);
+    };
+    m->f0 = rt(m->f0);
+    m->f90 = rt(m->f90);
+    // oct_encode (pack.hlsl:86-90) in halves, oct_decode (pack.hlsl:94-99)
+    const float sa = (std::fabs(N->x) + std::fabs(N->y)) + std::fabs(N->z);
+    float x = N->x / sa, y = N->y / sa;
+    if (!(N->z / sa >= 0.0f)) {
+        const float wx = (1.0f - std::fabs(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        const float wy = (1.0f - std::fabs(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = wx;
+        y = wy;
+    }
+    x = qh(x);
+    y = qh(y);
+    F3 n = f3(x, y, (1.0f - std::fabs(x)) - std::fabs(y));
+    const float t = std::min(std::max(-n.z, 0.0f), 1.0f);
+    n.x = n.x + (n.x >= 0.0f ? -t : t);
+    n.y = n.y + (n.y >= 0.0f ? -t : t);
+    *N = normalize(n);
+    mat_alpha(*m, *rough, (float)unorm8(aniso) / 255.0f);
+}
+
 PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st,
                    uint32_t pixel, uint32_t sample, const CacheMode* cm = nullptr) {
     PathOut out{};
@@ -1585,6 +1643,7 @@ PathOut trace_path(const Scene& s, const ora_camera& cam, const ora_settings& st
         float rough;
         uint32_t mid;
         hit_surface(s, tri, h, P, ray.d, &N, &m, &rough, &mid);
+        if (st.flags & kOraGbufferQuantize) gbuffer_round_trip(&N, &m, &rough, s.materials[kMatFloats * mid + 7]);
         F3 V = normalize(ray.o - P);
         if (cm && (int)b == cm->bounce) {
             if (!cm->train) {

@@ -145,3 +145,43 @@ def test_decoded_normals_and_tangents(oracle_lib):
             worst_n = max(worst_n, np.linalg.norm(Nd - n_w))
             worst_t = max(worst_t, np.linalg.norm(Td - t_w))
     assert worst_n < 2e-3 and worst_t < 4e-3, (worst_n, worst_t)
+
+
+# ----------------------------------------------------- drift mode of the restatement
+QUANTIZE = oracle.GBUFFER_QUANTIZE
+
+
+def test_drift_mode_keeps_exactly_representable_surfaces(oracle_lib):
+    """On a plane whose surface survives the GBuffer exactly (half-exact base colour and
+    roughness, zero f0/f90 and anisotropy, normal +Y), the drift mode renders the same image;
+    a base colour that is not a half moves it by half precision only."""
+    cam = scenes.make_camera((0.3, 3.0, 0.4), (0.0, 0.0, 0.0), 24, 18, 70.0)
+    exact = oracle_lib.OracleScene(plane_scene(True, base=(0.5, 0.25, 0.75), rough=0.5, f0=(0, 0, 0), f90=(0, 0, 0),
+                                               aniso=0.0))
+    a = exact.render32(cam, 3, 5, spp=2)["sums"]
+    b = exact.render32(cam, 3, 5, spp=2, flags=QUANTIZE)["sums"]
+    assert a.sum() > 0 and np.array_equal(a, b)
+    off = oracle_lib.OracleScene(plane_scene(True, base=(0.3, 0.3, 0.3), rough=0.5, f0=(0, 0, 0), f90=(0, 0, 0),
+                                             aniso=0.0))
+    a = off.render32(cam, 2, 5, spp=1)["sums"].astype(np.float64)
+    b = off.render32(cam, 2, 5, spp=1, flags=QUANTIZE)["sums"].astype(np.float64)
+    rel = np.abs(b - a) / np.maximum(a, 1e-12)
+    lit_px = a[:, 0] > 1e-3
+    # one segment: the image scales with the base colour, 0.3 -> half(0.3) = 0.300048828125
+    want = abs(float(np.float32(0.3).astype(np.float16)) / float(np.float32(0.3)) - 1)
+    assert lit_px.all()
+    np.testing.assert_allclose(rel[lit_px], want, rtol=1e-3)
+
+
+def test_drift_of_the_reference_gbuffer_on_c0(oracle_lib, c0_scene):
+    """The reference's GBuffer quantisation against the FP32 path on the C0 room: a small,
+    non-zero difference of the mean (f90 = 1 does not survive the unorm16 split: it reads
+    back as 2046/2047)."""
+    cam = scenes.camera_c0(48, 36)
+    o = oracle_lib.OracleScene(c0_scene)
+    a = o.render32(cam, 5, 9, spp=4)["sums"].astype(np.float64)
+    b = o.render32(cam, 5, 9, spp=4, flags=QUANTIZE)["sums"].astype(np.float64)
+    assert np.isfinite(b).all()
+    drift = abs(b.sum() / a.sum() - 1)
+    print(f"c0 drift of the mean radiance: {drift:.2e}", flush=True)
+    assert 0 < drift < 2e-2
