@@ -657,6 +657,12 @@ bi_pt_status bi_pt_set_rect_light_textures(bi_pt_ctx* ctx, const bi_pt_rect_text
             src += n;
         }
     }
+    auto same_bytes = [](const auto& a, const auto& b) {
+        return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(a[0])) == 0);
+    };
+    if (same_bytes(recs, ctx->rect_tex_host) && same_bytes(levels, ctx->rect_tex_levels_host) &&
+        same_bytes(pool, ctx->rect_tex_pool_host))
+        return BI_PT_OK;   // unchanged (set every frame): the running image continues
     BI_HIP_TRY(ctx, hipSetDevice(ctx->device));
     BI_TRY(dalloc(ctx, ctx->rect_tex, std::max<size_t>(recs.size(), 1)));
     BI_TRY(dalloc(ctx, ctx->rect_tex_levels, std::max<size_t>(levels.size(), 1)));
@@ -669,6 +675,9 @@ bi_pt_status bi_pt_set_rect_light_textures(bi_pt_ctx* ctx, const bi_pt_rect_text
         BI_HIP_TRY(ctx, hipMemcpy(ctx->rect_tex_pool.p, pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
     }
     ctx->num_rect_tex = num_textures;
+    ctx->rect_tex_host.swap(recs);
+    ctx->rect_tex_levels_host.swap(levels);
+    ctx->rect_tex_pool_host.swap(pool);
     ctx->acc_count = 0;
     ctx->scene_version++;
     return BI_PT_OK;

@@ -182,6 +182,9 @@ struct bi_pt_ctx : bipt::Owner {
     bipt::DevBuf<bipt::TexRec> rect_tex_levels;
     bipt::DevBuf<uint32_t> rect_tex_pool;
     uint32_t num_rect_tex = 0;
+    std::vector<bipt::RectTexRec> rect_tex_host;      // what the device holds, to skip unchanged sets
+    std::vector<bipt::TexRec> rect_tex_levels_host;
+    std::vector<uint32_t> rect_tex_pool_host;
     // sky
     bipt::DevBuf<float4> sky_tex;
     bipt::DevBuf<float> sky_w;

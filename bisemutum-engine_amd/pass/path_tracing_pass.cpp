@@ -23,9 +23,35 @@ void PathTracingPass::check(bi_pt_status s) const {
 // path_tracing.cpp:183-222: binds the light and sky contexts for the coming frame.
 void PathTracingPass::update_params(LightsContext& lights_ctx, SkyboxContext& skybox_ctx,
                                     PathTracingSettings const&) {
+    // directional and point lights share four slots: drop the old point lights first only
+    // when the new directional ones would not fit beside them (unchanged lights keep the
+    // running image)
+    if (lights_ctx.dir_lights.size() + num_point_lights_ > 4) {
+        check(bi_pt_set_point_lights(ctx_, nullptr, 0));
+        num_point_lights_ = 0;
+    }
     check(bi_pt_set_lights(ctx_, lights_ctx.dir_lights.data(), (uint32_t)lights_ctx.dir_lights.size()));
-    check(bi_pt_set_sky(ctx_, skybox_ctx.rgba32f.empty() ? nullptr : skybox_ctx.rgba32f.data(), skybox_ctx.width,
-                        skybox_ctx.height, skybox_ctx.color, skybox_ctx.rotation));
+    check(bi_pt_set_point_lights(ctx_, lights_ctx.point_lights.empty() ? nullptr : lights_ctx.point_lights.data(),
+                                 (uint32_t)lights_ctx.point_lights.size()));
+    num_point_lights_ = lights_ctx.point_lights.size();
+    if (!lights_ctx.ltc_norm.empty()) {
+        for (const auto& t : lights_ctx.ltc_matrix)
+            if (t.size() != 8u * 8u * 64u * 4u)
+                throw PathTracingError(BI_PT_ERROR_INVALID_ARGUMENT, "LTC matrix table size");
+        if (lights_ctx.ltc_norm.size() != 8u * 8u * 64u * 2u)
+            throw PathTracingError(BI_PT_ERROR_INVALID_ARGUMENT, "LTC norm table size");
+        check(bi_pt_set_ltc_luts(ctx_, lights_ctx.ltc_matrix[0].data(), lights_ctx.ltc_matrix[1].data(),
+                                 lights_ctx.ltc_matrix[2].data(), lights_ctx.ltc_norm.data()));
+    }
+    check(bi_pt_set_rect_light_textures(ctx_, lights_ctx.rect_light_textures.data(),
+                                        (uint32_t)lights_ctx.rect_light_textures.size()));
+    check(bi_pt_set_rect_lights(ctx_, lights_ctx.rect_lights.empty() ? nullptr : lights_ctx.rect_lights.data(),
+                                (uint32_t)lights_ctx.rect_lights.size()));
+    const float* sky = skybox_ctx.rgba32f.empty() ? nullptr : skybox_ctx.rgba32f.data();
+    if (skybox_ctx.cube)
+        check(bi_pt_set_sky_cube(ctx_, sky, sky ? skybox_ctx.width : 0u, skybox_ctx.color, skybox_ctx.rotation));
+    else
+        check(bi_pt_set_sky(ctx_, sky, skybox_ctx.width, skybox_ctx.height, skybox_ctx.color, skybox_ctx.rotation));
 }
 
 void PathTracingPass::ensure_outputs(uint32_t width, uint32_t height) {

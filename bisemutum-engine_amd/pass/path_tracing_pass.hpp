@@ -5,8 +5,11 @@
 //   PathTracingPass::update_params(LightsContext&, SkyboxContext&, PathTracingSettings const&)
 //   PathTracingPass::render(Camera const&, InputData const&, PathTracingSettings const&) -> OutputData
 // with the engine's types reduced to what the pass reads:
-//   LightsContext        -> the DirLightData array (src/renderer/context/lights.hpp:12-22, 92-131)
-//   SkyboxContext        -> equirect texels, colour*strength, rotation (src/scene_basic/skybox.cpp:43-73)
+//   LightsContext        -> the DirLightData, PointLightData and RectLightData arrays, the rect
+//                           lights' textures and the LTC tables (src/renderer/context/lights.hpp:
+//                           12-22,65-131, lights.cpp:26-37,120-243)
+//   SkyboxContext        -> the skybox cube (or an equirect), colour*strength, rotation
+//                           (src/scene_basic/skybox.cpp:43-73)
 //   PathTracingSettings  -> include/bisemutum/renderer/basic.hpp:76-81 (denoise is unused by the pass)
 //   InputData            -> the scene that the TLAS is built from (the reference takes a
 //                           ready AccelerationStructureHandle; here the pass builds its LBVH)
@@ -41,11 +44,18 @@ struct PathTracingSettings {
 };
 
 struct LightsContext {
-    std::vector<bi_pt_dir_light> dir_lights;   // DirLightData records
+    std::vector<bi_pt_dir_light> dir_lights;       // DirLightData records
+    std::vector<bi_pt_point_light> point_lights;   // PointLightData records (with dir_lights, at most 4)
+    std::vector<bi_pt_rect_light> rect_lights;     // RectLightData records (at most 16)
+    std::vector<bi_pt_rect_texture> rect_light_textures;   // what texture_index >= 0 names
+    // LTC tables (ltc_matrix_lut0..2 RGBA32F, ltc_norm_lut RG32F, 8 x 8 x 64 texels each);
+    // empty: not loaded (then rect_lights must be empty)
+    std::vector<float> ltc_matrix[3], ltc_norm;
 };
 
 struct SkyboxContext {
-    std::vector<float> rgba32f;   // equirect texels, row 0 = +Y pole (empty: black sky)
+    std::vector<float> rgba32f;   // equirect texels, row 0 = +Y pole, or six cube faces (empty: black sky)
+    bool cube = false;            // rgba32f holds the skybox cube: six width x width faces (height unused)
     uint32_t width = 0, height = 0;
     float color[3] = {0.0f, 0.0f, 0.0f};
     float rotation[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -91,6 +101,7 @@ private:
 
     bi_pt_ctx* ctx_ = nullptr;
     uint64_t scene_version_ = ~0ull;
+    size_t num_point_lights_ = 0;   // point lights last given to the context
     const SceneGeometry* scene_ptr_ = nullptr;
     float* color_ = nullptr;
     float* depth_ = nullptr;

@@ -146,6 +146,12 @@ def test_textured_rect_lights_match_oracle(oracle_lib):
     got = g.sums(cam.width * cam.height)
     want = oracle_lib.OracleScene(sc).render32(cam, 5, 0x7E57, spp=2, flags=bi_pt.FLAG_JITTER)["sums"]
     assert radiance_check(got, want)
+    # setting the same textures again (as a renderer does every frame) keeps the running image
+    g.set_rect_light_textures(sc.rect_textures)
+    g.render(cam, spp=1, max_bounces=5, seed=0x7E57, flags=bi_pt.FLAG_JITTER, accumulate=True)
+    cont = oracle_lib.OracleScene(sc).render32(cam, 5, 0x7E57, spp=1, sample_base=2, sums=want.copy(),
+                                               flags=bi_pt.FLAG_JITTER)["sums"]
+    assert radiance_check(g.sums(cam.width * cam.height), cont)
     # the textures change the image
     plain = oracle_lib.OracleScene(dataclasses.replace(sc, rect_textures=[], rect_lights=np.stack(
         [rect_c0().rect_lights[0], rect_c0().rect_lights[1], sc.rect_lights[2]]))).render32(
